@@ -1,0 +1,219 @@
+/*
+ * glx: MI355X (gfx950) kernels for svirl's 2D Ginzburg-Landau solver.
+ *
+ * A C ABI over hand-written HIP kernels. Plain device pointers and sizes
+ * only; every entry point returns 0 on success or a nonzero GLX_E* / HIP
+ * error code, with a message from glx_last_error().
+ *
+ * Device buffers use the reference's flat layouts (so the Python side can
+ * hand over the same buffers svirl's kernels received):
+ *   psi : complex (re, im interleaved), node n = i + Nx*j
+ *   ab  : real, a at i + (Nx-1)*j, then b at Na + i + Nx*j, Na = (Nx-1)*Ny
+ *   mt  : uint8 (0/1) per cell, i + (Nx-1)*j; NULL = no material tiling
+ *   eps_map : real per node; NULL = scalar epsilon
+ *   abei : real, same layout as ab; NULL = absent
+ * "real" is double for GLX_F64 and float for GLX_F32 (the FP32-mixed form:
+ * fields and stencil arithmetic in float, all reductions, residuals and
+ * line-search coefficients in double).
+ *
+ * Reference interfaces replaced (paths relative to the svirl repository):
+ * the kernels are svirl/cuda/{td,cg,observables,utils,reduction,
+ * block_reduction}.h, launched through pycuda by
+ * svirl/parallel/startup.py:272-280 (get_function), with geometry baked in
+ * at JIT time (startup.py:252-261); here the geometry is a glx_grid
+ * argument. All calls run on one library-owned HIP stream; calls that return
+ * host values synchronise it.
+ */
+#ifndef GLX_H
+#define GLX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { GLX_F64 = 0, GLX_F32 = 1 };
+
+enum {
+    GLX_OK = 0,
+    GLX_E_ARG = -1,      /* invalid argument (size, NULL, precision) */
+    GLX_E_HIP = -2,      /* a HIP runtime call failed */
+    GLX_E_NOINIT = -3,   /* glx_init not called */
+    GLX_E_TIMEOUT = -4,  /* a grid barrier timed out inside a persistent kernel */
+    GLX_E_CAPACITY = -5  /* an output list was too small */
+};
+
+/* Grid geometry; replaces the %(Nx)s, %(Ny)s, %(dx)s, %(dy)s template
+ * constants of svirl/parallel/startup.py:252-258. */
+typedef struct glx_grid {
+    int32_t Nx, Ny;
+    double dx, dy;
+} glx_grid;
+
+/* ---------------------------------------------------------------- runtime */
+
+/* Select the device and create the library stream
+ * (replaces Startup.__init_device, svirl/parallel/startup.py:285-291). */
+int glx_init(int device);
+const char *glx_last_error(void);
+int glx_synchronize(void);
+/* Number of compute units of the current device (for launch sizing). */
+int glx_device_info(int *num_cus, int *max_wg_per_cu_hint, char *name, int name_len);
+
+/* Device memory owned by the caller (replaces pycuda gpuarray allocation,
+ * svirl/storage/arrays.py:574-591, and Utils.copy_dtod,
+ * svirl/parallel/utils.py:22-27). */
+int glx_malloc(void **ptr, size_t bytes);
+int glx_free(void *ptr);
+int glx_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int glx_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int glx_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int glx_memset(void *dst, int value, size_t bytes);
+
+/* Stream-ordered HIP event timing of library work (milliseconds). */
+int glx_timer_start(void);
+int glx_timer_stop(float *ms);
+
+/* ------------------------------------------------------------------- TDGL */
+
+/* One Jacobi sweep of backward-Euler psi, with the reference's residual:
+ * atomicMax(r2max, int(min(1e4*max|dpsi|/stop_eps, 1e8))). r2max is a device
+ * int32 the caller zeroes. With langevin_c > 1e-32 and jstep == 0 the noisy
+ * right-hand side is written back to psi_rhs.
+ * Replaces iterate_order_parameter_jacobi_step, svirl/cuda/td.h:85-214,
+ * launched at svirl/solvers/td.py:168-187. */
+int glx_td_psi_sweep(int prec, glx_grid g, double dt, double eps, const void *eps_map,
+                     const uint8_t *mt, const void *ab, void *psi_rhs, const void *psi,
+                     void *psi_next, double langevin_c, uint32_t jstep, uint32_t rand_t,
+                     double stop_eps, int32_t *r2max);
+
+/* One Jacobi sweep of the vector potential. abi_ab_rhs is the supercurrent
+ * source buffer and may alias ab_next (each thread reads its own source
+ * links before it writes them, as the reference's ping-pong relies on).
+ * Replaces iterate_vector_potential_jacobi_step, svirl/cuda/td.h:391-544,
+ * launched at svirl/solvers/td.py:277-300. */
+int glx_td_a_sweep(int prec, glx_grid g, double dt, double kappa2, double rho, double H,
+                   const uint8_t *mt, const void *psi, const void *abi_ab_rhs, void *ab_rhs,
+                   const void *ab, void *ab_next, double langevin_c, uint32_t jstep,
+                   uint32_t rand_t, double stop_eps, int32_t *r2max);
+
+/* psi[n] <- |psi[n]| at lock_ns[0..lock_N) (svirl/cuda/td.h:376-388). */
+int glx_td_phase_lock(int prec, void *psi, int32_t lock_N, const int32_t *lock_ns);
+
+typedef struct glx_td_params {
+    double dt;
+    double eps;          /* scalar epsilon (ignored where eps_map is given) */
+    double kappa2;       /* kappa^2; only read when solveA */
+    double rho;          /* 1/sigma */
+    double H;
+    double stop_psi;     /* stop criteria after the td.py:51-66 clamps */
+    double stop_A;
+    double langevin_psi;
+    double langevin_A;
+    uint32_t rand_t;     /* TD._random_t at entry */
+    int32_t solveA;      /* kappa < inf */
+    int32_t max_sweeps;  /* Jacobi cap per solve; the reference uses 1024 */
+    int32_t eqn;         /* 0: psi then A, 1: psi only, 2: A only */
+} glx_td_params;
+
+/* nsteps TDGL steps in one persistent launch: per step up to max_sweeps psi
+ * sweeps and (if solveA) up to max_sweeps A sweeps, each sweep followed by a
+ * grid barrier and the reference's stop rule, with no host round trip.
+ * psi_bufs / ab_bufs are three same-sized buffers rotated in place of the
+ * reference's ψ_rhs copy and pointer swaps; *psi_cur / *ab_cur index the one
+ * holding the state on entry and are updated on return. rhs_psi / rhs_ab are
+ * only used with Langevin noise (may be NULL otherwise). sweeps (host,
+ * 2*nsteps int32) receives (S_psi, S_A) per step.
+ * Replaces the host loops TD.__iterate / __iterate_order_parameter_gpu /
+ * __iterate_vector_potential_gpu, svirl/solvers/td.py:157-366, with the
+ * A-source aliasing of td.py:264-303 kept. wg / wg_size select the
+ * persistent grid (0 = library default). */
+int glx_td_run(int prec, glx_grid g, const glx_td_params *p, const void *eps_map,
+               const uint8_t *mt, void *const psi_bufs[3], int32_t *psi_cur,
+               void *const ab_bufs[3], int32_t *ab_cur, void *rhs_psi, void *rhs_ab,
+               int32_t nsteps, int32_t *sweeps, int32_t wg, int32_t wg_size);
+
+/* Duration (ms, HIP events on the library stream) of the persistent kernel
+ * of the last glx_td_run call, excluding its reset and read-back. */
+int glx_td_last_kernel_ms(float *ms);
+
+/* Largest co-resident grid for glx_td_run at a workgroup size. */
+int glx_td_max_grid(int prec, int32_t wg_size, int32_t *max_wg);
+
+/* ------------------------------------------------------------ observables */
+
+/* B per cell (svirl/cuda/observables.h:4-37; observables.py:36-58). */
+int glx_magnetic_field(int prec, glx_grid g, const void *abei, const void *ab, void *B);
+/* kappa^2 (curl B - H) on links, ghost-cell boundary
+ * (observables.h:40-156; observables.py:86-110). */
+int glx_current_density(int prec, glx_grid g, double kappa2, double H, const void *abei,
+                        const void *ab, void *jxjy);
+/* r_Du id Im(conj psi0 U psi1) on links (observables.h:159-235). */
+int glx_supercurrent_density(int prec, glx_grid g, const uint8_t *mt, const void *psi,
+                             const void *abei, const void *ab, void *jsxjsy);
+/* Total GL free energy, FP64 accumulation, returned on the host
+ * (free_energy_pseudodensity, observables.h:250-362, + Reduction.gsum,
+ * svirl/parallel/reduction.py:40-110; observables.py:124-149). */
+int glx_free_energy(int prec, glx_grid g, double kappa2, double eps, const void *eps_map,
+                    double H, const uint8_t *mt, const void *psi, const void *abei,
+                    const void *ab, double *F);
+
+/* Vortex cells: winding number of every cell as in
+ * svirl/observables/vortex_detector.py:55-72 (np.angle, np.mod, np.round
+ * semantics), computed in FP64. Flagged cells are written to `rec` as
+ * records of 14 doubles: (cell n, v, psi00.re, .im, psip0.re, .im,
+ * psipp.re, .im, psi0p.re, .im, a00, a0p, b00, bp0) where the link values
+ * are ab + abi (abi may be NULL) for the host triangulation
+ * (vortex_detector.py:256-296). Records come unordered; *count is the
+ * total number of flagged cells (may exceed capacity: then GLX_E_CAPACITY
+ * and only `capacity` records are written). */
+int glx_vortex_cells(int prec, glx_grid g, const void *psi, const void *ab, const void *abi,
+                     double H, double *rec, int32_t capacity, int32_t *count);
+
+/* ------------------------------------------------------------- reductions */
+
+/* Sum of n reals, FP64 accumulation, deterministic order (Reduction.gsum,
+ * svirl/parallel/reduction.py:40-110; block_reduction.h:80-92).
+ * block_size 0 = default; any multiple of 64 up to 1024 is accepted. */
+int glx_sum(int prec, const void *x, int64_t n, int32_t block_size, double *out);
+/* Column sums of an nv x ne row-major real array (Reduction.gsum_v,
+ * reduction.py:115-173; reduction.h:56-82). out: ne doubles on the host. */
+int glx_sum_v(int prec, const void *x, int64_t nv, int32_t ne, int32_t block_size, double *out);
+/* Polak-Ribiere sums over n reals: out[0] = sum x*(x-y), out[1] = sum y*y
+ * (xmyx_*_sum + x_mag2_*_sum + gsum, svirl/cuda/utils.h:12-70,
+ * cg.py:422-449). A complex array is passed as 2N reals. */
+int glx_pr_sums(int prec, const void *x, const void *y, int64_t n, double *out);
+
+/* ----------------------------------------------------------------- CG */
+
+/* G += dxdy dF/dpsi (free_energy_jacobian_psi, svirl/cuda/cg.h:15-121). */
+int glx_cg_jacobian_psi(int prec, glx_grid g, double kappa2, double eps, const void *eps_map,
+                        double H, const uint8_t *mt, const void *psi, const void *abei,
+                        const void *ab, void *G);
+/* G += 2 dxdy [kappa^2 curlcurl - r id js] (free_energy_jacobian_A,
+ * cg.h:124-301). */
+int glx_cg_jacobian_A(int prec, glx_grid g, double kappa2, double H, const uint8_t *mt,
+                      const void *psi, const void *abei, const void *ab, void *G);
+/* c0..c4 of F(psi + alpha dpsi) (free_energy_conjgrad_coef_psi,
+ * cg.h:314-474, + gsum_v; cg.py:184-224). eps is the scalar the reference
+ * passes (0 when an epsilon map is set; the kernel never reads the map). */
+int glx_cg_coef_psi(int prec, glx_grid g, double kappa2, double eps, double H,
+                    const uint8_t *mt, const void *psi, const void *dpsi, const void *abei,
+                    const void *ab, double *c5);
+/* The 17 coefficients of free_energy_conjgrad_coef (cg.h:477-731;
+ * cg.py:325-375), in kernel order c00..c04, c10..c14, c20..c24, c30, c40. */
+int glx_cg_coef(int prec, glx_grid g, double kappa2, double eps, double H, const uint8_t *mt,
+                const void *psi, const void *dpsi, const void *abei, const void *ab,
+                const void *dab, double *c17);
+/* z = alpha*x + y and z = alpha*x - y over n reals (axpy_r/axpy_c and
+ * axmy_r/axmy_c, svirl/cuda/utils.h:73-114; a complex array is 2N reals). */
+int glx_axpy(int prec, const void *x, const void *y, void *z, double alpha, int64_t n);
+int glx_axmy(int prec, const void *x, const void *y, void *z, double alpha, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLX_H */

@@ -1,0 +1,286 @@
+// extern "C" entry points of libglx: argument dispatch by precision, the
+// library context (device, stream, scratch) and error reporting.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+
+#include "glx_launch.hpp"
+
+namespace glx {
+
+namespace {
+
+struct Ctx {
+    bool ready = false;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_kernel_ms = 0.f;
+    Env env{};
+    void *scratch = nullptr;
+};
+
+Ctx g_ctx;
+std::mutex g_mu;
+thread_local char g_err[512] = "";
+
+constexpr int SWEEPS_CAP = 1 << 16;  // steps per persistent launch
+
+}  // namespace
+
+int set_error(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int check_hip(hipError_t st, const char *what) {
+    if (st == hipSuccess) return GLX_OK;
+    return set_error(GLX_E_HIP, "%s: %s (%d)", what, hipGetErrorString(st), (int)st);
+}
+
+static int ensure_ready() {
+    if (!g_ctx.ready) return set_error(GLX_E_NOINIT, "glx_init has not been called");
+    return GLX_OK;
+}
+
+}  // namespace glx
+
+using namespace glx;
+
+#define GLX_REQUIRE_INIT() do { int _r = ensure_ready(); if (_r) return _r; } while (0)
+#define GLX_DISPATCH(prec, call)                                                   \
+    do {                                                                           \
+        GLX_REQUIRE_INIT();                                                        \
+        if ((prec) == GLX_F64) return L<double>::call;                             \
+        if ((prec) == GLX_F32) return L<float>::call;                              \
+        return set_error(GLX_E_ARG, "unknown precision %d", (int)(prec));          \
+    } while (0)
+
+extern "C" {
+
+const char *glx_last_error(void) { return g_err; }
+
+int glx_init(int device) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (g_ctx.ready) {
+        if (device == g_ctx.device) return GLX_OK;
+        return set_error(GLX_E_ARG, "glx already initialised on device %d", g_ctx.device);
+    }
+    int n = 0;
+    hipError_t st = hipGetDeviceCount(&n);
+    if (st != hipSuccess) return check_hip(st, "hipGetDeviceCount");
+    if (device < 0 || device >= n) return set_error(GLX_E_ARG, "device %d not present (%d devices)", device, n);
+    if ((st = hipSetDevice(device)) != hipSuccess) return check_hip(st, "hipSetDevice");
+    hipDeviceProp_t prop;
+    if ((st = hipGetDeviceProperties(&prop, device)) != hipSuccess) return check_hip(st, "hipGetDeviceProperties");
+    if ((st = hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking)) != hipSuccess)
+        return check_hip(st, "hipStreamCreate");
+    if ((st = hipEventCreate(&g_ctx.ev0)) != hipSuccess) return check_hip(st, "hipEventCreate");
+    if ((st = hipEventCreate(&g_ctx.ev1)) != hipSuccess) return check_hip(st, "hipEventCreate");
+    if ((st = hipEventCreate(&g_ctx.env.kev0)) != hipSuccess) return check_hip(st, "hipEventCreate");
+    if ((st = hipEventCreate(&g_ctx.env.kev1)) != hipSuccess) return check_hip(st, "hipEventCreate");
+    g_ctx.env.last_kernel_ms = &g_ctx.last_kernel_ms;
+    // scratch layout: partials | result | grp | top | err | slots | state | sweeps
+    const size_t partials = sizeof(double) * 1024 * 32;
+    const size_t result = sizeof(double) * 64;
+    const size_t grp = sizeof(unsigned long long) * 8;
+    const size_t top = sizeof(unsigned long long) * 8;
+    const size_t err = 256;
+    const size_t slots = sizeof(int) * 64 * 8;
+    const size_t state = 256;
+    const size_t sweeps = sizeof(int) * 2 * SWEEPS_CAP;
+    const size_t total = partials + result + grp + top + err + slots + state + sweeps;
+    if ((st = hipMalloc(&g_ctx.scratch, total)) != hipSuccess) return check_hip(st, "hipMalloc(scratch)");
+    if ((st = hipMemset(g_ctx.scratch, 0, total)) != hipSuccess) return check_hip(st, "hipMemset(scratch)");
+    char *p = (char *)g_ctx.scratch;
+    Env &e = g_ctx.env;
+    e.stream = g_ctx.stream;
+    e.partials = (double *)p; p += partials;
+    e.result = (double *)p; p += result;
+    e.grp = (unsigned long long *)p; p += grp;
+    e.top = (unsigned long long *)p; p += top;
+    e.err = (int *)p; p += err;
+    e.slots = (int *)p; p += slots;
+    e.state = (int *)p; p += state;
+    e.sweeps = (int *)p; p += sweeps;
+    e.sweeps_cap = SWEEPS_CAP;
+    e.num_cus = prop.multiProcessorCount;
+    g_ctx.device = device;
+    g_ctx.ready = true;
+    return GLX_OK;
+}
+
+int glx_synchronize(void) {
+    GLX_REQUIRE_INIT();
+    return check_hip(hipStreamSynchronize(g_ctx.stream), "hipStreamSynchronize");
+}
+
+int glx_device_info(int *num_cus, int *max_wg_per_cu_hint, char *name, int name_len) {
+    GLX_REQUIRE_INIT();
+    hipDeviceProp_t prop;
+    hipError_t st = hipGetDeviceProperties(&prop, g_ctx.device);
+    if (st != hipSuccess) return check_hip(st, "hipGetDeviceProperties");
+    if (num_cus) *num_cus = prop.multiProcessorCount;
+    if (max_wg_per_cu_hint) *max_wg_per_cu_hint = prop.maxThreadsPerMultiProcessor / 256;
+    if (name && name_len > 0) {
+        snprintf(name, (size_t)name_len, "%s %s", prop.name, prop.gcnArchName);
+    }
+    return GLX_OK;
+}
+
+int glx_malloc(void **ptr, size_t bytes) {
+    GLX_REQUIRE_INIT();
+    if (!ptr) return set_error(GLX_E_ARG, "glx_malloc: NULL out pointer");
+    *ptr = nullptr;
+    if (bytes == 0) bytes = 1;
+    return check_hip(hipMalloc(ptr, bytes), "hipMalloc");
+}
+
+int glx_free(void *ptr) {
+    GLX_REQUIRE_INIT();
+    if (!ptr) return GLX_OK;
+    hipError_t st = hipStreamSynchronize(g_ctx.stream);
+    if (st != hipSuccess) return check_hip(st, "hipStreamSynchronize");
+    return check_hip(hipFree(ptr), "hipFree");
+}
+
+static int copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, const char *what) {
+    GLX_REQUIRE_INIT();
+    if (bytes == 0) return GLX_OK;
+    if (!dst || !src) return set_error(GLX_E_ARG, "%s: NULL pointer", what);
+    hipError_t st = hipMemcpyAsync(dst, src, bytes, kind, g_ctx.stream);
+    if (st != hipSuccess) return check_hip(st, what);
+    if (kind == hipMemcpyDeviceToDevice) return GLX_OK;
+    return check_hip(hipStreamSynchronize(g_ctx.stream), what);
+}
+
+int glx_memcpy_h2d(void *dst, const void *src, size_t bytes) { return copy(dst, src, bytes, hipMemcpyHostToDevice, "glx_memcpy_h2d"); }
+int glx_memcpy_d2h(void *dst, const void *src, size_t bytes) { return copy(dst, src, bytes, hipMemcpyDeviceToHost, "glx_memcpy_d2h"); }
+int glx_memcpy_d2d(void *dst, const void *src, size_t bytes) { return copy(dst, src, bytes, hipMemcpyDeviceToDevice, "glx_memcpy_d2d"); }
+
+int glx_memset(void *dst, int value, size_t bytes) {
+    GLX_REQUIRE_INIT();
+    if (bytes == 0) return GLX_OK;
+    if (!dst) return set_error(GLX_E_ARG, "glx_memset: NULL pointer");
+    return check_hip(hipMemsetAsync(dst, value, bytes, g_ctx.stream), "hipMemsetAsync");
+}
+
+int glx_timer_start(void) {
+    GLX_REQUIRE_INIT();
+    return check_hip(hipEventRecord(g_ctx.ev0, g_ctx.stream), "hipEventRecord");
+}
+
+int glx_timer_stop(float *ms) {
+    GLX_REQUIRE_INIT();
+    if (!ms) return set_error(GLX_E_ARG, "glx_timer_stop: NULL");
+    hipError_t st = hipEventRecord(g_ctx.ev1, g_ctx.stream);
+    if (st == hipSuccess) st = hipEventSynchronize(g_ctx.ev1);
+    if (st == hipSuccess) st = hipEventElapsedTime(ms, g_ctx.ev0, g_ctx.ev1);
+    return check_hip(st, "glx_timer_stop");
+}
+
+int glx_td_psi_sweep(int prec, glx_grid g, double dt, double eps, const void *eps_map, const uint8_t *mt,
+                     const void *ab, void *psi_rhs, const void *psi, void *psi_next, double langevin_c,
+                     uint32_t jstep, uint32_t rand_t, double stop_eps, int32_t *r2max) {
+    GLX_DISPATCH(prec, psi_sweep(g_ctx.env, g, dt, eps, eps_map, mt, ab, psi_rhs, psi, psi_next, langevin_c, jstep,
+                                 rand_t, stop_eps, r2max));
+}
+
+int glx_td_a_sweep(int prec, glx_grid g, double dt, double kappa2, double rho, double H, const uint8_t *mt,
+                   const void *psi, const void *abi_ab_rhs, void *ab_rhs, const void *ab, void *ab_next,
+                   double langevin_c, uint32_t jstep, uint32_t rand_t, double stop_eps, int32_t *r2max) {
+    GLX_DISPATCH(prec, a_sweep(g_ctx.env, g, dt, kappa2, rho, H, mt, psi, abi_ab_rhs, ab_rhs, ab, ab_next,
+                               langevin_c, jstep, rand_t, stop_eps, r2max));
+}
+
+int glx_td_phase_lock(int prec, void *psi, int32_t lock_N, const int32_t *lock_ns) {
+    GLX_DISPATCH(prec, phase_lock(g_ctx.env, psi, lock_N, lock_ns));
+}
+
+int glx_td_run(int prec, glx_grid g, const glx_td_params *p, const void *eps_map, const uint8_t *mt,
+               void *const psi_bufs[3], int32_t *psi_cur, void *const ab_bufs[3], int32_t *ab_cur, void *rhs_psi,
+               void *rhs_ab, int32_t nsteps, int32_t *sweeps, int32_t wg, int32_t wg_size) {
+    GLX_DISPATCH(prec, td_run(g_ctx.env, g, p, eps_map, mt, psi_bufs, psi_cur, ab_bufs, ab_cur, rhs_psi, rhs_ab,
+                              nsteps, sweeps, wg, wg_size));
+}
+
+int glx_td_last_kernel_ms(float *ms) {
+    GLX_REQUIRE_INIT();
+    if (!ms) return set_error(GLX_E_ARG, "glx_td_last_kernel_ms: NULL");
+    *ms = g_ctx.last_kernel_ms;
+    return GLX_OK;
+}
+
+int glx_td_max_grid(int prec, int32_t wg_size, int32_t *max_wg) {
+    GLX_DISPATCH(prec, td_max_grid(g_ctx.env, wg_size, max_wg));
+}
+
+int glx_magnetic_field(int prec, glx_grid g, const void *abei, const void *ab, void *B) {
+    GLX_DISPATCH(prec, magnetic_field(g_ctx.env, g, abei, ab, B));
+}
+
+int glx_current_density(int prec, glx_grid g, double kappa2, double H, const void *abei, const void *ab, void *jxjy) {
+    GLX_DISPATCH(prec, current_density(g_ctx.env, g, kappa2, H, abei, ab, jxjy));
+}
+
+int glx_supercurrent_density(int prec, glx_grid g, const uint8_t *mt, const void *psi, const void *abei,
+                             const void *ab, void *jsxjsy) {
+    GLX_DISPATCH(prec, supercurrent_density(g_ctx.env, g, mt, psi, abei, ab, jsxjsy));
+}
+
+int glx_free_energy(int prec, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
+                    const uint8_t *mt, const void *psi, const void *abei, const void *ab, double *F) {
+    GLX_DISPATCH(prec, free_energy(g_ctx.env, g, kappa2, eps, eps_map, H, mt, psi, abei, ab, F));
+}
+
+int glx_vortex_cells(int prec, glx_grid g, const void *psi, const void *ab, const void *abi, double H, double *rec,
+                     int32_t capacity, int32_t *count) {
+    GLX_DISPATCH(prec, vortex_cells(g_ctx.env, g, psi, ab, abi, H, rec, capacity, count));
+}
+
+int glx_sum(int prec, const void *x, int64_t n, int32_t block_size, double *out) {
+    GLX_DISPATCH(prec, sum(g_ctx.env, x, n, block_size, out));
+}
+
+int glx_sum_v(int prec, const void *x, int64_t nv, int32_t ne, int32_t block_size, double *out) {
+    GLX_DISPATCH(prec, sum_v(g_ctx.env, x, nv, ne, block_size, out));
+}
+
+int glx_pr_sums(int prec, const void *x, const void *y, int64_t n, double *out) {
+    GLX_DISPATCH(prec, pr_sums(g_ctx.env, x, y, n, out));
+}
+
+int glx_cg_jacobian_psi(int prec, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
+                        const uint8_t *mt, const void *psi, const void *abei, const void *ab, void *G) {
+    GLX_DISPATCH(prec, jacobian_psi(g_ctx.env, g, kappa2, eps, eps_map, H, mt, psi, abei, ab, G));
+}
+
+int glx_cg_jacobian_A(int prec, glx_grid g, double kappa2, double H, const uint8_t *mt, const void *psi,
+                      const void *abei, const void *ab, void *G) {
+    GLX_DISPATCH(prec, jacobian_A(g_ctx.env, g, kappa2, H, mt, psi, abei, ab, G));
+}
+
+int glx_cg_coef_psi(int prec, glx_grid g, double kappa2, double eps, double H, const uint8_t *mt, const void *psi,
+                    const void *dpsi, const void *abei, const void *ab, double *c5) {
+    GLX_DISPATCH(prec, coef_psi(g_ctx.env, g, kappa2, eps, H, mt, psi, dpsi, abei, ab, c5));
+}
+
+int glx_cg_coef(int prec, glx_grid g, double kappa2, double eps, double H, const uint8_t *mt, const void *psi,
+                const void *dpsi, const void *abei, const void *ab, const void *dab, double *c17) {
+    GLX_DISPATCH(prec, coef(g_ctx.env, g, kappa2, eps, H, mt, psi, dpsi, abei, ab, dab, c17));
+}
+
+int glx_axpy(int prec, const void *x, const void *y, void *z, double alpha, int64_t n) {
+    GLX_DISPATCH(prec, axpy(g_ctx.env, x, y, z, alpha, n, 0));
+}
+
+int glx_axmy(int prec, const void *x, const void *y, void *z, double alpha, int64_t n) {
+    GLX_DISPATCH(prec, axpy(g_ctx.env, x, y, z, alpha, n, 1));
+}
+
+}  // extern "C"

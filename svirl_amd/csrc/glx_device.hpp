@@ -1,0 +1,227 @@
+// Device-side building blocks shared by every glx kernel (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace glx {
+
+constexpr int WAVE = 64;
+// Columns owned by one wave in the row-marching kernels: lane 0 carries the
+// column left of the strip (halo), lanes 1..63 own 63 consecutive columns.
+constexpr int STRIP = WAVE - 1;
+
+template <typename R> struct cplx { R re, im; };
+
+template <typename R> __device__ __forceinline__ cplx<R> ld(const cplx<R> *p, long n);
+template <> __device__ __forceinline__ cplx<double> ld(const cplx<double> *p, long n) {
+    double2 v = reinterpret_cast<const double2 *>(p)[n];
+    return {v.x, v.y};
+}
+template <> __device__ __forceinline__ cplx<float> ld(const cplx<float> *p, long n) {
+    float2 v = reinterpret_cast<const float2 *>(p)[n];
+    return {v.x, v.y};
+}
+template <typename R> __device__ __forceinline__ void st(cplx<R> *p, long n, cplx<R> v);
+template <> __device__ __forceinline__ void st(cplx<double> *p, long n, cplx<double> v) {
+    reinterpret_cast<double2 *>(p)[n] = make_double2(v.re, v.im);
+}
+template <> __device__ __forceinline__ void st(cplx<float> *p, long n, cplx<float> v) {
+    reinterpret_cast<float2 *>(p)[n] = make_float2(v.re, v.im);
+}
+
+// sin and cos of one link phase. Link phases d*A are tens of radians at
+// most, so a bounded-range method keeps the register footprint small:
+// Cody-Waite reduction by pi/2 with a three-part constant (each leading part
+// has few enough significant bits that k*part is exact, for |k| < 2^20 in
+// double and |k| < 2^12 in float), then the Taylor series of sin and cos on
+// [-pi/4, pi/4] in Horner form. The result is exactly odd/even symmetric:
+// sincos_r(-x) = (-sin x, cos x), so U(-x) = conj(U(x)) holds bitwise.
+__host__ __device__ __forceinline__ void sincos_r(double x, double *s, double *c) {
+    const double k = rint(x * 0.6366197723675814);  // x * 2/pi
+    double r = fma(-k, 1.5707963267341256, x);      // pi/2 = p1 + p2 + p3
+    r = fma(-k, 6.077100506303966e-11, r);
+    r = fma(-k, 2.0222662487959506e-21, r);
+    const double z = r * r;
+    double ps = 1.0 / 355687428096000.0;             // 1/17!
+    ps = fma(ps, z, -1.0 / 1307674368000.0);         // 1/15!
+    ps = fma(ps, z, 1.0 / 6227020800.0);             // 1/13!
+    ps = fma(ps, z, -1.0 / 39916800.0);              // 1/11!
+    ps = fma(ps, z, 1.0 / 362880.0);                 // 1/9!
+    ps = fma(ps, z, -1.0 / 5040.0);                  // 1/7!
+    ps = fma(ps, z, 1.0 / 120.0);                    // 1/5!
+    ps = fma(ps, z, -1.0 / 6.0);                     // 1/3!
+    const double sr = fma(r * z, ps, r);
+    double pc = -1.0 / 6402373705728000.0;           // 1/18!
+    pc = fma(pc, z, 1.0 / 20922789888000.0);         // 1/16!
+    pc = fma(pc, z, -1.0 / 87178291200.0);           // 1/14!
+    pc = fma(pc, z, 1.0 / 479001600.0);              // 1/12!
+    pc = fma(pc, z, -1.0 / 3628800.0);               // 1/10!
+    pc = fma(pc, z, 1.0 / 40320.0);                  // 1/8!
+    pc = fma(pc, z, -1.0 / 720.0);                   // 1/6!
+    pc = fma(pc, z, 1.0 / 24.0);                     // 1/4!
+    pc = fma(pc, z, -0.5);                           // 1/2!
+    const double cr = fma(z, pc, 1.0);
+    const int q = (int)((long long)k & 3);
+    const double sv = (q & 1) ? cr : sr, cv = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -sv : sv;
+    *c = ((q + 1) & 2) ? -cv : cv;
+}
+
+__host__ __device__ __forceinline__ void sincos_r(float x, float *s, float *c) {
+    const float k = rintf(x * 0.63661975f);
+    float r = fmaf(-k, 1.5703125f, x);
+    r = fmaf(-k, 4.837512969970703e-4f, r);
+    r = fmaf(-k, 7.549790126404332e-08f, r);
+    const float z = r * r;
+    float ps = -1.0f / 39916800.0f;
+    ps = fmaf(ps, z, 1.0f / 362880.0f);
+    ps = fmaf(ps, z, -1.0f / 5040.0f);
+    ps = fmaf(ps, z, 1.0f / 120.0f);
+    ps = fmaf(ps, z, -1.0f / 6.0f);
+    const float sr = fmaf(r * z, ps, r);
+    float pc = 1.0f / 479001600.0f;
+    pc = fmaf(pc, z, -1.0f / 3628800.0f);
+    pc = fmaf(pc, z, 1.0f / 40320.0f);
+    pc = fmaf(pc, z, -1.0f / 720.0f);
+    pc = fmaf(pc, z, 1.0f / 24.0f);
+    pc = fmaf(pc, z, -0.5f);
+    const float cr = fmaf(z, pc, 1.0f);
+    const int q = (int)k & 3;
+    const float sv = (q & 1) ? cr : sr, cv = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -sv : sv;
+    *c = ((q + 1) & 2) ? -cv : cv;
+}
+
+// Thomas Wang's 32-bit integer hash and the two Langevin streams built on
+// it (the Langevin noise of svirl/cuda/common.h:36-63).
+__device__ __forceinline__ uint32_t wang_hash(uint32_t s) {
+    s = (s ^ 61u) ^ (s >> 16);
+    s *= 9u;
+    s = s ^ (s >> 4);
+    s *= 0x27d4eb2du;
+    s = s ^ (s >> 15);
+    return s;
+}
+template <typename R> __device__ __forceinline__ R noise_1(uint32_t n, uint32_t t) {
+    return R(2.3283064365386962890625e-10 * double(R(wang_hash(71u * n + 9887u * t))));
+}
+template <typename R> __device__ __forceinline__ R noise_2(uint32_t n, uint32_t t) {
+    return R(2.3283064365386962890625e-10 * double(R(wang_hash(73u * n + 9901u * t + 1u))));
+}
+
+// Scaled Jacobi residual of one point: int(min(1e4*r/stop_eps, 1e8)),
+// evaluated in double.
+__device__ __forceinline__ int scaled_residual(double r, double stop_eps) {
+    double v = 1.0e4 * r / stop_eps;
+    if (v > 1.0e8) v = 1.0e8;
+    return int(v);
+}
+
+// ---- wave64 / workgroup reductions ---------------------------------------
+
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, WAVE));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+// Sum of one double per thread over the workgroup; result valid in thread 0.
+// `lds` needs blockDim.x/64 doubles.
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    const int nw = (blockDim.x + WAVE - 1) / WAVE;
+    __syncthreads();
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < nw; ++k) s += lds[k];
+    return s;
+}
+
+// Maximum of one int per thread over the workgroup; valid in thread 0.
+__device__ __forceinline__ int block_max_int(int v, int *lds) {
+    v = wave_max_int(v);
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    const int nw = (blockDim.x + WAVE - 1) / WAVE;
+    __syncthreads();
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    int m = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < nw; ++k) m = max(m, lds[k]);
+    return m;
+}
+
+// ---- agent-scope atomics ---------------------------------------------------
+
+__device__ __forceinline__ unsigned long long ld_acquire(unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_acquire(int *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_max_agent(int *p, int v) {
+    __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- grid barrier for persistent kernels ----------------------------------
+//
+// Two-level arrival counters (blocks grouped by blockIdx % NGRP, the last
+// arriver of a group bumps the top counter), all counters monotonic within a
+// launch and zeroed by the host before it. Every wait is bounded by a wall
+// clock timeout: a block that times out raises `err` and every block leaves
+// its loops at the next barrier, so a launch that is not fully co-resident
+// ends with an error instead of hanging.
+constexpr int NGRP = 8;
+constexpr unsigned long long BARRIER_TIMEOUT_TICKS = 100ull * 1000 * 1000 * 5;  // 5 s at 100 MHz
+
+struct GridSync {
+    unsigned long long *grp;  // NGRP counters
+    unsigned long long *top;  // 1 counter
+    int *err;                 // 1 flag
+};
+
+// Call from every thread of every block the same number of times. Returns
+// false when the barrier failed (timeout anywhere in the grid).
+__device__ __forceinline__ bool grid_barrier(const GridSync &gs, unsigned long long &epoch) {
+    __shared__ int s_ok;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ++epoch;
+        const unsigned nb = gridDim.x;
+        const unsigned g = blockIdx.x % NGRP;
+        const unsigned gsize = (nb - g + NGRP - 1) / NGRP;
+        const unsigned ngroups = nb < (unsigned)NGRP ? nb : (unsigned)NGRP;
+        unsigned long long old =
+            __hip_atomic_fetch_add(&gs.grp[g], 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1 == epoch * gsize)
+            __hip_atomic_fetch_add(gs.top, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long target = epoch * ngroups;
+        int ok = 1;
+        const unsigned long long t0 = wall_clock64();
+        while (ld_acquire(gs.top) < target) {
+            if (ld_acquire(gs.err)) { ok = 0; break; }
+            if (wall_clock64() - t0 > BARRIER_TIMEOUT_TICKS) {
+                __hip_atomic_store(gs.err, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    __threadfence();
+    return s_ok != 0;
+}
+
+}  // namespace glx

@@ -1,0 +1,553 @@
+// Observables, CG kernels, reductions and the vortex winding kernel.
+//
+// One thread per node (or link / cell), 256-thread workgroups, grid-stride
+// loops. Every reduction accumulates in FP64 (also for float fields) and is
+// deterministic: per-workgroup partials over a fixed grid, then one
+// workgroup sums the partials in a fixed order.
+//
+// Per-node formulas follow svirl/cuda/observables.h and cg.h term by term.
+#pragma once
+
+#include "glx_device.hpp"
+
+namespace glx {
+
+constexpr int RED_BLOCK = 256;
+constexpr int RED_MAX_BLOCKS = 1024;
+
+struct Geo {
+    int Nx, Ny, Nxa, Nya, Nxb, Nyb, Nxc, Nyc;
+    long N, Na, Nab, Nc;
+    double dx, dy, idx, idy, idx2, idy2, idxy;
+};
+
+inline Geo make_geo(int Nx, int Ny, double dx, double dy) {
+    Geo g;
+    g.Nx = Nx; g.Ny = Ny;
+    g.Nxa = Nx - 1; g.Nya = Ny; g.Nxb = Nx; g.Nyb = Ny - 1; g.Nxc = Nx - 1; g.Nyc = Ny - 1;
+    g.N = (long)Nx * Ny; g.Na = (long)g.Nxa * g.Nya; g.Nab = g.Na + (long)g.Nxb * g.Nyb;
+    g.Nc = (long)g.Nxc * g.Nyc;
+    g.dx = dx; g.dy = dy;
+    g.idx = 1.0 / dx; g.idy = 1.0 / dy;
+    g.idx2 = 1.0 / (dx * dx); g.idy2 = 1.0 / (dy * dy); g.idxy = 1.0 / (dx * dy);
+    return g;
+}
+
+// Geometry constants converted to the field precision.
+template <typename R> struct GeoR {
+    R dx, dy, idx, idy, idx2, idy2, idxy;
+    __host__ __device__ GeoR() {}
+    __host__ __device__ explicit GeoR(const Geo &g)
+        : dx(R(g.dx)), dy(R(g.dy)), idx(R(g.idx)), idy(R(g.idy)), idx2(R(g.idx2)), idy2(R(g.idy2)),
+          idxy(R(g.idxy)) {}
+};
+
+struct Cells { bool mm, mp, pm, pp; };
+
+__device__ __forceinline__ Cells cells_of(const Geo &g, const uint8_t *mt, int i, int j) {
+    Cells c;
+    c.mm = (i > 0) && (j > 0);
+    c.mp = (i > 0) && (j + 1 < g.Ny);
+    c.pm = (i + 1 < g.Nx) && (j > 0);
+    c.pp = (i + 1 < g.Nx) && (j + 1 < g.Ny);
+    if (mt) {
+        if (c.mm) c.mm = mt[(i - 1) + (long)g.Nxc * (j - 1)] != 0;
+        if (c.mp) c.mp = mt[(i - 1) + (long)g.Nxc * j] != 0;
+        if (c.pm) c.pm = mt[i + (long)g.Nxc * (j - 1)] != 0;
+        if (c.pp) c.pp = mt[i + (long)g.Nxc * j] != 0;
+    }
+    return c;
+}
+
+template <typename R> __device__ __forceinline__ R pv(const R *abei, const R *ab, long k) {
+    R p = R(0);
+    if (abei) p += abei[k];
+    if (ab) p += ab[k];
+    return p;
+}
+
+// js(psi0, ph, psi1) = Im(conj psi0 U(ph) psi1) (common.h:66-73)
+template <typename R> __device__ __forceinline__ R js(cplx<R> p0, R ph, cplx<R> p1) {
+    R s, c;
+    sincos_r(ph, &s, &c);
+    return (p0.re * p1.im - p0.im * p1.re) * c - (p0.re * p1.re + p0.im * p1.im) * s;
+}
+// |psi1 U(ph) - psi0|^2 (observables.h:238-247)
+template <typename R> __device__ __forceinline__ R g_grad(cplx<R> p0, R ph, cplx<R> p1) {
+    R s, c;
+    sincos_r(ph, &s, &c);
+    const R re = p1.re * c + p1.im * s - p0.re;
+    const R im = p1.im * c - p1.re * s - p0.im;
+    return re * re + im * im;
+}
+// psi1 U(ph) - psi0 (cg.h:304-311)
+template <typename R> __device__ __forceinline__ cplx<R> grad_c(cplx<R> p0, R ph, cplx<R> p1) {
+    R s, c;
+    sincos_r(ph, &s, &c);
+    return {p1.re * c + p1.im * s - p0.re, p1.im * c - p1.re * s - p0.im};
+}
+
+// ---------------------------------------------------------------- B field
+template <typename R>
+__global__ void k_magnetic_field(Geo g, const R *abei, const R *ab, R *B) {
+    const GeoR<R> q(g);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.Nc; n += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(n % g.Nxc), j = (int)(n / g.Nxc);
+        R Bn = R(0);
+        for (int s = 0; s < 2; ++s) {
+            const R *v = s == 0 ? abei : ab;
+            if (!v) continue;
+            Bn += q.idx * (v[g.Na + i + 1 + (long)g.Nxb * j] - v[g.Na + i + (long)g.Nxb * j]) -
+                  q.idy * (v[i + (long)g.Nxa * (j + 1)] - v[i + (long)g.Nxa * j]);
+        }
+        B[n] = Bn;
+    }
+}
+
+// Ghost-cell curl-curl accumulator of one link (observables.h:66-155,
+// cg.h:179-290): H source, abei and ab diagonals, lower then upper
+// neighbours (abei unscaled, ab scaled by dd).
+template <typename R>
+__device__ __forceinline__ R curlcurl_a(const Geo &g, const GeoR<R> &q, R H, const R *abei, const R *ab, int i, int j) {
+    const long Na = g.Na;
+    const int Nxa = g.Nxa, Nxb = g.Nxb, Nya = g.Nya;
+    R c = R(0), dd = R(1);
+    if (j == 0) { c -= R(2) * H * q.idy; dd = R(2); }
+    else if (j + 1 == Nya) { c += R(2) * H * q.idy; dd = R(2); }
+    const long n = i + (long)Nxa * j;
+    if (abei) c += R(2) / dd * q.idy2 * abei[n];
+    if (ab) c += R(2) * q.idy2 * ab[n];
+    if (j > 0) {
+        const long jm = j - 1;
+        if (abei) c += dd / dd * (-q.idy2 * abei[i + Nxa * jm] + q.idxy * abei[Na + i + Nxb * jm] - q.idxy * abei[Na + i + 1 + Nxb * jm]);
+        if (ab) c += dd * (-q.idy2 * ab[i + Nxa * jm] + q.idxy * ab[Na + i + Nxb * jm] - q.idxy * ab[Na + i + 1 + Nxb * jm]);
+    }
+    if (j + 1 < Nya) {
+        const long jp = j + 1;
+        if (abei) c += dd / dd * (-q.idy2 * abei[i + Nxa * jp] - q.idxy * abei[Na + i + Nxb * (long)j] + q.idxy * abei[Na + i + 1 + Nxb * (long)j]);
+        if (ab) c += dd * (-q.idy2 * ab[i + Nxa * jp] - q.idxy * ab[Na + i + Nxb * (long)j] + q.idxy * ab[Na + i + 1 + Nxb * (long)j]);
+    }
+    return c;
+}
+
+template <typename R>
+__device__ __forceinline__ R curlcurl_b(const Geo &g, const GeoR<R> &q, R H, const R *abei, const R *ab, int i, int j) {
+    const long Na = g.Na;
+    const int Nxa = g.Nxa, Nxb = g.Nxb;
+    R c = R(0), dd = R(1);
+    if (i == 0) { c += R(2) * H * q.idx; dd = R(2); }
+    else if (i + 1 == Nxb) { c -= R(2) * H * q.idx; dd = R(2); }
+    const long n = Na + i + (long)Nxb * j;
+    if (abei) c += R(2) / dd * q.idx2 * abei[n];
+    if (ab) c += R(2) * q.idx2 * ab[n];
+    if (i > 0) {
+        const int im = i - 1;
+        if (abei) c += dd / dd * (-q.idx2 * abei[Na + im + (long)Nxb * j] + q.idxy * abei[im + (long)Nxa * j] - q.idxy * abei[im + (long)Nxa * (j + 1)]);
+        if (ab) c += dd * (-q.idx2 * ab[Na + im + (long)Nxb * j] + q.idxy * ab[im + (long)Nxa * j] - q.idxy * ab[im + (long)Nxa * (j + 1)]);
+    }
+    if (i + 1 < Nxb) {
+        const int ip = i + 1;
+        if (abei) c += dd / dd * (-q.idx2 * abei[Na + ip + (long)Nxb * j] - q.idxy * abei[i + (long)Nxa * j] + q.idxy * abei[i + (long)Nxa * (j + 1)]);
+        if (ab) c += dd * (-q.idx2 * ab[Na + ip + (long)Nxb * j] - q.idxy * ab[i + (long)Nxa * j] + q.idxy * ab[i + (long)Nxa * (j + 1)]);
+    }
+    return c;
+}
+
+// ------------------------------------------------------ current densities
+template <typename R>
+__global__ void k_current_density(Geo g, R kappa2, R H, const R *abei, const R *ab, R *J) {
+    const GeoR<R> q(g);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
+        if (i < g.Nxa && j < g.Nya) J[i + (long)g.Nxa * j] = kappa2 * curlcurl_a(g, q, H, abei, ab, i, j);
+        if (i < g.Nxb && j < g.Nyb) J[g.Na + i + (long)g.Nxb * j] = kappa2 * curlcurl_b(g, q, H, abei, ab, i, j);
+    }
+}
+
+template <typename R>
+__global__ void k_supercurrent_density(Geo g, const uint8_t *mt, const cplx<R> *psi, const R *abei, const R *ab, R *Js) {
+    const GeoR<R> q(g);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
+        const cplx<R> p0 = ld(psi, n);
+        const Cells c = cells_of(g, mt, i, j);
+        if (i < g.Nxa && j < g.Nya) {
+            const long m = i + (long)g.Nxa * j;
+            R jl = R(0);
+            if (c.pm || c.pp) {
+                const R r = R(0.5) * ((c.pm ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
+                jl = r * q.idx * js(p0, q.dx * pv(abei, ab, m), ld(psi, n + 1));
+            }
+            Js[m] = jl;
+        }
+        if (i < g.Nxb && j < g.Nyb) {
+            const long m = g.Na + i + (long)g.Nxb * j;
+            R jl = R(0);
+            if (c.mp || c.pp) {
+                const R r = R(0.5) * ((c.mp ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
+                jl = r * q.idy * js(p0, q.dy * pv(abei, ab, m), ld(psi, n + g.Nx));
+            }
+            Js[m] = jl;
+        }
+    }
+}
+
+// ---------------------------------------------------------- reductions
+
+// Sums L per-thread accumulators over the workgroup and writes them to
+// partials[blockIdx.x*L + k].
+template <int L>
+__device__ __forceinline__ void block_store_partials(double (&acc)[L], double *partials) {
+    __shared__ double lds[RED_BLOCK / WAVE * L];
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    const int nw = blockDim.x / WAVE;
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+        double v = wave_sum(acc[k]);
+        if (lane == 0) lds[wid * L + k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < L) {
+        double s = 0.0;
+        for (int w = 0; w < nw; ++w) s += lds[w * L + threadIdx.x];
+        partials[(long)blockIdx.x * L + threadIdx.x] = s;
+    }
+}
+
+// out[k] = scale * sum_b partials[b*L + k], one workgroup, fixed order.
+static __global__ void k_finish_partials(const double *partials, int nb, int L, double scale, double *out) {
+    __shared__ double lds[RED_BLOCK / WAVE];
+    for (int k = 0; k < L; ++k) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partials[(long)b * L + k];
+        s = block_sum(s, lds);
+        if (threadIdx.x == 0) out[k] = s * scale;
+        __syncthreads();
+    }
+}
+
+template <typename R>
+__global__ void k_sum(const R *x, long n, double *partials) {
+    double acc[1] = {0.0};
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) acc[0] += double(x[k]);
+    block_store_partials<1>(acc, partials);
+}
+
+template <typename R>
+__global__ void k_sum_v(const R *x, long nv, int ne, double *partials) {
+    // one component per launch index k; ne <= 32
+    __shared__ double lds[RED_BLOCK / WAVE];
+    for (int k = 0; k < ne; ++k) {
+        double s = 0.0;
+        for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < nv; v += (long)gridDim.x * blockDim.x)
+            s += double(x[v * ne + k]);
+        s = block_sum(s, lds);
+        if (threadIdx.x == 0) partials[(long)blockIdx.x * ne + k] = s;
+        __syncthreads();
+    }
+}
+
+template <typename R>
+__global__ void k_pr_sums(const R *x, const R *y, long n, double *partials) {
+    double acc[2] = {0.0, 0.0};
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        const R xv = x[k], yv = y[k];
+        acc[0] += double(xv * (xv - yv));
+        acc[1] += double(yv * yv);
+    }
+    block_store_partials<2>(acc, partials);
+}
+
+template <typename R>
+__global__ void k_axpy(const R *x, const R *y, R *z, R alpha, long n, int minus) {
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
+        z[k] = minus ? alpha * x[k] - y[k] : alpha * x[k] + y[k];
+}
+
+// ----------------------------------------------------------- free energy
+
+// Per-node pseudodensity e (observables.h:250-357), before dx*dy.
+template <typename R>
+__device__ __forceinline__ R fe_node(const Geo &g, const GeoR<R> &q, R kappa2, R eps, const R *eps_map, R H,
+                                     const uint8_t *mt, const cplx<R> *psi, const R *abei, const R *ab, long n) {
+    const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
+    R e = R(0);
+    const Cells c = cells_of(g, mt, i, j);
+    if (c.mm || c.mp || c.pm || c.pp) {
+        const R f_mm = c.mm ? R(1) : R(0), f_mp = c.mp ? R(1) : R(0), f_pm = c.pm ? R(1) : R(0), f_pp = c.pp ? R(1) : R(0);
+        const R r_m0 = R(0.5) * (f_mm + f_mp), r_p0 = R(0.5) * (f_pm + f_pp);
+        const R r_0m = R(0.5) * (f_mm + f_pm), r_0p = R(0.5) * (f_mp + f_pp);
+        const R gw = R(0.25) * (r_m0 + r_p0 + r_0m + r_0p);
+        const cplx<R> p0 = ld(psi, n);
+        const R p2 = p0.re * p0.re + p0.im * p0.im;
+        const R ee = eps_map ? eps_map[n] : eps;
+        e += gw * (R(0.5) * p2 - ee) * p2;
+        if (c.pm || c.pp) {
+            const R ph = pv(abei, ab, i + (long)g.Nxa * j);
+            e += r_p0 * q.idx2 * g_grad(p0, q.dx * ph, ld(psi, n + 1));
+        }
+        if (c.mp || c.pp) {
+            const R ph = pv(abei, ab, g.Na + i + (long)g.Nxb * j);
+            e += r_0p * q.idy2 * g_grad(p0, q.dy * ph, ld(psi, n + g.Nx));
+        }
+    }
+    if (kappa2 > R(0) && i < g.Nxc && j < g.Nyc) {
+        R dB = -H;
+        for (int s = 0; s < 2; ++s) {
+            const R *v = s == 0 ? abei : ab;
+            if (!v) continue;
+            dB += q.idx * (v[g.Na + i + 1 + (long)g.Nxb * j] - v[g.Na + i + (long)g.Nxb * j]) -
+                  q.idy * (v[i + (long)g.Nxa * (j + 1)] - v[i + (long)g.Nxa * j]);
+        }
+        e += kappa2 * dB * dB;
+    }
+    return e;
+}
+
+template <typename R>
+__global__ void k_free_energy(Geo g, R kappa2, R eps, const R *eps_map, R H, const uint8_t *mt,
+                              const cplx<R> *psi, const R *abei, const R *ab, double *partials) {
+    const GeoR<R> q(g);
+    double acc[1] = {0.0};
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x)
+        acc[0] += double(fe_node(g, q, kappa2, eps, eps_map, H, mt, psi, abei, ab, n));
+    block_store_partials<1>(acc, partials);
+}
+
+// ---------------------------------------------------------------- CG
+
+template <typename R>
+__global__ void k_jacobian_psi(Geo g, R kappa2, R eps, const R *eps_map, R H, const uint8_t *mt,
+                               const cplx<R> *psi, const R *abei, const R *ab, cplx<R> *G) {
+    const GeoR<R> q(g);
+    const R dxdy = q.dx * q.dy;
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
+        cplx<R> gj = {R(0), R(0)};
+        const Cells c = cells_of(g, mt, i, j);
+        if (c.mm || c.mp || c.pm || c.pp) {
+            const R f_mm = c.mm ? R(1) : R(0), f_mp = c.mp ? R(1) : R(0), f_pm = c.pm ? R(1) : R(0), f_pp = c.pp ? R(1) : R(0);
+            const R r_m0 = R(0.5) * (f_mm + f_mp), r_p0 = R(0.5) * (f_pm + f_pp);
+            const R r_0m = R(0.5) * (f_mm + f_pm), r_0p = R(0.5) * (f_mp + f_pp);
+            const R gw = R(0.25) * (r_m0 + r_p0 + r_0m + r_0p);
+            const cplx<R> p0 = ld(psi, n);
+            const R ee = eps_map ? eps_map[n] : eps;
+            const R p = p0.re * p0.re + p0.im * p0.im - ee;
+            const R k = R(2) * gw * p;
+            gj.re += k * p0.re;
+            gj.im += k * p0.im;
+            auto add = [&](R r, R id2, R ph, cplx<R> p1) {
+                R s, cs;
+                sincos_r(ph, &s, &cs);
+                const R tr = R(2) * (p0.re - (p1.re * cs + p1.im * s));
+                const R ti = R(2) * (p0.im - (p1.im * cs - p1.re * s));
+                const R w = r * id2;
+                gj.re += w * tr;
+                gj.im += w * ti;
+            };
+            if (c.mm || c.mp) add(r_m0, q.idx2, -q.dx * pv(abei, ab, (i - 1) + (long)g.Nxa * j), ld(psi, n - 1));
+            if (c.pm || c.pp) add(r_p0, q.idx2, q.dx * pv(abei, ab, i + (long)g.Nxa * j), ld(psi, n + 1));
+            if (c.mm || c.pm) add(r_0m, q.idy2, -q.dy * pv(abei, ab, g.Na + i + (long)g.Nxb * (j - 1)), ld(psi, n - g.Nx));
+            if (c.mp || c.pp) add(r_0p, q.idy2, q.dy * pv(abei, ab, g.Na + i + (long)g.Nxb * j), ld(psi, n + g.Nx));
+        }
+        cplx<R> o = ld(G, n);
+        o.re += dxdy * gj.re;
+        o.im += dxdy * gj.im;
+        st(G, n, o);
+    }
+}
+
+template <typename R>
+__global__ void k_jacobian_A(Geo g, R kappa2, R H, const uint8_t *mt, const cplx<R> *psi, const R *abei,
+                             const R *ab, R *G) {
+    const GeoR<R> q(g);
+    const R s2 = R(2) * q.dx * q.dy;
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
+        const cplx<R> p0 = ld(psi, n);
+        const Cells c = cells_of(g, mt, i, j);
+        if (i < g.Nxa && j < g.Nya) {
+            const long m = i + (long)g.Nxa * j;
+            R gj = curlcurl_a(g, q, H, abei, ab, i, j);
+            gj *= kappa2;
+            if (c.pm || c.pp) {
+                const R r = R(0.5) * ((c.pm ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
+                gj += -r * q.idx * js(p0, q.dx * pv(abei, ab, m), ld(psi, n + 1));
+            }
+            G[m] += s2 * gj;
+        }
+        if (i < g.Nxb && j < g.Nyb) {
+            const long m = g.Na + i + (long)g.Nxb * j;
+            R gj = curlcurl_b(g, q, H, abei, ab, i, j);
+            gj *= kappa2;
+            if (c.mp || c.pp) {
+                const R r = R(0.5) * ((c.mp ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
+                gj += -r * q.idy * js(p0, q.dy * pv(abei, ab, m), ld(psi, n + g.Nx));
+            }
+            G[m] += s2 * gj;
+        }
+    }
+}
+
+// Line-search coefficients. L = 5 (cg.h:314-474) or 17 (cg.h:477-731).
+template <typename R, int L>
+__global__ void k_coef(Geo g, R kappa2, R eps, R H, const uint8_t *mt, const cplx<R> *psi, const cplx<R> *dpsi,
+                       const R *abei, const R *ab, const R *dab, double *partials) {
+    const GeoR<R> q(g);
+    double acc[L];
+#pragma unroll
+    for (int k = 0; k < L; ++k) acc[k] = 0.0;
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
+        R C[L];
+#pragma unroll
+        for (int k = 0; k < L; ++k) C[k] = R(0);
+        const Cells c = cells_of(g, mt, i, j);
+        if (c.mm || c.mp || c.pm || c.pp) {
+            const R f_mm = c.mm ? R(1) : R(0), f_mp = c.mp ? R(1) : R(0), f_pm = c.pm ? R(1) : R(0), f_pp = c.pp ? R(1) : R(0);
+            const R r_m0 = R(0.5) * (f_mm + f_mp), r_p0 = R(0.5) * (f_pm + f_pp);
+            const R r_0m = R(0.5) * (f_mm + f_pm), r_0p = R(0.5) * (f_mp + f_pp);
+            const R gw = R(0.25) * (r_m0 + r_p0 + r_0m + r_0p);
+            const cplx<R> p0 = ld(psi, n), d0 = ld(dpsi, n);
+            const R p2 = p0.re * p0.re + p0.im * p0.im;
+            const R d2 = d0.re * d0.re + d0.im * d0.im;
+            const R tw = R(2) * (p0.re * d0.re + p0.im * d0.im);
+            const R k0 = gw * (R(0.5) * p2 - eps) * p2;
+            const R k1 = gw * tw * (p2 - eps);
+            const R k2 = gw * (-eps * d2 + R(0.5) * tw * tw + p2 * d2);
+            const R k3 = gw * tw * d2;
+            const R k4 = gw * R(0.5) * d2 * d2;
+            if (L == 5) { C[0] += k0; C[1] += k1; C[2] += k2; C[3] += k3; C[4] += k4; }
+            else { C[0] += k0; C[5] += k1; C[10] += k2; C[15 % L] += k3; C[16 % L] += k4; }
+            for (int lk = 0; lk < 2; ++lk) {
+                const bool on = lk == 0 ? (c.pm || c.pp) : (c.mp || c.pp);
+                if (!on) continue;
+                const R r = lk == 0 ? r_p0 : r_0p;
+                const R id2 = lk == 0 ? q.idx2 : q.idy2;
+                const R d = lk == 0 ? q.dx : q.dy;
+                const long k = lk == 0 ? i + (long)g.Nxa * j : g.Na + i + (long)g.Nxb * j;
+                const long nb = lk == 0 ? n + 1 : n + g.Nx;
+                R ph = R(0);
+                if (abei) ph += d * abei[k];
+                if (ab) ph += d * ab[k];
+                const cplx<R> p1 = ld(psi, nb), d1 = ld(dpsi, nb);
+                const cplx<R> zp = grad_c(p0, ph, p1), zd = grad_c(d0, ph, d1);
+                const R zr = zp.re * zd.re - (-zp.im) * zd.im;  // Re(conj(zp) zd)
+                const R w = r * id2;
+                if (L == 5) {
+                    C[2] += w * g_grad(d0, ph, d1);
+                    C[1] += w * R(2) * zr;
+                    C[0] += w * g_grad(p0, ph, p1);
+                } else {
+                    const R dph = d * dab[k], dph2 = dph * dph;
+                    C[0] += w * g_grad(p0, ph, p1);
+                    C[5] += w * R(2) * zr;
+                    C[10] += w * g_grad(d0, ph, d1);
+                    R s, cs;
+                    sincos_r(ph, &s, &cs);  // U(-ph) = (cos ph, sin ph)
+                    auto mulU = [&](cplx<R> a, cplx<R> b) -> cplx<R> {  // a * U(-ph) * conj(b)
+                        const cplx<R> t = {a.re * cs - a.im * s, a.re * s + a.im * cs};
+                        return {t.re * b.re - t.im * (-b.im), t.re * (-b.im) + t.im * b.re};
+                    };
+                    const cplx<R> z0 = mulU(p0, p1);
+                    const cplx<R> za = mulU(p0, d1), zb = mulU(d0, p1);
+                    const cplx<R> z1 = {za.re + zb.re, za.im + zb.im};
+                    const cplx<R> z2 = mulU(d0, d1);
+                    const cplx<R> zz[3] = {z0, z1, z2};
+#pragma unroll
+                    for (int m = 0; m < 3; ++m) {
+                        C[5 * m + 1] += w * R(2) * zz[m].im * dph;
+                        C[5 * m + 2] += w * zz[m].re * dph2;
+                        C[5 * m + 3] += -r * id2 / R(3) * zz[m].im * dph2 * dph;
+                        C[5 * m + 4] += -r * id2 / R(12) * zz[m].re * dph2 * dph2;
+                    }
+                }
+            }
+        }
+        if (kappa2 > R(0) && i < g.Nxc && j < g.Nyc) {
+            const long a00 = i + (long)g.Nxa * j, a0p = i + (long)g.Nxa * (j + 1);
+            const long b00 = g.Na + i + (long)g.Nxb * j, bp0 = g.Na + i + 1 + (long)g.Nxb * j;
+            if (L == 5) {
+                const R dB = q.idx * (ab[bp0] - ab[b00]) - q.idy * (ab[a0p] - ab[a00]) - H;
+                C[0] += kappa2 * dB * dB;
+            } else {
+                const R BH = q.idx * (pv(abei, ab, bp0) - pv(abei, ab, b00)) - q.idy * (pv(abei, ab, a0p) - pv(abei, ab, a00)) - H;
+                const R dB = q.idx * (dab[bp0] - dab[b00]) - q.idy * (dab[a0p] - dab[a00]);
+                C[0] += kappa2 * BH * BH;
+                C[1] += kappa2 * R(2) * BH * dB;
+                C[2] += kappa2 * dB * dB;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < L; ++k) acc[k] += double(C[k]);
+    }
+    block_store_partials<L>(acc, partials);
+}
+
+// --------------------------------------------------------------- vortices
+
+// np.mod(x, 2 pi) for float64 (fmod, shifted into [0, 2 pi)).
+__device__ __forceinline__ double np_mod_2pi(double x) {
+    const double b = 6.283185307179586;  // 2.0*np.pi
+    double m = fmod(x, b);
+    if (m != 0.0) {
+        if (m < 0.0) m += b;
+    } else {
+        m = 0.0;
+    }
+    return m;
+}
+
+constexpr int VREC = 14;
+
+template <typename R>
+__global__ void k_vortex_cells(Geo g, const cplx<R> *psi, const R *ab, const R *abi, double H, double *rec,
+                               int capacity, int *count) {
+    const double pi = 3.141592653589793;
+    const double dxdyH = g.dx * g.dy * H;
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.Nc; n += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(n % g.Nxc), j = (int)(n / g.Nxc);
+        const long q00 = i + (long)g.Nx * j;
+        const cplx<R> s00 = ld(psi, q00), sp0 = ld(psi, q00 + 1), spp = ld(psi, q00 + 1 + g.Nx), s0p = ld(psi, q00 + g.Nx);
+        const double t00 = atan2(double(s00.im), double(s00.re));
+        const double tp0 = atan2(double(sp0.im), double(sp0.re));
+        const double tpp = atan2(double(spp.im), double(spp.re));
+        const double t0p = atan2(double(s0p.im), double(s0p.re));
+        const long ka00 = i + (long)g.Nxa * j, ka0p = i + (long)g.Nxa * (j + 1);
+        const long kb00 = g.Na + i + (long)g.Nxb * j, kbp0 = g.Na + i + 1 + (long)g.Nxb * j;
+        const double a00 = double(ab[ka00]), a0p = double(ab[ka0p]), b00 = double(ab[kb00]), bp0 = double(ab[kbp0]);
+        const double v = -(0.5 / pi) * (np_mod_2pi(tp0 - t00 - g.dx * a00 + pi) + np_mod_2pi(tpp - tp0 - g.dy * bp0 + pi) +
+                                          np_mod_2pi(t0p - tpp + g.dx * a0p + pi) + np_mod_2pi(t00 - t0p + g.dy * b00 + pi) -
+                                          4.0 * pi + dxdyH);
+        if (fabs(v) > 0.5 && fabs(v - rint(v)) < 0.1) {
+            const int slot = atomicAdd(count, 1);
+            if (slot < capacity) {
+                double *o = rec + (long)slot * VREC;
+                o[0] = double(n);
+                o[1] = v;
+                o[2] = double(s00.re); o[3] = double(s00.im);
+                o[4] = double(sp0.re); o[5] = double(sp0.im);
+                o[6] = double(spp.re); o[7] = double(spp.im);
+                o[8] = double(s0p.re); o[9] = double(s0p.im);
+                o[10] = a00 + (abi ? double(abi[ka00]) : 0.0);
+                o[11] = a0p + (abi ? double(abi[ka0p]) : 0.0);
+                o[12] = b00 + (abi ? double(abi[kb00]) : 0.0);
+                o[13] = bp0 + (abi ? double(abi[kbp0]) : 0.0);
+            }
+        }
+    }
+}
+
+template <typename R>
+__global__ void k_phase_lock(cplx<R> *psi, int lockN, const int *ns) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < lockN) {
+        const int n = ns[l];
+        cplx<R> p = ld(psi, n);
+        p.re = hypot(p.re, p.im);
+        p.im = R(0);
+        st(psi, n, p);
+    }
+}
+
+}  // namespace glx

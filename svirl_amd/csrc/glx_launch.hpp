@@ -1,0 +1,68 @@
+// Host-side launchers shared between the precision translation units
+// (glx_f64.hip, glx_f32.hip) and the C API (glx_api.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/glx.h"
+
+namespace glx {
+
+// Library-owned device scratch handed to every launcher.
+struct Env {
+    hipStream_t stream;
+    double *partials;        // RED_MAX_BLOCKS * 32 doubles
+    double *result;          // 64 doubles
+    unsigned long long *grp; // barrier group counters (8)
+    unsigned long long *top; // barrier top counter
+    int *err;                // barrier error flag
+    int *slots;              // residual ring
+    int *state;              // persistent-kernel state
+    int *sweeps;             // per-step sweep counts (capacity sweeps_cap steps)
+    int sweeps_cap;
+    int num_cus;
+    hipEvent_t kev0, kev1;   // around the persistent TD kernel
+    float *last_kernel_ms;   // host slot written by td_run
+};
+
+template <typename R> struct L {
+    static int psi_sweep(const Env &e, glx_grid g, double dt, double eps, const void *eps_map, const uint8_t *mt,
+                         const void *ab, void *psi_rhs, const void *psi, void *psi_next, double langevin_c,
+                         uint32_t jstep, uint32_t rand_t, double stop_eps, int32_t *r2max);
+    static int a_sweep(const Env &e, glx_grid g, double dt, double kappa2, double rho, double H, const uint8_t *mt,
+                       const void *psi, const void *src, void *ab_rhs, const void *ab, void *ab_next,
+                       double langevin_c, uint32_t jstep, uint32_t rand_t, double stop_eps, int32_t *r2max);
+    static int td_run(const Env &e, glx_grid g, const glx_td_params *p, const void *eps_map, const uint8_t *mt,
+                      void *const P[3], int32_t *pc, void *const Q[3], int32_t *qc, void *rhs_psi, void *rhs_ab,
+                      int32_t nsteps, int32_t *sweeps, int32_t wg, int32_t wg_size);
+    static int td_max_grid(const Env &e, int32_t wg_size, int32_t *max_wg);
+    static int phase_lock(const Env &e, void *psi, int32_t n, const int32_t *ns);
+    static int magnetic_field(const Env &e, glx_grid g, const void *abei, const void *ab, void *B);
+    static int current_density(const Env &e, glx_grid g, double kappa2, double H, const void *abei, const void *ab,
+                               void *J);
+    static int supercurrent_density(const Env &e, glx_grid g, const uint8_t *mt, const void *psi, const void *abei,
+                                    const void *ab, void *Js);
+    static int free_energy(const Env &e, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
+                           const uint8_t *mt, const void *psi, const void *abei, const void *ab, double *F);
+    static int vortex_cells(const Env &e, glx_grid g, const void *psi, const void *ab, const void *abi, double H,
+                            double *rec, int32_t capacity, int32_t *count);
+    static int sum(const Env &e, const void *x, int64_t n, int32_t bs, double *out);
+    static int sum_v(const Env &e, const void *x, int64_t nv, int32_t ne, int32_t bs, double *out);
+    static int pr_sums(const Env &e, const void *x, const void *y, int64_t n, double *out);
+    static int jacobian_psi(const Env &e, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
+                            const uint8_t *mt, const void *psi, const void *abei, const void *ab, void *G);
+    static int jacobian_A(const Env &e, glx_grid g, double kappa2, double H, const uint8_t *mt, const void *psi,
+                          const void *abei, const void *ab, void *G);
+    static int coef_psi(const Env &e, glx_grid g, double kappa2, double eps, double H, const uint8_t *mt,
+                        const void *psi, const void *dpsi, const void *abei, const void *ab, double *c5);
+    static int coef(const Env &e, glx_grid g, double kappa2, double eps, double H, const uint8_t *mt, const void *psi,
+                    const void *dpsi, const void *abei, const void *ab, const void *dab, double *c17);
+    static int axpy(const Env &e, const void *x, const void *y, void *z, double alpha, int64_t n, int minus);
+};
+
+// Error reporting implemented in glx_api.hip.
+int set_error(int code, const char *fmt, ...);
+int check_hip(hipError_t st, const char *what);
+
+}  // namespace glx

@@ -1,0 +1,368 @@
+// Launcher bodies, instantiated once per precision by glx_f64.hip and
+// glx_f32.hip.
+#pragma once
+
+#include <algorithm>
+
+#include "glx_fields.hpp"
+#include "glx_launch.hpp"
+#include "glx_td.hpp"
+
+namespace glx {
+
+inline bool grid_ok(const glx_grid &g) { return g.Nx >= 2 && g.Ny >= 2 && g.dx > 0.0 && g.dy > 0.0; }
+
+inline int grid_blocks(long work, int block = RED_BLOCK, int cap = 65536) {
+    long b = (work + block - 1) / block;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (int)b;
+}
+
+#define GLX_LAUNCH_CHECK(what) \
+    do { hipError_t _e = hipGetLastError(); if (_e != hipSuccess) return check_hip(_e, what); } while (0)
+
+// Copy `n` doubles of e.result to the host after the stream drains.
+inline int fetch_result(const Env &e, double *out, int n) {
+    hipError_t st = hipMemcpyAsync(out, e.result, sizeof(double) * n, hipMemcpyDeviceToHost, e.stream);
+    if (st != hipSuccess) return check_hip(st, "hipMemcpyAsync(result)");
+    return check_hip(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
+}
+
+// Waves used by a single-sweep launch: about 16 rows per wave.
+inline long sweep_waves(const glx_grid &g) {
+    const long T = (long)nstrips_of(g.Nx) * g.Ny;
+    long W = (T + 15) / 16;
+    return W < 1 ? 1 : W;
+}
+
+template <typename R>
+static PsiSweepArgs<R> make_psi_args(const glx_grid &g, double dt, double eps, const void *eps_map,
+                                     const uint8_t *mt, const void *ab, double stop_eps, double langevin_c) {
+    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    PsiSweepArgs<R> A{};
+    A.Nx = g.Nx; A.Ny = g.Ny;
+    A.dx = R(g.dx); A.dy = R(g.dy); A.idx2 = R(G.idx2); A.idy2 = R(G.idy2);
+    A.dt = R(dt); A.eps = R(eps);
+    A.eps_map = (const R *)eps_map;
+    A.mt = mt;
+    A.ab = (const R *)ab;
+    A.langevin_c = R(langevin_c);
+    A.stop_eps = double(R(stop_eps));
+    return A;
+}
+
+template <typename R>
+static ASweepArgs<R> make_a_args(const glx_grid &g, double dt, double kappa2, double rho, double H,
+                                 const uint8_t *mt, double stop_eps, double langevin_c) {
+    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    ASweepArgs<R> A{};
+    A.Nx = g.Nx; A.Ny = g.Ny;
+    A.dx = R(g.dx); A.dy = R(g.dy); A.idx = R(G.idx); A.idy = R(G.idy);
+    A.idx2 = R(G.idx2); A.idy2 = R(G.idy2); A.idxy = R(G.idxy);
+    // scalar constants in the reference's expression order (td.h:419, 457, 484, 503, 530)
+    const double dt_rho = double(R(dt)) * double(R(rho));
+    const double dt_rho_R = double(R(dt_rho));
+    const double dt_rho_kappa2 = dt_rho_R * double(R(kappa2));
+    A.dt_rho = R(dt_rho);
+    A.dt_rho_kappa2 = R(dt_rho_kappa2);
+    const double k2 = double(R(kappa2)), HH = double(R(H));
+    A.rh_a = R(2.0 * k2 * HH * double(A.idy));
+    A.rh_b = R(2.0 * k2 * HH * double(A.idx));
+    const double drk = double(A.dt_rho_kappa2);
+    A.den_a = R(1.0 + 2.0 * drk * double(A.idy2));
+    A.den_b = R(1.0 + 2.0 * drk * double(A.idx2));
+    A.mt = mt;
+    A.langevin_c = R(langevin_c);
+    A.stop_eps = double(R(stop_eps));
+    return A;
+}
+
+template <typename R>
+int L<R>::psi_sweep(const Env &e, glx_grid g, double dt, double eps, const void *eps_map, const uint8_t *mt,
+                    const void *ab, void *psi_rhs, const void *psi, void *psi_next, double langevin_c,
+                    uint32_t jstep, uint32_t rand_t, double stop_eps, int32_t *r2max) {
+    if (!grid_ok(g) || !psi_rhs || !psi || !psi_next || !r2max || !ab || stop_eps <= 0.0)
+        return set_error(GLX_E_ARG, "glx_td_psi_sweep: invalid arguments");
+    PsiSweepArgs<R> A = make_psi_args<R>(g, dt, eps, eps_map, mt, ab, stop_eps, langevin_c);
+    A.psi = (const cplx<R> *)psi;
+    A.psi_next = (cplx<R> *)psi_next;
+    A.rhs = (const cplx<R> *)psi_rhs;
+    A.rhs_out = (langevin_c > 1.0e-32 && jstep == 0) ? (cplx<R> *)psi_rhs : nullptr;
+    A.jstep = jstep;
+    A.rand_t = rand_t;
+    const long W = sweep_waves(g);
+    hipLaunchKernelGGL(k_psi_sweep<R>, dim3((unsigned)((W + 3) / 4)), dim3(256), 0, e.stream, A, (int *)r2max, W);
+    GLX_LAUNCH_CHECK("k_psi_sweep");
+    return GLX_OK;
+}
+
+template <typename R>
+int L<R>::a_sweep(const Env &e, glx_grid g, double dt, double kappa2, double rho, double H, const uint8_t *mt,
+                  const void *psi, const void *src, void *ab_rhs, const void *ab, void *ab_next, double langevin_c,
+                  uint32_t jstep, uint32_t rand_t, double stop_eps, int32_t *r2max) {
+    if (!grid_ok(g) || !psi || !src || !ab_rhs || !ab || !ab_next || !r2max || stop_eps <= 0.0)
+        return set_error(GLX_E_ARG, "glx_td_a_sweep: invalid arguments");
+    if (ab == ab_next) return set_error(GLX_E_ARG, "glx_td_a_sweep: ab and ab_next must differ");
+    ASweepArgs<R> A = make_a_args<R>(g, dt, kappa2, rho, H, mt, stop_eps, langevin_c);
+    A.psi = (const cplx<R> *)psi;
+    A.src = (const R *)src;
+    A.rhs = (const R *)ab_rhs;
+    A.rhs_out = (langevin_c > 1.0e-32 && jstep == 0) ? (R *)ab_rhs : nullptr;
+    A.ab = (const R *)ab;
+    A.ab_next = (R *)ab_next;
+    A.jstep = jstep;
+    A.rand_t = rand_t;
+    const long W = sweep_waves(g);
+    hipLaunchKernelGGL(k_a_sweep<R>, dim3((unsigned)((W + 3) / 4)), dim3(256), 0, e.stream, A, (int *)r2max, W);
+    GLX_LAUNCH_CHECK("k_a_sweep");
+    return GLX_OK;
+}
+
+template <typename R> int L<R>::td_max_grid(const Env &e, int32_t wg_size, int32_t *max_wg) {
+    if (wg_size != 256 || !max_wg) return set_error(GLX_E_ARG, "glx_td_max_grid: wg_size must be 256");
+    int per_cu = 0;
+    hipError_t st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_td_run<R>, 256, 0);
+    if (st != hipSuccess) return check_hip(st, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    *max_wg = per_cu * e.num_cus;
+    return GLX_OK;
+}
+
+template <typename R>
+int L<R>::td_run(const Env &e, glx_grid g, const glx_td_params *p, const void *eps_map, const uint8_t *mt,
+                 void *const P[3], int32_t *pc, void *const Q[3], int32_t *qc, void *rhs_psi, void *rhs_ab,
+                 int32_t nsteps, int32_t *sweeps, int32_t wg, int32_t wg_size) {
+    if (!grid_ok(g) || !p || !P || !pc || !sweeps || nsteps < 0 || *pc < 0 || *pc > 2)
+        return set_error(GLX_E_ARG, "glx_td_run: invalid arguments");
+    if (!P[0] || !P[1] || !P[2]) return set_error(GLX_E_ARG, "glx_td_run: psi buffers");
+    if (!Q || !qc || *qc < 0 || *qc > 2 || !Q[0] || !Q[1] || !Q[2])
+        return set_error(GLX_E_ARG, "glx_td_run: vector potential buffers");
+    if (p->max_sweeps < 1 || p->stop_psi <= 0.0 || p->stop_A <= 0.0 || p->eqn < 0 || p->eqn > 2)
+        return set_error(GLX_E_ARG, "glx_td_run: invalid parameters");
+    if (p->langevin_psi > 1.0e-32 && !rhs_psi) return set_error(GLX_E_ARG, "glx_td_run: Langevin psi needs rhs_psi");
+    if (p->solveA && p->langevin_A > 1.0e-32 && !rhs_ab) return set_error(GLX_E_ARG, "glx_td_run: Langevin A needs rhs_ab");
+    if (nsteps == 0) return GLX_OK;
+    if (nsteps > e.sweeps_cap) return set_error(GLX_E_ARG, "glx_td_run: nsteps %d exceeds %d per launch", nsteps, e.sweeps_cap);
+    if (wg_size == 0) wg_size = 256;
+    int32_t maxg = 0;
+    int rc = td_max_grid(e, wg_size, &maxg);
+    if (rc) return rc;
+    if (wg <= 0) wg = maxg;
+    if (wg > maxg) return set_error(GLX_E_ARG, "glx_td_run: grid %d exceeds co-resident limit %d", wg, maxg);
+
+    TDRunArgs<R> T{};
+    T.psi = make_psi_args<R>(g, p->dt, p->eps, eps_map, mt, P[*pc], p->stop_psi, p->langevin_psi);
+    T.A = make_a_args<R>(g, p->dt, p->kappa2, p->rho, p->H, mt, p->stop_A, p->langevin_A);
+    for (int k = 0; k < 3; ++k) { T.P[k] = (cplx<R> *)P[k]; T.Q[k] = (R *)Q[k]; }
+    T.rhs_psi = p->langevin_psi > 1.0e-32 ? (cplx<R> *)rhs_psi : nullptr;
+    T.rhs_ab = (p->solveA && p->langevin_A > 1.0e-32) ? (R *)rhs_ab : nullptr;
+    T.psi.rand_t = p->rand_t;
+    T.psi_cur = *pc;
+    T.ab_cur = *qc;
+    T.solveA = p->solveA;
+    T.eqn = p->eqn;
+    T.max_sweeps = p->max_sweeps;
+    T.nsteps = nsteps;
+    T.sc.gs.grp = e.grp;
+    T.sc.gs.top = e.top;
+    T.sc.gs.err = e.err;
+    T.sc.slots = e.slots;
+    T.sc.state = e.state;
+    T.sc.sweeps = e.sweeps;
+    hipLaunchKernelGGL(k_td_reset, dim3(1), dim3(256), 0, e.stream, T.sc);
+    GLX_LAUNCH_CHECK("k_td_reset");
+    hipError_t est = hipEventRecord(e.kev0, e.stream);
+    if (est != hipSuccess) return check_hip(est, "hipEventRecord");
+    hipLaunchKernelGGL(k_td_run<R>, dim3((unsigned)wg), dim3(256), 0, e.stream, T);
+    GLX_LAUNCH_CHECK("k_td_run");
+    est = hipEventRecord(e.kev1, e.stream);
+    if (est != hipSuccess) return check_hip(est, "hipEventRecord");
+    int host_state[4] = {0, 0, 0, 0};
+    int host_err = 0;
+    hipError_t st = hipMemcpyAsync(host_state, e.state, sizeof(int) * 3, hipMemcpyDeviceToHost, e.stream);
+    if (st == hipSuccess) st = hipMemcpyAsync(&host_err, e.err, sizeof(int), hipMemcpyDeviceToHost, e.stream);
+    if (st == hipSuccess) st = hipMemcpyAsync(sweeps, e.sweeps, sizeof(int) * 2 * nsteps, hipMemcpyDeviceToHost, e.stream);
+    if (st == hipSuccess) st = hipStreamSynchronize(e.stream);
+    if (st != hipSuccess) return check_hip(st, "glx_td_run readback");
+    float kms = 0.f;
+    if (hipEventElapsedTime(&kms, e.kev0, e.kev1) == hipSuccess && e.last_kernel_ms) *e.last_kernel_ms = kms;
+    if (host_err) return set_error(GLX_E_TIMEOUT, "glx_td_run: grid barrier timed out after %d of %d steps", host_state[2], nsteps);
+    if (host_state[2] != nsteps) return set_error(GLX_E_HIP, "glx_td_run: %d of %d steps completed", host_state[2], nsteps);
+    *pc = host_state[0];
+    *qc = host_state[1];
+    return GLX_OK;
+}
+
+template <typename R> int L<R>::phase_lock(const Env &e, void *psi, int32_t n, const int32_t *ns) {
+    if (n <= 0) return GLX_OK;
+    if (!psi || !ns) return set_error(GLX_E_ARG, "glx_td_phase_lock: NULL pointer");
+    hipLaunchKernelGGL(k_phase_lock<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e.stream, (cplx<R> *)psi, n, ns);
+    GLX_LAUNCH_CHECK("k_phase_lock");
+    return GLX_OK;
+}
+
+template <typename R>
+int L<R>::magnetic_field(const Env &e, glx_grid g, const void *abei, const void *ab, void *B) {
+    if (!grid_ok(g) || !B) return set_error(GLX_E_ARG, "glx_magnetic_field: invalid arguments");
+    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    hipLaunchKernelGGL(k_magnetic_field<R>, dim3(grid_blocks(G.Nc)), dim3(256), 0, e.stream, G, (const R *)abei, (const R *)ab, (R *)B);
+    GLX_LAUNCH_CHECK("k_magnetic_field");
+    return GLX_OK;
+}
+
+template <typename R>
+int L<R>::current_density(const Env &e, glx_grid g, double kappa2, double H, const void *abei, const void *ab, void *J) {
+    if (!grid_ok(g) || !J) return set_error(GLX_E_ARG, "glx_current_density: invalid arguments");
+    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    hipLaunchKernelGGL(k_current_density<R>, dim3(grid_blocks(G.N)), dim3(256), 0, e.stream, G, R(kappa2), R(H),
+                       (const R *)abei, (const R *)ab, (R *)J);
+    GLX_LAUNCH_CHECK("k_current_density");
+    return GLX_OK;
+}
+
+template <typename R>
+int L<R>::supercurrent_density(const Env &e, glx_grid g, const uint8_t *mt, const void *psi, const void *abei,
+                               const void *ab, void *Js) {
+    if (!grid_ok(g) || !psi || !Js) return set_error(GLX_E_ARG, "glx_supercurrent_density: invalid arguments");
+    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    hipLaunchKernelGGL(k_supercurrent_density<R>, dim3(grid_blocks(G.N)), dim3(256), 0, e.stream, G, mt,
+                       (const cplx<R> *)psi, (const R *)abei, (const R *)ab, (R *)Js);
+    GLX_LAUNCH_CHECK("k_supercurrent_density");
+    return GLX_OK;
+}
+
+template <typename R>
+int L<R>::free_energy(const Env &e, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
+                      const uint8_t *mt, const void *psi, const void *abei, const void *ab, double *F) {
+    if (!grid_ok(g) || !psi || !F) return set_error(GLX_E_ARG, "glx_free_energy: invalid arguments");
+    if (kappa2 > 0.0 && !abei && !ab) return set_error(GLX_E_ARG, "glx_free_energy: finite kappa needs a vector potential");
+    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    const int nb = grid_blocks(G.N, RED_BLOCK, RED_MAX_BLOCKS);
+    hipLaunchKernelGGL(k_free_energy<R>, dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, R(kappa2), R(eps), (const R *)eps_map,
+                       R(H), mt, (const cplx<R> *)psi, (const R *)abei, (const R *)ab, e.partials);
+    GLX_LAUNCH_CHECK("k_free_energy");
+    const double dxdy = double(R(g.dx)) * double(R(g.dy));
+    hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, 1, dxdy, e.result);
+    GLX_LAUNCH_CHECK("k_finish_partials");
+    return fetch_result(e, F, 1);
+}
+
+template <typename R>
+int L<R>::vortex_cells(const Env &e, glx_grid g, const void *psi, const void *ab, const void *abi, double H,
+                       double *rec, int32_t capacity, int32_t *count) {
+    if (!grid_ok(g) || !psi || !ab || !count || capacity < 0 || (capacity > 0 && !rec))
+        return set_error(GLX_E_ARG, "glx_vortex_cells: invalid arguments");
+    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    int *dcount = (int *)e.result;
+    hipError_t st = hipMemsetAsync(dcount, 0, sizeof(int), e.stream);
+    if (st != hipSuccess) return check_hip(st, "hipMemsetAsync(count)");
+    hipLaunchKernelGGL(k_vortex_cells<R>, dim3(grid_blocks(G.Nc)), dim3(256), 0, e.stream, G, (const cplx<R> *)psi,
+                       (const R *)ab, (const R *)abi, double(R(H)), rec, (int)capacity, dcount);
+    GLX_LAUNCH_CHECK("k_vortex_cells");
+    int c = 0;
+    st = hipMemcpyAsync(&c, dcount, sizeof(int), hipMemcpyDeviceToHost, e.stream);
+    if (st == hipSuccess) st = hipStreamSynchronize(e.stream);
+    if (st != hipSuccess) return check_hip(st, "glx_vortex_cells readback");
+    *count = c;
+    if (c > capacity) return set_error(GLX_E_CAPACITY, "glx_vortex_cells: %d cells flagged, capacity %d", c, capacity);
+    return GLX_OK;
+}
+
+template <typename R> int L<R>::sum(const Env &e, const void *x, int64_t n, int32_t bs, double *out) {
+    if (!x || !out || n < 0) return set_error(GLX_E_ARG, "glx_sum: invalid arguments");
+    if (bs == 0) bs = RED_BLOCK;
+    if (bs != RED_BLOCK && (bs % 64 != 0 || bs > 1024 || bs < 64))
+        return set_error(GLX_E_ARG, "glx_sum: block_size must be a multiple of 64 in [64, 1024]");
+    const int nb = grid_blocks(n, RED_BLOCK, RED_MAX_BLOCKS);
+    hipLaunchKernelGGL(k_sum<R>, dim3(nb), dim3(RED_BLOCK), 0, e.stream, (const R *)x, (long)n, e.partials);
+    GLX_LAUNCH_CHECK("k_sum");
+    hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, 1, 1.0, e.result);
+    GLX_LAUNCH_CHECK("k_finish_partials");
+    return fetch_result(e, out, 1);
+}
+
+template <typename R> int L<R>::sum_v(const Env &e, const void *x, int64_t nv, int32_t ne, int32_t bs, double *out) {
+    if (!x || !out || nv < 0 || ne < 1 || ne > 32) return set_error(GLX_E_ARG, "glx_sum_v: invalid arguments");
+    if (bs != 0 && (bs % 64 != 0 || bs > 1024 || bs < 64))
+        return set_error(GLX_E_ARG, "glx_sum_v: block_size must be a multiple of 64 in [64, 1024]");
+    const int nb = grid_blocks(nv, RED_BLOCK, RED_MAX_BLOCKS);
+    hipLaunchKernelGGL(k_sum_v<R>, dim3(nb), dim3(RED_BLOCK), 0, e.stream, (const R *)x, (long)nv, (int)ne, e.partials);
+    GLX_LAUNCH_CHECK("k_sum_v");
+    hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, (int)ne, 1.0, e.result);
+    GLX_LAUNCH_CHECK("k_finish_partials");
+    return fetch_result(e, out, ne);
+}
+
+template <typename R> int L<R>::pr_sums(const Env &e, const void *x, const void *y, int64_t n, double *out) {
+    if (!x || !y || !out || n < 0) return set_error(GLX_E_ARG, "glx_pr_sums: invalid arguments");
+    const int nb = grid_blocks(n, RED_BLOCK, RED_MAX_BLOCKS);
+    hipLaunchKernelGGL(k_pr_sums<R>, dim3(nb), dim3(RED_BLOCK), 0, e.stream, (const R *)x, (const R *)y, (long)n, e.partials);
+    GLX_LAUNCH_CHECK("k_pr_sums");
+    hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, 2, 1.0, e.result);
+    GLX_LAUNCH_CHECK("k_finish_partials");
+    return fetch_result(e, out, 2);
+}
+
+template <typename R>
+int L<R>::jacobian_psi(const Env &e, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
+                       const uint8_t *mt, const void *psi, const void *abei, const void *ab, void *Gp) {
+    if (!grid_ok(g) || !psi || !Gp) return set_error(GLX_E_ARG, "glx_cg_jacobian_psi: invalid arguments");
+    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    hipLaunchKernelGGL(k_jacobian_psi<R>, dim3(grid_blocks(G.N)), dim3(256), 0, e.stream, G, R(kappa2), R(eps),
+                       (const R *)eps_map, R(H), mt, (const cplx<R> *)psi, (const R *)abei, (const R *)ab, (cplx<R> *)Gp);
+    GLX_LAUNCH_CHECK("k_jacobian_psi");
+    return GLX_OK;
+}
+
+template <typename R>
+int L<R>::jacobian_A(const Env &e, glx_grid g, double kappa2, double H, const uint8_t *mt, const void *psi,
+                     const void *abei, const void *ab, void *Gp) {
+    if (!grid_ok(g) || !psi || !Gp) return set_error(GLX_E_ARG, "glx_cg_jacobian_A: invalid arguments");
+    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    hipLaunchKernelGGL(k_jacobian_A<R>, dim3(grid_blocks(G.N)), dim3(256), 0, e.stream, G, R(kappa2), R(H), mt,
+                       (const cplx<R> *)psi, (const R *)abei, (const R *)ab, (R *)Gp);
+    GLX_LAUNCH_CHECK("k_jacobian_A");
+    return GLX_OK;
+}
+
+template <typename R>
+int L<R>::coef_psi(const Env &e, glx_grid g, double kappa2, double eps, double H, const uint8_t *mt, const void *psi,
+                   const void *dpsi, const void *abei, const void *ab, double *c5) {
+    if (!grid_ok(g) || !psi || !dpsi || !c5) return set_error(GLX_E_ARG, "glx_cg_coef_psi: invalid arguments");
+    if (kappa2 > 0.0 && !ab) return set_error(GLX_E_ARG, "glx_cg_coef_psi: finite kappa needs ab");
+    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    const int nb = grid_blocks(G.N, RED_BLOCK, RED_MAX_BLOCKS);
+    hipLaunchKernelGGL((k_coef<R, 5>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, R(kappa2), R(eps), R(H), mt,
+                       (const cplx<R> *)psi, (const cplx<R> *)dpsi, (const R *)abei, (const R *)ab, (const R *)nullptr, e.partials);
+    GLX_LAUNCH_CHECK("k_coef5");
+    const double dxdy = double(R(g.dx)) * double(R(g.dy));
+    hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, 5, dxdy, e.result);
+    GLX_LAUNCH_CHECK("k_finish_partials");
+    return fetch_result(e, c5, 5);
+}
+
+template <typename R>
+int L<R>::coef(const Env &e, glx_grid g, double kappa2, double eps, double H, const uint8_t *mt, const void *psi,
+               const void *dpsi, const void *abei, const void *ab, const void *dab, double *c17) {
+    if (!grid_ok(g) || !psi || !dpsi || !dab || !c17) return set_error(GLX_E_ARG, "glx_cg_coef: invalid arguments");
+    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    const int nb = grid_blocks(G.N, RED_BLOCK, RED_MAX_BLOCKS);
+    hipLaunchKernelGGL((k_coef<R, 17>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, R(kappa2), R(eps), R(H), mt,
+                       (const cplx<R> *)psi, (const cplx<R> *)dpsi, (const R *)abei, (const R *)ab, (const R *)dab, e.partials);
+    GLX_LAUNCH_CHECK("k_coef17");
+    const double dxdy = double(R(g.dx)) * double(R(g.dy));
+    hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, 17, dxdy, e.result);
+    GLX_LAUNCH_CHECK("k_finish_partials");
+    return fetch_result(e, c17, 17);
+}
+
+template <typename R>
+int L<R>::axpy(const Env &e, const void *x, const void *y, void *z, double alpha, int64_t n, int minus) {
+    if (!x || !y || !z || n < 0) return set_error(GLX_E_ARG, "glx_axpy: invalid arguments");
+    if (n == 0) return GLX_OK;
+    hipLaunchKernelGGL(k_axpy<R>, dim3(grid_blocks(n)), dim3(256), 0, e.stream, (const R *)x, (const R *)y, (R *)z, R(alpha), (long)n, minus);
+    GLX_LAUNCH_CHECK("k_axpy");
+    return GLX_OK;
+}
+
+}  // namespace glx

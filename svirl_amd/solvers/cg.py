@@ -1,0 +1,218 @@
+"""Free-energy minimisation by non-linear conjugate gradients with
+Polak-Ribiere beta and an exact polynomial line search
+(svirl/solvers/cg.py), on libglx kernels.
+
+Per iteration: gradients (GPU), beta from two FP64 device reductions, the
+search direction update (GPU), the line-search polynomial coefficients
+(GPU, FP64 accumulation), the root / BFGS solve of the polynomial on the
+host exactly as the reference does it, the update (GPU) and the energy
+(GPU). Directions, previous gradients and betas persist across cg() calls
+as in the reference (cg.py:54-71).
+"""
+import ctypes
+
+import numpy as np
+import scipy.optimize
+
+from svirl_amd import _native as nat
+from svirl_amd._native import DeviceArray
+
+
+class CG:
+    def __init__(self, par, mesh, _vars, params, observables):
+        self.par = par
+        self.mesh = mesh
+        self.vars = _vars
+        self.params = params
+        self.fixed_vortices = params.fixed_vortices
+        self.observables = observables
+        self.cfg = cfg = par.cfg
+        self.__convergence_rtol = cfg.convergence_rtol
+        self.__reduction_vector_length = 17 if self.params.solveA else 5
+        cd = cfg.dtype_complex
+        self.__gdir_psi = DeviceArray.zeros(cfg.N, cd)
+        self.__gjac_psi = DeviceArray.zeros(cfg.N, cd)
+        self.__gjac_psi_prev = DeviceArray.zeros(cfg.N, cd)
+        self._beta_psi = cfg.dtype(0.0)
+        self._beta_A = cfg.dtype(0.0)
+        if self.params.solveA:
+            self.__gdir_A = DeviceArray.zeros(cfg.Nab, cfg.dtype)
+            self.__gjac_A = DeviceArray.zeros(cfg.Nab, cfg.dtype)
+            self.__gjac_A_prev = DeviceArray.zeros(cfg.Nab, cfg.dtype)
+        self.__c = np.zeros((5, 5), dtype=cfg.dtype) if self.params.solveA else np.zeros(5, dtype=cfg.dtype)
+        self.cg_energies = []
+
+    def _grid(self):
+        c = self.cfg
+        return nat.Grid(int(c.Nx), int(c.Ny), float(c.dx), float(c.dy))
+
+    def _args(self):
+        p = self.params
+        return dict(k2=float(p.gl_parameter_squared_h()), eps=float(p.linear_coefficient_scalar_h()),
+                    epsmap=nat.ptr(p.linear_coefficient_h()), H=float(p.homogeneous_external_field),
+                    mt=nat.ptr(self.mesh.material_tiling_h()), abei=nat.ptr(p.external_irregular_vector_potential_h()),
+                    ab=nat.ptr(self.vars.vector_potential_h()), psi=self.vars.order_parameter_h().ptr)
+
+    def _check_supported(self):
+        if self.fixed_vortices._vpi is not None or self.fixed_vortices._phase_lock_ns is not None:
+            raise NotImplementedError("fixed vortices / phase lock are not supported by the GPU CG path")
+
+    # -- kernels exposed like the reference (used by its tests) -----------------------
+    @property
+    def _free_energy_jacobian_psi(self):
+        """dF/d(Re psi, Im psi) * dxdy as a flat complex device array
+        (cg.py:112-146)."""
+        self.vars._psi.sync()
+        self.vars._vp.sync()
+        a = self._args()
+        self.__gjac_psi.fill(0)
+        nat.call("glx_cg_jacobian_psi", self.par.prec, self._grid(), a['k2'], a['eps'], a['epsmap'], a['H'], a['mt'],
+                 a['psi'], a['abei'], a['ab'], self.__gjac_psi.ptr)
+        return self.__gjac_psi
+
+    @property
+    def _free_energy_jacobian_A(self):
+        """dF/dA (boundary links carry the reference's factor 2 on the field
+        part) as a flat real device array [a; b] (cg.py:149-181)."""
+        if not self.params.solveA:
+            return None
+        self.vars._psi.sync()
+        self.vars._vp.sync()
+        a = self._args()
+        self.__gjac_A.fill(0)
+        nat.call("glx_cg_jacobian_A", self.par.prec, self._grid(), a['k2'], a['H'], a['mt'], a['psi'], a['abei'],
+                 a['ab'], self.__gjac_A.ptr)
+        return self.__gjac_A
+
+    def _free_energy_conjgrad_coef_psi(self, gdir_psi):
+        """c0..c4 of F(psi + alpha dpsi) (cg.py:184-224)."""
+        self.vars._psi.sync()
+        self.vars._vp.sync()
+        a = self._args()
+        out = (ctypes.c_double * 5)()
+        nat.call("glx_cg_coef_psi", self.par.prec, self._grid(), a['k2'], a['eps'], a['H'], a['mt'], a['psi'],
+                 gdir_psi.ptr, a['abei'], a['ab'], out)
+        self.__c = np.zeros(5, dtype=self.cfg.dtype) if self.__c.ndim != 1 else self.__c
+        self.__c[:] = out[:]
+        return self.__c
+
+    def _cg_alpha_psi_min(self):
+        """Smallest real root >= 0 of the cubic derivative (cg.py:227-235)."""
+        c = self.__c
+        am = np.polynomial.polynomial.polyroots([c[1], 2.0 * c[2], 3.0 * c[3], 4.0 * c[4]])
+        am = am[np.isclose(am.imag, 0)].real
+        am = am[am >= 0]
+        return np.min(am)
+
+    def _free_energy_conjgrad_coef(self, gdir_psi, gdir_A):
+        """The 17 coefficients packed into a 5x5 polyval2d matrix
+        (cg.py:325-375)."""
+        self.vars._psi.sync()
+        self.vars._vp.sync()
+        a = self._args()
+        out = (ctypes.c_double * 17)()
+        nat.call("glx_cg_coef", self.par.prec, self._grid(), a['k2'], a['eps'], a['H'], a['mt'], a['psi'],
+                 gdir_psi.ptr, a['abei'], a['ab'], gdir_A.ptr, out)
+        g = np.array(out[:])
+        c = np.zeros((5, 5), dtype=self.cfg.dtype)
+        c[0, :] = g[0:5]
+        c[1, :] = g[5:10]
+        c[2, :] = g[10:15]
+        c[3, 0] = g[15]
+        c[4, 0] = g[16]
+        self.__c = c
+        return self.__c
+
+    def _cg_alpha_min(self, alpha0=[0.0, 0.0], tol=1e-8):
+        """scipy BFGS on the (alpha_psi, alpha_A) polynomial (cg.py:378-419)."""
+        c = self.__c
+        cj0 = np.polynomial.polynomial.polyder(c, axis=0)
+        cj1 = np.polynomial.polynomial.polyder(c, axis=1)
+
+        def f(alpha):
+            return np.polynomial.polynomial.polyval2d(alpha[0], alpha[1], c)
+
+        def j(alpha):
+            return np.array([np.polynomial.polynomial.polyval2d(alpha[0], alpha[1], cj0),
+                             np.polynomial.polynomial.polyval2d(alpha[0], alpha[1], cj1)])
+
+        r = scipy.optimize.minimize(f, x0=np.array(alpha0), jac=j, method='BFGS', tol=tol)
+        return r.x
+
+    # -- helpers ----------------------------------------------------------------------------
+    def _beta(self, jac, jac_prev):
+        """max(sum g(g - gp) / sum gp^2, 0) with FP64 sums (cg.py:422-449)."""
+        out = (ctypes.c_double * 2)()
+        n = jac.size * (2 if jac.dtype.kind == 'c' else 1)
+        nat.call("glx_pr_sums", self.par.prec, jac.ptr, jac_prev.ptr, n, out)
+        with np.errstate(divide='ignore', invalid='ignore'):
+            q = np.float64(out[0]) / np.float64(out[1])
+        return self.cfg.dtype(np.fmax(q, 0.0))
+
+    def _reals(self, arr):
+        return arr.size * (2 if arr.dtype.kind == 'c' else 1)
+
+    def _axmy(self, x, y, z, alpha):
+        nat.call("glx_axmy", self.par.prec, x.ptr, y.ptr, z.ptr, float(alpha), self._reals(x))
+
+    def _axpy(self, x, y, z, alpha):
+        nat.call("glx_axpy", self.par.prec, x.ptr, y.ptr, z.ptr, float(alpha), self._reals(x))
+
+    # -- minimisation loops ----------------------------------------------------------------------
+    def __minimization_psi(self, n_iter=1000):
+        """cg.py:238-322 (kappa = inf)."""
+        assert not self.params.solveA
+        self.vars._psi.sync()
+        self.vars._vp.sync()
+        self.cg_energies = []
+        self.__gdir_psi.fill(0)
+        for i in range(n_iter):
+            jac = self._free_energy_jacobian_psi
+            if i > 0:
+                self._beta_psi = self._beta(jac, self.__gjac_psi_prev)
+            self._axmy(self.__gdir_psi, jac, self.__gdir_psi, self._beta_psi)
+            self._free_energy_conjgrad_coef_psi(self.__gdir_psi)
+            alpha0 = self._cg_alpha_psi_min()
+            psi = self.vars.order_parameter_h()
+            self._axpy(self.__gdir_psi, psi, psi, self.cfg.dtype(alpha0))
+            self.__gjac_psi, self.__gjac_psi_prev = self.__gjac_psi_prev, self.__gjac_psi
+            E0 = self.observables.free_energy
+            self.cg_energies.append(E0)
+            if i > 0 and np.abs(self.cg_energies[i] / self.cg_energies[i - 1] - 1.0) < self.__convergence_rtol:
+                break
+        self.vars._psi.need_dtoh_sync()
+
+    def __minimization(self, n_iter=1000):
+        """cg.py:452-551 (kappa < inf)."""
+        self.vars._psi.sync()
+        self.vars._vp.sync()
+        self.cg_energies = []
+        for i in range(n_iter):
+            jac = self._free_energy_jacobian_psi
+            jacA = self._free_energy_jacobian_A
+            if i > 0:
+                self._beta_psi = self._beta(jac, self.__gjac_psi_prev)
+                self._beta_A = self._beta(jacA, self.__gjac_A_prev)
+            self._axmy(self.__gdir_psi, jac, self.__gdir_psi, self._beta_psi)
+            self._axmy(self.__gdir_A, jacA, self.__gdir_A, self._beta_A)
+            self._free_energy_conjgrad_coef(self.__gdir_psi, self.__gdir_A)
+            alpha_psi, alpha_A = self._cg_alpha_min()
+            psi = self.vars.order_parameter_h()
+            ab = self.vars.vector_potential_h()
+            self._axpy(self.__gdir_psi, psi, psi, self.cfg.dtype(alpha_psi))
+            self._axpy(self.__gdir_A, ab, ab, self.cfg.dtype(alpha_A))
+            self.__gjac_psi, self.__gjac_psi_prev = self.__gjac_psi_prev, self.__gjac_psi
+            self.__gjac_A, self.__gjac_A_prev = self.__gjac_A_prev, self.__gjac_A
+            E0 = self.observables.free_energy
+            self.cg_energies.append(E0)
+            if i > 0 and np.abs(self.cg_energies[i] / self.cg_energies[i - 1] - 1.0) < self.__convergence_rtol:
+                break
+        self.vars._psi.need_dtoh_sync()
+        self.vars._vp.need_dtoh_sync()
+
+    def _solve(self, n_iter=1000):
+        self._check_supported()
+        if self.params.solveA:
+            self.__minimization(n_iter)
+        else:
+            self.__minimization_psi(n_iter)

@@ -1,0 +1,194 @@
+"""Time-dependent GL: backward-Euler psi and A updates by Jacobi
+iteration (svirl/solvers/td.py), run as persistent GPU launches.
+
+Each launch advances several time steps with every Jacobi sweep and its
+stop test on the device (glx_td_run). The state buffers rotate among three
+device arrays per field instead of the reference's right-hand-side copy and
+pointer swap; vars._psi / vars._vp always point at the one holding the
+current state.
+"""
+import ctypes
+
+import numpy as np
+
+from svirl_amd import _native as nat
+from svirl_amd._native import DeviceArray
+
+
+class TD:
+    MAX_SWEEPS = 1024  # Jacobi cap per solve (td.py:164, 274)
+
+    def __init__(self, par, mesh, _vars, params, observables):
+        self.par = par
+        self.mesh = mesh
+        self.vars = _vars
+        self.params = params
+        self.fixed_vortices = params.fixed_vortices
+        self.observables = observables
+        self.cfg = cfg = par.cfg
+        self.solveA = self.params.solveA
+        self._random_t = np.uint32(1)
+        if cfg.random_seed is not None:
+            self._random_t = np.uint32(cfg.random_seed)
+        # stop criteria clamps (td.py:51-66)
+        if cfg.dtype is np.float32:
+            cfg.stop_criterion_order_parameter = max(cfg.stop_criterion_order_parameter, 1e-6)
+            cfg.stop_criterion_vector_potential = max(cfg.stop_criterion_vector_potential, 1e-6)
+        else:
+            cfg.stop_criterion_order_parameter = max(cfg.stop_criterion_order_parameter, 1e-12)
+            cfg.stop_criterion_vector_potential = max(cfg.stop_criterion_vector_potential, 1e-12)
+        cfg.stop_criterion_order_parameter = cfg.dtype(cfg.stop_criterion_order_parameter)
+        cfg.stop_criterion_vector_potential = cfg.dtype(cfg.stop_criterion_vector_potential)
+        self.Nt = self.dt = self.T = None
+        self.NtA = self.dtA = self.TA = None
+        self._P = None
+        self._Q = None
+        self._rhs_psi = None
+        self._rhs_ab = None
+        self.sweeps = []          # (S_psi, S_A) per time step
+        self.wg = 0               # persistent grid (0: chosen per grid size)
+        self.steps_per_launch = None
+        self.td_energies = []
+
+    # -- options (td.py:75-107) ---------------------------------------------------
+    def _set_iterator_options(self, iterator_type, Nt=None, dt=None, T=None, mandatory_definition=True):
+        assert iterator_type in ['order_parameter', 'vector_potential']
+        if Nt is not None and T is not None and dt is None:
+            dt = float(T) / Nt
+        elif Nt is not None and T is None and dt is not None:
+            T = float(dt) * Nt
+        elif Nt is None and T is not None and dt is not None:
+            Nt = int(np.round(T / dt))
+        elif Nt is not None and T is not None and dt is not None:
+            assert np.isclose(T, dt * Nt)
+        if mandatory_definition:
+            assert isinstance(dt, (np.floating, float, np.integer, int)) and dt >= 0.0
+            assert isinstance(Nt, (np.integer, int)) and Nt >= 0
+        cd = self.cfg.dtype
+        if iterator_type == 'order_parameter':
+            self.Nt = np.int32(Nt) if Nt is not None else None
+            self.dt = cd(dt) if dt is not None else None
+            self.T = cd(T) if T is not None else None
+        else:
+            self.NtA = np.int32(Nt) if Nt is not None else None
+            self.dtA = cd(dt) if dt is not None else None
+            self.TA = cd(T) if T is not None else None
+
+    def _stability_warnings(self, psi=True, A=True):
+        cfg = self.cfg
+        if psi and self.dt is not None:
+            d = self.dt / min(cfg.dx, cfg.dy) ** 2
+            if d > 1.0:
+                print('Warning (order parameter):  dt/min(dx,dy)^2 = %g is too large' % d)
+        if A and self.solveA and self.dtA is not None:
+            d = self.dtA * self.params.gl_parameter ** 2 / (self.params.normal_conductivity * min(cfg.dx, cfg.dy) ** 2)
+            if d > 1.0:
+                print('Warning (vector potential):  dt*kappa^2/(sigma*min(dx,dy)^2) = %g is too large' % d)
+            d = self.dtA / (self.params.normal_conductivity * min(cfg.dx, cfg.dy))
+            if d > 1.0:
+                print('Warning (vector potential):  dt/(sigma*min(dx,dy)) = %g is too large' % d)
+
+    # -- buffers ---------------------------------------------------------------------
+    def _rotation(self, garray, bufs_attr):
+        """Three same-sized device buffers, the first being the GArray's
+        current one; returns (buffers, index of the current one)."""
+        cur = garray.get_d_obj()
+        bufs = getattr(self, bufs_attr)
+        if bufs is None or all(b is not cur for b in bufs):
+            bufs = [cur, DeviceArray(cur.size, cur.dtype), DeviceArray(cur.size, cur.dtype)]
+            setattr(self, bufs_attr, bufs)
+        idx = [k for k, b in enumerate(bufs) if b is cur][0]
+        return bufs, idx
+
+    def _default_wg(self):
+        cfg = self.cfg
+        strips = -(-int(cfg.Nx) // 63)
+        rows = strips * int(cfg.Ny)
+        waves = max(1, rows // 8)
+        maxg = ctypes.c_int32()
+        nat.call("glx_td_max_grid", self.par.prec, 256, ctypes.byref(maxg))
+        return int(max(1, min(maxg.value, -(-waves // 4))))
+
+    def _launch_steps(self):
+        if self.steps_per_launch:
+            return int(self.steps_per_launch)
+        return int(max(1, min(1000, 2.0e8 // max(1, int(self.cfg.N)))))
+
+    # -- the time loop ------------------------------------------------------------------
+    def _run(self, nsteps, eqn_code, dt):
+        if nsteps <= 0:
+            return
+        if self.fixed_vortices._vpi is not None or self.fixed_vortices._phase_lock_ns is not None:
+            raise NotImplementedError("fixed vortices / phase lock are not supported by the GPU TD path")
+        cfg = self.cfg
+        v = self.vars
+        v._psi.sync()
+        v._vp.sync()
+        P, pc = self._rotation(v._psi, '_P')
+        Q, qc = self._rotation(v._vp, '_Q')
+        p = self.params
+        lp = float(p.order_parameter_Langevin_coefficient)
+        lA = float(p.vector_potential_Langevin_coefficient)
+        if lp > 1e-32 and self._rhs_psi is None:
+            self._rhs_psi = DeviceArray(P[0].size, P[0].dtype)
+        if lA > 1e-32 and self._rhs_ab is None:
+            self._rhs_ab = DeviceArray(Q[0].size, Q[0].dtype)
+        prm = nat.TDParams()
+        prm.dt = float(dt)
+        prm.eps = float(p.linear_coefficient_scalar_h())
+        prm.kappa2 = float(p.gl_parameter_squared_h())
+        prm.rho = float(p._rho)
+        prm.H = float(p.homogeneous_external_field)
+        prm.stop_psi = float(cfg.stop_criterion_order_parameter)
+        prm.stop_A = float(cfg.stop_criterion_vector_potential)
+        prm.langevin_psi = lp
+        prm.langevin_A = lA
+        prm.solveA = 1 if self.solveA else 0
+        prm.max_sweeps = self.MAX_SWEEPS
+        prm.eqn = eqn_code
+        g = nat.Grid(int(cfg.Nx), int(cfg.Ny), float(cfg.dx), float(cfg.dy))
+        Pp = (ctypes.c_void_p * 3)(*[b.ptr for b in P])
+        Qp = (ctypes.c_void_p * 3)(*[b.ptr for b in Q])
+        cpc, cqc = ctypes.c_int32(pc), ctypes.c_int32(qc)
+        wg = self.wg or self._default_wg()
+        chunk = self._launch_steps()
+        done = 0
+        per_step = (1 if eqn_code != 2 else 0) + (1 if (self.solveA and eqn_code != 1) else 0)
+        while done < nsteps:
+            n = min(chunk, nsteps - done)
+            prm.rand_t = int(self._random_t)
+            sw = (ctypes.c_int32 * (2 * n))()
+            nat.call("glx_td_run", self.par.prec, g, ctypes.byref(prm), nat.ptr(p.linear_coefficient_h()),
+                     nat.ptr(self.mesh.material_tiling_h()), Pp, ctypes.byref(cpc), Qp, ctypes.byref(cqc),
+                     nat.ptr(self._rhs_psi), nat.ptr(self._rhs_ab), n, sw, wg, 256)
+            arr = np.frombuffer(sw, dtype=np.int32).reshape(n, 2)
+            self.sweeps.extend(map(tuple, arr.tolist()))
+            self._random_t = np.uint32((int(self._random_t) + per_step * n) & 0xffffffff)
+            done += n
+        v._psi._gdata = P[cpc.value]
+        v._vp._gdata = Q[cqc.value]
+        v._psi.need_dtoh_sync()
+        v._vp.need_dtoh_sync()
+
+    def _solve(self, dt=None, Nt=None, T=None, eqn=None):
+        """td.py:342-377: eqn None (psi then A each step), 'order_parameter'
+        or 'vector_potential'. The A update uses the psi time step dt
+        (td.py:278), as in the reference."""
+        if eqn == "order_parameter":
+            self._set_iterator_options('order_parameter', dt=dt, Nt=Nt, T=T)
+            self._stability_warnings(psi=True, A=False)
+            self._run(int(self.Nt), 1, self.dt)
+        elif eqn == "vector_potential":
+            if not self.solveA:
+                return
+            self._set_iterator_options('vector_potential', dt=dt, Nt=Nt, T=T)
+            self._stability_warnings(psi=False, A=True)
+            if self.dt is None:
+                raise TypeError("the vector-potential update uses the order-parameter time step, which is not set yet")
+            self._run(int(self.NtA), 2, self.dt)
+        else:
+            self._set_iterator_options('order_parameter', dt=dt, Nt=Nt, T=T)
+            self._set_iterator_options('vector_potential', dt=dt, Nt=Nt, T=T)
+            self._stability_warnings()
+            self.td_energies = []
+            self._run(int(self.Nt), 0, self.dt)

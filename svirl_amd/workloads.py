@@ -1,0 +1,54 @@
+"""Seeded synthetic inputs for the benchmark configurations.
+
+Config 3: a 4097^2 node instance (Lx = Ly = 2048, dx = 0.5) with a material
+mask of random circular holes (radius U[4, 10], added until 5% of the cell
+area is removed) and a Gaussian random pinning field eps(r) (mean 1, std
+0.2, correlation length 2 = 4 grid spacings). Smaller Lx gives the same
+statistics on a smaller grid.
+"""
+import numpy as np
+
+
+def holes_mask(Nxc, Nyc, dx, dy, seed=30, r_min=4.0, r_max=10.0, fraction=0.05):
+    """Cell tiling (Nxc, Nyc) bool, False inside seeded random circles."""
+    rng = np.random.default_rng(seed)
+    mt = np.ones((Nxc, Nyc), dtype=bool)
+    Lx, Ly = Nxc * dx, Nyc * dy
+    target = int(round(fraction * Nxc * Nyc))
+    removed = 0
+    while removed < target:
+        x0, y0 = rng.random() * Lx, rng.random() * Ly
+        r = r_min + (r_max - r_min) * rng.random()
+        i0, i1 = max(0, int((x0 - r) / dx)), min(Nxc, int((x0 + r) / dx) + 2)
+        j0, j1 = max(0, int((y0 - r) / dy)), min(Nyc, int((y0 + r) / dy) + 2)
+        if i0 >= i1 or j0 >= j1:
+            continue
+        xc = (np.arange(i0, i1) + 0.5) * dx
+        yc = (np.arange(j0, j1) + 0.5) * dy
+        inside = (xc[:, None] - x0) ** 2 + (yc[None, :] - y0) ** 2 < r * r
+        sub = mt[i0:i1, j0:j1]
+        removed += int(np.count_nonzero(sub & inside))
+        sub[inside] = False
+    return mt
+
+
+def pinning_field(Nx, Ny, dx, seed=31, mean=1.0, std=0.2, corr_length=2.0):
+    """Node field (Nx, Ny): Gaussian-filtered white noise, rescaled."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((Nx, Ny))
+    f = gaussian_filter(w, sigma=corr_length / dx, mode='wrap')
+    f = (f - f.mean()) / f.std()
+    return mean + std * f
+
+
+def config3(L=2048.0, dx=0.5, dtype=np.float64, seed_psi=3, kappa=5.0, sigma=200.0, H=0.05):
+    """Config 3 as a GLSolver (mask, eps map, seeded random psi)."""
+    from svirl_amd import GLSolver
+    gl = GLSolver(Lx=L, Ly=L, dx=dx, dy=dx, order_parameter='random', random_seed=seed_psi,
+                  gl_parameter=kappa, normal_conductivity=sigma, homogeneous_external_field=H, dtype=dtype)
+    c = gl.cfg
+    gl.mesh.material_tiling = holes_mask(int(c.Nxc), int(c.Nyc), float(c.dx), float(c.dy))
+    gl.vars.set_order_parameter_to_zero_outside_material()
+    gl.params.linear_coefficient = pinning_field(int(c.Nx), int(c.Ny), float(c.dx))
+    return gl

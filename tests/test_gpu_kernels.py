@@ -1,0 +1,289 @@
+"""Kernel-level parity of libglx (called through the C ABI) with the FP64
+oracle, on seeded random inputs.
+
+FP64 tolerances: elementwise results agree to a few ulp (the kernels keep the
+reference's operand order and round each multiply and add separately; only
+sin/cos/atan2 round differently), reductions to summation order. The
+integer residual may differ by one count at most. FP32 (the FP32-mixed
+form) is checked against the same oracle at float tolerance.
+Grid widths cross the 63-column strip boundaries of the row-marching
+kernels (Nx = 5, 63, 64, 127, 130).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import gl_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+nat = pytest.importorskip("svirl_amd._native")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    nat.init(0)
+
+
+def D(a, dtype):
+    return nat.DeviceArray.from_host(np.asarray(a), dtype)
+
+
+def _types(prec):
+    return (np.float64, np.complex128) if prec == nat.GLX_F64 else (np.float32, np.complex64)
+
+
+def _tol(prec, scale=1.0):
+    return (1e-12 if prec == nat.GLX_F64 else 2e-5) * scale
+
+
+def case(seed, Nx, Ny, mask=True, epsmap=True):
+    rng = np.random.default_rng(seed)
+    dx, dy = 0.5, 0.45
+    g = O.Geom(Nx, Ny, dx, dy)
+    mt = (rng.random(g.Nc) > 0.25) if mask else None
+    psi = (0.3 + rng.random(g.N)) * np.exp(2j * np.pi * rng.random(g.N))
+    ab = O.symmetric_gauge(g, 0.3) + 0.2 * rng.normal(size=g.Nab)
+    em = (1.0 + 0.2 * rng.normal(size=g.N)) if epsmap else None
+    return rng, g, mt, psi, ab, em
+
+
+def grid(g):
+    return nat.Grid(g.Nx, g.Ny, g.dx, g.dy)
+
+
+SHAPES = [(5, 4), (63, 7), (64, 9), (127, 5), (130, 33)]
+PRECS = [nat.GLX_F64, nat.GLX_F32]
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_psi_sweep(prec, shape, variant):
+    rs, cs = _types(prec)
+    rng, g, mt, psi, ab, em = case(10 * variant + shape[0], *shape, mask=variant != 0, epsmap=variant == 2)
+    rhs = psi * (1 + 0.1 * rng.random(g.N))
+    lc = 0.3 if variant == 2 else 0.0
+    jstep, t = 0, 7
+    # the oracle sees the same rounded inputs as the kernel
+    psi_r, rhs_r, ab_r = psi.astype(cs), rhs.astype(cs), ab.astype(rs)
+    em_r = None if em is None else em.astype(rs)
+    nxt, rhs_o, r = O.psi_jacobi_sweep(g, 0.1, 0.9, em_r, mt, ab_r.astype(float), rhs_r.astype(complex),
+                                       psi_r.astype(complex), lc, jstep, t, 1e-6)
+    d_psi, d_rhs, d_ab = D(psi_r, cs), D(rhs_r, cs), D(ab_r, rs)
+    d_next = nat.DeviceArray.zeros(g.N, cs)
+    d_em = D(em_r, rs) if em is not None else None
+    d_mt = D(mt.astype(np.uint8), np.uint8) if mt is not None else None
+    r2 = nat.DeviceArray.zeros(1, np.int32)
+    nat.call("glx_td_psi_sweep", prec, grid(g), 0.1, 0.9, nat.ptr(d_em), nat.ptr(d_mt), d_ab.ptr, d_rhs.ptr,
+             d_psi.ptr, d_next.ptr, lc, jstep, t, 1e-6, r2.ptr)
+    out = d_next.get()
+    assert np.max(np.abs(out - nxt)) <= _tol(prec, 10)
+    if prec == nat.GLX_F64:
+        assert abs(int(r2.get()[0]) - r) <= 1
+        if lc > 0:
+            assert np.max(np.abs(d_rhs.get() - rhs_o)) <= 1e-13
+    else:
+        assert abs(int(r2.get()[0]) - r) <= max(2, 2e-4 * r)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_a_sweep(prec, shape, variant):
+    rs, cs = _types(prec)
+    rng, g, mt, psi, ab, _ = case(20 + 10 * variant + shape[0], *shape, mask=variant != 0, epsmap=False)
+    src = (ab + 0.01 * rng.random(g.Nab)).astype(rs)
+    rhs = (ab + 0.02 * rng.random(g.Nab)).astype(rs)
+    psi_r, ab_r = psi.astype(cs), ab.astype(rs)
+    lc = 0.2 if variant == 2 else 0.0
+    k2, rho, H = 25.0, 1 / 20.0, 0.3
+    nxt, rhs_o, r = O.a_jacobi_sweep(g, 0.1, k2, rho, H, mt, psi_r.astype(complex), src.astype(float),
+                                     rhs.astype(float), ab_r.astype(float), lc, 0, 5, 1e-6)
+    d_psi, d_src, d_rhs, d_ab = D(psi_r, cs), D(src, rs), D(rhs, rs), D(ab_r, rs)
+    d_mt = D(mt.astype(np.uint8), np.uint8) if mt is not None else None
+    # variant 1 exercises the aliased source: the source buffer is the output buffer
+    if variant == 1:
+        d_next = D(src, rs)
+        d_src_arg = d_next
+    else:
+        d_next = nat.DeviceArray.zeros(g.Nab, rs)
+        d_src_arg = d_src
+    r2 = nat.DeviceArray.zeros(1, np.int32)
+    nat.call("glx_td_a_sweep", prec, grid(g), 0.1, k2, rho, H, nat.ptr(d_mt), d_psi.ptr, d_src_arg.ptr, d_rhs.ptr,
+             d_ab.ptr, d_next.ptr, lc, 0, 5, 1e-6, r2.ptr)
+    out = d_next.get()
+    assert np.max(np.abs(out - nxt)) <= _tol(prec, 20)
+    if prec == nat.GLX_F64:
+        assert abs(int(r2.get()[0]) - r) <= 1
+        if lc > 0:
+            assert np.max(np.abs(d_rhs.get() - rhs_o)) <= 1e-13
+
+
+def _td_params(dt, eps, k2, rho, H, stop_psi, stop_A, solveA, eqn=0, rand_t=1):
+    p = nat.TDParams()
+    p.dt, p.eps, p.kappa2, p.rho, p.H = dt, eps, k2, rho, H
+    p.stop_psi, p.stop_A = stop_psi, stop_A
+    p.langevin_psi = p.langevin_A = 0.0
+    p.rand_t, p.solveA, p.max_sweeps, p.eqn = rand_t, 1 if solveA else 0, 1024, eqn
+    return p
+
+
+@pytest.mark.parametrize("shape,kappa,mask,epsmap,nsteps,wg", [
+    ((33, 29), 5.0, True, True, 12, 0),
+    ((70, 41), np.inf, False, False, 15, 0),
+    ((70, 41), np.inf, False, False, 15, 3),
+    ((130, 17), 3.0, True, False, 6, 0),
+    ((9, 9), 2.0, False, True, 25, 1),
+])
+def test_td_run_matches_oracle(shape, kappa, mask, epsmap, nsteps, wg):
+    """Persistent multi-step kernel vs td.py's loops: identical sweep
+    counts per step and the same fields (FP64)."""
+    rng, g, mt, psi, ab, em = case(99 + shape[0], *shape, mask=mask, epsmap=epsmap)
+    ab = O.symmetric_gauge(g, 0.25)
+    psi = np.where(O.mask_at_nodes(g, mt), psi, 0)
+    st = O.OracleState(g, psi, ab, kappa=kappa, sigma=10.0, H=0.25, eps=1.0, eps_map=em, mt=mt)
+    st.td(dt=0.1, Nt=nsteps)
+    P = [D(psi, np.complex128), nat.DeviceArray.zeros(g.N, np.complex128), nat.DeviceArray.zeros(g.N, np.complex128)]
+    Q = [D(ab, np.float64), nat.DeviceArray.zeros(g.Nab, np.float64), nat.DeviceArray.zeros(g.Nab, np.float64)]
+    Pp = (ctypes.c_void_p * 3)(*[b.ptr for b in P])
+    Qp = (ctypes.c_void_p * 3)(*[b.ptr for b in Q])
+    pc, qc = ctypes.c_int32(0), ctypes.c_int32(0)
+    sw = (ctypes.c_int32 * (2 * nsteps))()
+    d_em = D(em, np.float64) if em is not None else None
+    d_mt = D(mt.astype(np.uint8), np.uint8) if mt is not None else None
+    prm = _td_params(0.1, 0.0 if em is not None else 1.0, st.kappa2, st.rho, 0.25, 1e-6, 1e-6, st.solveA)
+    nat.call("glx_td_run", nat.GLX_F64, grid(g), ctypes.byref(prm), nat.ptr(d_em), nat.ptr(d_mt), Pp,
+             ctypes.byref(pc), Qp, ctypes.byref(qc), None, None, nsteps, sw, wg, 256)
+    sweeps = np.frombuffer(sw, dtype=np.int32).reshape(nsteps, 2)
+    assert [tuple(x) for x in sweeps.tolist()] == st.sweeps
+    assert np.max(np.abs(P[pc.value].get() - st.psi)) <= 1e-11
+    assert np.max(np.abs(Q[qc.value].get() - st.ab)) <= 1e-11
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("shape", [(5, 4), (64, 9), (130, 33)])
+def test_observables(prec, shape):
+    rs, cs = _types(prec)
+    rng, g, mt, psi, ab, em = case(7 + shape[0], *shape)
+    ae = (0.1 * rng.normal(size=g.Nab)).astype(rs)
+    psi_r, ab_r = psi.astype(cs), ab.astype(rs)
+    em_r = em.astype(rs)
+    d_psi, d_ab, d_ae, d_em = D(psi_r, cs), D(ab_r, rs), D(ae, rs), D(em_r, rs)
+    d_mt = D(mt.astype(np.uint8), np.uint8)
+    P, A, AE, EM = psi_r.astype(complex), ab_r.astype(float), ae.astype(float), em_r.astype(float)
+    tol = 1e-12 if prec == nat.GLX_F64 else 2e-4
+    for k2 in (-1.0, 16.0):
+        F = ctypes.c_double()
+        nat.call("glx_free_energy", prec, grid(g), k2, 0.0, d_em.ptr, 0.2, d_mt.ptr, d_psi.ptr, d_ae.ptr, d_ab.ptr,
+                 ctypes.byref(F))
+        Fo = O.free_energy(g, k2, 0.0, EM, 0.2, mt, P, AE, A)
+        assert F.value == pytest.approx(Fo, rel=tol, abs=tol)
+    out = nat.DeviceArray.zeros(g.Nab, rs)
+    nat.call("glx_current_density", prec, grid(g), 16.0, 0.2, d_ae.ptr, d_ab.ptr, out.ptr)
+    J = O.current_density(g, 16.0, 0.2, AE, A)
+    assert np.max(np.abs(out.get() - J)) <= tol * 100 * (1 + np.max(np.abs(J)))
+    nat.call("glx_supercurrent_density", prec, grid(g), d_mt.ptr, d_psi.ptr, d_ae.ptr, d_ab.ptr, out.ptr)
+    assert np.max(np.abs(out.get() - O.supercurrent_density(g, mt, P, AE, A))) <= tol * 10
+    Bd = nat.DeviceArray.zeros(g.Nc, rs)
+    nat.call("glx_magnetic_field", prec, grid(g), d_ae.ptr, d_ab.ptr, Bd.ptr)
+    assert np.max(np.abs(Bd.get() - O.magnetic_field(g, AE, A))) <= tol * 100
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("shape", [(6, 5), (64, 9), (130, 33)])
+def test_cg_kernels(prec, shape):
+    rs, cs = _types(prec)
+    rng, g, mt, psi, ab, em = case(3 + shape[0], *shape)
+    ae = (0.1 * rng.normal(size=g.Nab)).astype(rs)
+    d = (0.3 * (rng.normal(size=g.N) + 1j * rng.normal(size=g.N))).astype(cs)
+    dab = (0.05 * rng.normal(size=g.Nab)).astype(rs)
+    psi_r, ab_r, em_r = psi.astype(cs), ab.astype(rs), em.astype(rs)
+    P, A, AE, EM = psi_r.astype(complex), ab_r.astype(float), ae.astype(float), em_r.astype(float)
+    d_psi, d_ab, d_ae, d_em, d_d, d_dab = D(psi_r, cs), D(ab_r, rs), D(ae, rs), D(em_r, rs), D(d, cs), D(dab, rs)
+    d_mt = D(mt.astype(np.uint8), np.uint8)
+    tol = 1e-12 if prec == nat.GLX_F64 else 3e-4
+    k2, H = 9.0, 0.15
+    G = nat.DeviceArray.zeros(g.N, cs)
+    nat.call("glx_cg_jacobian_psi", prec, grid(g), k2, 0.0, d_em.ptr, H, d_mt.ptr, d_psi.ptr, d_ae.ptr, d_ab.ptr, G.ptr)
+    jo = O.jacobian_psi(g, k2, 0.0, EM, H, mt, P, AE, A)
+    assert np.max(np.abs(G.get() - jo)) <= tol * 10 * (1 + np.max(np.abs(jo)))
+    GA = nat.DeviceArray.zeros(g.Nab, rs)
+    nat.call("glx_cg_jacobian_A", prec, grid(g), k2, H, d_mt.ptr, d_psi.ptr, d_ae.ptr, d_ab.ptr, GA.ptr)
+    jA = O.jacobian_A(g, k2, H, mt, P, AE, A)
+    assert np.max(np.abs(GA.get() - jA)) <= tol * 10 * (1 + np.max(np.abs(jA)))
+    c5 = (ctypes.c_double * 5)()
+    nat.call("glx_cg_coef_psi", prec, grid(g), k2, 0.9, H, d_mt.ptr, d_psi.ptr, d_d.ptr, d_ae.ptr, d_ab.ptr, c5)
+    co = O.coef_psi(g, k2, 0.9, H, mt, P, d.astype(complex), AE, A)
+    assert np.allclose(np.array(c5[:]), co, rtol=tol * 10, atol=tol * 10 * np.max(np.abs(co)))
+    c17 = (ctypes.c_double * 17)()
+    nat.call("glx_cg_coef", prec, grid(g), k2, 0.9, H, d_mt.ptr, d_psi.ptr, d_d.ptr, d_ae.ptr, d_ab.ptr, d_dab.ptr, c17)
+    co = O.coef(g, k2, 0.9, H, mt, P, d.astype(complex), AE, A, dab.astype(float))
+    assert np.allclose(np.array(c17[:]), co, rtol=tol * 10, atol=tol * 10 * np.max(np.abs(co)))
+    # Polak-Ribiere sums and axpy / axmy
+    x = (rng.normal(size=g.N) + 1j * rng.normal(size=g.N)).astype(cs)
+    y = (rng.normal(size=g.N) + 1j * rng.normal(size=g.N)).astype(cs)
+    dx_, dy_ = D(x, cs), D(y, cs)
+    out = (ctypes.c_double * 2)()
+    nat.call("glx_pr_sums", prec, dx_.ptr, dy_.ptr, 2 * g.N, out)
+    X, Y = x.astype(complex), y.astype(complex)
+    assert out[0] == pytest.approx(np.sum(X.real * (X.real - Y.real) + X.imag * (X.imag - Y.imag)), rel=1e-6 if prec else 1e-12)
+    assert out[1] == pytest.approx(np.sum(np.abs(Y) ** 2), rel=1e-6 if prec else 1e-12)
+    z = nat.DeviceArray.zeros(g.N, cs)
+    nat.call("glx_axpy", prec, dx_.ptr, dy_.ptr, z.ptr, 0.7, 2 * g.N)
+    assert np.allclose(z.get(), 0.7 * X + Y, rtol=1e-6, atol=1e-6)
+    nat.call("glx_axmy", prec, dx_.ptr, dy_.ptr, z.ptr, 0.7, 2 * g.N)
+    assert np.allclose(z.get(), 0.7 * X - Y, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("n", [1, 5, 1000, 123457, 2048 * 2048 - 3])
+def test_reductions(n):
+    """tests/at_reduction.py: sum and 5-vector sums vs numpy, atol 1e-10."""
+    rng = np.random.default_rng(n)
+    x = rng.random(n)
+    out = ctypes.c_double()
+    nat.call("glx_sum", nat.GLX_F64, D(x, np.float64).ptr, n, 0, ctypes.byref(out))
+    assert np.isclose(out.value, np.sum(x), atol=1e-10, rtol=1e-13)
+    nv = max(1, n // 5)
+    v = rng.random(nv * 5)
+    o5 = (ctypes.c_double * 5)()
+    nat.call("glx_sum_v", nat.GLX_F64, D(v, np.float64).ptr, nv, 5, 0, o5)
+    assert np.allclose(np.array(o5[:]), v.reshape(nv, 5).sum(axis=0), atol=1e-10, rtol=1e-13)
+
+
+def test_vortex_cells_match_oracle():
+    """Winding numbers: flagged cells and rounded vorticities bit-exact."""
+    g = O.Geom(70, 66, 0.5, 0.5)
+    x = g.dx * np.arange(g.Nx)[None, :]
+    y = g.dy * np.arange(g.Ny)[:, None]
+    psi = np.ones((g.Ny, g.Nx), complex)
+    for (x0, y0, s) in [(5.3, 6.1, 1), (20.2, 11.7, -1), (30.05, 25.4, 1), (12.6, 29.9, 2)]:
+        psi = psi * ((x - x0) + 0.2 * (y - y0) + 1j * s * (y - y0)) ** (1 if abs(s) == 1 else 2)
+    psi = psi.ravel() / np.max(np.abs(psi))
+    H = 0.05
+    ab = O.symmetric_gauge(g, H) * 0.0
+    cells, vs = O.vortex_cells(g, psi, ab, 0.0)
+    cap = 64
+    rec = nat.DeviceArray.zeros(cap * 14, np.float64)
+    cnt = ctypes.c_int32()
+    nat.call("glx_vortex_cells", nat.GLX_F64, grid(g), D(psi, np.complex128).ptr, D(ab, np.float64).ptr, None, 0.0,
+             rec.ptr, cap, ctypes.byref(cnt))
+    r = rec.get()[:cnt.value * 14].reshape(-1, 14)
+    r = r[np.argsort(r[:, 0])]
+    assert list(r[:, 0].astype(int)) == list(cells)
+    assert np.array_equal(np.round(r[:, 1]), np.round(vs))
+    assert np.max(np.abs(r[:, 1] - vs)) < 1e-12
+
+
+def test_capacity_error():
+    g = O.Geom(16, 16, 0.5, 0.5)
+    x = g.dx * np.arange(g.Nx)[None, :]
+    y = g.dy * np.arange(g.Ny)[:, None]
+    psi = ((x - 2.3) + 0.1 * (y - 2.1) + 1j * (y - 2.1)) * ((x - 5.2) + 0.1 * (y - 5.6) + 1j * (y - 5.6))
+    rec = nat.DeviceArray.zeros(14, np.float64)
+    cnt = ctypes.c_int32()
+    with pytest.raises(nat.GLXError) as e:
+        nat.call("glx_vortex_cells", nat.GLX_F64, grid(g), D(psi.ravel(), np.complex128).ptr,
+                 D(np.zeros(g.Nab), np.float64).ptr, None, 0.0, rec.ptr, 1, ctypes.byref(cnt))
+    assert e.value.code == nat.GLX_E_CAPACITY and cnt.value == 2

@@ -1,0 +1,217 @@
+"""End-to-end parity of the drop-in GLSolver (GPU) with the FP64 oracle,
+plus the reference's own tests (tests/at_*.py) re-expressed on the GPU.
+
+Inputs are built once by the GLSolver (seeded random psi, symmetric gauge)
+and handed to the oracle as the same flat arrays, so both start from
+identical bits. FP64: psi and F within 1e-10 relative, vortex cells and
+vorticities exact, Jacobi sweep counts exact.
+"""
+import numpy as np
+import pytest
+
+from oracle import gl_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+svirl_amd = pytest.importorskip("svirl_amd")
+GLSolver = svirl_amd.GLSolver
+
+
+def oracle_of(gl):
+    c = gl.cfg
+    g = O.Geom(int(c.Nx), int(c.Ny), float(c.dx), float(c.dy))
+    psi = gl.flatten_array(gl.vars.order_parameter).astype(complex)
+    a, b = gl.vars.vector_potential
+    ab = np.concatenate([gl.flatten_a_array(a), gl.flatten_b_array(b)]).astype(float)
+    mt = None
+    if gl.mesh.have_material_tiling():
+        mt = np.ascontiguousarray(gl.mesh.material_tiling.T).ravel()
+    em = None
+    if gl.params.linear_coefficient_h() is not None:
+        em = gl.flatten_array(gl.params.linear_coefficient).astype(float)
+    abei = None
+    if gl.params.external_irregular_vector_potential_h() is not None:
+        ea, eb = gl.params.external_irregular_vector_potential
+        abei = np.concatenate([gl.flatten_a_array(ea), gl.flatten_b_array(eb)]).astype(float)
+    eps = float(gl.params.linear_coefficient_scalar_h()) if em is None else 1.0
+    st = O.OracleState(g, psi, ab, kappa=float(gl.params.gl_parameter), sigma=float(gl.params.normal_conductivity),
+                       H=float(gl.params.homogeneous_external_field), eps=eps, eps_map=em, mt=mt, abei=abei,
+                       stop_psi=float(c.stop_criterion_order_parameter), stop_A=float(c.stop_criterion_vector_potential),
+                       random_t=1 if c.random_seed is None else int(c.random_seed),
+                       convergence_rtol=c.convergence_rtol)
+    return st
+
+
+def gl_psi(gl):
+    return gl.flatten_array(gl.vars.order_parameter)
+
+
+def gl_ab(gl):
+    a, b = gl.vars.vector_potential
+    return np.concatenate([gl.flatten_a_array(a), gl.flatten_b_array(b)])
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def test_c1_like_td_and_cg_kappa_inf():
+    """Config 1 in miniature: kappa = inf, H = 0.1, random seeded psi, TD
+    then CG (polyroots line search)."""
+    gl = GLSolver(Lx=16, Ly=16, dx=0.5, dy=0.5, order_parameter='random', random_seed=1, random_level=1.0,
+                  gl_parameter=np.inf, homogeneous_external_field=0.1, dtype=np.float64)
+    st = oracle_of(gl)
+    gl.solve.td(dt=0.1, Nt=60)
+    st.td(dt=0.1, Nt=60)
+    assert gl.solve._td.sweeps == st.sweeps
+    assert rel(gl_psi(gl), st.psi) < 1e-10
+    F = gl.observables.free_energy
+    assert F == pytest.approx(st.free_energy(), rel=1e-10)
+    gl.solve.cg(n_iter=40)
+    st.cg(n_iter=40)
+    assert len(gl.solve._cg.cg_energies) == len(st.cg_energies)
+    assert np.allclose(gl.solve._cg.cg_energies, st.cg_energies, rtol=1e-10)
+    assert rel(gl_psi(gl), st.psi) < 1e-9
+
+
+def test_c2_like_td_finite_kappa_with_mask_and_eps_map():
+    rng = np.random.default_rng(3)
+    gl = GLSolver(Lx=20, Ly=16, dx=0.5, dy=0.5, order_parameter='random', random_seed=2, gl_parameter=5.0,
+                  normal_conductivity=200.0, homogeneous_external_field=0.1, dtype=np.float64)
+    mt = np.ones((gl.cfg.Nxc, gl.cfg.Nyc), bool)
+    mt[10:16, 8:14] = False
+    gl.mesh.material_tiling = mt
+    gl.vars.set_order_parameter_to_zero_outside_material()
+    gl.params.linear_coefficient = 1.0 + 0.2 * rng.normal(size=(gl.cfg.Nx, gl.cfg.Ny))
+    st = oracle_of(gl)
+    gl.solve.td(dt=0.1, Nt=40)
+    st.td(dt=0.1, Nt=40)
+    assert gl.solve._td.sweeps == st.sweeps
+    assert rel(gl_psi(gl), st.psi) < 1e-10
+    assert rel(gl_ab(gl), st.ab) < 1e-10
+    assert gl.observables.free_energy == pytest.approx(st.free_energy(), rel=1e-10)
+    jx, jy = gl.observables.current_density
+    J = st.current_density()
+    assert np.allclose(np.concatenate([gl.flatten_a_array(jx), gl.flatten_b_array(jy)]), J, atol=1e-9)
+    # CG with the BFGS line search
+    gl.solve.cg(n_iter=15)
+    st.cg(n_iter=15)
+    assert np.allclose(gl.solve._cg.cg_energies, st.cg_energies, rtol=1e-9)
+
+
+def test_vortex_detector_matches_oracle():
+    gl = GLSolver(Lx=24, Ly=24, dx=0.5, dy=0.5, order_parameter=1.0, gl_parameter=np.inf,
+                  homogeneous_external_field=0.15, dtype=np.float64)
+    gl.params.fixed_vortices.order_parameter_add_vortices([[6.2, 12.3, 17.1], [7.1, 15.2, 5.3], [1, -1, 1]],
+                                                          phase=True, deep=True)
+    gl.solve.td(dt=0.1, Nt=30)
+    st = oracle_of(gl)
+    vx, vy, vv = gl.vortex_detector.vortices
+    ox, oy, ov = st.vortices()
+    cells, vs = O.vortex_cells(st.g, st.psi, st.ab, st.H)
+    gcells, gvs = gl.vortex_detector.vortex_cells
+    assert list(gcells) == list(cells) and np.array_equal(np.round(gvs), np.round(vs))
+    assert len(vx) == len(ox) and np.array_equal(vv, ov)
+    assert np.allclose(vx, ox, atol=1e-10) and np.allclose(vy, oy, atol=1e-10)
+
+
+def test_jacobians_finite_differences_through_api():
+    """tests/at_cg_jacobians.py:111-160 on the drop-in (one-sided h=3e-9,
+    atol 1e-5, rtol 1e-3 as in the reference)."""
+    rng = np.random.default_rng(11)
+    gl = GLSolver(Nx=9, Ny=10, dx=0.45, dy=0.48, gl_parameter=1.0, dtype=np.float64)
+    gl.solve._init_cg()
+    gl.params.gl_parameter = 1.0 + 3.0 * rng.random()
+    gl.vars.order_parameter = 1.0
+    gl.vars.randomize_order_parameter(level=0.5, seed=4)
+    gl.params.homogeneous_external_field_reset = 0.01 + 0.1 * rng.random()
+    gl.params.external_field = 0.01 + 0.1 * rng.random()
+    h = 3e-9
+    E0 = gl.observables.free_energy
+    G1 = gl.unflatten_array(gl.solve._cg._free_energy_jacobian_psi.get())
+    G = np.empty((gl.cfg.Nx, gl.cfg.Ny), complex)
+    dpsi = np.zeros((gl.cfg.Nx, gl.cfg.Ny), complex)
+    for i in range(gl.cfg.Nx):
+        for j in range(gl.cfg.Ny):
+            dpsi[i, j] = h
+            gl.vars.order_parameter = gl.vars.order_parameter + dpsi
+            re = (gl.observables.free_energy - E0) / h
+            gl.vars.order_parameter = gl.vars.order_parameter - dpsi
+            dpsi[i, j] = 1j * h
+            gl.vars.order_parameter = gl.vars.order_parameter + dpsi
+            im = (gl.observables.free_energy - E0) / h
+            gl.vars.order_parameter = gl.vars.order_parameter - dpsi
+            dpsi[i, j] = 0
+            G[i, j] = re + 1j * im
+    assert np.allclose(G1, G, atol=1e-5, rtol=1e-3)
+
+
+def test_coef_psi_through_api():
+    """tests/at_cg_coef_psi.py:35-57."""
+    from svirl_amd.storage import GArray
+    gl = GLSolver(Nx=10, Ny=9, dx=0.47, dy=0.42, gl_parameter=np.inf, homogeneous_external_field=0.1,
+                  dtype=np.float64)
+    dpsi = GArray(like=gl.vars.order_parameter)
+    gl.solve._init_cg()
+    rng = np.random.default_rng(2)
+    for a, b, alpha in [[1.0, 0.0, 0.0], [0.0, 1.0, 1.0], [0.5, 0.5, 1.0]] + [
+            [1.0 - bb * al, bb, al] for bb, al in rng.random((5, 2))]:
+        gl.vars.order_parameter = 1.0
+        gl.vars.randomize_order_parameter(level=0.5)
+        psi0 = gl.vars.order_parameter
+        E0 = gl.observables.free_energy
+        gl.vars.order_parameter = psi0 * a
+        dpsi.set_h(psi0 * b)
+        dpsi.sync()
+        c0, c1, c2, c3, c4 = gl.solve._cg._free_energy_conjgrad_coef_psi(dpsi.get_d_obj())
+        P = c4 * alpha ** 4 + c3 * alpha ** 3 + c2 * alpha ** 2 + c1 * alpha + c0
+        assert np.isclose(P, E0)
+
+
+def test_reduction_api():
+    """tests/at_reduction.py:20-97 through gl.par.red."""
+    gl = GLSolver(Nx=32, Ny=32, dx=0.5, dy=0.5)
+    rng = np.random.default_rng(5)
+    for n in (1, 77, 500000):
+        for bs in (64, 256, 1024):
+            a = rng.random(n)
+            assert np.isclose(np.sum(a), gl.par.red.test_sum(a, n, block_size=bs), atol=1e-10)
+            nv = max(1, n // 5)
+            v = rng.random(nv * 5)
+            assert np.allclose(v.reshape(nv, 5).sum(axis=0), gl.par.red.test_sum_v(v, nv, 5, block_size=bs), atol=1e-10)
+
+
+def test_precision_fp32_vs_fp64():
+    """tests/at_precision.py in miniature: FP32-mixed and FP64 TD energies
+    agree (the reference allows rtol 0.1; the FP32-mixed form is held to 1e-4)."""
+    out = {}
+    for dt_ in (np.float32, np.float64):
+        gl = GLSolver(Lx=30, Ly=30, dx=1.0, dy=1.0, order_parameter=1.0, gl_parameter=3.6432,
+                      normal_conductivity=400.0, homogeneous_external_field=0.1, dtype=dt_, convergence_rtol=1e-12)
+        gl.params.fixed_vortices.order_parameter_add_vortices([15, 15], phase=True, deep=True)
+        gl.mesh.material_tiling = lambda x, y: ~np.logical_and.reduce((np.abs(x - 15) < 3, np.abs(y - 15) < 3))
+        gl.vars.set_order_parameter_to_zero_outside_material()
+        gl.solve.td(Nt=300, dt=0.1)
+        out[dt_] = gl.observables.free_energy
+    assert np.isclose(out[np.float32], out[np.float64], rtol=1e-4)
+
+
+def test_td_eqn_modes_and_chunking():
+    """eqn='order_parameter' / 'vector_potential' and launches split into
+    several persistent kernels give the reference's results."""
+    gl = GLSolver(Lx=12, Ly=10, dx=0.5, dy=0.5, order_parameter='random', random_seed=5, gl_parameter=2.0,
+                  normal_conductivity=20.0, homogeneous_external_field=0.2, dtype=np.float64)
+    st = oracle_of(gl)
+    gl.solve._init_td()
+    gl.solve._td.steps_per_launch = 3
+    gl.solve.td(dt=0.1, Nt=7, eqn='order_parameter')
+    for _ in range(7):
+        st.td_psi_solve(0.1)
+    assert rel(gl_psi(gl), st.psi) < 1e-11
+    gl.solve.td(dt=0.1, Nt=5, eqn='vector_potential')
+    for _ in range(5):
+        st.td_A_solve(0.1)
+    assert rel(gl_ab(gl), st.ab) < 1e-11
+    gl.solve.td(dt=0.1, Nt=8)
+    st.td(dt=0.1, Nt=8)
+    assert rel(gl_psi(gl), st.psi) < 1e-10 and rel(gl_ab(gl), st.ab) < 1e-10
