@@ -184,8 +184,11 @@ def grad_c(psi0, ph, psi1):
 
 def scaled_residual(r, stop_eps):
     """int32(min(1e4 r / stop_eps, 1e8)) (td.h:205-213, 541-543)."""
-    rr = 1.0e4 * r / stop_eps
+    with np.errstate(over='ignore', invalid='ignore'):
+        rr = 1.0e4 * r / stop_eps
     rr = np.where(rr > 1.0e8, 1.0e8, rr)
+    # a NaN residual converts to 0 (the GPU float-to-int conversion of NaN)
+    rr = np.nan_to_num(rr, nan=0.0)
     return int(np.max(np.trunc(rr))) if np.size(rr) else 0
 
 

@@ -143,7 +143,7 @@ def test_td_run_matches_oracle(shape, kappa, mask, epsmap, nsteps, wg):
     rng, g, mt, psi, ab, em = case(99 + shape[0], *shape, mask=mask, epsmap=epsmap)
     ab = O.symmetric_gauge(g, 0.25)
     psi = np.where(O.mask_at_nodes(g, mt), psi, 0)
-    st = O.OracleState(g, psi, ab, kappa=kappa, sigma=10.0, H=0.25, eps=1.0, eps_map=em, mt=mt)
+    st = O.OracleState(g, psi, ab, kappa=kappa, sigma=100.0, H=0.25, eps=1.0, eps_map=em, mt=mt)
     st.td(dt=0.1, Nt=nsteps)
     P = [D(psi, np.complex128), nat.DeviceArray.zeros(g.N, np.complex128), nat.DeviceArray.zeros(g.N, np.complex128)]
     Q = [D(ab, np.float64), nat.DeviceArray.zeros(g.Nab, np.float64), nat.DeviceArray.zeros(g.Nab, np.float64)]
@@ -243,12 +243,14 @@ def test_reductions(n):
     rng = np.random.default_rng(n)
     x = rng.random(n)
     out = ctypes.c_double()
-    nat.call("glx_sum", nat.GLX_F64, D(x, np.float64).ptr, n, 0, ctypes.byref(out))
+    dxv = D(x, np.float64)  # keep the device array alive across the call
+    nat.call("glx_sum", nat.GLX_F64, dxv.ptr, n, 0, ctypes.byref(out))
     assert np.isclose(out.value, np.sum(x), atol=1e-10, rtol=1e-13)
     nv = max(1, n // 5)
     v = rng.random(nv * 5)
     o5 = (ctypes.c_double * 5)()
-    nat.call("glx_sum_v", nat.GLX_F64, D(v, np.float64).ptr, nv, 5, 0, o5)
+    dv = D(v, np.float64)
+    nat.call("glx_sum_v", nat.GLX_F64, dv.ptr, nv, 5, 0, o5)
     assert np.allclose(np.array(o5[:]), v.reshape(nv, 5).sum(axis=0), atol=1e-10, rtol=1e-13)
 
 
@@ -267,7 +269,8 @@ def test_vortex_cells_match_oracle():
     cap = 64
     rec = nat.DeviceArray.zeros(cap * 14, np.float64)
     cnt = ctypes.c_int32()
-    nat.call("glx_vortex_cells", nat.GLX_F64, grid(g), D(psi, np.complex128).ptr, D(ab, np.float64).ptr, None, 0.0,
+    d_psi, d_ab = D(psi, np.complex128), D(ab, np.float64)
+    nat.call("glx_vortex_cells", nat.GLX_F64, grid(g), d_psi.ptr, d_ab.ptr, None, 0.0,
              rec.ptr, cap, ctypes.byref(cnt))
     r = rec.get()[:cnt.value * 14].reshape(-1, 14)
     r = r[np.argsort(r[:, 0])]
@@ -283,7 +286,8 @@ def test_capacity_error():
     psi = ((x - 2.3) + 0.1 * (y - 2.1) + 1j * (y - 2.1)) * ((x - 5.2) + 0.1 * (y - 5.6) + 1j * (y - 5.6))
     rec = nat.DeviceArray.zeros(14, np.float64)
     cnt = ctypes.c_int32()
+    d_psi, d_ab = D(psi.ravel(), np.complex128), D(np.zeros(g.Nab), np.float64)
     with pytest.raises(nat.GLXError) as e:
-        nat.call("glx_vortex_cells", nat.GLX_F64, grid(g), D(psi.ravel(), np.complex128).ptr,
-                 D(np.zeros(g.Nab), np.float64).ptr, None, 0.0, rec.ptr, 1, ctypes.byref(cnt))
+        nat.call("glx_vortex_cells", nat.GLX_F64, grid(g), d_psi.ptr, d_ab.ptr, None, 0.0, rec.ptr, 1,
+                 ctypes.byref(cnt))
     assert e.value.code == nat.GLX_E_CAPACITY and cnt.value == 2
