@@ -85,16 +85,17 @@ int glx_init(int device) {
     if ((st = hipEventCreate(&g_ctx.env.kev0)) != hipSuccess) return check_hip(st, "hipEventCreate");
     if ((st = hipEventCreate(&g_ctx.env.kev1)) != hipSuccess) return check_hip(st, "hipEventCreate");
     g_ctx.env.last_kernel_ms = &g_ctx.last_kernel_ms;
-    // scratch layout: partials | result | grp | top | err | slots | state | sweeps
+    // scratch layout: partials | result | grp | top | rel | err | slots | state | sweeps
     const size_t partials = sizeof(double) * 1024 * 32;
     const size_t result = sizeof(double) * 64;
-    const size_t grp = sizeof(unsigned long long) * 8;
-    const size_t top = sizeof(unsigned long long) * 8;
-    const size_t err = 256;
+    const size_t grp = sizeof(unsigned long long) * 8 * 64;
+    const size_t top = 512;
+    const size_t rel = sizeof(unsigned long long) * 8 * 64;
+    const size_t err = 512;
     const size_t slots = sizeof(int) * 64 * 8;
-    const size_t state = 256;
+    const size_t state = 512;
     const size_t sweeps = sizeof(int) * 2 * SWEEPS_CAP;
-    const size_t total = partials + result + grp + top + err + slots + state + sweeps;
+    const size_t total = partials + result + grp + top + rel + err + slots + state + sweeps;
     if ((st = hipMalloc(&g_ctx.scratch, total)) != hipSuccess) return check_hip(st, "hipMalloc(scratch)");
     if ((st = hipMemset(g_ctx.scratch, 0, total)) != hipSuccess) return check_hip(st, "hipMemset(scratch)");
     char *p = (char *)g_ctx.scratch;
@@ -104,6 +105,7 @@ int glx_init(int device) {
     e.result = (double *)p; p += result;
     e.grp = (unsigned long long *)p; p += grp;
     e.top = (unsigned long long *)p; p += top;
+    e.rel = (unsigned long long *)p; p += rel;
     e.err = (int *)p; p += err;
     e.slots = (int *)p; p += slots;
     e.state = (int *)p; p += state;

@@ -174,41 +174,53 @@ __device__ __forceinline__ void atomic_max_agent(int *p, int v) {
 
 // ---- grid barrier for persistent kernels ----------------------------------
 //
-// Two-level arrival counters (blocks grouped by blockIdx % NGRP, the last
-// arriver of a group bumps the top counter), all counters monotonic within a
-// launch and zeroed by the host before it. Every wait is bounded by a wall
-// clock timeout: a block that times out raises `err` and every block leaves
-// its loops at the next barrier, so a launch that is not fully co-resident
-// ends with an error instead of hanging.
+// Blocks are grouped by blockIdx % NGRP. Arrival: one agent-scope atomic per
+// block on its group counter; the last arriver of a group bumps the top
+// counter; the last group to arrive writes the epoch into every group's
+// release flag; each block polls only its own group's flag. Counters and
+// flags sit on separate 512-byte lines so that the atomics of different
+// groups do not serialise on one memory channel. All counters are
+// monotonic (epochs carry over between launches). Every wait is bounded by
+// a wall-clock timeout: a block that times out raises `err`, every block
+// leaves its loops at the next barrier and the host reports an error, so a
+// launch that is not fully co-resident cannot hang.
 constexpr int NGRP = 8;
+constexpr int GS_STRIDE = 64;  // u64 elements between counters (512 bytes)
 constexpr unsigned long long BARRIER_TIMEOUT_TICKS = 100ull * 1000 * 1000 * 5;  // 5 s at 100 MHz
 
 struct GridSync {
-    unsigned long long *grp;  // NGRP counters
+    unsigned long long *grp;  // NGRP counters, GS_STRIDE apart
     unsigned long long *top;  // 1 counter
+    unsigned long long *rel;  // NGRP release flags, GS_STRIDE apart
     int *err;                 // 1 flag
 };
 
-// Call from every thread of every block the same number of times. Returns
-// false when the barrier failed (timeout anywhere in the grid).
+// Call from every thread of every block the same number of times; `epoch`
+// is the number of barriers already passed (only thread 0's copy is used).
+// Returns false when the barrier failed (timeout anywhere in the grid).
 __device__ __forceinline__ bool grid_barrier(const GridSync &gs, unsigned long long &epoch) {
     __shared__ int s_ok;
-    __threadfence();
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have reached the cache
     __syncthreads();
     if (threadIdx.x == 0) {
+        __threadfence();  // release the workgroup's stores to the whole device
         ++epoch;
         const unsigned nb = gridDim.x;
         const unsigned g = blockIdx.x % NGRP;
         const unsigned gsize = (nb - g + NGRP - 1) / NGRP;
         const unsigned ngroups = nb < (unsigned)NGRP ? nb : (unsigned)NGRP;
-        unsigned long long old =
-            __hip_atomic_fetch_add(&gs.grp[g], 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old + 1 == epoch * gsize)
-            __hip_atomic_fetch_add(gs.top, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned long long target = epoch * ngroups;
+        const unsigned long long old =
+            __hip_atomic_fetch_add(gs.grp + g * GS_STRIDE, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1 == epoch * gsize) {
+            const unsigned long long t =
+                __hip_atomic_fetch_add(gs.top, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (t + 1 == epoch * ngroups)
+                for (unsigned k = 0; k < ngroups; ++k)
+                    __hip_atomic_store(gs.rel + k * GS_STRIDE, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
         int ok = 1;
         const unsigned long long t0 = wall_clock64();
-        while (ld_acquire(gs.top) < target) {
+        while (ld_acquire(gs.rel + g * GS_STRIDE) < epoch) {
             if (ld_acquire(gs.err)) { ok = 0; break; }
             if (wall_clock64() - t0 > BARRIER_TIMEOUT_TICKS) {
                 __hip_atomic_store(gs.err, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -217,10 +229,10 @@ __device__ __forceinline__ bool grid_barrier(const GridSync &gs, unsigned long l
             }
             __builtin_amdgcn_s_sleep(1);
         }
+        __threadfence();  // acquire: later loads of the workgroup see the grid's stores
         s_ok = ok;
     }
     __syncthreads();
-    __threadfence();
     return s_ok != 0;
 }
 

@@ -14,8 +14,9 @@ struct Env {
     hipStream_t stream;
     double *partials;        // RED_MAX_BLOCKS * 32 doubles
     double *result;          // 64 doubles
-    unsigned long long *grp; // barrier group counters (8)
+    unsigned long long *grp; // barrier group counters (NGRP, 512 B apart)
     unsigned long long *top; // barrier top counter
+    unsigned long long *rel; // barrier release flags (NGRP, 512 B apart)
     int *err;                // barrier error flag
     int *slots;              // residual ring
     int *state;              // persistent-kernel state

@@ -119,12 +119,20 @@ int L<R>::a_sweep(const Env &e, glx_grid g, double dt, double kappa2, double rho
     return GLX_OK;
 }
 
+// The FP32-mixed run stores A as A - A_gauge(H) (exact change of variable:
+// the symmetric gauge is a fixed point of the A sweep with psi = 0), so that
+// the iterates stay small and float-resolved; FP64 runs keep the plain form.
+template <typename R> struct RunForm { static constexpr bool gauge = false; };
+template <> struct RunForm<float> { static constexpr bool gauge = true; };
+
 template <typename R> int L<R>::td_max_grid(const Env &e, int32_t wg_size, int32_t *max_wg) {
     if (wg_size != 256 || !max_wg) return set_error(GLX_E_ARG, "glx_td_max_grid: wg_size must be 256");
-    int per_cu = 0;
-    hipError_t st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_td_run<R>, 256, 0);
+    constexpr bool G = RunForm<R>::gauge;
+    int a = 0, b = 0;
+    hipError_t st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_td_psi_solve<R, G>, 256, 0);
+    if (st == hipSuccess) st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_td_a_solve<R, G>, 256, 0);
     if (st != hipSuccess) return check_hip(st, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-    *max_wg = per_cu * e.num_cus;
+    *max_wg = (a < b ? a : b) * e.num_cus;
     return GLX_OK;
 }
 
@@ -149,47 +157,82 @@ int L<R>::td_run(const Env &e, glx_grid g, const glx_td_params *p, const void *e
     if (rc) return rc;
     if (wg <= 0) wg = maxg;
     if (wg > maxg) return set_error(GLX_E_ARG, "glx_td_run: grid %d exceeds co-resident limit %d", wg, maxg);
+    constexpr bool G = RunForm<R>::gauge;
 
-    TDRunArgs<R> T{};
-    T.psi = make_psi_args<R>(g, p->dt, p->eps, eps_map, mt, P[*pc], p->stop_psi, p->langevin_psi);
+    TDSolveArgs<R> T{};
+    T.psi = make_psi_args<R>(g, p->dt, p->eps, eps_map, mt, nullptr, p->stop_psi, p->langevin_psi);
     T.A = make_a_args<R>(g, p->dt, p->kappa2, p->rho, p->H, mt, p->stop_A, p->langevin_A);
+    const double Lx = g.dx * (g.Nx - 1), Ly = g.dy * (g.Ny - 1);
+    const double gH = G ? p->H : 0.0;
+    T.psi.gH = T.A.gH = gH;
+    T.psi.gdx = T.A.gdx = g.dx;
+    T.psi.gdy = T.A.gdy = g.dy;
+    T.psi.gx0 = T.A.gx0 = 0.5 * Lx;
+    T.psi.gy0 = T.A.gy0 = 0.5 * Ly;
+    if (G) {  // the gauge absorbs the boundary H source of the A sweep
+        T.A.rh_a = R(0);
+        T.A.rh_b = R(0);
+    }
     for (int k = 0; k < 3; ++k) { T.P[k] = (cplx<R> *)P[k]; T.Q[k] = (R *)Q[k]; }
     T.rhs_psi = p->langevin_psi > 1.0e-32 ? (cplx<R> *)rhs_psi : nullptr;
     T.rhs_ab = (p->solveA && p->langevin_A > 1.0e-32) ? (R *)rhs_ab : nullptr;
-    T.psi.rand_t = p->rand_t;
-    T.psi_cur = *pc;
-    T.ab_cur = *qc;
-    T.solveA = p->solveA;
-    T.eqn = p->eqn;
     T.max_sweeps = p->max_sweeps;
-    T.nsteps = nsteps;
     T.sc.gs.grp = e.grp;
     T.sc.gs.top = e.top;
+    T.sc.gs.rel = e.rel;
     T.sc.gs.err = e.err;
     T.sc.slots = e.slots;
     T.sc.state = e.state;
     T.sc.sweeps = e.sweeps;
-    hipLaunchKernelGGL(k_td_reset, dim3(1), dim3(256), 0, e.stream, T.sc);
+    const long Nab = (long)(g.Nx - 1) * g.Ny + (long)g.Nx * (g.Ny - 1);
+
+    hipLaunchKernelGGL(k_td_reset, dim3(1), dim3(256), 0, e.stream, T.sc, (int)*pc, (int)*qc);
     GLX_LAUNCH_CHECK("k_td_reset");
+    if (G) {
+        hipLaunchKernelGGL(k_gauge_shift<R>, dim3(grid_blocks(Nab)), dim3(256), 0, e.stream, T.Q[*qc], g.Nx, g.Ny, gH,
+                           g.dx, g.dy, 0.5 * Lx, 0.5 * Ly, -1.0);
+        GLX_LAUNCH_CHECK("k_gauge_shift");
+    }
     hipError_t est = hipEventRecord(e.kev0, e.stream);
     if (est != hipSuccess) return check_hip(est, "hipEventRecord");
-    hipLaunchKernelGGL(k_td_run<R>, dim3((unsigned)wg), dim3(256), 0, e.stream, T);
-    GLX_LAUNCH_CHECK("k_td_run");
+    uint32_t rt = p->rand_t;
+    const bool do_psi = p->eqn != 2, do_A = p->solveA && p->eqn != 1;
+    for (int step = 0; step < nsteps; ++step) {
+        T.step = step;
+        if (do_psi) {
+            T.psi.rand_t = rt++;
+            hipLaunchKernelGGL((k_td_psi_solve<R, G>), dim3((unsigned)wg), dim3(256), 0, e.stream, T);
+            GLX_LAUNCH_CHECK("k_td_psi_solve");
+        }
+        if (do_A) {
+            T.A.rand_t = rt++;
+            hipLaunchKernelGGL((k_td_a_solve<R, G>), dim3((unsigned)wg), dim3(256), 0, e.stream, T);
+            GLX_LAUNCH_CHECK("k_td_a_solve");
+        }
+    }
     est = hipEventRecord(e.kev1, e.stream);
     if (est != hipSuccess) return check_hip(est, "hipEventRecord");
-    int host_state[4] = {0, 0, 0, 0};
+    if (G) {
+        hipLaunchKernelGGL(k_gauge_shift_state<R>, dim3(grid_blocks(Nab)), dim3(256), 0, e.stream, T.Q[0], T.Q[1], T.Q[2],
+                           e.state, g.Nx, g.Ny, gH, g.dx, g.dy, 0.5 * Lx, 0.5 * Ly, 1.0);
+        GLX_LAUNCH_CHECK("k_gauge_shift_state");
+    }
+    int host_state[8] = {0};
     int host_err = 0;
-    hipError_t st = hipMemcpyAsync(host_state, e.state, sizeof(int) * 3, hipMemcpyDeviceToHost, e.stream);
+    hipError_t st = hipMemcpyAsync(host_state, e.state, sizeof(int) * 8, hipMemcpyDeviceToHost, e.stream);
     if (st == hipSuccess) st = hipMemcpyAsync(&host_err, e.err, sizeof(int), hipMemcpyDeviceToHost, e.stream);
     if (st == hipSuccess) st = hipMemcpyAsync(sweeps, e.sweeps, sizeof(int) * 2 * nsteps, hipMemcpyDeviceToHost, e.stream);
     if (st == hipSuccess) st = hipStreamSynchronize(e.stream);
     if (st != hipSuccess) return check_hip(st, "glx_td_run readback");
     float kms = 0.f;
     if (hipEventElapsedTime(&kms, e.kev0, e.kev1) == hipSuccess && e.last_kernel_ms) *e.last_kernel_ms = kms;
-    if (host_err) return set_error(GLX_E_TIMEOUT, "glx_td_run: grid barrier timed out after %d of %d steps", host_state[2], nsteps);
-    if (host_state[2] != nsteps) return set_error(GLX_E_HIP, "glx_td_run: %d of %d steps completed", host_state[2], nsteps);
-    *pc = host_state[0];
-    *qc = host_state[1];
+    if (host_err) return set_error(GLX_E_TIMEOUT, "glx_td_run: grid barrier timed out in step %d of %d", host_state[ST_ERRSTEP], nsteps);
+    if (!do_psi)
+        for (int k = 0; k < nsteps; ++k) sweeps[2 * k] = 0;
+    if (!do_A)
+        for (int k = 0; k < nsteps; ++k) sweeps[2 * k + 1] = 0;
+    *pc = host_state[ST_PSI];
+    *qc = host_state[ST_AB];
     return GLX_OK;
 }
 

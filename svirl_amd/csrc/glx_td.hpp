@@ -32,6 +32,9 @@ template <typename R> struct PsiSweepArgs {
     R langevin_c;
     uint32_t jstep, rand_t;
     double stop_eps;
+    // gauge-shifted storage (FP32-mixed run): the buffers hold A - A_g with
+    // A_g the symmetric gauge of field gH; link phases add A_g back in double
+    double gH, gdx, gdy, gx0, gy0;
 };
 
 template <typename R> struct ASweepArgs {
@@ -50,7 +53,20 @@ template <typename R> struct ASweepArgs {
     R langevin_c;
     uint32_t jstep, rand_t;
     double stop_eps;
+    double gH, gdx, gdy, gx0, gy0;  // as in PsiSweepArgs
 };
+
+// Symmetric gauge of field H (the one svirl builds, params.py:159-167) on
+// link a(c, j) and b(c, j): a_g = -H/2 (y_j - Ly/2), b_g = H/2 (x_c - Lx/2).
+__device__ __forceinline__ double gauge_a(double H, double dy, double y0, int j) { return -0.5 * H * (double(j) * dy - y0); }
+__device__ __forceinline__ double gauge_b(double H, double dx, double x0, int c) { return 0.5 * H * (double(c) * dx - x0); }
+
+// Link phase d*A of a stored value v: plain, or with the gauge added back.
+template <typename R, bool GAUGE>
+__device__ __forceinline__ R link_phase(R d, R v, double g) {
+    if (GAUGE) return R(double(d) * (g + double(v)));
+    return d * v;
+}
 
 template <typename R> __device__ __forceinline__ R shfl_up1(R v) { return __shfl_up(v, 1, WAVE); }
 template <typename R> __device__ __forceinline__ R shfl_dn1(R v) { return __shfl_down(v, 1, WAVE); }
@@ -60,9 +76,45 @@ __device__ __forceinline__ bool cell_at(const uint8_t *mt, int Nxc, int Nyc, int
     return mt ? (mt[c + (long)Nxc * j] != 0) : true;
 }
 
+// Loads of one row for the psi sweep: psi at row j+1, a and b at row j,
+// the cell (c, j), the right-hand side and epsilon at row j, and (lane 63)
+// psi right of the strip at row j. The sweep issues the loads of row j+1
+// before it computes row j, so one row of loads per wave is in flight
+// during the arithmetic.
+template <typename R> struct PsiRow {
+    cplx<R> pp, rhs, pr;
+    R av, bv, e;
+    bool cell;
+};
+
+template <typename R>
+__device__ __forceinline__ PsiRow<R> psi_load_row(const PsiSweepArgs<R> &A, int c, int lane, bool col, bool own, int j) {
+    const int Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1;
+    const long rowoff = (long)Nx * j;
+    PsiRow<R> L;
+    L.pp = {R(0), R(0)};
+    L.rhs = {R(0), R(0)};
+    L.pr = {R(0), R(0)};
+    L.av = R(0);
+    L.bv = R(0);
+    L.e = A.eps;
+    if (col) {
+        if (j + 1 < Ny) L.pp = ld(A.psi, c + rowoff + Nx);
+        if (c < Nxa) L.av = A.ab[c + (long)Nxa * j];
+        if (j < Ny - 1) L.bv = A.ab[(long)Nxa * Ny + c + rowoff];
+    }
+    L.cell = cell_at(A.mt, Nx - 1, Ny - 1, c, j);
+    if (own) {
+        L.rhs = ld(A.rhs, c + rowoff);
+        if (A.eps_map) L.e = A.eps_map[c + rowoff];
+    }
+    if (lane == WAVE - 1 && c + 1 < Nx) L.pr = ld(A.psi, c + 1 + rowoff);
+    return L;
+}
+
 // One psi sweep over rows [ja, jb) of strip s. Returns the lane's largest
 // scaled residual.
-template <typename R>
+template <typename R, bool GAUGE>
 __device__ int psi_rows(const PsiSweepArgs<R> &A, int s, int ja, int jb) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1, Nxc = Nx - 1, Nyc = Ny - 1;
@@ -70,7 +122,6 @@ __device__ int psi_rows(const PsiSweepArgs<R> &A, int s, int ja, int jb) {
     const int c = s * STRIP - 1 + lane;
     const bool col = (c >= 0) && (c < Nx);
     const bool own = col && lane >= 1;
-    const R *a = A.ab;
     const R *b = A.ab + Na;
     const bool noisy = A.rhs_out != nullptr;
     int rmax = 0;
@@ -78,35 +129,27 @@ __device__ int psi_rows(const PsiSweepArgs<R> &A, int s, int ja, int jb) {
     // window: psi at rows j-1, j of this column; U of b(c, j-1); cell (c, j-1)
     cplx<R> pm = {R(0), R(0)}, pc = {R(0), R(0)};
     R cbm = R(1), sbm = R(0);
-    bool cellm = false;
     if (col) {
         pc = ld(A.psi, c + (long)Nx * ja);
         if (ja > 0) {
             pm = ld(A.psi, c + (long)Nx * (ja - 1));
-            sincos_r(A.dy * b[c + (long)Nx * (ja - 1)], &sbm, &cbm);
+            sincos_r(link_phase<R, GAUGE>(A.dy, b[c + (long)Nx * (ja - 1)], gauge_b(A.gH, A.gdx, A.gx0, c)), &sbm, &cbm);
         }
     }
-    cellm = cell_at(A.mt, Nxc, Nyc, c, ja - 1);
+    bool cellm = cell_at(A.mt, Nxc, Nyc, c, ja - 1);
+    PsiRow<R> nx = psi_load_row(A, c, lane, col, own, ja);
 
     for (int j = ja; j < jb; ++j) {
         const long rowoff = (long)Nx * j;
-        cplx<R> pp = {R(0), R(0)};
-        R av = R(0), bv = R(0);
-        if (col) {
-            if (j + 1 < Ny) pp = ld(A.psi, c + rowoff + Nx);
-            if (c < Nxa) av = a[c + (long)Nxa * j];
-            if (j < Ny - 1) bv = b[c + rowoff];
-        }
-        const bool cellc = cell_at(A.mt, Nxc, Nyc, c, j);
-        cplx<R> rhs = {R(0), R(0)};
-        R e = A.eps;
-        if (own) {
-            rhs = ld(A.rhs, c + rowoff);
-            if (A.eps_map) e = A.eps_map[c + rowoff];
-        }
+        const PsiRow<R> cu = nx;
+        if (j + 1 < jb) nx = psi_load_row(A, c, lane, col, own, j + 1);
+        const cplx<R> pp = cu.pp;
+        const bool cellc = cu.cell;
+        cplx<R> rhs = cu.rhs;
+        const R e = cu.e;
         R sa, ca, sb, cb;
-        sincos_r(A.dx * av, &sa, &ca);
-        sincos_r(A.dy * bv, &sb, &cb);
+        sincos_r(link_phase<R, GAUGE>(A.dx, cu.av, gauge_a(A.gH, A.gdy, A.gy0, j)), &sa, &ca);
+        sincos_r(link_phase<R, GAUGE>(A.dy, cu.bv, gauge_b(A.gH, A.gdx, A.gx0, c)), &sb, &cb);
 
         // neighbours from adjacent lanes
         const R pl_re = shfl_up1(pc.re), pl_im = shfl_up1(pc.im);
@@ -114,10 +157,9 @@ __device__ int psi_rows(const PsiSweepArgs<R> &A, int s, int ja, int jb) {
         const R cal = shfl_up1(ca), sal = shfl_up1(sa);
         const bool cl_m = shfl_up1((int)cellm) != 0;  // cell (c-1, j-1)
         const bool cl_c = shfl_up1((int)cellc) != 0;  // cell (c-1, j)
-        if (lane == WAVE - 1 && c + 1 < Nx) {
-            cplx<R> q = ld(A.psi, c + 1 + rowoff);
-            pr_re = q.re;
-            pr_im = q.im;
+        if (lane == WAVE - 1) {
+            pr_re = cu.pr.re;
+            pr_im = cu.pr.im;
         }
 
         if (own) {
@@ -166,9 +208,48 @@ __device__ int psi_rows(const PsiSweepArgs<R> &A, int s, int ja, int jb) {
     return rmax;
 }
 
+// Loads of one row for the A sweep: a(c, j+1), b(c, j), psi(c, j+1), the
+// cell (c, j), the owned links' source and right-hand side at row j, and
+// (lane 63) b and psi right of the strip at row j.
+template <typename R> struct ARow {
+    R ap, bc, sa, sb, ra, rb, brc;
+    cplx<R> pp, pr;
+    bool cell;
+};
+
+template <typename R>
+__device__ __forceinline__ ARow<R> a_load_row(const ASweepArgs<R> &A, int c, int lane, bool col, bool own, int j) {
+    const int Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1;
+    const long Na = (long)Nxa * Ny;
+    const long rowoff = (long)Nx * j;
+    const R *a = A.ab;
+    const R *b = A.ab + Na;
+    ARow<R> L;
+    L.ap = R(0); L.bc = R(0); L.sa = R(0); L.sb = R(0); L.ra = R(0); L.rb = R(0); L.brc = R(0);
+    L.pp = {R(0), R(0)};
+    L.pr = {R(0), R(0)};
+    if (col) {
+        if (c < Nxa && j + 1 < Ny) L.ap = a[c + (long)Nxa * (j + 1)];
+        if (j < Ny - 1) {
+            L.bc = b[c + rowoff];
+            L.pp = ld(A.psi, c + rowoff + Nx);
+        }
+    }
+    L.cell = cell_at(A.mt, Nx - 1, Ny - 1, c, j);
+    const long na = c + (long)Nxa * j;
+    const long nb = Na + c + rowoff;
+    if (own && c < Nxa) { L.sa = A.src[na]; L.ra = A.rhs[na]; }
+    if (own && j < Ny - 1) { L.sb = A.src[nb]; L.rb = A.rhs[nb]; }
+    if (lane == WAVE - 1 && c + 1 < Nx) {
+        if (j < Ny - 1) L.brc = b[c + 1 + rowoff];
+        L.pr = ld(A.psi, c + 1 + rowoff);
+    }
+    return L;
+}
+
 // One A sweep over rows [ja, jb) of strip s (both link families of every
 // owned node).
-template <typename R>
+template <typename R, bool GAUGE>
 __device__ int a_rows(const ASweepArgs<R> &A, int s, int ja, int jb) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1, Nxc = Nx - 1, Nyc = Ny - 1;
@@ -196,49 +277,40 @@ __device__ int a_rows(const ASweepArgs<R> &A, int s, int ja, int jb) {
     }
     if (lane == WAVE - 1 && c + 1 < Nx && ja > 0) brm = b[c + 1 + (long)Nx * (ja - 1)];
     bool cellm = cell_at(A.mt, Nxc, Nyc, c, ja - 1);
+    ARow<R> nx = a_load_row(A, c, lane, col, own, ja);
 
     for (int j = ja; j < jb; ++j) {
-        const long rowoff = (long)Nx * j;
-        R ap = R(0), bc = R(0);
-        cplx<R> pp = {R(0), R(0)};
-        if (col) {
-            if (hasa && j + 1 < Ny) ap = a[c + (long)Nxa * (j + 1)];
-            if (j < Ny - 1) {
-                bc = b[c + rowoff];
-                pp = ld(A.psi, c + rowoff + Nx);
-            }
-        }
-        const bool cellc = cell_at(A.mt, Nxc, Nyc, c, j);
+        const ARow<R> cu = nx;
+        if (j + 1 < jb) nx = a_load_row(A, c, lane, col, own, j + 1);
+        const R ap = cu.ap, bc = cu.bc;
+        const cplx<R> pp = cu.pp;
+        const bool cellc = cu.cell;
         // right column values
         R brc = shfl_dn1(bc), brm_s = shfl_dn1(bm);
         R pr_re = shfl_dn1(pc.re), pr_im = shfl_dn1(pc.im);
-        if (lane == WAVE - 1 && c + 1 < Nx) {
+        if (lane == WAVE - 1) {
             brm_s = brm;
-            brc = (j < Ny - 1) ? b[c + 1 + rowoff] : R(0);
-            cplx<R> q = ld(A.psi, c + 1 + rowoff);
-            pr_re = q.re;
-            pr_im = q.im;
+            brc = cu.brc;
+            pr_re = cu.pr.re;
+            pr_im = cu.pr.im;
             brm = brc;
         }
         // left column values
         const R alc = shfl_up1(ac), alp = shfl_up1(ap), blc = shfl_up1(bc);
         const bool cl_c = shfl_up1((int)cellc) != 0;  // cell (c-1, j)
 
-        // source links and right-hand sides of the owned links
-        R sa_src = R(0), sb_src = R(0), ra = R(0), rb = R(0);
         const long na = c + (long)Nxa * j;
-        const long nb = Na + c + rowoff;
+        const long nb = Na + c + (long)Nx * j;
         const bool owna = own && c < Nxa;
         const bool ownb = own && j < Ny - 1;
-        if (owna) { sa_src = A.src[na]; ra = A.rhs[na]; }
-        if (ownb) { sb_src = A.src[nb]; rb = A.rhs[nb]; }
+        R ra = cu.ra, rb = cu.rb;
         if (noisy && A.jstep == 0 && A.langevin_c > R(1.0e-32)) {
             if (owna) ra = ra + A.langevin_c * R(double(noise_1<R>((uint32_t)na, A.rand_t)) - 0.5);
             if (ownb) rb = rb + A.langevin_c * R(double(noise_2<R>((uint32_t)nb, A.rand_t)) - 0.5);
         }
         R ssa, csa, ssb, csb;
-        sincos_r(A.dx * sa_src, &ssa, &csa);
-        sincos_r(A.dy * sb_src, &ssb, &csb);
+        sincos_r(link_phase<R, GAUGE>(A.dx, cu.sa, gauge_a(A.gH, A.gdy, A.gy0, j)), &ssa, &csa);
+        sincos_r(link_phase<R, GAUGE>(A.dy, cu.sb, gauge_b(A.gH, A.gdx, A.gx0, c)), &ssb, &csb);
 
         double r = 0.0;
         if (owna) {
@@ -268,8 +340,7 @@ __device__ int a_rows(const ASweepArgs<R> &A, int s, int ja, int jb) {
                 jl = A.idy * ((x0 * y1 - y0 * x1) * csb - (x0 * x1 + y0 * y1) * ssb);
             }
             R t1 = R(0), t2 = R(0);
-            const R blc_ = blc;
-            if (c > 0) t1 = A.idx2 * blc_ - A.idxy * alc + A.idxy * alp;
+            if (c > 0) t1 = A.idx2 * blc - A.idxy * alc + A.idxy * alp;
             if (c + 1 < Nx) t2 = A.idx2 * brc + A.idxy * ac - A.idxy * ap;
             const R v = (rb + A.dt_rho * (jl + rh) + A.dt_rho_kappa2 * dd * (t1 + t2)) / A.den_b;
             if (noisy && A.jstep == 0) A.rhs_out[nb] = rb;
@@ -313,7 +384,7 @@ __global__ void __launch_bounds__(256) k_psi_sweep(PsiSweepArgs<R> A, int *r2max
     __shared__ int lds[4];
     const long w = (long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     int rmax = 0;
-    if (w < W) for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rmax = max(rmax, psi_rows(A, s, ja, jb)); });
+    if (w < W) for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rmax = max(rmax, psi_rows<R, false>(A, s, ja, jb)); });
     rmax = block_max_int(rmax, lds);
     if (threadIdx.x == 0) atomic_max_agent(r2max, rmax);
 }
@@ -323,143 +394,196 @@ __global__ void __launch_bounds__(256) k_a_sweep(ASweepArgs<R> A, int *r2max, lo
     __shared__ int lds[4];
     const long w = (long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     int rmax = 0;
-    if (w < W) for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rmax = max(rmax, a_rows(A, s, ja, jb)); });
+    if (w < W) for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rmax = max(rmax, a_rows<R, false>(A, s, ja, jb)); });
     rmax = block_max_int(rmax, lds);
     if (threadIdx.x == 0) atomic_max_agent(r2max, rmax);
 }
 
-// ---- persistent multi-step kernel ------------------------------------------
+// ---- persistent solve kernels ----------------------------------------------
+//
+// One launch runs one Jacobi solve (psi or A) to the reference's stop rule:
+// every sweep ends at a grid barrier after which every workgroup reads the
+// same residual slots and takes the same branch. Per-step launches are
+// enqueued back to back; the buffer holding the current state, the barrier
+// epoch and the global sweep counter carry over between launches in device
+// memory, so there is no reset launch and no host round trip per step.
 
 constexpr int SLOT_RING = 64;  // residual slots, one per sweep, NGRP ints each
 
+enum { ST_PSI = 0, ST_AB = 1, ST_EPOCH = 2, ST_GSWEEP = 4, ST_ERRSTEP = 6 };
+
 struct TDScratch {
     GridSync gs;
-    int *slots;   // SLOT_RING * NGRP
-    int *state;   // [0] psi_cur, [1] ab_cur, [2] steps done
-    int *sweeps;  // 2 * nsteps
+    int *slots;             // SLOT_RING * NGRP
+    int *state;             // ST_* fields (64-bit epoch and sweep counters at ST_EPOCH, ST_GSWEEP)
+    int *sweeps;            // 2 * nsteps
 };
 
-template <typename R> struct TDRunArgs {
-    PsiSweepArgs<R> psi;   // per-sweep pointers are filled in the kernel
+template <typename R> struct TDSolveArgs {
+    PsiSweepArgs<R> psi;    // per-sweep pointers are filled in the kernel
     ASweepArgs<R> A;
     cplx<R> *P[3];
     R *Q[3];
-    cplx<R> *rhs_psi;      // Langevin rhs scratch or NULL
+    cplx<R> *rhs_psi;       // Langevin rhs scratch or NULL
     R *rhs_ab;
-    int psi_cur, ab_cur;
-    int solveA, eqn, max_sweeps, nsteps;
+    int max_sweeps;
+    int step;               // index into sweeps[]
     TDScratch sc;
 };
 
-// Residual of the sweep just finished, read after the barrier.
-__device__ __forceinline__ int slot_max(int *slots, unsigned long long gsweep) {
-    int *s = slots + (gsweep % SLOT_RING) * NGRP;
-    int m = 0;
-#pragma unroll
-    for (int k = 0; k < NGRP; ++k) m = max(m, ld_acquire(s + k));
-    return m;
+__device__ __forceinline__ unsigned long long *state64(int *state, int k) {
+    return reinterpret_cast<unsigned long long *>(state + k);
 }
 
-template <typename R>
-__global__ void __launch_bounds__(256) k_td_run(TDRunArgs<R> T) {
+// Fold the workgroup's residual into the slot of this sweep, meet at the
+// barrier, read the sweep's grid-wide residual. Block 0 clears the slot 32
+// sweeps ahead (every block has finished reading it).
+__device__ __forceinline__ bool sweep_sync(const TDScratch &sc, int rmax, unsigned long long &epoch,
+                                           unsigned long long gsweep, int *lds, int &r_out) {
+    rmax = block_max_int(rmax, lds);
+    if (threadIdx.x == 0) {
+        atomic_max_agent(sc.slots + (gsweep % SLOT_RING) * NGRP + blockIdx.x % NGRP, rmax);
+        if (blockIdx.x == 0) {
+            int *z = sc.slots + ((gsweep + SLOT_RING / 2) % SLOT_RING) * NGRP;
+            for (int k = 0; k < NGRP; ++k) __hip_atomic_store(z + k, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    if (!grid_barrier(sc.gs, epoch)) return false;
+    if (threadIdx.x == 0) {
+        int *sl = sc.slots + (gsweep % SLOT_RING) * NGRP;
+        int m = 0;
+        for (int k = 0; k < NGRP; ++k) m = max(m, ld_acquire(sl + k));
+        lds[8] = m;
+    }
+    __syncthreads();
+    r_out = lds[8];
+    return true;
+}
+
+template <typename R, bool GAUGE>
+__global__ void __launch_bounds__(256) k_td_psi_solve(TDSolveArgs<R> T) {
     __shared__ int lds[16];
     const long W = (long)gridDim.x * (blockDim.x / WAVE);
     const long w = (long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    unsigned long long epoch = 0, gsweep = 0;
-    int pc = T.psi_cur, qc = T.ab_cur;
-    uint32_t rand_t = T.psi.rand_t;
-    int *slots = T.sc.slots;
-
-    auto sweep_done = [&](int rmax, int &r_out) -> bool {
-        rmax = block_max_int(rmax, lds);
-        if (threadIdx.x == 0) {
-            atomic_max_agent(slots + (gsweep % SLOT_RING) * NGRP + blockIdx.x % NGRP, rmax);
-            if (blockIdx.x == 0) {
-                int *z = slots + ((gsweep + SLOT_RING / 2) % SLOT_RING) * NGRP;
-                for (int k = 0; k < NGRP; ++k)
-                    __hip_atomic_store(z + k, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
+    int *st = T.sc.state;
+    const int pc = st[ST_PSI], qc = st[ST_AB];
+    const unsigned long long e0 = *state64(st, ST_EPOCH), g0 = *state64(st, ST_GSWEEP);
+    unsigned long long epoch = e0, gsweep = g0;
+    const int o1 = (pc + 1) % 3, o2 = (pc + 2) % 3;
+    PsiSweepArgs<R> A = T.psi;
+    A.ab = T.Q[qc];
+    int in = pc, out = o1, nsw = 0;
+    for (int j = 0; j < T.max_sweeps; ++j) {
+        A.jstep = (uint32_t)j;
+        A.psi = T.P[in];
+        A.psi_next = T.P[out];
+        A.rhs = (T.rhs_psi && j > 0) ? T.rhs_psi : T.P[pc];
+        A.rhs_out = T.rhs_psi;
+        int rmax = 0;
+        for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rmax = max(rmax, psi_rows<R, GAUGE>(A, s, ja, jb)); });
+        int r = 0;
+        if (!sweep_sync(T.sc, rmax, epoch, gsweep, lds, r)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) st[ST_ERRSTEP] = T.step + 1;
+            return;
         }
-        if (!grid_barrier(T.sc.gs, epoch)) return false;
-        if (threadIdx.x == 0) lds[8] = slot_max(slots, gsweep);
-        __syncthreads();
-        r_out = lds[8];
         ++gsweep;
-        return true;
-    };
-
-    for (int step = 0; step < T.nsteps; ++step) {
-        int spsi = 0, sA = 0;
-        if (T.eqn != 2) {
-            // psi solve: rhs = P[pc] (or the Langevin scratch), ping-pong over the other two
-            const int o1 = (pc + 1) % 3, o2 = (pc + 2) % 3;
-            PsiSweepArgs<R> A = T.psi;
-            A.rand_t = rand_t;
-            A.ab = T.Q[qc];  // the vector potential is fixed during the psi solve
-            int in = pc, out = o1;
-            for (int j = 0; j < T.max_sweeps; ++j) {
-                A.jstep = (uint32_t)j;
-                A.psi = T.P[in];
-                A.psi_next = T.P[out];
-                A.rhs = (T.rhs_psi && j > 0) ? T.rhs_psi : T.P[pc];
-                A.rhs_out = T.rhs_psi;
-                int rmax = 0;
-                for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rmax = max(rmax, psi_rows(A, s, ja, jb)); });
-                int r = 0;
-                if (!sweep_done(rmax, r)) return;
-                spsi = j + 1;
-                in = out;
-                out = (out == o1) ? o2 : o1;
-                if (r < 10000) break;
-            }
-            pc = in;
-            rand_t += 1u;
-        }
-        if (T.solveA && T.eqn != 1) {
-            const int o1 = (qc + 1) % 3, o2 = (qc + 2) % 3;
-            ASweepArgs<R> A = T.A;
-            A.rand_t = rand_t;
-            A.psi = T.P[pc];
-            int in = qc, out = o1;
-            for (int j = 0; j < T.max_sweeps; ++j) {
-                A.jstep = (uint32_t)j;
-                A.ab = T.Q[in];
-                A.ab_next = T.Q[out];
-                // source A^(2*floor(j/2)): the input on even sweeps, A^n on
-                // sweep 1, the output buffer's previous contents on odd j >= 3
-                A.src = (j % 2 == 0) ? T.Q[in] : (j == 1 ? T.Q[qc] : T.Q[out]);
-                A.rhs = (T.rhs_ab && j > 0) ? T.rhs_ab : T.Q[qc];
-                A.rhs_out = T.rhs_ab;
-                int rmax = 0;
-                for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rmax = max(rmax, a_rows(A, s, ja, jb)); });
-                int r = 0;
-                if (!sweep_done(rmax, r)) return;
-                sA = j + 1;
-                in = out;
-                out = (out == o1) ? o2 : o1;
-                if (r < 10000) break;
-            }
-            qc = in;
-            rand_t += 1u;
-        }
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            T.sc.sweeps[2 * step] = spsi;
-            T.sc.sweeps[2 * step + 1] = sA;
-            T.sc.state[0] = pc;
-            T.sc.state[1] = qc;
-            T.sc.state[2] = step + 1;
-        }
+        nsw = j + 1;
+        in = out;
+        out = (out == o1) ? o2 : o1;
+        if (r < 10000) break;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st[ST_PSI] = in;
+        *state64(st, ST_EPOCH) = epoch;
+        *state64(st, ST_GSWEEP) = gsweep;
+        T.sc.sweeps[2 * T.step] = nsw;
     }
 }
 
-static __global__ void k_td_reset(TDScratch sc) {
+template <typename R, bool GAUGE>
+__global__ void __launch_bounds__(256) k_td_a_solve(TDSolveArgs<R> T) {
+    __shared__ int lds[16];
+    const long W = (long)gridDim.x * (blockDim.x / WAVE);
+    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    int *st = T.sc.state;
+    const int pc = st[ST_PSI], qc = st[ST_AB];
+    const unsigned long long e0 = *state64(st, ST_EPOCH), g0 = *state64(st, ST_GSWEEP);
+    unsigned long long epoch = e0, gsweep = g0;
+    const int o1 = (qc + 1) % 3, o2 = (qc + 2) % 3;
+    ASweepArgs<R> A = T.A;
+    A.psi = T.P[pc];
+    int in = qc, out = o1, nsw = 0;
+    for (int j = 0; j < T.max_sweeps; ++j) {
+        A.jstep = (uint32_t)j;
+        A.ab = T.Q[in];
+        A.ab_next = T.Q[out];
+        // source A^(2*floor(j/2)): the input on even sweeps, A^n on sweep 1,
+        // the output buffer's previous contents on odd j >= 3
+        A.src = (j % 2 == 0) ? T.Q[in] : (j == 1 ? T.Q[qc] : T.Q[out]);
+        A.rhs = (T.rhs_ab && j > 0) ? T.rhs_ab : T.Q[qc];
+        A.rhs_out = T.rhs_ab;
+        int rmax = 0;
+        for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rmax = max(rmax, a_rows<R, GAUGE>(A, s, ja, jb)); });
+        int r = 0;
+        if (!sweep_sync(T.sc, rmax, epoch, gsweep, lds, r)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) st[ST_ERRSTEP] = T.step + 1;
+            return;
+        }
+        ++gsweep;
+        nsw = j + 1;
+        in = out;
+        out = (out == o1) ? o2 : o1;
+        if (r < 10000) break;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st[ST_AB] = in;
+        *state64(st, ST_EPOCH) = epoch;
+        *state64(st, ST_GSWEEP) = gsweep;
+        T.sc.sweeps[2 * T.step + 1] = nsw;
+    }
+}
+
+// Zero the barrier counters, residual slots and run state (before the first
+// solve of a run, or when the grid size changes) and set the buffer indices.
+static __global__ void k_td_reset(TDScratch sc, int pc, int qc) {
     const int t = threadIdx.x;
     for (int k = t; k < SLOT_RING * NGRP; k += blockDim.x) sc.slots[k] = 0;
-    if (t < NGRP) sc.gs.grp[t] = 0ull;
+    for (int k = t; k < NGRP * GS_STRIDE; k += blockDim.x) { sc.gs.grp[k] = 0ull; sc.gs.rel[k] = 0ull; }
     if (t == 0) {
         *sc.gs.top = 0ull;
         *sc.gs.err = 0;
-        sc.state[2] = 0;
+        sc.state[ST_PSI] = pc;
+        sc.state[ST_AB] = qc;
+        *state64(sc.state, ST_EPOCH) = 0ull;
+        *state64(sc.state, ST_GSWEEP) = 0ull;
+        sc.state[ST_ERRSTEP] = 0;
+    }
+}
+
+// A <-> A - A_g on every link, in place (sign -1: remove, +1: add back).
+template <typename R>
+__global__ void k_gauge_shift(R *ab, int Nx, int Ny, double H, double dx, double dy, double x0, double y0, double sign) {
+    const long Na = (long)(Nx - 1) * Ny, Nab = Na + (long)Nx * (Ny - 1);
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < Nab; k += (long)gridDim.x * blockDim.x) {
+        double g;
+        if (k < Na) g = gauge_a(H, dy, y0, (int)(k / (Nx - 1)));
+        else g = gauge_b(H, dx, x0, (int)((k - Na) % Nx));
+        ab[k] = R(double(ab[k]) + sign * g);
+    }
+}
+
+// k_gauge_shift on the buffer holding the run's current vector potential.
+template <typename R>
+__global__ void k_gauge_shift_state(R *q0, R *q1, R *q2, const int *state, int Nx, int Ny, double H, double dx,
+                                    double dy, double x0, double y0, double sign) {
+    const int k = state[ST_AB];
+    R *ab = k == 0 ? q0 : (k == 1 ? q1 : q2);
+    const long Na = (long)(Nx - 1) * Ny, Nab = Na + (long)Nx * (Ny - 1);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < Nab; n += (long)gridDim.x * blockDim.x) {
+        double g;
+        if (n < Na) g = gauge_a(H, dy, y0, (int)(n / (Nx - 1)));
+        else g = gauge_b(H, dx, x0, (int)((n - Na) % Nx));
+        ab[n] = R(double(ab[n]) + sign * g);
     }
 }
 

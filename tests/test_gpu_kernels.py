@@ -162,6 +162,66 @@ def test_td_run_matches_oracle(shape, kappa, mask, epsmap, nsteps, wg):
     assert np.max(np.abs(Q[qc.value].get() - st.ab)) <= 1e-11
 
 
+def _td_run_f32(g, psi, ab, em, mt, prm, nsteps, wg=0):
+    P = [D(psi, np.complex64), nat.DeviceArray.zeros(g.N, np.complex64), nat.DeviceArray.zeros(g.N, np.complex64)]
+    Q = [D(ab, np.float32), nat.DeviceArray.zeros(g.Nab, np.float32), nat.DeviceArray.zeros(g.Nab, np.float32)]
+    Pp = (ctypes.c_void_p * 3)(*[b.ptr for b in P])
+    Qp = (ctypes.c_void_p * 3)(*[b.ptr for b in Q])
+    pc, qc = ctypes.c_int32(0), ctypes.c_int32(0)
+    sw = (ctypes.c_int32 * (2 * nsteps))()
+    d_em = D(em, np.float32) if em is not None else None
+    d_mt = D(mt.astype(np.uint8), np.uint8) if mt is not None else None
+    nat.call("glx_td_run", nat.GLX_F32, grid(g), ctypes.byref(prm), nat.ptr(d_em), nat.ptr(d_mt), Pp,
+             ctypes.byref(pc), Qp, ctypes.byref(qc), None, None, nsteps, sw, wg, 256)
+    sweeps = np.frombuffer(sw, dtype=np.int32).reshape(nsteps, 2).copy()
+    return P[pc.value].get().astype(np.complex128), Q[qc.value].get().astype(np.float64), sweeps
+
+
+@pytest.mark.parametrize("shape,kappa,mask,epsmap,nsteps", [
+    ((33, 29), 5.0, True, True, 12),
+    ((70, 41), np.inf, False, False, 15),
+    ((130, 17), 3.0, True, False, 6),
+])
+def test_td_run_fp32_mixed(shape, kappa, mask, epsmap, nsteps):
+    """FP32-mixed persistent run (A stored relative to the symmetric gauge
+    inside the run) against the FP64 oracle: fields within float
+    resolution, sweep counts within one of the FP64 counts."""
+    rng, g, mt, psi, ab, em = case(99 + shape[0], *shape, mask=mask, epsmap=epsmap)
+    ab = O.symmetric_gauge(g, 0.25) + 0.05 * rng.normal(size=g.Nab)
+    psi = np.where(O.mask_at_nodes(g, mt), psi, 0)
+    psi32 = psi.astype(np.complex64).astype(np.complex128)
+    ab32 = ab.astype(np.float32).astype(np.float64)
+    em32 = em.astype(np.float32).astype(np.float64) if em is not None else None
+    st = O.OracleState(g, psi32, ab32, kappa=kappa, sigma=100.0, H=0.25, eps=1.0, eps_map=em32, mt=mt)
+    st.td(dt=0.1, Nt=nsteps)
+    prm = _td_params(0.1, 0.0 if em is not None else 1.0, st.kappa2, st.rho, 0.25, 1e-6, 1e-6, st.solveA)
+    p, a, sweeps = _td_run_f32(g, psi32, ab32, em32, mt, prm, nsteps)
+    ref = np.array(st.sweeps)
+    assert np.all(np.abs(sweeps - ref) <= 1), (sweeps.tolist(), st.sweeps)
+    assert np.max(np.abs(p - st.psi)) <= 2e-5
+    assert np.max(np.abs(a - st.ab)) <= 2e-5
+
+
+def test_td_run_fp32_large_potential():
+    """|A| ~ 26 at the edges (float ulp 1.9e-6, above the 1e-6 stop
+    threshold): the FP32-mixed run still converges in a few sweeps per step
+    and tracks the FP64 oracle."""
+    g = O.Geom(513, 41, 0.5, 0.5)
+    H = 0.4
+    psi = O.random_psi(g.N, 1.0, seed=5)
+    ab = O.symmetric_gauge(g, H)
+    st = O.OracleState(g, psi, ab, kappa=5.0, sigma=100.0, H=H)
+    st.td(dt=0.1, Nt=4)
+    psi32 = psi.astype(np.complex64).astype(np.complex128)
+    prm = _td_params(0.1, 1.0, st.kappa2, st.rho, H, 1e-6, 1e-6, True)
+    p, a, sweeps = _td_run_f32(g, psi32, ab.astype(np.float32).astype(np.float64), None, None, prm, 4)
+    ref = np.array(st.sweeps)
+    assert np.all(sweeps[:, 1] <= ref[:, 1] + 1), (sweeps.tolist(), st.sweeps)
+    assert np.all(np.abs(sweeps[:, 0] - ref[:, 0]) <= 1), (sweeps.tolist(), st.sweeps)
+    assert np.max(np.abs(p - st.psi)) <= 5e-5
+    assert np.max(np.abs(a - st.ab)) <= 2e-5 * (1 + np.max(np.abs(st.ab)))
+
+
 @pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("shape", [(5, 4), (64, 9), (130, 33)])
 def test_observables(prec, shape):
