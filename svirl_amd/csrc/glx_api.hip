@@ -85,17 +85,15 @@ int glx_init(int device) {
     if ((st = hipEventCreate(&g_ctx.env.kev0)) != hipSuccess) return check_hip(st, "hipEventCreate");
     if ((st = hipEventCreate(&g_ctx.env.kev1)) != hipSuccess) return check_hip(st, "hipEventCreate");
     g_ctx.env.last_kernel_ms = &g_ctx.last_kernel_ms;
-    // scratch layout: partials | result | grp | top | rel | err | slots | state | sweeps
+    // scratch layout: partials | result | arrive | rel | err | state | sweeps
     const size_t partials = sizeof(double) * 1024 * 32;
     const size_t result = sizeof(double) * 64;
-    const size_t grp = sizeof(unsigned long long) * 8 * 64;
-    const size_t top = 512;
-    const size_t rel = sizeof(unsigned long long) * 8 * 64;
+    const size_t arrive = sizeof(unsigned long long) * MAX_GRID_BLOCKS;
+    const size_t rel = sizeof(unsigned long long) * NREL * REL_STRIDE;
     const size_t err = 512;
-    const size_t slots = sizeof(int) * 64 * 8;
     const size_t state = 512;
     const size_t sweeps = sizeof(int) * 2 * SWEEPS_CAP;
-    const size_t total = partials + result + grp + top + rel + err + slots + state + sweeps;
+    const size_t total = partials + result + arrive + rel + err + state + sweeps;
     if ((st = hipMalloc(&g_ctx.scratch, total)) != hipSuccess) return check_hip(st, "hipMalloc(scratch)");
     if ((st = hipMemset(g_ctx.scratch, 0, total)) != hipSuccess) return check_hip(st, "hipMemset(scratch)");
     char *p = (char *)g_ctx.scratch;
@@ -103,11 +101,9 @@ int glx_init(int device) {
     e.stream = g_ctx.stream;
     e.partials = (double *)p; p += partials;
     e.result = (double *)p; p += result;
-    e.grp = (unsigned long long *)p; p += grp;
-    e.top = (unsigned long long *)p; p += top;
+    e.arrive = (unsigned long long *)p; p += arrive;
     e.rel = (unsigned long long *)p; p += rel;
     e.err = (int *)p; p += err;
-    e.slots = (int *)p; p += slots;
     e.state = (int *)p; p += state;
     e.sweeps = (int *)p; p += sweeps;
     e.sweeps_cap = SWEEPS_CAP;

@@ -4,14 +4,20 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "glx_limits.hpp"
+
 namespace glx {
 
 constexpr int WAVE = 64;
-// Columns owned by one wave in the row-marching kernels: lane 0 carries the
-// column left of the strip (halo), lanes 1..63 own 63 consecutive columns.
-constexpr int STRIP = WAVE - 1;
+// Columns owned by one wave in the row-marching kernels: lanes 0 and 63 carry
+// the columns left and right of the strip (halos), lanes 1..62 own 62
+// consecutive columns.
+constexpr int STRIP = WAVE - 2;
 
 template <typename R> struct cplx { R re, im; };
+
+// Index of the wave within its workgroup, as a wave-uniform (scalar) value.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE); }
 
 template <typename R> __device__ __forceinline__ cplx<R> ld(const cplx<R> *p, long n);
 template <> __device__ __forceinline__ cplx<double> ld(const cplx<double> *p, long n) {
@@ -162,77 +168,109 @@ __device__ __forceinline__ int block_max_int(int v, int *lds) {
 
 // ---- agent-scope atomics ---------------------------------------------------
 
-__device__ __forceinline__ unsigned long long ld_acquire(unsigned long long *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ int ld_acquire(int *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ void atomic_max_agent(int *p, int v) {
     __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// ---- grid barrier for persistent kernels ----------------------------------
+// ---- grid barrier with a max-reduction, for persistent kernels ----------
 //
-// Blocks are grouped by blockIdx % NGRP. Arrival: one agent-scope atomic per
-// block on its group counter; the last arriver of a group bumps the top
-// counter; the last group to arrive writes the epoch into every group's
-// release flag; each block polls only its own group's flag. Counters and
-// flags sit on separate 512-byte lines so that the atomics of different
-// groups do not serialise on one memory channel. All counters are
-// monotonic (epochs carry over between launches). Every wait is bounded by
-// a wall-clock timeout: a block that times out raises `err`, every block
-// leaves its loops at the next barrier and the host reports an error, so a
-// launch that is not fully co-resident cannot hang.
-constexpr int NGRP = 8;
-constexpr int GS_STRIDE = 64;  // u64 elements between counters (512 bytes)
+// Arrival: each workgroup makes its stores visible device-wide (release
+// fence), then writes (epoch << 32 | value) into its own word of `arrive`.
+// Workgroup 0 polls all arrival words with plain device-coherent loads (all
+// of its threads in parallel), takes the max of the values once every word
+// carries the current epoch, and writes (epoch << 32 | max) into NREL release
+// words; every other workgroup polls one release word. No two workgroups
+// contend for one atomic, polls never invalidate the caches (one acquire
+// fence follows a successful poll), and the cost is a few memory round trips
+// whatever the number of workgroups. A poll that waits longer than
+// BARRIER_TIMEOUT_TICKS sets `err`, which every poller also watches, so a
+// grid that is not co-resident fails instead of hanging.
 constexpr unsigned long long BARRIER_TIMEOUT_TICKS = 100ull * 1000 * 1000 * 5;  // 5 s at 100 MHz
 
 struct GridSync {
-    unsigned long long *grp;  // NGRP counters, GS_STRIDE apart
-    unsigned long long *top;  // 1 counter
-    unsigned long long *rel;  // NGRP release flags, GS_STRIDE apart
-    int *err;                 // 1 flag
+    unsigned long long *arrive;  // MAX_GRID_BLOCKS words
+    unsigned long long *rel;     // NREL words, REL_STRIDE apart
+    int *err;                    // 1 flag
 };
 
-// Call from every thread of every block the same number of times; `epoch`
-// is the number of barriers already passed (only thread 0's copy is used).
-// Returns false when the barrier failed (timeout anywhere in the grid).
-__device__ __forceinline__ bool grid_barrier(const GridSync &gs, unsigned long long &epoch) {
-    __shared__ int s_ok;
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have reached the cache
+__device__ __forceinline__ unsigned long long ld_relaxed(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_relaxed(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Call from every thread of every workgroup the same number of times with the
+// same `epoch` (barriers already passed; incremented here). `v` is the
+// workgroup's value (thread 0's is used): two 16-bit fields. On success
+// returns true and sets `vmax` (all threads) to the field-wise max over the
+// grid; returns false when a poll timed out anywhere in the grid.
+__device__ __forceinline__ bool grid_sync_max(const GridSync &gs, unsigned long long &epoch, int v, int &vmax) {
+    __shared__ int s_red[16];
+    __shared__ int s_ok, s_max;
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have left the wave
     __syncthreads();
+    ++epoch;
+    const unsigned long long tag = epoch << 32;
     if (threadIdx.x == 0) {
-        __threadfence();  // release the workgroup's stores to the whole device
-        ++epoch;
-        const unsigned nb = gridDim.x;
-        const unsigned g = blockIdx.x % NGRP;
-        const unsigned gsize = (nb - g + NGRP - 1) / NGRP;
-        const unsigned ngroups = nb < (unsigned)NGRP ? nb : (unsigned)NGRP;
-        const unsigned long long old =
-            __hip_atomic_fetch_add(gs.grp + g * GS_STRIDE, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old + 1 == epoch * gsize) {
-            const unsigned long long t =
-                __hip_atomic_fetch_add(gs.top, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            if (t + 1 == epoch * ngroups)
-                for (unsigned k = 0; k < ngroups; ++k)
-                    __hip_atomic_store(gs.rel + k * GS_STRIDE, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        int ok = 1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(gs.arrive + blockIdx.x, tag | (unsigned)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (blockIdx.x == 0) {
         const unsigned long long t0 = wall_clock64();
-        while (ld_acquire(gs.rel + g * GS_STRIDE) < epoch) {
-            if (ld_acquire(gs.err)) { ok = 0; break; }
+        int ok = 1, m = 0, mh = 0;
+        for (;;) {
+            int all = 1;
+            m = 0;
+            mh = 0;
+            for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x) {
+                const unsigned long long a = ld_relaxed(gs.arrive + k);
+                if ((a >> 32) != epoch) all = 0;
+                else {
+                    m = max(m, (int)(unsigned)(a & 0xffffull));
+                    mh = max(mh, (int)(unsigned)((a >> 16) & 0xffffull));
+                }
+            }
+            int abort = 0;
+            if (threadIdx.x == 0) abort = (wall_clock64() - t0 > BARRIER_TIMEOUT_TICKS) || ld_relaxed(gs.err);
+            if (__syncthreads_or(abort)) { ok = 0; break; }
+            if (__syncthreads_and(all)) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (ok) {  // uniform over the workgroup
+            m = block_max_int(m, s_red);
+            mh = block_max_int(mh, s_red);
+            if (threadIdx.x == 0) s_max = (mh << 16) | m;
+            __syncthreads();
+            if (threadIdx.x < NREL) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+                __hip_atomic_store(gs.rel + threadIdx.x * REL_STRIDE, tag | (unsigned)s_max, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else if (threadIdx.x == 0) {
+            __hip_atomic_store(gs.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (threadIdx.x == 0) s_ok = ok;
+    } else if (threadIdx.x == 0) {
+        const unsigned long long *r = gs.rel + (blockIdx.x % NREL) * REL_STRIDE;
+        const unsigned long long t0 = wall_clock64();
+        int ok = 1;
+        unsigned long long a;
+        while (((a = ld_relaxed(r)) >> 32) != epoch) {
+            if (ld_relaxed(gs.err)) { ok = 0; break; }
             if (wall_clock64() - t0 > BARRIER_TIMEOUT_TICKS) {
-                __hip_atomic_store(gs.err, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(gs.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        __threadfence();  // acquire: later loads of the workgroup see the grid's stores
         s_ok = ok;
+        s_max = (int)(unsigned)(a & 0xffffffffull);
     }
+    if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     __syncthreads();
+    vmax = s_max;
     return s_ok != 0;
 }
 

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "../../include/glx.h"
+#include "glx_limits.hpp"
 
 namespace glx {
 
@@ -14,11 +15,9 @@ struct Env {
     hipStream_t stream;
     double *partials;        // RED_MAX_BLOCKS * 32 doubles
     double *result;          // 64 doubles
-    unsigned long long *grp; // barrier group counters (NGRP, 512 B apart)
-    unsigned long long *top; // barrier top counter
-    unsigned long long *rel; // barrier release flags (NGRP, 512 B apart)
-    int *err;                // barrier error flag
-    int *slots;              // residual ring
+    unsigned long long *arrive; // grid barrier arrival words (MAX_GRID_BLOCKS)
+    unsigned long long *rel;    // grid barrier release words (NREL, REL_STRIDE apart)
+    int *err;                   // grid barrier error flag
     int *state;              // persistent-kernel state
     int *sweeps;             // per-step sweep counts (capacity sweeps_cap steps)
     int sweeps_cap;

@@ -10,7 +10,10 @@
 
 namespace glx {
 
-inline bool grid_ok(const glx_grid &g) { return g.Nx >= 2 && g.Ny >= 2 && g.dx > 0.0 && g.dy > 0.0; }
+// Nodes are addressed with 32-bit offsets: Nx * Ny is capped at 2^30.
+inline bool grid_ok(const glx_grid &g) {
+    return g.Nx >= 2 && g.Ny >= 2 && (long)g.Nx * g.Ny <= (1L << 30) && g.dx > 0.0 && g.dy > 0.0;
+}
 
 inline int grid_blocks(long work, int block = RED_BLOCK, int cap = 65536) {
     long b = (work + block - 1) / block;
@@ -31,7 +34,7 @@ inline int fetch_result(const Env &e, double *out, int n) {
 
 // Waves used by a single-sweep launch: about 16 rows per wave.
 inline long sweep_waves(const glx_grid &g) {
-    const long T = (long)nstrips_of(g.Nx) * g.Ny;
+    const long T = (long)nstrips_of<1>(g.Nx) * g.Ny;
     long W = (T + 15) / 16;
     return W < 1 ? 1 : W;
 }
@@ -132,7 +135,8 @@ template <typename R> int L<R>::td_max_grid(const Env &e, int32_t wg_size, int32
     hipError_t st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_td_psi_solve<R, G>, 256, 0);
     if (st == hipSuccess) st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_td_a_solve<R, G>, 256, 0);
     if (st != hipSuccess) return check_hip(st, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-    *max_wg = (a < b ? a : b) * e.num_cus;
+    int m = (a < b ? a : b) * e.num_cus;
+    *max_wg = m < MAX_GRID_BLOCKS ? m : MAX_GRID_BLOCKS;
     return GLX_OK;
 }
 
@@ -177,11 +181,9 @@ int L<R>::td_run(const Env &e, glx_grid g, const glx_td_params *p, const void *e
     T.rhs_psi = p->langevin_psi > 1.0e-32 ? (cplx<R> *)rhs_psi : nullptr;
     T.rhs_ab = (p->solveA && p->langevin_A > 1.0e-32) ? (R *)rhs_ab : nullptr;
     T.max_sweeps = p->max_sweeps;
-    T.sc.gs.grp = e.grp;
-    T.sc.gs.top = e.top;
+    T.sc.gs.arrive = e.arrive;
     T.sc.gs.rel = e.rel;
     T.sc.gs.err = e.err;
-    T.sc.slots = e.slots;
     T.sc.state = e.state;
     T.sc.sweeps = e.sweeps;
     const long Nab = (long)(g.Nx - 1) * g.Ny + (long)g.Nx * (g.Ny - 1);

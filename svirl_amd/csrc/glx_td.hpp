@@ -1,14 +1,13 @@
 // TDGL Jacobi sweeps for psi and A on gfx950.
 //
-// Layout of the work: the grid is cut into vertical strips of 63 columns;
-// one wave marches down a range of rows of one strip. Lane 0 of a wave holds
-// the column left of the strip (halo), lanes 1..63 own one column each.
-// Each lane keeps a three-row window of its column in registers, so every
-// field value is read from HBM once per sweep; horizontal neighbours come
-// from adjacent lanes by cross-lane shuffles, and only lane 63 loads the
-// column right of the strip. Each lane evaluates sin/cos once per link it
-// carries (2 per node per sweep) and hands the values to its right
-// neighbour.
+// Layout of the work: the grid is cut into vertical strips of 62 columns;
+// one wave marches down a range of rows of one strip. Lanes 0 and 63 of a
+// wave hold the columns left and right of the strip (halos), lanes 1..62 own
+// one column each. Each lane keeps a three-row window of its column in
+// registers, so every field value is read from HBM once per sweep;
+// horizontal neighbours come from adjacent lanes by cross-lane shuffles.
+// Each lane evaluates sin/cos once per link it carries (2 per node per
+// sweep) and hands the values to its right neighbour.
 //
 // Arithmetic follows svirl/cuda/td.h term by term (same operand order, no
 // reassociation); FP64 instantiations are compiled without FMA contraction.
@@ -76,127 +75,163 @@ __device__ __forceinline__ bool cell_at(const uint8_t *mt, int Nxc, int Nyc, int
     return mt ? (mt[c + (long)Nxc * j] != 0) : true;
 }
 
-// Loads of one row for the psi sweep: psi at row j+1, a and b at row j,
-// the cell (c, j), the right-hand side and epsilon at row j, and (lane 63)
-// psi right of the strip at row j. The sweep issues the loads of row j+1
-// before it computes row j, so one row of loads per wave is in flight
-// during the arithmetic.
+// Rows are loaded in batches of PIPE_ROWS: the loads of batch k+1 are issued
+// before batch k is computed, and the two batches live in separately named
+// registers (the loop body is unrolled over the pair), so no loaded value is
+// copied before it is used and 1 to 2 batches of loads per wave stay in
+// flight during the arithmetic. Loads beyond the wave's last row are clamped
+// to that row (cache hits, never used).
+template <typename R> struct Pipe { static constexpr int ROWS = 2; };
+template <> struct Pipe<double> { static constexpr int ROWS = 1; };
+
+// Per-lane column context of a strip: c may lie outside the grid (halo lanes
+// at the edges); cc and ca are c clamped into the node and link ranges, so
+// that every load of a row is issued unconditionally from a valid address
+// (straight-line loads let the compiler count outstanding loads exactly and
+// keep the next batch in flight); values that do not exist are masked at
+// their use instead.
+struct ColCtx {
+    int c;
+    uint32_t cc, ca;
+    bool col, own, hasa;
+};
+
+// Strips of WAVE - 2*HALO columns; lanes [HALO, WAVE - HALO) own a column.
+template <int HALO = 1>
+__device__ __forceinline__ ColCtx col_ctx(int Nx, int s) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    ColCtx X;
+    X.c = s * (WAVE - 2 * HALO) - HALO + lane;
+    X.col = (X.c >= 0) && (X.c < Nx);
+    X.own = X.col && lane >= HALO && lane < WAVE - HALO;
+    X.hasa = X.col && X.c < Nx - 1;
+    X.cc = (uint32_t)min(max(X.c, 0), Nx - 1);
+    X.ca = min(X.cc, (uint32_t)(Nx - 2));
+    return X;
+}
+
+// Raw loads of one row for the psi sweep: psi at row j+1, a and b at row j,
+// the mask byte of cell (c, j), the right-hand side and epsilon at row j.
 template <typename R> struct PsiRow {
-    cplx<R> pp, rhs, pr;
+    cplx<R> pp, rhs;
     R av, bv, e;
-    bool cell;
+    int cell;
 };
 
 template <typename R>
-__device__ __forceinline__ PsiRow<R> psi_load_row(const PsiSweepArgs<R> &A, int c, int lane, bool col, bool own, int j) {
-    const int Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1;
-    const long rowoff = (long)Nx * j;
+__device__ __forceinline__ PsiRow<R> psi_load_row(const PsiSweepArgs<R> &A, const ColCtx &X, const uint8_t *mtp,
+                                                  const R *emp, int j) {
+    const uint32_t Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1;
+    const uint32_t jr = min((uint32_t)j, Ny - 1), jp = min((uint32_t)j + 1, Ny - 1), jq = min((uint32_t)j, Ny - 2);
     PsiRow<R> L;
-    L.pp = {R(0), R(0)};
-    L.rhs = {R(0), R(0)};
-    L.pr = {R(0), R(0)};
-    L.av = R(0);
-    L.bv = R(0);
-    L.e = A.eps;
-    if (col) {
-        if (j + 1 < Ny) L.pp = ld(A.psi, c + rowoff + Nx);
-        if (c < Nxa) L.av = A.ab[c + (long)Nxa * j];
-        if (j < Ny - 1) L.bv = A.ab[(long)Nxa * Ny + c + rowoff];
-    }
-    L.cell = cell_at(A.mt, Nx - 1, Ny - 1, c, j);
-    if (own) {
-        L.rhs = ld(A.rhs, c + rowoff);
-        if (A.eps_map) L.e = A.eps_map[c + rowoff];
-    }
-    if (lane == WAVE - 1 && c + 1 < Nx) L.pr = ld(A.psi, c + 1 + rowoff);
+    L.pp = ld(A.psi, X.cc + Nx * jp);
+    L.av = A.ab[X.ca + Nxa * jr];
+    L.bv = A.ab[Nxa * Ny + X.cc + Nx * jq];
+    L.cell = mtp[X.ca + Nxa * jq];
+    L.rhs = ld(A.rhs, X.cc + Nx * jr);
+    L.e = emp[X.cc + Nx * jr];
     return L;
 }
 
+// Backward-Euler Jacobi update of one psi node (svirl/cuda/td.h:85-214):
+// neighbours pl, pr, pm, pp (left, right, down, up), the link factors
+// (c, s) = (cos, sin) of dx a_left, dx a, dy b_down, dy b, and the four
+// adjacent cells (mm = (c-1, j-1), mp = (c-1, j), pm = (c, j-1), pp = (c, j)).
+template <typename R>
+__device__ __forceinline__ cplx<R> psi_node(const PsiSweepArgs<R> &A, const cplx<R> &rhs, R e, R pl_re, R pl_im,
+                                            R pr_re, R pr_im, const cplx<R> &pm, const cplx<R> &pp, R cal, R sal,
+                                            R ca, R sa, R cbm, R sbm, R cb, R sb, bool mm, bool mp, bool pmc,
+                                            bool ppc) {
+    const R rm0 = (mm || mp) ? R(1) : R(0);
+    const R rp0 = (pmc || ppc) ? R(1) : R(0);
+    const R r0m = (mm || pmc) ? R(1) : R(0);
+    const R r0p = (mp || ppc) ? R(1) : R(0);
+    // U(-dx a_left) psi_left, U(dx a) psi_right, U(-dy b_down) psi_down, U(dy b) psi_up
+    R m0r = R(0), m0i = R(0), p0r = R(0), p0i = R(0);
+    R q0mr = R(0), q0mi = R(0), q0pr = R(0), q0pi = R(0);
+    if (mm || mp) { m0r = cal * pl_re - sal * pl_im; m0i = cal * pl_im + sal * pl_re; }
+    if (pmc || ppc) { p0r = ca * pr_re + sa * pr_im; p0i = ca * pr_im - sa * pr_re; }
+    if (mm || pmc) { q0mr = cbm * pm.re - sbm * pm.im; q0mi = cbm * pm.im + sbm * pm.re; }
+    if (mp || ppc) { q0pr = cb * pp.re + sb * pp.im; q0pi = cb * pp.im - sb * pp.re; }
+    R q = rhs.re * rhs.re + rhs.im * rhs.im - e;
+    q = q + (A.idx2 * (rm0 + rp0) + A.idy2 * (r0m + r0p));
+    const R diag = R(1) + A.dt * q;
+    const R nre = rhs.re + A.dt * (A.idx2 * (m0r + p0r) + A.idy2 * (q0mr + q0pr));
+    const R nim = rhs.im + A.dt * (A.idx2 * (m0i + p0i) + A.idy2 * (q0mi + q0pi));
+    return {nre / diag, nim / diag};
+}
+
+// The reference's Langevin term added to the right-hand side at node n.
+template <typename R>
+__device__ __forceinline__ void psi_noise(const PsiSweepArgs<R> &A, uint32_t n, cplx<R> &rhs) {
+    const R n1 = R(double(noise_1<R>(n, A.rand_t)) - 0.5);
+    const R n2 = R(double(noise_2<R>(n, A.rand_t)) - 0.5);
+    rhs.re = rhs.re + A.langevin_c * n1;
+    rhs.im = rhs.im + A.langevin_c * n2;
+}
+
 // One psi sweep over rows [ja, jb) of strip s. Returns the lane's largest
-// scaled residual.
+// |delta| (max of real and imaginary parts); the stop rule's scaled integer
+// residual is monotone in it, so it is applied once per wave by the caller.
 template <typename R, bool GAUGE>
-__device__ int psi_rows(const PsiSweepArgs<R> &A, int s, int ja, int jb) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1, Nxc = Nx - 1, Nyc = Ny - 1;
-    const long Na = (long)Nxa * Ny;
-    const int c = s * STRIP - 1 + lane;
-    const bool col = (c >= 0) && (c < Nx);
-    const bool own = col && lane >= 1;
-    const R *b = A.ab + Na;
+__device__ __forceinline__ double psi_rows(const PsiSweepArgs<R> &A, int s, int ja, int jb) {
+    constexpr int B = Pipe<R>::ROWS;
+    const ColCtx X = col_ctx<1>(A.Nx, s);
+    const int c = X.c;
+    const bool own = X.own;
+    const int Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1;
+    const R *b = A.ab + (long)Nxa * Ny;
     const bool noisy = A.rhs_out != nullptr;
-    int rmax = 0;
+    const bool has_mt = A.mt != nullptr;
+    const uint8_t *mtp = has_mt ? A.mt : reinterpret_cast<const uint8_t *>(A.ab);     // any readable bytes
+    const R *emp = A.eps_map ? A.eps_map : reinterpret_cast<const R *>(A.rhs);         // when absent
+    const double gb = gauge_b(A.gH, A.gdx, A.gx0, c);
+    double rr = 0.0;
 
     // window: psi at rows j-1, j of this column; U of b(c, j-1); cell (c, j-1)
     cplx<R> pm = {R(0), R(0)}, pc = {R(0), R(0)};
     R cbm = R(1), sbm = R(0);
-    if (col) {
+    int cellm = 0;
+    if (X.col) {
         pc = ld(A.psi, c + (long)Nx * ja);
         if (ja > 0) {
             pm = ld(A.psi, c + (long)Nx * (ja - 1));
-            sincos_r(link_phase<R, GAUGE>(A.dy, b[c + (long)Nx * (ja - 1)], gauge_b(A.gH, A.gdx, A.gx0, c)), &sbm, &cbm);
+            sincos_r(link_phase<R, GAUGE>(A.dy, b[c + (long)Nx * (ja - 1)], gb), &sbm, &cbm);
+            if (c < Nxa) cellm = has_mt ? int(A.mt[c + (long)Nxa * (ja - 1)]) : 1;
         }
     }
-    bool cellm = cell_at(A.mt, Nxc, Nyc, c, ja - 1);
-    PsiRow<R> nx = psi_load_row(A, c, lane, col, own, ja);
 
-    for (int j = ja; j < jb; ++j) {
+    auto row = [&](const PsiRow<R> &cu, int j) {
         const long rowoff = (long)Nx * j;
-        const PsiRow<R> cu = nx;
-        if (j + 1 < jb) nx = psi_load_row(A, c, lane, col, own, j + 1);
-        const cplx<R> pp = cu.pp;
-        const bool cellc = cu.cell;
+        const cplx<R> pp = (X.col && j + 1 < Ny) ? cu.pp : cplx<R>{R(0), R(0)};
+        const R av = X.hasa ? cu.av : R(0);
+        const R bv = (X.col && j < Ny - 1) ? cu.bv : R(0);
+        const int cellc = (X.hasa && j < Ny - 1) ? (has_mt ? cu.cell : 1) : 0;
         cplx<R> rhs = cu.rhs;
-        const R e = cu.e;
+        const R e = A.eps_map ? cu.e : A.eps;
         R sa, ca, sb, cb;
-        sincos_r(link_phase<R, GAUGE>(A.dx, cu.av, gauge_a(A.gH, A.gdy, A.gy0, j)), &sa, &ca);
-        sincos_r(link_phase<R, GAUGE>(A.dy, cu.bv, gauge_b(A.gH, A.gdx, A.gx0, c)), &sb, &cb);
+        sincos_r(link_phase<R, GAUGE>(A.dx, av, gauge_a(A.gH, A.gdy, A.gy0, j)), &sa, &ca);
+        sincos_r(link_phase<R, GAUGE>(A.dy, bv, gb), &sb, &cb);
 
         // neighbours from adjacent lanes
         const R pl_re = shfl_up1(pc.re), pl_im = shfl_up1(pc.im);
-        R pr_re = shfl_dn1(pc.re), pr_im = shfl_dn1(pc.im);
+        const R pr_re = shfl_dn1(pc.re), pr_im = shfl_dn1(pc.im);
         const R cal = shfl_up1(ca), sal = shfl_up1(sa);
-        const bool cl_m = shfl_up1((int)cellm) != 0;  // cell (c-1, j-1)
-        const bool cl_c = shfl_up1((int)cellc) != 0;  // cell (c-1, j)
-        if (lane == WAVE - 1) {
-            pr_re = cu.pr.re;
-            pr_im = cu.pr.im;
-        }
+        const bool cl_m = shfl_up1(cellm) != 0;  // cell (c-1, j-1)
+        const bool cl_c = shfl_up1(cellc) != 0;  // cell (c-1, j)
 
         if (own) {
-            const bool mm = cl_m, mp = cl_c, pmc = cellm, ppc = cellc;
+            const bool mm = cl_m, mp = cl_c, pmc = cellm != 0, ppc = cellc != 0;
             const bool inside = mm || mp || pmc || ppc;
             cplx<R> nxt = {R(0), R(0)};
             if (inside) {
-                if (noisy && A.jstep == 0 && A.langevin_c > R(1.0e-32)) {
-                    const uint32_t n = (uint32_t)(c + rowoff);
-                    const R n1 = R(double(noise_1<R>(n, A.rand_t)) - 0.5);
-                    const R n2 = R(double(noise_2<R>(n, A.rand_t)) - 0.5);
-                    rhs.re = rhs.re + A.langevin_c * n1;
-                    rhs.im = rhs.im + A.langevin_c * n2;
-                }
-                const R rm0 = (mm || mp) ? R(1) : R(0);
-                const R rp0 = (pmc || ppc) ? R(1) : R(0);
-                const R r0m = (mm || pmc) ? R(1) : R(0);
-                const R r0p = (mp || ppc) ? R(1) : R(0);
-                // U(-dx a_left) psi_left, U(dx a) psi_right, U(-dy b_down) psi_down, U(dy b) psi_up
-                R m0r = R(0), m0i = R(0), p0r = R(0), p0i = R(0);
-                R q0mr = R(0), q0mi = R(0), q0pr = R(0), q0pi = R(0);
-                if (mm || mp) { m0r = cal * pl_re - sal * pl_im; m0i = cal * pl_im + sal * pl_re; }
-                if (pmc || ppc) { p0r = ca * pr_re + sa * pr_im; p0i = ca * pr_im - sa * pr_re; }
-                if (mm || pmc) { q0mr = cbm * pm.re - sbm * pm.im; q0mi = cbm * pm.im + sbm * pm.re; }
-                if (mp || ppc) { q0pr = cb * pp.re + sb * pp.im; q0pi = cb * pp.im - sb * pp.re; }
-                R q = rhs.re * rhs.re + rhs.im * rhs.im - e;
-                q = q + (A.idx2 * (rm0 + rp0) + A.idy2 * (r0m + r0p));
-                const R diag = R(1) + A.dt * q;
-                const R nre = rhs.re + A.dt * (A.idx2 * (m0r + p0r) + A.idy2 * (q0mr + q0pr));
-                const R nim = rhs.im + A.dt * (A.idx2 * (m0i + p0i) + A.idy2 * (q0mi + q0pi));
-                nxt.re = nre / diag;
-                nxt.im = nim / diag;
+                if (noisy && A.jstep == 0 && A.langevin_c > R(1.0e-32)) psi_noise(A, (uint32_t)(c + rowoff), rhs);
+                nxt = psi_node(A, rhs, e, pl_re, pl_im, pr_re, pr_im, pm, pp, cal, sal, ca, sa, cbm, sbm, cb, sb, mm,
+                               mp, pmc, ppc);
             }
             if (noisy && A.jstep == 0) st(A.rhs_out, c + rowoff, rhs);
             st(A.psi_next, c + rowoff, nxt);
-            const double r = fmax(fabs(double(nxt.re) - double(pc.re)), fabs(double(nxt.im) - double(pc.im)));
-            rmax = max(rmax, scaled_residual(r, A.stop_eps));
+            rr = fmax(rr, fmax(fabs(double(nxt.re) - double(pc.re)), fabs(double(nxt.im) - double(pc.im))));
         }
         // roll the window
         pm = pc;
@@ -204,100 +239,227 @@ __device__ int psi_rows(const PsiSweepArgs<R> &A, int s, int ja, int jb) {
         cbm = cb;
         sbm = sb;
         cellm = cellc;
+    };
+
+    PsiRow<R> P0[B], P1[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) P0[k] = psi_load_row(A, X, mtp, emp, min(ja + k, jb - 1));
+    for (int j = ja; j < jb; j += 2 * B) {
+#pragma unroll
+        for (int k = 0; k < B; ++k) P1[k] = psi_load_row(A, X, mtp, emp, min(j + B + k, jb - 1));
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            if (j + k < jb) row(P0[k], j + k);
+#pragma unroll
+        for (int k = 0; k < B; ++k) P0[k] = psi_load_row(A, X, mtp, emp, min(j + 2 * B + k, jb - 1));
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            if (j + B + k < jb) row(P1[k], j + B + k);
     }
-    return rmax;
+    return rr;
 }
 
-// Loads of one row for the A sweep: a(c, j+1), b(c, j), psi(c, j+1), the
-// cell (c, j), the owned links' source and right-hand side at row j, and
-// (lane 63) b and psi right of the strip at row j.
+// Two consecutive psi sweeps j and j+1 in one pass over rows [ja, jb) of
+// strip s (strips of WAVE - 4 columns, two halo lanes a side): the first
+// sweep runs one row ahead of the second and on lanes 1..62, so the second
+// finds its three rows and both lateral neighbours of psi^{j+1} in
+// registers. Link factors and cells are computed once for both sweeps;
+// psi^{j+1} never leaves registers. Writes psi^{j+2} (and, at jstep 0, the
+// Langevin right-hand side); returns the lane's largest |delta| of each sweep
+// in r1 and r2. The arithmetic of each node update is that of psi_rows.
+template <typename R, bool GAUGE>
+__device__ __forceinline__ void psi_rows2(const PsiSweepArgs<R> &A, int s, int ja, int jb, double &r1, double &r2) {
+    constexpr int B = Pipe<R>::ROWS;
+    const ColCtx X = col_ctx<2>(A.Nx, s);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int c = X.c;
+    const bool own = X.own;                                  // lanes 2..61: outputs
+    const bool mid = X.col && lane >= 1 && lane < WAVE - 1;  // lanes 1..62: first sweep
+    const int Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1;
+    const R *b = A.ab + (long)Nxa * Ny;
+    const bool noisy = A.rhs_out != nullptr;
+    const bool noise_on = noisy && A.jstep == 0 && A.langevin_c > R(1.0e-32);
+    const bool has_mt = A.mt != nullptr;
+    const uint8_t *mtp = has_mt ? A.mt : reinterpret_cast<const uint8_t *>(A.ab);
+    const R *emp = A.eps_map ? A.eps_map : reinterpret_cast<const R *>(A.rhs);
+    const double gb = gauge_b(A.gH, A.gdx, A.gx0, c);
+    const cplx<R> zero = {R(0), R(0)};
+    const int t0 = ja > 0 ? ja - 1 : 0;
+
+    // first sweep window: psi^j at rows t-1, t; U of b(t-1); cell (c, t-1)
+    cplx<R> pm = zero, pc = zero;
+    R cb1 = R(1), sb1 = R(0);
+    int cell1 = 0;
+    if (X.col) {
+        pc = ld(A.psi, c + (long)Nx * t0);
+        if (t0 > 0) {
+            pm = ld(A.psi, c + (long)Nx * (t0 - 1));
+            sincos_r(link_phase<R, GAUGE>(A.dy, b[c + (long)Nx * (t0 - 1)], gb), &sb1, &cb1);
+            if (c < Nxa) cell1 = has_mt ? int(A.mt[c + (long)Nxa * (t0 - 1)]) : 1;
+        }
+    }
+    // second sweep window (row u = t - 1): psi^{j+1} at u-1, u; U of a(u),
+    // b(u-1); cells (c, u-1), (c-1, u-1), (c-1, u); rhs and eps at u
+    cplx<R> qm = zero, qc = zero, rhs_u = zero;
+    R ca2 = R(1), sa2 = R(0), cal2 = R(1), sal2 = R(0), cb2 = R(1), sb2 = R(0), e_u = R(0);
+    int cell2 = 0, cl2m = 0, cl2c = 0;
+
+    auto row = [&](const PsiRow<R> &cu, int t) {
+        const long rowoff = (long)Nx * t;
+        const bool real = t < Ny;  // t == Ny: past the last row (second sweep only)
+        const cplx<R> pp = (X.col && t + 1 < Ny) ? cu.pp : zero;
+        const R av = X.hasa ? cu.av : R(0);
+        const R bv = (X.col && t < Ny - 1) ? cu.bv : R(0);
+        const int cellc = (X.hasa && t < Ny - 1) ? (has_mt ? cu.cell : 1) : 0;
+        cplx<R> rhs = cu.rhs;
+        const R e = A.eps_map ? cu.e : A.eps;
+        R sa, ca, sb, cb;
+        sincos_r(link_phase<R, GAUGE>(A.dx, av, gauge_a(A.gH, A.gdy, A.gy0, t)), &sa, &ca);
+        sincos_r(link_phase<R, GAUGE>(A.dy, bv, gb), &sb, &cb);
+        const R cal = shfl_up1(ca), sal = shfl_up1(sa);
+        const int cl_m = shfl_up1(cell1);  // cell (c-1, t-1)
+        const int cl_c = shfl_up1(cellc);  // cell (c-1, t)
+
+        // first sweep at row t -> psi^{j+1}(t)
+        cplx<R> q = zero;
+        {
+            const R pl_re = shfl_up1(pc.re), pl_im = shfl_up1(pc.im);
+            const R pr_re = shfl_dn1(pc.re), pr_im = shfl_dn1(pc.im);
+            const bool mm = cl_m != 0, mp = cl_c != 0, pmc = cell1 != 0, ppc = cellc != 0;
+            if (mid && real && (mm || mp || pmc || ppc)) {
+                if (noise_on) psi_noise(A, (uint32_t)(c + rowoff), rhs);
+                q = psi_node(A, rhs, e, pl_re, pl_im, pr_re, pr_im, pm, pp, cal, sal, ca, sa, cb1, sb1, cb, sb, mm, mp,
+                             pmc, ppc);
+            }
+            if (own && real && t >= ja && t < jb) {
+                if (noisy && A.jstep == 0) st(A.rhs_out, c + rowoff, rhs);
+                r1 = fmax(r1, fmax(fabs(double(q.re) - double(pc.re)), fabs(double(q.im) - double(pc.im))));
+            }
+        }
+        // second sweep at row u = t - 1 -> psi^{j+2}(u)
+        const int u = t - 1;
+        if (u >= ja) {
+            const R pl_re = shfl_up1(qc.re), pl_im = shfl_up1(qc.im);
+            const R pr_re = shfl_dn1(qc.re), pr_im = shfl_dn1(qc.im);
+            const bool mm = cl2m != 0, mp = cl2c != 0, pmc = cell2 != 0, ppc = cell1 != 0;
+            if (own) {
+                cplx<R> nxt = zero;
+                if (mm || mp || pmc || ppc)
+                    nxt = psi_node(A, rhs_u, e_u, pl_re, pl_im, pr_re, pr_im, qm, q, cal2, sal2, ca2, sa2, cb2, sb2,
+                                   cb1, sb1, mm, mp, pmc, ppc);
+                st(A.psi_next, c + (long)Nx * u, nxt);
+                r2 = fmax(r2, fmax(fabs(double(nxt.re) - double(qc.re)), fabs(double(nxt.im) - double(qc.im))));
+            }
+        }
+        // roll both windows
+        qm = qc;
+        qc = q;
+        cb2 = cb1;
+        sb2 = sb1;
+        ca2 = ca;
+        sa2 = sa;
+        cal2 = cal;
+        sal2 = sal;
+        cell2 = cell1;
+        cl2m = cl_m;
+        cl2c = cl_c;
+        rhs_u = rhs;
+        e_u = e;
+        pm = pc;
+        pc = pp;
+        cb1 = cb;
+        sb1 = sb;
+        cell1 = cellc;
+    };
+
+    const int t1 = jb + 1;  // rows t in [t0, t1)
+    PsiRow<R> P0[B], P1[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) P0[k] = psi_load_row(A, X, mtp, emp, min(t0 + k, t1 - 1));
+    for (int t = t0; t < t1; t += 2 * B) {
+#pragma unroll
+        for (int k = 0; k < B; ++k) P1[k] = psi_load_row(A, X, mtp, emp, min(t + B + k, t1 - 1));
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            if (t + k < t1) row(P0[k], t + k);
+#pragma unroll
+        for (int k = 0; k < B; ++k) P0[k] = psi_load_row(A, X, mtp, emp, min(t + 2 * B + k, t1 - 1));
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            if (t + B + k < t1) row(P1[k], t + B + k);
+    }
+}
+
+// Raw loads of one row for the A sweep: a(c, j+1), b(c, j), psi(c, j+1), the
+// mask byte of cell (c, j), and the owned links' source and right-hand side
+// at row j.
 template <typename R> struct ARow {
-    R ap, bc, sa, sb, ra, rb, brc;
-    cplx<R> pp, pr;
-    bool cell;
+    R ap, bc, sa, sb, ra, rb;
+    cplx<R> pp;
+    int cell;
 };
 
 template <typename R>
-__device__ __forceinline__ ARow<R> a_load_row(const ASweepArgs<R> &A, int c, int lane, bool col, bool own, int j) {
-    const int Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1;
-    const long Na = (long)Nxa * Ny;
-    const long rowoff = (long)Nx * j;
-    const R *a = A.ab;
-    const R *b = A.ab + Na;
+__device__ __forceinline__ ARow<R> a_load_row(const ASweepArgs<R> &A, const ColCtx &X, const uint8_t *mtp, int j) {
+    const uint32_t Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1, Na = Nxa * Ny;
+    const uint32_t jp = min((uint32_t)j + 1, Ny - 1), jq = min((uint32_t)j, Ny - 2);
     ARow<R> L;
-    L.ap = R(0); L.bc = R(0); L.sa = R(0); L.sb = R(0); L.ra = R(0); L.rb = R(0); L.brc = R(0);
-    L.pp = {R(0), R(0)};
-    L.pr = {R(0), R(0)};
-    if (col) {
-        if (c < Nxa && j + 1 < Ny) L.ap = a[c + (long)Nxa * (j + 1)];
-        if (j < Ny - 1) {
-            L.bc = b[c + rowoff];
-            L.pp = ld(A.psi, c + rowoff + Nx);
-        }
-    }
-    L.cell = cell_at(A.mt, Nx - 1, Ny - 1, c, j);
-    const long na = c + (long)Nxa * j;
-    const long nb = Na + c + rowoff;
-    if (own && c < Nxa) { L.sa = A.src[na]; L.ra = A.rhs[na]; }
-    if (own && j < Ny - 1) { L.sb = A.src[nb]; L.rb = A.rhs[nb]; }
-    if (lane == WAVE - 1 && c + 1 < Nx) {
-        if (j < Ny - 1) L.brc = b[c + 1 + rowoff];
-        L.pr = ld(A.psi, c + 1 + rowoff);
-    }
+    L.ap = A.ab[X.ca + Nxa * jp];
+    L.bc = A.ab[Na + X.cc + Nx * jq];
+    L.pp = ld(A.psi, X.cc + Nx * jp);
+    L.cell = mtp[X.ca + Nxa * jq];
+    const uint32_t na = X.ca + Nxa * (uint32_t)j, nb = Na + X.cc + Nx * jq;
+    L.sa = A.src[na];
+    L.ra = A.rhs[na];
+    L.sb = A.src[nb];
+    L.rb = A.rhs[nb];
     return L;
 }
 
 // One A sweep over rows [ja, jb) of strip s (both link families of every
-// owned node).
+// owned node). Returns the lane's largest |delta|.
 template <typename R, bool GAUGE>
-__device__ int a_rows(const ASweepArgs<R> &A, int s, int ja, int jb) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1, Nxc = Nx - 1, Nyc = Ny - 1;
+__device__ __forceinline__ double a_rows(const ASweepArgs<R> &A, int s, int ja, int jb) {
+    constexpr int B = Pipe<R>::ROWS;
+    const ColCtx X = col_ctx<1>(A.Nx, s);
+    const int c = X.c;
+    const bool own = X.own, hasa = X.hasa;
+    const int Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1;
     const long Na = (long)Nxa * Ny;
-    const int c = s * STRIP - 1 + lane;
-    const bool col = (c >= 0) && (c < Nx);
-    const bool own = col && lane >= 1;
-    const bool hasa = col && c < Nxa;
     const R *a = A.ab;
     const R *b = A.ab + Na;
     const bool noisy = A.rhs_out != nullptr;
-    int rmax = 0;
+    const bool has_mt = A.mt != nullptr;
+    const uint8_t *mtp = has_mt ? A.mt : reinterpret_cast<const uint8_t *>(A.ab);  // any readable bytes when absent
+    const double gb = gauge_b(A.gH, A.gdx, A.gx0, c);
+    double rr = 0.0;
 
-    // window of this column: a(c, j-1), a(c, j), b(c, j-1), psi(c, j)
+    // window of this column: a(c, j-1), a(c, j), b(c, j-1), psi(c, j), cell (c, j-1)
     R am = R(0), ac = R(0), bm = R(0);
-    R brm = R(0);  // b(c+1, j-1) (lane 63 only; other lanes use shuffles)
     cplx<R> pc = {R(0), R(0)};
-    if (col) {
+    int cellm = 0;
+    if (X.col) {
         pc = ld(A.psi, c + (long)Nx * ja);
         if (hasa) ac = a[c + (long)Nxa * ja];
         if (ja > 0) {
             if (hasa) am = a[c + (long)Nxa * (ja - 1)];
             bm = b[c + (long)Nx * (ja - 1)];
+            if (hasa) cellm = has_mt ? int(A.mt[c + (long)Nxa * (ja - 1)]) : 1;
         }
     }
-    if (lane == WAVE - 1 && c + 1 < Nx && ja > 0) brm = b[c + 1 + (long)Nx * (ja - 1)];
-    bool cellm = cell_at(A.mt, Nxc, Nyc, c, ja - 1);
-    ARow<R> nx = a_load_row(A, c, lane, col, own, ja);
 
-    for (int j = ja; j < jb; ++j) {
-        const ARow<R> cu = nx;
-        if (j + 1 < jb) nx = a_load_row(A, c, lane, col, own, j + 1);
-        const R ap = cu.ap, bc = cu.bc;
-        const cplx<R> pp = cu.pp;
-        const bool cellc = cu.cell;
+    auto row = [&](const ARow<R> &cu, int j) {
+        const R ap = (hasa && j + 1 < Ny) ? cu.ap : R(0);
+        const R bc = (X.col && j < Ny - 1) ? cu.bc : R(0);
+        const cplx<R> pp = (X.col && j < Ny - 1) ? cu.pp : cplx<R>{R(0), R(0)};
+        const int cellv = (hasa && j < Ny - 1) ? (has_mt ? cu.cell : 1) : 0;
+        const bool cellc = cellv != 0;
         // right column values
-        R brc = shfl_dn1(bc), brm_s = shfl_dn1(bm);
-        R pr_re = shfl_dn1(pc.re), pr_im = shfl_dn1(pc.im);
-        if (lane == WAVE - 1) {
-            brm_s = brm;
-            brc = cu.brc;
-            pr_re = cu.pr.re;
-            pr_im = cu.pr.im;
-            brm = brc;
-        }
+        const R brc = shfl_dn1(bc), brm = shfl_dn1(bm);
+        const R pr_re = shfl_dn1(pc.re), pr_im = shfl_dn1(pc.im);
         // left column values
         const R alc = shfl_up1(ac), alp = shfl_up1(ap), blc = shfl_up1(bc);
-        const bool cl_c = shfl_up1((int)cellc) != 0;  // cell (c-1, j)
+        const bool cl_c = shfl_up1(cellv) != 0;  // cell (c-1, j)
 
         const long na = c + (long)Nxa * j;
         const long nb = Na + c + (long)Nx * j;
@@ -310,25 +472,24 @@ __device__ int a_rows(const ASweepArgs<R> &A, int s, int ja, int jb) {
         }
         R ssa, csa, ssb, csb;
         sincos_r(link_phase<R, GAUGE>(A.dx, cu.sa, gauge_a(A.gH, A.gdy, A.gy0, j)), &ssa, &csa);
-        sincos_r(link_phase<R, GAUGE>(A.dy, cu.sb, gauge_b(A.gH, A.gdx, A.gx0, c)), &ssb, &csb);
+        sincos_r(link_phase<R, GAUGE>(A.dy, cu.sb, gb), &ssb, &csb);
 
-        double r = 0.0;
         if (owna) {
             R rh = R(0), dd = R(1);
             if (j == 0) { rh = A.rh_a; dd = R(2); }
             else if (j + 1 == Ny) { rh = -A.rh_a; dd = R(2); }
             R jl = R(0);
-            if (cellm || cellc) {  // cells (c, j-1), (c, j)
+            if (cellm != 0 || cellc) {  // cells (c, j-1), (c, j)
                 const R x0 = pc.re, y0 = pc.im, x1 = pr_re, y1 = pr_im;
                 jl = A.idx * ((x0 * y1 - y0 * x1) * csa - (x0 * x1 + y0 * y1) * ssa);
             }
             R t1 = R(0), t2 = R(0);
-            if (j > 0) t1 = A.idy2 * am - A.idxy * bm + A.idxy * brm_s;
+            if (j > 0) t1 = A.idy2 * am - A.idxy * bm + A.idxy * brm;
             if (j + 1 < Ny) t2 = A.idy2 * ap + A.idxy * bc - A.idxy * brc;
             const R v = (ra + A.dt_rho * (jl + rh) + A.dt_rho_kappa2 * dd * (t1 + t2)) / A.den_a;
             if (noisy && A.jstep == 0) A.rhs_out[na] = ra;
             A.ab_next[na] = v;
-            r = fabs(double(v) - double(ac));
+            rr = fmax(rr, fabs(double(v) - double(ac)));
         }
         if (ownb) {
             R rh = R(0), dd = R(1);
@@ -345,23 +506,40 @@ __device__ int a_rows(const ASweepArgs<R> &A, int s, int ja, int jb) {
             const R v = (rb + A.dt_rho * (jl + rh) + A.dt_rho_kappa2 * dd * (t1 + t2)) / A.den_b;
             if (noisy && A.jstep == 0) A.rhs_out[nb] = rb;
             A.ab_next[nb] = v;
-            r = fmax(r, fabs(double(v) - double(bc)));
+            rr = fmax(rr, fabs(double(v) - double(bc)));
         }
-        if (own) rmax = max(rmax, scaled_residual(r, A.stop_eps));
         // roll
         am = ac;
         ac = ap;
         bm = bc;
         pc = pp;
-        cellm = cellc;
+        cellm = cellv;
+    };
+
+    ARow<R> P0[B], P1[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) P0[k] = a_load_row(A, X, mtp, min(ja + k, jb - 1));
+    for (int j = ja; j < jb; j += 2 * B) {
+#pragma unroll
+        for (int k = 0; k < B; ++k) P1[k] = a_load_row(A, X, mtp, min(j + B + k, jb - 1));
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            if (j + k < jb) row(P0[k], j + k);
+#pragma unroll
+        for (int k = 0; k < B; ++k) P0[k] = a_load_row(A, X, mtp, min(j + 2 * B + k, jb - 1));
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            if (j + B + k < jb) row(P1[k], j + B + k);
     }
-    return rmax;
+    return rr;
 }
 
 // ---- work partition --------------------------------------------------------
 // A grid of W waves shares T = nstrips*Ny wave-rows; wave w takes the
 // contiguous global rows [w*T/W, (w+1)*T/W), which may span two strips.
-__host__ __device__ inline int nstrips_of(int Nx) { return (Nx + STRIP - 1) / STRIP; }
+template <int HALO = 1> __host__ __device__ inline int nstrips_of(int Nx) {
+    return (Nx + WAVE - 2 * HALO - 1) / (WAVE - 2 * HALO);
+}
 
 template <typename F>
 __device__ __forceinline__ void for_rows(int Nx, int Ny, long w, long W, F &&f) {
@@ -377,45 +555,60 @@ __device__ __forceinline__ void for_rows(int Nx, int Ny, long w, long W, F &&f) 
     }
 }
 
+// Row-major partition for the persistent solves: the rows are cut into
+// S = max(1, W / nstrips) segments and the work items k = (segment, strip)
+// are numbered strip-fastest; wave w takes items w, w + U, ... with
+// U = min(W, S * nstrips). Waves of consecutive index then sweep the same rows
+// of adjacent strips at the same time (whole grid rows are read together);
+// spare waves (w >= U) idle.
+template <int HALO, typename F>
+__device__ __forceinline__ void for_rows_rm(int Nx, int Ny, long w, long W, F &&f) {
+    const long nst = nstrips_of<HALO>(Nx);
+    const long S = W >= nst ? W / nst : 1;
+    const long U = W < S * nst ? W : S * nst;
+    for (long k = w; k < S * nst; k += U) {
+        const long seg = k / nst;
+        const int ja = (int)((seg * Ny) / S), jb = (int)(((seg + 1) * Ny) / S);
+        if (ja < jb) f((int)(k % nst), ja, jb);
+    }
+}
+
 // ---- single-sweep kernels ---------------------------------------------------
 
 template <typename R>
 __global__ void __launch_bounds__(256) k_psi_sweep(PsiSweepArgs<R> A, int *r2max, long W) {
     __shared__ int lds[4];
-    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    int rmax = 0;
-    if (w < W) for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rmax = max(rmax, psi_rows<R, false>(A, s, ja, jb)); });
-    rmax = block_max_int(rmax, lds);
+    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
+    double rr = 0.0;
+    if (w < W) for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, psi_rows<R, false>(A, s, ja, jb)); });
+    const int rmax = block_max_int(scaled_residual(rr, A.stop_eps), lds);
     if (threadIdx.x == 0) atomic_max_agent(r2max, rmax);
 }
 
 template <typename R>
 __global__ void __launch_bounds__(256) k_a_sweep(ASweepArgs<R> A, int *r2max, long W) {
     __shared__ int lds[4];
-    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    int rmax = 0;
-    if (w < W) for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rmax = max(rmax, a_rows<R, false>(A, s, ja, jb)); });
-    rmax = block_max_int(rmax, lds);
+    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
+    double rr = 0.0;
+    if (w < W) for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, a_rows<R, false>(A, s, ja, jb)); });
+    const int rmax = block_max_int(scaled_residual(rr, A.stop_eps), lds);
     if (threadIdx.x == 0) atomic_max_agent(r2max, rmax);
 }
 
 // ---- persistent solve kernels ----------------------------------------------
 //
 // One launch runs one Jacobi solve (psi or A) to the reference's stop rule:
-// every sweep ends at a grid barrier after which every workgroup reads the
-// same residual slots and takes the same branch. Per-step launches are
-// enqueued back to back; the buffer holding the current state, the barrier
-// epoch and the global sweep counter carry over between launches in device
-// memory, so there is no reset launch and no host round trip per step.
+// every sweep ends at a grid barrier that also reduces the sweep's residual,
+// so every workgroup sees the same value and takes the same branch.
+// Per-step launches are enqueued back to back; the buffer holding the
+// current state and the barrier epoch carry over between launches in device
+// memory, so there is no host round trip per step.
 
-constexpr int SLOT_RING = 64;  // residual slots, one per sweep, NGRP ints each
-
-enum { ST_PSI = 0, ST_AB = 1, ST_EPOCH = 2, ST_GSWEEP = 4, ST_ERRSTEP = 6 };
+enum { ST_PSI = 0, ST_AB = 1, ST_EPOCH = 2, ST_ERRSTEP = 6, ST_PRED_PSI = 7 };
 
 struct TDScratch {
     GridSync gs;
-    int *slots;             // SLOT_RING * NGRP
-    int *state;             // ST_* fields (64-bit epoch and sweep counters at ST_EPOCH, ST_GSWEEP)
+    int *state;             // ST_* fields (64-bit epoch at ST_EPOCH)
     int *sweeps;            // 2 * nsteps
 };
 
@@ -431,71 +624,89 @@ template <typename R> struct TDSolveArgs {
     TDScratch sc;
 };
 
+// P[k] for a run-time k without indexing the kernel-argument array (which
+// would copy it to scratch memory).
+template <typename T> __device__ __forceinline__ T *pick3(T *const (&P)[3], int k) {
+    return k == 0 ? P[0] : (k == 1 ? P[1] : P[2]);
+}
+
 __device__ __forceinline__ unsigned long long *state64(int *state, int k) {
     return reinterpret_cast<unsigned long long *>(state + k);
 }
 
-// Fold the workgroup's residual into the slot of this sweep, meet at the
-// barrier, read the sweep's grid-wide residual. Block 0 clears the slot 32
-// sweeps ahead (every block has finished reading it).
-__device__ __forceinline__ bool sweep_sync(const TDScratch &sc, int rmax, unsigned long long &epoch,
-                                           unsigned long long gsweep, int *lds, int &r_out) {
-    rmax = block_max_int(rmax, lds);
-    if (threadIdx.x == 0) {
-        atomic_max_agent(sc.slots + (gsweep % SLOT_RING) * NGRP + blockIdx.x % NGRP, rmax);
-        if (blockIdx.x == 0) {
-            int *z = sc.slots + ((gsweep + SLOT_RING / 2) % SLOT_RING) * NGRP;
-            for (int k = 0; k < NGRP; ++k) __hip_atomic_store(z + k, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-    if (!grid_barrier(sc.gs, epoch)) return false;
-    if (threadIdx.x == 0) {
-        int *sl = sc.slots + (gsweep % SLOT_RING) * NGRP;
-        int m = 0;
-        for (int k = 0; k < NGRP; ++k) m = max(m, ld_acquire(sl + k));
-        lds[8] = m;
-    }
-    __syncthreads();
-    r_out = lds[8];
-    return true;
+// The workgroup's largest scaled residuals of up to two sweeps (each clamped
+// to 0xffff: only the comparison with 10000 steers the solve), grid barrier,
+// grid-wide residuals r1 (low 16 bits of r_out) and r2 (high 16 bits).
+__device__ __forceinline__ bool sweep_sync(const TDScratch &sc, int r1, int r2, unsigned long long &epoch, int *lds,
+                                           int &r_out) {
+    r1 = block_max_int(min(r1, 0xffff), lds);
+    r2 = block_max_int(min(r2, 0xffff), lds);
+    return grid_sync_max(sc.gs, epoch, (r2 << 16) | r1, r_out);
 }
 
+// The psi solve sweeps in fused pairs (psi_rows2: two sweeps per pass over
+// memory). A pair (j, j+1) reports both residuals; when sweep j already met
+// the stop rule, sweep j is recomputed alone from the pair's input, so the
+// result and the sweep count are those of sweep-by-sweep iteration. When the
+// previous step needed an odd number of sweeps, the solve opens with one
+// single sweep so that the pairs end where the solve is likely to end.
 template <typename R, bool GAUGE>
 __global__ void __launch_bounds__(256) k_td_psi_solve(TDSolveArgs<R> T) {
     __shared__ int lds[16];
     const long W = (long)gridDim.x * (blockDim.x / WAVE);
-    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
     int *st = T.sc.state;
     const int pc = st[ST_PSI], qc = st[ST_AB];
-    const unsigned long long e0 = *state64(st, ST_EPOCH), g0 = *state64(st, ST_GSWEEP);
-    unsigned long long epoch = e0, gsweep = g0;
+    const int pred = st[ST_PRED_PSI];
+    unsigned long long epoch = *state64(st, ST_EPOCH);
     const int o1 = (pc + 1) % 3, o2 = (pc + 2) % 3;
     PsiSweepArgs<R> A = T.psi;
-    A.ab = T.Q[qc];
-    int in = pc, out = o1, nsw = 0;
-    for (int j = 0; j < T.max_sweeps; ++j) {
+    A.ab = pick3(T.Q, qc);
+    A.rhs_out = T.rhs_psi;
+    int in = pc, out = o1, j = 0, nsw = 0;
+    bool done = false;
+    while (!done && j < T.max_sweeps) {
+        A.psi = pick3(T.P, in);
+        A.psi_next = pick3(T.P, out);
+        A.rhs = (T.rhs_psi && j > 0) ? T.rhs_psi : pick3(T.P, pc);
         A.jstep = (uint32_t)j;
-        A.psi = T.P[in];
-        A.psi_next = T.P[out];
-        A.rhs = (T.rhs_psi && j > 0) ? T.rhs_psi : T.P[pc];
-        A.rhs_out = T.rhs_psi;
-        int rmax = 0;
-        for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rmax = max(rmax, psi_rows<R, GAUGE>(A, s, ja, jb)); });
+        const bool pair = !(j == 0 && (pred & 1)) && j + 2 <= T.max_sweeps;
+        double r1 = 0.0, r2 = 0.0;
+        if (pair)
+            for_rows_rm<2>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { psi_rows2<R, GAUGE>(A, s, ja, jb, r1, r2); });
+        else
+            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { r1 = fmax(r1, psi_rows<R, GAUGE>(A, s, ja, jb)); });
         int r = 0;
-        if (!sweep_sync(T.sc, rmax, epoch, gsweep, lds, r)) {
+        if (!sweep_sync(T.sc, scaled_residual(r1, A.stop_eps), scaled_residual(r2, A.stop_eps), epoch, lds, r)) {
             if (blockIdx.x == 0 && threadIdx.x == 0) st[ST_ERRSTEP] = T.step + 1;
             return;
         }
-        ++gsweep;
-        nsw = j + 1;
+        const unsigned ru = (unsigned)r;
+        const bool c1 = (ru & 0xffffu) < 10000u, c2 = (ru >> 16) < 10000u;
+        if (pair && c1) {
+            // sweep j met the stop rule: redo it alone into the same output
+            A.psi_next = pick3(T.P, out);
+            double rr = 0.0;
+            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, psi_rows<R, GAUGE>(A, s, ja, jb)); });
+            int rdummy = 0;
+            if (!sweep_sync(T.sc, 0, 0, epoch, lds, rdummy)) {
+                if (blockIdx.x == 0 && threadIdx.x == 0) st[ST_ERRSTEP] = T.step + 1;
+                return;
+            }
+            nsw = j + 1;
+            done = true;
+        } else {
+            j += pair ? 2 : 1;
+            nsw = j;
+            done = pair ? c2 : c1;
+        }
         in = out;
         out = (out == o1) ? o2 : o1;
-        if (r < 10000) break;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st[ST_PSI] = in;
+        st[ST_PRED_PSI] = nsw;
         *state64(st, ST_EPOCH) = epoch;
-        *state64(st, ST_GSWEEP) = gsweep;
         T.sc.sweeps[2 * T.step] = nsw;
     }
 }
@@ -504,59 +715,55 @@ template <typename R, bool GAUGE>
 __global__ void __launch_bounds__(256) k_td_a_solve(TDSolveArgs<R> T) {
     __shared__ int lds[16];
     const long W = (long)gridDim.x * (blockDim.x / WAVE);
-    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
     int *st = T.sc.state;
     const int pc = st[ST_PSI], qc = st[ST_AB];
-    const unsigned long long e0 = *state64(st, ST_EPOCH), g0 = *state64(st, ST_GSWEEP);
-    unsigned long long epoch = e0, gsweep = g0;
+    unsigned long long epoch = *state64(st, ST_EPOCH);
     const int o1 = (qc + 1) % 3, o2 = (qc + 2) % 3;
     ASweepArgs<R> A = T.A;
-    A.psi = T.P[pc];
+    A.psi = pick3(T.P, pc);
     int in = qc, out = o1, nsw = 0;
     for (int j = 0; j < T.max_sweeps; ++j) {
         A.jstep = (uint32_t)j;
-        A.ab = T.Q[in];
-        A.ab_next = T.Q[out];
+        A.ab = pick3(T.Q, in);
+        A.ab_next = pick3(T.Q, out);
         // source A^(2*floor(j/2)): the input on even sweeps, A^n on sweep 1,
         // the output buffer's previous contents on odd j >= 3
-        A.src = (j % 2 == 0) ? T.Q[in] : (j == 1 ? T.Q[qc] : T.Q[out]);
-        A.rhs = (T.rhs_ab && j > 0) ? T.rhs_ab : T.Q[qc];
+        A.src = pick3(T.Q, (j % 2 == 0) ? in : (j == 1 ? qc : out));
+        A.rhs = (T.rhs_ab && j > 0) ? T.rhs_ab : pick3(T.Q, qc);
         A.rhs_out = T.rhs_ab;
-        int rmax = 0;
-        for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rmax = max(rmax, a_rows<R, GAUGE>(A, s, ja, jb)); });
+        double rr = 0.0;
+        for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, a_rows<R, GAUGE>(A, s, ja, jb)); });
         int r = 0;
-        if (!sweep_sync(T.sc, rmax, epoch, gsweep, lds, r)) {
+        if (!sweep_sync(T.sc, scaled_residual(rr, A.stop_eps), 0, epoch, lds, r)) {
             if (blockIdx.x == 0 && threadIdx.x == 0) st[ST_ERRSTEP] = T.step + 1;
             return;
         }
-        ++gsweep;
         nsw = j + 1;
         in = out;
         out = (out == o1) ? o2 : o1;
-        if (r < 10000) break;
+        if ((r & 0xffff) < 10000) break;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st[ST_AB] = in;
         *state64(st, ST_EPOCH) = epoch;
-        *state64(st, ST_GSWEEP) = gsweep;
         T.sc.sweeps[2 * T.step + 1] = nsw;
     }
 }
 
-// Zero the barrier counters, residual slots and run state (before the first
-// solve of a run, or when the grid size changes) and set the buffer indices.
+// Zero the barrier words and run state before the first solve of a run and
+// set the buffer indices.
 static __global__ void k_td_reset(TDScratch sc, int pc, int qc) {
     const int t = threadIdx.x;
-    for (int k = t; k < SLOT_RING * NGRP; k += blockDim.x) sc.slots[k] = 0;
-    for (int k = t; k < NGRP * GS_STRIDE; k += blockDim.x) { sc.gs.grp[k] = 0ull; sc.gs.rel[k] = 0ull; }
+    for (int k = t; k < MAX_GRID_BLOCKS; k += blockDim.x) sc.gs.arrive[k] = 0ull;
+    for (int k = t; k < NREL * REL_STRIDE; k += blockDim.x) sc.gs.rel[k] = 0ull;
     if (t == 0) {
-        *sc.gs.top = 0ull;
         *sc.gs.err = 0;
         sc.state[ST_PSI] = pc;
         sc.state[ST_AB] = qc;
         *state64(sc.state, ST_EPOCH) = 0ull;
-        *state64(sc.state, ST_GSWEEP) = 0ull;
         sc.state[ST_ERRSTEP] = 0;
+        sc.state[ST_PRED_PSI] = 0;
     }
 }
 

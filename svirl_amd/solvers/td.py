@@ -102,7 +102,7 @@ class TD:
 
     def _default_wg(self):
         cfg = self.cfg
-        strips = -(-int(cfg.Nx) // 63)
+        strips = -(-int(cfg.Nx) // 62)
         rows = strips * int(cfg.Ny)
         waves = max(1, rows // 8)
         maxg = ctypes.c_int32()
