@@ -1,19 +1,26 @@
-"""Benchmark: TDGL node-steps/s at 4097^2 nodes (Config 3), FP32-mixed and
-FP64, with the roofline of the persistent TD kernel and a CPU baseline.
+"""Benchmark: TDGL node-steps/s and CG iterations/s at 4097^2 nodes (Config 3),
+FP32-mixed and FP64, with the roofline of the dominant kernel and a CPU
+baseline.
 
 Workload (Config 3): Lx = Ly = 2048, dx = 0.5 (4097^2 nodes), kappa = 5,
 sigma = 200, H = 0.05, dt = 0.05, seeded random psi, a mask of random
 circular holes (5% of the area) and a Gaussian random pinning field
-eps(r) (std 0.2). A "step" is one TDGL time step: the psi Jacobi solve and
-the A Jacobi solve, each swept to the reference's stop rule.
+eps(r) (std 0.2). A TD "step" is one TDGL time step: the psi Jacobi solve
+and the A Jacobi solve, each swept to the reference's stop rule. A CG
+iteration is one Polak-Ribiere step with its exact polynomial line search.
 
-value = node-steps/s of the FP32-mixed form (whole job, one GPU), timed
-over K steps after W warm-up steps with the fields resident in HBM. The FP64
-form is timed the same way and reported beside it. roofline.achieved =
-algorithmic HBM bytes of the timed launch / its kernel duration (HIP events
-on the library stream); bytes per node per sweep: psi 8s + e + m, A 8s + m
-(even sweeps and sweep 1) or 10s + m (odd sweeps >= 3), s = 4 or 8,
-e = s (eps map), m = 1 (mask).
+value = node-steps/s of the FP32-mixed TD (whole job, one GPU), timed over
+K steps after W warm-up steps with the fields resident in HBM. The FP64 form
+is timed the same way and reported beside it; CG iterations/s follow.
+
+roofline: the dominant kernel is the persistent psi solve (k_td_psi_solve,
+one launch per step). A second run of the same workload brackets every
+solve launch with HIP events on the library stream (glx_td_profile);
+achieved = algorithmic bytes of the psi launches / their summed duration.
+Algorithmic bytes per node per psi sweep: 8s + e + m (s = 4 or 8, e = s
+for the eps map, m = 1 for the mask) times the logged sweep counts (the
+fused two-sweep passes move less; `traffic` is the measured HBM bytes per
+launch from the committed rocprofv3 counter summary, when present).
 """
 import argparse
 import ctypes
@@ -28,24 +35,28 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "td_psi_solve_4097_f32_pmc.json")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def bytes_per_step(sweeps, s, e, m, N):
-    """Algorithmic HBM bytes of the given (S_psi, S_A) steps."""
+def psi_bytes(sweeps, s, e, m, N):
+    """Algorithmic HBM bytes of the psi solves of the given (S_psi, S_A) steps."""
+    return sum(sp for sp, _ in sweeps) * (8 * s + e + m) * N
+
+
+def a_bytes(sweeps, s, m, N):
     total = 0
-    for sp, sa in sweeps:
-        total += sp * (8 * s + e + m)
+    for _, sa in sweeps:
         for j in range(sa):
             total += (8 * s + m) if (j % 2 == 0 or j == 1) else (10 * s + m)
     return total * N
 
 
-def run_td(gl, dt, warmup, steps):
-    """Warm up, then time `steps` TD steps in one persistent launch."""
+def run_td(gl, dt, warmup, steps, profile=False):
+    """Warm up, then run `steps` TD steps in one glx_td_run call."""
     from svirl_amd import _native as nat
     gl.solve._init_td()
     td = gl.solve._td
@@ -55,13 +66,49 @@ def run_td(gl, dt, warmup, steps):
     nat.call("glx_synchronize")
     n0 = len(td.sweeps)
     td.steps_per_launch = steps
+    nat.call("glx_td_profile", 1 if profile else 0)
     t0 = time.perf_counter()
     gl.solve.td(dt=dt, Nt=steps)
     nat.call("glx_synchronize")
     wall = time.perf_counter() - t0
+    nat.call("glx_td_profile", 0)
     kms = ctypes.c_float()
     nat.call("glx_td_last_kernel_ms", ctypes.byref(kms))
-    return td.sweeps[n0:], wall, kms.value / 1e3
+    pm, am = ctypes.c_double(), ctypes.c_double()
+    pn, an = ctypes.c_int32(), ctypes.c_int32()
+    nat.call("glx_td_solve_ms", ctypes.byref(pm), ctypes.byref(pn), ctypes.byref(am), ctypes.byref(an))
+    return dict(sweeps=td.sweeps[n0:], wall=wall, kernel_s=kms.value / 1e3,
+                psi_s=pm.value / 1e3, psi_n=pn.value, a_s=am.value / 1e3, a_n=an.value)
+
+
+def run_cg(gl, iters):
+    """`iters` CG iterations (the stop rule is disabled by a zero tolerance)."""
+    from svirl_amd import _native as nat
+    gl.solve._init_cg()
+    cg = gl.solve._cg
+    saved = cg._CG__convergence_rtol
+    cg._CG__convergence_rtol = 0.0
+    gl.solve.cg(n_iter=2)  # warm-up
+    nat.call("glx_synchronize")
+    t0 = time.perf_counter()
+    gl.solve.cg(n_iter=iters)
+    nat.call("glx_synchronize")
+    t = time.perf_counter() - t0
+    cg._CG__convergence_rtol = saved
+    return dict(iters=len(cg.cg_energies), wall=t, F=float(cg.cg_energies[-1]))
+
+
+def calibrate_copy(n=1 << 26, reps=3):
+    """FP32 axpy over n reals (reads 8n bytes, writes 4n), for scaling the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
+    from svirl_amd import _native as nat
+    x = nat.DeviceArray.zeros(n, np.float32)
+    y = nat.DeviceArray.zeros(n, np.float32)
+    for _ in range(reps):
+        nat.call("glx_axpy", nat.GLX_F32, x.ptr, y.ptr, y.ptr, 0.5, n)
+    nat.call("glx_synchronize")
+    x.free()
+    y.free()
 
 
 def cpu_baseline(L, dt, steps):
@@ -86,14 +133,30 @@ def cpu_baseline(L, dt, steps):
     return g.N * steps / t, t, g.N, st.sweeps[1:]
 
 
+def pmc_traffic():
+    """HBM bytes per k_td_psi_solve<float> launch from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration)."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            d = json.load(f)
+        return d.get("bytes_per_launch"), d.get("algorithmic_bytes_per_launch")
+    except (OSError, ValueError):
+        return None, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--L", type=float, default=2048.0, help="box size (2048 = 4097^2 nodes)")
+    ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--cg-iters", type=int, default=20)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-cg", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run FP32 axpy dispatches of known size first (counter calibration)")
     ap.add_argument("--cpu-L", type=float, default=512.0)
     ap.add_argument("--cpu-steps", type=int, default=3)
     args = ap.parse_args()
@@ -107,23 +170,47 @@ def main():
     nat.init(0)
     info = nat.device_info()
     log(f"device: {info}")
+    if args.calibrate:
+        calibrate_copy()
+        log("[calib] 3 FP32 axpy dispatches over 2^26 reals")
     dt = 0.05
     res = {}
     for prec in (["fp32"] if args.skip_fp64 else ["fp32", "fp64"]):
         dtype = np.float32 if prec == "fp32" else np.float64
+        s = 4 if prec == "fp32" else 8
         log(f"[{prec}] building Config 3 state, L={args.L}")
         gl = workloads.config3(L=args.L, dtype=dtype)
         N = int(gl.cfg.N)
         log(f"[{prec}] N = {N}; warmup {args.warmup} steps, timing {args.steps}")
-        sweeps, wall, ksec = run_td(gl, dt, args.warmup, args.steps)
-        s = 4 if prec == "fp32" else 8
-        B = bytes_per_step(sweeps, s, s, 1, N)
-        sp = np.array(sweeps)
-        res[prec] = dict(N=N, wall=wall, kernel_s=ksec, bytes=B, sweeps_psi=float(sp[:, 0].mean()),
-                         sweeps_A=float(sp[:, 1].mean()), node_steps_per_s=N * args.steps / wall,
-                         gbs=B / ksec / 1e9, F=gl.observables.free_energy_fp64())
-        log(f"[{prec}] {res[prec]['node_steps_per_s']:.4g} node-steps/s, {res[prec]['gbs']:.1f} GB/s, "
-            f"sweeps/step psi {res[prec]['sweeps_psi']:.2f} A {res[prec]['sweeps_A']:.2f}")
+        r = run_td(gl, dt, args.warmup, args.steps)
+        sp = np.array(r["sweeps"])
+        out = dict(N=N, wall=r["wall"], kernel_s=r["kernel_s"], sweeps_psi=float(sp[:, 0].mean()),
+                   sweeps_A=float(sp[:, 1].mean()), node_steps_per_s=N * args.steps / r["wall"])
+        out["all_gbs"] = (psi_bytes(r["sweeps"], s, s, 1, N) + a_bytes(r["sweeps"], s, 1, N)) / r["kernel_s"] / 1e9
+        log(f"[{prec}] {out['node_steps_per_s']:.4g} node-steps/s ({1e3 * r['wall'] / args.steps:.3f} ms/step wall, "
+            f"{1e3 * r['kernel_s'] / args.steps:.3f} ms/step on the GPU), "
+            f"sweeps/step psi {out['sweeps_psi']:.2f} A {out['sweeps_A']:.2f}")
+        if args.steps <= 10:
+            log(f"[{prec}] sweeps per step (psi, A): {r['sweeps']}")
+        if args.profile_steps:
+            p = run_td(gl, dt, 0, args.profile_steps, profile=True)
+            pb = psi_bytes(p["sweeps"], s, s, 1, N)
+            ab = a_bytes(p["sweeps"], s, 1, N)
+            out["psi_launch_us"] = 1e6 * p["psi_s"] / max(1, p["psi_n"])
+            out["a_launch_us"] = 1e6 * p["a_s"] / max(1, p["a_n"])
+            out["psi_gbs"] = pb / p["psi_s"] / 1e9
+            out["a_gbs"] = ab / p["a_s"] / 1e9 if p["a_s"] > 0 else None
+            out["psi_bytes_per_launch"] = pb / max(1, p["psi_n"])
+            log(f"[{prec}] psi solve {out['psi_launch_us']:.1f} us/launch, {out['psi_gbs']:.1f} GB/s algorithmic; "
+                f"A solve {out['a_launch_us']:.1f} us/launch")
+        out["F_td"] = gl.observables.free_energy_fp64()
+        if not args.skip_cg:
+            c = run_cg(gl, args.cg_iters)
+            out["cg_iters_per_s"] = c["iters"] / c["wall"]
+            out["cg_ms_per_iter"] = 1e3 * c["wall"] / c["iters"]
+            out["F_cg"] = c["F"]
+            log(f"[{prec}] CG {out['cg_iters_per_s']:.2f} iters/s ({out['cg_ms_per_iter']:.2f} ms/iter)")
+        res[prec] = out
         del gl
 
     cpu = None
@@ -137,8 +224,16 @@ def main():
         log(f"[cpu] {rate:.4g} node-steps/s")
 
     r32 = res["fp32"]
+    traffic, alg_pmc = pmc_traffic()
+    roof = {"bound": "hbm", "achieved": r32.get("psi_gbs"), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (r32["psi_gbs"] / HBM_PEAK_GBS) if r32.get("psi_gbs") else None,
+            "traffic": traffic,
+            "kernel": "k_td_psi_solve<float, gauge-shifted A>",
+            "launch_us": r32.get("psi_launch_us"),
+            "algorithmic_bytes_per_launch": r32.get("psi_bytes_per_launch"),
+            "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if traffic else None}
     out = {
-        "metric": "TDGL node-steps/s at 4097^2 (FP32-mixed; FP64 beside it)",
+        "metric": "TDGL node-steps/s at 4097^2 (FP32-mixed; FP64 and CG iters/s beside it)",
         "value": r32["node_steps_per_s"],
         "unit": "node-steps/s",
         "n_gpus": 1,
@@ -148,23 +243,31 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 fields, f64 reductions/residuals",
+        "dtype": "f32",
         "data": "synthetic (seeded Config 3 inputs)",
         "config": {"workload": "Config 3: 4097^2 nodes, kappa=5, sigma=200, H=0.05, dt=0.05, 5% circular holes, "
                                "Gaussian pinning eps std 0.2", "nodes": r32["N"], "L": args.L},
-        "roofline": {"bound": "hbm", "achieved": r32["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": r32["gbs"] / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_td_run<float>", "kernel_s": r32["kernel_s"], "algorithmic_bytes": r32["bytes"]},
+        "roofline": roof,
         "sweeps_per_step": {"psi": r32["sweeps_psi"], "A": r32["sweeps_A"]},
+        "a_solve": {"launch_us": r32.get("a_launch_us"), "achieved_GBs": r32.get("a_gbs")},
+        "td_all_kernels_GBs": r32["all_gbs"],
         "cpu_baseline": cpu,
     }
+    if "cg_iters_per_s" in r32:
+        out["cg"] = {"fp32_iters_per_s": r32["cg_iters_per_s"], "fp32_ms_per_iter": r32["cg_ms_per_iter"]}
     if "fp64" in res:
         r64 = res["fp64"]
         out["fp64"] = {"node_steps_per_s": r64["node_steps_per_s"], "ms_per_step": 1e3 * r64["wall"] / args.steps,
-                       "achieved_GBs": r64["gbs"], "frac": r64["gbs"] / HBM_PEAK_GBS,
-                       "sweeps_per_step": {"psi": r64["sweeps_psi"], "A": r64["sweeps_A"]}, "F": r64["F"]}
+                       "psi_solve_GBs": r64.get("psi_gbs"),
+                       "psi_solve_frac": (r64["psi_gbs"] / HBM_PEAK_GBS) if r64.get("psi_gbs") else None,
+                       "psi_launch_us": r64.get("psi_launch_us"),
+                       "sweeps_per_step": {"psi": r64["sweeps_psi"], "A": r64["sweeps_A"]}, "F": r64["F_td"]}
         out["fp32_over_fp64"] = r32["node_steps_per_s"] / r64["node_steps_per_s"]
-        out["fp32_F_rel_err_vs_fp64"] = abs(r32["F"] - r64["F"]) / abs(r64["F"])
+        out["fp32_F_rel_err_vs_fp64"] = abs(r32["F_td"] - r64["F_td"]) / abs(r64["F_td"])
+        if "cg_iters_per_s" in r64:
+            out["cg"]["fp64_iters_per_s"] = r64["cg_iters_per_s"]
+            out["cg"]["fp64_ms_per_iter"] = r64["cg_ms_per_iter"]
+            out["cg"]["fp32_F_rel_err_vs_fp64"] = abs(r32["F_cg"] - r64["F_cg"]) / abs(r64["F_cg"])
     print(json.dumps(out), flush=True)
 
 

@@ -139,6 +139,14 @@ int glx_td_run(int prec, glx_grid g, const glx_td_params *p, const void *eps_map
  * of the last glx_td_run call, excluding its reset and read-back. */
 int glx_td_last_kernel_ms(float *ms);
 
+/* Per-launch timing of glx_td_run's solve kernels (no counterpart in the
+ * reference; a measurement hook). While enabled, glx_td_run brackets every
+ * psi-solve and A-solve launch with HIP events on the library stream;
+ * glx_td_solve_ms returns the summed durations (ms) and launch counts of the
+ * last glx_td_run call (zeros when profiling was off). */
+int glx_td_profile(int32_t enable);
+int glx_td_solve_ms(double *psi_ms, int32_t *psi_launches, double *a_ms, int32_t *a_launches);
+
 /* Largest co-resident grid for glx_td_run at a workgroup size. */
 int glx_td_max_grid(int prec, int32_t wg_size, int32_t *max_wg);
 

@@ -62,6 +62,8 @@ _PROTOS = {
                    ctypes.POINTER(c_void_p), P_i32, c_void_p, c_void_p, c_i32, P_i32, c_i32, c_i32],
     "glx_td_max_grid": [c_int, c_i32, P_i32],
     "glx_td_last_kernel_ms": [ctypes.POINTER(ctypes.c_float)],
+    "glx_td_profile": [c_i32],
+    "glx_td_solve_ms": [ctypes.POINTER(c_double), P_i32, ctypes.POINTER(c_double), P_i32],
     "glx_magnetic_field": [c_int, Grid, c_void_p, c_void_p, c_void_p],
     "glx_current_density": [c_int, Grid, c_double, c_double, c_void_p, c_void_p, c_void_p],
     "glx_supercurrent_density": [c_int, Grid, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],

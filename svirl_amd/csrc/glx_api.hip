@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <vector>
 
 #include "glx_launch.hpp"
 
@@ -18,6 +19,10 @@ struct Ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_kernel_ms = 0.f;
+    double solve_ms[2] = {0.0, 0.0};
+    int solve_n[2] = {0, 0};
+    std::vector<hipEvent_t> tev;
+    hipDeviceProp_t prop{};
     Env env{};
     void *scratch = nullptr;
 };
@@ -85,6 +90,8 @@ int glx_init(int device) {
     if ((st = hipEventCreate(&g_ctx.env.kev0)) != hipSuccess) return check_hip(st, "hipEventCreate");
     if ((st = hipEventCreate(&g_ctx.env.kev1)) != hipSuccess) return check_hip(st, "hipEventCreate");
     g_ctx.env.last_kernel_ms = &g_ctx.last_kernel_ms;
+    g_ctx.env.solve_ms = g_ctx.solve_ms;
+    g_ctx.env.solve_n = g_ctx.solve_n;
     // scratch layout: partials | result | arrive | rel | err | state | sweeps
     const size_t partials = sizeof(double) * 1024 * 32;
     const size_t result = sizeof(double) * 64;
@@ -108,6 +115,7 @@ int glx_init(int device) {
     e.sweeps = (int *)p; p += sweeps;
     e.sweeps_cap = SWEEPS_CAP;
     e.num_cus = prop.multiProcessorCount;
+    g_ctx.prop = prop;
     g_ctx.device = device;
     g_ctx.ready = true;
     return GLX_OK;
@@ -120,9 +128,7 @@ int glx_synchronize(void) {
 
 int glx_device_info(int *num_cus, int *max_wg_per_cu_hint, char *name, int name_len) {
     GLX_REQUIRE_INIT();
-    hipDeviceProp_t prop;
-    hipError_t st = hipGetDeviceProperties(&prop, g_ctx.device);
-    if (st != hipSuccess) return check_hip(st, "hipGetDeviceProperties");
+    const hipDeviceProp_t &prop = g_ctx.prop;  // queried once by glx_init
     if (num_cus) *num_cus = prop.multiProcessorCount;
     if (max_wg_per_cu_hint) *max_wg_per_cu_hint = prop.maxThreadsPerMultiProcessor / 256;
     if (name && name_len > 0) {
@@ -203,6 +209,19 @@ int glx_td_phase_lock(int prec, void *psi, int32_t lock_N, const int32_t *lock_n
 int glx_td_run(int prec, glx_grid g, const glx_td_params *p, const void *eps_map, const uint8_t *mt,
                void *const psi_bufs[3], int32_t *psi_cur, void *const ab_bufs[3], int32_t *ab_cur, void *rhs_psi,
                void *rhs_ab, int32_t nsteps, int32_t *sweeps, int32_t wg, int32_t wg_size) {
+    if (g_ctx.ready && g_ctx.env.td_profile && nsteps > 0 && nsteps <= SWEEPS_CAP) {
+        // grow the event pool to two events per solve launch
+        std::lock_guard<std::mutex> lk(g_mu);
+        const size_t need = 4 * (size_t)nsteps;
+        while (g_ctx.tev.size() < need) {
+            hipEvent_t ev;
+            hipError_t st = hipEventCreate(&ev);
+            if (st != hipSuccess) return check_hip(st, "hipEventCreate");
+            g_ctx.tev.push_back(ev);
+        }
+        g_ctx.env.tev = g_ctx.tev.data();
+        g_ctx.env.tev_cap = (int)g_ctx.tev.size();
+    }
     GLX_DISPATCH(prec, td_run(g_ctx.env, g, p, eps_map, mt, psi_bufs, psi_cur, ab_bufs, ab_cur, rhs_psi, rhs_ab,
                               nsteps, sweeps, wg, wg_size));
 }
@@ -211,6 +230,22 @@ int glx_td_last_kernel_ms(float *ms) {
     GLX_REQUIRE_INIT();
     if (!ms) return set_error(GLX_E_ARG, "glx_td_last_kernel_ms: NULL");
     *ms = g_ctx.last_kernel_ms;
+    return GLX_OK;
+}
+
+int glx_td_profile(int32_t enable) {
+    GLX_REQUIRE_INIT();
+    g_ctx.env.td_profile = enable != 0;
+    return GLX_OK;
+}
+
+int glx_td_solve_ms(double *psi_ms, int32_t *psi_launches, double *a_ms, int32_t *a_launches) {
+    GLX_REQUIRE_INIT();
+    if (!psi_ms || !psi_launches || !a_ms || !a_launches) return set_error(GLX_E_ARG, "glx_td_solve_ms: NULL");
+    *psi_ms = g_ctx.solve_ms[0];
+    *psi_launches = g_ctx.solve_n[0];
+    *a_ms = g_ctx.solve_ms[1];
+    *a_launches = g_ctx.solve_n[1];
     return GLX_OK;
 }
 

@@ -116,12 +116,40 @@ template <typename R> __device__ __forceinline__ R noise_2(uint32_t n, uint32_t 
     return R(2.3283064365386962890625e-10 * double(R(wang_hash(73u * n + 9901u * t + 1u))));
 }
 
-// Scaled Jacobi residual of one point: int(min(1e4*r/stop_eps, 1e8)),
-// evaluated in double.
-__device__ __forceinline__ int scaled_residual(double r, double stop_eps) {
+// Scaled Jacobi residual of the stop rule (svirl/cuda/td.h:123-131):
+// int(min(1e4 * r / stop_eps, 1e8)) with the quotient in double and, as in
+// the reference, stored in the working precision before the conversion.
+// r is the largest |delta| (non-decreasing in r, so it may be applied to a
+// maximum); stop_eps is already rounded to the working precision.
+template <typename R> __device__ __forceinline__ int scaled_residual(R r, double stop_eps);
+template <> __device__ __forceinline__ int scaled_residual<double>(double r, double stop_eps) {
     double v = 1.0e4 * r / stop_eps;
     if (v > 1.0e8) v = 1.0e8;
     return int(v);
+}
+template <> __device__ __forceinline__ int scaled_residual<float>(float r, double stop_eps) {
+    float v = float(1.0e4 * double(r) / stop_eps);
+    if (v > 1.0e8f) v = 1.0e8f;
+    return int(v);
+}
+
+// Lane i receives lane i-1 (up) or lane i+1 (down) of a wave64, by DPP
+// whole-wave shifts; the edge lane that has no source gets 0.
+__device__ __forceinline__ int dpp_up1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false); }
+__device__ __forceinline__ int dpp_dn1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false); }
+__device__ __forceinline__ int shfl_up1(int v) { return dpp_up1(v); }
+__device__ __forceinline__ int shfl_dn1(int v) { return dpp_dn1(v); }
+__device__ __forceinline__ float shfl_up1(float v) { return __int_as_float(dpp_up1(__float_as_int(v))); }
+__device__ __forceinline__ float shfl_dn1(float v) { return __int_as_float(dpp_dn1(__float_as_int(v))); }
+__device__ __forceinline__ double shfl_up1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_up1((int)(b & 0xffffffffll)), hi = dpp_up1((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double shfl_dn1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_dn1((int)(b & 0xffffffffll)), hi = dpp_dn1((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 // ---- wave64 / workgroup reductions ---------------------------------------

@@ -24,6 +24,14 @@ struct Env {
     int num_cus;
     hipEvent_t kev0, kev1;   // around the persistent TD kernel
     float *last_kernel_ms;   // host slot written by td_run
+    // per-launch timing of the TD solves (glx_td_profile): an event before
+    // and after every solve launch; td_run sums them per kind into solve_ms
+    // (psi, A) and counts the launches in solve_n
+    bool td_profile;
+    hipEvent_t *tev;
+    int tev_cap;
+    double *solve_ms;
+    int *solve_n;
 };
 
 template <typename R> struct L {

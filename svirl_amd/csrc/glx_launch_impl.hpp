@@ -131,11 +131,15 @@ template <> struct RunForm<float> { static constexpr bool gauge = true; };
 template <typename R> int L<R>::td_max_grid(const Env &e, int32_t wg_size, int32_t *max_wg) {
     if (wg_size != 256 || !max_wg) return set_error(GLX_E_ARG, "glx_td_max_grid: wg_size must be 256");
     constexpr bool G = RunForm<R>::gauge;
-    int a = 0, b = 0;
-    hipError_t st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_td_psi_solve<R, G>, 256, 0);
-    if (st == hipSuccess) st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_td_a_solve<R, G>, 256, 0);
-    if (st != hipSuccess) return check_hip(st, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-    int m = (a < b ? a : b) * e.num_cus;
+    static int per_cu = 0;  // co-resident workgroups per CU of both solve kernels (queried once)
+    if (per_cu <= 0) {
+        int a = 0, b = 0;
+        hipError_t st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_td_psi_solve<R, G>, 256, 0);
+        if (st == hipSuccess) st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_td_a_solve<R, G>, 256, 0);
+        if (st != hipSuccess) return check_hip(st, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        per_cu = a < b ? a : b;
+    }
+    int m = per_cu * e.num_cus;
     *max_wg = m < MAX_GRID_BLOCKS ? m : MAX_GRID_BLOCKS;
     return GLX_OK;
 }
@@ -199,17 +203,23 @@ int L<R>::td_run(const Env &e, glx_grid g, const glx_td_params *p, const void *e
     if (est != hipSuccess) return check_hip(est, "hipEventRecord");
     uint32_t rt = p->rand_t;
     const bool do_psi = p->eqn != 2, do_A = p->solveA && p->eqn != 1;
+    const bool prof = e.td_profile && e.tev && e.tev_cap >= 4 * nsteps;
+    int nev = 0;
     for (int step = 0; step < nsteps; ++step) {
         T.step = step;
         if (do_psi) {
             T.psi.rand_t = rt++;
+            if (prof) (void)hipEventRecord(e.tev[nev++], e.stream);
             hipLaunchKernelGGL((k_td_psi_solve<R, G>), dim3((unsigned)wg), dim3(256), 0, e.stream, T);
             GLX_LAUNCH_CHECK("k_td_psi_solve");
+            if (prof) (void)hipEventRecord(e.tev[nev++], e.stream);
         }
         if (do_A) {
             T.A.rand_t = rt++;
+            if (prof) (void)hipEventRecord(e.tev[nev++], e.stream);
             hipLaunchKernelGGL((k_td_a_solve<R, G>), dim3((unsigned)wg), dim3(256), 0, e.stream, T);
             GLX_LAUNCH_CHECK("k_td_a_solve");
+            if (prof) (void)hipEventRecord(e.tev[nev++], e.stream);
         }
     }
     est = hipEventRecord(e.kev1, e.stream);
@@ -228,6 +238,23 @@ int L<R>::td_run(const Env &e, glx_grid g, const glx_td_params *p, const void *e
     if (st != hipSuccess) return check_hip(st, "glx_td_run readback");
     float kms = 0.f;
     if (hipEventElapsedTime(&kms, e.kev0, e.kev1) == hipSuccess && e.last_kernel_ms) *e.last_kernel_ms = kms;
+    if (e.solve_ms && e.solve_n) {
+        e.solve_ms[0] = e.solve_ms[1] = 0.0;
+        e.solve_n[0] = e.solve_n[1] = 0;
+        if (prof) {
+            int k = 0;
+            for (int step = 0; step < nsteps; ++step)
+                for (int kind = 0; kind < 2; ++kind) {
+                    if ((kind == 0 && !do_psi) || (kind == 1 && !do_A)) continue;
+                    float ms = 0.f;
+                    if (hipEventElapsedTime(&ms, e.tev[k], e.tev[k + 1]) == hipSuccess) {
+                        e.solve_ms[kind] += ms;
+                        e.solve_n[kind] += 1;
+                    }
+                    k += 2;
+                }
+        }
+    }
     if (host_err) return set_error(GLX_E_TIMEOUT, "glx_td_run: grid barrier timed out in step %d of %d", host_state[ST_ERRSTEP], nsteps);
     if (!do_psi)
         for (int k = 0; k < nsteps; ++k) sweeps[2 * k] = 0;

@@ -13,6 +13,8 @@
 // reassociation); FP64 instantiations are compiled without FMA contraction.
 #pragma once
 
+#include <type_traits>
+
 #include "glx_device.hpp"
 
 namespace glx {
@@ -60,29 +62,29 @@ template <typename R> struct ASweepArgs {
 __device__ __forceinline__ double gauge_a(double H, double dy, double y0, int j) { return -0.5 * H * (double(j) * dy - y0); }
 __device__ __forceinline__ double gauge_b(double H, double dx, double x0, int c) { return 0.5 * H * (double(c) * dx - x0); }
 
-// Link phase d*A of a stored value v: plain, or with the gauge added back.
+// Link phase d*A of a stored value v: plain, or (gauge-shifted storage) with
+// the gauge's phase dg = d*A_g added back, dg being evaluated in double once
+// per row or column and rounded to the working precision.
 template <typename R, bool GAUGE>
-__device__ __forceinline__ R link_phase(R d, R v, double g) {
-    if (GAUGE) return R(double(d) * (g + double(v)));
+__device__ __forceinline__ R link_phase(R d, R v, R dg) {
+    if (GAUGE) return fma(d, v, dg);
     return d * v;
 }
-
-template <typename R> __device__ __forceinline__ R shfl_up1(R v) { return __shfl_up(v, 1, WAVE); }
-template <typename R> __device__ __forceinline__ R shfl_dn1(R v) { return __shfl_down(v, 1, WAVE); }
-
-__device__ __forceinline__ bool cell_at(const uint8_t *mt, int Nxc, int Nyc, int c, int j) {
-    if (c < 0 || c >= Nxc || j < 0 || j >= Nyc) return false;
-    return mt ? (mt[c + (long)Nxc * j] != 0) : true;
+template <typename R, typename Args> __device__ __forceinline__ R gauge_phase_a(const Args &A, int j) {
+    return R(double(A.dx) * gauge_a(A.gH, A.gdy, A.gy0, j));
+}
+template <typename R, typename Args> __device__ __forceinline__ R gauge_phase_b(const Args &A, int c) {
+    return R(double(A.dy) * gauge_b(A.gH, A.gdx, A.gx0, c));
 }
 
-// Rows are loaded in batches of PIPE_ROWS: the loads of batch k+1 are issued
-// before batch k is computed, and the two batches live in separately named
-// registers (the loop body is unrolled over the pair), so no loaded value is
-// copied before it is used and 1 to 2 batches of loads per wave stay in
-// flight during the arithmetic. Loads beyond the wave's last row are clamped
-// to that row (cache hits, never used).
-template <typename R> struct Pipe { static constexpr int ROWS = 2; };
-template <> struct Pipe<double> { static constexpr int ROWS = 1; };
+// Rows are loaded in batches of Pipe<R>::ROWS: the loads of batch k+1 are
+// issued before batch k is computed, and the two batches live in separately
+// named registers (the loop body is unrolled over the pair), so no loaded
+// value is copied before it is used and 1 to 2 batches of loads per wave stay
+// in flight during the arithmetic. Loads beyond the wave's last row are
+// clamped to that row (cache hits, never used).
+template <typename R> struct Pipe { static constexpr int ROWS = 2, ROWS2 = 1; };
+template <> struct Pipe<double> { static constexpr int ROWS = 1, ROWS2 = 1; };
 
 // Per-lane column context of a strip: c may lie outside the grid (halo lanes
 // at the edges); cc and ca are c clamped into the node and link ranges, so
@@ -158,7 +160,15 @@ __device__ __forceinline__ cplx<R> psi_node(const PsiSweepArgs<R> &A, const cplx
     const R diag = R(1) + A.dt * q;
     const R nre = rhs.re + A.dt * (A.idx2 * (m0r + p0r) + A.idy2 * (q0mr + q0pr));
     const R nim = rhs.im + A.dt * (A.idx2 * (m0i + p0i) + A.idy2 * (q0mi + q0pi));
-    return {nre / diag, nim / diag};
+    if constexpr (std::is_same<R, float>::value) {
+        // FP32: one reciprocal (hardware estimate + one Newton step, within
+        // 1 ulp) for both components instead of two IEEE divisions
+        float rd = __builtin_amdgcn_rcpf(diag);
+        rd = fmaf(rd, fmaf(-diag, rd, 1.0f), rd);
+        return {nre * rd, nim * rd};
+    } else {
+        return {nre / diag, nim / diag};
+    }
 }
 
 // The reference's Langevin term added to the right-hand side at node n.
@@ -174,7 +184,7 @@ __device__ __forceinline__ void psi_noise(const PsiSweepArgs<R> &A, uint32_t n, 
 // |delta| (max of real and imaginary parts); the stop rule's scaled integer
 // residual is monotone in it, so it is applied once per wave by the caller.
 template <typename R, bool GAUGE>
-__device__ __forceinline__ double psi_rows(const PsiSweepArgs<R> &A, int s, int ja, int jb) {
+__device__ __forceinline__ R psi_rows(const PsiSweepArgs<R> &A, int s, int ja, int jb) {
     constexpr int B = Pipe<R>::ROWS;
     const ColCtx X = col_ctx<1>(A.Nx, s);
     const int c = X.c;
@@ -185,8 +195,8 @@ __device__ __forceinline__ double psi_rows(const PsiSweepArgs<R> &A, int s, int 
     const bool has_mt = A.mt != nullptr;
     const uint8_t *mtp = has_mt ? A.mt : reinterpret_cast<const uint8_t *>(A.ab);     // any readable bytes
     const R *emp = A.eps_map ? A.eps_map : reinterpret_cast<const R *>(A.rhs);         // when absent
-    const double gb = gauge_b(A.gH, A.gdx, A.gx0, c);
-    double rr = 0.0;
+    const R gb = GAUGE ? gauge_phase_b<R>(A, c) : R(0);
+    R rr = R(0);
 
     // window: psi at rows j-1, j of this column; U of b(c, j-1); cell (c, j-1)
     cplx<R> pm = {R(0), R(0)}, pc = {R(0), R(0)};
@@ -210,7 +220,7 @@ __device__ __forceinline__ double psi_rows(const PsiSweepArgs<R> &A, int s, int 
         cplx<R> rhs = cu.rhs;
         const R e = A.eps_map ? cu.e : A.eps;
         R sa, ca, sb, cb;
-        sincos_r(link_phase<R, GAUGE>(A.dx, av, gauge_a(A.gH, A.gdy, A.gy0, j)), &sa, &ca);
+        sincos_r(link_phase<R, GAUGE>(A.dx, av, GAUGE ? gauge_phase_a<R>(A, j) : R(0)), &sa, &ca);
         sincos_r(link_phase<R, GAUGE>(A.dy, bv, gb), &sb, &cb);
 
         // neighbours from adjacent lanes
@@ -231,7 +241,7 @@ __device__ __forceinline__ double psi_rows(const PsiSweepArgs<R> &A, int s, int 
             }
             if (noisy && A.jstep == 0) st(A.rhs_out, c + rowoff, rhs);
             st(A.psi_next, c + rowoff, nxt);
-            rr = fmax(rr, fmax(fabs(double(nxt.re) - double(pc.re)), fabs(double(nxt.im) - double(pc.im))));
+            rr = fmax(rr, fmax(fabs(nxt.re - pc.re), fabs(nxt.im - pc.im)));
         }
         // roll the window
         pm = pc;
@@ -268,8 +278,8 @@ __device__ __forceinline__ double psi_rows(const PsiSweepArgs<R> &A, int s, int 
 // Langevin right-hand side); returns the lane's largest |delta| of each sweep
 // in r1 and r2. The arithmetic of each node update is that of psi_rows.
 template <typename R, bool GAUGE>
-__device__ __forceinline__ void psi_rows2(const PsiSweepArgs<R> &A, int s, int ja, int jb, double &r1, double &r2) {
-    constexpr int B = Pipe<R>::ROWS;
+__device__ __forceinline__ void psi_rows2(const PsiSweepArgs<R> &A, int s, int ja, int jb, R &r1, R &r2) {
+    constexpr int B = Pipe<R>::ROWS2;
     const ColCtx X = col_ctx<2>(A.Nx, s);
     const int lane = threadIdx.x & (WAVE - 1);
     const int c = X.c;
@@ -282,7 +292,7 @@ __device__ __forceinline__ void psi_rows2(const PsiSweepArgs<R> &A, int s, int j
     const bool has_mt = A.mt != nullptr;
     const uint8_t *mtp = has_mt ? A.mt : reinterpret_cast<const uint8_t *>(A.ab);
     const R *emp = A.eps_map ? A.eps_map : reinterpret_cast<const R *>(A.rhs);
-    const double gb = gauge_b(A.gH, A.gdx, A.gx0, c);
+    const R gb = GAUGE ? gauge_phase_b<R>(A, c) : R(0);
     const cplx<R> zero = {R(0), R(0)};
     const int t0 = ja > 0 ? ja - 1 : 0;
 
@@ -314,7 +324,7 @@ __device__ __forceinline__ void psi_rows2(const PsiSweepArgs<R> &A, int s, int j
         cplx<R> rhs = cu.rhs;
         const R e = A.eps_map ? cu.e : A.eps;
         R sa, ca, sb, cb;
-        sincos_r(link_phase<R, GAUGE>(A.dx, av, gauge_a(A.gH, A.gdy, A.gy0, t)), &sa, &ca);
+        sincos_r(link_phase<R, GAUGE>(A.dx, av, GAUGE ? gauge_phase_a<R>(A, t) : R(0)), &sa, &ca);
         sincos_r(link_phase<R, GAUGE>(A.dy, bv, gb), &sb, &cb);
         const R cal = shfl_up1(ca), sal = shfl_up1(sa);
         const int cl_m = shfl_up1(cell1);  // cell (c-1, t-1)
@@ -333,7 +343,7 @@ __device__ __forceinline__ void psi_rows2(const PsiSweepArgs<R> &A, int s, int j
             }
             if (own && real && t >= ja && t < jb) {
                 if (noisy && A.jstep == 0) st(A.rhs_out, c + rowoff, rhs);
-                r1 = fmax(r1, fmax(fabs(double(q.re) - double(pc.re)), fabs(double(q.im) - double(pc.im))));
+                r1 = fmax(r1, fmax(fabs(q.re - pc.re), fabs(q.im - pc.im)));
             }
         }
         // second sweep at row u = t - 1 -> psi^{j+2}(u)
@@ -348,7 +358,7 @@ __device__ __forceinline__ void psi_rows2(const PsiSweepArgs<R> &A, int s, int j
                     nxt = psi_node(A, rhs_u, e_u, pl_re, pl_im, pr_re, pr_im, qm, q, cal2, sal2, ca2, sa2, cb2, sb2,
                                    cb1, sb1, mm, mp, pmc, ppc);
                 st(A.psi_next, c + (long)Nx * u, nxt);
-                r2 = fmax(r2, fmax(fabs(double(nxt.re) - double(qc.re)), fabs(double(nxt.im) - double(qc.im))));
+                r2 = fmax(r2, fmax(fabs(nxt.re - qc.re), fabs(nxt.im - qc.im)));
             }
         }
         // roll both windows
@@ -419,7 +429,7 @@ __device__ __forceinline__ ARow<R> a_load_row(const ASweepArgs<R> &A, const ColC
 // One A sweep over rows [ja, jb) of strip s (both link families of every
 // owned node). Returns the lane's largest |delta|.
 template <typename R, bool GAUGE>
-__device__ __forceinline__ double a_rows(const ASweepArgs<R> &A, int s, int ja, int jb) {
+__device__ __forceinline__ R a_rows(const ASweepArgs<R> &A, int s, int ja, int jb) {
     constexpr int B = Pipe<R>::ROWS;
     const ColCtx X = col_ctx<1>(A.Nx, s);
     const int c = X.c;
@@ -431,8 +441,8 @@ __device__ __forceinline__ double a_rows(const ASweepArgs<R> &A, int s, int ja, 
     const bool noisy = A.rhs_out != nullptr;
     const bool has_mt = A.mt != nullptr;
     const uint8_t *mtp = has_mt ? A.mt : reinterpret_cast<const uint8_t *>(A.ab);  // any readable bytes when absent
-    const double gb = gauge_b(A.gH, A.gdx, A.gx0, c);
-    double rr = 0.0;
+    const R gb = GAUGE ? gauge_phase_b<R>(A, c) : R(0);
+    R rr = R(0);
 
     // window of this column: a(c, j-1), a(c, j), b(c, j-1), psi(c, j), cell (c, j-1)
     R am = R(0), ac = R(0), bm = R(0);
@@ -471,7 +481,7 @@ __device__ __forceinline__ double a_rows(const ASweepArgs<R> &A, int s, int ja, 
             if (ownb) rb = rb + A.langevin_c * R(double(noise_2<R>((uint32_t)nb, A.rand_t)) - 0.5);
         }
         R ssa, csa, ssb, csb;
-        sincos_r(link_phase<R, GAUGE>(A.dx, cu.sa, gauge_a(A.gH, A.gdy, A.gy0, j)), &ssa, &csa);
+        sincos_r(link_phase<R, GAUGE>(A.dx, cu.sa, GAUGE ? gauge_phase_a<R>(A, j) : R(0)), &ssa, &csa);
         sincos_r(link_phase<R, GAUGE>(A.dy, cu.sb, gb), &ssb, &csb);
 
         if (owna) {
@@ -489,7 +499,7 @@ __device__ __forceinline__ double a_rows(const ASweepArgs<R> &A, int s, int ja, 
             const R v = (ra + A.dt_rho * (jl + rh) + A.dt_rho_kappa2 * dd * (t1 + t2)) / A.den_a;
             if (noisy && A.jstep == 0) A.rhs_out[na] = ra;
             A.ab_next[na] = v;
-            rr = fmax(rr, fabs(double(v) - double(ac)));
+            rr = fmax(rr, fabs(v - ac));
         }
         if (ownb) {
             R rh = R(0), dd = R(1);
@@ -506,7 +516,7 @@ __device__ __forceinline__ double a_rows(const ASweepArgs<R> &A, int s, int ja, 
             const R v = (rb + A.dt_rho * (jl + rh) + A.dt_rho_kappa2 * dd * (t1 + t2)) / A.den_b;
             if (noisy && A.jstep == 0) A.rhs_out[nb] = rb;
             A.ab_next[nb] = v;
-            rr = fmax(rr, fabs(double(v) - double(bc)));
+            rr = fmax(rr, fabs(v - bc));
         }
         // roll
         am = ac;
@@ -555,6 +565,18 @@ __device__ __forceinline__ void for_rows(int Nx, int Ny, long w, long W, F &&f) 
     }
 }
 
+// Workgroups are dispatched to the NXCD accelerator dies round-robin by
+// blockIdx, and each die has its own L2. The persistent solves number their
+// work by xcd_block(), which gives the workgroups of one die a contiguous
+// range, so that waves sharing halo rows, halo columns and cache lines of
+// adjacent strips mostly sit on the same die and share its L2.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_block() {
+    const int G = gridDim.x, b = blockIdx.x;
+    if (G % NXCD) return b;
+    return (b % NXCD) * (G / NXCD) + b / NXCD;
+}
+
 // Row-major partition for the persistent solves: the rows are cut into
 // S = max(1, W / nstrips) segments and the work items k = (segment, strip)
 // are numbered strip-fastest; wave w takes items w, w + U, ... with
@@ -579,9 +601,9 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_psi_sweep(PsiSweepArgs<R> A, int *r2max, long W) {
     __shared__ int lds[4];
     const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
-    double rr = 0.0;
+    R rr = R(0);
     if (w < W) for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, psi_rows<R, false>(A, s, ja, jb)); });
-    const int rmax = block_max_int(scaled_residual(rr, A.stop_eps), lds);
+    const int rmax = block_max_int(scaled_residual<R>(rr, A.stop_eps), lds);
     if (threadIdx.x == 0) atomic_max_agent(r2max, rmax);
 }
 
@@ -589,9 +611,9 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_a_sweep(ASweepArgs<R> A, int *r2max, long W) {
     __shared__ int lds[4];
     const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
-    double rr = 0.0;
+    R rr = R(0);
     if (w < W) for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, a_rows<R, false>(A, s, ja, jb)); });
-    const int rmax = block_max_int(scaled_residual(rr, A.stop_eps), lds);
+    const int rmax = block_max_int(scaled_residual<R>(rr, A.stop_eps), lds);
     if (threadIdx.x == 0) atomic_max_agent(r2max, rmax);
 }
 
@@ -654,7 +676,7 @@ template <typename R, bool GAUGE>
 __global__ void __launch_bounds__(256) k_td_psi_solve(TDSolveArgs<R> T) {
     __shared__ int lds[16];
     const long W = (long)gridDim.x * (blockDim.x / WAVE);
-    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
+    const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
     int *st = T.sc.state;
     const int pc = st[ST_PSI], qc = st[ST_AB];
     const int pred = st[ST_PRED_PSI];
@@ -671,13 +693,13 @@ __global__ void __launch_bounds__(256) k_td_psi_solve(TDSolveArgs<R> T) {
         A.rhs = (T.rhs_psi && j > 0) ? T.rhs_psi : pick3(T.P, pc);
         A.jstep = (uint32_t)j;
         const bool pair = !(j == 0 && (pred & 1)) && j + 2 <= T.max_sweeps;
-        double r1 = 0.0, r2 = 0.0;
+        R r1 = R(0), r2 = R(0);
         if (pair)
             for_rows_rm<2>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { psi_rows2<R, GAUGE>(A, s, ja, jb, r1, r2); });
         else
             for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { r1 = fmax(r1, psi_rows<R, GAUGE>(A, s, ja, jb)); });
         int r = 0;
-        if (!sweep_sync(T.sc, scaled_residual(r1, A.stop_eps), scaled_residual(r2, A.stop_eps), epoch, lds, r)) {
+        if (!sweep_sync(T.sc, scaled_residual<R>(r1, A.stop_eps), scaled_residual<R>(r2, A.stop_eps), epoch, lds, r)) {
             if (blockIdx.x == 0 && threadIdx.x == 0) st[ST_ERRSTEP] = T.step + 1;
             return;
         }
@@ -686,7 +708,7 @@ __global__ void __launch_bounds__(256) k_td_psi_solve(TDSolveArgs<R> T) {
         if (pair && c1) {
             // sweep j met the stop rule: redo it alone into the same output
             A.psi_next = pick3(T.P, out);
-            double rr = 0.0;
+            R rr = R(0);
             for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, psi_rows<R, GAUGE>(A, s, ja, jb)); });
             int rdummy = 0;
             if (!sweep_sync(T.sc, 0, 0, epoch, lds, rdummy)) {
@@ -715,7 +737,7 @@ template <typename R, bool GAUGE>
 __global__ void __launch_bounds__(256) k_td_a_solve(TDSolveArgs<R> T) {
     __shared__ int lds[16];
     const long W = (long)gridDim.x * (blockDim.x / WAVE);
-    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
+    const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
     int *st = T.sc.state;
     const int pc = st[ST_PSI], qc = st[ST_AB];
     unsigned long long epoch = *state64(st, ST_EPOCH);
@@ -732,10 +754,10 @@ __global__ void __launch_bounds__(256) k_td_a_solve(TDSolveArgs<R> T) {
         A.src = pick3(T.Q, (j % 2 == 0) ? in : (j == 1 ? qc : out));
         A.rhs = (T.rhs_ab && j > 0) ? T.rhs_ab : pick3(T.Q, qc);
         A.rhs_out = T.rhs_ab;
-        double rr = 0.0;
+        R rr = R(0);
         for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, a_rows<R, GAUGE>(A, s, ja, jb)); });
         int r = 0;
-        if (!sweep_sync(T.sc, scaled_residual(rr, A.stop_eps), 0, epoch, lds, r)) {
+        if (!sweep_sync(T.sc, scaled_residual<R>(rr, A.stop_eps), 0, epoch, lds, r)) {
             if (blockIdx.x == 0 && threadIdx.x == 0) st[ST_ERRSTEP] = T.step + 1;
             return;
         }

@@ -101,13 +101,16 @@ class TD:
         return bufs, idx
 
     def _default_wg(self):
+        """Persistent grid: two workgroups per CU (the measured optimum at
+        4097^2), fewer for small grids (about 32 rows per wave)."""
         cfg = self.cfg
         strips = -(-int(cfg.Nx) // 62)
         rows = strips * int(cfg.Ny)
-        waves = max(1, rows // 8)
+        waves = max(1, rows // 32)
         maxg = ctypes.c_int32()
         nat.call("glx_td_max_grid", self.par.prec, 256, ctypes.byref(maxg))
-        return int(max(1, min(maxg.value, -(-waves // 4))))
+        cus = nat.device_info()["num_cus"]
+        return int(max(1, min(maxg.value, 2 * cus, -(-waves // 4))))
 
     def _launch_steps(self):
         if self.steps_per_launch:
