@@ -219,6 +219,14 @@ int glx_cg_coef(int prec, glx_grid g, double kappa2, double eps, double H, const
  * axmy_r/axmy_c, svirl/cuda/utils.h:73-114; a complex array is 2N reals). */
 int glx_axpy(int prec, const void *x, const void *y, void *z, double alpha, int64_t n);
 int glx_axmy(int prec, const void *x, const void *y, void *z, double alpha, int64_t n);
+/* Polak-Ribiere beta left in device memory, as the reference keeps it
+ * (__compute_beta_psi/_A, cg.py:422-449; divide_scalars_positive,
+ * utils.h:139-146): *beta_dev = max(sum x*(x-y) / sum y*y, 0), sums in FP64,
+ * the quotient rounded to the working precision and stored as a double. */
+int glx_cg_pr_beta(int prec, const void *jac, const void *jac_prev, int64_t n, double *beta_dev);
+/* z = alpha*x - y with alpha read from device memory (the reference's
+ * axmy_c/axmy_r called with the device beta, cg.py:283-290). */
+int glx_axmy_dev(int prec, const void *x, const void *y, void *z, const double *alpha_dev, int64_t n);
 
 #ifdef __cplusplus
 }

@@ -82,6 +82,8 @@ _PROTOS = {
                     c_void_p, P_double],
     "glx_axpy": [c_int, c_void_p, c_void_p, c_void_p, c_double, c_i64],
     "glx_axmy": [c_int, c_void_p, c_void_p, c_void_p, c_double, c_i64],
+    "glx_cg_pr_beta": [c_int, c_void_p, c_void_p, c_i64, c_void_p],
+    "glx_axmy_dev": [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_i64],
 }
 
 EXPORTS = tuple(_PROTOS) + ("glx_last_error",)

@@ -308,6 +308,14 @@ int glx_cg_coef(int prec, glx_grid g, double kappa2, double eps, double H, const
     GLX_DISPATCH(prec, coef(g_ctx.env, g, kappa2, eps, H, mt, psi, dpsi, abei, ab, dab, c17));
 }
 
+int glx_cg_pr_beta(int prec, const void *jac, const void *jac_prev, int64_t n, double *beta_dev) {
+    GLX_DISPATCH(prec, pr_beta(g_ctx.env, jac, jac_prev, n, beta_dev));
+}
+
+int glx_axmy_dev(int prec, const void *x, const void *y, void *z, const double *alpha_dev, int64_t n) {
+    GLX_DISPATCH(prec, axmy_dev(g_ctx.env, x, y, z, alpha_dev, n));
+}
+
 int glx_axpy(int prec, const void *x, const void *y, void *z, double alpha, int64_t n) {
     GLX_DISPATCH(prec, axpy(g_ctx.env, x, y, z, alpha, n, 0));
 }

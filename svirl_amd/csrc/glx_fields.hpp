@@ -258,6 +258,29 @@ __global__ void k_pr_sums(const R *x, const R *y, long n, double *partials) {
     block_store_partials<2>(acc, partials);
 }
 
+// Polak-Ribiere beta from the two partial sums of k_pr_sums, left in device
+// memory (cg.py:422-449, utils.h:139-146): beta = max(num/den, 0) in FP64,
+// rounded to the working precision as the reference's real_t scalar.
+template <typename R>
+__global__ void k_pr_beta_finish(const double *partials, int nb, double *beta) {
+    __shared__ double lds[RED_BLOCK / WAVE];
+    double num = 0.0, den = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) num += partials[(long)b * 2];
+    num = block_sum(num, lds);
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) den += partials[(long)b * 2 + 1];
+    den = block_sum(den, lds);
+    if (threadIdx.x == 0) *beta = double(R(fmax(num / den, 0.0)));
+}
+
+// z = alpha * x - y with alpha read from device memory.
+template <typename R>
+__global__ void k_axmy_dev(const R *x, const R *y, R *z, const double *alpha, long n) {
+    const R a = R(*alpha);
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
+        z[k] = a * x[k] - y[k];
+}
+
 template <typename R>
 __global__ void k_axpy(const R *x, const R *y, R *z, R alpha, long n, int minus) {
     for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)

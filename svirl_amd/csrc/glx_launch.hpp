@@ -66,6 +66,8 @@ template <typename R> struct L {
                         const void *psi, const void *dpsi, const void *abei, const void *ab, double *c5);
     static int coef(const Env &e, glx_grid g, double kappa2, double eps, double H, const uint8_t *mt, const void *psi,
                     const void *dpsi, const void *abei, const void *ab, const void *dab, double *c17);
+    static int pr_beta(const Env &e, const void *x, const void *y, int64_t n, double *beta_dev);
+    static int axmy_dev(const Env &e, const void *x, const void *y, void *z, const double *alpha_dev, int64_t n);
     static int axpy(const Env &e, const void *x, const void *y, void *z, double alpha, int64_t n, int minus);
 };
 

@@ -122,50 +122,47 @@ int L<R>::a_sweep(const Env &e, glx_grid g, double dt, double kappa2, double rho
     return GLX_OK;
 }
 
-// The FP32-mixed run stores A as A - A_gauge(H) (exact change of variable:
-// the symmetric gauge is a fixed point of the A sweep with psi = 0), so that
-// the iterates stay small and float-resolved; FP64 runs keep the plain form.
-template <typename R> struct RunForm { static constexpr bool gauge = false; };
-template <> struct RunForm<float> { static constexpr bool gauge = true; };
+// The FP32-mixed run stores A as A - A_gauge(H) while it evolves A (exact
+// change of variable: the symmetric gauge is a fixed point of the A sweep
+// with psi = 0), so that the iterates stay small and float-resolved. FP64
+// runs, and FP32 runs that do not evolve A, keep the plain form (A is then
+// left bit-for-bit untouched).
+template <typename R> struct RunForm { static constexpr bool has_gauge = false; };
+template <> struct RunForm<float> { static constexpr bool has_gauge = true; };
+
+template <typename R, bool G> static int solve_occupancy(int *per_cu) {
+    int a = 0, b = 0;
+    hipError_t st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_td_psi_solve<R, G>, 256, 0);
+    if (st == hipSuccess) st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_td_a_solve<R, G>, 256, 0);
+    if (st != hipSuccess) return check_hip(st, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    *per_cu = a < b ? a : b;
+    return GLX_OK;
+}
 
 template <typename R> int L<R>::td_max_grid(const Env &e, int32_t wg_size, int32_t *max_wg) {
     if (wg_size != 256 || !max_wg) return set_error(GLX_E_ARG, "glx_td_max_grid: wg_size must be 256");
-    constexpr bool G = RunForm<R>::gauge;
-    static int per_cu = 0;  // co-resident workgroups per CU of both solve kernels (queried once)
+    static int per_cu = 0;  // co-resident workgroups per CU of every solve kernel (queried once)
     if (per_cu <= 0) {
-        int a = 0, b = 0;
-        hipError_t st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_td_psi_solve<R, G>, 256, 0);
-        if (st == hipSuccess) st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_td_a_solve<R, G>, 256, 0);
-        if (st != hipSuccess) return check_hip(st, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-        per_cu = a < b ? a : b;
+        int m = 0;
+        int rc = solve_occupancy<R, false>(&m);
+        if (rc) return rc;
+        if constexpr (RunForm<R>::has_gauge) {
+            int mg = 0;
+            rc = solve_occupancy<R, true>(&mg);
+            if (rc) return rc;
+            if (mg < m) m = mg;
+        }
+        per_cu = m;
     }
     int m = per_cu * e.num_cus;
     *max_wg = m < MAX_GRID_BLOCKS ? m : MAX_GRID_BLOCKS;
     return GLX_OK;
 }
 
-template <typename R>
-int L<R>::td_run(const Env &e, glx_grid g, const glx_td_params *p, const void *eps_map, const uint8_t *mt,
-                 void *const P[3], int32_t *pc, void *const Q[3], int32_t *qc, void *rhs_psi, void *rhs_ab,
-                 int32_t nsteps, int32_t *sweeps, int32_t wg, int32_t wg_size) {
-    if (!grid_ok(g) || !p || !P || !pc || !sweeps || nsteps < 0 || *pc < 0 || *pc > 2)
-        return set_error(GLX_E_ARG, "glx_td_run: invalid arguments");
-    if (!P[0] || !P[1] || !P[2]) return set_error(GLX_E_ARG, "glx_td_run: psi buffers");
-    if (!Q || !qc || *qc < 0 || *qc > 2 || !Q[0] || !Q[1] || !Q[2])
-        return set_error(GLX_E_ARG, "glx_td_run: vector potential buffers");
-    if (p->max_sweeps < 1 || p->stop_psi <= 0.0 || p->stop_A <= 0.0 || p->eqn < 0 || p->eqn > 2)
-        return set_error(GLX_E_ARG, "glx_td_run: invalid parameters");
-    if (p->langevin_psi > 1.0e-32 && !rhs_psi) return set_error(GLX_E_ARG, "glx_td_run: Langevin psi needs rhs_psi");
-    if (p->solveA && p->langevin_A > 1.0e-32 && !rhs_ab) return set_error(GLX_E_ARG, "glx_td_run: Langevin A needs rhs_ab");
-    if (nsteps == 0) return GLX_OK;
-    if (nsteps > e.sweeps_cap) return set_error(GLX_E_ARG, "glx_td_run: nsteps %d exceeds %d per launch", nsteps, e.sweeps_cap);
-    if (wg_size == 0) wg_size = 256;
-    int32_t maxg = 0;
-    int rc = td_max_grid(e, wg_size, &maxg);
-    if (rc) return rc;
-    if (wg <= 0) wg = maxg;
-    if (wg > maxg) return set_error(GLX_E_ARG, "glx_td_run: grid %d exceeds co-resident limit %d", wg, maxg);
-    constexpr bool G = RunForm<R>::gauge;
+template <typename R, bool G>
+static int td_run_form(const Env &e, glx_grid g, const glx_td_params *p, const void *eps_map, const uint8_t *mt,
+                       void *const P[3], int32_t *pc, void *const Q[3], int32_t *qc, void *rhs_psi, void *rhs_ab,
+                       int32_t nsteps, int32_t *sweeps, int32_t wg) {
 
     TDSolveArgs<R> T{};
     T.psi = make_psi_args<R>(g, p->dt, p->eps, eps_map, mt, nullptr, p->stop_psi, p->langevin_psi);
@@ -263,6 +260,35 @@ int L<R>::td_run(const Env &e, glx_grid g, const glx_td_params *p, const void *e
     *pc = host_state[ST_PSI];
     *qc = host_state[ST_AB];
     return GLX_OK;
+}
+
+
+template <typename R>
+int L<R>::td_run(const Env &e, glx_grid g, const glx_td_params *p, const void *eps_map, const uint8_t *mt,
+                 void *const P[3], int32_t *pc, void *const Q[3], int32_t *qc, void *rhs_psi, void *rhs_ab,
+                 int32_t nsteps, int32_t *sweeps, int32_t wg, int32_t wg_size) {
+    if (!grid_ok(g) || !p || !P || !pc || !sweeps || nsteps < 0 || *pc < 0 || *pc > 2)
+        return set_error(GLX_E_ARG, "glx_td_run: invalid arguments");
+    if (!P[0] || !P[1] || !P[2]) return set_error(GLX_E_ARG, "glx_td_run: psi buffers");
+    if (!Q || !qc || *qc < 0 || *qc > 2 || !Q[0] || !Q[1] || !Q[2])
+        return set_error(GLX_E_ARG, "glx_td_run: vector potential buffers");
+    if (p->max_sweeps < 1 || p->stop_psi <= 0.0 || p->stop_A <= 0.0 || p->eqn < 0 || p->eqn > 2)
+        return set_error(GLX_E_ARG, "glx_td_run: invalid parameters");
+    if (p->langevin_psi > 1.0e-32 && !rhs_psi) return set_error(GLX_E_ARG, "glx_td_run: Langevin psi needs rhs_psi");
+    if (p->solveA && p->langevin_A > 1.0e-32 && !rhs_ab) return set_error(GLX_E_ARG, "glx_td_run: Langevin A needs rhs_ab");
+    if (nsteps == 0) return GLX_OK;
+    if (nsteps > e.sweeps_cap) return set_error(GLX_E_ARG, "glx_td_run: nsteps %d exceeds %d per launch", nsteps, e.sweeps_cap);
+    if (wg_size == 0) wg_size = 256;
+    int32_t maxg = 0;
+    int rc = td_max_grid(e, wg_size, &maxg);
+    if (rc) return rc;
+    if (wg <= 0) wg = maxg;
+    if (wg > maxg) return set_error(GLX_E_ARG, "glx_td_run: grid %d exceeds co-resident limit %d", wg, maxg);
+    const bool evolves_A = p->solveA && p->eqn != 1;
+    if constexpr (RunForm<R>::has_gauge)
+        if (evolves_A)
+            return td_run_form<R, true>(e, g, p, eps_map, mt, P, pc, Q, qc, rhs_psi, rhs_ab, nsteps, sweeps, wg);
+    return td_run_form<R, false>(e, g, p, eps_map, mt, P, pc, Q, qc, rhs_psi, rhs_ab, nsteps, sweeps, wg);
 }
 
 template <typename R> int L<R>::phase_lock(const Env &e, void *psi, int32_t n, const int32_t *ns) {
@@ -426,6 +452,26 @@ int L<R>::coef(const Env &e, glx_grid g, double kappa2, double eps, double H, co
     hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, 17, dxdy, e.result);
     GLX_LAUNCH_CHECK("k_finish_partials");
     return fetch_result(e, c17, 17);
+}
+
+template <typename R> int L<R>::pr_beta(const Env &e, const void *x, const void *y, int64_t n, double *beta_dev) {
+    if (!x || !y || !beta_dev || n < 0) return set_error(GLX_E_ARG, "glx_cg_pr_beta: invalid arguments");
+    const int nb = grid_blocks(n, RED_BLOCK, RED_MAX_BLOCKS);
+    hipLaunchKernelGGL(k_pr_sums<R>, dim3(nb), dim3(RED_BLOCK), 0, e.stream, (const R *)x, (const R *)y, (long)n, e.partials);
+    GLX_LAUNCH_CHECK("k_pr_sums");
+    hipLaunchKernelGGL(k_pr_beta_finish<R>, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, beta_dev);
+    GLX_LAUNCH_CHECK("k_pr_beta_finish");
+    return GLX_OK;
+}
+
+template <typename R>
+int L<R>::axmy_dev(const Env &e, const void *x, const void *y, void *z, const double *alpha_dev, int64_t n) {
+    if (!x || !y || !z || !alpha_dev || n < 0) return set_error(GLX_E_ARG, "glx_axmy_dev: invalid arguments");
+    if (n == 0) return GLX_OK;
+    hipLaunchKernelGGL(k_axmy_dev<R>, dim3(grid_blocks(n)), dim3(256), 0, e.stream, (const R *)x, (const R *)y, (R *)z,
+                       alpha_dev, (long)n);
+    GLX_LAUNCH_CHECK("k_axmy_dev");
+    return GLX_OK;
 }
 
 template <typename R>

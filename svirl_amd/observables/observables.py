@@ -20,15 +20,15 @@ class Observables:
         return nat.Grid(int(c.Nx), int(c.Ny), float(c.dx), float(c.dy))
 
     def _sync_device(self):
-        self.vars._psi.sync()
-        self.vars._vp.sync()
+        self.vars._psi.sync_d()
+        self.vars._vp.sync_d()
         if self.params._vpei is not None:
-            self.params._vpei.sync()
+            self.params._vpei.sync_d()
 
     @property
     def superfluid_density(self):
         """|psi|^2 on nodes (observables.py:29-33)."""
-        self.vars._psi.sync()
+        self.vars._psi.sync_d()
         psi = self.vars._psi.get_h()
         return np.square(psi.real) + np.square(psi.imag)
 

@@ -63,8 +63,8 @@ class VortexDetector:
         """Flagged cells from the GPU winding kernel, ascending n."""
         c = self.cfg
         g = nat.Grid(int(c.Nx), int(c.Ny), float(c.dx), float(c.dy))
-        self.vars._psi.sync()
-        self.vars._vp.sync()
+        self.vars._psi.sync_d()
+        self.vars._vp.sync_d()
         abi = self.fixed_vortices.irregular_vector_potential_h()
         while True:
             if self._rec is None or self._rec.size < self._capacity * _VREC:

@@ -33,8 +33,8 @@ class CG:
         self.__gdir_psi = DeviceArray.zeros(cfg.N, cd)
         self.__gjac_psi = DeviceArray.zeros(cfg.N, cd)
         self.__gjac_psi_prev = DeviceArray.zeros(cfg.N, cd)
-        self._beta_psi = cfg.dtype(0.0)
-        self._beta_A = cfg.dtype(0.0)
+        # beta stays in device memory (one FP64 slot each), as in the reference
+        self.__gbeta = DeviceArray.zeros(2, np.float64)
         if self.params.solveA:
             self.__gdir_A = DeviceArray.zeros(cfg.Nab, cfg.dtype)
             self.__gjac_A = DeviceArray.zeros(cfg.Nab, cfg.dtype)
@@ -62,8 +62,8 @@ class CG:
     def _free_energy_jacobian_psi(self):
         """dF/d(Re psi, Im psi) * dxdy as a flat complex device array
         (cg.py:112-146)."""
-        self.vars._psi.sync()
-        self.vars._vp.sync()
+        self.vars._psi.sync_d()
+        self.vars._vp.sync_d()
         a = self._args()
         self.__gjac_psi.fill(0)
         nat.call("glx_cg_jacobian_psi", self.par.prec, self._grid(), a['k2'], a['eps'], a['epsmap'], a['H'], a['mt'],
@@ -76,8 +76,8 @@ class CG:
         part) as a flat real device array [a; b] (cg.py:149-181)."""
         if not self.params.solveA:
             return None
-        self.vars._psi.sync()
-        self.vars._vp.sync()
+        self.vars._psi.sync_d()
+        self.vars._vp.sync_d()
         a = self._args()
         self.__gjac_A.fill(0)
         nat.call("glx_cg_jacobian_A", self.par.prec, self._grid(), a['k2'], a['H'], a['mt'], a['psi'], a['abei'],
@@ -86,8 +86,8 @@ class CG:
 
     def _free_energy_conjgrad_coef_psi(self, gdir_psi):
         """c0..c4 of F(psi + alpha dpsi) (cg.py:184-224)."""
-        self.vars._psi.sync()
-        self.vars._vp.sync()
+        self.vars._psi.sync_d()
+        self.vars._vp.sync_d()
         a = self._args()
         out = (ctypes.c_double * 5)()
         nat.call("glx_cg_coef_psi", self.par.prec, self._grid(), a['k2'], a['eps'], a['H'], a['mt'], a['psi'],
@@ -107,8 +107,8 @@ class CG:
     def _free_energy_conjgrad_coef(self, gdir_psi, gdir_A):
         """The 17 coefficients packed into a 5x5 polyval2d matrix
         (cg.py:325-375)."""
-        self.vars._psi.sync()
-        self.vars._vp.sync()
+        self.vars._psi.sync_d()
+        self.vars._vp.sync_d()
         a = self._args()
         out = (ctypes.c_double * 17)()
         nat.call("glx_cg_coef", self.par.prec, self._grid(), a['k2'], a['eps'], a['H'], a['mt'], a['psi'],
@@ -140,20 +140,29 @@ class CG:
         return r.x
 
     # -- helpers ----------------------------------------------------------------------------
-    def _beta(self, jac, jac_prev):
-        """max(sum g(g - gp) / sum gp^2, 0) with FP64 sums (cg.py:422-449)."""
-        out = (ctypes.c_double * 2)()
-        n = jac.size * (2 if jac.dtype.kind == 'c' else 1)
-        nat.call("glx_pr_sums", self.par.prec, jac.ptr, jac_prev.ptr, n, out)
-        with np.errstate(divide='ignore', invalid='ignore'):
-            q = np.float64(out[0]) / np.float64(out[1])
-        return self.cfg.dtype(np.fmax(q, 0.0))
+    @property
+    def _beta_psi(self):
+        """Polak-Ribiere beta of the psi direction (device scalar, read back)."""
+        return self.cfg.dtype(self.__gbeta.get()[0])
+
+    @property
+    def _beta_A(self):
+        return self.cfg.dtype(self.__gbeta.get()[1])
+
+    def _beta_ptr(self, k):
+        return ctypes.c_void_p(self.__gbeta.ptr + 8 * k)
+
+    def _beta(self, jac, jac_prev, k):
+        """beta_k = max(sum g(g - gp) / sum gp^2, 0), FP64 sums, left on the
+        device (cg.py:422-449)."""
+        nat.call("glx_cg_pr_beta", self.par.prec, jac.ptr, jac_prev.ptr, self._reals(jac), self._beta_ptr(k))
+
+    def _axmy_beta(self, x, y, z, k):
+        """z = beta_k x - y with the device beta (cg.py:283-290)."""
+        nat.call("glx_axmy_dev", self.par.prec, x.ptr, y.ptr, z.ptr, self._beta_ptr(k), self._reals(x))
 
     def _reals(self, arr):
         return arr.size * (2 if arr.dtype.kind == 'c' else 1)
-
-    def _axmy(self, x, y, z, alpha):
-        nat.call("glx_axmy", self.par.prec, x.ptr, y.ptr, z.ptr, float(alpha), self._reals(x))
 
     def _axpy(self, x, y, z, alpha):
         nat.call("glx_axpy", self.par.prec, x.ptr, y.ptr, z.ptr, float(alpha), self._reals(x))
@@ -162,15 +171,15 @@ class CG:
     def __minimization_psi(self, n_iter=1000):
         """cg.py:238-322 (kappa = inf)."""
         assert not self.params.solveA
-        self.vars._psi.sync()
-        self.vars._vp.sync()
+        self.vars._psi.sync_d()
+        self.vars._vp.sync_d()
         self.cg_energies = []
         self.__gdir_psi.fill(0)
         for i in range(n_iter):
             jac = self._free_energy_jacobian_psi
             if i > 0:
-                self._beta_psi = self._beta(jac, self.__gjac_psi_prev)
-            self._axmy(self.__gdir_psi, jac, self.__gdir_psi, self._beta_psi)
+                self._beta(jac, self.__gjac_psi_prev, 0)
+            self._axmy_beta(self.__gdir_psi, jac, self.__gdir_psi, 0)
             self._free_energy_conjgrad_coef_psi(self.__gdir_psi)
             alpha0 = self._cg_alpha_psi_min()
             psi = self.vars.order_parameter_h()
@@ -184,17 +193,17 @@ class CG:
 
     def __minimization(self, n_iter=1000):
         """cg.py:452-551 (kappa < inf)."""
-        self.vars._psi.sync()
-        self.vars._vp.sync()
+        self.vars._psi.sync_d()
+        self.vars._vp.sync_d()
         self.cg_energies = []
         for i in range(n_iter):
             jac = self._free_energy_jacobian_psi
             jacA = self._free_energy_jacobian_A
             if i > 0:
-                self._beta_psi = self._beta(jac, self.__gjac_psi_prev)
-                self._beta_A = self._beta(jacA, self.__gjac_A_prev)
-            self._axmy(self.__gdir_psi, jac, self.__gdir_psi, self._beta_psi)
-            self._axmy(self.__gdir_A, jacA, self.__gdir_A, self._beta_A)
+                self._beta(jac, self.__gjac_psi_prev, 0)
+                self._beta(jacA, self.__gjac_A_prev, 1)
+            self._axmy_beta(self.__gdir_psi, jac, self.__gdir_psi, 0)
+            self._axmy_beta(self.__gdir_A, jacA, self.__gdir_A, 1)
             self._free_energy_conjgrad_coef(self.__gdir_psi, self.__gdir_A)
             alpha_psi, alpha_A = self._cg_alpha_min()
             psi = self.vars.order_parameter_h()

@@ -125,8 +125,8 @@ class TD:
             raise NotImplementedError("fixed vortices / phase lock are not supported by the GPU TD path")
         cfg = self.cfg
         v = self.vars
-        v._psi.sync()
-        v._vp.sync()
+        v._psi.sync_d()
+        v._vp.sync_d()
         P, pc = self._rotation(v._psi, '_P')
         Q, qc = self._rotation(v._vp, '_Q')
         p = self.params

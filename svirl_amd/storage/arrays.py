@@ -106,6 +106,14 @@ class GArray:
             self._gdata.set(self._data)
         self._sync_status = 0
 
+    def sync_d(self):
+        """Make the device copy current (host-to-device copy if the host is
+        newer); a newer device copy stays where it is. For kernels that only
+        read the device copy."""
+        if self._on == self.on_host_and_device and self._sync_status < 0:
+            self._gdata.set(self._data)
+            self._sync_status = 0
+
     def need_dtoh_sync(self):
         if self._on == self.on_host_and_device:
             self._sync_status = 1

@@ -202,6 +202,17 @@ def test_td_run_fp32_mixed(shape, kappa, mask, epsmap, nsteps):
     assert np.max(np.abs(a - st.ab)) <= 2e-5
 
 
+def test_td_run_fp32_psi_only_leaves_A_untouched():
+    """kappa = inf (no A solve): the FP32 run uses the plain form, so the
+    vector potential comes back bit for bit."""
+    rng, g, mt, psi, ab, em = case(5, 70, 41, mask=False, epsmap=False)
+    ab32 = O.symmetric_gauge(g, 0.25).astype(np.float32).astype(np.float64)
+    prm = _td_params(0.1, 1.0, -1.0, 1.0, 0.25, 1e-6, 1e-6, False)
+    p, a, sweeps = _td_run_f32(g, psi.astype(np.complex64).astype(np.complex128), ab32, None, None, prm, 3)
+    assert np.array_equal(a, ab32)
+    assert np.all(sweeps[:, 1] == 0) and np.all(sweeps[:, 0] > 0)
+
+
 def test_td_run_fp32_large_potential():
     """|A| ~ 26 at the edges (float ulp 1.9e-6, above the 1e-6 stop
     threshold): the FP32-mixed run still converges in a few sweeps per step
@@ -312,6 +323,31 @@ def test_reductions(n):
     dv = D(v, np.float64)
     nat.call("glx_sum_v", nat.GLX_F64, dv.ptr, nv, 5, 0, o5)
     assert np.allclose(np.array(o5[:]), v.reshape(nv, 5).sum(axis=0), atol=1e-10, rtol=1e-13)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("n", [1, 1000, 300001])
+def test_pr_beta_device(prec, n):
+    """glx_cg_pr_beta leaves max(sum g(g-gp)/sum gp^2, 0) in device memory
+    (the oracle's pr_beta, FP64 sums, rounded to the working precision);
+    glx_axmy_dev applies it as the reference's axmy does; 0/0 gives 0."""
+    rs, _ = _types(prec)
+    rng = np.random.default_rng(n)
+    g1 = rng.normal(size=n).astype(rs)
+    g0 = (g1 + 0.3 * rng.normal(size=n)).astype(rs)
+    d1, d0 = D(g1, rs), D(g0, rs)
+    beta = nat.DeviceArray.zeros(1, np.float64)
+    nat.call("glx_cg_pr_beta", prec, d1.ptr, d0.ptr, n, beta.ptr)
+    want = float(rs(O.pr_beta(g1.astype(np.float64), g0.astype(np.float64))))
+    got = float(beta.get()[0])
+    assert got == pytest.approx(want, rel=1e-12 if prec == nat.GLX_F64 else 1e-6)
+    dirv = rng.normal(size=n).astype(rs)
+    dd = D(dirv, rs)
+    nat.call("glx_axmy_dev", prec, dd.ptr, d1.ptr, dd.ptr, beta.ptr, n)
+    assert np.allclose(dd.get(), rs(got) * dirv - g1, rtol=1e-6, atol=1e-6)
+    z = nat.DeviceArray.zeros(n, rs)
+    nat.call("glx_cg_pr_beta", prec, z.ptr, z.ptr, n, beta.ptr)
+    assert beta.get()[0] == 0.0
 
 
 def test_vortex_cells_match_oracle():
