@@ -135,6 +135,37 @@ int glx_td_run(int prec, glx_grid g, const glx_td_params *p, const void *eps_map
                void *const ab_bufs[3], int32_t *ab_cur, void *rhs_psi, void *rhs_ab,
                int32_t nsteps, int32_t *sweeps, int32_t wg, int32_t wg_size);
 
+/* ------------------------------------------------------------ batched TD */
+
+/* One instance of a batched TD run (no counterpart in the reference, which
+ * runs one GLSolver per process; this is the batched form of glx_td_run).
+ * Each instance has its own grid, parameters, epsilon map, mask and buffer
+ * triplets; psi[psi_cur] / ab[ab_cur] hold its state on entry and exit.
+ * Langevin noise and eqn modes other than 0 are not supported here. */
+typedef struct glx_batch_instance {
+    glx_grid g;
+    glx_td_params p;
+    const void *eps_map;
+    const uint8_t *mt;
+    void *psi[3];
+    void *ab[3];
+    int32_t psi_cur;
+    int32_t ab_cur;
+} glx_batch_instance;
+
+/* Advance n instances by nsteps TD steps. Every step is ONE persistent
+ * launch in which each instance is swept by its own team of co-resident
+ * workgroups (sized to its grid) with team-local barriers, so instances
+ * stop their Jacobi solves independently (ragged sweep counts). With
+ * use_graph != 0, the step launches are captured once into a HIP graph and
+ * replayed. wg_per_cu (co-resident workgroups per CU, 0 = as many as fit)
+ * and min_rows (fewest rows per wave when sizing teams, 0 = 16) are the
+ * tunable launch shape. sweeps (host, nsteps*n*2 int32, may be NULL)
+ * receives (S_psi, S_A) per step and instance. Instances must not share
+ * buffers; n must not exceed the number of co-resident workgroups. */
+int glx_batch_td_run(int prec, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
+                     int32_t use_graph, int32_t wg_per_cu, int32_t min_rows);
+
 /* Duration (ms, HIP events on the library stream) of the persistent kernel
  * of the last glx_td_run call, excluding its reset and read-back. */
 int glx_td_last_kernel_ms(float *ms);

@@ -40,6 +40,12 @@ class TDParams(ctypes.Structure):
     ]
 
 
+class BatchInstance(ctypes.Structure):
+    """glx_batch_instance."""
+    _fields_ = [("g", Grid), ("p", TDParams), ("eps_map", c_void_p), ("mt", c_void_p),
+                ("psi", c_void_p * 3), ("ab", c_void_p * 3), ("psi_cur", c_i32), ("ab_cur", c_i32)]
+
+
 # name -> (argtypes); every function returns int
 _PROTOS = {
     "glx_init": [c_int],
@@ -63,6 +69,7 @@ _PROTOS = {
     "glx_td_max_grid": [c_int, c_i32, P_i32],
     "glx_td_last_kernel_ms": [ctypes.POINTER(ctypes.c_float)],
     "glx_td_profile": [c_i32],
+    "glx_batch_td_run": [c_int, ctypes.POINTER(BatchInstance), c_i32, c_i32, P_i32, c_i32, c_i32, c_i32],
     "glx_td_solve_ms": [ctypes.POINTER(c_double), P_i32, ctypes.POINTER(c_double), P_i32],
     "glx_magnetic_field": [c_int, Grid, c_void_p, c_void_p, c_void_p],
     "glx_current_density": [c_int, Grid, c_double, c_double, c_void_p, c_void_p, c_void_p],

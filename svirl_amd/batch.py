@@ -1,0 +1,120 @@
+"""Batched TDGL: many GLSolver instances advanced together.
+
+The reference runs one GLSolver per process, one time step at a time
+(svirl/solvers/td.py:342-366). Here a list of solvers of the same precision
+(each with its own grid, kappa, field, epsilon map and mask) advances by one
+persistent launch per time step (glx_batch_td_run): every instance is swept
+by its own team of workgroups with team-local barriers, so each instance
+keeps its own Jacobi sweep counts and stop rule, exactly as if it had been
+run alone. The step launches are replayed from a captured HIP graph.
+
+The launch shape (workgroups per CU, fewest rows per wave when sizing the
+teams) is picked by a small autotuner: the first steps of a run time each
+candidate shape on the real state (the results do not depend on the shape)
+and the rest of the run uses the fastest; the choice is cached per batch
+signature.
+
+    from svirl_amd import GLSolver
+    from svirl_amd.batch import BatchTD
+    sims = [GLSolver(...) for _ in range(256)]
+    BatchTD(sims).td(dt=0.1, Nt=5000)
+"""
+import ctypes
+
+import numpy as np
+
+from svirl_amd import _native as nat
+from svirl_amd._native import BatchInstance
+
+# (signature) -> (wg_per_cu, min_rows)
+_TUNED = {}
+
+CANDIDATES = [(0, 16), (2, 16), (0, 8), (0, 32), (1, 16)]
+
+
+class BatchTD:
+    """Advance a list of GLSolver objects together (FP32 or FP64, all the
+    same). use_graph: replay the step launches from a HIP graph.
+    autotune: pick the launch shape from CANDIDATES on the first steps."""
+
+    def __init__(self, solvers, use_graph=True, autotune=True, tune_steps=4):
+        if not solvers:
+            raise ValueError("BatchTD needs at least one solver")
+        dt0 = solvers[0].cfg.dtype
+        if any(s.cfg.dtype is not dt0 for s in solvers):
+            raise ValueError("all solvers of a batch must share one dtype")
+        self.solvers = list(solvers)
+        self.prec = nat.prec_of(dt0)
+        self.use_graph = bool(use_graph)
+        self.autotune = bool(autotune)
+        self.tune_steps = int(tune_steps)
+        self.shape = None
+        self.timings = {}
+        self.last_kernel_ms = 0.0
+
+    def _signature(self):
+        return tuple((int(s.cfg.Nx), int(s.cfg.Ny), bool(s.params.solveA)) for s in self.solvers) + (self.prec,)
+
+    def _tds(self, dt, Nt):
+        tds = []
+        for s in self.solvers:
+            s.solve._init_td()
+            td = s.solve._td
+            td._set_iterator_options('order_parameter', dt=dt, Nt=Nt)
+            td._set_iterator_options('vector_potential', dt=dt, Nt=Nt)
+            td._stability_warnings()
+            tds.append(td)
+        return tds
+
+    def _run(self, tds, nsteps, shape):
+        n = len(tds)
+        arr = (BatchInstance * n)()
+        bufs = []
+        for k, td in enumerate(tds):
+            g, prm, P, pc, Q, qc = td._prepare(0, td.dt)
+            if prm.langevin_psi > 1e-32 or prm.langevin_A > 1e-32:
+                raise NotImplementedError("Langevin noise is not supported in batched runs")
+            s = self.solvers[k]
+            I = arr[k]
+            I.g = g
+            I.p = prm
+            I.eps_map = nat.ptr(s.params.linear_coefficient_h())
+            I.mt = nat.ptr(s.mesh.material_tiling_h())
+            for j in range(3):
+                I.psi[j] = P[j].ptr
+                I.ab[j] = Q[j].ptr
+            I.psi_cur = pc
+            I.ab_cur = qc
+            bufs.append((P, Q))
+        sw = (ctypes.c_int32 * (2 * n * nsteps))()
+        nat.call("glx_batch_td_run", self.prec, arr, n, nsteps, sw, 1 if self.use_graph else 0, shape[0], shape[1])
+        kms = ctypes.c_float()
+        nat.call("glx_td_last_kernel_ms", ctypes.byref(kms))
+        self.last_kernel_ms = kms.value
+        sweeps = np.frombuffer(sw, dtype=np.int32).reshape(nsteps, n, 2)
+        for k, td in enumerate(tds):
+            P, Q = bufs[k]
+            td._finish(P, arr[k].psi_cur, Q, arr[k].ab_cur, sweeps[:, k, :], 0)
+        return kms.value
+
+    def td(self, dt=0.1, Nt=1000):
+        """Nt time steps of dt for every instance (td.py:342-366 per
+        instance)."""
+        tds = self._tds(dt, Nt)
+        done = 0
+        sig = self._signature()
+        if self.shape is None:
+            self.shape = _TUNED.get(sig)
+        if self.shape is None and self.autotune and Nt >= self.tune_steps * len(CANDIDATES) + 1:
+            for cand in CANDIDATES:
+                ms = self._run(tds, self.tune_steps, cand)
+                self.timings[cand] = ms / self.tune_steps
+                done += self.tune_steps
+            self.shape = min(self.timings, key=self.timings.get)
+            _TUNED[sig] = self.shape
+        if self.shape is None:
+            self.shape = CANDIDATES[0]
+        if Nt - done > 0:
+            self._run(tds, Nt - done, self.shape)
+        for s in self.solvers:
+            s.solve.vortices_detected = False

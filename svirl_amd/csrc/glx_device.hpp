@@ -228,31 +228,50 @@ __device__ __forceinline__ int ld_relaxed(const int *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Call from every thread of every workgroup the same number of times with the
-// same `epoch` (barriers already passed; incremented here). `v` is the
-// workgroup's value (thread 0's is used): two 16-bit fields. On success
-// returns true and sets `vmax` (all threads) to the field-wise max over the
-// grid; returns false when a poll timed out anywhere in the grid.
-__device__ __forceinline__ bool grid_sync_max(const GridSync &gs, unsigned long long &epoch, int v, int &vmax) {
+// A team: the workgroups [b0, b0 + nb) that sweep one grid together (the
+// whole launch for a single instance, a slice of it for one instance of a
+// batch). Its master is workgroup b0; its release words are rel[k *
+// REL_STRIDE], k < nrel.
+struct Team {
+    int b0, nb, nrel;
+    unsigned long long *rel;
+};
+
+// Call from every thread of every workgroup of the team the same number of
+// times with the same `epoch` (barriers the team has passed; incremented
+// here). `v` is the workgroup's value (thread 0's is used): two 16-bit
+// fields. On success returns true and sets `vmax` (all threads) to the
+// field-wise max over the team; returns false when a poll timed out anywhere
+// in the grid. A one-workgroup team synchronises in the workgroup only.
+__device__ __forceinline__ bool team_sync_max(const GridSync &gs, const Team &tm, unsigned long long &epoch, int v,
+                                              int &vmax) {
     __shared__ int s_red[16];
     __shared__ int s_ok, s_max;
+    ++epoch;
+    if (tm.nb == 1) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_max = v;
+        __syncthreads();
+        vmax = s_max;
+        return true;
+    }
     __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have left the wave
     __syncthreads();
-    ++epoch;
     const unsigned long long tag = epoch << 32;
+    const int local = (int)blockIdx.x - tm.b0;
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __hip_atomic_store(gs.arrive + blockIdx.x, tag | (unsigned)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (blockIdx.x == 0) {
+    if (local == 0) {
         const unsigned long long t0 = wall_clock64();
         int ok = 1, m = 0, mh = 0;
         for (;;) {
             int all = 1;
             m = 0;
             mh = 0;
-            for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x) {
-                const unsigned long long a = ld_relaxed(gs.arrive + k);
+            for (int k = threadIdx.x; k < tm.nb; k += blockDim.x) {
+                const unsigned long long a = ld_relaxed(gs.arrive + tm.b0 + k);
                 if ((a >> 32) != epoch) all = 0;
                 else {
                     m = max(m, (int)(unsigned)(a & 0xffffull));
@@ -270,9 +289,9 @@ __device__ __forceinline__ bool grid_sync_max(const GridSync &gs, unsigned long 
             mh = block_max_int(mh, s_red);
             if (threadIdx.x == 0) s_max = (mh << 16) | m;
             __syncthreads();
-            if (threadIdx.x < NREL) {
+            if (threadIdx.x < tm.nrel) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-                __hip_atomic_store(gs.rel + threadIdx.x * REL_STRIDE, tag | (unsigned)s_max, __ATOMIC_RELAXED,
+                __hip_atomic_store(tm.rel + threadIdx.x * REL_STRIDE, tag | (unsigned)s_max, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             }
         } else if (threadIdx.x == 0) {
@@ -280,7 +299,7 @@ __device__ __forceinline__ bool grid_sync_max(const GridSync &gs, unsigned long 
         }
         if (threadIdx.x == 0) s_ok = ok;
     } else if (threadIdx.x == 0) {
-        const unsigned long long *r = gs.rel + (blockIdx.x % NREL) * REL_STRIDE;
+        const unsigned long long *r = tm.rel + (local % tm.nrel) * REL_STRIDE;
         const unsigned long long t0 = wall_clock64();
         int ok = 1;
         unsigned long long a;
@@ -300,6 +319,12 @@ __device__ __forceinline__ bool grid_sync_max(const GridSync &gs, unsigned long 
     __syncthreads();
     vmax = s_max;
     return s_ok != 0;
+}
+
+// The whole launch as one team.
+__device__ __forceinline__ bool grid_sync_max(const GridSync &gs, unsigned long long &epoch, int v, int &vmax) {
+    const Team tm{0, (int)gridDim.x, NREL, gs.rel};
+    return team_sync_max(gs, tm, epoch, v, vmax);
 }
 
 }  // namespace glx

@@ -44,6 +44,8 @@ template <typename R> struct L {
     static int td_run(const Env &e, glx_grid g, const glx_td_params *p, const void *eps_map, const uint8_t *mt,
                       void *const P[3], int32_t *pc, void *const Q[3], int32_t *qc, void *rhs_psi, void *rhs_ab,
                       int32_t nsteps, int32_t *sweeps, int32_t wg, int32_t wg_size);
+    static int batch_td_run(const Env &e, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
+                            int32_t use_graph, int32_t wg_per_cu, int32_t min_rows);
     static int td_max_grid(const Env &e, int32_t wg_size, int32_t *max_wg);
     static int phase_lock(const Env &e, void *psi, int32_t n, const int32_t *ns);
     static int magnetic_field(const Env &e, glx_grid g, const void *abei, const void *ab, void *B);

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <algorithm>
+#include <vector>
 
 #include "glx_fields.hpp"
 #include "glx_launch.hpp"
@@ -289,6 +290,217 @@ int L<R>::td_run(const Env &e, glx_grid g, const glx_td_params *p, const void *e
         if (evolves_A)
             return td_run_form<R, true>(e, g, p, eps_map, mt, P, pc, Q, qc, rhs_psi, rhs_ab, nsteps, sweeps, wg);
     return td_run_form<R, false>(e, g, p, eps_map, mt, P, pc, Q, qc, rhs_psi, rhs_ab, nsteps, sweeps, wg);
+}
+
+// ---- batched TD ---------------------------------------------------------------
+
+// Per-instance solve arguments, as td_run_form builds them for one grid.
+template <typename R>
+static TDSolveArgs<R> batch_args(const glx_batch_instance &I, bool gauge_form) {
+    const glx_grid &g = I.g;
+    const glx_td_params *p = &I.p;
+    TDSolveArgs<R> T{};
+    T.psi = make_psi_args<R>(g, p->dt, p->eps, I.eps_map, I.mt, nullptr, p->stop_psi, 0.0);
+    T.A = make_a_args<R>(g, p->dt, p->kappa2, p->rho, p->H, I.mt, p->stop_A, 0.0);
+    const bool G = gauge_form && p->solveA;
+    const double gH = G ? p->H : 0.0;
+    T.psi.gH = T.A.gH = gH;
+    T.psi.gdx = T.A.gdx = g.dx;
+    T.psi.gdy = T.A.gdy = g.dy;
+    T.psi.gx0 = T.A.gx0 = 0.5 * g.dx * (g.Nx - 1);
+    T.psi.gy0 = T.A.gy0 = 0.5 * g.dy * (g.Ny - 1);
+    if (G) {
+        T.A.rh_a = R(0);
+        T.A.rh_b = R(0);
+    }
+    for (int k = 0; k < 3; ++k) { T.P[k] = (cplx<R> *)I.psi[k]; T.Q[k] = (R *)I.ab[k]; }
+    T.rhs_psi = nullptr;
+    T.rhs_ab = nullptr;
+    T.max_sweeps = p->max_sweeps;
+    T.solveA = p->solveA ? 1 : 0;
+    return T;
+}
+
+// Teams of consecutive workgroups, sized to each instance's wave-rows: the
+// largest scale f with sum_i clamp(floor(f*rows_i), 1, cap_i) <= G, where
+// cap_i keeps at least min_rows rows per wave.
+static void plan_teams(const std::vector<long> &rows, int G, int min_rows, std::vector<int> &nb) {
+    const size_t n = rows.size();
+    nb.assign(n, 1);
+    auto total = [&](double f, std::vector<int> *out) {
+        long t = 0;
+        for (size_t i = 0; i < n; ++i) {
+            const long cap = std::max(1L, rows[i] / (4L * min_rows));
+            long k = (long)(f * (double)rows[i]);
+            k = std::min(std::max(k, 1L), cap);
+            if (out) (*out)[i] = (int)k;
+            t += k;
+        }
+        return t;
+    };
+    double lo = 0.0, hi = 1.0;
+    while (total(hi, nullptr) <= G && hi < 1e12) hi *= 2.0;
+    for (int it = 0; it < 100; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (total(mid, nullptr) <= G) lo = mid;
+        else hi = mid;
+    }
+    total(lo, &nb);
+}
+
+template <typename R, bool G> static int batch_occupancy(int *per_cu) {
+    hipError_t st = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_batch_td_step<R, G>, 256, 0);
+    return st == hipSuccess ? GLX_OK : check_hip(st, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+}
+
+template <typename R, bool G>
+static int batch_run_form(const Env &e, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
+                          int32_t use_graph, int32_t wg_per_cu, int32_t min_rows) {
+    int per_cu = 0;
+    int rc = batch_occupancy<R, G>(&per_cu);
+    if (rc) return rc;
+    if (wg_per_cu > 0 && wg_per_cu < per_cu) per_cu = wg_per_cu;
+    const int Gb = std::min(per_cu * e.num_cus, MAX_GRID_BLOCKS);
+    if (n > Gb) return set_error(GLX_E_ARG, "glx_batch_td_run: %d instances exceed %d co-resident workgroups", n, Gb);
+    std::vector<long> rows(n);
+    for (int i = 0; i < n; ++i) rows[i] = (long)nstrips_of<1>(inst[i].g.Nx) * inst[i].g.Ny;
+    std::vector<int> nb;
+    plan_teams(rows, Gb, min_rows > 0 ? min_rows : 16, nb);
+    std::vector<int> team_of_block;
+    std::vector<int2> teams(n);
+    for (int i = 0; i < n; ++i) {
+        teams[i] = make_int2((int)team_of_block.size(), nb[i]);
+        for (int k = 0; k < nb[i]; ++k) team_of_block.push_back(i);
+    }
+    const int grid = (int)team_of_block.size();
+    std::vector<TDSolveArgs<R>> args(n);
+    std::vector<int> state(8 * (size_t)n, 0);
+    for (int i = 0; i < n; ++i) {
+        args[i] = batch_args<R>(inst[i], G);
+        state[8 * i + ST_PSI] = inst[i].psi_cur;
+        state[8 * i + ST_AB] = inst[i].ab_cur;
+    }
+    // one device block: args | team_of_block | teams | rel | state | sweeps
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t s_args = al(sizeof(TDSolveArgs<R>) * n), s_tob = al(sizeof(int) * grid), s_teams = al(sizeof(int2) * n);
+    const size_t s_rel = al(sizeof(unsigned long long) * REL_STRIDE * n), s_state = al(sizeof(int) * 8 * n);
+    const size_t s_sw = al(sizeof(int) * 2 * (size_t)n * nsteps);
+    char *blk = nullptr;
+    hipError_t st = hipMalloc((void **)&blk, s_args + s_tob + s_teams + s_rel + s_state + s_sw);
+    if (st != hipSuccess) return check_hip(st, "hipMalloc(batch)");
+    struct Free { char *p; ~Free() { if (p) (void)hipFree(p); } } guard{blk};
+    char *q = blk;
+    BatchArgs<R> B{};
+    B.args = (const TDSolveArgs<R> *)q; q += s_args;
+    B.team_of_block = (const int *)q; q += s_tob;
+    B.teams = (const int2 *)q; q += s_teams;
+    B.rel = (unsigned long long *)q; q += s_rel;
+    B.state = (int *)q; q += s_state;
+    B.sweeps = (int *)q;
+    B.gs.arrive = e.arrive;
+    B.gs.rel = e.rel;
+    B.gs.err = e.err;
+    B.n = n;
+    st = hipMemcpyAsync((void *)B.args, args.data(), sizeof(TDSolveArgs<R>) * n, hipMemcpyHostToDevice, e.stream);
+    if (st == hipSuccess)
+        st = hipMemcpyAsync((void *)B.team_of_block, team_of_block.data(), sizeof(int) * grid, hipMemcpyHostToDevice, e.stream);
+    if (st == hipSuccess) st = hipMemcpyAsync((void *)B.teams, teams.data(), sizeof(int2) * n, hipMemcpyHostToDevice, e.stream);
+    if (st == hipSuccess) st = hipMemcpyAsync(B.state, state.data(), sizeof(int) * 8 * n, hipMemcpyHostToDevice, e.stream);
+    if (st != hipSuccess) return check_hip(st, "glx_batch_td_run upload");
+    hipLaunchKernelGGL(k_batch_reset, dim3(std::max(1, grid_blocks(std::max(grid, n)))), dim3(256), 0, e.stream, e.arrive,
+                       grid, B.rel, n, e.err);
+    GLX_LAUNCH_CHECK("k_batch_reset");
+    if (G)
+        for (int i = 0; i < n; ++i) {
+            if (!inst[i].p.solveA) continue;
+            const glx_grid &g = inst[i].g;
+            const long Nab = (long)(g.Nx - 1) * g.Ny + (long)g.Nx * (g.Ny - 1);
+            hipLaunchKernelGGL(k_gauge_shift<R>, dim3(grid_blocks(Nab)), dim3(256), 0, e.stream,
+                               (R *)inst[i].ab[inst[i].ab_cur], g.Nx, g.Ny, inst[i].p.H, g.dx, g.dy,
+                               0.5 * g.dx * (g.Nx - 1), 0.5 * g.dy * (g.Ny - 1), -1.0);
+            GLX_LAUNCH_CHECK("k_gauge_shift");
+        }
+    st = hipEventRecord(e.kev0, e.stream);
+    if (st != hipSuccess) return check_hip(st, "hipEventRecord");
+    int launched = 0;
+    if (use_graph && nsteps > 1) {
+        const int K = std::min(nsteps, 64);
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        st = hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal);
+        if (st != hipSuccess) return check_hip(st, "hipStreamBeginCapture");
+        for (int k = 0; k < K; ++k) hipLaunchKernelGGL((k_batch_td_step<R, G>), dim3(grid), dim3(256), 0, e.stream, B);
+        st = hipStreamEndCapture(e.stream, &graph);
+        if (st != hipSuccess) return check_hip(st, "hipStreamEndCapture");
+        st = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (st != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return check_hip(st, "hipGraphInstantiate");
+        }
+        for (; launched + K <= nsteps && st == hipSuccess; launched += K) st = hipGraphLaunch(exec, e.stream);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        if (st != hipSuccess) return check_hip(st, "hipGraphLaunch");
+    }
+    for (; launched < nsteps; ++launched) {
+        hipLaunchKernelGGL((k_batch_td_step<R, G>), dim3(grid), dim3(256), 0, e.stream, B);
+        GLX_LAUNCH_CHECK("k_batch_td_step");
+    }
+    st = hipEventRecord(e.kev1, e.stream);
+    if (st != hipSuccess) return check_hip(st, "hipEventRecord");
+    if (G)
+        for (int i = 0; i < n; ++i) {
+            if (!inst[i].p.solveA) continue;
+            const glx_grid &g = inst[i].g;
+            const long Nab = (long)(g.Nx - 1) * g.Ny + (long)g.Nx * (g.Ny - 1);
+            hipLaunchKernelGGL(k_gauge_shift_state<R>, dim3(grid_blocks(Nab)), dim3(256), 0, e.stream, (R *)inst[i].ab[0],
+                               (R *)inst[i].ab[1], (R *)inst[i].ab[2], B.state + 8 * i, g.Nx, g.Ny, inst[i].p.H, g.dx,
+                               g.dy, 0.5 * g.dx * (g.Nx - 1), 0.5 * g.dy * (g.Ny - 1), 1.0);
+            GLX_LAUNCH_CHECK("k_gauge_shift_state");
+        }
+    int host_err = 0;
+    std::vector<int> sw_host(sweeps ? 2 * (size_t)n * nsteps : 0);
+    st = hipMemcpyAsync(state.data(), B.state, sizeof(int) * 8 * n, hipMemcpyDeviceToHost, e.stream);
+    if (st == hipSuccess) st = hipMemcpyAsync(&host_err, e.err, sizeof(int), hipMemcpyDeviceToHost, e.stream);
+    if (st == hipSuccess && sweeps)
+        st = hipMemcpyAsync(sw_host.data(), B.sweeps, sizeof(int) * sw_host.size(), hipMemcpyDeviceToHost, e.stream);
+    if (st == hipSuccess) st = hipStreamSynchronize(e.stream);
+    if (st != hipSuccess) return check_hip(st, "glx_batch_td_run readback");
+    float kms = 0.f;
+    if (hipEventElapsedTime(&kms, e.kev0, e.kev1) == hipSuccess && e.last_kernel_ms) *e.last_kernel_ms = kms;
+    if (host_err) return set_error(GLX_E_TIMEOUT, "glx_batch_td_run: a team barrier timed out");
+    for (int i = 0; i < n; ++i) {
+        inst[i].psi_cur = state[8 * i + ST_PSI];
+        inst[i].ab_cur = state[8 * i + ST_AB];
+    }
+    if (sweeps) std::copy(sw_host.begin(), sw_host.end(), sweeps);
+    return GLX_OK;
+}
+
+template <typename R>
+int L<R>::batch_td_run(const Env &e, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
+                       int32_t use_graph, int32_t wg_per_cu, int32_t min_rows) {
+    if (!inst || n < 1 || nsteps < 0) return set_error(GLX_E_ARG, "glx_batch_td_run: invalid arguments");
+    if ((long)n * nsteps > (1L << 26)) return set_error(GLX_E_ARG, "glx_batch_td_run: n*nsteps too large");
+    bool any_A = false;
+    for (int i = 0; i < n; ++i) {
+        const glx_batch_instance &I = inst[i];
+        const glx_td_params &p = I.p;
+        if (!grid_ok(I.g)) return set_error(GLX_E_ARG, "glx_batch_td_run: instance %d: invalid grid", i);
+        if (p.max_sweeps < 1 || p.stop_psi <= 0.0 || p.stop_A <= 0.0 || p.eqn != 0)
+            return set_error(GLX_E_ARG, "glx_batch_td_run: instance %d: invalid parameters", i);
+        if (p.langevin_psi > 1.0e-32 || p.langevin_A > 1.0e-32)
+            return set_error(GLX_E_ARG, "glx_batch_td_run: instance %d: Langevin noise is not supported in batches", i);
+        if (I.psi_cur < 0 || I.psi_cur > 2 || I.ab_cur < 0 || I.ab_cur > 2)
+            return set_error(GLX_E_ARG, "glx_batch_td_run: instance %d: buffer index", i);
+        for (int k = 0; k < 3; ++k)
+            if (!I.psi[k] || !I.ab[k]) return set_error(GLX_E_ARG, "glx_batch_td_run: instance %d: NULL buffer", i);
+        any_A = any_A || p.solveA;
+    }
+    if (nsteps == 0) return GLX_OK;
+    if constexpr (RunForm<R>::has_gauge)
+        if (any_A) return batch_run_form<R, true>(e, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows);
+    return batch_run_form<R, false>(e, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows);
 }
 
 template <typename R> int L<R>::phase_lock(const Env &e, void *psi, int32_t n, const int32_t *ns) {

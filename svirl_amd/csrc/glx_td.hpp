@@ -626,7 +626,7 @@ __global__ void __launch_bounds__(256) k_a_sweep(ASweepArgs<R> A, int *r2max, lo
 // current state and the barrier epoch carry over between launches in device
 // memory, so there is no host round trip per step.
 
-enum { ST_PSI = 0, ST_AB = 1, ST_EPOCH = 2, ST_ERRSTEP = 6, ST_PRED_PSI = 7 };
+enum { ST_PSI = 0, ST_AB = 1, ST_EPOCH = 2, ST_STEP = 4, ST_ERRSTEP = 6, ST_PRED_PSI = 7 };
 
 struct TDScratch {
     GridSync gs;
@@ -643,6 +643,7 @@ template <typename R> struct TDSolveArgs {
     R *rhs_ab;
     int max_sweeps;
     int step;               // index into sweeps[]
+    int solveA;             // batched runs: the instance has an A solve
     TDScratch sc;
 };
 
@@ -657,30 +658,32 @@ __device__ __forceinline__ unsigned long long *state64(int *state, int k) {
 }
 
 // The workgroup's largest scaled residuals of up to two sweeps (each clamped
-// to 0xffff: only the comparison with 10000 steers the solve), grid barrier,
-// grid-wide residuals r1 (low 16 bits of r_out) and r2 (high 16 bits).
-__device__ __forceinline__ bool sweep_sync(const TDScratch &sc, int r1, int r2, unsigned long long &epoch, int *lds,
-                                           int &r_out) {
+// to 0xffff: only the comparison with 10000 steers the solve), team barrier,
+// team-wide residuals r1 (low 16 bits of r_out) and r2 (high 16 bits).
+__device__ __forceinline__ bool sweep_sync(const GridSync &gs, const Team &tm, int r1, int r2,
+                                           unsigned long long &epoch, int *lds, int &r_out) {
     r1 = block_max_int(min(r1, 0xffff), lds);
     r2 = block_max_int(min(r2, 0xffff), lds);
-    return grid_sync_max(sc.gs, epoch, (r2 << 16) | r1, r_out);
+    return team_sync_max(gs, tm, epoch, (r2 << 16) | r1, r_out);
 }
 
-// The psi solve sweeps in fused pairs (psi_rows2: two sweeps per pass over
-// memory). A pair (j, j+1) reports both residuals; when sweep j already met
-// the stop rule, sweep j is recomputed alone from the pair's input, so the
-// result and the sweep count are those of sweep-by-sweep iteration. When the
-// previous step needed an odd number of sweeps, the solve opens with one
-// single sweep so that the pairs end where the solve is likely to end.
+// Result of one solve, identical in every workgroup of the team.
+struct SolveOut {
+    int cur;    // buffer index of the final iterate
+    int nsw;    // sweeps
+    bool ok;    // false: a barrier timed out
+};
+
+// The psi solve of one TD step on the team's waves (w of W): fused pairs
+// (psi_rows2: two sweeps per pass over memory). A pair (j, j+1) reports both
+// residuals; when sweep j already met the stop rule, sweep j is recomputed
+// alone from the pair's input, so the result and the sweep count are those
+// of sweep-by-sweep iteration. When the previous step needed an odd number
+// of sweeps (pred), the solve opens with one single sweep so that the pairs
+// end where the solve is likely to end.
 template <typename R, bool GAUGE>
-__global__ void __launch_bounds__(256) k_td_psi_solve(TDSolveArgs<R> T) {
-    __shared__ int lds[16];
-    const long W = (long)gridDim.x * (blockDim.x / WAVE);
-    const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
-    int *st = T.sc.state;
-    const int pc = st[ST_PSI], qc = st[ST_AB];
-    const int pred = st[ST_PRED_PSI];
-    unsigned long long epoch = *state64(st, ST_EPOCH);
+__device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const GridSync &gs, const Team &tm, long w,
+                                              long W, int pc, int qc, int pred, unsigned long long &epoch, int *lds) {
     const int o1 = (pc + 1) % 3, o2 = (pc + 2) % 3;
     PsiSweepArgs<R> A = T.psi;
     A.ab = pick3(T.Q, qc);
@@ -699,22 +702,16 @@ __global__ void __launch_bounds__(256) k_td_psi_solve(TDSolveArgs<R> T) {
         else
             for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { r1 = fmax(r1, psi_rows<R, GAUGE>(A, s, ja, jb)); });
         int r = 0;
-        if (!sweep_sync(T.sc, scaled_residual<R>(r1, A.stop_eps), scaled_residual<R>(r2, A.stop_eps), epoch, lds, r)) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) st[ST_ERRSTEP] = T.step + 1;
-            return;
-        }
+        if (!sweep_sync(gs, tm, scaled_residual<R>(r1, A.stop_eps), scaled_residual<R>(r2, A.stop_eps), epoch, lds, r))
+            return {in, nsw, false};
         const unsigned ru = (unsigned)r;
         const bool c1 = (ru & 0xffffu) < 10000u, c2 = (ru >> 16) < 10000u;
         if (pair && c1) {
             // sweep j met the stop rule: redo it alone into the same output
-            A.psi_next = pick3(T.P, out);
             R rr = R(0);
             for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, psi_rows<R, GAUGE>(A, s, ja, jb)); });
             int rdummy = 0;
-            if (!sweep_sync(T.sc, 0, 0, epoch, lds, rdummy)) {
-                if (blockIdx.x == 0 && threadIdx.x == 0) st[ST_ERRSTEP] = T.step + 1;
-                return;
-            }
+            if (!sweep_sync(gs, tm, 0, 0, epoch, lds, rdummy)) return {in, nsw, false};
             nsw = j + 1;
             done = true;
         } else {
@@ -725,22 +722,14 @@ __global__ void __launch_bounds__(256) k_td_psi_solve(TDSolveArgs<R> T) {
         in = out;
         out = (out == o1) ? o2 : o1;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st[ST_PSI] = in;
-        st[ST_PRED_PSI] = nsw;
-        *state64(st, ST_EPOCH) = epoch;
-        T.sc.sweeps[2 * T.step] = nsw;
-    }
+    return {in, nsw, true};
 }
 
+// The A solve of one TD step on the team's waves, sweep by sweep, with the
+// reference's source schedule A^(2*floor(j/2)) (td.py:264-303).
 template <typename R, bool GAUGE>
-__global__ void __launch_bounds__(256) k_td_a_solve(TDSolveArgs<R> T) {
-    __shared__ int lds[16];
-    const long W = (long)gridDim.x * (blockDim.x / WAVE);
-    const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
-    int *st = T.sc.state;
-    const int pc = st[ST_PSI], qc = st[ST_AB];
-    unsigned long long epoch = *state64(st, ST_EPOCH);
+__device__ __forceinline__ SolveOut a_solve(const TDSolveArgs<R> &T, const GridSync &gs, const Team &tm, long w,
+                                            long W, int pc, int qc, unsigned long long &epoch, int *lds) {
     const int o1 = (qc + 1) % 3, o2 = (qc + 2) % 3;
     ASweepArgs<R> A = T.A;
     A.psi = pick3(T.P, pc);
@@ -757,20 +746,118 @@ __global__ void __launch_bounds__(256) k_td_a_solve(TDSolveArgs<R> T) {
         R rr = R(0);
         for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, a_rows<R, GAUGE>(A, s, ja, jb)); });
         int r = 0;
-        if (!sweep_sync(T.sc, scaled_residual<R>(rr, A.stop_eps), 0, epoch, lds, r)) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) st[ST_ERRSTEP] = T.step + 1;
-            return;
-        }
+        if (!sweep_sync(gs, tm, scaled_residual<R>(rr, A.stop_eps), 0, epoch, lds, r)) return {in, nsw, false};
         nsw = j + 1;
         in = out;
         out = (out == o1) ? o2 : o1;
         if ((r & 0xffff) < 10000) break;
     }
+    return {in, nsw, true};
+}
+
+// Single instance: one launch per solve over the whole grid; the state
+// carries the buffer indices, the barrier epoch and the sweep-count
+// prediction between launches.
+template <typename R, bool GAUGE>
+__global__ void __launch_bounds__(256) k_td_psi_solve(TDSolveArgs<R> T) {
+    __shared__ int lds[16];
+    const long W = (long)gridDim.x * (blockDim.x / WAVE);
+    const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
+    int *st = T.sc.state;
+    unsigned long long epoch = *state64(st, ST_EPOCH);
+    const Team tm{0, (int)gridDim.x, NREL, T.sc.gs.rel};
+    const SolveOut o = psi_solve<R, GAUGE>(T, T.sc.gs, tm, w, W, st[ST_PSI], st[ST_AB], st[ST_PRED_PSI], epoch, lds);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st[ST_AB] = in;
+        if (!o.ok) {
+            st[ST_ERRSTEP] = T.step + 1;
+            return;
+        }
+        st[ST_PSI] = o.cur;
+        st[ST_PRED_PSI] = o.nsw;
         *state64(st, ST_EPOCH) = epoch;
-        T.sc.sweeps[2 * T.step + 1] = nsw;
+        T.sc.sweeps[2 * T.step] = o.nsw;
     }
+}
+
+template <typename R, bool GAUGE>
+__global__ void __launch_bounds__(256) k_td_a_solve(TDSolveArgs<R> T) {
+    __shared__ int lds[16];
+    const long W = (long)gridDim.x * (blockDim.x / WAVE);
+    const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
+    int *st = T.sc.state;
+    unsigned long long epoch = *state64(st, ST_EPOCH);
+    const Team tm{0, (int)gridDim.x, NREL, T.sc.gs.rel};
+    const SolveOut o = a_solve<R, GAUGE>(T, T.sc.gs, tm, w, W, st[ST_PSI], st[ST_AB], epoch, lds);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (!o.ok) {
+            st[ST_ERRSTEP] = T.step + 1;
+            return;
+        }
+        st[ST_AB] = o.cur;
+        *state64(st, ST_EPOCH) = epoch;
+        T.sc.sweeps[2 * T.step + 1] = o.nsw;
+    }
+}
+
+// ---- batched TD ---------------------------------------------------------------
+//
+// Many independent instances advance one TD step per launch. Each instance
+// owns a team of consecutive workgroups (sized to its grid by the host) and
+// runs its psi solve and then its A solve with team-local barriers, so its
+// sweep counts are its own. The per-instance state (buffer indices, barrier
+// epoch, step counter, sweep-count prediction) lives in device memory, so
+// every step launch is identical and a run can be replayed from a captured
+// HIP graph.
+
+template <typename R> struct BatchArgs {
+    const TDSolveArgs<R> *args;  // per instance
+    const int *team_of_block;    // per workgroup: its instance, or -1 (idle)
+    const int2 *teams;           // per instance: first workgroup, workgroups
+    unsigned long long *rel;     // per instance: one release word, REL_STRIDE apart
+    int *state;                  // per instance: 8 ints (ST_*)
+    int *sweeps;                 // [step][instance][2]
+    GridSync gs;                 // arrival words (per workgroup) and error flag
+    int n;
+};
+
+template <typename R, bool GAUGE>
+__global__ void __launch_bounds__(256) k_batch_td_step(BatchArgs<R> B) {
+    __shared__ int lds[16];
+    const int inst = B.team_of_block[blockIdx.x];
+    if (inst < 0) return;
+    const int2 tb = B.teams[inst];
+    const Team tm{tb.x, tb.y, 1, B.rel + (long)inst * REL_STRIDE};
+    const long W = (long)tb.y * (blockDim.x / WAVE);
+    const long w = (long)(blockIdx.x - tb.x) * (blockDim.x / WAVE) + wave_id();
+    const TDSolveArgs<R> &T = B.args[inst];
+    int *st = B.state + 8 * inst;
+    unsigned long long epoch = *state64(st, ST_EPOCH);
+    const int step = st[ST_STEP];
+    const int qc = st[ST_AB];
+    const SolveOut o = psi_solve<R, GAUGE>(T, B.gs, tm, w, W, st[ST_PSI], qc, st[ST_PRED_PSI], epoch, lds);
+    SolveOut oa{qc, 0, o.ok};
+    if (o.ok && T.solveA) oa = a_solve<R, GAUGE>(T, B.gs, tm, w, W, o.cur, qc, epoch, lds);
+    if ((int)blockIdx.x == tb.x && threadIdx.x == 0) {
+        if (!oa.ok) {
+            st[ST_ERRSTEP] = step + 1;
+            return;
+        }
+        st[ST_PSI] = o.cur;
+        st[ST_AB] = oa.cur;
+        st[ST_PRED_PSI] = o.nsw;
+        st[ST_STEP] = step + 1;
+        *state64(st, ST_EPOCH) = epoch;
+        B.sweeps[((long)step * B.n + inst) * 2] = o.nsw;
+        B.sweeps[((long)step * B.n + inst) * 2 + 1] = oa.nsw;
+    }
+}
+
+// Zero the arrival words of the first nb workgroups, n release words and
+// the error flag (a kernel, so that nothing here is a memset node).
+static __global__ void k_batch_reset(unsigned long long *arrive, int nb, unsigned long long *rel, int n, int *err) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nb; k += gridDim.x * blockDim.x) arrive[k] = 0ull;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) rel[(long)k * REL_STRIDE] = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *err = 0;
 }
 
 // Zero the barrier words and run state before the first solve of a run and

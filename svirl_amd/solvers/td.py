@@ -118,9 +118,8 @@ class TD:
         return int(max(1, min(1000, 2.0e8 // max(1, int(self.cfg.N)))))
 
     # -- the time loop ------------------------------------------------------------------
-    def _run(self, nsteps, eqn_code, dt):
-        if nsteps <= 0:
-            return
+    def _prepare(self, eqn_code, dt):
+        """Buffers and glx_td_params of a run (shared with batched runs)."""
         if self.fixed_vortices._vpi is not None or self.fixed_vortices._phase_lock_ns is not None:
             raise NotImplementedError("fixed vortices / phase lock are not supported by the GPU TD path")
         cfg = self.cfg
@@ -149,14 +148,35 @@ class TD:
         prm.solveA = 1 if self.solveA else 0
         prm.max_sweeps = self.MAX_SWEEPS
         prm.eqn = eqn_code
+        prm.rand_t = int(self._random_t)
         g = nat.Grid(int(cfg.Nx), int(cfg.Ny), float(cfg.dx), float(cfg.dy))
+        return g, prm, P, pc, Q, qc
+
+    def _finish(self, P, pc, Q, qc, sweeps, eqn_code):
+        """Record a run's sweep counts and point the variables at the final
+        buffers."""
+        arr = np.asarray(sweeps, dtype=np.int32).reshape(-1, 2)
+        self.sweeps.extend(map(tuple, arr.tolist()))
+        per_step = (1 if eqn_code != 2 else 0) + (1 if (self.solveA and eqn_code != 1) else 0)
+        self._random_t = np.uint32((int(self._random_t) + per_step * len(arr)) & 0xffffffff)
+        v = self.vars
+        v._psi._gdata = P[pc]
+        v._vp._gdata = Q[qc]
+        v._psi.need_dtoh_sync()
+        v._vp.need_dtoh_sync()
+
+    # -- the time loop ------------------------------------------------------------------
+    def _run(self, nsteps, eqn_code, dt):
+        if nsteps <= 0:
+            return
+        g, prm, P, pc, Q, qc = self._prepare(eqn_code, dt)
+        p = self.params
         Pp = (ctypes.c_void_p * 3)(*[b.ptr for b in P])
         Qp = (ctypes.c_void_p * 3)(*[b.ptr for b in Q])
         cpc, cqc = ctypes.c_int32(pc), ctypes.c_int32(qc)
         wg = self.wg or self._default_wg()
         chunk = self._launch_steps()
         done = 0
-        per_step = (1 if eqn_code != 2 else 0) + (1 if (self.solveA and eqn_code != 1) else 0)
         while done < nsteps:
             n = min(chunk, nsteps - done)
             prm.rand_t = int(self._random_t)
@@ -164,14 +184,8 @@ class TD:
             nat.call("glx_td_run", self.par.prec, g, ctypes.byref(prm), nat.ptr(p.linear_coefficient_h()),
                      nat.ptr(self.mesh.material_tiling_h()), Pp, ctypes.byref(cpc), Qp, ctypes.byref(cqc),
                      nat.ptr(self._rhs_psi), nat.ptr(self._rhs_ab), n, sw, wg, 256)
-            arr = np.frombuffer(sw, dtype=np.int32).reshape(n, 2)
-            self.sweeps.extend(map(tuple, arr.tolist()))
-            self._random_t = np.uint32((int(self._random_t) + per_step * n) & 0xffffffff)
+            self._finish(P, cpc.value, Q, cqc.value, np.frombuffer(sw, dtype=np.int32), eqn_code)
             done += n
-        v._psi._gdata = P[cpc.value]
-        v._vp._gdata = Q[cqc.value]
-        v._psi.need_dtoh_sync()
-        v._vp.need_dtoh_sync()
 
     def _solve(self, dt=None, Nt=None, T=None, eqn=None):
         """td.py:342-377: eqn None (psi then A each step), 'order_parameter'
