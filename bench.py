@@ -98,6 +98,66 @@ def run_cg(gl, iters):
     return dict(iters=len(cg.cg_energies), wall=t, F=float(cg.cg_energies[-1]))
 
 
+def run_batch(args):
+    """Config 4: n instances (FP32-mixed), batched steps vs the same
+    instances run one after another (each through its own persistent
+    launches), and the batched FP32 result against the batched FP64 form."""
+    from svirl_amd import _native as nat
+    from svirl_amd import workloads
+    from svirl_amd.batch import CANDIDATES, BatchTD
+    log(f"[batch] building {args.batch_n} Config 4 instances")
+    sims = workloads.config4(n=args.batch_n, dtype=np.float32)
+    nodes = sum(int(s.cfg.N) for s in sims)
+    b = BatchTD(sims, use_graph=True, autotune=True)
+    b.td(dt=0.1, Nt=b.tune_steps * len(CANDIDATES) + 1)  # autotune on the first steps
+    nat.call("glx_synchronize")
+    t0 = time.perf_counter()
+    b.td(dt=0.1, Nt=args.batch_steps)
+    nat.call("glx_synchronize")
+    tb = time.perf_counter() - t0
+    rate_b = nodes * args.batch_steps / tb
+    log(f"[batch] {len(sims)} instances, {nodes} nodes: {rate_b:.4g} node-steps/s batched "
+        f"({1e3 * tb / args.batch_steps:.3f} ms/step, shape {b.shape})")
+    for s in sims:
+        s.solve.td(dt=0.1, Nt=1)  # warm each instance's own launches
+    nat.call("glx_synchronize")
+    t0 = time.perf_counter()
+    for s in sims:
+        s.solve.td(dt=0.1, Nt=args.batch_solo_steps)
+    nat.call("glx_synchronize")
+    ts = time.perf_counter() - t0
+    rate_s = nodes * args.batch_solo_steps / ts
+    log(f"[batch] one instance at a time: {rate_s:.4g} node-steps/s; speedup {rate_b / rate_s:.2f}x")
+    sw = np.array([np.mean(np.array(s.solve._td.sweeps), axis=0) for s in sims])
+    res = {"instances": len(sims), "nodes": nodes, "node_steps_per_s": rate_b, "ms_per_step": 1e3 * tb / args.batch_steps,
+           "per_instance_node_steps_per_s": rate_s, "speedup_vs_per_instance": rate_b / rate_s,
+           "tuned_shape": {"wg_per_cu": b.shape[0], "min_rows": b.shape[1]},
+           "tuning_ms_per_step": {f"{k[0]},{k[1]}": v for k, v in b.timings.items()},
+           "mean_sweeps_per_step": {"psi": float(sw[:, 0].mean()), "A": float(sw[:, 1].mean())},
+           "dtype": "f32", "graph": True}
+    del sims
+    # precision: the first 32 instances, FP32-mixed vs FP64, batched, same steps
+    k = min(32, args.batch_n)
+    s32 = workloads.config4(n=args.batch_n, dtype=np.float32)[:k]
+    s64 = workloads.config4(n=args.batch_n, dtype=np.float64)[:k]
+    BatchTD(s32, autotune=False).td(dt=0.1, Nt=args.batch_steps)
+    BatchTD(s64, autotune=False).td(dt=0.1, Nt=args.batch_steps)
+    fe, pe, je = [], [], []
+    for a, c in zip(s32, s64):
+        fa, fc = a.observables.free_energy_fp64(), c.observables.free_energy_fp64()
+        fe.append(abs(fa - fc) / abs(fc))
+        pa = np.asarray(a.vars.order_parameter, np.complex128)
+        pc = np.asarray(c.vars.order_parameter, np.complex128)
+        pe.append(np.linalg.norm(pa - pc) / np.linalg.norm(pc))
+        ja = np.concatenate([np.ravel(x) for x in a.observables.current_density]).astype(np.float64)
+        jc = np.concatenate([np.ravel(x) for x in c.observables.current_density]).astype(np.float64)
+        je.append(np.linalg.norm(ja - jc) / max(np.linalg.norm(jc), 1e-300))
+    res["fp32_vs_fp64"] = {"instances": k, "after_steps": args.batch_steps, "F_rel_max": float(np.max(fe)),
+                           "psi_L2_rel_max": float(np.max(pe)), "J_L2_rel_max": float(np.max(je))}
+    log(f"[batch] FP32 vs FP64 over {k} instances: F {np.max(fe):.2e}, psi {np.max(pe):.2e}, J {np.max(je):.2e}")
+    return res
+
+
 def calibrate_copy(n=1 << 26, reps=3):
     """FP32 axpy over n reals (reads 8n bytes, writes 4n), for scaling the
     rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
@@ -155,6 +215,10 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-cg", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--batch-n", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=60)
+    ap.add_argument("--batch-solo-steps", type=int, default=10)
     ap.add_argument("--calibrate", action="store_true",
                     help="run FP32 axpy dispatches of known size first (counter calibration)")
     ap.add_argument("--cpu-L", type=float, default=512.0)
@@ -204,6 +268,10 @@ def main():
             log(f"[{prec}] psi solve {out['psi_launch_us']:.1f} us/launch, {out['psi_gbs']:.1f} GB/s algorithmic; "
                 f"A solve {out['a_launch_us']:.1f} us/launch")
         out["F_td"] = gl.observables.free_energy_fp64()
+        if not args.skip_fp64:
+            jx, jy = gl.observables.current_density
+            out["psi_h"] = np.asarray(gl.vars.order_parameter, dtype=np.complex128).ravel()
+            out["J_h"] = np.concatenate([np.asarray(jx, dtype=np.float64).ravel(), np.asarray(jy, dtype=np.float64).ravel()])
         if not args.skip_cg:
             c = run_cg(gl, args.cg_iters)
             out["cg_iters_per_s"] = c["iters"] / c["wall"]
@@ -212,6 +280,10 @@ def main():
             log(f"[{prec}] CG {out['cg_iters_per_s']:.2f} iters/s ({out['cg_ms_per_iter']:.2f} ms/iter)")
         res[prec] = out
         del gl
+
+    batch = None
+    if not args.skip_batch:
+        batch = run_batch(args)
 
     cpu = None
     if not args.skip_cpu:
@@ -253,6 +325,8 @@ def main():
         "td_all_kernels_GBs": r32["all_gbs"],
         "cpu_baseline": cpu,
     }
+    if batch:
+        out["batch"] = batch
     if "cg_iters_per_s" in r32:
         out["cg"] = {"fp32_iters_per_s": r32["cg_iters_per_s"], "fp32_ms_per_iter": r32["cg_ms_per_iter"]}
     if "fp64" in res:
@@ -264,6 +338,12 @@ def main():
                        "sweeps_per_step": {"psi": r64["sweeps_psi"], "A": r64["sweeps_A"]}, "F": r64["F_td"]}
         out["fp32_over_fp64"] = r32["node_steps_per_s"] / r64["node_steps_per_s"]
         out["fp32_F_rel_err_vs_fp64"] = abs(r32["F_td"] - r64["F_td"]) / abs(r64["F_td"])
+        out["fp32_vs_fp64"] = {
+            "after_steps": args.warmup + args.steps + args.profile_steps,
+            "F_rel": abs(r32["F_td"] - r64["F_td"]) / abs(r64["F_td"]),
+            "psi_L2_rel": float(np.linalg.norm(r32["psi_h"] - r64["psi_h"]) / np.linalg.norm(r64["psi_h"])),
+            "J_L2_rel": float(np.linalg.norm(r32["J_h"] - r64["J_h"]) / np.linalg.norm(r64["J_h"])),
+            "reference": "FP64 GPU form on the same inputs (validated against the FP64 oracle by the tests)"}
         if "cg_iters_per_s" in r64:
             out["cg"]["fp64_iters_per_s"] = r64["cg_iters_per_s"]
             out["cg"]["fp64_ms_per_iter"] = r64["cg_ms_per_iter"]

@@ -52,3 +52,23 @@ def config3(L=2048.0, dx=0.5, dtype=np.float64, seed_psi=3, kappa=5.0, sigma=200
     gl.vars.set_order_parameter_to_zero_outside_material()
     gl.params.linear_coefficient = pinning_field(int(c.Nx), int(c.Ny), float(c.dx))
     return gl
+
+
+def config4(n=256, dtype=np.float32, seed=40, sizes=(32, 64, 128, 256, 512), dx=0.5, sigma=200.0):
+    """Config 4: n independent instances for a batched run. Lx and Ly are
+    drawn independently from `sizes` (powers of two by default), H uniform
+    in [0.02, 0.5], kappa from {2, 5, inf}; each instance has its own seeded
+    random psi and pinning field eps(r) (mean 1, std 0.2, correlation 2)."""
+    from svirl_amd import GLSolver
+    rng = np.random.default_rng(seed)
+    sims = []
+    for k in range(n):
+        Lx, Ly = float(rng.choice(sizes)), float(rng.choice(sizes))
+        H = float(rng.uniform(0.02, 0.5))
+        kappa = float(rng.choice([2.0, 5.0, np.inf]))
+        gl = GLSolver(Lx=Lx, Ly=Ly, dx=dx, dy=dx, order_parameter='random', random_seed=seed * 1000 + k,
+                      gl_parameter=kappa, normal_conductivity=sigma, homogeneous_external_field=H, dtype=dtype)
+        c = gl.cfg
+        gl.params.linear_coefficient = pinning_field(int(c.Nx), int(c.Ny), dx, seed=seed * 1000 + k)
+        sims.append(gl)
+    return sims
