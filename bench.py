@@ -194,14 +194,14 @@ def cpu_baseline(L, dt, steps):
 
 
 def pmc_traffic():
-    """HBM bytes per k_td_psi_solve<float> launch from the committed counter
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration)."""
+    """Measured HBM bytes per node per psi sweep of k_td_psi_solve<float>
+    from the committed counter summary (FETCH_SIZE + WRITE_SIZE, scaled by
+    the copy calibration), or None."""
     try:
         with open(PMC_SUMMARY) as f:
-            d = json.load(f)
-        return d.get("bytes_per_launch"), d.get("algorithmic_bytes_per_launch")
+            return json.load(f).get("bytes_per_node_sweep")
     except (OSError, ValueError):
-        return None, None
+        return None
 
 
 def main():
@@ -296,10 +296,17 @@ def main():
         log(f"[cpu] {rate:.4g} node-steps/s")
 
     r32 = res["fp32"]
-    traffic, alg_pmc = pmc_traffic()
+    per_node_sweep = pmc_traffic()
+    traffic = None
+    if per_node_sweep and r32.get("psi_bytes_per_launch"):
+        # measured bytes per node-sweep x this run's node-sweeps per launch
+        traffic = per_node_sweep * r32["psi_bytes_per_launch"] / 37.0
     roof = {"bound": "hbm", "achieved": r32.get("psi_gbs"), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": (r32["psi_gbs"] / HBM_PEAK_GBS) if r32.get("psi_gbs") else None,
             "traffic": traffic,
+            "traffic_bytes_per_node_sweep": per_node_sweep,
+            "algorithmic_bytes_per_node_sweep": 37,
+            "physical_GBs": (traffic / (r32["psi_launch_us"] * 1e-6) / 1e9) if traffic else None,
             "kernel": "k_td_psi_solve<float, gauge-shifted A>",
             "launch_us": r32.get("psi_launch_us"),
             "algorithmic_bytes_per_launch": r32.get("psi_bytes_per_launch"),
