@@ -166,6 +166,37 @@ typedef struct glx_batch_instance {
 int glx_batch_td_run(int prec, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
                      int32_t use_graph, int32_t wg_per_cu, int32_t min_rows);
 
+/* ------------------------------------------------------------ batched CG */
+
+/* One instance of a batched CG minimisation (no counterpart in the
+ * reference; the batched form of cg.py:452-551 / 238-322). kappa2 <= 0
+ * means kappa = inf (psi only, ab may be NULL); all instances of one call
+ * must agree on that. eps_map (may be NULL) is used by the gradient and by
+ * the line-search coefficients. iters, converged and grad_rel are outputs. */
+typedef struct glx_batch_cg_instance {
+    glx_grid g;
+    double kappa2;
+    double eps;
+    double H;
+    const void *eps_map;
+    const uint8_t *mt;
+    void *psi;
+    void *ab;
+    int32_t iters;
+    int32_t converged;
+    double grad_rel;
+} glx_batch_cg_instance;
+
+/* Minimise n instances by Polak-Ribiere CG with an exact polynomial line
+ * search: each instance is owned by one workgroup that runs whole
+ * iterations (gradient, FP64 PR sums and gradient norm, direction, the 17
+ * (or 5) FP64 line-search coefficients, an FP64 2-D quartic minimisation on
+ * the device, update) and stops on its own when ||grad F|| <= rtol *
+ * ||grad F||_0 or after max_iters (a ragged batch). iters_per_launch
+ * iterations run per kernel launch (0 = 50). */
+int glx_batch_cg_run(int prec, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,
+                     int32_t iters_per_launch);
+
 /* Duration (ms, HIP events on the library stream) of the persistent kernel
  * of the last glx_td_run call, excluding its reset and read-back. */
 int glx_td_last_kernel_ms(float *ms);

@@ -46,6 +46,13 @@ class BatchInstance(ctypes.Structure):
                 ("psi", c_void_p * 3), ("ab", c_void_p * 3), ("psi_cur", c_i32), ("ab_cur", c_i32)]
 
 
+class BatchCGInstance(ctypes.Structure):
+    """glx_batch_cg_instance."""
+    _fields_ = [("g", Grid), ("kappa2", c_double), ("eps", c_double), ("H", c_double), ("eps_map", c_void_p),
+                ("mt", c_void_p), ("psi", c_void_p), ("ab", c_void_p), ("iters", c_i32), ("converged", c_i32),
+                ("grad_rel", c_double)]
+
+
 # name -> (argtypes); every function returns int
 _PROTOS = {
     "glx_init": [c_int],
@@ -70,6 +77,7 @@ _PROTOS = {
     "glx_td_last_kernel_ms": [ctypes.POINTER(ctypes.c_float)],
     "glx_td_profile": [c_i32],
     "glx_batch_td_run": [c_int, ctypes.POINTER(BatchInstance), c_i32, c_i32, P_i32, c_i32, c_i32, c_i32],
+    "glx_batch_cg_run": [c_int, ctypes.POINTER(BatchCGInstance), c_i32, c_i32, c_double, c_i32],
     "glx_td_solve_ms": [ctypes.POINTER(c_double), P_i32, ctypes.POINTER(c_double), P_i32],
     "glx_magnetic_field": [c_int, Grid, c_void_p, c_void_p, c_void_p],
     "glx_current_density": [c_int, Grid, c_double, c_double, c_void_p, c_void_p, c_void_p],

@@ -18,13 +18,18 @@ signature.
     from svirl_amd.batch import BatchTD
     sims = [GLSolver(...) for _ in range(256)]
     BatchTD(sims).td(dt=0.1, Nt=5000)
+
+BatchCG minimises many instances at once by the reference's Polak-Ribiere
+CG (cg.py:452-551): one workgroup per instance runs whole iterations with
+FP64 reductions and an FP64 device line search, and each instance stops on
+its own gradient-norm test.
 """
 import ctypes
 
 import numpy as np
 
 from svirl_amd import _native as nat
-from svirl_amd._native import BatchInstance
+from svirl_amd._native import BatchCGInstance, BatchInstance
 
 # (signature) -> (wg_per_cu, min_rows)
 _TUNED = {}
@@ -118,3 +123,54 @@ class BatchTD:
             self._run(tds, Nt - done, self.shape)
         for s in self.solvers:
             s.solve.vortices_detected = False
+
+
+class BatchCG:
+    """Free-energy minimisation of a list of GLSolver objects (one dtype;
+    all kappa < inf or all kappa = inf). Each instance stops when
+    ||grad F|| <= rtol * ||grad F||_0 or after max_iter iterations."""
+
+    def __init__(self, solvers):
+        if not solvers:
+            raise ValueError("BatchCG needs at least one solver")
+        dt0 = solvers[0].cfg.dtype
+        if any(s.cfg.dtype is not dt0 for s in solvers):
+            raise ValueError("all solvers of a batch must share one dtype")
+        if len({bool(s.params.solveA) for s in solvers}) != 1:
+            raise ValueError("all solvers of a batch must have kappa < inf, or all kappa = inf")
+        self.solvers = list(solvers)
+        self.prec = nat.prec_of(dt0)
+        self.last_kernel_ms = 0.0
+
+    def cg(self, max_iter=3000, rtol=1e-8, iters_per_launch=50):
+        """Returns (iterations, converged flags, final relative gradient
+        norms) per instance."""
+        n = len(self.solvers)
+        arr = (BatchCGInstance * n)()
+        for k, gl in enumerate(self.solvers):
+            c, p, v = gl.cfg, gl.params, gl.vars
+            if p.fixed_vortices._vpi is not None or p.fixed_vortices._phase_lock_ns is not None:
+                raise NotImplementedError("fixed vortices are not supported by the batched CG")
+            v._psi.sync_d()
+            v._vp.sync_d()
+            I = arr[k]
+            I.g = nat.Grid(int(c.Nx), int(c.Ny), float(c.dx), float(c.dy))
+            I.kappa2 = float(p.gl_parameter_squared_h())
+            I.eps = float(p.linear_coefficient_scalar_h())
+            I.H = float(p.homogeneous_external_field)
+            I.eps_map = nat.ptr(p.linear_coefficient_h())
+            I.mt = nat.ptr(gl.mesh.material_tiling_h())
+            I.psi = v.order_parameter_h().ptr
+            I.ab = nat.ptr(v.vector_potential_h()) if p.solveA else None
+        nat.call("glx_batch_cg_run", self.prec, arr, n, int(max_iter), float(rtol), int(iters_per_launch))
+        kms = ctypes.c_float()
+        nat.call("glx_td_last_kernel_ms", ctypes.byref(kms))
+        self.last_kernel_ms = kms.value
+        for gl in self.solvers:
+            gl.vars._psi.need_dtoh_sync()
+            gl.vars._vp.need_dtoh_sync()
+            gl.solve.vortices_detected = False
+        iters = np.array([arr[k].iters for k in range(n)])
+        conv = np.array([bool(arr[k].converged) for k in range(n)])
+        grel = np.array([arr[k].grad_rel for k in range(n)])
+        return iters, conv, grel

@@ -238,6 +238,11 @@ int glx_batch_td_run(int prec, glx_batch_instance *inst, int32_t n, int32_t nste
     GLX_DISPATCH(prec, batch_td_run(g_ctx.env, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows));
 }
 
+int glx_batch_cg_run(int prec, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,
+                     int32_t iters_per_launch) {
+    GLX_DISPATCH(prec, batch_cg_run(g_ctx.env, inst, n, max_iters, rtol, iters_per_launch));
+}
+
 int glx_td_profile(int32_t enable) {
     GLX_REQUIRE_INIT();
     g_ctx.env.td_profile = enable != 0;

@@ -339,44 +339,93 @@ __global__ void k_free_energy(Geo g, R kappa2, R eps, const R *eps_map, R H, con
 
 // ---------------------------------------------------------------- CG
 
+// dF/dpsi at node n before the dxdy factor (free_energy_jacobian_psi,
+// cg.h:15-121).
+template <typename R>
+__device__ __forceinline__ cplx<R> jac_psi_node(const Geo &g, const GeoR<R> &q, R eps, const R *eps_map,
+                                                const uint8_t *mt, const cplx<R> *psi, const R *abei, const R *ab,
+                                                long n) {
+    const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
+    cplx<R> gj = {R(0), R(0)};
+    const Cells c = cells_of(g, mt, i, j);
+    if (c.mm || c.mp || c.pm || c.pp) {
+        const R f_mm = c.mm ? R(1) : R(0), f_mp = c.mp ? R(1) : R(0), f_pm = c.pm ? R(1) : R(0), f_pp = c.pp ? R(1) : R(0);
+        const R r_m0 = R(0.5) * (f_mm + f_mp), r_p0 = R(0.5) * (f_pm + f_pp);
+        const R r_0m = R(0.5) * (f_mm + f_pm), r_0p = R(0.5) * (f_mp + f_pp);
+        const R gw = R(0.25) * (r_m0 + r_p0 + r_0m + r_0p);
+        const cplx<R> p0 = ld(psi, n);
+        const R ee = eps_map ? eps_map[n] : eps;
+        const R p = p0.re * p0.re + p0.im * p0.im - ee;
+        const R k = R(2) * gw * p;
+        gj.re += k * p0.re;
+        gj.im += k * p0.im;
+        auto add = [&](R r, R id2, R ph, cplx<R> p1) {
+            R s, cs;
+            sincos_r(ph, &s, &cs);
+            const R tr = R(2) * (p0.re - (p1.re * cs + p1.im * s));
+            const R ti = R(2) * (p0.im - (p1.im * cs - p1.re * s));
+            const R w = r * id2;
+            gj.re += w * tr;
+            gj.im += w * ti;
+        };
+        if (c.mm || c.mp) add(r_m0, q.idx2, -q.dx * pv(abei, ab, (i - 1) + (long)g.Nxa * j), ld(psi, n - 1));
+        if (c.pm || c.pp) add(r_p0, q.idx2, q.dx * pv(abei, ab, i + (long)g.Nxa * j), ld(psi, n + 1));
+        if (c.mm || c.pm) add(r_0m, q.idy2, -q.dy * pv(abei, ab, g.Na + i + (long)g.Nxb * (j - 1)), ld(psi, n - g.Nx));
+        if (c.mp || c.pp) add(r_0p, q.idy2, q.dy * pv(abei, ab, g.Na + i + (long)g.Nxb * j), ld(psi, n + g.Nx));
+    }
+    return gj;
+}
+
 template <typename R>
 __global__ void k_jacobian_psi(Geo g, R kappa2, R eps, const R *eps_map, R H, const uint8_t *mt,
                                const cplx<R> *psi, const R *abei, const R *ab, cplx<R> *G) {
     const GeoR<R> q(g);
     const R dxdy = q.dx * q.dy;
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
-        cplx<R> gj = {R(0), R(0)};
-        const Cells c = cells_of(g, mt, i, j);
-        if (c.mm || c.mp || c.pm || c.pp) {
-            const R f_mm = c.mm ? R(1) : R(0), f_mp = c.mp ? R(1) : R(0), f_pm = c.pm ? R(1) : R(0), f_pp = c.pp ? R(1) : R(0);
-            const R r_m0 = R(0.5) * (f_mm + f_mp), r_p0 = R(0.5) * (f_pm + f_pp);
-            const R r_0m = R(0.5) * (f_mm + f_pm), r_0p = R(0.5) * (f_mp + f_pp);
-            const R gw = R(0.25) * (r_m0 + r_p0 + r_0m + r_0p);
-            const cplx<R> p0 = ld(psi, n);
-            const R ee = eps_map ? eps_map[n] : eps;
-            const R p = p0.re * p0.re + p0.im * p0.im - ee;
-            const R k = R(2) * gw * p;
-            gj.re += k * p0.re;
-            gj.im += k * p0.im;
-            auto add = [&](R r, R id2, R ph, cplx<R> p1) {
-                R s, cs;
-                sincos_r(ph, &s, &cs);
-                const R tr = R(2) * (p0.re - (p1.re * cs + p1.im * s));
-                const R ti = R(2) * (p0.im - (p1.im * cs - p1.re * s));
-                const R w = r * id2;
-                gj.re += w * tr;
-                gj.im += w * ti;
-            };
-            if (c.mm || c.mp) add(r_m0, q.idx2, -q.dx * pv(abei, ab, (i - 1) + (long)g.Nxa * j), ld(psi, n - 1));
-            if (c.pm || c.pp) add(r_p0, q.idx2, q.dx * pv(abei, ab, i + (long)g.Nxa * j), ld(psi, n + 1));
-            if (c.mm || c.pm) add(r_0m, q.idy2, -q.dy * pv(abei, ab, g.Na + i + (long)g.Nxb * (j - 1)), ld(psi, n - g.Nx));
-            if (c.mp || c.pp) add(r_0p, q.idy2, q.dy * pv(abei, ab, g.Na + i + (long)g.Nxb * j), ld(psi, n + g.Nx));
-        }
+        const cplx<R> gj = jac_psi_node(g, q, eps, eps_map, mt, psi, abei, ab, n);
         cplx<R> o = ld(G, n);
         o.re += dxdy * gj.re;
         o.im += dxdy * gj.im;
         st(G, n, o);
+    }
+}
+
+// dF/dA of the a link (i, j) and the b link (i, j) at node n, before the
+// 2 dxdy factor (free_energy_jacobian_A, cg.h:124-301); has_a / has_b say
+// which links exist. The reference's field part on boundary links is twice
+// the derivative of F (ghost cell with dd = 2); bscale multiplies it there:
+// 1 keeps the reference's Jacobian, 1/2 gives the exact gradient.
+template <typename R>
+__device__ __forceinline__ void jac_A_node(const Geo &g, const GeoR<R> &q, R kappa2, R H, const uint8_t *mt,
+                                           const cplx<R> *psi, const R *abei, const R *ab, long n, R &ga, R &gb,
+                                           bool &has_a, bool &has_b, R bscale = R(1)) {
+    const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
+    const cplx<R> p0 = ld(psi, n);
+    const Cells c = cells_of(g, mt, i, j);
+    has_a = i < g.Nxa && j < g.Nya;
+    has_b = i < g.Nxb && j < g.Nyb;
+    ga = gb = R(0);
+    if (has_a) {
+        const long m = i + (long)g.Nxa * j;
+        R gj = curlcurl_a(g, q, H, abei, ab, i, j);
+        if (j == 0 || j + 1 == g.Nya) gj *= bscale;
+        gj *= kappa2;
+        if (c.pm || c.pp) {
+            const R r = R(0.5) * ((c.pm ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
+            gj += -r * q.idx * js(p0, q.dx * pv(abei, ab, m), ld(psi, n + 1));
+        }
+        ga = gj;
+    }
+    if (has_b) {
+        const long m = g.Na + i + (long)g.Nxb * j;
+        R gj = curlcurl_b(g, q, H, abei, ab, i, j);
+        if (i == 0 || i + 1 == g.Nxb) gj *= bscale;
+        gj *= kappa2;
+        if (c.mp || c.pp) {
+            const R r = R(0.5) * ((c.mp ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
+            gj += -r * q.idy * js(p0, q.dy * pv(abei, ab, m), ld(psi, n + g.Nx));
+        }
+        gb = gj;
     }
 }
 
@@ -386,33 +435,103 @@ __global__ void k_jacobian_A(Geo g, R kappa2, R H, const uint8_t *mt, const cplx
     const GeoR<R> q(g);
     const R s2 = R(2) * q.dx * q.dy;
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
+        R ga, gb;
+        bool has_a, has_b;
+        jac_A_node(g, q, kappa2, H, mt, psi, abei, ab, n, ga, gb, has_a, has_b);
         const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
-        const cplx<R> p0 = ld(psi, n);
-        const Cells c = cells_of(g, mt, i, j);
-        if (i < g.Nxa && j < g.Nya) {
-            const long m = i + (long)g.Nxa * j;
-            R gj = curlcurl_a(g, q, H, abei, ab, i, j);
-            gj *= kappa2;
-            if (c.pm || c.pp) {
-                const R r = R(0.5) * ((c.pm ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
-                gj += -r * q.idx * js(p0, q.dx * pv(abei, ab, m), ld(psi, n + 1));
-            }
-            G[m] += s2 * gj;
-        }
-        if (i < g.Nxb && j < g.Nyb) {
-            const long m = g.Na + i + (long)g.Nxb * j;
-            R gj = curlcurl_b(g, q, H, abei, ab, i, j);
-            gj *= kappa2;
-            if (c.mp || c.pp) {
-                const R r = R(0.5) * ((c.mp ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
-                gj += -r * q.idy * js(p0, q.dy * pv(abei, ab, m), ld(psi, n + g.Nx));
-            }
-            G[m] += s2 * gj;
-        }
+        if (has_a) G[i + (long)g.Nxa * j] += s2 * ga;
+        if (has_b) G[g.Na + i + (long)g.Nxb * j] += s2 * gb;
     }
 }
 
 // Line-search coefficients. L = 5 (cg.h:314-474) or 17 (cg.h:477-731).
+// One node's contributions are added to C (before the dxdy factor). The
+// reference's kernels read only the scalar eps (eps_map is NULL on the
+// drop-in path); the batched CG passes the map.
+template <typename R, int L>
+__device__ __forceinline__ void coef_node(const Geo &g, const GeoR<R> &q, R kappa2, R eps_s, const R *eps_map, R H,
+                                          const uint8_t *mt, const cplx<R> *psi, const cplx<R> *dpsi, const R *abei,
+                                          const R *ab, const R *dab, long n, R (&C)[L]) {
+    const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
+    const Cells c = cells_of(g, mt, i, j);
+    if (c.mm || c.mp || c.pm || c.pp) {
+        const R f_mm = c.mm ? R(1) : R(0), f_mp = c.mp ? R(1) : R(0), f_pm = c.pm ? R(1) : R(0), f_pp = c.pp ? R(1) : R(0);
+        const R r_m0 = R(0.5) * (f_mm + f_mp), r_p0 = R(0.5) * (f_pm + f_pp);
+        const R r_0m = R(0.5) * (f_mm + f_pm), r_0p = R(0.5) * (f_mp + f_pp);
+        const R gw = R(0.25) * (r_m0 + r_p0 + r_0m + r_0p);
+        const cplx<R> p0 = ld(psi, n), d0 = ld(dpsi, n);
+        const R p2 = p0.re * p0.re + p0.im * p0.im;
+        const R d2 = d0.re * d0.re + d0.im * d0.im;
+        const R tw = R(2) * (p0.re * d0.re + p0.im * d0.im);
+        const R eps = eps_map ? eps_map[n] : eps_s;
+        const R k0 = gw * (R(0.5) * p2 - eps) * p2;
+        const R k1 = gw * tw * (p2 - eps);
+        const R k2 = gw * (-eps * d2 + R(0.5) * tw * tw + p2 * d2);
+        const R k3 = gw * tw * d2;
+        const R k4 = gw * R(0.5) * d2 * d2;
+        if (L == 5) { C[0] += k0; C[1] += k1; C[2] += k2; C[3] += k3; C[4] += k4; }
+        else { C[0] += k0; C[5] += k1; C[10] += k2; C[15 % L] += k3; C[16 % L] += k4; }
+        for (int lk = 0; lk < 2; ++lk) {
+            const bool on = lk == 0 ? (c.pm || c.pp) : (c.mp || c.pp);
+            if (!on) continue;
+            const R r = lk == 0 ? r_p0 : r_0p;
+            const R id2 = lk == 0 ? q.idx2 : q.idy2;
+            const R d = lk == 0 ? q.dx : q.dy;
+            const long k = lk == 0 ? i + (long)g.Nxa * j : g.Na + i + (long)g.Nxb * j;
+            const long nb = lk == 0 ? n + 1 : n + g.Nx;
+            R ph = R(0);
+            if (abei) ph += d * abei[k];
+            if (ab) ph += d * ab[k];
+            const cplx<R> p1 = ld(psi, nb), d1 = ld(dpsi, nb);
+            const cplx<R> zp = grad_c(p0, ph, p1), zd = grad_c(d0, ph, d1);
+            const R zr = zp.re * zd.re - (-zp.im) * zd.im;  // Re(conj(zp) zd)
+            const R w = r * id2;
+            if (L == 5) {
+                C[2] += w * g_grad(d0, ph, d1);
+                C[1] += w * R(2) * zr;
+                C[0] += w * g_grad(p0, ph, p1);
+            } else {
+                const R dph = d * dab[k], dph2 = dph * dph;
+                C[0] += w * g_grad(p0, ph, p1);
+                C[5] += w * R(2) * zr;
+                C[10] += w * g_grad(d0, ph, d1);
+                R s, cs;
+                sincos_r(ph, &s, &cs);  // U(-ph) = (cos ph, sin ph)
+                auto mulU = [&](cplx<R> a, cplx<R> b) -> cplx<R> {  // a * U(-ph) * conj(b)
+                    const cplx<R> t = {a.re * cs - a.im * s, a.re * s + a.im * cs};
+                    return {t.re * b.re - t.im * (-b.im), t.re * (-b.im) + t.im * b.re};
+                };
+                const cplx<R> z0 = mulU(p0, p1);
+                const cplx<R> za = mulU(p0, d1), zb = mulU(d0, p1);
+                const cplx<R> z1 = {za.re + zb.re, za.im + zb.im};
+                const cplx<R> z2 = mulU(d0, d1);
+                const cplx<R> zz[3] = {z0, z1, z2};
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    C[5 * m + 1] += w * R(2) * zz[m].im * dph;
+                    C[5 * m + 2] += w * zz[m].re * dph2;
+                    C[5 * m + 3] += -r * id2 / R(3) * zz[m].im * dph2 * dph;
+                    C[5 * m + 4] += -r * id2 / R(12) * zz[m].re * dph2 * dph2;
+                }
+            }
+        }
+    }
+    if (kappa2 > R(0) && i < g.Nxc && j < g.Nyc) {
+        const long a00 = i + (long)g.Nxa * j, a0p = i + (long)g.Nxa * (j + 1);
+        const long b00 = g.Na + i + (long)g.Nxb * j, bp0 = g.Na + i + 1 + (long)g.Nxb * j;
+        if (L == 5) {
+            const R dB = q.idx * (ab[bp0] - ab[b00]) - q.idy * (ab[a0p] - ab[a00]) - H;
+            C[0] += kappa2 * dB * dB;
+        } else {
+            const R BH = q.idx * (pv(abei, ab, bp0) - pv(abei, ab, b00)) - q.idy * (pv(abei, ab, a0p) - pv(abei, ab, a00)) - H;
+            const R dB = q.idx * (dab[bp0] - dab[b00]) - q.idy * (dab[a0p] - dab[a00]);
+            C[0] += kappa2 * BH * BH;
+            C[1] += kappa2 * R(2) * BH * dB;
+            C[2] += kappa2 * dB * dB;
+        }
+    }
+}
+
 template <typename R, int L>
 __global__ void k_coef(Geo g, R kappa2, R eps, R H, const uint8_t *mt, const cplx<R> *psi, const cplx<R> *dpsi,
                        const R *abei, const R *ab, const R *dab, double *partials) {
@@ -421,86 +540,10 @@ __global__ void k_coef(Geo g, R kappa2, R eps, R H, const uint8_t *mt, const cpl
 #pragma unroll
     for (int k = 0; k < L; ++k) acc[k] = 0.0;
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
         R C[L];
 #pragma unroll
         for (int k = 0; k < L; ++k) C[k] = R(0);
-        const Cells c = cells_of(g, mt, i, j);
-        if (c.mm || c.mp || c.pm || c.pp) {
-            const R f_mm = c.mm ? R(1) : R(0), f_mp = c.mp ? R(1) : R(0), f_pm = c.pm ? R(1) : R(0), f_pp = c.pp ? R(1) : R(0);
-            const R r_m0 = R(0.5) * (f_mm + f_mp), r_p0 = R(0.5) * (f_pm + f_pp);
-            const R r_0m = R(0.5) * (f_mm + f_pm), r_0p = R(0.5) * (f_mp + f_pp);
-            const R gw = R(0.25) * (r_m0 + r_p0 + r_0m + r_0p);
-            const cplx<R> p0 = ld(psi, n), d0 = ld(dpsi, n);
-            const R p2 = p0.re * p0.re + p0.im * p0.im;
-            const R d2 = d0.re * d0.re + d0.im * d0.im;
-            const R tw = R(2) * (p0.re * d0.re + p0.im * d0.im);
-            const R k0 = gw * (R(0.5) * p2 - eps) * p2;
-            const R k1 = gw * tw * (p2 - eps);
-            const R k2 = gw * (-eps * d2 + R(0.5) * tw * tw + p2 * d2);
-            const R k3 = gw * tw * d2;
-            const R k4 = gw * R(0.5) * d2 * d2;
-            if (L == 5) { C[0] += k0; C[1] += k1; C[2] += k2; C[3] += k3; C[4] += k4; }
-            else { C[0] += k0; C[5] += k1; C[10] += k2; C[15 % L] += k3; C[16 % L] += k4; }
-            for (int lk = 0; lk < 2; ++lk) {
-                const bool on = lk == 0 ? (c.pm || c.pp) : (c.mp || c.pp);
-                if (!on) continue;
-                const R r = lk == 0 ? r_p0 : r_0p;
-                const R id2 = lk == 0 ? q.idx2 : q.idy2;
-                const R d = lk == 0 ? q.dx : q.dy;
-                const long k = lk == 0 ? i + (long)g.Nxa * j : g.Na + i + (long)g.Nxb * j;
-                const long nb = lk == 0 ? n + 1 : n + g.Nx;
-                R ph = R(0);
-                if (abei) ph += d * abei[k];
-                if (ab) ph += d * ab[k];
-                const cplx<R> p1 = ld(psi, nb), d1 = ld(dpsi, nb);
-                const cplx<R> zp = grad_c(p0, ph, p1), zd = grad_c(d0, ph, d1);
-                const R zr = zp.re * zd.re - (-zp.im) * zd.im;  // Re(conj(zp) zd)
-                const R w = r * id2;
-                if (L == 5) {
-                    C[2] += w * g_grad(d0, ph, d1);
-                    C[1] += w * R(2) * zr;
-                    C[0] += w * g_grad(p0, ph, p1);
-                } else {
-                    const R dph = d * dab[k], dph2 = dph * dph;
-                    C[0] += w * g_grad(p0, ph, p1);
-                    C[5] += w * R(2) * zr;
-                    C[10] += w * g_grad(d0, ph, d1);
-                    R s, cs;
-                    sincos_r(ph, &s, &cs);  // U(-ph) = (cos ph, sin ph)
-                    auto mulU = [&](cplx<R> a, cplx<R> b) -> cplx<R> {  // a * U(-ph) * conj(b)
-                        const cplx<R> t = {a.re * cs - a.im * s, a.re * s + a.im * cs};
-                        return {t.re * b.re - t.im * (-b.im), t.re * (-b.im) + t.im * b.re};
-                    };
-                    const cplx<R> z0 = mulU(p0, p1);
-                    const cplx<R> za = mulU(p0, d1), zb = mulU(d0, p1);
-                    const cplx<R> z1 = {za.re + zb.re, za.im + zb.im};
-                    const cplx<R> z2 = mulU(d0, d1);
-                    const cplx<R> zz[3] = {z0, z1, z2};
-#pragma unroll
-                    for (int m = 0; m < 3; ++m) {
-                        C[5 * m + 1] += w * R(2) * zz[m].im * dph;
-                        C[5 * m + 2] += w * zz[m].re * dph2;
-                        C[5 * m + 3] += -r * id2 / R(3) * zz[m].im * dph2 * dph;
-                        C[5 * m + 4] += -r * id2 / R(12) * zz[m].re * dph2 * dph2;
-                    }
-                }
-            }
-        }
-        if (kappa2 > R(0) && i < g.Nxc && j < g.Nyc) {
-            const long a00 = i + (long)g.Nxa * j, a0p = i + (long)g.Nxa * (j + 1);
-            const long b00 = g.Na + i + (long)g.Nxb * j, bp0 = g.Na + i + 1 + (long)g.Nxb * j;
-            if (L == 5) {
-                const R dB = q.idx * (ab[bp0] - ab[b00]) - q.idy * (ab[a0p] - ab[a00]) - H;
-                C[0] += kappa2 * dB * dB;
-            } else {
-                const R BH = q.idx * (pv(abei, ab, bp0) - pv(abei, ab, b00)) - q.idy * (pv(abei, ab, a0p) - pv(abei, ab, a00)) - H;
-                const R dB = q.idx * (dab[bp0] - dab[b00]) - q.idy * (dab[a0p] - dab[a00]);
-                C[0] += kappa2 * BH * BH;
-                C[1] += kappa2 * R(2) * BH * dB;
-                C[2] += kappa2 * dB * dB;
-            }
-        }
+        coef_node<R, L>(g, q, kappa2, eps, (const R *)nullptr, H, mt, psi, dpsi, abei, ab, dab, n, C);
 #pragma unroll
         for (int k = 0; k < L; ++k) acc[k] += double(C[k]);
     }

@@ -8,6 +8,7 @@
 #include "glx_fields.hpp"
 #include "glx_launch.hpp"
 #include "glx_td.hpp"
+#include "glx_batch_cg.hpp"
 
 namespace glx {
 
@@ -501,6 +502,100 @@ int L<R>::batch_td_run(const Env &e, glx_batch_instance *inst, int32_t n, int32_
     if constexpr (RunForm<R>::has_gauge)
         if (any_A) return batch_run_form<R, true>(e, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows);
     return batch_run_form<R, false>(e, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows);
+}
+
+// ---- batched CG -----------------------------------------------------------------
+
+template <typename R, int LL>
+static int batch_cg_form(const Env &e, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,
+                         int32_t per_launch) {
+    const bool withA = LL == 17;
+    // scratch per instance: jac[2], dir (N complex each) and, with A, jacA[2], dirA (Nab reals each)
+    std::vector<size_t> off(n + 1, 0);
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    for (int i = 0; i < n; ++i) {
+        const glx_grid &g = inst[i].g;
+        const size_t N = (size_t)g.Nx * g.Ny, Nab = (size_t)(g.Nx - 1) * g.Ny + (size_t)g.Nx * (g.Ny - 1);
+        off[i + 1] = off[i] + 3 * al(N * sizeof(cplx<R>)) + (withA ? 3 * al(Nab * sizeof(R)) : 0);
+    }
+    const size_t s_inst = al(sizeof(CGInst<R>) * n), s_state = al(sizeof(double) * 8 * n);
+    char *blk = nullptr;
+    hipError_t st = hipMalloc((void **)&blk, s_inst + s_state + off[n]);
+    if (st != hipSuccess) return check_hip(st, "hipMalloc(batch cg)");
+    struct Free { char *p; ~Free() { if (p) (void)hipFree(p); } } guard{blk};
+    CGInst<R> *d_inst = (CGInst<R> *)blk;
+    double *d_state = (double *)(blk + s_inst);
+    char *work = blk + s_inst + s_state;
+    std::vector<CGInst<R>> h(n);
+    for (int i = 0; i < n; ++i) {
+        const glx_batch_cg_instance &I = inst[i];
+        const glx_grid &g = I.g;
+        const size_t N = (size_t)g.Nx * g.Ny, Nab = (size_t)(g.Nx - 1) * g.Ny + (size_t)g.Nx * (g.Ny - 1);
+        CGInst<R> &c = h[i];
+        c.g = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+        c.kappa2 = R(I.kappa2);
+        c.eps = R(I.eps);
+        c.H = R(I.H);
+        c.eps_map = (const R *)I.eps_map;
+        c.mt = I.mt;
+        c.psi = (cplx<R> *)I.psi;
+        c.ab = withA ? (R *)I.ab : nullptr;
+        char *w = work + off[i];
+        for (int k = 0; k < 2; ++k) { c.jac[k] = (cplx<R> *)w; w += al(N * sizeof(cplx<R>)); }
+        c.dir = (cplx<R> *)w; w += al(N * sizeof(cplx<R>));
+        for (int k = 0; k < 2; ++k) { c.jacA[k] = withA ? (R *)w : nullptr; if (withA) w += al(Nab * sizeof(R)); }
+        c.dirA = withA ? (R *)w : nullptr;
+    }
+    st = hipMemcpyAsync(d_inst, h.data(), sizeof(CGInst<R>) * n, hipMemcpyHostToDevice, e.stream);
+    if (st == hipSuccess) st = hipMemsetAsync(d_state, 0, sizeof(double) * 8 * n + off[n], e.stream);
+    if (st != hipSuccess) return check_hip(st, "glx_batch_cg_run upload");
+    int per_cu = 0;
+    st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_cg<R, LL>, 256, 0);
+    if (st != hipSuccess) return check_hip(st, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    const int grid = std::max(1, std::min(n, per_cu * e.num_cus));
+    const int chunk = per_launch > 0 ? per_launch : 50;
+    std::vector<double> hs(8 * (size_t)n);
+    st = hipEventRecord(e.kev0, e.stream);
+    if (st != hipSuccess) return check_hip(st, "hipEventRecord");
+    for (int done = 0; done < max_iters; done += chunk) {
+        hipLaunchKernelGGL((k_batch_cg<R, LL>), dim3(grid), dim3(256), 0, e.stream, d_inst, d_state, n,
+                           std::min(chunk, max_iters - done), max_iters, rtol);
+        GLX_LAUNCH_CHECK("k_batch_cg");
+        st = hipMemcpyAsync(hs.data(), d_state, sizeof(double) * 8 * n, hipMemcpyDeviceToHost, e.stream);
+        if (st == hipSuccess) st = hipStreamSynchronize(e.stream);
+        if (st != hipSuccess) return check_hip(st, "glx_batch_cg_run readback");
+        bool all = true;
+        for (int i = 0; i < n && all; ++i) all = hs[8 * i + CG_DONE] != 0.0 || hs[8 * i + CG_IT] >= max_iters;
+        if (all) break;
+    }
+    st = hipEventRecord(e.kev1, e.stream);
+    if (st == hipSuccess) st = hipStreamSynchronize(e.stream);
+    if (st != hipSuccess) return check_hip(st, "hipEventRecord");
+    float kms = 0.f;
+    if (hipEventElapsedTime(&kms, e.kev0, e.kev1) == hipSuccess && e.last_kernel_ms) *e.last_kernel_ms = kms;
+    for (int i = 0; i < n; ++i) {
+        inst[i].iters = (int32_t)hs[8 * i + CG_IT];
+        inst[i].converged = hs[8 * i + CG_DONE] != 0.0 ? 1 : 0;
+        inst[i].grad_rel = hs[8 * i + CG_GREL];
+    }
+    return GLX_OK;
+}
+
+template <typename R>
+int L<R>::batch_cg_run(const Env &e, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,
+                       int32_t iters_per_launch) {
+    if (!inst || n < 1 || max_iters < 0 || !(rtol >= 0.0)) return set_error(GLX_E_ARG, "glx_batch_cg_run: invalid arguments");
+    const bool withA = inst[0].kappa2 > 0.0;
+    for (int i = 0; i < n; ++i) {
+        const glx_batch_cg_instance &I = inst[i];
+        if (!grid_ok(I.g) || !I.psi) return set_error(GLX_E_ARG, "glx_batch_cg_run: instance %d: invalid grid or psi", i);
+        if ((I.kappa2 > 0.0) != withA)
+            return set_error(GLX_E_ARG, "glx_batch_cg_run: instances must all have kappa < inf or all kappa = inf");
+        if (withA && !I.ab) return set_error(GLX_E_ARG, "glx_batch_cg_run: instance %d: NULL vector potential", i);
+    }
+    if (max_iters == 0) return GLX_OK;
+    if (withA) return batch_cg_form<R, 17>(e, inst, n, max_iters, rtol, iters_per_launch);
+    return batch_cg_form<R, 5>(e, inst, n, max_iters, rtol, iters_per_launch);
 }
 
 template <typename R> int L<R>::phase_lock(const Env &e, void *psi, int32_t n, const int32_t *ns) {
