@@ -158,6 +158,37 @@ def run_batch(args):
     return res
 
 
+def run_batch_cg(args):
+    """Config 5: n instances of 257^2 (kappa = 5) minimised by the batched
+    CG, FP32-mixed then FP64, up to --cg-batch-iters iterations with the
+    ragged stop ||grad F|| <= 1e-8 ||grad F||_0."""
+    from svirl_amd import workloads
+    from svirl_amd.batch import BatchCG
+    res = {"instances": args.cg_batch_n, "max_iters": args.cg_batch_iters, "rtol": 1e-8}
+    F = {}
+    for name, dtype in (("fp32", np.float32), ("fp64", np.float64)):
+        log(f"[batch-cg] building {args.cg_batch_n} Config 5 instances ({name})")
+        sims = workloads.config5(n=args.cg_batch_n, dtype=dtype)
+        nodes = sum(int(s.cfg.N) for s in sims)
+        b = BatchCG(sims)
+        b.cg(max_iter=2, rtol=0.0)  # warm-up (two iterations from the random state)
+        t0 = time.perf_counter()
+        iters, conv, grel = b.cg(max_iter=args.cg_batch_iters, rtol=1e-8)
+        t = time.perf_counter() - t0
+        total = int(iters.sum())
+        res[name] = {"instance_iters_per_s": total / t, "node_iters_per_s": float(np.sum(iters * (nodes / len(sims)))) / t,
+                     "wall_s": t, "converged": int(conv.sum()), "iters_min": int(iters.min()),
+                     "iters_max": int(iters.max()), "grad_rel_median": float(np.median(grel))}
+        F[name] = np.array([s.observables.free_energy_fp64() for s in sims[:64]])
+        log(f"[batch-cg] {name}: {total / t:.4g} instance-iterations/s, {res[name]['node_iters_per_s']:.4g} "
+            f"node-iterations/s, {int(conv.sum())} converged, median grad_rel {np.median(grel):.2e}")
+        del sims, b
+    res["fp32_vs_fp64_F_rel_max"] = float(np.max(np.abs(F["fp32"] - F["fp64"]) / np.abs(F["fp64"])))
+    res["fp32_vs_fp64_F_rel_median"] = float(np.median(np.abs(F["fp32"] - F["fp64"]) / np.abs(F["fp64"])))
+    log(f"[batch-cg] FP32 vs FP64 free energy (first 64 instances): max rel {res['fp32_vs_fp64_F_rel_max']:.2e}")
+    return res
+
+
 def calibrate_copy(n=1 << 26, reps=3):
     """FP32 axpy over n reals (reads 8n bytes, writes 4n), for scaling the
     rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
@@ -219,6 +250,9 @@ def main():
     ap.add_argument("--batch-n", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=60)
     ap.add_argument("--batch-solo-steps", type=int, default=10)
+    ap.add_argument("--skip-batch-cg", action="store_true")
+    ap.add_argument("--cg-batch-n", type=int, default=1024)
+    ap.add_argument("--cg-batch-iters", type=int, default=300)
     ap.add_argument("--calibrate", action="store_true",
                     help="run FP32 axpy dispatches of known size first (counter calibration)")
     ap.add_argument("--cpu-L", type=float, default=512.0)
@@ -284,6 +318,9 @@ def main():
     batch = None
     if not args.skip_batch:
         batch = run_batch(args)
+    batch_cg = None
+    if not args.skip_batch_cg:
+        batch_cg = run_batch_cg(args)
 
     cpu = None
     if not args.skip_cpu:
@@ -334,6 +371,8 @@ def main():
     }
     if batch:
         out["batch"] = batch
+    if batch_cg:
+        out["batch_cg"] = batch_cg
     if "cg_iters_per_s" in r32:
         out["cg"] = {"fp32_iters_per_s": r32["cg_iters_per_s"], "fp32_ms_per_iter": r32["cg_ms_per_iter"]}
     if "fp64" in res:

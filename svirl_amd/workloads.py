@@ -72,3 +72,20 @@ def config4(n=256, dtype=np.float32, seed=40, sizes=(32, 64, 128, 256, 512), dx=
         gl.params.linear_coefficient = pinning_field(int(c.Nx), int(c.Ny), dx, seed=seed * 1000 + k)
         sims.append(gl)
     return sims
+
+
+def config5(n=1024, dtype=np.float32, seed=50, L=128.0, dx=0.5, kappa=5.0):
+    """Config 5: n instances of (2L/dx+1)^2 nodes (257^2 by default) for a
+    batched CG minimisation: kappa = 5, H uniform in [0.05, 0.3], each with
+    its own seeded random psi and pinning field eps(r) (std 0.2)."""
+    from svirl_amd import GLSolver
+    rng = np.random.default_rng(seed)
+    sims = []
+    for k in range(n):
+        H = float(rng.uniform(0.05, 0.3))
+        gl = GLSolver(Lx=L, Ly=L, dx=dx, dy=dx, order_parameter='random', random_seed=seed * 10000 + k,
+                      gl_parameter=kappa, homogeneous_external_field=H, dtype=dtype)
+        c = gl.cfg
+        gl.params.linear_coefficient = pinning_field(int(c.Nx), int(c.Ny), dx, seed=seed * 10000 + k)
+        sims.append(gl)
+    return sims
