@@ -197,6 +197,25 @@ typedef struct glx_batch_cg_instance {
 int glx_batch_cg_run(int prec, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,
                      int32_t iters_per_launch);
 
+/* One psi Jacobi solve to the reference's stop rule, on the device
+ * (TD.__iterate_order_parameter_gpu, td.py:157-218: the sweep loop, its
+ * r2max read-back per sweep replaced by the device barrier). ab is read
+ * only; psi_bufs / *psi_cur as in glx_td_run; *sweeps receives the sweep
+ * count. p->eqn is ignored. */
+int glx_td_psi_solve(int prec, glx_grid g, const glx_td_params *p, const void *eps_map, const uint8_t *mt,
+                     const void *ab, void *const psi_bufs[3], int32_t *psi_cur, void *rhs_psi, int32_t *sweeps);
+
+/* One A Jacobi solve (TD.__iterate_vector_potential_gpu, td.py:252-325),
+ * with the reference's source aliasing A^(2*floor(j/2)); psi is read only. */
+int glx_td_a_solve(int prec, glx_grid g, const glx_td_params *p, const uint8_t *mt, const void *psi,
+                   void *const ab_bufs[3], int32_t *ab_cur, void *rhs_ab, int32_t *sweeps);
+
+/* One TDGL step, psi then (if p->solveA) A (TD.__iterate, td.py:356-361);
+ * sweeps[0..1] = (S_psi, S_A). Equivalent to glx_td_run with nsteps = 1. */
+int glx_td_step(int prec, glx_grid g, const glx_td_params *p, const void *eps_map, const uint8_t *mt,
+                void *const psi_bufs[3], int32_t *psi_cur, void *const ab_bufs[3], int32_t *ab_cur,
+                void *rhs_psi, void *rhs_ab, int32_t *sweeps);
+
 /* Duration (ms, HIP events on the library stream) of the persistent kernel
  * of the last glx_td_run call, excluding its reset and read-back. */
 int glx_td_last_kernel_ms(float *ms);

@@ -74,6 +74,12 @@ _PROTOS = {
     "glx_td_run": [c_int, Grid, ctypes.POINTER(TDParams), c_void_p, c_void_p, ctypes.POINTER(c_void_p), P_i32,
                    ctypes.POINTER(c_void_p), P_i32, c_void_p, c_void_p, c_i32, P_i32, c_i32, c_i32],
     "glx_td_max_grid": [c_int, c_i32, P_i32],
+    "glx_td_psi_solve": [c_int, Grid, ctypes.POINTER(TDParams), c_void_p, c_void_p, c_void_p,
+                         ctypes.POINTER(c_void_p), P_i32, c_void_p, P_i32],
+    "glx_td_a_solve": [c_int, Grid, ctypes.POINTER(TDParams), c_void_p, c_void_p, ctypes.POINTER(c_void_p), P_i32,
+                       c_void_p, P_i32],
+    "glx_td_step": [c_int, Grid, ctypes.POINTER(TDParams), c_void_p, c_void_p, ctypes.POINTER(c_void_p), P_i32,
+                    ctypes.POINTER(c_void_p), P_i32, c_void_p, c_void_p, P_i32],
     "glx_td_last_kernel_ms": [ctypes.POINTER(ctypes.c_float)],
     "glx_td_profile": [c_i32],
     "glx_batch_td_run": [c_int, ctypes.POINTER(BatchInstance), c_i32, c_i32, P_i32, c_i32, c_i32, c_i32],

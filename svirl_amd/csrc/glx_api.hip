@@ -226,6 +226,42 @@ int glx_td_run(int prec, glx_grid g, const glx_td_params *p, const void *eps_map
                               nsteps, sweeps, wg, wg_size));
 }
 
+int glx_td_psi_solve(int prec, glx_grid g, const glx_td_params *p, const void *eps_map, const uint8_t *mt,
+                     const void *ab, void *const psi_bufs[3], int32_t *psi_cur, void *rhs_psi, int32_t *sweeps) {
+    if (!p || !ab || !sweeps) return set_error(GLX_E_ARG, "glx_td_psi_solve: NULL argument");
+    glx_td_params q = *p;
+    q.eqn = 1;
+    q.langevin_A = 0.0;
+    void *const Q[3] = {const_cast<void *>(ab), const_cast<void *>(ab), const_cast<void *>(ab)};
+    int32_t qc = 0, sw[2] = {0, 0};
+    const int rc = glx_td_run(prec, g, &q, eps_map, mt, psi_bufs, psi_cur, Q, &qc, rhs_psi, nullptr, 1, sw, 0, 256);
+    if (rc == GLX_OK) *sweeps = sw[0];
+    return rc;
+}
+
+int glx_td_a_solve(int prec, glx_grid g, const glx_td_params *p, const uint8_t *mt, const void *psi,
+                   void *const ab_bufs[3], int32_t *ab_cur, void *rhs_ab, int32_t *sweeps) {
+    if (!p || !psi || !sweeps) return set_error(GLX_E_ARG, "glx_td_a_solve: NULL argument");
+    glx_td_params q = *p;
+    q.eqn = 2;
+    q.solveA = 1;
+    q.langevin_psi = 0.0;
+    void *const P[3] = {const_cast<void *>(psi), const_cast<void *>(psi), const_cast<void *>(psi)};
+    int32_t pc = 0, sw[2] = {0, 0};
+    const int rc = glx_td_run(prec, g, &q, nullptr, mt, P, &pc, ab_bufs, ab_cur, nullptr, rhs_ab, 1, sw, 0, 256);
+    if (rc == GLX_OK) *sweeps = sw[1];
+    return rc;
+}
+
+int glx_td_step(int prec, glx_grid g, const glx_td_params *p, const void *eps_map, const uint8_t *mt,
+                void *const psi_bufs[3], int32_t *psi_cur, void *const ab_bufs[3], int32_t *ab_cur,
+                void *rhs_psi, void *rhs_ab, int32_t *sweeps) {
+    if (!p) return set_error(GLX_E_ARG, "glx_td_step: NULL parameters");
+    glx_td_params q = *p;
+    q.eqn = 0;
+    return glx_td_run(prec, g, &q, eps_map, mt, psi_bufs, psi_cur, ab_bufs, ab_cur, rhs_psi, rhs_ab, 1, sweeps, 0, 256);
+}
+
 int glx_td_last_kernel_ms(float *ms) {
     GLX_REQUIRE_INIT();
     if (!ms) return set_error(GLX_E_ARG, "glx_td_last_kernel_ms: NULL");

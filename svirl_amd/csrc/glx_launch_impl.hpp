@@ -284,7 +284,7 @@ int L<R>::td_run(const Env &e, glx_grid g, const glx_td_params *p, const void *e
     int32_t maxg = 0;
     int rc = td_max_grid(e, wg_size, &maxg);
     if (rc) return rc;
-    if (wg <= 0) wg = maxg;
+    if (wg <= 0) wg = std::min(maxg, 2 * e.num_cus);  // two workgroups per CU: the measured optimum
     if (wg > maxg) return set_error(GLX_E_ARG, "glx_td_run: grid %d exceeds co-resident limit %d", wg, maxg);
     const bool evolves_A = p->solveA && p->eqn != 1;
     if constexpr (RunForm<R>::has_gauge)

@@ -122,3 +122,45 @@ def test_cg_kinf_13_through_api():
     gl.solve.cg(n_iter=8)
     assert np.allclose(gl.solve._cg.cg_energies, f["energies"], rtol=1e-11)
     assert np.max(np.abs(gl.flatten_array(gl.vars.order_parameter) - f["psi"])) < 1e-10
+
+
+def test_solve_and_step_entry_points_match_td_run():
+    """glx_td_psi_solve then glx_td_a_solve, and glx_td_step, give bit for
+    bit the state and sweep counts of one glx_td_run step (FP64)."""
+    f = load("td_k5_mask")
+    psi_ref, ab_ref, sw_ref = run_td(f, 1, float(f["kappa"]) ** 2, 1.0 / float(f["sigma"]), float(f["H"]), 0.0,
+                                     f["eps_map"], f["mt"])
+    g = nat.Grid(int(f["Nx"]), int(f["Ny"]), float(f["dx"]), float(f["dy"]))
+    N = int(f["Nx"]) * int(f["Ny"])
+    prm = nat.TDParams()
+    prm.dt, prm.eps, prm.kappa2, prm.rho, prm.H = float(f["dt"]), 0.0, float(f["kappa"]) ** 2, 1.0 / float(f["sigma"]), float(f["H"])
+    prm.stop_psi, prm.stop_A = 1e-6, 1e-6
+    prm.langevin_psi = prm.langevin_A = 0.0
+    prm.rand_t, prm.solveA, prm.max_sweeps, prm.eqn = 1, 1, 1024, 0
+    em = D(f["eps_map"], np.float64)
+    mt = D(f["mt"].astype(np.uint8), np.uint8)
+    # psi solve, then A solve
+    P = [D(f["psi0"], np.complex128)] + [nat.DeviceArray.zeros(N, np.complex128) for _ in range(2)]
+    Q = [D(f["ab0"], np.float64)] + [nat.DeviceArray.zeros(f["ab0"].size, np.float64) for _ in range(2)]
+    Pp = (ctypes.c_void_p * 3)(*[b.ptr for b in P])
+    Qp = (ctypes.c_void_p * 3)(*[b.ptr for b in Q])
+    pc, qc, sp, sa = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(), ctypes.c_int32()
+    nat.call("glx_td_psi_solve", nat.GLX_F64, g, ctypes.byref(prm), em.ptr, mt.ptr, Q[0].ptr, Pp, ctypes.byref(pc),
+             None, ctypes.byref(sp))
+    nat.call("glx_td_a_solve", nat.GLX_F64, g, ctypes.byref(prm), mt.ptr, P[pc.value].ptr, Qp, ctypes.byref(qc),
+             None, ctypes.byref(sa))
+    assert (sp.value, sa.value) == tuple(sw_ref[0])
+    assert np.array_equal(P[pc.value].get(), psi_ref)
+    assert np.array_equal(Q[qc.value].get(), ab_ref)
+    # one step
+    P2 = [D(f["psi0"], np.complex128)] + [nat.DeviceArray.zeros(N, np.complex128) for _ in range(2)]
+    Q2 = [D(f["ab0"], np.float64)] + [nat.DeviceArray.zeros(f["ab0"].size, np.float64) for _ in range(2)]
+    Pp2 = (ctypes.c_void_p * 3)(*[b.ptr for b in P2])
+    Qp2 = (ctypes.c_void_p * 3)(*[b.ptr for b in Q2])
+    pc2, qc2 = ctypes.c_int32(0), ctypes.c_int32(0)
+    sw = (ctypes.c_int32 * 2)()
+    nat.call("glx_td_step", nat.GLX_F64, g, ctypes.byref(prm), em.ptr, mt.ptr, Pp2, ctypes.byref(pc2), Qp2,
+             ctypes.byref(qc2), None, None, sw)
+    assert tuple(sw) == tuple(sw_ref[0])
+    assert np.array_equal(P2[pc2.value].get(), psi_ref)
+    assert np.array_equal(Q2[qc2.value].get(), ab_ref)
