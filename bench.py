@@ -109,7 +109,7 @@ def run_batch(args):
     sims = workloads.config4(n=args.batch_n, dtype=np.float32)
     nodes = sum(int(s.cfg.N) for s in sims)
     b = BatchTD(sims, use_graph=True, autotune=True)
-    b.td(dt=0.1, Nt=b.tune_steps * len(CANDIDATES) + 1)  # autotune on the first steps
+    b.td(dt=0.1, Nt=b.tune_steps * (len(CANDIDATES) + 1) + 1)  # autotune on the first steps
     nat.call("glx_synchronize")
     t0 = time.perf_counter()
     b.td(dt=0.1, Nt=args.batch_steps)

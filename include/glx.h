@@ -155,16 +155,18 @@ typedef struct glx_batch_instance {
 
 /* Advance n instances by nsteps TD steps. Every step is ONE persistent
  * launch in which each instance is swept by its own team of co-resident
- * workgroups (sized to its grid) with team-local barriers, so instances
- * stop their Jacobi solves independently (ragged sweep counts). With
- * use_graph != 0, the step launches are captured once into a HIP graph and
- * replayed. wg_per_cu (co-resident workgroups per CU, 0 = as many as fit)
- * and min_rows (fewest rows per wave when sizing teams, 0 = 16) are the
- * tunable launch shape. sweeps (host, nsteps*n*2 int32, may be NULL)
- * receives (S_psi, S_A) per step and instance. Instances must not share
- * buffers; n must not exceed the number of co-resident workgroups. */
+ * workgroups with team-local barriers, so instances stop their Jacobi
+ * solves independently (ragged sweep counts). Teams are sized in
+ * proportion to each instance's wave-rows times weight[i] (its relative
+ * sweeps per step, e.g. from an earlier run; NULL = 1). With use_graph != 0,
+ * the step launches are captured once into a HIP graph and replayed.
+ * wg_per_cu (co-resident workgroups per CU, 0 = as many as fit) and
+ * min_rows (fewest rows per wave when sizing teams, 0 = 16) are the tunable
+ * launch shape. sweeps (host, nsteps*n*2 int32, may be NULL) receives
+ * (S_psi, S_A) per step and instance. Instances must not share buffers; n
+ * must not exceed the number of co-resident workgroups. */
 int glx_batch_td_run(int prec, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
-                     int32_t use_graph, int32_t wg_per_cu, int32_t min_rows);
+                     int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight);
 
 /* ------------------------------------------------------------ batched CG */
 

@@ -12,7 +12,9 @@ The launch shape (workgroups per CU, fewest rows per wave when sizing the
 teams) is picked by a small autotuner: the first steps of a run time each
 candidate shape on the real state (the results do not depend on the shape)
 and the rest of the run uses the fastest; the choice is cached per batch
-signature.
+signature. Teams are sized by each instance's wave-rows times its sweeps
+per step in the previous run, so instances that iterate more get more
+workgroups.
 
     from svirl_amd import GLSolver
     from svirl_amd.batch import BatchTD
@@ -56,6 +58,7 @@ class BatchTD:
         self.shape = None
         self.timings = {}
         self.last_kernel_ms = 0.0
+        self.weights = None   # per-instance sweeps per step of the last run (team sizing)
 
     def _signature(self):
         return tuple((int(s.cfg.Nx), int(s.cfg.Ny), bool(s.params.solveA)) for s in self.solvers) + (self.prec,)
@@ -92,11 +95,17 @@ class BatchTD:
             I.ab_cur = qc
             bufs.append((P, Q))
         sw = (ctypes.c_int32 * (2 * n * nsteps))()
-        nat.call("glx_batch_td_run", self.prec, arr, n, nsteps, sw, 1 if self.use_graph else 0, shape[0], shape[1])
+        wt = None
+        if self.weights is not None:
+            wt = np.ascontiguousarray(self.weights, dtype=np.float64)
+        nat.call("glx_batch_td_run", self.prec, arr, n, nsteps, sw, 1 if self.use_graph else 0, shape[0], shape[1],
+                 wt.ctypes.data if wt is not None else None)
         kms = ctypes.c_float()
         nat.call("glx_td_last_kernel_ms", ctypes.byref(kms))
         self.last_kernel_ms = kms.value
         sweeps = np.frombuffer(sw, dtype=np.int32).reshape(nsteps, n, 2)
+        # the next run sizes the teams by each instance's sweeps per step
+        self.weights = np.maximum(sweeps.sum(axis=2).mean(axis=0), 1.0)
         for k, td in enumerate(tds):
             P, Q = bufs[k]
             td._finish(P, arr[k].psi_cur, Q, arr[k].ab_cur, sweeps[:, k, :], 0)
@@ -110,7 +119,10 @@ class BatchTD:
         sig = self._signature()
         if self.shape is None:
             self.shape = _TUNED.get(sig)
-        if self.shape is None and self.autotune and Nt >= self.tune_steps * len(CANDIDATES) + 1:
+        if self.shape is None and self.autotune and Nt >= self.tune_steps * (len(CANDIDATES) + 1) + 1:
+            if self.weights is None:  # learn the per-instance sweep counts first
+                self._run(tds, self.tune_steps, CANDIDATES[0])
+                done += self.tune_steps
             for cand in CANDIDATES:
                 ms = self._run(tds, self.tune_steps, cand)
                 self.timings[cand] = ms / self.tune_steps

@@ -270,8 +270,8 @@ int glx_td_last_kernel_ms(float *ms) {
 }
 
 int glx_batch_td_run(int prec, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
-                     int32_t use_graph, int32_t wg_per_cu, int32_t min_rows) {
-    GLX_DISPATCH(prec, batch_td_run(g_ctx.env, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows));
+                     int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight) {
+    GLX_DISPATCH(prec, batch_td_run(g_ctx.env, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows, weight));
 }
 
 int glx_batch_cg_run(int prec, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,

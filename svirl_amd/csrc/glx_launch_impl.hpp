@@ -322,17 +322,19 @@ static TDSolveArgs<R> batch_args(const glx_batch_instance &I, bool gauge_form) {
     return T;
 }
 
-// Teams of consecutive workgroups, sized to each instance's wave-rows: the
-// largest scale f with sum_i clamp(floor(f*rows_i), 1, cap_i) <= G, where
-// cap_i keeps at least min_rows rows per wave.
-static void plan_teams(const std::vector<long> &rows, int G, int min_rows, std::vector<int> &nb) {
+// Teams of consecutive workgroups, sized to each instance's work (its
+// wave-rows times its relative sweep count): the largest scale f with
+// sum_i clamp(floor(f*work_i), 1, cap_i) <= G, where cap_i keeps at least
+// min_rows rows per wave.
+static void plan_teams(const std::vector<long> &rows, const std::vector<double> &work, int G, int min_rows,
+                       std::vector<int> &nb) {
     const size_t n = rows.size();
     nb.assign(n, 1);
     auto total = [&](double f, std::vector<int> *out) {
         long t = 0;
         for (size_t i = 0; i < n; ++i) {
             const long cap = std::max(1L, rows[i] / (4L * min_rows));
-            long k = (long)(f * (double)rows[i]);
+            long k = (long)(f * work[i]);
             k = std::min(std::max(k, 1L), cap);
             if (out) (*out)[i] = (int)k;
             t += k;
@@ -356,7 +358,7 @@ template <typename R, bool G> static int batch_occupancy(int *per_cu) {
 
 template <typename R, bool G>
 static int batch_run_form(const Env &e, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
-                          int32_t use_graph, int32_t wg_per_cu, int32_t min_rows) {
+                          int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight) {
     int per_cu = 0;
     int rc = batch_occupancy<R, G>(&per_cu);
     if (rc) return rc;
@@ -364,9 +366,13 @@ static int batch_run_form(const Env &e, glx_batch_instance *inst, int32_t n, int
     const int Gb = std::min(per_cu * e.num_cus, MAX_GRID_BLOCKS);
     if (n > Gb) return set_error(GLX_E_ARG, "glx_batch_td_run: %d instances exceed %d co-resident workgroups", n, Gb);
     std::vector<long> rows(n);
-    for (int i = 0; i < n; ++i) rows[i] = (long)nstrips_of<1>(inst[i].g.Nx) * inst[i].g.Ny;
+    std::vector<double> work(n);
+    for (int i = 0; i < n; ++i) {
+        rows[i] = (long)nstrips_of<1>(inst[i].g.Nx) * inst[i].g.Ny;
+        work[i] = (double)rows[i] * ((weight && weight[i] > 0.0) ? weight[i] : 1.0);
+    }
     std::vector<int> nb;
-    plan_teams(rows, Gb, min_rows > 0 ? min_rows : 16, nb);
+    plan_teams(rows, work, Gb, min_rows > 0 ? min_rows : 16, nb);
     std::vector<int> team_of_block;
     std::vector<int2> teams(n);
     for (int i = 0; i < n; ++i) {
@@ -480,7 +486,7 @@ static int batch_run_form(const Env &e, glx_batch_instance *inst, int32_t n, int
 
 template <typename R>
 int L<R>::batch_td_run(const Env &e, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
-                       int32_t use_graph, int32_t wg_per_cu, int32_t min_rows) {
+                       int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight) {
     if (!inst || n < 1 || nsteps < 0) return set_error(GLX_E_ARG, "glx_batch_td_run: invalid arguments");
     if ((long)n * nsteps > (1L << 26)) return set_error(GLX_E_ARG, "glx_batch_td_run: n*nsteps too large");
     bool any_A = false;
@@ -500,8 +506,8 @@ int L<R>::batch_td_run(const Env &e, glx_batch_instance *inst, int32_t n, int32_
     }
     if (nsteps == 0) return GLX_OK;
     if constexpr (RunForm<R>::has_gauge)
-        if (any_A) return batch_run_form<R, true>(e, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows);
-    return batch_run_form<R, false>(e, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows);
+        if (any_A) return batch_run_form<R, true>(e, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows, weight);
+    return batch_run_form<R, false>(e, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows, weight);
 }
 
 // ---- batched CG -----------------------------------------------------------------

@@ -139,6 +139,21 @@ class CG:
         r = scipy.optimize.minimize(f, x0=np.array(alpha0), jac=j, method='BFGS', tol=tol)
         return r.x
 
+    # -- hidden state (checkpoint / resume) ------------------------------------------------
+    def _state(self):
+        """The state that carries over between cg() calls in the reference
+        (cg.py:54-71): the search directions and the betas."""
+        d = {"cg_dir_psi": self.__gdir_psi.get(), "cg_beta": self.__gbeta.get()}
+        if self.params.solveA:
+            d["cg_dir_A"] = self.__gdir_A.get()
+        return d
+
+    def _set_state(self, d):
+        self.__gdir_psi.set(d["cg_dir_psi"])
+        self.__gbeta.set(d["cg_beta"])
+        if self.params.solveA and "cg_dir_A" in d:
+            self.__gdir_A.set(d["cg_dir_A"])
+
     # -- helpers ----------------------------------------------------------------------------
     @property
     def _beta_psi(self):

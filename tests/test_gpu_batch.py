@@ -88,6 +88,7 @@ def test_batch_equals_single_runs_and_continues():
     alone = [make(k, *c) for k, c in enumerate(CASES)]
     b = BatchTD(sims, use_graph=True, autotune=True, tune_steps=1)
     b.td(dt=0.1, Nt=7)
+    assert b.weights is not None
     b.td(dt=0.1, Nt=3)
     assert b.shape is not None
     for gl, g1 in zip(sims, alone):

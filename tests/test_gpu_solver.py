@@ -215,3 +215,30 @@ def test_td_eqn_modes_and_chunking():
     gl.solve.td(dt=0.1, Nt=8)
     st.td(dt=0.1, Nt=8)
     assert rel(gl_psi(gl), st.psi) < 1e-10 and rel(gl_ab(gl), st.ab) < 1e-10
+
+
+def test_snapshot_resume_is_bit_exact(tmp_path):
+    """Checkpoint after TD + CG, continue; a fresh solver loaded from the
+    checkpoint continues to the same bits (psi, A, energies, sweeps)."""
+    from svirl_amd import GLSolver, snapshot
+
+    def make():
+        return GLSolver(Nx=33, Ny=29, dx=0.5, dy=0.5, order_parameter='random', random_seed=4, gl_parameter=5.0,
+                        normal_conductivity=100.0, homogeneous_external_field=0.15, dtype=np.float64)
+    a = make()
+    a.solve.td(dt=0.1, Nt=5)
+    a.solve.cg(n_iter=3)
+    p = str(tmp_path / "snap.npz")
+    snapshot.save(a, p)
+    a.solve.td(dt=0.1, Nt=4)
+    a.solve.cg(n_iter=3)
+    b = make()
+    snapshot.load(b, p)
+    b.solve.td(dt=0.1, Nt=4)
+    b.solve.cg(n_iter=3)
+    assert np.array_equal(a.vars.order_parameter, b.vars.order_parameter)
+    aa, ab_ = a.vars.vector_potential
+    ba, bb = b.vars.vector_potential
+    assert np.array_equal(aa, ba) and np.array_equal(ab_, bb)
+    assert a.solve._cg.cg_energies == b.solve._cg.cg_energies
+    assert a.solve._td.sweeps[-4:] == b.solve._td.sweeps[-4:]
