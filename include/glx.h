@@ -302,6 +302,19 @@ int glx_cg_coef(int prec, glx_grid g, double kappa2, double eps, double H, const
  * axmy_r/axmy_c, svirl/cuda/utils.h:73-114; a complex array is 2N reals). */
 int glx_axpy(int prec, const void *x, const void *y, void *z, double alpha, int64_t n);
 int glx_axmy(int prec, const void *x, const void *y, void *z, double alpha, int64_t n);
+/* The gradient pass of one CG iteration, fused: both Jacobians
+ * (free_energy_jacobian_psi / _A, cg.h:15-301; written, which equals the
+ * reference's += into a zeroed buffer), the free energy of the state
+ * (observables.py:124-149, returned in *F, FP64) and the Polak-Ribiere sums
+ * against jac_prev / jacA_prev, whose betas max(num/den, 0) go to
+ * beta_next_dev[0..1] (device, FP64 rounded to the working precision,
+ * cg.py:422-449). kappa2 <= 0: psi only (jacA, jacA_prev unused; the
+ * psi Jacobian still reads the fixed vector potential). One
+ * pass over the state replaces the reference's gradient, beta and energy
+ * launches (cg.py:458-549). */
+int glx_cg_gradient_pass(int prec, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
+                         const uint8_t *mt, const void *psi, const void *abei, const void *ab, void *jac,
+                         const void *jac_prev, void *jacA, const void *jacA_prev, double *beta_next_dev, double *F);
 /* Polak-Ribiere beta left in device memory, as the reference keeps it
  * (__compute_beta_psi/_A, cg.py:422-449; divide_scalars_positive,
  * utils.h:139-146): *beta_dev = max(sum x*(x-y) / sum y*y, 0), sums in FP64,

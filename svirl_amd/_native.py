@@ -104,6 +104,8 @@ _PROTOS = {
     "glx_axpy": [c_int, c_void_p, c_void_p, c_void_p, c_double, c_i64],
     "glx_axmy": [c_int, c_void_p, c_void_p, c_void_p, c_double, c_i64],
     "glx_cg_pr_beta": [c_int, c_void_p, c_void_p, c_i64, c_void_p],
+    "glx_cg_gradient_pass": [c_int, Grid, c_double, c_double, c_void_p, c_double, c_void_p, c_void_p, c_void_p,
+                             c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double)],
     "glx_axmy_dev": [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_i64],
 }
 

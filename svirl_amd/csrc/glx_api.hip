@@ -354,6 +354,13 @@ int glx_cg_coef(int prec, glx_grid g, double kappa2, double eps, double H, const
     GLX_DISPATCH(prec, coef(g_ctx.env, g, kappa2, eps, H, mt, psi, dpsi, abei, ab, dab, c17));
 }
 
+int glx_cg_gradient_pass(int prec, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
+                         const uint8_t *mt, const void *psi, const void *abei, const void *ab, void *jac,
+                         const void *jac_prev, void *jacA, const void *jacA_prev, double *beta_next_dev, double *F) {
+    GLX_DISPATCH(prec, cg_gradient_pass(g_ctx.env, g, kappa2, eps, eps_map, H, mt, psi, abei, ab, jac, jac_prev, jacA,
+                                        jacA_prev, beta_next_dev, F));
+}
+
 int glx_cg_pr_beta(int prec, const void *jac, const void *jac_prev, int64_t n, double *beta_dev) {
     GLX_DISPATCH(prec, pr_beta(g_ctx.env, jac, jac_prev, n, beta_dev));
 }

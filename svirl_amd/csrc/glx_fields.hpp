@@ -444,6 +444,73 @@ __global__ void k_jacobian_A(Geo g, R kappa2, R H, const uint8_t *mt, const cplx
     }
 }
 
+// One pass of the CG loop over the state (phase 1 of an iteration): writes
+// both Jacobians (overwriting, equal to the reference's += into a zeroed
+// buffer), and accumulates the free energy of the state and the two pairs
+// of Polak-Ribiere sums against the previous Jacobians. partials per
+// workgroup: F, num_psi, den_psi, num_A, den_A. withA: kappa < inf.
+template <typename R, bool withA>
+__global__ void k_cg_gradient_pass(Geo g, R kappa2, R eps, const R *eps_map, R H, const uint8_t *mt,
+                                   const cplx<R> *psi, const R *abei, const R *ab, cplx<R> *jac,
+                                   const cplx<R> *jac_prev, R *jacA, const R *jacA_prev, double *partials) {
+    const GeoR<R> q(g);
+    const R dxdy = q.dx * q.dy, s2 = R(2) * q.dx * q.dy;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
+        acc[0] += double(fe_node(g, q, kappa2, eps, eps_map, H, mt, psi, abei, ab, n));
+        const cplx<R> gj = jac_psi_node(g, q, eps, eps_map, mt, psi, abei, ab, n);
+        const cplx<R> gv = {dxdy * gj.re, dxdy * gj.im};
+        const cplx<R> gp = ld(jac_prev, n);
+        st(jac, n, gv);
+        acc[1] += double(gv.re * (gv.re - gp.re));
+        acc[1] += double(gv.im * (gv.im - gp.im));
+        acc[2] += double(gp.re * gp.re);
+        acc[2] += double(gp.im * gp.im);
+        if (withA) {
+            R ga, gb;
+            bool ha, hb;
+            jac_A_node(g, q, kappa2, H, mt, psi, abei, ab, n, ga, gb, ha, hb);
+            const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
+            if (ha) {
+                const long la = i + (long)g.Nxa * j;
+                const R v = s2 * ga, vp = jacA_prev[la];
+                jacA[la] = v;
+                acc[3] += double(v * (v - vp));
+                acc[4] += double(vp * vp);
+            }
+            if (hb) {
+                const long lb = g.Na + i + (long)g.Nxb * j;
+                const R v = s2 * gb, vp = jacA_prev[lb];
+                jacA[lb] = v;
+                acc[3] += double(v * (v - vp));
+                acc[4] += double(vp * vp);
+            }
+        }
+    }
+    block_store_partials<5>(acc, partials);
+}
+
+// Finish of k_cg_gradient_pass: out[0] = dxdy * F (to be read by the host);
+// beta_next[0..1] = max(num/den, 0) for psi and A, in FP64 rounded to the
+// working precision (cg.py:422-449), left in device memory for the next
+// direction update.
+template <typename R>
+__global__ void k_cg_gradient_finish(const double *partials, int nb, double dxdy, double *out, double *beta_next) {
+    __shared__ double lds[RED_BLOCK / WAVE];
+    double v[5];
+    for (int k = 0; k < 5; ++k) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partials[(long)b * 5 + k];
+        v[k] = block_sum(s, lds);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = v[0] * dxdy;
+        beta_next[0] = double(R(fmax(v[1] / v[2], 0.0)));
+        beta_next[1] = double(R(fmax(v[3] / v[4], 0.0)));
+    }
+}
+
 // Line-search coefficients. L = 5 (cg.h:314-474) or 17 (cg.h:477-731).
 // One node's contributions are added to C (before the dxdy factor). The
 // reference's kernels read only the scalar eps (eps_map is NULL on the

@@ -767,6 +767,33 @@ int L<R>::coef(const Env &e, glx_grid g, double kappa2, double eps, double H, co
     return fetch_result(e, c17, 17);
 }
 
+template <typename R>
+int L<R>::cg_gradient_pass(const Env &e, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
+                           const uint8_t *mt, const void *psi, const void *abei, const void *ab, void *jac,
+                           const void *jac_prev, void *jacA, const void *jacA_prev, double *beta_next_dev, double *F) {
+    const bool withA = kappa2 > 0.0;  // the psi Jacobian reads A in both cases
+    if (!grid_ok(g) || !psi || !jac || !jac_prev || !beta_next_dev || !F)
+        return set_error(GLX_E_ARG, "glx_cg_gradient_pass: invalid arguments");
+    if (withA && (!(abei || ab) || !jacA || !jacA_prev))
+        return set_error(GLX_E_ARG, "glx_cg_gradient_pass: finite kappa needs the vector potential and its Jacobians");
+    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    const int nb = grid_blocks(G.N, RED_BLOCK, RED_MAX_BLOCKS);
+    if (withA)
+        hipLaunchKernelGGL((k_cg_gradient_pass<R, true>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, R(kappa2), R(eps),
+                           (const R *)eps_map, R(H), mt, (const cplx<R> *)psi, (const R *)abei, (const R *)ab,
+                           (cplx<R> *)jac, (const cplx<R> *)jac_prev, (R *)jacA, (const R *)jacA_prev, e.partials);
+    else
+        hipLaunchKernelGGL((k_cg_gradient_pass<R, false>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, R(kappa2), R(eps),
+                           (const R *)eps_map, R(H), mt, (const cplx<R> *)psi, (const R *)abei, (const R *)ab,
+                           (cplx<R> *)jac, (const cplx<R> *)jac_prev, (R *)nullptr, (const R *)nullptr, e.partials);
+    GLX_LAUNCH_CHECK("k_cg_gradient_pass");
+    const double dxdy = double(R(g.dx)) * double(R(g.dy));
+    hipLaunchKernelGGL(k_cg_gradient_finish<R>, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, dxdy, e.result,
+                       beta_next_dev);
+    GLX_LAUNCH_CHECK("k_cg_gradient_finish");
+    return fetch_result(e, F, 1);
+}
+
 template <typename R> int L<R>::pr_beta(const Env &e, const void *x, const void *y, int64_t n, double *beta_dev) {
     if (!x || !y || !beta_dev || n < 0) return set_error(GLX_E_ARG, "glx_cg_pr_beta: invalid arguments");
     const int nb = grid_blocks(n, RED_BLOCK, RED_MAX_BLOCKS);

@@ -33,8 +33,9 @@ class CG:
         self.__gdir_psi = DeviceArray.zeros(cfg.N, cd)
         self.__gjac_psi = DeviceArray.zeros(cfg.N, cd)
         self.__gjac_psi_prev = DeviceArray.zeros(cfg.N, cd)
-        # beta stays in device memory (one FP64 slot each), as in the reference
-        self.__gbeta = DeviceArray.zeros(2, np.float64)
+        # beta stays in device memory, as in the reference: [beta_psi, beta_A,
+        # staged beta_psi, staged beta_A] (FP64 slots)
+        self.__gbeta = DeviceArray.zeros(4, np.float64)
         if self.params.solveA:
             self.__gdir_A = DeviceArray.zeros(cfg.Nab, cfg.dtype)
             self.__gjac_A = DeviceArray.zeros(cfg.Nab, cfg.dtype)
@@ -183,56 +184,66 @@ class CG:
         nat.call("glx_axpy", self.par.prec, x.ptr, y.ptr, z.ptr, float(alpha), self._reals(x))
 
     # -- minimisation loops ----------------------------------------------------------------------
-    def __minimization_psi(self, n_iter=1000):
-        """cg.py:238-322 (kappa = inf)."""
-        assert not self.params.solveA
+    #
+    # Both loops follow cg.py:238-322 / 452-551 step for step, with the
+    # gradient, the energy and the Polak-Ribiere sums of a state fused into
+    # one pass (glx_cg_gradient_pass): the pass after the update of
+    # iteration i gives E_i for the stop test and the gradient and the
+    # (staged) beta of iteration i+1. The staged beta only becomes the
+    # solver's beta when iteration i+1 runs, so the state left behind when
+    # the loop stops is the reference's.
+    def __gradient_pass(self, withA):
+        a = self._args()
+        F = ctypes.c_double()
+        nat.call("glx_cg_gradient_pass", self.par.prec, self._grid(), a['k2'], a['eps'], a['epsmap'], a['H'], a['mt'],
+                 a['psi'], a['abei'], a['ab'], self.__gjac_psi.ptr, self.__gjac_psi_prev.ptr,
+                 self.__gjac_A.ptr if withA else None, self.__gjac_A_prev.ptr if withA else None,
+                 self._beta_ptr(2), ctypes.byref(F))
+        return F.value
+
+    def __commit_beta(self):
+        nat.call("glx_memcpy_d2d", self._beta_ptr(0), self._beta_ptr(2), 16)
+
+    def __loop(self, n_iter, withA):
         self.vars._psi.sync_d()
         self.vars._vp.sync_d()
         self.cg_energies = []
-        self.__gdir_psi.fill(0)
+        self.__gradient_pass(withA)  # gradient of the starting state
         for i in range(n_iter):
-            jac = self._free_energy_jacobian_psi
             if i > 0:
-                self._beta(jac, self.__gjac_psi_prev, 0)
-            self._axmy_beta(self.__gdir_psi, jac, self.__gdir_psi, 0)
-            self._free_energy_conjgrad_coef_psi(self.__gdir_psi)
-            alpha0 = self._cg_alpha_psi_min()
+                self.__commit_beta()
+            self._axmy_beta(self.__gdir_psi, self.__gjac_psi, self.__gdir_psi, 0)
             psi = self.vars.order_parameter_h()
-            self._axpy(self.__gdir_psi, psi, psi, self.cfg.dtype(alpha0))
+            if withA:
+                self._axmy_beta(self.__gdir_A, self.__gjac_A, self.__gdir_A, 1)
+                self._free_energy_conjgrad_coef(self.__gdir_psi, self.__gdir_A)
+                alpha_psi, alpha_A = self._cg_alpha_min()
+                ab = self.vars.vector_potential_h()
+                self._axpy(self.__gdir_psi, psi, psi, self.cfg.dtype(alpha_psi))
+                self._axpy(self.__gdir_A, ab, ab, self.cfg.dtype(alpha_A))
+                self.__gjac_A, self.__gjac_A_prev = self.__gjac_A_prev, self.__gjac_A
+            else:
+                self._free_energy_conjgrad_coef_psi(self.__gdir_psi)
+                alpha0 = self._cg_alpha_psi_min()
+                self._axpy(self.__gdir_psi, psi, psi, self.cfg.dtype(alpha0))
             self.__gjac_psi, self.__gjac_psi_prev = self.__gjac_psi_prev, self.__gjac_psi
-            E0 = self.observables.free_energy
+            E0 = self.cfg.dtype(self.__gradient_pass(withA))
             self.cg_energies.append(E0)
             if i > 0 and np.abs(self.cg_energies[i] / self.cg_energies[i - 1] - 1.0) < self.__convergence_rtol:
                 break
         self.vars._psi.need_dtoh_sync()
+        if withA:
+            self.vars._vp.need_dtoh_sync()
+
+    def __minimization_psi(self, n_iter=1000):
+        """cg.py:238-322 (kappa = inf): the direction restarts from zero."""
+        assert not self.params.solveA
+        self.__gdir_psi.fill(0)
+        self.__loop(n_iter, False)
 
     def __minimization(self, n_iter=1000):
-        """cg.py:452-551 (kappa < inf)."""
-        self.vars._psi.sync_d()
-        self.vars._vp.sync_d()
-        self.cg_energies = []
-        for i in range(n_iter):
-            jac = self._free_energy_jacobian_psi
-            jacA = self._free_energy_jacobian_A
-            if i > 0:
-                self._beta(jac, self.__gjac_psi_prev, 0)
-                self._beta(jacA, self.__gjac_A_prev, 1)
-            self._axmy_beta(self.__gdir_psi, jac, self.__gdir_psi, 0)
-            self._axmy_beta(self.__gdir_A, jacA, self.__gdir_A, 1)
-            self._free_energy_conjgrad_coef(self.__gdir_psi, self.__gdir_A)
-            alpha_psi, alpha_A = self._cg_alpha_min()
-            psi = self.vars.order_parameter_h()
-            ab = self.vars.vector_potential_h()
-            self._axpy(self.__gdir_psi, psi, psi, self.cfg.dtype(alpha_psi))
-            self._axpy(self.__gdir_A, ab, ab, self.cfg.dtype(alpha_A))
-            self.__gjac_psi, self.__gjac_psi_prev = self.__gjac_psi_prev, self.__gjac_psi
-            self.__gjac_A, self.__gjac_A_prev = self.__gjac_A_prev, self.__gjac_A
-            E0 = self.observables.free_energy
-            self.cg_energies.append(E0)
-            if i > 0 and np.abs(self.cg_energies[i] / self.cg_energies[i - 1] - 1.0) < self.__convergence_rtol:
-                break
-        self.vars._psi.need_dtoh_sync()
-        self.vars._vp.need_dtoh_sync()
+        """cg.py:452-551 (kappa < inf): directions and betas carry over."""
+        self.__loop(n_iter, True)
 
     def _solve(self, n_iter=1000):
         self._check_supported()
