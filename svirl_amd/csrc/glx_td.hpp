@@ -426,6 +426,45 @@ __device__ __forceinline__ ARow<R> a_load_row(const ASweepArgs<R> &A, const ColC
     return L;
 }
 
+// Backward-Euler Jacobi update of the a link (c, j) (td.h:366-410): its
+// supercurrent jl, the right-hand side ra, and the neighbouring links am =
+// a(c, j-1), bm = b(c, j-1), brm = b(c+1, j-1), ap = a(c, j+1), bc = b(c, j),
+// brc = b(c+1, j).
+template <typename R>
+__device__ __forceinline__ R a_link_update(const ASweepArgs<R> &A, int j, R ra, R jl, R am, R bm, R brm, R ap, R bc,
+                                           R brc) {
+    const int Ny = A.Ny;
+    R rh = R(0), dd = R(1);
+    if (j == 0) { rh = A.rh_a; dd = R(2); }
+    else if (j + 1 == Ny) { rh = -A.rh_a; dd = R(2); }
+    R t1 = R(0), t2 = R(0);
+    if (j > 0) t1 = A.idy2 * am - A.idxy * bm + A.idxy * brm;
+    if (j + 1 < Ny) t2 = A.idy2 * ap + A.idxy * bc - A.idxy * brc;
+    return (ra + A.dt_rho * (jl + rh) + A.dt_rho_kappa2 * dd * (t1 + t2)) / A.den_a;
+}
+
+// The b link (c, j) (td.h:412-456): blc = b(c-1, j), alc = a(c-1, j),
+// alp = a(c-1, j+1), brc = b(c+1, j), ac = a(c, j), ap = a(c, j+1).
+template <typename R>
+__device__ __forceinline__ R b_link_update(const ASweepArgs<R> &A, int c, R rb, R jl, R blc, R alc, R alp, R brc, R ac,
+                                           R ap) {
+    const int Nx = A.Nx;
+    R rh = R(0), dd = R(1);
+    if (c == 0) { rh = -A.rh_b; dd = R(2); }
+    else if (c + 1 == Nx) { rh = A.rh_b; dd = R(2); }
+    R t1 = R(0), t2 = R(0);
+    if (c > 0) t1 = A.idx2 * blc - A.idxy * alc + A.idxy * alp;
+    if (c + 1 < Nx) t2 = A.idx2 * brc + A.idxy * ac - A.idxy * ap;
+    return (rb + A.dt_rho * (jl + rh) + A.dt_rho_kappa2 * dd * (t1 + t2)) / A.den_b;
+}
+
+// Supercurrent of a link, id * Im(conj psi0 U psi1) in the reference's form.
+template <typename R>
+__device__ __forceinline__ R link_current(R id, cplx<R> p0, R x1, R y1, R cs, R sn) {
+    const R x0 = p0.re, y0 = p0.im;
+    return id * ((x0 * y1 - y0 * x1) * cs - (x0 * x1 + y0 * y1) * sn);
+}
+
 // One A sweep over rows [ja, jb) of strip s (both link families of every
 // owned node). Returns the lane's largest |delta|.
 template <typename R, bool GAUGE>
@@ -485,35 +524,17 @@ __device__ __forceinline__ R a_rows(const ASweepArgs<R> &A, int s, int ja, int j
         sincos_r(link_phase<R, GAUGE>(A.dy, cu.sb, gb), &ssb, &csb);
 
         if (owna) {
-            R rh = R(0), dd = R(1);
-            if (j == 0) { rh = A.rh_a; dd = R(2); }
-            else if (j + 1 == Ny) { rh = -A.rh_a; dd = R(2); }
-            R jl = R(0);
-            if (cellm != 0 || cellc) {  // cells (c, j-1), (c, j)
-                const R x0 = pc.re, y0 = pc.im, x1 = pr_re, y1 = pr_im;
-                jl = A.idx * ((x0 * y1 - y0 * x1) * csa - (x0 * x1 + y0 * y1) * ssa);
-            }
-            R t1 = R(0), t2 = R(0);
-            if (j > 0) t1 = A.idy2 * am - A.idxy * bm + A.idxy * brm;
-            if (j + 1 < Ny) t2 = A.idy2 * ap + A.idxy * bc - A.idxy * brc;
-            const R v = (ra + A.dt_rho * (jl + rh) + A.dt_rho_kappa2 * dd * (t1 + t2)) / A.den_a;
+            // cells (c, j-1), (c, j)
+            const R jl = (cellm != 0 || cellc) ? link_current(A.idx, pc, pr_re, pr_im, csa, ssa) : R(0);
+            const R v = a_link_update(A, j, ra, jl, am, bm, brm, ap, bc, brc);
             if (noisy && A.jstep == 0) A.rhs_out[na] = ra;
             A.ab_next[na] = v;
             rr = fmax(rr, fabs(v - ac));
         }
         if (ownb) {
-            R rh = R(0), dd = R(1);
-            if (c == 0) { rh = -A.rh_b; dd = R(2); }
-            else if (c + 1 == Nx) { rh = A.rh_b; dd = R(2); }
-            R jl = R(0);
-            if (cl_c || cellc) {  // cells (c-1, j), (c, j)
-                const R x0 = pc.re, y0 = pc.im, x1 = pp.re, y1 = pp.im;
-                jl = A.idy * ((x0 * y1 - y0 * x1) * csb - (x0 * x1 + y0 * y1) * ssb);
-            }
-            R t1 = R(0), t2 = R(0);
-            if (c > 0) t1 = A.idx2 * blc - A.idxy * alc + A.idxy * alp;
-            if (c + 1 < Nx) t2 = A.idx2 * brc + A.idxy * ac - A.idxy * ap;
-            const R v = (rb + A.dt_rho * (jl + rh) + A.dt_rho_kappa2 * dd * (t1 + t2)) / A.den_b;
+            // cells (c-1, j), (c, j)
+            const R jl = (cl_c || cellc) ? link_current(A.idy, pc, pp.re, pp.im, csb, ssb) : R(0);
+            const R v = b_link_update(A, c, rb, jl, blc, alc, alp, brc, ac, ap);
             if (noisy && A.jstep == 0) A.rhs_out[nb] = rb;
             A.ab_next[nb] = v;
             rr = fmax(rr, fabs(v - bc));
@@ -542,6 +563,143 @@ __device__ __forceinline__ R a_rows(const ASweepArgs<R> &A, int s, int ja, int j
             if (j + B + k < jb) row(P1[k], j + B + k);
     }
     return rr;
+}
+
+// Two consecutive A sweeps j and j+1 (j even) in one pass over rows
+// [ja, jb) of strip s, strips of WAVE - 4 columns as in psi_rows2. Over such
+// a pair the supercurrent source is the pair's input for both sweeps
+// (A^(2*floor(j/2)) = A^j), so each link's supercurrent is evaluated once;
+// the first sweep runs one row ahead of the second on lanes 1..62 and its
+// result A^{j+1} stays in registers. Writes A^{j+2} (and, at jstep 0, the
+// Langevin right-hand side); r1 and r2 get the lane's largest |delta| of
+// each sweep. Each link update is that of a_rows.
+template <typename R, bool GAUGE>
+__device__ __forceinline__ void a_rows2(const ASweepArgs<R> &A, int s, int ja, int jb, R &r1, R &r2) {
+    constexpr int B = Pipe<R>::ROWS2;
+    const ColCtx X = col_ctx<2>(A.Nx, s);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int c = X.c;
+    const bool own = X.own, hasa = X.hasa;
+    const bool mid = X.col && lane >= 1 && lane < WAVE - 1;
+    const int Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1;
+    const long Na = (long)Nxa * Ny;
+    const R *a = A.ab;
+    const R *b = A.ab + Na;
+    const bool noisy = A.rhs_out != nullptr;
+    const bool noise_on = noisy && A.jstep == 0 && A.langevin_c > R(1.0e-32);
+    const bool has_mt = A.mt != nullptr;
+    const uint8_t *mtp = has_mt ? A.mt : reinterpret_cast<const uint8_t *>(A.ab);
+    const R gb = GAUGE ? gauge_phase_b<R>(A, c) : R(0);
+    const cplx<R> zero = {R(0), R(0)};
+    const int t0 = ja > 0 ? ja - 1 : 0;
+
+    // first sweep window (row t): a(c, t-1), a(c, t), b(c, t-1), psi(c, t), cell (c, t-1)
+    R am = R(0), ac = R(0), bm = R(0);
+    cplx<R> pc = zero;
+    int cellm = 0;
+    if (X.col) {
+        pc = ld(A.psi, c + (long)Nx * t0);
+        if (hasa) ac = a[c + (long)Nxa * t0];
+        if (t0 > 0) {
+            if (hasa) am = a[c + (long)Nxa * (t0 - 1)];
+            bm = b[c + (long)Nx * (t0 - 1)];
+            if (hasa) cellm = has_mt ? int(A.mt[c + (long)Nxa * (t0 - 1)]) : 1;
+        }
+    }
+    // second sweep window (row u = t - 1): A^{j+1} at a(c, u-1), a(c, u),
+    // b(c, u-1), b(c, u); supercurrents and right-hand sides of row u
+    R am1 = R(0), ac1 = R(0), bm1 = R(0), bc1 = R(0), jla1 = R(0), jlb1 = R(0), ra1 = R(0), rb1 = R(0);
+
+    auto row = [&](const ARow<R> &cu, int t) {
+        const bool real = t < Ny;
+        const R ap = (hasa && t + 1 < Ny) ? cu.ap : R(0);
+        const R bc = (X.col && t < Ny - 1) ? cu.bc : R(0);
+        const cplx<R> pp = (X.col && t < Ny - 1) ? cu.pp : zero;
+        const int cellv = (hasa && t < Ny - 1) ? (has_mt ? cu.cell : 1) : 0;
+        const bool cellc = cellv != 0;
+        const R brc = shfl_dn1(bc), brm = shfl_dn1(bm);
+        const R pr_re = shfl_dn1(pc.re), pr_im = shfl_dn1(pc.im);
+        const R alc = shfl_up1(ac), alp = shfl_up1(ap), blc = shfl_up1(bc);
+        const bool cl_c = shfl_up1(cellv) != 0;  // cell (c-1, t)
+        const long na = c + (long)Nxa * t;
+        const long nb = Na + c + (long)Nx * t;
+        const bool hasa_t = hasa && real;
+        const bool hasb_t = X.col && t < Ny - 1;
+        R ra = cu.ra, rb = cu.rb;
+        if (noise_on) {
+            if (mid && hasa_t) ra = ra + A.langevin_c * R(double(noise_1<R>((uint32_t)na, A.rand_t)) - 0.5);
+            if (mid && hasb_t) rb = rb + A.langevin_c * R(double(noise_2<R>((uint32_t)nb, A.rand_t)) - 0.5);
+        }
+        R ssa, csa, ssb, csb;
+        sincos_r(link_phase<R, GAUGE>(A.dx, cu.sa, GAUGE ? gauge_phase_a<R>(A, t) : R(0)), &ssa, &csa);
+        sincos_r(link_phase<R, GAUGE>(A.dy, cu.sb, gb), &ssb, &csb);
+        const R jla = (cellm != 0 || cellc) ? link_current(A.idx, pc, pr_re, pr_im, csa, ssa) : R(0);
+        const R jlb = (cl_c || cellc) ? link_current(A.idy, pc, pp.re, pp.im, csb, ssb) : R(0);
+
+        // first sweep at row t -> A^{j+1}(t)
+        R va = R(0), vb = R(0);
+        if (mid && hasa_t) va = a_link_update(A, t, ra, jla, am, bm, brm, ap, bc, brc);
+        if (mid && hasb_t) vb = b_link_update(A, c, rb, jlb, blc, alc, alp, brc, ac, ap);
+        if (own && t >= ja && t < jb) {
+            if (hasa_t) {
+                if (noisy && A.jstep == 0) A.rhs_out[na] = ra;
+                r1 = fmax(r1, fabs(va - ac));
+            }
+            if (hasb_t) {
+                if (noisy && A.jstep == 0) A.rhs_out[nb] = rb;
+                r1 = fmax(r1, fabs(vb - bc));
+            }
+        }
+        // second sweep at row u = t - 1 -> A^{j+2}(u)
+        const int u = t - 1;
+        {
+            const R brc1 = shfl_dn1(bc1), brm1 = shfl_dn1(bm1);
+            const R alc1 = shfl_up1(ac1), alp1 = shfl_up1(va), blc1 = shfl_up1(bc1);
+            if (u >= ja && own) {
+                if (hasa) {
+                    const R v = a_link_update(A, u, ra1, jla1, am1, bm1, brm1, va, bc1, brc1);
+                    A.ab_next[c + (long)Nxa * u] = v;
+                    r2 = fmax(r2, fabs(v - ac1));
+                }
+                if (u < Ny - 1) {
+                    const R v = b_link_update(A, c, rb1, jlb1, blc1, alc1, alp1, brc1, ac1, va);
+                    A.ab_next[Na + c + (long)Nx * u] = v;
+                    r2 = fmax(r2, fabs(v - bc1));
+                }
+            }
+        }
+        // roll both windows
+        am1 = ac1;
+        ac1 = va;
+        bm1 = bc1;
+        bc1 = vb;
+        jla1 = jla;
+        jlb1 = jlb;
+        ra1 = ra;
+        rb1 = rb;
+        am = ac;
+        ac = ap;
+        bm = bc;
+        pc = pp;
+        cellm = cellv;
+    };
+
+    const int t1 = jb + 1;  // rows t in [t0, t1)
+    ARow<R> P0[B], P1[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) P0[k] = a_load_row(A, X, mtp, min(min(t0 + k, t1 - 1), Ny - 1));
+    for (int t = t0; t < t1; t += 2 * B) {
+#pragma unroll
+        for (int k = 0; k < B; ++k) P1[k] = a_load_row(A, X, mtp, min(min(t + B + k, t1 - 1), Ny - 1));
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            if (t + k < t1) row(P0[k], t + k);
+#pragma unroll
+        for (int k = 0; k < B; ++k) P0[k] = a_load_row(A, X, mtp, min(min(t + 2 * B + k, t1 - 1), Ny - 1));
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            if (t + B + k < t1) row(P1[k], t + B + k);
+    }
 }
 
 // ---- work partition --------------------------------------------------------
@@ -626,7 +784,7 @@ __global__ void __launch_bounds__(256) k_a_sweep(ASweepArgs<R> A, int *r2max, lo
 // current state and the barrier epoch carry over between launches in device
 // memory, so there is no host round trip per step.
 
-enum { ST_PSI = 0, ST_AB = 1, ST_EPOCH = 2, ST_STEP = 4, ST_ERRSTEP = 6, ST_PRED_PSI = 7 };
+enum { ST_PSI = 0, ST_AB = 1, ST_EPOCH = 2, ST_STEP = 4, ST_PRED_A = 5, ST_ERRSTEP = 6, ST_PRED_PSI = 7 };
 
 struct TDScratch {
     GridSync gs;
@@ -725,32 +883,56 @@ __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const Gri
     return {in, nsw, true};
 }
 
-// The A solve of one TD step on the team's waves, sweep by sweep, with the
-// reference's source schedule A^(2*floor(j/2)) (td.py:264-303).
+// The A solve of one TD step on the team's waves, with the reference's
+// source schedule A^(2*floor(j/2)) (td.py:264-303). Sweeps run in fused
+// pairs (a_rows2) starting at even j, where both sweeps take their source
+// from the pair's input; as in psi_solve, a pair whose first sweep met the
+// stop rule is redone as that single sweep. Single sweeps (a_rows) are used
+// at odd j, at the last sweep allowed, and at j = pred - 1 when the previous
+// step needed pred sweeps with pred odd.
 template <typename R, bool GAUGE>
 __device__ __forceinline__ SolveOut a_solve(const TDSolveArgs<R> &T, const GridSync &gs, const Team &tm, long w,
-                                            long W, int pc, int qc, unsigned long long &epoch, int *lds) {
+                                            long W, int pc, int qc, int pred, unsigned long long &epoch, int *lds) {
     const int o1 = (qc + 1) % 3, o2 = (qc + 2) % 3;
     ASweepArgs<R> A = T.A;
     A.psi = pick3(T.P, pc);
-    int in = qc, out = o1, nsw = 0;
-    for (int j = 0; j < T.max_sweeps; ++j) {
+    A.rhs_out = T.rhs_ab;
+    int in = qc, out = o1, j = 0, nsw = 0;
+    bool done = false;
+    while (!done && j < T.max_sweeps) {
         A.jstep = (uint32_t)j;
         A.ab = pick3(T.Q, in);
         A.ab_next = pick3(T.Q, out);
         // source A^(2*floor(j/2)): the input on even sweeps, A^n on sweep 1,
-        // the output buffer's previous contents on odd j >= 3
+        // the output buffer's previous contents on odd j >= 3 (sweep j - 1
+        // was then a single sweep that read it)
         A.src = pick3(T.Q, (j % 2 == 0) ? in : (j == 1 ? qc : out));
         A.rhs = (T.rhs_ab && j > 0) ? T.rhs_ab : pick3(T.Q, qc);
-        A.rhs_out = T.rhs_ab;
-        R rr = R(0);
-        for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, a_rows<R, GAUGE>(A, s, ja, jb)); });
+        const bool pair = (j % 2 == 0) && j + 1 != pred && j + 2 <= T.max_sweeps;
+        R r1 = R(0), r2 = R(0);
+        if (pair)
+            for_rows_rm<2>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { a_rows2<R, GAUGE>(A, s, ja, jb, r1, r2); });
+        else
+            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { r1 = fmax(r1, a_rows<R, GAUGE>(A, s, ja, jb)); });
         int r = 0;
-        if (!sweep_sync(gs, tm, scaled_residual<R>(rr, A.stop_eps), 0, epoch, lds, r)) return {in, nsw, false};
-        nsw = j + 1;
+        if (!sweep_sync(gs, tm, scaled_residual<R>(r1, A.stop_eps), scaled_residual<R>(r2, A.stop_eps), epoch, lds, r))
+            return {in, nsw, false};
+        const unsigned ru = (unsigned)r;
+        const bool c1 = (ru & 0xffffu) < 10000u, c2 = (ru >> 16) < 10000u;
+        if (pair && c1) {
+            R rr = R(0);
+            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, a_rows<R, GAUGE>(A, s, ja, jb)); });
+            int rdummy = 0;
+            if (!sweep_sync(gs, tm, 0, 0, epoch, lds, rdummy)) return {in, nsw, false};
+            nsw = j + 1;
+            done = true;
+        } else {
+            j += pair ? 2 : 1;
+            nsw = j;
+            done = pair ? c2 : c1;
+        }
         in = out;
         out = (out == o1) ? o2 : o1;
-        if ((r & 0xffff) < 10000) break;
     }
     return {in, nsw, true};
 }
@@ -787,13 +969,14 @@ __global__ void __launch_bounds__(256) k_td_a_solve(TDSolveArgs<R> T) {
     int *st = T.sc.state;
     unsigned long long epoch = *state64(st, ST_EPOCH);
     const Team tm{0, (int)gridDim.x, NREL, T.sc.gs.rel};
-    const SolveOut o = a_solve<R, GAUGE>(T, T.sc.gs, tm, w, W, st[ST_PSI], st[ST_AB], epoch, lds);
+    const SolveOut o = a_solve<R, GAUGE>(T, T.sc.gs, tm, w, W, st[ST_PSI], st[ST_AB], st[ST_PRED_A], epoch, lds);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (!o.ok) {
             st[ST_ERRSTEP] = T.step + 1;
             return;
         }
         st[ST_AB] = o.cur;
+        st[ST_PRED_A] = o.nsw;
         *state64(st, ST_EPOCH) = epoch;
         T.sc.sweeps[2 * T.step + 1] = o.nsw;
     }
@@ -836,7 +1019,7 @@ __global__ void __launch_bounds__(256) k_batch_td_step(BatchArgs<R> B) {
     const int qc = st[ST_AB];
     const SolveOut o = psi_solve<R, GAUGE>(T, B.gs, tm, w, W, st[ST_PSI], qc, st[ST_PRED_PSI], epoch, lds);
     SolveOut oa{qc, 0, o.ok};
-    if (o.ok && T.solveA) oa = a_solve<R, GAUGE>(T, B.gs, tm, w, W, o.cur, qc, epoch, lds);
+    if (o.ok && T.solveA) oa = a_solve<R, GAUGE>(T, B.gs, tm, w, W, o.cur, qc, st[ST_PRED_A], epoch, lds);
     if ((int)blockIdx.x == tb.x && threadIdx.x == 0) {
         if (!oa.ok) {
             st[ST_ERRSTEP] = step + 1;
@@ -845,6 +1028,7 @@ __global__ void __launch_bounds__(256) k_batch_td_step(BatchArgs<R> B) {
         st[ST_PSI] = o.cur;
         st[ST_AB] = oa.cur;
         st[ST_PRED_PSI] = o.nsw;
+        st[ST_PRED_A] = oa.nsw;
         st[ST_STEP] = step + 1;
         *state64(st, ST_EPOCH) = epoch;
         B.sweeps[((long)step * B.n + inst) * 2] = o.nsw;
@@ -873,6 +1057,7 @@ static __global__ void k_td_reset(TDScratch sc, int pc, int qc) {
         *state64(sc.state, ST_EPOCH) = 0ull;
         sc.state[ST_ERRSTEP] = 0;
         sc.state[ST_PRED_PSI] = 0;
+        sc.state[ST_PRED_A] = 0;
     }
 }
 
