@@ -77,14 +77,22 @@ template <typename R, typename Args> __device__ __forceinline__ R gauge_phase_b(
     return R(double(A.dy) * gauge_b(A.gH, A.gdx, A.gx0, c));
 }
 
-// Rows are loaded in batches of Pipe<R>::ROWS: the loads of batch k+1 are
-// issued before batch k is computed, and the two batches live in separately
-// named registers (the loop body is unrolled over the pair), so no loaded
-// value is copied before it is used and 1 to 2 batches of loads per wave stay
-// in flight during the arithmetic. Loads beyond the wave's last row are
-// clamped to that row (cache hits, never used).
-template <typename R> struct Pipe { static constexpr int ROWS = 2, ROWS2 = 1; };
-template <> struct Pipe<double> { static constexpr int ROWS = 1, ROWS2 = 1; };
+// Rows are loaded in batches of Pipe<R>::ROWS (single sweeps), ROWS2 (fused
+// psi pairs) or AROWS2 (fused A pairs): the loads of batch k+1 are issued
+// before batch k is computed, and the two batches live in separately named
+// registers (the loop body is unrolled over the pair), so no loaded value is
+// copied before it is used and 1 to 2 batches of loads per wave stay in
+// flight during the arithmetic. The persistent solves run two waves per
+// SIMD, which leaves registers for two-row batches (measured at 4097^2: FP32
+// psi pairs 93 -> 85 us per sweep, FP64 162 -> 155; deeper batches and
+// FP32 A pairs gain nothing). Loads beyond the wave's last row are clamped
+// to that row (cache hits, never used).
+// The batched kernel runs its solves with SH (shallow) batches, which keep
+// its register count within two waves per SIMD.
+template <typename R, bool SH = false> struct Pipe { static constexpr int ROWS = 2, ROWS2 = 2, AROWS2 = 1; };
+template <> struct Pipe<double, false> { static constexpr int ROWS = 2, ROWS2 = 2, AROWS2 = 2; };
+template <> struct Pipe<float, true> { static constexpr int ROWS = 2, ROWS2 = 1, AROWS2 = 1; };
+template <> struct Pipe<double, true> { static constexpr int ROWS = 1, ROWS2 = 1, AROWS2 = 1; };
 
 // Per-lane column context of a strip: c may lie outside the grid (halo lanes
 // at the edges); cc and ca are c clamped into the node and link ranges, so
@@ -183,9 +191,9 @@ __device__ __forceinline__ void psi_noise(const PsiSweepArgs<R> &A, uint32_t n, 
 // One psi sweep over rows [ja, jb) of strip s. Returns the lane's largest
 // |delta| (max of real and imaginary parts); the stop rule's scaled integer
 // residual is monotone in it, so it is applied once per wave by the caller.
-template <typename R, bool GAUGE>
+template <typename R, bool GAUGE, bool SH = false>
 __device__ __forceinline__ R psi_rows(const PsiSweepArgs<R> &A, int s, int ja, int jb) {
-    constexpr int B = Pipe<R>::ROWS;
+    constexpr int B = Pipe<R, SH>::ROWS;
     const ColCtx X = col_ctx<1>(A.Nx, s);
     const int c = X.c;
     const bool own = X.own;
@@ -277,9 +285,9 @@ __device__ __forceinline__ R psi_rows(const PsiSweepArgs<R> &A, int s, int ja, i
 // psi^{j+1} never leaves registers. Writes psi^{j+2} (and, at jstep 0, the
 // Langevin right-hand side); returns the lane's largest |delta| of each sweep
 // in r1 and r2. The arithmetic of each node update is that of psi_rows.
-template <typename R, bool GAUGE>
+template <typename R, bool GAUGE, bool SH = false>
 __device__ __forceinline__ void psi_rows2(const PsiSweepArgs<R> &A, int s, int ja, int jb, R &r1, R &r2) {
-    constexpr int B = Pipe<R>::ROWS2;
+    constexpr int B = Pipe<R, SH>::ROWS2;
     const ColCtx X = col_ctx<2>(A.Nx, s);
     const int lane = threadIdx.x & (WAVE - 1);
     const int c = X.c;
@@ -467,9 +475,9 @@ __device__ __forceinline__ R link_current(R id, cplx<R> p0, R x1, R y1, R cs, R 
 
 // One A sweep over rows [ja, jb) of strip s (both link families of every
 // owned node). Returns the lane's largest |delta|.
-template <typename R, bool GAUGE>
+template <typename R, bool GAUGE, bool SH = false>
 __device__ __forceinline__ R a_rows(const ASweepArgs<R> &A, int s, int ja, int jb) {
-    constexpr int B = Pipe<R>::ROWS;
+    constexpr int B = Pipe<R, SH>::ROWS;
     const ColCtx X = col_ctx<1>(A.Nx, s);
     const int c = X.c;
     const bool own = X.own, hasa = X.hasa;
@@ -573,9 +581,9 @@ __device__ __forceinline__ R a_rows(const ASweepArgs<R> &A, int s, int ja, int j
 // result A^{j+1} stays in registers. Writes A^{j+2} (and, at jstep 0, the
 // Langevin right-hand side); r1 and r2 get the lane's largest |delta| of
 // each sweep. Each link update is that of a_rows.
-template <typename R, bool GAUGE>
+template <typename R, bool GAUGE, bool SH = false>
 __device__ __forceinline__ void a_rows2(const ASweepArgs<R> &A, int s, int ja, int jb, R &r1, R &r2) {
-    constexpr int B = Pipe<R>::ROWS2;
+    constexpr int B = Pipe<R, SH>::AROWS2;
     const ColCtx X = col_ctx<2>(A.Nx, s);
     const int lane = threadIdx.x & (WAVE - 1);
     const int c = X.c;
@@ -760,7 +768,7 @@ __global__ void __launch_bounds__(256) k_psi_sweep(PsiSweepArgs<R> A, int *r2max
     __shared__ int lds[4];
     const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
     R rr = R(0);
-    if (w < W) for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, psi_rows<R, false>(A, s, ja, jb)); });
+    if (w < W) for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, psi_rows<R, false, true>(A, s, ja, jb)); });
     const int rmax = block_max_int(scaled_residual<R>(rr, A.stop_eps), lds);
     if (threadIdx.x == 0) atomic_max_agent(r2max, rmax);
 }
@@ -770,7 +778,7 @@ __global__ void __launch_bounds__(256) k_a_sweep(ASweepArgs<R> A, int *r2max, lo
     __shared__ int lds[4];
     const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
     R rr = R(0);
-    if (w < W) for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, a_rows<R, false>(A, s, ja, jb)); });
+    if (w < W) for_rows(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, a_rows<R, false, true>(A, s, ja, jb)); });
     const int rmax = block_max_int(scaled_residual<R>(rr, A.stop_eps), lds);
     if (threadIdx.x == 0) atomic_max_agent(r2max, rmax);
 }
@@ -839,7 +847,7 @@ struct SolveOut {
 // of sweep-by-sweep iteration. When the previous step needed an odd number
 // of sweeps (pred), the solve opens with one single sweep so that the pairs
 // end where the solve is likely to end.
-template <typename R, bool GAUGE>
+template <typename R, bool GAUGE, bool SH = false>
 __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const GridSync &gs, const Team &tm, long w,
                                               long W, int pc, int qc, int pred, unsigned long long &epoch, int *lds) {
     const int o1 = (pc + 1) % 3, o2 = (pc + 2) % 3;
@@ -856,9 +864,9 @@ __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const Gri
         const bool pair = !(j == 0 && (pred & 1)) && j + 2 <= T.max_sweeps;
         R r1 = R(0), r2 = R(0);
         if (pair)
-            for_rows_rm<2>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { psi_rows2<R, GAUGE>(A, s, ja, jb, r1, r2); });
+            for_rows_rm<2>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { psi_rows2<R, GAUGE, SH>(A, s, ja, jb, r1, r2); });
         else
-            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { r1 = fmax(r1, psi_rows<R, GAUGE>(A, s, ja, jb)); });
+            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { r1 = fmax(r1, psi_rows<R, GAUGE, SH>(A, s, ja, jb)); });
         int r = 0;
         if (!sweep_sync(gs, tm, scaled_residual<R>(r1, A.stop_eps), scaled_residual<R>(r2, A.stop_eps), epoch, lds, r))
             return {in, nsw, false};
@@ -867,7 +875,7 @@ __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const Gri
         if (pair && c1) {
             // sweep j met the stop rule: redo it alone into the same output
             R rr = R(0);
-            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, psi_rows<R, GAUGE>(A, s, ja, jb)); });
+            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, psi_rows<R, GAUGE, SH>(A, s, ja, jb)); });
             int rdummy = 0;
             if (!sweep_sync(gs, tm, 0, 0, epoch, lds, rdummy)) return {in, nsw, false};
             nsw = j + 1;
@@ -890,7 +898,7 @@ __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const Gri
 // stop rule is redone as that single sweep. Single sweeps (a_rows) are used
 // at odd j, at the last sweep allowed, and at j = pred - 1 when the previous
 // step needed pred sweeps with pred odd.
-template <typename R, bool GAUGE>
+template <typename R, bool GAUGE, bool SH = false>
 __device__ __forceinline__ SolveOut a_solve(const TDSolveArgs<R> &T, const GridSync &gs, const Team &tm, long w,
                                             long W, int pc, int qc, int pred, unsigned long long &epoch, int *lds) {
     const int o1 = (qc + 1) % 3, o2 = (qc + 2) % 3;
@@ -911,9 +919,9 @@ __device__ __forceinline__ SolveOut a_solve(const TDSolveArgs<R> &T, const GridS
         const bool pair = (j % 2 == 0) && j + 1 != pred && j + 2 <= T.max_sweeps;
         R r1 = R(0), r2 = R(0);
         if (pair)
-            for_rows_rm<2>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { a_rows2<R, GAUGE>(A, s, ja, jb, r1, r2); });
+            for_rows_rm<2>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { a_rows2<R, GAUGE, SH>(A, s, ja, jb, r1, r2); });
         else
-            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { r1 = fmax(r1, a_rows<R, GAUGE>(A, s, ja, jb)); });
+            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { r1 = fmax(r1, a_rows<R, GAUGE, SH>(A, s, ja, jb)); });
         int r = 0;
         if (!sweep_sync(gs, tm, scaled_residual<R>(r1, A.stop_eps), scaled_residual<R>(r2, A.stop_eps), epoch, lds, r))
             return {in, nsw, false};
@@ -921,7 +929,7 @@ __device__ __forceinline__ SolveOut a_solve(const TDSolveArgs<R> &T, const GridS
         const bool c1 = (ru & 0xffffu) < 10000u, c2 = (ru >> 16) < 10000u;
         if (pair && c1) {
             R rr = R(0);
-            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, a_rows<R, GAUGE>(A, s, ja, jb)); });
+            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, a_rows<R, GAUGE, SH>(A, s, ja, jb)); });
             int rdummy = 0;
             if (!sweep_sync(gs, tm, 0, 0, epoch, lds, rdummy)) return {in, nsw, false};
             nsw = j + 1;
@@ -1017,9 +1025,9 @@ __global__ void __launch_bounds__(256) k_batch_td_step(BatchArgs<R> B) {
     unsigned long long epoch = *state64(st, ST_EPOCH);
     const int step = st[ST_STEP];
     const int qc = st[ST_AB];
-    const SolveOut o = psi_solve<R, GAUGE>(T, B.gs, tm, w, W, st[ST_PSI], qc, st[ST_PRED_PSI], epoch, lds);
+    const SolveOut o = psi_solve<R, GAUGE, true>(T, B.gs, tm, w, W, st[ST_PSI], qc, st[ST_PRED_PSI], epoch, lds);
     SolveOut oa{qc, 0, o.ok};
-    if (o.ok && T.solveA) oa = a_solve<R, GAUGE>(T, B.gs, tm, w, W, o.cur, qc, st[ST_PRED_A], epoch, lds);
+    if (o.ok && T.solveA) oa = a_solve<R, GAUGE, true>(T, B.gs, tm, w, W, o.cur, qc, st[ST_PRED_A], epoch, lds);
     if ((int)blockIdx.x == tb.x && threadIdx.x == 0) {
         if (!oa.ok) {
             st[ST_ERRSTEP] = step + 1;
