@@ -143,40 +143,56 @@ __device__ __forceinline__ PsiRow<R> psi_load_row(const PsiSweepArgs<R> &A, cons
     return L;
 }
 
-// Backward-Euler Jacobi update of one psi node (svirl/cuda/td.h:85-214):
-// neighbours pl, pr, pm, pp (left, right, down, up), the link factors
-// (c, s) = (cos, sin) of dx a_left, dx a, dy b_down, dy b, and the four
-// adjacent cells (mm = (c-1, j-1), mp = (c-1, j), pm = (c, j-1), pp = (c, j)).
+// Backward-Euler Jacobi update of one psi node (svirl/cuda/td.h:85-214),
+// in two parts. psi_den: the diagonal 1 + dt (|rhs|^2 - eps + sum of the
+// link weights / h^2) from the four adjacent cells (mm = (c-1, j-1),
+// mp = (c-1, j), pmc = (c, j-1), ppc = (c, j)); FP32 returns its reciprocal
+// (hardware estimate + one Newton step, within 1 ulp, one reciprocal for
+// both components instead of two IEEE divisions), FP64 the diagonal itself.
+// It depends on the sweep only through rhs, so the two sweeps of a fused
+// pair share it.
 template <typename R>
-__device__ __forceinline__ cplx<R> psi_node(const PsiSweepArgs<R> &A, const cplx<R> &rhs, R e, R pl_re, R pl_im,
-                                            R pr_re, R pr_im, const cplx<R> &pm, const cplx<R> &pp, R cal, R sal,
-                                            R ca, R sa, R cbm, R sbm, R cb, R sb, bool mm, bool mp, bool pmc,
-                                            bool ppc) {
+__device__ __forceinline__ R psi_den(const PsiSweepArgs<R> &A, const cplx<R> &rhs, R e, bool mm, bool mp, bool pmc,
+                                     bool ppc) {
     const R rm0 = (mm || mp) ? R(1) : R(0);
     const R rp0 = (pmc || ppc) ? R(1) : R(0);
     const R r0m = (mm || pmc) ? R(1) : R(0);
     const R r0p = (mp || ppc) ? R(1) : R(0);
-    // U(-dx a_left) psi_left, U(dx a) psi_right, U(-dy b_down) psi_down, U(dy b) psi_up
-    R m0r = R(0), m0i = R(0), p0r = R(0), p0i = R(0);
-    R q0mr = R(0), q0mi = R(0), q0pr = R(0), q0pi = R(0);
-    if (mm || mp) { m0r = cal * pl_re - sal * pl_im; m0i = cal * pl_im + sal * pl_re; }
-    if (pmc || ppc) { p0r = ca * pr_re + sa * pr_im; p0i = ca * pr_im - sa * pr_re; }
-    if (mm || pmc) { q0mr = cbm * pm.re - sbm * pm.im; q0mi = cbm * pm.im + sbm * pm.re; }
-    if (mp || ppc) { q0pr = cb * pp.re + sb * pp.im; q0pi = cb * pp.im - sb * pp.re; }
     R q = rhs.re * rhs.re + rhs.im * rhs.im - e;
     q = q + (A.idx2 * (rm0 + rp0) + A.idy2 * (r0m + r0p));
     const R diag = R(1) + A.dt * q;
+    if constexpr (std::is_same<R, float>::value) {
+        float rd = __builtin_amdgcn_rcpf(diag);
+        return fmaf(rd, fmaf(-diag, rd, 1.0f), rd);
+    } else {
+        return diag;
+    }
+}
+
+// psi_apply: the off-diagonal part and the division, from the neighbours pl,
+// pr, pm, pp (left, right, down, up) and the link factors (c, s) = (cos, sin)
+// of dx a_left, dx a, dy b_down, dy b. A link that touches no material cell
+// carries the factors (0, 0) (mask_link), which makes its term 0 as the
+// reference's link weight 0 does.
+template <typename R>
+__device__ __forceinline__ cplx<R> psi_apply(const PsiSweepArgs<R> &A, const cplx<R> &rhs, R den, R pl_re, R pl_im,
+                                             R pr_re, R pr_im, const cplx<R> &pm, const cplx<R> &pp, R cal, R sal,
+                                             R ca, R sa, R cbm, R sbm, R cb, R sb) {
+    // U(-dx a_left) psi_left, U(dx a) psi_right, U(-dy b_down) psi_down, U(dy b) psi_up
+    const R m0r = cal * pl_re - sal * pl_im, m0i = cal * pl_im + sal * pl_re;
+    const R p0r = ca * pr_re + sa * pr_im, p0i = ca * pr_im - sa * pr_re;
+    const R q0mr = cbm * pm.re - sbm * pm.im, q0mi = cbm * pm.im + sbm * pm.re;
+    const R q0pr = cb * pp.re + sb * pp.im, q0pi = cb * pp.im - sb * pp.re;
     const R nre = rhs.re + A.dt * (A.idx2 * (m0r + p0r) + A.idy2 * (q0mr + q0pr));
     const R nim = rhs.im + A.dt * (A.idx2 * (m0i + p0i) + A.idy2 * (q0mi + q0pi));
-    if constexpr (std::is_same<R, float>::value) {
-        // FP32: one reciprocal (hardware estimate + one Newton step, within
-        // 1 ulp) for both components instead of two IEEE divisions
-        float rd = __builtin_amdgcn_rcpf(diag);
-        rd = fmaf(rd, fmaf(-diag, rd, 1.0f), rd);
-        return {nre * rd, nim * rd};
-    } else {
-        return {nre / diag, nim / diag};
-    }
+    if constexpr (std::is_same<R, float>::value) return {nre * den, nim * den};
+    else return {nre / den, nim / den};
+}
+
+// Link factors of a link that touches no material cell become (0, 0).
+template <typename R> __device__ __forceinline__ void mask_link(bool on, R &c, R &s) {
+    c = on ? c : R(0);
+    s = on ? s : R(0);
 }
 
 // The reference's Langevin term added to the right-hand side at node n.
@@ -218,6 +234,7 @@ __device__ __forceinline__ R psi_rows(const PsiSweepArgs<R> &A, int s, int ja, i
             if (c < Nxa) cellm = has_mt ? int(A.mt[c + (long)Nxa * (ja - 1)]) : 1;
         }
     }
+    mask_link(shfl_up1(cellm) != 0 || cellm != 0, cbm, sbm);  // b(c, ja-1): cells (c-1, ja-1), (c, ja-1)
 
     auto row = [&](const PsiRow<R> &cu, int j) {
         const long rowoff = (long)Nx * j;
@@ -230,13 +247,15 @@ __device__ __forceinline__ R psi_rows(const PsiSweepArgs<R> &A, int s, int ja, i
         R sa, ca, sb, cb;
         sincos_r(link_phase<R, GAUGE>(A.dx, av, GAUGE ? gauge_phase_a<R>(A, j) : R(0)), &sa, &ca);
         sincos_r(link_phase<R, GAUGE>(A.dy, bv, gb), &sb, &cb);
+        const bool cl_m = shfl_up1(cellm) != 0;  // cell (c-1, j-1)
+        const bool cl_c = shfl_up1(cellc) != 0;  // cell (c-1, j)
+        mask_link(cellm != 0 || cellc != 0, ca, sa);  // a(c, j): cells (c, j-1), (c, j)
+        mask_link(cl_c || cellc != 0, cb, sb);        // b(c, j): cells (c-1, j), (c, j)
 
         // neighbours from adjacent lanes
         const R pl_re = shfl_up1(pc.re), pl_im = shfl_up1(pc.im);
         const R pr_re = shfl_dn1(pc.re), pr_im = shfl_dn1(pc.im);
         const R cal = shfl_up1(ca), sal = shfl_up1(sa);
-        const bool cl_m = shfl_up1(cellm) != 0;  // cell (c-1, j-1)
-        const bool cl_c = shfl_up1(cellc) != 0;  // cell (c-1, j)
 
         if (own) {
             const bool mm = cl_m, mp = cl_c, pmc = cellm != 0, ppc = cellc != 0;
@@ -244,8 +263,8 @@ __device__ __forceinline__ R psi_rows(const PsiSweepArgs<R> &A, int s, int ja, i
             cplx<R> nxt = {R(0), R(0)};
             if (inside) {
                 if (noisy && A.jstep == 0 && A.langevin_c > R(1.0e-32)) psi_noise(A, (uint32_t)(c + rowoff), rhs);
-                nxt = psi_node(A, rhs, e, pl_re, pl_im, pr_re, pr_im, pm, pp, cal, sal, ca, sa, cbm, sbm, cb, sb, mm,
-                               mp, pmc, ppc);
+                nxt = psi_apply(A, rhs, psi_den(A, rhs, e, mm, mp, pmc, ppc), pl_re, pl_im, pr_re, pr_im, pm, pp, cal,
+                                sal, ca, sa, cbm, sbm, cb, sb);
             }
             if (noisy && A.jstep == 0) st(A.rhs_out, c + rowoff, rhs);
             st(A.psi_next, c + rowoff, nxt);
@@ -316,11 +335,12 @@ __device__ __forceinline__ void psi_rows2(const PsiSweepArgs<R> &A, int s, int j
             if (c < Nxa) cell1 = has_mt ? int(A.mt[c + (long)Nxa * (t0 - 1)]) : 1;
         }
     }
+    mask_link(shfl_up1(cell1) != 0 || cell1 != 0, cb1, sb1);  // b(c, t0-1)
     // second sweep window (row u = t - 1): psi^{j+1} at u-1, u; U of a(u),
-    // b(u-1); cells (c, u-1), (c-1, u-1), (c-1, u); rhs and eps at u
+    // b(u-1); rhs, diagonal and material test of node (c, u)
     cplx<R> qm = zero, qc = zero, rhs_u = zero;
-    R ca2 = R(1), sa2 = R(0), cal2 = R(1), sal2 = R(0), cb2 = R(1), sb2 = R(0), e_u = R(0);
-    int cell2 = 0, cl2m = 0, cl2c = 0;
+    R ca2 = R(0), sa2 = R(0), cal2 = R(0), sal2 = R(0), cb2 = R(0), sb2 = R(0), den_u = R(1);
+    bool in_u = false;
 
     auto row = [&](const PsiRow<R> &cu, int t) {
         const long rowoff = (long)Nx * t;
@@ -334,20 +354,24 @@ __device__ __forceinline__ void psi_rows2(const PsiSweepArgs<R> &A, int s, int j
         R sa, ca, sb, cb;
         sincos_r(link_phase<R, GAUGE>(A.dx, av, GAUGE ? gauge_phase_a<R>(A, t) : R(0)), &sa, &ca);
         sincos_r(link_phase<R, GAUGE>(A.dy, bv, gb), &sb, &cb);
+        const bool mm = shfl_up1(cell1) != 0;  // cell (c-1, t-1)
+        const bool mp = shfl_up1(cellc) != 0;  // cell (c-1, t)
+        const bool pmc = cell1 != 0, ppc = cellc != 0;
+        mask_link(pmc || ppc, ca, sa);  // a(c, t): cells (c, t-1), (c, t)
+        mask_link(mp || ppc, cb, sb);   // b(c, t): cells (c-1, t), (c, t)
         const R cal = shfl_up1(ca), sal = shfl_up1(sa);
-        const int cl_m = shfl_up1(cell1);  // cell (c-1, t-1)
-        const int cl_c = shfl_up1(cellc);  // cell (c-1, t)
+        const bool inside = mm || mp || pmc || ppc;
 
         // first sweep at row t -> psi^{j+1}(t)
         cplx<R> q = zero;
+        R den = R(1);
         {
             const R pl_re = shfl_up1(pc.re), pl_im = shfl_up1(pc.im);
             const R pr_re = shfl_dn1(pc.re), pr_im = shfl_dn1(pc.im);
-            const bool mm = cl_m != 0, mp = cl_c != 0, pmc = cell1 != 0, ppc = cellc != 0;
-            if (mid && real && (mm || mp || pmc || ppc)) {
+            if (mid && real && inside) {
                 if (noise_on) psi_noise(A, (uint32_t)(c + rowoff), rhs);
-                q = psi_node(A, rhs, e, pl_re, pl_im, pr_re, pr_im, pm, pp, cal, sal, ca, sa, cb1, sb1, cb, sb, mm, mp,
-                             pmc, ppc);
+                den = psi_den(A, rhs, e, mm, mp, pmc, ppc);
+                q = psi_apply(A, rhs, den, pl_re, pl_im, pr_re, pr_im, pm, pp, cal, sal, ca, sa, cb1, sb1, cb, sb);
             }
             if (own && real && t >= ja && t < jb) {
                 if (noisy && A.jstep == 0) st(A.rhs_out, c + rowoff, rhs);
@@ -359,12 +383,11 @@ __device__ __forceinline__ void psi_rows2(const PsiSweepArgs<R> &A, int s, int j
         if (u >= ja) {
             const R pl_re = shfl_up1(qc.re), pl_im = shfl_up1(qc.im);
             const R pr_re = shfl_dn1(qc.re), pr_im = shfl_dn1(qc.im);
-            const bool mm = cl2m != 0, mp = cl2c != 0, pmc = cell2 != 0, ppc = cell1 != 0;
             if (own) {
                 cplx<R> nxt = zero;
-                if (mm || mp || pmc || ppc)
-                    nxt = psi_node(A, rhs_u, e_u, pl_re, pl_im, pr_re, pr_im, qm, q, cal2, sal2, ca2, sa2, cb2, sb2,
-                                   cb1, sb1, mm, mp, pmc, ppc);
+                if (in_u)
+                    nxt = psi_apply(A, rhs_u, den_u, pl_re, pl_im, pr_re, pr_im, qm, q, cal2, sal2, ca2, sa2, cb2, sb2,
+                                    cb1, sb1);
                 st(A.psi_next, c + (long)Nx * u, nxt);
                 r2 = fmax(r2, fmax(fabs(nxt.re - qc.re), fabs(nxt.im - qc.im)));
             }
@@ -378,11 +401,9 @@ __device__ __forceinline__ void psi_rows2(const PsiSweepArgs<R> &A, int s, int j
         sa2 = sa;
         cal2 = cal;
         sal2 = sal;
-        cell2 = cell1;
-        cl2m = cl_m;
-        cl2c = cl_c;
+        in_u = inside;
+        den_u = den;
         rhs_u = rhs;
-        e_u = e;
         pm = pc;
         pc = pp;
         cb1 = cb;
