@@ -36,6 +36,22 @@ template <> __device__ __forceinline__ void st(cplx<float> *p, long n, cplx<floa
     reinterpret_cast<float2 *>(p)[n] = make_float2(v.re, v.im);
 }
 
+// Element at byte offset off (a per-lane 32-bit value) from a wave-uniform
+// row pointer: the compiler keeps base + lane offset per array in registers
+// and adds the scalar row offset with one vector instruction per access.
+template <typename T> __device__ __forceinline__ T at(const T *row, uint32_t off) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(row) + off);
+}
+template <typename T> __device__ __forceinline__ void st_at(T *row, uint32_t off, T v) {
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(row) + off) = v;
+}
+template <typename R> __device__ __forceinline__ cplx<R> ld_at(const cplx<R> *row, uint32_t off) {
+    return ld(reinterpret_cast<const cplx<R> *>(reinterpret_cast<const char *>(row) + off), 0);
+}
+template <typename R> __device__ __forceinline__ void st_at(cplx<R> *row, uint32_t off, cplx<R> v) {
+    st(reinterpret_cast<cplx<R> *>(reinterpret_cast<char *>(row) + off), 0, v);
+}
+
 // sin and cos of one link phase. Link phases d*A are tens of radians at
 // most, so a bounded-range method keeps the register footprint small:
 // Cody-Waite reduction by pi/2 with a three-part constant (each leading part

@@ -77,6 +77,8 @@ static ASweepArgs<R> make_a_args(const glx_grid &g, double dt, double kappa2, do
     const double drk = double(A.dt_rho_kappa2);
     A.den_a = R(1.0 + 2.0 * drk * double(A.idy2));
     A.den_b = R(1.0 + 2.0 * drk * double(A.idx2));
+    A.iden_a = R(1.0 / double(A.den_a));
+    A.iden_b = R(1.0 / double(A.den_b));
     A.mt = mt;
     A.langevin_c = R(langevin_c);
     A.stop_eps = double(R(stop_eps));

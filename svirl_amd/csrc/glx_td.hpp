@@ -44,6 +44,7 @@ template <typename R> struct ASweepArgs {
     R dt_rho, dt_rho_kappa2;
     R rh_a, rh_b;            // 2 kappa^2 H idy, 2 kappa^2 H idx
     R den_a, den_b;          // 1 + 2 dt rho kappa^2 idy2 (idx2)
+    R iden_a, iden_b;        // 1/den_a, 1/den_b (FP32 multiplies by them instead of dividing)
     const uint8_t *mt;
     const cplx<R> *psi;
     const R *src;            // supercurrent source (may alias ab_next)
@@ -133,13 +134,14 @@ __device__ __forceinline__ PsiRow<R> psi_load_row(const PsiSweepArgs<R> &A, cons
                                                   const R *emp, int j) {
     const uint32_t Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1;
     const uint32_t jr = min((uint32_t)j, Ny - 1), jp = min((uint32_t)j + 1, Ny - 1), jq = min((uint32_t)j, Ny - 2);
+    const uint32_t oc = X.cc * (uint32_t)sizeof(R), oa = X.ca * (uint32_t)sizeof(R);
     PsiRow<R> L;
-    L.pp = ld(A.psi, X.cc + Nx * jp);
-    L.av = A.ab[X.ca + Nxa * jr];
-    L.bv = A.ab[Nxa * Ny + X.cc + Nx * jq];
-    L.cell = mtp[X.ca + Nxa * jq];
-    L.rhs = ld(A.rhs, X.cc + Nx * jr);
-    L.e = emp[X.cc + Nx * jr];
+    L.pp = ld_at(A.psi + (size_t)Nx * jp, 2 * oc);
+    L.av = at(A.ab + (size_t)Nxa * jr, oa);
+    L.bv = at(A.ab + (size_t)Nxa * Ny + (size_t)Nx * jq, oc);
+    L.cell = at(mtp + (size_t)Nxa * jq, X.ca);
+    L.rhs = ld_at(A.rhs + (size_t)Nx * jr, 2 * oc);
+    L.e = at(emp + (size_t)Nx * jr, oc);
     return L;
 }
 
@@ -267,7 +269,7 @@ __device__ __forceinline__ R psi_rows(const PsiSweepArgs<R> &A, int s, int ja, i
                                 sal, ca, sa, cbm, sbm, cb, sb);
             }
             if (noisy && A.jstep == 0) st(A.rhs_out, c + rowoff, rhs);
-            st(A.psi_next, c + rowoff, nxt);
+            st_at(A.psi_next + rowoff, 2 * X.cc * (uint32_t)sizeof(R), nxt);
             rr = fmax(rr, fmax(fabs(nxt.re - pc.re), fabs(nxt.im - pc.im)));
         }
         // roll the window
@@ -388,7 +390,7 @@ __device__ __forceinline__ void psi_rows2(const PsiSweepArgs<R> &A, int s, int j
                 if (in_u)
                     nxt = psi_apply(A, rhs_u, den_u, pl_re, pl_im, pr_re, pr_im, qm, q, cal2, sal2, ca2, sa2, cb2, sb2,
                                     cb1, sb1);
-                st(A.psi_next, c + (long)Nx * u, nxt);
+                st_at(A.psi_next + (size_t)Nx * u, 2 * X.cc * (uint32_t)sizeof(R), nxt);
                 r2 = fmax(r2, fmax(fabs(nxt.re - qc.re), fabs(nxt.im - qc.im)));
             }
         }
@@ -442,23 +444,25 @@ template <typename R>
 __device__ __forceinline__ ARow<R> a_load_row(const ASweepArgs<R> &A, const ColCtx &X, const uint8_t *mtp, int j) {
     const uint32_t Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1, Na = Nxa * Ny;
     const uint32_t jp = min((uint32_t)j + 1, Ny - 1), jq = min((uint32_t)j, Ny - 2);
+    const uint32_t oc = X.cc * (uint32_t)sizeof(R), oa = X.ca * (uint32_t)sizeof(R);
+    const size_t ra = (size_t)Nxa * (uint32_t)j, rb = (size_t)Na + (size_t)Nx * jq;
     ARow<R> L;
-    L.ap = A.ab[X.ca + Nxa * jp];
-    L.bc = A.ab[Na + X.cc + Nx * jq];
-    L.pp = ld(A.psi, X.cc + Nx * jp);
-    L.cell = mtp[X.ca + Nxa * jq];
-    const uint32_t na = X.ca + Nxa * (uint32_t)j, nb = Na + X.cc + Nx * jq;
-    L.sa = A.src[na];
-    L.ra = A.rhs[na];
-    L.sb = A.src[nb];
-    L.rb = A.rhs[nb];
+    L.ap = at(A.ab + (size_t)Nxa * jp, oa);
+    L.bc = at(A.ab + rb, oc);
+    L.pp = ld_at(A.psi + (size_t)Nx * jp, 2 * oc);
+    L.cell = at(mtp + (size_t)Nxa * jq, X.ca);
+    L.sa = at(A.src + ra, oa);
+    L.ra = at(A.rhs + ra, oa);
+    L.sb = at(A.src + rb, oc);
+    L.rb = at(A.rhs + rb, oc);
     return L;
 }
 
 // Backward-Euler Jacobi update of the a link (c, j) (td.h:366-410): its
 // supercurrent jl, the right-hand side ra, and the neighbouring links am =
 // a(c, j-1), bm = b(c, j-1), brm = b(c+1, j-1), ap = a(c, j+1), bc = b(c, j),
-// brc = b(c+1, j).
+// brc = b(c+1, j). FP32 multiplies by the rounded reciprocal of the
+// constant denominator (within 1 ulp of the division).
 template <typename R>
 __device__ __forceinline__ R a_link_update(const ASweepArgs<R> &A, int j, R ra, R jl, R am, R bm, R brm, R ap, R bc,
                                            R brc) {
@@ -469,7 +473,9 @@ __device__ __forceinline__ R a_link_update(const ASweepArgs<R> &A, int j, R ra, 
     R t1 = R(0), t2 = R(0);
     if (j > 0) t1 = A.idy2 * am - A.idxy * bm + A.idxy * brm;
     if (j + 1 < Ny) t2 = A.idy2 * ap + A.idxy * bc - A.idxy * brc;
-    return (ra + A.dt_rho * (jl + rh) + A.dt_rho_kappa2 * dd * (t1 + t2)) / A.den_a;
+    const R num = ra + A.dt_rho * (jl + rh) + A.dt_rho_kappa2 * dd * (t1 + t2);
+    if constexpr (std::is_same<R, float>::value) return num * A.iden_a;
+    else return num / A.den_a;
 }
 
 // The b link (c, j) (td.h:412-456): blc = b(c-1, j), alc = a(c-1, j),
@@ -484,7 +490,9 @@ __device__ __forceinline__ R b_link_update(const ASweepArgs<R> &A, int c, R rb, 
     R t1 = R(0), t2 = R(0);
     if (c > 0) t1 = A.idx2 * blc - A.idxy * alc + A.idxy * alp;
     if (c + 1 < Nx) t2 = A.idx2 * brc + A.idxy * ac - A.idxy * ap;
-    return (rb + A.dt_rho * (jl + rh) + A.dt_rho_kappa2 * dd * (t1 + t2)) / A.den_b;
+    const R num = rb + A.dt_rho * (jl + rh) + A.dt_rho_kappa2 * dd * (t1 + t2);
+    if constexpr (std::is_same<R, float>::value) return num * A.iden_b;
+    else return num / A.den_b;
 }
 
 // Supercurrent of a link, id * Im(conj psi0 U psi1) in the reference's form.
@@ -557,7 +565,7 @@ __device__ __forceinline__ R a_rows(const ASweepArgs<R> &A, int s, int ja, int j
             const R jl = (cellm != 0 || cellc) ? link_current(A.idx, pc, pr_re, pr_im, csa, ssa) : R(0);
             const R v = a_link_update(A, j, ra, jl, am, bm, brm, ap, bc, brc);
             if (noisy && A.jstep == 0) A.rhs_out[na] = ra;
-            A.ab_next[na] = v;
+            st_at(A.ab_next + (size_t)Nxa * j, X.cc * (uint32_t)sizeof(R), v);
             rr = fmax(rr, fabs(v - ac));
         }
         if (ownb) {
@@ -565,7 +573,7 @@ __device__ __forceinline__ R a_rows(const ASweepArgs<R> &A, int s, int ja, int j
             const R jl = (cl_c || cellc) ? link_current(A.idy, pc, pp.re, pp.im, csb, ssb) : R(0);
             const R v = b_link_update(A, c, rb, jl, blc, alc, alp, brc, ac, ap);
             if (noisy && A.jstep == 0) A.rhs_out[nb] = rb;
-            A.ab_next[nb] = v;
+            st_at(A.ab_next + (size_t)Na + (size_t)Nx * j, X.cc * (uint32_t)sizeof(R), v);
             rr = fmax(rr, fabs(v - bc));
         }
         // roll
@@ -687,12 +695,12 @@ __device__ __forceinline__ void a_rows2(const ASweepArgs<R> &A, int s, int ja, i
             if (u >= ja && own) {
                 if (hasa) {
                     const R v = a_link_update(A, u, ra1, jla1, am1, bm1, brm1, va, bc1, brc1);
-                    A.ab_next[c + (long)Nxa * u] = v;
+                    st_at(A.ab_next + (size_t)Nxa * u, X.cc * (uint32_t)sizeof(R), v);
                     r2 = fmax(r2, fabs(v - ac1));
                 }
                 if (u < Ny - 1) {
                     const R v = b_link_update(A, c, rb1, jlb1, blc1, alc1, alp1, brc1, ac1, va);
-                    A.ab_next[Na + c + (long)Nx * u] = v;
+                    st_at(A.ab_next + (size_t)Na + (size_t)Nx * u, X.cc * (uint32_t)sizeof(R), v);
                     r2 = fmax(r2, fabs(v - bc1));
                 }
             }
