@@ -95,24 +95,28 @@ __host__ __device__ __forceinline__ void sincos_r(float x, float *s, float *c) {
     float r = fmaf(-k, 1.5703125f, x);
     r = fmaf(-k, 4.837512969970703e-4f, r);
     r = fmaf(-k, 7.549790126404332e-08f, r);
+    // Taylor terms through r^9 (sin) and r^10 (cos): the first omitted terms,
+    // (pi/4)^11/11! = 1.7e-9 and (pi/4)^12/12! = 1.1e-10, are far below half
+    // an FP32 ulp
     const float z = r * r;
-    float ps = -1.0f / 39916800.0f;
-    ps = fmaf(ps, z, 1.0f / 362880.0f);
+    float ps = 1.0f / 362880.0f;
     ps = fmaf(ps, z, -1.0f / 5040.0f);
     ps = fmaf(ps, z, 1.0f / 120.0f);
     ps = fmaf(ps, z, -1.0f / 6.0f);
     const float sr = fmaf(r * z, ps, r);
-    float pc = 1.0f / 479001600.0f;
-    pc = fmaf(pc, z, -1.0f / 3628800.0f);
+    float pc = -1.0f / 3628800.0f;
     pc = fmaf(pc, z, 1.0f / 40320.0f);
     pc = fmaf(pc, z, -1.0f / 720.0f);
     pc = fmaf(pc, z, 1.0f / 24.0f);
     pc = fmaf(pc, z, -0.5f);
     const float cr = fmaf(z, pc, 1.0f);
-    const int q = (int)k & 3;
-    const float sv = (q & 1) ? cr : sr, cv = (q & 1) ? sr : cr;
-    *s = (q & 2) ? -sv : sv;
-    *c = ((q + 1) & 2) ? -cv : cv;
+    // quadrant q = k mod 4: swap on odd q, sign of sin from bit 1 of q, sign
+    // of cos from bit 1 of q + 1, applied to the sign bit directly
+    const uint32_t q = (uint32_t)(int)k;
+    const bool odd = (q & 1u) != 0u;
+    const float sv = odd ? cr : sr, cv = odd ? sr : cr;
+    *s = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, sv) ^ ((q << 30) & 0x80000000u));
+    *c = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, cv) ^ (((q + 1u) << 30) & 0x80000000u));
 }
 
 // Thomas Wang's 32-bit integer hash and the two Langevin streams built on
