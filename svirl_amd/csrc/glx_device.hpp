@@ -259,8 +259,9 @@ struct Team {
 
 // Call from every thread of every workgroup of the team the same number of
 // times with the same `epoch` (barriers the team has passed; incremented
-// here). `v` is the workgroup's value (thread 0's is used): two 16-bit
-// fields. On success returns true and sets `vmax` (all threads) to the
+// here). `v` is the workgroup's value (thread 0's is used): three 10-bit
+// fields (bits 0-9, 10-19, 20-29). On success returns true and sets `vmax`
+// (all threads) to the
 // field-wise max over the team; returns false when a poll timed out anywhere
 // in the grid. A one-workgroup team synchronises in the workgroup only.
 __device__ __forceinline__ bool team_sync_max(const GridSync &gs, const Team &tm, unsigned long long &epoch, int v,
@@ -285,17 +286,17 @@ __device__ __forceinline__ bool team_sync_max(const GridSync &gs, const Team &tm
     }
     if (local == 0) {
         const unsigned long long t0 = wall_clock64();
-        int ok = 1, m = 0, mh = 0;
+        int ok = 1, m0 = 0, m1 = 0, m2 = 0;
         for (;;) {
             int all = 1;
-            m = 0;
-            mh = 0;
+            m0 = m1 = m2 = 0;
             for (int k = threadIdx.x; k < tm.nb; k += blockDim.x) {
                 const unsigned long long a = ld_relaxed(gs.arrive + tm.b0 + k);
                 if ((a >> 32) != epoch) all = 0;
                 else {
-                    m = max(m, (int)(unsigned)(a & 0xffffull));
-                    mh = max(mh, (int)(unsigned)((a >> 16) & 0xffffull));
+                    m0 = max(m0, (int)(unsigned)(a & 0x3ffull));
+                    m1 = max(m1, (int)(unsigned)((a >> 10) & 0x3ffull));
+                    m2 = max(m2, (int)(unsigned)((a >> 20) & 0x3ffull));
                 }
             }
             int abort = 0;
@@ -305,9 +306,10 @@ __device__ __forceinline__ bool team_sync_max(const GridSync &gs, const Team &tm
             __builtin_amdgcn_s_sleep(1);
         }
         if (ok) {  // uniform over the workgroup
-            m = block_max_int(m, s_red);
-            mh = block_max_int(mh, s_red);
-            if (threadIdx.x == 0) s_max = (mh << 16) | m;
+            m0 = block_max_int(m0, s_red);
+            m1 = block_max_int(m1, s_red);
+            m2 = block_max_int(m2, s_red);
+            if (threadIdx.x == 0) s_max = (m2 << 20) | (m1 << 10) | m0;
             __syncthreads();
             if (threadIdx.x < tm.nrel) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");

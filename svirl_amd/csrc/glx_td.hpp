@@ -95,6 +95,9 @@ template <> struct Pipe<double, false> { static constexpr int ROWS = 2, ROWS2 = 
 template <> struct Pipe<float, true> { static constexpr int ROWS = 2, ROWS2 = 1, AROWS2 = 1; };
 template <> struct Pipe<double, true> { static constexpr int ROWS = 1, ROWS2 = 1, AROWS2 = 1; };
 
+// Sweeps per fused psi pass in the persistent single-instance solve.
+template <typename R> struct Fuse { static constexpr int PSI = 3; };
+
 // Per-lane column context of a strip: c may lie outside the grid (halo lanes
 // at the edges); cc and ca are c clamped into the node and link ranges, so
 // that every load of a row is issued unconditionally from a valid address
@@ -414,6 +417,167 @@ __device__ __forceinline__ void psi_rows2(const PsiSweepArgs<R> &A, int s, int j
     };
 
     const int t1 = jb + 1;  // rows t in [t0, t1)
+    PsiRow<R> P0[B], P1[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) P0[k] = psi_load_row(A, X, mtp, emp, min(t0 + k, t1 - 1));
+    for (int t = t0; t < t1; t += 2 * B) {
+#pragma unroll
+        for (int k = 0; k < B; ++k) P1[k] = psi_load_row(A, X, mtp, emp, min(t + B + k, t1 - 1));
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            if (t + k < t1) row(P0[k], t + k);
+#pragma unroll
+        for (int k = 0; k < B; ++k) P0[k] = psi_load_row(A, X, mtp, emp, min(t + 2 * B + k, t1 - 1));
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            if (t + B + k < t1) row(P1[k], t + B + k);
+    }
+}
+
+// Three consecutive psi sweeps j, j+1, j+2 in one pass over rows [ja, jb) of
+// strip s (strips of WAVE - 6 columns, three halo lanes a side), built like
+// psi_rows2 with one more stage: sweep j runs at row t on lanes 1..62,
+// sweep j+1 at row t-1 on lanes 2..61 and sweep j+2 at row t-2 on lanes
+// 3..60, each from the previous stage's rows kept in registers. Link
+// factors, cells and node diagonals are computed once for the three sweeps.
+// Writes psi^{j+3} (and, at jstep 0, the Langevin right-hand side); r1, r2
+// and r3 get the lane's largest |delta| of each sweep.
+template <typename R, bool GAUGE, bool SH = false>
+__device__ __forceinline__ void psi_rows3(const PsiSweepArgs<R> &A, int s, int ja, int jb, R &r1, R &r2, R &r3) {
+    constexpr int B = Pipe<R, SH>::ROWS2;
+    const ColCtx X = col_ctx<3>(A.Nx, s);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int c = X.c;
+    const bool own = X.own;                                   // lanes 3..60: outputs
+    const bool mid1 = X.col && lane >= 1 && lane < WAVE - 1;  // lanes 1..62: sweep j
+    const bool mid2 = X.col && lane >= 2 && lane < WAVE - 2;  // lanes 2..61: sweep j+1
+    const int Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1;
+    const R *b = A.ab + (long)Nxa * Ny;
+    const bool noisy = A.rhs_out != nullptr;
+    const bool noise_on = noisy && A.jstep == 0 && A.langevin_c > R(1.0e-32);
+    const bool has_mt = A.mt != nullptr;
+    const uint8_t *mtp = has_mt ? A.mt : reinterpret_cast<const uint8_t *>(A.ab);
+    const R *emp = A.eps_map ? A.eps_map : reinterpret_cast<const R *>(A.rhs);
+    const R gb = GAUGE ? gauge_phase_b<R>(A, c) : R(0);
+    const cplx<R> zero = {R(0), R(0)};
+    const int t0 = ja > 1 ? ja - 2 : 0;
+
+    // sweep j window: psi^j at rows t-1, t; U of b(t-1); cell (c, t-1)
+    cplx<R> pm = zero, pc = zero;
+    R cb1 = R(1), sb1 = R(0);
+    int cell1 = 0;
+    if (X.col) {
+        pc = ld(A.psi, c + (long)Nx * t0);
+        if (t0 > 0) {
+            pm = ld(A.psi, c + (long)Nx * (t0 - 1));
+            sincos_r(link_phase<R, GAUGE>(A.dy, b[c + (long)Nx * (t0 - 1)], gb), &sb1, &cb1);
+            if (c < Nxa) cell1 = has_mt ? int(A.mt[c + (long)Nxa * (t0 - 1)]) : 1;
+        }
+    }
+    mask_link(shfl_up1(cell1) != 0 || cell1 != 0, cb1, sb1);  // b(c, t0-1)
+    // sweep j+1 window (row u = t - 1): psi^{j+1} at u-1, u; U of a(u),
+    // b(u-1); rhs, diagonal and material test of node (c, u)
+    cplx<R> qm = zero, qc = zero, rhs_u = zero;
+    R ca2 = R(0), sa2 = R(0), cal2 = R(0), sal2 = R(0), cb2 = R(0), sb2 = R(0), den_u = R(1);
+    bool in_u = false;
+    // sweep j+2 window (row v = t - 2): psi^{j+2} at v-1, v; U of a(v),
+    // b(v-1); rhs, diagonal and material test of node (c, v)
+    cplx<R> wm = zero, wc = zero, rhs_v = zero;
+    R ca3 = R(0), sa3 = R(0), cal3 = R(0), sal3 = R(0), cb3 = R(0), sb3 = R(0), den_v = R(1);
+    bool in_v = false;
+
+    auto row = [&](const PsiRow<R> &cu, int t) {
+        const long rowoff = (long)Nx * t;
+        const bool real = t < Ny;  // t >= Ny: past the last row (later sweeps only)
+        const cplx<R> pp = (X.col && t + 1 < Ny) ? cu.pp : zero;
+        const R av = X.hasa ? cu.av : R(0);
+        const R bv = (X.col && t < Ny - 1) ? cu.bv : R(0);
+        const int cellc = (X.hasa && t < Ny - 1) ? (has_mt ? cu.cell : 1) : 0;
+        cplx<R> rhs = cu.rhs;
+        const R e = A.eps_map ? cu.e : A.eps;
+        R sa, ca, sb, cb;
+        sincos_r(link_phase<R, GAUGE>(A.dx, av, GAUGE ? gauge_phase_a<R>(A, t) : R(0)), &sa, &ca);
+        sincos_r(link_phase<R, GAUGE>(A.dy, bv, gb), &sb, &cb);
+        const bool mm = shfl_up1(cell1) != 0;  // cell (c-1, t-1)
+        const bool mp = shfl_up1(cellc) != 0;  // cell (c-1, t)
+        const bool pmc = cell1 != 0, ppc = cellc != 0;
+        mask_link(pmc || ppc, ca, sa);  // a(c, t): cells (c, t-1), (c, t)
+        mask_link(mp || ppc, cb, sb);   // b(c, t): cells (c-1, t), (c, t)
+        const R cal = shfl_up1(ca), sal = shfl_up1(sa);
+        const bool inside = mm || mp || pmc || ppc;
+
+        // sweep j at row t -> psi^{j+1}(t)
+        cplx<R> q = zero;
+        R den = R(1);
+        {
+            const R pl_re = shfl_up1(pc.re), pl_im = shfl_up1(pc.im);
+            const R pr_re = shfl_dn1(pc.re), pr_im = shfl_dn1(pc.im);
+            if (mid1 && real && inside) {
+                if (noise_on) psi_noise(A, (uint32_t)(c + rowoff), rhs);
+                den = psi_den(A, rhs, e, mm, mp, pmc, ppc);
+                q = psi_apply(A, rhs, den, pl_re, pl_im, pr_re, pr_im, pm, pp, cal, sal, ca, sa, cb1, sb1, cb, sb);
+            }
+            if (own && real && t >= ja && t < jb) {
+                if (noisy && A.jstep == 0) st(A.rhs_out, c + rowoff, rhs);
+                r1 = fmax(r1, fmax(fabs(q.re - pc.re), fabs(q.im - pc.im)));
+            }
+        }
+        // sweep j+1 at row u = t - 1 -> psi^{j+2}(u)
+        const int u = t - 1;
+        cplx<R> w = zero;
+        if (u >= 0 && u < Ny) {
+            const R pl_re = shfl_up1(qc.re), pl_im = shfl_up1(qc.im);
+            const R pr_re = shfl_dn1(qc.re), pr_im = shfl_dn1(qc.im);
+            if (mid2 && in_u)
+                w = psi_apply(A, rhs_u, den_u, pl_re, pl_im, pr_re, pr_im, qm, q, cal2, sal2, ca2, sa2, cb2, sb2, cb1,
+                              sb1);
+            if (own && u >= ja && u < jb) r2 = fmax(r2, fmax(fabs(w.re - qc.re), fabs(w.im - qc.im)));
+        }
+        // sweep j+2 at row v = t - 2 -> psi^{j+3}(v)
+        const int v = t - 2;
+        if (v >= ja) {
+            const R pl_re = shfl_up1(wc.re), pl_im = shfl_up1(wc.im);
+            const R pr_re = shfl_dn1(wc.re), pr_im = shfl_dn1(wc.im);
+            if (own) {
+                cplx<R> nxt = zero;
+                if (in_v)
+                    nxt = psi_apply(A, rhs_v, den_v, pl_re, pl_im, pr_re, pr_im, wm, w, cal3, sal3, ca3, sa3, cb3, sb3,
+                                    cb2, sb2);
+                st_at(A.psi_next + (size_t)Nx * v, 2 * X.cc * (uint32_t)sizeof(R), nxt);
+                r3 = fmax(r3, fmax(fabs(nxt.re - wc.re), fabs(nxt.im - wc.im)));
+            }
+        }
+        // roll the three windows
+        wm = wc;
+        wc = w;
+        cb3 = cb2;
+        sb3 = sb2;
+        ca3 = ca2;
+        sa3 = sa2;
+        cal3 = cal2;
+        sal3 = sal2;
+        in_v = in_u;
+        den_v = den_u;
+        rhs_v = rhs_u;
+        qm = qc;
+        qc = q;
+        cb2 = cb1;
+        sb2 = sb1;
+        ca2 = ca;
+        sa2 = sa;
+        cal2 = cal;
+        sal2 = sal;
+        in_u = inside;
+        den_u = den;
+        rhs_u = rhs;
+        pm = pc;
+        pc = pp;
+        cb1 = cb;
+        sb1 = sb;
+        cell1 = cellc;
+    };
+
+    const int t1 = jb + 2;  // rows t in [t0, t1)
     PsiRow<R> P0[B], P1[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) P0[k] = psi_load_row(A, X, mtp, emp, min(t0 + k, t1 - 1));
@@ -852,14 +1016,21 @@ __device__ __forceinline__ unsigned long long *state64(int *state, int k) {
     return reinterpret_cast<unsigned long long *>(state + k);
 }
 
-// The workgroup's largest scaled residuals of up to two sweeps (each clamped
-// to 0xffff: only the comparison with 10000 steers the solve), team barrier,
-// team-wide residuals r1 (low 16 bits of r_out) and r2 (high 16 bits).
-__device__ __forceinline__ bool sweep_sync(const GridSync &gs, const Team &tm, int r1, int r2,
+// The scaled residuals of up to three sweeps of one pass go through the team
+// barrier as 10-bit fields min(r, 16383) >> 4 (only the comparison with
+// 10000 steers the solve, and r < 10000 iff the field is below 625 =
+// 10000 / 16). sweep_sync: the workgroup's largest residuals, team barrier,
+// team-wide fields in r_out; res_conv(r_out, k): sweep k of the pass met the
+// stop rule everywhere.
+__device__ __forceinline__ int res_field(int r) { return min(r, 16383) >> 4; }
+__device__ __forceinline__ bool res_conv(int packed, int k) { return ((packed >> (10 * k)) & 0x3ff) < 625; }
+
+__device__ __forceinline__ bool sweep_sync(const GridSync &gs, const Team &tm, int r1, int r2, int r3,
                                            unsigned long long &epoch, int *lds, int &r_out) {
-    r1 = block_max_int(min(r1, 0xffff), lds);
-    r2 = block_max_int(min(r2, 0xffff), lds);
-    return team_sync_max(gs, tm, epoch, (r2 << 16) | r1, r_out);
+    r1 = block_max_int(res_field(r1), lds);
+    r2 = block_max_int(res_field(r2), lds);
+    r3 = block_max_int(res_field(r3), lds);
+    return team_sync_max(gs, tm, epoch, (r3 << 20) | (r2 << 10) | r1, r_out);
 }
 
 // Result of one solve, identical in every workgroup of the team.
@@ -869,51 +1040,64 @@ struct SolveOut {
     bool ok;    // false: a barrier timed out
 };
 
-// The psi solve of one TD step on the team's waves (w of W): fused pairs
-// (psi_rows2: two sweeps per pass over memory). A pair (j, j+1) reports both
-// residuals; when sweep j already met the stop rule, sweep j is recomputed
-// alone from the pair's input, so the result and the sweep count are those
-// of sweep-by-sweep iteration. When the previous step needed an odd number
-// of sweeps (pred), the solve opens with one single sweep so that the pairs
-// end where the solve is likely to end.
+// The psi solve of one TD step on the team's waves (w of W), in passes of
+// KMAX fused sweeps (psi_rows3 or psi_rows2: several sweeps per pass over
+// memory). A pass reports each sweep's residual; when an earlier sweep of the
+// pass already met the stop rule, the solve is redone up to that sweep from
+// the pass's input (psi_rows or psi_rows2 into the same output), so the
+// result and the sweep count are those of sweep-by-sweep iteration. The
+// first pass is shortened to pred mod KMAX sweeps (pred: the previous step's
+// count) so that the full passes end where the solve is likely to end.
 template <typename R, bool GAUGE, bool SH = false>
 __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const GridSync &gs, const Team &tm, long w,
                                               long W, int pc, int qc, int pred, unsigned long long &epoch, int *lds) {
+    constexpr int KMAX = SH ? 2 : Fuse<R>::PSI;
     const int o1 = (pc + 1) % 3, o2 = (pc + 2) % 3;
     PsiSweepArgs<R> A = T.psi;
     A.ab = pick3(T.Q, qc);
     A.rhs_out = T.rhs_psi;
     int in = pc, out = o1, j = 0, nsw = 0;
     bool done = false;
+    auto pass = [&](int k, R &r1, R &r2, R &r3) {
+        if constexpr (KMAX == 3) {
+            if (k == 3) {
+                for_rows_rm<3>(A.Nx, A.Ny, w, W,
+                               [&](int s, int ja, int jb) { psi_rows3<R, GAUGE, SH>(A, s, ja, jb, r1, r2, r3); });
+                return;
+            }
+        }
+        if (k == 2)
+            for_rows_rm<2>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { psi_rows2<R, GAUGE, SH>(A, s, ja, jb, r1, r2); });
+        else
+            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { r1 = fmax(r1, psi_rows<R, GAUGE, SH>(A, s, ja, jb)); });
+    };
     while (!done && j < T.max_sweeps) {
         A.psi = pick3(T.P, in);
         A.psi_next = pick3(T.P, out);
         A.rhs = (T.rhs_psi && j > 0) ? T.rhs_psi : pick3(T.P, pc);
         A.jstep = (uint32_t)j;
-        const bool pair = !(j == 0 && (pred & 1)) && j + 2 <= T.max_sweeps;
-        R r1 = R(0), r2 = R(0);
-        if (pair)
-            for_rows_rm<2>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { psi_rows2<R, GAUGE, SH>(A, s, ja, jb, r1, r2); });
-        else
-            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { r1 = fmax(r1, psi_rows<R, GAUGE, SH>(A, s, ja, jb)); });
+        int k = (j == 0 && pred % KMAX != 0) ? pred % KMAX : KMAX;
+        k = min(k, T.max_sweeps - j);
+        R r1 = R(0), r2 = R(0), r3 = R(0);
+        pass(k, r1, r2, r3);
         int r = 0;
-        if (!sweep_sync(gs, tm, scaled_residual<R>(r1, A.stop_eps), scaled_residual<R>(r2, A.stop_eps), epoch, lds, r))
+        if (!sweep_sync(gs, tm, scaled_residual<R>(r1, A.stop_eps), scaled_residual<R>(r2, A.stop_eps),
+                        scaled_residual<R>(r3, A.stop_eps), epoch, lds, r))
             return {in, nsw, false};
-        const unsigned ru = (unsigned)r;
-        const bool c1 = (ru & 0xffffu) < 10000u, c2 = (ru >> 16) < 10000u;
-        if (pair && c1) {
-            // sweep j met the stop rule: redo it alone into the same output
-            R rr = R(0);
-            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, psi_rows<R, GAUGE, SH>(A, s, ja, jb)); });
+        // the first sweep of the pass that met the stop rule, or k
+        const int m = (k > 1 && res_conv(r, 0)) ? 1 : ((k > 2 && res_conv(r, 1)) ? 2 : k);
+        if (m < k) {
+            // redo the pass up to sweep m into the same output
+            R d1 = R(0), d2 = R(0), d3 = R(0);
+            pass(m, d1, d2, d3);
             int rdummy = 0;
-            if (!sweep_sync(gs, tm, 0, 0, epoch, lds, rdummy)) return {in, nsw, false};
-            nsw = j + 1;
+            if (!sweep_sync(gs, tm, 0, 0, 0, epoch, lds, rdummy)) return {in, nsw, false};
             done = true;
         } else {
-            j += pair ? 2 : 1;
-            nsw = j;
-            done = pair ? c2 : c1;
+            done = res_conv(r, k - 1);
         }
+        j += m;
+        nsw = j;
         in = out;
         out = (out == o1) ? o2 : o1;
     }
@@ -952,15 +1136,14 @@ __device__ __forceinline__ SolveOut a_solve(const TDSolveArgs<R> &T, const GridS
         else
             for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { r1 = fmax(r1, a_rows<R, GAUGE, SH>(A, s, ja, jb)); });
         int r = 0;
-        if (!sweep_sync(gs, tm, scaled_residual<R>(r1, A.stop_eps), scaled_residual<R>(r2, A.stop_eps), epoch, lds, r))
+        if (!sweep_sync(gs, tm, scaled_residual<R>(r1, A.stop_eps), scaled_residual<R>(r2, A.stop_eps), 0, epoch, lds, r))
             return {in, nsw, false};
-        const unsigned ru = (unsigned)r;
-        const bool c1 = (ru & 0xffffu) < 10000u, c2 = (ru >> 16) < 10000u;
+        const bool c1 = res_conv(r, 0), c2 = res_conv(r, 1);
         if (pair && c1) {
             R rr = R(0);
             for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, a_rows<R, GAUGE, SH>(A, s, ja, jb)); });
             int rdummy = 0;
-            if (!sweep_sync(gs, tm, 0, 0, epoch, lds, rdummy)) return {in, nsw, false};
+            if (!sweep_sync(gs, tm, 0, 0, 0, epoch, lds, rdummy)) return {in, nsw, false};
             nsw = j + 1;
             done = true;
         } else {
