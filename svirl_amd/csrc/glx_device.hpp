@@ -36,6 +36,14 @@ template <> __device__ __forceinline__ void st(cplx<float> *p, long n, cplx<floa
     reinterpret_cast<float2 *>(p)[n] = make_float2(v.re, v.im);
 }
 
+// A wave-uniform value kept in a vector register: the row kernels hold many
+// loop-invariant scalars, and the scalar register file overflows into
+// lane spills inside the hot loops if all of them live there.
+template <typename T> __device__ __forceinline__ T in_vgpr(T x) {
+    asm("" : "+v"(x));
+    return x;
+}
+
 // Element at byte offset off (a per-lane 32-bit value) from a wave-uniform
 // row pointer: the compiler keeps base + lane offset per array in registers
 // and adds the scalar row offset with one vector instruction per access.

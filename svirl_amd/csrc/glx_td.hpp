@@ -51,7 +51,8 @@ template <typename R> struct ASweepArgs {
     const R *rhs;            // right-hand side read
     R *rhs_out;              // Langevin: noisy rhs written here at jstep 0 (else NULL)
     const R *ab;             // A^j
-    R *ab_next;              // A^{j+1}
+    R *ab_next;              // A^{j+1} (a fused pass: its last sweep's result)
+    R *ab_mid;               // three-sweep pass: A^{j+2} (the next sweep's source)
     R langevin_c;
     uint32_t jstep, rand_t;
     double stop_eps;
@@ -90,10 +91,10 @@ template <typename R, typename Args> __device__ __forceinline__ R gauge_phase_b(
 // to that row (cache hits, never used).
 // The batched kernel runs its solves with SH (shallow) batches, which keep
 // its register count within two waves per SIMD.
-template <typename R, bool SH = false> struct Pipe { static constexpr int ROWS = 2, ROWS2 = 2, AROWS2 = 1; };
-template <> struct Pipe<double, false> { static constexpr int ROWS = 2, ROWS2 = 2, AROWS2 = 2; };
-template <> struct Pipe<float, true> { static constexpr int ROWS = 2, ROWS2 = 1, AROWS2 = 1; };
-template <> struct Pipe<double, true> { static constexpr int ROWS = 1, ROWS2 = 1, AROWS2 = 1; };
+template <typename R, bool SH = false> struct Pipe { static constexpr int ROWS = 2, ROWS2 = 2, AROWS2 = 1, AROWS3 = 1; };
+template <> struct Pipe<double, false> { static constexpr int ROWS = 2, ROWS2 = 2, AROWS2 = 2, AROWS3 = 1; };
+template <> struct Pipe<float, true> { static constexpr int ROWS = 2, ROWS2 = 1, AROWS2 = 1, AROWS3 = 1; };
+template <> struct Pipe<double, true> { static constexpr int ROWS = 1, ROWS2 = 1, AROWS2 = 1, AROWS3 = 1; };
 
 // Sweeps per fused psi pass in the persistent single-instance solve.
 template <typename R> struct Fuse { static constexpr int PSI = 3; };
@@ -622,6 +623,33 @@ __device__ __forceinline__ ARow<R> a_load_row(const ASweepArgs<R> &A, const ColC
     return L;
 }
 
+// The A sweep arguments for a row kernel. In the FP32 persistent solves the
+// constants of the link updates are held in vector registers (see in_vgpr);
+// FP64 keeps them in scalar registers (two vector registers each would push
+// its kernels past two waves per SIMD), and so do the batched kernels.
+template <typename R, bool SH>
+__device__ __forceinline__ const ASweepArgs<R> &a_row_args(const ASweepArgs<R> &A0, ASweepArgs<R> &tmp) {
+    if constexpr (std::is_same<R, float>::value && !SH) {
+        tmp = A0;
+        tmp.dx = in_vgpr(tmp.dx);
+        tmp.dy = in_vgpr(tmp.dy);
+        tmp.idx = in_vgpr(tmp.idx);
+        tmp.idy = in_vgpr(tmp.idy);
+        tmp.idx2 = in_vgpr(tmp.idx2);
+        tmp.idy2 = in_vgpr(tmp.idy2);
+        tmp.idxy = in_vgpr(tmp.idxy);
+        tmp.dt_rho = in_vgpr(tmp.dt_rho);
+        tmp.dt_rho_kappa2 = in_vgpr(tmp.dt_rho_kappa2);
+        tmp.rh_a = in_vgpr(tmp.rh_a);
+        tmp.rh_b = in_vgpr(tmp.rh_b);
+        tmp.iden_a = in_vgpr(tmp.iden_a);
+        tmp.iden_b = in_vgpr(tmp.iden_b);
+        return tmp;
+    } else {
+        return A0;
+    }
+}
+
 // Backward-Euler Jacobi update of the a link (c, j) (td.h:366-410): its
 // supercurrent jl, the right-hand side ra, and the neighbouring links am =
 // a(c, j-1), bm = b(c, j-1), brm = b(c+1, j-1), ap = a(c, j+1), bc = b(c, j),
@@ -669,7 +697,9 @@ __device__ __forceinline__ R link_current(R id, cplx<R> p0, R x1, R y1, R cs, R 
 // One A sweep over rows [ja, jb) of strip s (both link families of every
 // owned node). Returns the lane's largest |delta|.
 template <typename R, bool GAUGE, bool SH = false>
-__device__ __forceinline__ R a_rows(const ASweepArgs<R> &A, int s, int ja, int jb) {
+__device__ __forceinline__ R a_rows(const ASweepArgs<R> &A0, int s, int ja, int jb) {
+    ASweepArgs<R> tmp;
+    const ASweepArgs<R> &A = a_row_args<R, SH>(A0, tmp);
     constexpr int B = Pipe<R, SH>::ROWS;
     const ColCtx X = col_ctx<1>(A.Nx, s);
     const int c = X.c;
@@ -775,7 +805,9 @@ __device__ __forceinline__ R a_rows(const ASweepArgs<R> &A, int s, int ja, int j
 // Langevin right-hand side); r1 and r2 get the lane's largest |delta| of
 // each sweep. Each link update is that of a_rows.
 template <typename R, bool GAUGE, bool SH = false>
-__device__ __forceinline__ void a_rows2(const ASweepArgs<R> &A, int s, int ja, int jb, R &r1, R &r2) {
+__device__ __forceinline__ void a_rows2(const ASweepArgs<R> &A0, int s, int ja, int jb, R &r1, R &r2) {
+    ASweepArgs<R> tmp;
+    const ASweepArgs<R> &A = a_row_args<R, SH>(A0, tmp);
     constexpr int B = Pipe<R, SH>::AROWS2;
     const ColCtx X = col_ctx<2>(A.Nx, s);
     const int lane = threadIdx.x & (WAVE - 1);
@@ -886,6 +918,195 @@ __device__ __forceinline__ void a_rows2(const ASweepArgs<R> &A, int s, int ja, i
     };
 
     const int t1 = jb + 1;  // rows t in [t0, t1)
+    ARow<R> P0[B], P1[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) P0[k] = a_load_row(A, X, mtp, min(min(t0 + k, t1 - 1), Ny - 1));
+    for (int t = t0; t < t1; t += 2 * B) {
+#pragma unroll
+        for (int k = 0; k < B; ++k) P1[k] = a_load_row(A, X, mtp, min(min(t + B + k, t1 - 1), Ny - 1));
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            if (t + k < t1) row(P0[k], t + k);
+#pragma unroll
+        for (int k = 0; k < B; ++k) P0[k] = a_load_row(A, X, mtp, min(min(t + 2 * B + k, t1 - 1), Ny - 1));
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            if (t + B + k < t1) row(P1[k], t + B + k);
+    }
+}
+
+// Three consecutive A sweeps j, j+1, j+2 (j even) in one pass over rows
+// [ja, jb) of strip s, strips of WAVE - 6 columns. Sweeps j and j+1 take
+// their supercurrent from the pass's input A^j, sweep j+2 from A^{j+2}, the
+// output of sweep j+1, which the pass holds in registers (the source
+// schedule A^(2*floor(j/2))). Sweep j runs at row t on lanes 1..62, sweep j+1
+// at row t-1 on lanes 2..61, sweep j+2 at row t-2 on lanes 3..60. Writes
+// A^{j+3} to ab_next and A^{j+2} to ab_mid (the source of sweep j+3 should
+// the solve continue); r1, r2, r3 get the lane's largest |delta| of each
+// sweep. Each link update is that of a_rows.
+template <typename R, bool GAUGE, bool SH = false>
+__device__ __forceinline__ void a_rows3(const ASweepArgs<R> &A0, int s, int ja, int jb, R &r1, R &r2, R &r3) {
+    ASweepArgs<R> tmp;
+    const ASweepArgs<R> &A = a_row_args<R, SH>(A0, tmp);
+    constexpr int B = Pipe<R, SH>::AROWS3;
+    const ColCtx X = col_ctx<3>(A.Nx, s);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int c = X.c;
+    const bool own = X.own, hasa = X.hasa;
+    const bool mid1 = X.col && lane >= 1 && lane < WAVE - 1;
+    const bool mid2 = X.col && lane >= 2 && lane < WAVE - 2;
+    const int Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1;
+    const long Na = (long)Nxa * Ny;
+    const R *a = A.ab;
+    const R *b = A.ab + Na;
+    const bool noisy = A.rhs_out != nullptr;
+    const bool noise_on = noisy && A.jstep == 0 && A.langevin_c > R(1.0e-32);
+    const bool has_mt = A.mt != nullptr;
+    const uint8_t *mtp = has_mt ? A.mt : reinterpret_cast<const uint8_t *>(A.ab);
+    const R gb = GAUGE ? gauge_phase_b<R>(A, c) : R(0);
+    const cplx<R> zero = {R(0), R(0)};
+    const uint32_t oc = X.cc * (uint32_t)sizeof(R);
+    const int t0 = ja > 1 ? ja - 2 : 0;
+
+    // sweep j window (row t): a(c, t-1), a(c, t), b(c, t-1), psi(c, t), cell (c, t-1)
+    R am = R(0), ac = R(0), bm = R(0);
+    cplx<R> pc = zero;
+    int cellm = 0;
+    if (X.col) {
+        pc = ld(A.psi, c + (long)Nx * t0);
+        if (hasa) ac = a[c + (long)Nxa * t0];
+        if (t0 > 0) {
+            if (hasa) am = a[c + (long)Nxa * (t0 - 1)];
+            bm = b[c + (long)Nx * (t0 - 1)];
+            if (hasa) cellm = has_mt ? int(A.mt[c + (long)Nxa * (t0 - 1)]) : 1;
+        }
+    }
+    // sweep j+1 window (row u = t - 1): A^{j+1} at a(c, u-1), a(c, u),
+    // b(c, u-1), b(c, u); supercurrents and right-hand sides of row u
+    R am1 = R(0), ac1 = R(0), bm1 = R(0), bc1 = R(0), jla1 = R(0), jlb1 = R(0), ra1 = R(0), rb1 = R(0);
+    // sweep j+2 window (row v = t - 2): A^{j+2} at a(c, v-1), a(c, v),
+    // b(c, v-1), b(c, v); psi(c, v), psi(c, v+1); right-hand sides and
+    // supercurrent switches of row v
+    R am2 = R(0), ac2 = R(0), bm2 = R(0), bc2 = R(0), ra2 = R(0), rb2 = R(0);
+    cplx<R> pv = zero, pu = zero;
+    bool ena1 = false, enb1 = false, ena2 = false, enb2 = false;
+
+    auto row = [&](const ARow<R> &cu, int t) {
+        const bool real = t < Ny;
+        const R ap = (hasa && t + 1 < Ny) ? cu.ap : R(0);
+        const R bc = (X.col && t < Ny - 1) ? cu.bc : R(0);
+        const cplx<R> pp = (X.col && t < Ny - 1) ? cu.pp : zero;
+        const int cellv = (hasa && t < Ny - 1) ? (has_mt ? cu.cell : 1) : 0;
+        const bool cellc = cellv != 0;
+        const R brc = shfl_dn1(bc), brm = shfl_dn1(bm);
+        const R pr_re = shfl_dn1(pc.re), pr_im = shfl_dn1(pc.im);
+        const R alc = shfl_up1(ac), alp = shfl_up1(ap), blc = shfl_up1(bc);
+        const bool cl_c = shfl_up1(cellv) != 0;  // cell (c-1, t)
+        const long na = c + (long)Nxa * t;
+        const long nb = Na + c + (long)Nx * t;
+        const bool hasa_t = hasa && real;
+        const bool hasb_t = X.col && t < Ny - 1;
+        const bool ena = cellm != 0 || cellc;  // a(c, t) touches the material
+        const bool enb = cl_c || cellc;        // b(c, t) touches the material
+        R ra = cu.ra, rb = cu.rb;
+        if (noise_on) {
+            if (mid1 && hasa_t) ra = ra + A.langevin_c * R(double(noise_1<R>((uint32_t)na, A.rand_t)) - 0.5);
+            if (mid1 && hasb_t) rb = rb + A.langevin_c * R(double(noise_2<R>((uint32_t)nb, A.rand_t)) - 0.5);
+        }
+        R ssa, csa, ssb, csb;
+        sincos_r(link_phase<R, GAUGE>(A.dx, cu.sa, GAUGE ? gauge_phase_a<R>(A, t) : R(0)), &ssa, &csa);
+        sincos_r(link_phase<R, GAUGE>(A.dy, cu.sb, gb), &ssb, &csb);
+        const R jla = ena ? link_current(A.idx, pc, pr_re, pr_im, csa, ssa) : R(0);
+        const R jlb = enb ? link_current(A.idy, pc, pp.re, pp.im, csb, ssb) : R(0);
+
+        // sweep j at row t -> A^{j+1}(t)
+        R va = R(0), vb = R(0);
+        if (mid1 && hasa_t) va = a_link_update(A, t, ra, jla, am, bm, brm, ap, bc, brc);
+        if (mid1 && hasb_t) vb = b_link_update(A, c, rb, jlb, blc, alc, alp, brc, ac, ap);
+        if (own && t >= ja && t < jb) {
+            if (hasa_t) {
+                if (noisy && A.jstep == 0) A.rhs_out[na] = ra;
+                r1 = fmax(r1, fabs(va - ac));
+            }
+            if (hasb_t) {
+                if (noisy && A.jstep == 0) A.rhs_out[nb] = rb;
+                r1 = fmax(r1, fabs(vb - bc));
+            }
+        }
+        // sweep j+1 at row u = t - 1 -> A^{j+2}(u)
+        const int u = t - 1;
+        R wa = R(0), wb = R(0);
+        {
+            const R brc1 = shfl_dn1(bc1), brm1 = shfl_dn1(bm1);
+            const R alc1 = shfl_up1(ac1), alp1 = shfl_up1(va), blc1 = shfl_up1(bc1);
+            if (mid2 && u >= 0 && u < Ny) {
+                if (hasa) wa = a_link_update(A, u, ra1, jla1, am1, bm1, brm1, va, bc1, brc1);
+                if (u < Ny - 1) wb = b_link_update(A, c, rb1, jlb1, blc1, alc1, alp1, brc1, ac1, va);
+            }
+            if (own && u >= ja && u < jb) {
+                if (hasa) {
+                    st_at(A.ab_mid + (size_t)Nxa * u, oc, wa);
+                    r2 = fmax(r2, fabs(wa - ac1));
+                }
+                if (u < Ny - 1) {
+                    st_at(A.ab_mid + (size_t)Na + (size_t)Nx * u, oc, wb);
+                    r2 = fmax(r2, fabs(wb - bc1));
+                }
+            }
+        }
+        // sweep j+2 at row v = t - 2 -> A^{j+3}(v), source A^{j+2}
+        const int v = t - 2;
+        {
+            const R brc2 = shfl_dn1(bc2), brm2 = shfl_dn1(bm2);
+            const R alc2 = shfl_up1(ac2), alp2 = shfl_up1(wa), blc2 = shfl_up1(bc2);
+            const R pvr_re = shfl_dn1(pv.re), pvr_im = shfl_dn1(pv.im);
+            if (v >= ja && own) {
+                R s2a, c2a, s2b, c2b;
+                sincos_r(link_phase<R, GAUGE>(A.dx, ac2, GAUGE ? gauge_phase_a<R>(A, v) : R(0)), &s2a, &c2a);
+                sincos_r(link_phase<R, GAUGE>(A.dy, bc2, gb), &s2b, &c2b);
+                if (hasa) {
+                    const R jl = ena2 ? link_current(A.idx, pv, pvr_re, pvr_im, c2a, s2a) : R(0);
+                    const R x = a_link_update(A, v, ra2, jl, am2, bm2, brm2, wa, bc2, brc2);
+                    st_at(A.ab_next + (size_t)Nxa * v, oc, x);
+                    r3 = fmax(r3, fabs(x - ac2));
+                }
+                if (v < Ny - 1) {
+                    const R jl = enb2 ? link_current(A.idy, pv, pu.re, pu.im, c2b, s2b) : R(0);
+                    const R x = b_link_update(A, c, rb2, jl, blc2, alc2, alp2, brc2, ac2, wa);
+                    st_at(A.ab_next + (size_t)Na + (size_t)Nx * v, oc, x);
+                    r3 = fmax(r3, fabs(x - bc2));
+                }
+            }
+        }
+        // roll the three windows
+        am2 = ac2;
+        ac2 = wa;
+        bm2 = bc2;
+        bc2 = wb;
+        ra2 = ra1;
+        rb2 = rb1;
+        ena2 = ena1;
+        enb2 = enb1;
+        pv = pu;
+        pu = pc;
+        am1 = ac1;
+        ac1 = va;
+        bm1 = bc1;
+        bc1 = vb;
+        jla1 = jla;
+        jlb1 = jlb;
+        ra1 = ra;
+        rb1 = rb;
+        ena1 = ena;
+        enb1 = enb;
+        am = ac;
+        ac = ap;
+        bm = bc;
+        pc = pp;
+        cellm = cellv;
+    };
+
+    const int t1 = jb + 2;  // rows t in [t0, t1)
     ARow<R> P0[B], P1[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) P0[k] = a_load_row(A, X, mtp, min(min(t0 + k, t1 - 1), Ny - 1));
@@ -1025,11 +1246,12 @@ __device__ __forceinline__ unsigned long long *state64(int *state, int k) {
 __device__ __forceinline__ int res_field(int r) { return min(r, 16383) >> 4; }
 __device__ __forceinline__ bool res_conv(int packed, int k) { return ((packed >> (10 * k)) & 0x3ff) < 625; }
 
+template <int K = 3>
 __device__ __forceinline__ bool sweep_sync(const GridSync &gs, const Team &tm, int r1, int r2, int r3,
                                            unsigned long long &epoch, int *lds, int &r_out) {
-    r1 = block_max_int(res_field(r1), lds);
-    r2 = block_max_int(res_field(r2), lds);
-    r3 = block_max_int(res_field(r3), lds);
+    r1 = K >= 1 ? block_max_int(res_field(r1), lds) : 0;  // K: fields carried (passes of at most K sweeps)
+    r2 = K >= 2 ? block_max_int(res_field(r2), lds) : 0;
+    r3 = K >= 3 ? block_max_int(res_field(r3), lds) : 0;
     return team_sync_max(gs, tm, epoch, (r3 << 20) | (r2 << 10) | r1, r_out);
 }
 
@@ -1081,17 +1303,18 @@ __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const Gri
         R r1 = R(0), r2 = R(0), r3 = R(0);
         pass(k, r1, r2, r3);
         int r = 0;
-        if (!sweep_sync(gs, tm, scaled_residual<R>(r1, A.stop_eps), scaled_residual<R>(r2, A.stop_eps),
-                        scaled_residual<R>(r3, A.stop_eps), epoch, lds, r))
+        if (!sweep_sync<KMAX>(gs, tm, scaled_residual<R>(r1, A.stop_eps), scaled_residual<R>(r2, A.stop_eps),
+                              scaled_residual<R>(r3, A.stop_eps), epoch, lds, r))
             return {in, nsw, false};
         // the first sweep of the pass that met the stop rule, or k
         const int m = (k > 1 && res_conv(r, 0)) ? 1 : ((k > 2 && res_conv(r, 1)) ? 2 : k);
         if (m < k) {
-            // redo the pass up to sweep m into the same output
+            // redo the pass up to sweep m into the same output (m is 1 unless
+            // the pass had three sweeps)
             R d1 = R(0), d2 = R(0), d3 = R(0);
-            pass(m, d1, d2, d3);
+            pass(KMAX == 3 && m == 2 ? 2 : 1, d1, d2, d3);
             int rdummy = 0;
-            if (!sweep_sync(gs, tm, 0, 0, 0, epoch, lds, rdummy)) return {in, nsw, false};
+            if (!sweep_sync<0>(gs, tm, 0, 0, 0, epoch, lds, rdummy)) return {in, nsw, false};
             done = true;
         } else {
             done = res_conv(r, k - 1);
@@ -1107,50 +1330,66 @@ __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const Gri
 // The A solve of one TD step on the team's waves, with the reference's
 // source schedule A^(2*floor(j/2)) (td.py:264-303). Sweeps run in fused
 // pairs (a_rows2) starting at even j, where both sweeps take their source
-// from the pair's input; as in psi_solve, a pair whose first sweep met the
-// stop rule is redone as that single sweep. Single sweeps (a_rows) are used
-// at odd j, at the last sweep allowed, and at j = pred - 1 when the previous
-// step needed pred sweeps with pred odd.
+// from the pair's input. When the previous step needed an odd number pred >= 3
+// of sweeps, the solve opens with a three-sweep pass (a_rows3, which also
+// stores A^2, the source of sweep 3). Single sweeps (a_rows) are used at odd
+// j, at the last sweep allowed, and at j = pred - 1. As in psi_solve, a pass
+// whose earlier sweep met the stop rule is redone up to that sweep.
 template <typename R, bool GAUGE, bool SH = false>
 __device__ __forceinline__ SolveOut a_solve(const TDSolveArgs<R> &T, const GridSync &gs, const Team &tm, long w,
                                             long W, int pc, int qc, int pred, unsigned long long &epoch, int *lds) {
+    constexpr bool TRIPLE = !SH;
     const int o1 = (qc + 1) % 3, o2 = (qc + 2) % 3;
     ASweepArgs<R> A = T.A;
     A.psi = pick3(T.P, pc);
     A.rhs_out = T.rhs_ab;
     int in = qc, out = o1, j = 0, nsw = 0;
     bool done = false;
+    auto pass = [&](int k, R &r1, R &r2, R &r3) {
+        if constexpr (TRIPLE) {
+            if (k == 3) {
+                for_rows_rm<3>(A.Nx, A.Ny, w, W,
+                               [&](int s, int ja, int jb) { a_rows3<R, GAUGE, SH>(A, s, ja, jb, r1, r2, r3); });
+                return;
+            }
+        }
+        if (k == 2)
+            for_rows_rm<2>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { a_rows2<R, GAUGE, SH>(A, s, ja, jb, r1, r2); });
+        else
+            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { r1 = fmax(r1, a_rows<R, GAUGE, SH>(A, s, ja, jb)); });
+    };
     while (!done && j < T.max_sweeps) {
         A.jstep = (uint32_t)j;
         A.ab = pick3(T.Q, in);
         A.ab_next = pick3(T.Q, out);
+        if constexpr (TRIPLE) A.ab_mid = pick3(T.Q, out == o1 ? o2 : o1);
         // source A^(2*floor(j/2)): the input on even sweeps, A^n on sweep 1,
-        // the output buffer's previous contents on odd j >= 3 (sweep j - 1
-        // was then a single sweep that read it)
+        // the output buffer's previous contents on odd j >= 3 (A^(j-1), left
+        // there by the single sweep j - 1 or by the three-sweep pass)
         A.src = pick3(T.Q, (j % 2 == 0) ? in : (j == 1 ? qc : out));
         A.rhs = (T.rhs_ab && j > 0) ? T.rhs_ab : pick3(T.Q, qc);
-        const bool pair = (j % 2 == 0) && j + 1 != pred && j + 2 <= T.max_sweeps;
-        R r1 = R(0), r2 = R(0);
-        if (pair)
-            for_rows_rm<2>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { a_rows2<R, GAUGE, SH>(A, s, ja, jb, r1, r2); });
-        else
-            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { r1 = fmax(r1, a_rows<R, GAUGE, SH>(A, s, ja, jb)); });
+        int k = 2;
+        if (j % 2 == 1) k = 1;
+        else if (TRIPLE && j == 0 && (pred & 1) && pred >= 3 && T.max_sweeps >= 3) k = 3;
+        else if (j + 1 == pred || j + 2 > T.max_sweeps) k = 1;
+        R r1 = R(0), r2 = R(0), r3 = R(0);
+        pass(k, r1, r2, r3);
         int r = 0;
-        if (!sweep_sync(gs, tm, scaled_residual<R>(r1, A.stop_eps), scaled_residual<R>(r2, A.stop_eps), 0, epoch, lds, r))
+        if (!sweep_sync<TRIPLE ? 3 : 2>(gs, tm, scaled_residual<R>(r1, A.stop_eps), scaled_residual<R>(r2, A.stop_eps),
+                                        scaled_residual<R>(r3, A.stop_eps), epoch, lds, r))
             return {in, nsw, false};
-        const bool c1 = res_conv(r, 0), c2 = res_conv(r, 1);
-        if (pair && c1) {
-            R rr = R(0);
-            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { rr = fmax(rr, a_rows<R, GAUGE, SH>(A, s, ja, jb)); });
+        const int m = (k > 1 && res_conv(r, 0)) ? 1 : ((k > 2 && res_conv(r, 1)) ? 2 : k);
+        if (m < k) {
+            R d1 = R(0), d2 = R(0), d3 = R(0);
+            pass(TRIPLE && m == 2 ? 2 : 1, d1, d2, d3);  // m is 1 unless the pass had three sweeps
             int rdummy = 0;
-            if (!sweep_sync(gs, tm, 0, 0, 0, epoch, lds, rdummy)) return {in, nsw, false};
-            nsw = j + 1;
+            if (!sweep_sync<0>(gs, tm, 0, 0, 0, epoch, lds, rdummy)) return {in, nsw, false};
             done = true;
         } else {
-            j += pair ? 2 : 1;
-            nsw = j;
-            done = pair ? c2 : c1;
+            done = res_conv(r, k - 1);
         }
+        j += m;
+        nsw = j;
         in = out;
         out = (out == o1) ? o2 : o1;
     }
