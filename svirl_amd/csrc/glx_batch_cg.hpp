@@ -140,7 +140,8 @@ __global__ void __launch_bounds__(256) k_batch_cg(const CGInst<R> *insts, double
                     bool ha, hb;
                     jac_A_node(g, q, I.kappa2, I.H, I.mt, I.psi, (const R *)nullptr, I.ab, m, ga, gb, ha, hb,
                                R(0.5));
-                    const int i = (int)(m % g.Nx), j = (int)(m / g.Nx);
+                    int i, j;
+                    node_ij(g, m, i, j);
                     if (ha) {
                         const long la = i + (long)g.Nxa * j;
                         const R v = s2 * ga, vp = jprevA[la];

@@ -19,6 +19,7 @@ struct Geo {
     int Nx, Ny, Nxa, Nya, Nxb, Nyb, Nxc, Nyc;
     long N, Na, Nab, Nc;
     double dx, dy, idx, idy, idx2, idy2, idxy;
+    double inx;  // 1 / Nx (node_ij)
 };
 
 inline Geo make_geo(int Nx, int Ny, double dx, double dy) {
@@ -30,7 +31,26 @@ inline Geo make_geo(int Nx, int Ny, double dx, double dy) {
     g.dx = dx; g.dy = dy;
     g.idx = 1.0 / dx; g.idy = 1.0 / dy;
     g.idx2 = 1.0 / (dx * dx); g.idy2 = 1.0 / (dy * dy); g.idxy = 1.0 / (dx * dy);
+    g.inx = 1.0 / Nx;
     return g;
+}
+
+// (i, j) of node n = i + Nx j without an integer division: for n < 2^30 the
+// double product n * (1/Nx) is within 2^-22 of the quotient, so its
+// truncation is off by at most one, which one comparison corrects.
+__device__ __forceinline__ void node_ij(const Geo &g, long n, int &i, int &j) {
+    const int nn = (int)n;
+    int jj = (int)((double)nn * g.inx);
+    int ii = nn - jj * g.Nx;
+    if (ii < 0) {
+        ii += g.Nx;
+        --jj;
+    } else if (ii >= g.Nx) {
+        ii -= g.Nx;
+        ++jj;
+    }
+    i = ii;
+    j = jj;
 }
 
 // Geometry constants converted to the field precision.
@@ -158,7 +178,8 @@ template <typename R>
 __global__ void k_current_density(Geo g, R kappa2, R H, const R *abei, const R *ab, R *J) {
     const GeoR<R> q(g);
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
+        int i, j;
+        node_ij(g, n, i, j);
         if (i < g.Nxa && j < g.Nya) J[i + (long)g.Nxa * j] = kappa2 * curlcurl_a(g, q, H, abei, ab, i, j);
         if (i < g.Nxb && j < g.Nyb) J[g.Na + i + (long)g.Nxb * j] = kappa2 * curlcurl_b(g, q, H, abei, ab, i, j);
     }
@@ -168,7 +189,8 @@ template <typename R>
 __global__ void k_supercurrent_density(Geo g, const uint8_t *mt, const cplx<R> *psi, const R *abei, const R *ab, R *Js) {
     const GeoR<R> q(g);
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
+        int i, j;
+        node_ij(g, n, i, j);
         const cplx<R> p0 = ld(psi, n);
         const Cells c = cells_of(g, mt, i, j);
         if (i < g.Nxa && j < g.Nya) {
@@ -293,7 +315,8 @@ __global__ void k_axpy(const R *x, const R *y, R *z, R alpha, long n, int minus)
 template <typename R>
 __device__ __forceinline__ R fe_node(const Geo &g, const GeoR<R> &q, R kappa2, R eps, const R *eps_map, R H,
                                      const uint8_t *mt, const cplx<R> *psi, const R *abei, const R *ab, long n) {
-    const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
+    int i, j;
+    node_ij(g, n, i, j);
     R e = R(0);
     const Cells c = cells_of(g, mt, i, j);
     if (c.mm || c.mp || c.pm || c.pp) {
@@ -345,7 +368,8 @@ template <typename R>
 __device__ __forceinline__ cplx<R> jac_psi_node(const Geo &g, const GeoR<R> &q, R eps, const R *eps_map,
                                                 const uint8_t *mt, const cplx<R> *psi, const R *abei, const R *ab,
                                                 long n) {
-    const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
+    int i, j;
+    node_ij(g, n, i, j);
     cplx<R> gj = {R(0), R(0)};
     const Cells c = cells_of(g, mt, i, j);
     if (c.mm || c.mp || c.pm || c.pp) {
@@ -399,7 +423,8 @@ template <typename R>
 __device__ __forceinline__ void jac_A_node(const Geo &g, const GeoR<R> &q, R kappa2, R H, const uint8_t *mt,
                                            const cplx<R> *psi, const R *abei, const R *ab, long n, R &ga, R &gb,
                                            bool &has_a, bool &has_b, R bscale = R(1)) {
-    const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
+    int i, j;
+    node_ij(g, n, i, j);
     const cplx<R> p0 = ld(psi, n);
     const Cells c = cells_of(g, mt, i, j);
     has_a = i < g.Nxa && j < g.Nya;
@@ -438,9 +463,119 @@ __global__ void k_jacobian_A(Geo g, R kappa2, R H, const uint8_t *mt, const cplx
         R ga, gb;
         bool has_a, has_b;
         jac_A_node(g, q, kappa2, H, mt, psi, abei, ab, n, ga, gb, has_a, has_b);
-        const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
+        int i, j;
+        node_ij(g, n, i, j);
         if (has_a) G[i + (long)g.Nxa * j] += s2 * ga;
         if (has_b) G[g.Na + i + (long)g.Nxb * j] += s2 * gb;
+    }
+}
+
+// The free-energy density, dF/dpsi and dF/dA of node n in one evaluation:
+// fe_node, jac_psi_node and jac_A_node fused. The cells, psi and its
+// neighbours and the sin/cos of the four link phases are loaded and
+// evaluated once (the right and up links serve all three); every quantity
+// keeps the operation order of the separate functions, so the results are
+// theirs bit for bit.
+template <typename R, bool withA>
+__device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kappa2, R eps, const R *eps_map, R H,
+                                          const uint8_t *mt, const cplx<R> *psi, const R *abei, const R *ab, long n,
+                                          int i, int j, R &e, cplx<R> &gj, R &ga, R &gb, bool &has_a, bool &has_b) {
+    const Cells c = cells_of(g, mt, i, j);
+    const cplx<R> p0 = ld(psi, n);
+    const cplx<R> zero = {R(0), R(0)};
+    const bool lr = c.pm || c.pp, lu = c.mp || c.pp;  // the right link a(i, j) and the up link b(i, j) are active
+    R sr = R(0), cr = R(1), su = R(0), cu = R(1);
+    cplx<R> pr = zero, pu = zero;
+    if (lr) {
+        pr = ld(psi, n + 1);
+        sincos_r(q.dx * pv(abei, ab, i + (long)g.Nxa * j), &sr, &cr);
+    }
+    if (lu) {
+        pu = ld(psi, n + g.Nx);
+        sincos_r(q.dy * pv(abei, ab, g.Na + i + (long)g.Nxb * j), &su, &cu);
+    }
+    // |psi1 U - psi0|^2 and Im(conj psi0 U psi1) with U's (sin, cos) given
+    auto gg = [&](R s, R cs, cplx<R> p1) {
+        const R re = p1.re * cs + p1.im * s - p0.re;
+        const R im = p1.im * cs - p1.re * s - p0.im;
+        return re * re + im * im;
+    };
+    auto jsu = [&](R s, R cs, cplx<R> p1) {
+        return (p0.re * p1.im - p0.im * p1.re) * cs - (p0.re * p1.re + p0.im * p1.im) * s;
+    };
+    e = R(0);
+    gj = zero;
+    if (c.mm || c.mp || c.pm || c.pp) {
+        const R f_mm = c.mm ? R(1) : R(0), f_mp = c.mp ? R(1) : R(0), f_pm = c.pm ? R(1) : R(0), f_pp = c.pp ? R(1) : R(0);
+        const R r_m0 = R(0.5) * (f_mm + f_mp), r_p0 = R(0.5) * (f_pm + f_pp);
+        const R r_0m = R(0.5) * (f_mm + f_pm), r_0p = R(0.5) * (f_mp + f_pp);
+        const R gw = R(0.25) * (r_m0 + r_p0 + r_0m + r_0p);
+        const R ee = eps_map ? eps_map[n] : eps;
+        // fe_node
+        const R p2 = p0.re * p0.re + p0.im * p0.im;
+        e += gw * (R(0.5) * p2 - ee) * p2;
+        if (lr) e += r_p0 * q.idx2 * gg(sr, cr, pr);
+        if (lu) e += r_0p * q.idy2 * gg(su, cu, pu);
+        // jac_psi_node
+        const R p = p0.re * p0.re + p0.im * p0.im - ee;
+        const R k = R(2) * gw * p;
+        gj.re += k * p0.re;
+        gj.im += k * p0.im;
+        auto add = [&](R r, R id2, R s, R cs, cplx<R> p1) {
+            const R tr = R(2) * (p0.re - (p1.re * cs + p1.im * s));
+            const R ti = R(2) * (p0.im - (p1.im * cs - p1.re * s));
+            const R w = r * id2;
+            gj.re += w * tr;
+            gj.im += w * ti;
+        };
+        if (c.mm || c.mp) {
+            R s, cs;
+            sincos_r(-q.dx * pv(abei, ab, (i - 1) + (long)g.Nxa * j), &s, &cs);
+            add(r_m0, q.idx2, s, cs, ld(psi, n - 1));
+        }
+        if (lr) add(r_p0, q.idx2, sr, cr, pr);
+        if (c.mm || c.pm) {
+            R s, cs;
+            sincos_r(-q.dy * pv(abei, ab, g.Na + i + (long)g.Nxb * (j - 1)), &s, &cs);
+            add(r_0m, q.idy2, s, cs, ld(psi, n - g.Nx));
+        }
+        if (lu) add(r_0p, q.idy2, su, cu, pu);
+    }
+    // fe_node: the cell's magnetic energy
+    if (kappa2 > R(0) && i < g.Nxc && j < g.Nyc) {
+        R dB = -H;
+        for (int sv = 0; sv < 2; ++sv) {
+            const R *v = sv == 0 ? abei : ab;
+            if (!v) continue;
+            dB += q.idx * (v[g.Na + i + 1 + (long)g.Nxb * j] - v[g.Na + i + (long)g.Nxb * j]) -
+                  q.idy * (v[i + (long)g.Nxa * (j + 1)] - v[i + (long)g.Nxa * j]);
+        }
+        e += kappa2 * dB * dB;
+    }
+    // jac_A_node (bscale 1)
+    has_a = has_b = false;
+    ga = gb = R(0);
+    if (withA) {
+        has_a = i < g.Nxa && j < g.Nya;
+        has_b = i < g.Nxb && j < g.Nyb;
+        if (has_a) {
+            R t = curlcurl_a(g, q, H, abei, ab, i, j);
+            t *= kappa2;
+            if (lr) {
+                const R r = R(0.5) * ((c.pm ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
+                t += -r * q.idx * jsu(sr, cr, pr);
+            }
+            ga = t;
+        }
+        if (has_b) {
+            R t = curlcurl_b(g, q, H, abei, ab, i, j);
+            t *= kappa2;
+            if (lu) {
+                const R r = R(0.5) * ((c.mp ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
+                t += -r * q.idy * jsu(su, cu, pu);
+            }
+            gb = t;
+        }
     }
 }
 
@@ -457,33 +592,35 @@ __global__ void k_cg_gradient_pass(Geo g, R kappa2, R eps, const R *eps_map, R H
     const R dxdy = q.dx * q.dy, s2 = R(2) * q.dx * q.dy;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
-        acc[0] += double(fe_node(g, q, kappa2, eps, eps_map, H, mt, psi, abei, ab, n));
-        const cplx<R> gj = jac_psi_node(g, q, eps, eps_map, mt, psi, abei, ab, n);
-        const cplx<R> gv = {dxdy * gj.re, dxdy * gj.im};
+        int i, j;
+        node_ij(g, n, i, j);
+        R e, ga, gb;
+        cplx<R> gj;
+        bool ha, hb;
+        grad_node<R, withA>(g, q, kappa2, eps, eps_map, H, mt, psi, abei, ab, n, i, j, e, gj, ga, gb, ha, hb);
+        // every load of the node before its stores
         const cplx<R> gp = ld(jac_prev, n);
+        const long la = i + (long)g.Nxa * j, lb = g.Na + i + (long)g.Nxb * j;
+        const R vpa = (withA && ha) ? jacA_prev[la] : R(0), vpb = (withA && hb) ? jacA_prev[lb] : R(0);
+        acc[0] += double(e);
+        const cplx<R> gv = {dxdy * gj.re, dxdy * gj.im};
         st(jac, n, gv);
         acc[1] += double(gv.re * (gv.re - gp.re));
         acc[1] += double(gv.im * (gv.im - gp.im));
         acc[2] += double(gp.re * gp.re);
         acc[2] += double(gp.im * gp.im);
         if (withA) {
-            R ga, gb;
-            bool ha, hb;
-            jac_A_node(g, q, kappa2, H, mt, psi, abei, ab, n, ga, gb, ha, hb);
-            const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
             if (ha) {
-                const long la = i + (long)g.Nxa * j;
-                const R v = s2 * ga, vp = jacA_prev[la];
+                const R v = s2 * ga;
                 jacA[la] = v;
-                acc[3] += double(v * (v - vp));
-                acc[4] += double(vp * vp);
+                acc[3] += double(v * (v - vpa));
+                acc[4] += double(vpa * vpa);
             }
             if (hb) {
-                const long lb = g.Na + i + (long)g.Nxb * j;
-                const R v = s2 * gb, vp = jacA_prev[lb];
+                const R v = s2 * gb;
                 jacA[lb] = v;
-                acc[3] += double(v * (v - vp));
-                acc[4] += double(vp * vp);
+                acc[3] += double(v * (v - vpb));
+                acc[4] += double(vpb * vpb);
             }
         }
     }
@@ -519,7 +656,8 @@ template <typename R, int L>
 __device__ __forceinline__ void coef_node(const Geo &g, const GeoR<R> &q, R kappa2, R eps_s, const R *eps_map, R H,
                                           const uint8_t *mt, const cplx<R> *psi, const cplx<R> *dpsi, const R *abei,
                                           const R *ab, const R *dab, long n, R (&C)[L]) {
-    const int i = (int)(n % g.Nx), j = (int)(n / g.Nx);
+    int i, j;
+    node_ij(g, n, i, j);
     const Cells c = cells_of(g, mt, i, j);
     if (c.mm || c.mp || c.pm || c.pp) {
         const R f_mm = c.mm ? R(1) : R(0), f_mp = c.mp ? R(1) : R(0), f_pm = c.pm ? R(1) : R(0), f_pp = c.pp ? R(1) : R(0);
