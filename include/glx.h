@@ -323,6 +323,17 @@ int glx_cg_pr_beta(int prec, const void *jac, const void *jac_prev, int64_t n, d
 /* z = alpha*x - y with alpha read from device memory (the reference's
  * axmy_c/axmy_r called with the device beta, cg.py:283-290). */
 int glx_axmy_dev(int prec, const void *x, const void *y, void *z, const double *alpha_dev, int64_t n);
+/* The line search and update of one CG iteration on the device, for the
+ * FP32-mixed form: the line-search coefficients (as glx_cg_coef, or
+ * glx_cg_coef_psi when dab is NULL; cg.py:184-224, 325-375), their
+ * minimum from (0, 0) found in FP64 on the device (the role of the
+ * reference's numpy polyroots / scipy BFGS, cg.py:227-235, 378-419; a
+ * damped Newton descent on the quartic), and psi += alpha_psi * dpsi,
+ * ab += alpha_A * dab (cg.py:296, 535-536). alpha_dev: two device doubles
+ * (alpha_psi, alpha_A) written; the coefficients are left in the library's
+ * result buffer. No host synchronisation. */
+int glx_cg_line_step(int prec, glx_grid g, double kappa2, double eps, double H, const uint8_t *mt, void *psi,
+                     const void *dpsi, const void *abei, void *ab, const void *dab, double *alpha_dev);
 
 #ifdef __cplusplus
 }

@@ -107,6 +107,8 @@ _PROTOS = {
     "glx_cg_gradient_pass": [c_int, Grid, c_double, c_double, c_void_p, c_double, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double)],
     "glx_axmy_dev": [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_i64],
+    "glx_cg_line_step": [c_int, Grid, c_double, c_double, c_double, c_void_p, c_void_p, c_void_p, c_void_p,
+                         c_void_p, c_void_p, c_void_p],
 }
 
 EXPORTS = tuple(_PROTOS) + ("glx_last_error",)

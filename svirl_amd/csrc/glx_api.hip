@@ -369,6 +369,11 @@ int glx_axmy_dev(int prec, const void *x, const void *y, void *z, const double *
     GLX_DISPATCH(prec, axmy_dev(g_ctx.env, x, y, z, alpha_dev, n));
 }
 
+int glx_cg_line_step(int prec, glx_grid g, double kappa2, double eps, double H, const uint8_t *mt, void *psi,
+                     const void *dpsi, const void *abei, void *ab, const void *dab, double *alpha_dev) {
+    GLX_DISPATCH(prec, cg_line_step(g_ctx.env, g, kappa2, eps, H, mt, psi, dpsi, abei, ab, dab, alpha_dev));
+}
+
 int glx_axpy(int prec, const void *x, const void *y, void *z, double alpha, int64_t n) {
     GLX_DISPATCH(prec, axpy(g_ctx.env, x, y, z, alpha, n, 0));
 }

@@ -34,64 +34,6 @@ template <typename R> struct CGInst {
 // initial gradient norm, last relative gradient norm, last alphas.
 enum { CG_IT = 0, CG_DONE = 1, CG_BPSI = 2, CG_BA = 3, CG_G0 = 4, CG_GREL = 5, CG_APSI = 6, CG_AA = 7 };
 
-// P(x, y) = sum c[i][j] x^i y^j over i, j < 5 and its derivatives.
-__device__ __forceinline__ void poly2(const double (&c)[5][5], double x, double y, double &p, double &px, double &py,
-                                      double &pxx, double &pxy, double &pyy) {
-    double xi[5] = {1.0, x, x * x, x * x * x, x * x * x * x};
-    double yj[5] = {1.0, y, y * y, y * y * y, y * y * y * y};
-    p = px = py = pxx = pxy = pyy = 0.0;
-    for (int i = 0; i < 5; ++i)
-        for (int j = 0; j < 5; ++j) {
-            const double cij = c[i][j];
-            if (cij == 0.0) continue;
-            p += cij * xi[i] * yj[j];
-            if (i >= 1) px += cij * i * xi[i - 1] * yj[j];
-            if (j >= 1) py += cij * j * xi[i] * yj[j - 1];
-            if (i >= 2) pxx += cij * i * (i - 1) * xi[i - 2] * yj[j];
-            if (i >= 1 && j >= 1) pxy += cij * i * j * xi[i - 1] * yj[j - 1];
-            if (j >= 2) pyy += cij * j * (j - 1) * xi[i] * yj[j - 2];
-        }
-}
-
-// A local minimiser of P reached from (0, 0) by damped Newton steps
-// (gradient steps where the Hessian is not positive definite) with
-// Armijo backtracking, in FP64. With one_d, y stays 0 (the psi-only
-// quartic). Plays the part of the reference's numpy root / scipy BFGS
-// line search (cg.py:227-235, 378-419) on the device.
-__device__ __forceinline__ void quartic2_min(const double (&c)[5][5], bool one_d, double &x, double &y) {
-    x = 0.0;
-    y = 0.0;
-    for (int it = 0; it < 80; ++it) {
-        double p, px, py, pxx, pxy, pyy;
-        poly2(c, x, y, p, px, py, pxx, pxy, pyy);
-        if (one_d) { py = 0.0; pxy = 0.0; pyy = 1.0; }
-        const double gn = fabs(px) + fabs(py);
-        if (!(gn > 0.0)) break;
-        double dx, dy;
-        const double det = pxx * pyy - pxy * pxy;
-        if (pxx > 0.0 && det > 0.0) {
-            dx = -(pyy * px - pxy * py) / det;
-            dy = -(pxx * py - pxy * px) / det;
-        } else {
-            dx = -px;
-            dy = -py;
-        }
-        const double slope = px * dx + py * dy;
-        if (!(slope < 0.0)) { dx = -px; dy = -py; }
-        const double sl = px * dx + py * dy;
-        double t = 1.0, pn = 0.0, a, b, c2, d, e2;
-        for (int k = 0; k < 60; ++k) {
-            poly2(c, x + t * dx, y + t * dy, pn, a, b, c2, d, e2);
-            if (pn <= p + 1.0e-4 * t * sl) break;
-            t *= 0.5;
-        }
-        const double sx = t * dx, sy = t * dy;
-        x += sx;
-        y += sy;
-        if (fabs(sx) + fabs(sy) <= 1.0e-15 * (fabs(x) + fabs(y)) || !(pn < p)) break;
-    }
-}
-
 // Sum of L doubles per thread over the workgroup, into out (thread 0).
 template <int L> __device__ __forceinline__ void block_sums(double (&acc)[L], double *lds, double *out) {
     for (int k = 0; k < L; ++k) {

@@ -648,6 +648,111 @@ __global__ void k_cg_gradient_finish(const double *partials, int nb, double dxdy
     }
 }
 
+// P(x, y) = sum c[i][j] x^i y^j over i, j < 5 and its derivatives.
+__device__ __forceinline__ void poly2(const double (&c)[5][5], double x, double y, double &p, double &px, double &py,
+                                      double &pxx, double &pxy, double &pyy) {
+    double xi[5] = {1.0, x, x * x, x * x * x, x * x * x * x};
+    double yj[5] = {1.0, y, y * y, y * y * y, y * y * y * y};
+    p = px = py = pxx = pxy = pyy = 0.0;
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j) {
+            const double cij = c[i][j];
+            if (cij == 0.0) continue;
+            p += cij * xi[i] * yj[j];
+            if (i >= 1) px += cij * i * xi[i - 1] * yj[j];
+            if (j >= 1) py += cij * j * xi[i] * yj[j - 1];
+            if (i >= 2) pxx += cij * i * (i - 1) * xi[i - 2] * yj[j];
+            if (i >= 1 && j >= 1) pxy += cij * i * j * xi[i - 1] * yj[j - 1];
+            if (j >= 2) pyy += cij * j * (j - 1) * xi[i] * yj[j - 2];
+        }
+}
+
+// A local minimiser of P reached from (0, 0) by damped Newton steps
+// (gradient steps where the Hessian is not positive definite) with
+// Armijo backtracking, in FP64. With one_d, y stays 0 (the psi-only
+// quartic). Plays the part of the reference's numpy root / scipy BFGS
+// line search (cg.py:227-235, 378-419) on the device.
+__device__ __forceinline__ void quartic2_min(const double (&c)[5][5], bool one_d, double &x, double &y) {
+    x = 0.0;
+    y = 0.0;
+    for (int it = 0; it < 80; ++it) {
+        double p, px, py, pxx, pxy, pyy;
+        poly2(c, x, y, p, px, py, pxx, pxy, pyy);
+        if (one_d) { py = 0.0; pxy = 0.0; pyy = 1.0; }
+        const double gn = fabs(px) + fabs(py);
+        if (!(gn > 0.0)) break;
+        double dx, dy;
+        const double det = pxx * pyy - pxy * pxy;
+        if (pxx > 0.0 && det > 0.0) {
+            dx = -(pyy * px - pxy * py) / det;
+            dy = -(pxx * py - pxy * px) / det;
+        } else {
+            dx = -px;
+            dy = -py;
+        }
+        const double slope = px * dx + py * dy;
+        if (!(slope < 0.0)) { dx = -px; dy = -py; }
+        const double sl = px * dx + py * dy;
+        double t = 1.0, pn = 0.0, a, b, c2, d, e2;
+        for (int k = 0; k < 60; ++k) {
+            poly2(c, x + t * dx, y + t * dy, pn, a, b, c2, d, e2);
+            if (pn <= p + 1.0e-4 * t * sl) break;
+            t *= 0.5;
+        }
+        const double sx = t * dx, sy = t * dy;
+        x += sx;
+        y += sy;
+        if (fabs(sx) + fabs(sy) <= 1.0e-15 * (fabs(x) + fabs(y)) || !(pn < p)) break;
+    }
+}
+
+// Device line search of the FP32-mixed CG (glx_cg_line_step): the L
+// coefficient sums of k_coef (partials), times scale, packed as the
+// reference packs them (cg.py:369-373, c[i][0] for the psi-only quartic),
+// minimised in FP64 from (0, 0) by quartic2_min; the alphas go to
+// alpha[0..1] and the coefficients to c[0..L-1]. One workgroup.
+template <int L>
+__global__ void k_line_search(const double *partials, int nb, double scale, double *c, double *alpha) {
+    __shared__ double lds[RED_BLOCK / WAVE];
+    __shared__ double red[L];
+    for (int k = 0; k < L; ++k) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partials[(long)b * L + k];
+        s = block_sum(s, lds);
+        if (threadIdx.x == 0) red[k] = s * scale;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double c5[5][5] = {};
+        if (L == 17) {
+            for (int k = 0; k < 5; ++k) {
+                c5[0][k] = red[k];
+                c5[1][k] = red[5 + k];
+                c5[2][k] = red[10 + k];
+            }
+            c5[3][0] = red[15 % L];
+            c5[4][0] = red[16 % L];
+        } else {
+            for (int k = 0; k < 5; ++k) c5[k][0] = red[k];
+        }
+        double x, y;
+        quartic2_min(c5, L != 17, x, y);
+        alpha[0] = x;
+        alpha[1] = y;
+        for (int k = 0; k < L; ++k) c[k] = red[k];
+    }
+}
+
+// z = alpha*x + y with alpha read from device memory and rounded to the
+// working precision (the reference's axpy with the host alpha, cg.py:296,
+// 535-536).
+template <typename R>
+__global__ void k_axpy_dev(const R *x, const R *y, R *z, const double *alpha, long n) {
+    const R a = R(*alpha);
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
+        z[k] = a * x[k] + y[k];
+}
+
 // Line-search coefficients. L = 5 (cg.h:314-474) or 17 (cg.h:477-731).
 // One node's contributions are added to C (before the dxdy factor). The
 // reference's kernels read only the scalar eps (eps_map is NULL on the

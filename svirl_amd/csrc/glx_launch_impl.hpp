@@ -817,6 +817,44 @@ int L<R>::axmy_dev(const Env &e, const void *x, const void *y, void *z, const do
 }
 
 template <typename R>
+int L<R>::cg_line_step(const Env &e, glx_grid g, double kappa2, double eps, double H, const uint8_t *mt, void *psi,
+                       const void *dpsi, const void *abei, void *ab, const void *dab, double *alpha_dev) {
+    const bool withA = dab != nullptr;
+    if (!grid_ok(g) || !psi || !dpsi || !alpha_dev) return set_error(GLX_E_ARG, "glx_cg_line_step: invalid arguments");
+    if (withA && !ab) return set_error(GLX_E_ARG, "glx_cg_line_step: a vector potential direction needs ab");
+    if (kappa2 > 0.0 && !(abei || ab)) return set_error(GLX_E_ARG, "glx_cg_line_step: finite kappa needs ab");
+    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    const int nb = grid_blocks(G.N, RED_BLOCK, RED_MAX_BLOCKS);
+    const double dxdy = double(R(g.dx)) * double(R(g.dy));
+    if (withA) {
+        hipLaunchKernelGGL((k_coef<R, 17>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, R(kappa2), R(eps), R(H), mt,
+                           (const cplx<R> *)psi, (const cplx<R> *)dpsi, (const R *)abei, (const R *)ab, (const R *)dab,
+                           e.partials);
+        GLX_LAUNCH_CHECK("k_coef17");
+        hipLaunchKernelGGL(k_line_search<17>, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, dxdy, e.result,
+                           alpha_dev);
+    } else {
+        hipLaunchKernelGGL((k_coef<R, 5>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, R(kappa2), R(eps), R(H), mt,
+                           (const cplx<R> *)psi, (const cplx<R> *)dpsi, (const R *)abei, (const R *)ab,
+                           (const R *)nullptr, e.partials);
+        GLX_LAUNCH_CHECK("k_coef5");
+        hipLaunchKernelGGL(k_line_search<5>, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, dxdy, e.result,
+                           alpha_dev);
+    }
+    GLX_LAUNCH_CHECK("k_line_search");
+    const long n2 = 2 * G.N;
+    hipLaunchKernelGGL(k_axpy_dev<R>, dim3(grid_blocks(n2)), dim3(256), 0, e.stream, (const R *)dpsi, (const R *)psi,
+                       (R *)psi, (const double *)alpha_dev, n2);
+    GLX_LAUNCH_CHECK("k_axpy_dev");
+    if (withA) {
+        hipLaunchKernelGGL(k_axpy_dev<R>, dim3(grid_blocks(G.Nab)), dim3(256), 0, e.stream, (const R *)dab,
+                           (const R *)ab, (R *)ab, (const double *)alpha_dev + 1, (long)G.Nab);
+        GLX_LAUNCH_CHECK("k_axpy_dev");
+    }
+    return GLX_OK;
+}
+
+template <typename R>
 int L<R>::axpy(const Env &e, const void *x, const void *y, void *z, double alpha, int64_t n, int minus) {
     if (!x || !y || !z || n < 0) return set_error(GLX_E_ARG, "glx_axpy: invalid arguments");
     if (n == 0) return GLX_OK;

@@ -4,10 +4,17 @@ Polak-Ribiere beta and an exact polynomial line search
 
 Per iteration: gradients (GPU), beta from two FP64 device reductions, the
 search direction update (GPU), the line-search polynomial coefficients
-(GPU, FP64 accumulation), the root / BFGS solve of the polynomial on the
-host exactly as the reference does it, the update (GPU) and the energy
-(GPU). Directions, previous gradients and betas persist across cg() calls
-as in the reference (cg.py:54-71).
+(GPU, FP64 accumulation), the minimum of the polynomial, the update (GPU)
+and the energy (GPU). Directions, previous gradients and betas persist
+across cg() calls as in the reference (cg.py:54-71).
+
+The line search follows `line_search`: 'host' (the FP64 default) reads the
+coefficients back and solves them on the host exactly as the reference does
+(numpy polyroots / scipy BFGS), which keeps the FP64 form's trajectory the
+reference's; 'device' (the FP32-mixed default) minimises the same FP64
+coefficients on the device and applies the update there (glx_cg_line_step),
+so an iteration needs no host round trip but the energy read for the stop
+test.
 """
 import ctypes
 
@@ -41,6 +48,8 @@ class CG:
             self.__gjac_A = DeviceArray.zeros(cfg.Nab, cfg.dtype)
             self.__gjac_A_prev = DeviceArray.zeros(cfg.Nab, cfg.dtype)
         self.__c = np.zeros((5, 5), dtype=cfg.dtype) if self.params.solveA else np.zeros(5, dtype=cfg.dtype)
+        self.__galpha = DeviceArray.zeros(2, np.float64)
+        self.line_search = 'device' if np.dtype(cfg.dtype) == np.float32 else 'host'
         self.cg_energies = []
 
     def _grid(self):
@@ -201,6 +210,11 @@ class CG:
                  self._beta_ptr(2), ctypes.byref(F))
         return F.value
 
+    def __device_line_step(self, withA):
+        a = self._args()
+        nat.call("glx_cg_line_step", self.par.prec, self._grid(), a['k2'], a['eps'], a['H'], a['mt'], a['psi'],
+                 self.__gdir_psi.ptr, a['abei'], a['ab'], self.__gdir_A.ptr if withA else None, self.__galpha.ptr)
+
     def __commit_beta(self):
         nat.call("glx_memcpy_d2d", self._beta_ptr(0), self._beta_ptr(2), 16)
 
@@ -214,7 +228,13 @@ class CG:
                 self.__commit_beta()
             self._axmy_beta(self.__gdir_psi, self.__gjac_psi, self.__gdir_psi, 0)
             psi = self.vars.order_parameter_h()
-            if withA:
+            if self.line_search == 'device':
+                if withA:
+                    self._axmy_beta(self.__gdir_A, self.__gjac_A, self.__gdir_A, 1)
+                self.__device_line_step(withA)
+                if withA:
+                    self.__gjac_A, self.__gjac_A_prev = self.__gjac_A_prev, self.__gjac_A
+            elif withA:
                 self._axmy_beta(self.__gdir_A, self.__gjac_A, self.__gdir_A, 1)
                 self._free_energy_conjgrad_coef(self.__gdir_psi, self.__gdir_A)
                 alpha_psi, alpha_A = self._cg_alpha_min()

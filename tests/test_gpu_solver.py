@@ -242,3 +242,27 @@ def test_snapshot_resume_is_bit_exact(tmp_path):
     assert np.array_equal(aa, ba) and np.array_equal(ab_, bb)
     assert a.solve._cg.cg_energies == b.solve._cg.cg_energies
     assert a.solve._td.sweeps[-4:] == b.solve._td.sweeps[-4:]
+
+
+@pytest.mark.parametrize("kappa", [5.0, np.inf])
+def test_cg_device_line_search(kappa):
+    """The device line search (glx_cg_line_step, the FP32-mixed default)
+    lands on the line minimum the reference's host solve finds (scipy BFGS /
+    numpy polyroots): FP64 runs with either agree to 1e-8 in every energy;
+    the FP32-mixed run ends within 1e-5 of the FP64 energy."""
+    def make(dt_):
+        gl = GLSolver(Nx=65, Ny=57, dx=0.5, dy=0.5, order_parameter='random', random_seed=7, gl_parameter=kappa,
+                      normal_conductivity=100.0, homogeneous_external_field=0.1, dtype=dt_, convergence_rtol=0.0)
+        gl.solve.td(dt=0.1, Nt=5)
+        gl.solve._init_cg()
+        return gl
+    a, b, c = make(np.float64), make(np.float64), make(np.float32)
+    b.solve._cg.line_search = 'device'
+    assert a.solve._cg.line_search == 'host' and c.solve._cg.line_search == 'device'
+    for gl in (a, b, c):
+        gl.solve.cg(n_iter=30)
+    ea, eb, ec = (np.array(gl.solve._cg.cg_energies, dtype=np.float64) for gl in (a, b, c))
+    assert len(ea) == len(eb) == len(ec) == 30
+    assert np.allclose(ea, eb, rtol=1e-8)
+    assert np.all(np.diff(eb) <= 1e-12 * np.abs(eb[1:]))
+    assert abs(ec[-1] - ea[-1]) <= 1e-5 * abs(ea[-1])
