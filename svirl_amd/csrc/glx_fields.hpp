@@ -109,7 +109,7 @@ template <typename R> __device__ __forceinline__ cplx<R> grad_c(cplx<R> p0, R ph
 
 // ---------------------------------------------------------------- B field
 template <typename R>
-__global__ void k_magnetic_field(Geo g, const R *abei, const R *ab, R *B) {
+__global__ void __launch_bounds__(256) k_magnetic_field(Geo g, const R *abei, const R *ab, R *B) {
     const GeoR<R> q(g);
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.Nc; n += (long)gridDim.x * blockDim.x) {
         const int i = (int)(n % g.Nxc), j = (int)(n / g.Nxc);
@@ -175,7 +175,7 @@ __device__ __forceinline__ R curlcurl_b(const Geo &g, const GeoR<R> &q, R H, con
 
 // ------------------------------------------------------ current densities
 template <typename R>
-__global__ void k_current_density(Geo g, R kappa2, R H, const R *abei, const R *ab, R *J) {
+__global__ void __launch_bounds__(256) k_current_density(Geo g, R kappa2, R H, const R *abei, const R *ab, R *J) {
     const GeoR<R> q(g);
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
         int i, j;
@@ -186,7 +186,7 @@ __global__ void k_current_density(Geo g, R kappa2, R H, const R *abei, const R *
 }
 
 template <typename R>
-__global__ void k_supercurrent_density(Geo g, const uint8_t *mt, const cplx<R> *psi, const R *abei, const R *ab, R *Js) {
+__global__ void __launch_bounds__(256) k_supercurrent_density(Geo g, const uint8_t *mt, const cplx<R> *psi, const R *abei, const R *ab, R *Js) {
     const GeoR<R> q(g);
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
         int i, j;
@@ -237,7 +237,7 @@ __device__ __forceinline__ void block_store_partials(double (&acc)[L], double *p
 }
 
 // out[k] = scale * sum_b partials[b*L + k], one workgroup, fixed order.
-static __global__ void k_finish_partials(const double *partials, int nb, int L, double scale, double *out) {
+static __global__ void __launch_bounds__(256) k_finish_partials(const double *partials, int nb, int L, double scale, double *out) {
     __shared__ double lds[RED_BLOCK / WAVE];
     for (int k = 0; k < L; ++k) {
         double s = 0.0;
@@ -249,14 +249,14 @@ static __global__ void k_finish_partials(const double *partials, int nb, int L, 
 }
 
 template <typename R>
-__global__ void k_sum(const R *x, long n, double *partials) {
+__global__ void __launch_bounds__(256) k_sum(const R *x, long n, double *partials) {
     double acc[1] = {0.0};
     for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) acc[0] += double(x[k]);
     block_store_partials<1>(acc, partials);
 }
 
 template <typename R>
-__global__ void k_sum_v(const R *x, long nv, int ne, double *partials) {
+__global__ void __launch_bounds__(256) k_sum_v(const R *x, long nv, int ne, double *partials) {
     // one component per launch index k; ne <= 32
     __shared__ double lds[RED_BLOCK / WAVE];
     for (int k = 0; k < ne; ++k) {
@@ -270,7 +270,7 @@ __global__ void k_sum_v(const R *x, long nv, int ne, double *partials) {
 }
 
 template <typename R>
-__global__ void k_pr_sums(const R *x, const R *y, long n, double *partials) {
+__global__ void __launch_bounds__(256) k_pr_sums(const R *x, const R *y, long n, double *partials) {
     double acc[2] = {0.0, 0.0};
     for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
         const R xv = x[k], yv = y[k];
@@ -284,7 +284,7 @@ __global__ void k_pr_sums(const R *x, const R *y, long n, double *partials) {
 // memory (cg.py:422-449, utils.h:139-146): beta = max(num/den, 0) in FP64,
 // rounded to the working precision as the reference's real_t scalar.
 template <typename R>
-__global__ void k_pr_beta_finish(const double *partials, int nb, double *beta) {
+__global__ void __launch_bounds__(256) k_pr_beta_finish(const double *partials, int nb, double *beta) {
     __shared__ double lds[RED_BLOCK / WAVE];
     double num = 0.0, den = 0.0;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) num += partials[(long)b * 2];
@@ -297,14 +297,14 @@ __global__ void k_pr_beta_finish(const double *partials, int nb, double *beta) {
 
 // z = alpha * x - y with alpha read from device memory.
 template <typename R>
-__global__ void k_axmy_dev(const R *x, const R *y, R *z, const double *alpha, long n) {
+__global__ void __launch_bounds__(256) k_axmy_dev(const R *x, const R *y, R *z, const double *alpha, long n) {
     const R a = R(*alpha);
     for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
         z[k] = a * x[k] - y[k];
 }
 
 template <typename R>
-__global__ void k_axpy(const R *x, const R *y, R *z, R alpha, long n, int minus) {
+__global__ void __launch_bounds__(256) k_axpy(const R *x, const R *y, R *z, R alpha, long n, int minus) {
     for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
         z[k] = minus ? alpha * x[k] - y[k] : alpha * x[k] + y[k];
 }
@@ -351,7 +351,7 @@ __device__ __forceinline__ R fe_node(const Geo &g, const GeoR<R> &q, R kappa2, R
 }
 
 template <typename R>
-__global__ void k_free_energy(Geo g, R kappa2, R eps, const R *eps_map, R H, const uint8_t *mt,
+__global__ void __launch_bounds__(256) k_free_energy(Geo g, R kappa2, R eps, const R *eps_map, R H, const uint8_t *mt,
                               const cplx<R> *psi, const R *abei, const R *ab, double *partials) {
     const GeoR<R> q(g);
     double acc[1] = {0.0};
@@ -401,7 +401,7 @@ __device__ __forceinline__ cplx<R> jac_psi_node(const Geo &g, const GeoR<R> &q, 
 }
 
 template <typename R>
-__global__ void k_jacobian_psi(Geo g, R kappa2, R eps, const R *eps_map, R H, const uint8_t *mt,
+__global__ void __launch_bounds__(256) k_jacobian_psi(Geo g, R kappa2, R eps, const R *eps_map, R H, const uint8_t *mt,
                                const cplx<R> *psi, const R *abei, const R *ab, cplx<R> *G) {
     const GeoR<R> q(g);
     const R dxdy = q.dx * q.dy;
@@ -455,7 +455,7 @@ __device__ __forceinline__ void jac_A_node(const Geo &g, const GeoR<R> &q, R kap
 }
 
 template <typename R>
-__global__ void k_jacobian_A(Geo g, R kappa2, R H, const uint8_t *mt, const cplx<R> *psi, const R *abei,
+__global__ void __launch_bounds__(256) k_jacobian_A(Geo g, R kappa2, R H, const uint8_t *mt, const cplx<R> *psi, const R *abei,
                              const R *ab, R *G) {
     const GeoR<R> q(g);
     const R s2 = R(2) * q.dx * q.dy;
@@ -470,30 +470,123 @@ __global__ void k_jacobian_A(Geo g, R kappa2, R H, const uint8_t *mt, const cplx
     }
 }
 
+// The ten link values around node (i, j) that its free energy and both
+// Jacobians read: a(i, j), a(i-1, j), a(i, j-1), a(i, j+1), a(i-1, j+1) and
+// b(i, j), b(i, j-1), b(i-1, j), b(i+1, j), b(i+1, j-1). Loaded from clamped
+// (always valid) indices, all at once; a value is used only where the
+// reference's conditions say the link exists.
+template <typename R> struct Links10 {
+    R a00, am0, a0m, a0p, amp, b00, b0m, bm0, bp0, bpm;
+};
+
+template <typename R>
+__device__ __forceinline__ Links10<R> load_links10(const Geo &g, const R *v, int i, int j) {
+    const int xa0 = min(i, g.Nxa - 1), xam = max(min(i - 1, g.Nxa - 1), 0);
+    const int ya0 = j, yam = max(j - 1, 0), yap = min(j + 1, g.Nya - 1);
+    const int xb0 = i, xbm = max(i - 1, 0), xbp = min(i + 1, g.Nxb - 1);
+    const int yb0 = min(j, g.Nyb - 1), ybm = max(min(j - 1, g.Nyb - 1), 0);
+    const R *b = v + g.Na;
+    Links10<R> L;
+    L.a00 = v[xa0 + (long)g.Nxa * ya0];
+    L.am0 = v[xam + (long)g.Nxa * ya0];
+    L.a0m = v[xa0 + (long)g.Nxa * yam];
+    L.a0p = v[xa0 + (long)g.Nxa * yap];
+    L.amp = v[xam + (long)g.Nxa * yap];
+    L.b00 = b[xb0 + (long)g.Nxb * yb0];
+    L.b0m = b[xb0 + (long)g.Nxb * ybm];
+    L.bm0 = b[xbm + (long)g.Nxb * yb0];
+    L.bp0 = b[xbp + (long)g.Nxb * yb0];
+    L.bpm = b[xbp + (long)g.Nxb * ybm];
+    return L;
+}
+
+// curlcurl_a / curlcurl_b on loaded link values (E: abei, B: ab; he, hb:
+// present), same terms in the same order.
+template <typename R>
+__device__ __forceinline__ R curlcurl_a_v(const Geo &g, const GeoR<R> &q, R H, bool he, const Links10<R> &E, bool hb,
+                                          const Links10<R> &B, int j) {
+    R c = R(0), dd = R(1);
+    if (j == 0) { c -= R(2) * H * q.idy; dd = R(2); }
+    else if (j + 1 == g.Nya) { c += R(2) * H * q.idy; dd = R(2); }
+    if (he) c += R(2) / dd * q.idy2 * E.a00;
+    if (hb) c += R(2) * q.idy2 * B.a00;
+    if (j > 0) {
+        if (he) c += dd / dd * (-q.idy2 * E.a0m + q.idxy * E.b0m - q.idxy * E.bpm);
+        if (hb) c += dd * (-q.idy2 * B.a0m + q.idxy * B.b0m - q.idxy * B.bpm);
+    }
+    if (j + 1 < g.Nya) {
+        if (he) c += dd / dd * (-q.idy2 * E.a0p - q.idxy * E.b00 + q.idxy * E.bp0);
+        if (hb) c += dd * (-q.idy2 * B.a0p - q.idxy * B.b00 + q.idxy * B.bp0);
+    }
+    return c;
+}
+
+template <typename R>
+__device__ __forceinline__ R curlcurl_b_v(const Geo &g, const GeoR<R> &q, R H, bool he, const Links10<R> &E, bool hb,
+                                          const Links10<R> &B, int i) {
+    R c = R(0), dd = R(1);
+    if (i == 0) { c += R(2) * H * q.idx; dd = R(2); }
+    else if (i + 1 == g.Nxb) { c -= R(2) * H * q.idx; dd = R(2); }
+    if (he) c += R(2) / dd * q.idx2 * E.b00;
+    if (hb) c += R(2) * q.idx2 * B.b00;
+    if (i > 0) {
+        if (he) c += dd / dd * (-q.idx2 * E.bm0 + q.idxy * E.am0 - q.idxy * E.amp);
+        if (hb) c += dd * (-q.idx2 * B.bm0 + q.idxy * B.am0 - q.idxy * B.amp);
+    }
+    if (i + 1 < g.Nxb) {
+        if (he) c += dd / dd * (-q.idx2 * E.bp0 - q.idxy * E.a00 + q.idxy * E.a0p);
+        if (hb) c += dd * (-q.idx2 * B.bp0 - q.idxy * B.a00 + q.idxy * B.a0p);
+    }
+    return c;
+}
+
 // The free-energy density, dF/dpsi and dF/dA of node n in one evaluation:
-// fe_node, jac_psi_node and jac_A_node fused. The cells, psi and its
-// neighbours and the sin/cos of the four link phases are loaded and
-// evaluated once (the right and up links serve all three); every quantity
-// keeps the operation order of the separate functions, so the results are
-// theirs bit for bit.
+// fe_node, jac_psi_node and jac_A_node (bscale 1) fused. Every value the
+// three read (psi and its four neighbours, the cells, epsilon, the ten link
+// values of abei and ab) is loaded once, up front, from clamped indices, so
+// the loads of a node go out together instead of one dependent branch at a
+// time; the sin/cos of the four link phases are evaluated once. Every
+// quantity keeps the operation order of the separate functions, so the
+// results are theirs bit for bit.
 template <typename R, bool withA>
 __device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kappa2, R eps, const R *eps_map, R H,
                                           const uint8_t *mt, const cplx<R> *psi, const R *abei, const R *ab, long n,
                                           int i, int j, R &e, cplx<R> &gj, R &ga, R &gb, bool &has_a, bool &has_b) {
-    const Cells c = cells_of(g, mt, i, j);
+    // loads
+    const long row = (long)g.Nx * j;
     const cplx<R> p0 = ld(psi, n);
+    const cplx<R> pl = ld(psi, row + max(i - 1, 0)), pr = ld(psi, row + min(i + 1, g.Nx - 1));
+    const cplx<R> pd = ld(psi, (long)g.Nx * max(j - 1, 0) + i), pu = ld(psi, (long)g.Nx * min(j + 1, g.Ny - 1) + i);
+    const R ee = eps_map ? eps_map[n] : eps;
+    Cells c;
+    c.mm = (i > 0) && (j > 0);
+    c.mp = (i > 0) && (j + 1 < g.Ny);
+    c.pm = (i + 1 < g.Nx) && (j > 0);
+    c.pp = (i + 1 < g.Nx) && (j + 1 < g.Ny);
+    if (mt) {
+        const int cx0 = min(i, g.Nxc - 1), cxm = max(i - 1, 0), cy0 = min(j, g.Nyc - 1), cym = max(j - 1, 0);
+        const bool m_mm = mt[cxm + (long)g.Nxc * cym] != 0, m_mp = mt[cxm + (long)g.Nxc * cy0] != 0;
+        const bool m_pm = mt[cx0 + (long)g.Nxc * cym] != 0, m_pp = mt[cx0 + (long)g.Nxc * cy0] != 0;
+        c.mm = c.mm && m_mm;
+        c.mp = c.mp && m_mp;
+        c.pm = c.pm && m_pm;
+        c.pp = c.pp && m_pp;
+    }
+    const bool he = abei != nullptr, hb = ab != nullptr;
+    Links10<R> E{}, B{};
+    if (he) E = load_links10(g, abei, i, j);
+    if (hb) B = load_links10(g, ab, i, j);
+    auto pvv = [&](R ve, R vb) {  // pv(abei, ab, k) of loaded values
+        R p = R(0);
+        if (he) p += ve;
+        if (hb) p += vb;
+        return p;
+    };
     const cplx<R> zero = {R(0), R(0)};
     const bool lr = c.pm || c.pp, lu = c.mp || c.pp;  // the right link a(i, j) and the up link b(i, j) are active
     R sr = R(0), cr = R(1), su = R(0), cu = R(1);
-    cplx<R> pr = zero, pu = zero;
-    if (lr) {
-        pr = ld(psi, n + 1);
-        sincos_r(q.dx * pv(abei, ab, i + (long)g.Nxa * j), &sr, &cr);
-    }
-    if (lu) {
-        pu = ld(psi, n + g.Nx);
-        sincos_r(q.dy * pv(abei, ab, g.Na + i + (long)g.Nxb * j), &su, &cu);
-    }
+    if (lr) sincos_r(q.dx * pvv(E.a00, B.a00), &sr, &cr);
+    if (lu) sincos_r(q.dy * pvv(E.b00, B.b00), &su, &cu);
     // |psi1 U - psi0|^2 and Im(conj psi0 U psi1) with U's (sin, cos) given
     auto gg = [&](R s, R cs, cplx<R> p1) {
         const R re = p1.re * cs + p1.im * s - p0.re;
@@ -510,7 +603,6 @@ __device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kapp
         const R r_m0 = R(0.5) * (f_mm + f_mp), r_p0 = R(0.5) * (f_pm + f_pp);
         const R r_0m = R(0.5) * (f_mm + f_pm), r_0p = R(0.5) * (f_mp + f_pp);
         const R gw = R(0.25) * (r_m0 + r_p0 + r_0m + r_0p);
-        const R ee = eps_map ? eps_map[n] : eps;
         // fe_node
         const R p2 = p0.re * p0.re + p0.im * p0.im;
         e += gw * (R(0.5) * p2 - ee) * p2;
@@ -530,26 +622,22 @@ __device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kapp
         };
         if (c.mm || c.mp) {
             R s, cs;
-            sincos_r(-q.dx * pv(abei, ab, (i - 1) + (long)g.Nxa * j), &s, &cs);
-            add(r_m0, q.idx2, s, cs, ld(psi, n - 1));
+            sincos_r(-q.dx * pvv(E.am0, B.am0), &s, &cs);
+            add(r_m0, q.idx2, s, cs, pl);
         }
         if (lr) add(r_p0, q.idx2, sr, cr, pr);
         if (c.mm || c.pm) {
             R s, cs;
-            sincos_r(-q.dy * pv(abei, ab, g.Na + i + (long)g.Nxb * (j - 1)), &s, &cs);
-            add(r_0m, q.idy2, s, cs, ld(psi, n - g.Nx));
+            sincos_r(-q.dy * pvv(E.b0m, B.b0m), &s, &cs);
+            add(r_0m, q.idy2, s, cs, pd);
         }
         if (lu) add(r_0p, q.idy2, su, cu, pu);
     }
     // fe_node: the cell's magnetic energy
     if (kappa2 > R(0) && i < g.Nxc && j < g.Nyc) {
         R dB = -H;
-        for (int sv = 0; sv < 2; ++sv) {
-            const R *v = sv == 0 ? abei : ab;
-            if (!v) continue;
-            dB += q.idx * (v[g.Na + i + 1 + (long)g.Nxb * j] - v[g.Na + i + (long)g.Nxb * j]) -
-                  q.idy * (v[i + (long)g.Nxa * (j + 1)] - v[i + (long)g.Nxa * j]);
-        }
+        if (he) dB += q.idx * (E.bp0 - E.b00) - q.idy * (E.a0p - E.a00);
+        if (hb) dB += q.idx * (B.bp0 - B.b00) - q.idy * (B.a0p - B.a00);
         e += kappa2 * dB * dB;
     }
     // jac_A_node (bscale 1)
@@ -559,7 +647,7 @@ __device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kapp
         has_a = i < g.Nxa && j < g.Nya;
         has_b = i < g.Nxb && j < g.Nyb;
         if (has_a) {
-            R t = curlcurl_a(g, q, H, abei, ab, i, j);
+            R t = curlcurl_a_v(g, q, H, he, E, hb, B, j);
             t *= kappa2;
             if (lr) {
                 const R r = R(0.5) * ((c.pm ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
@@ -568,7 +656,7 @@ __device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kapp
             ga = t;
         }
         if (has_b) {
-            R t = curlcurl_b(g, q, H, abei, ab, i, j);
+            R t = curlcurl_b_v(g, q, H, he, E, hb, B, i);
             t *= kappa2;
             if (lu) {
                 const R r = R(0.5) * ((c.mp ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
@@ -585,7 +673,7 @@ __device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kapp
 // of Polak-Ribiere sums against the previous Jacobians. partials per
 // workgroup: F, num_psi, den_psi, num_A, den_A. withA: kappa < inf.
 template <typename R, bool withA>
-__global__ void k_cg_gradient_pass(Geo g, R kappa2, R eps, const R *eps_map, R H, const uint8_t *mt,
+__global__ void __launch_bounds__(256) k_cg_gradient_pass(Geo g, R kappa2, R eps, const R *eps_map, R H, const uint8_t *mt,
                                    const cplx<R> *psi, const R *abei, const R *ab, cplx<R> *jac,
                                    const cplx<R> *jac_prev, R *jacA, const R *jacA_prev, double *partials) {
     const GeoR<R> q(g);
@@ -632,7 +720,7 @@ __global__ void k_cg_gradient_pass(Geo g, R kappa2, R eps, const R *eps_map, R H
 // working precision (cg.py:422-449), left in device memory for the next
 // direction update.
 template <typename R>
-__global__ void k_cg_gradient_finish(const double *partials, int nb, double dxdy, double *out, double *beta_next) {
+__global__ void __launch_bounds__(256) k_cg_gradient_finish(const double *partials, int nb, double dxdy, double *out, double *beta_next) {
     __shared__ double lds[RED_BLOCK / WAVE];
     double v[5];
     for (int k = 0; k < 5; ++k) {
@@ -712,7 +800,7 @@ __device__ __forceinline__ void quartic2_min(const double (&c)[5][5], bool one_d
 // minimised in FP64 from (0, 0) by quartic2_min; the alphas go to
 // alpha[0..1] and the coefficients to c[0..L-1]. One workgroup.
 template <int L>
-__global__ void k_line_search(const double *partials, int nb, double scale, double *c, double *alpha) {
+__global__ void __launch_bounds__(256) k_line_search(const double *partials, int nb, double scale, double *c, double *alpha) {
     __shared__ double lds[RED_BLOCK / WAVE];
     __shared__ double red[L];
     for (int k = 0; k < L; ++k) {
@@ -747,7 +835,7 @@ __global__ void k_line_search(const double *partials, int nb, double scale, doub
 // working precision (the reference's axpy with the host alpha, cg.py:296,
 // 535-536).
 template <typename R>
-__global__ void k_axpy_dev(const R *x, const R *y, R *z, const double *alpha, long n) {
+__global__ void __launch_bounds__(256) k_axpy_dev(const R *x, const R *y, R *z, const double *alpha, long n) {
     const R a = R(*alpha);
     for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
         z[k] = a * x[k] + y[k];
@@ -756,24 +844,73 @@ __global__ void k_axpy_dev(const R *x, const R *y, R *z, const double *alpha, lo
 // Line-search coefficients. L = 5 (cg.h:314-474) or 17 (cg.h:477-731).
 // One node's contributions are added to C (before the dxdy factor). The
 // reference's kernels read only the scalar eps (eps_map is NULL on the
-// drop-in path); the batched CG passes the map.
+// drop-in path); the batched CG passes the map. Every value the node reads
+// (psi, dpsi and their right and up neighbours, the cells, epsilon, the four
+// link values a(i, j), a(i, j+1), b(i, j), b(i+1, j) of abei, ab and dab)
+// is loaded up front from clamped indices, and the link's sin/cos is
+// evaluated once for all its terms; the arithmetic is the reference's term
+// by term.
+template <typename R> struct Links4 {
+    R a00, a0p, b00, bp0;
+};
+
+template <typename R> __device__ __forceinline__ Links4<R> load_links4(const Geo &g, const R *v, int i, int j) {
+    const int xa = min(i, g.Nxa - 1), yap = min(j + 1, g.Nya - 1);
+    const int xbp = min(i + 1, g.Nxb - 1), yb = min(j, g.Nyb - 1);
+    const R *b = v + g.Na;
+    Links4<R> L;
+    L.a00 = v[xa + (long)g.Nxa * j];
+    L.a0p = v[xa + (long)g.Nxa * yap];
+    L.b00 = b[i + (long)g.Nxb * yb];
+    L.bp0 = b[xbp + (long)g.Nxb * yb];
+    return L;
+}
+
 template <typename R, int L>
 __device__ __forceinline__ void coef_node(const Geo &g, const GeoR<R> &q, R kappa2, R eps_s, const R *eps_map, R H,
                                           const uint8_t *mt, const cplx<R> *psi, const cplx<R> *dpsi, const R *abei,
                                           const R *ab, const R *dab, long n, R (&C)[L]) {
     int i, j;
     node_ij(g, n, i, j);
-    const Cells c = cells_of(g, mt, i, j);
+    // loads
+    const long nr = (long)g.Nx * j + min(i + 1, g.Nx - 1), nu = (long)g.Nx * min(j + 1, g.Ny - 1) + i;
+    const cplx<R> p0 = ld(psi, n), d0 = ld(dpsi, n);
+    const cplx<R> p_r = ld(psi, nr), d_r = ld(dpsi, nr), p_u = ld(psi, nu), d_u = ld(dpsi, nu);
+    const R eps = eps_map ? eps_map[n] : eps_s;
+    Cells c;
+    c.mm = (i > 0) && (j > 0);
+    c.mp = (i > 0) && (j + 1 < g.Ny);
+    c.pm = (i + 1 < g.Nx) && (j > 0);
+    c.pp = (i + 1 < g.Nx) && (j + 1 < g.Ny);
+    if (mt) {
+        const int cx0 = min(i, g.Nxc - 1), cxm = max(i - 1, 0), cy0 = min(j, g.Nyc - 1), cym = max(j - 1, 0);
+        const bool m_mm = mt[cxm + (long)g.Nxc * cym] != 0, m_mp = mt[cxm + (long)g.Nxc * cy0] != 0;
+        const bool m_pm = mt[cx0 + (long)g.Nxc * cym] != 0, m_pp = mt[cx0 + (long)g.Nxc * cy0] != 0;
+        c.mm = c.mm && m_mm;
+        c.mp = c.mp && m_mp;
+        c.pm = c.pm && m_pm;
+        c.pp = c.pp && m_pp;
+    }
+    const bool he = abei != nullptr, hb = ab != nullptr;
+    Links4<R> E{}, B{}, D{};
+    if (he) E = load_links4(g, abei, i, j);
+    if (hb) B = load_links4(g, ab, i, j);
+    if (L == 17) D = load_links4(g, dab, i, j);
+    auto pvv = [&](R ve, R vb) {  // pv(abei, ab, k) of loaded values
+        R p = R(0);
+        if (he) p += ve;
+        if (hb) p += vb;
+        return p;
+    };
+
     if (c.mm || c.mp || c.pm || c.pp) {
         const R f_mm = c.mm ? R(1) : R(0), f_mp = c.mp ? R(1) : R(0), f_pm = c.pm ? R(1) : R(0), f_pp = c.pp ? R(1) : R(0);
         const R r_m0 = R(0.5) * (f_mm + f_mp), r_p0 = R(0.5) * (f_pm + f_pp);
         const R r_0m = R(0.5) * (f_mm + f_pm), r_0p = R(0.5) * (f_mp + f_pp);
         const R gw = R(0.25) * (r_m0 + r_p0 + r_0m + r_0p);
-        const cplx<R> p0 = ld(psi, n), d0 = ld(dpsi, n);
         const R p2 = p0.re * p0.re + p0.im * p0.im;
         const R d2 = d0.re * d0.re + d0.im * d0.im;
         const R tw = R(2) * (p0.re * d0.re + p0.im * d0.im);
-        const R eps = eps_map ? eps_map[n] : eps_s;
         const R k0 = gw * (R(0.5) * p2 - eps) * p2;
         const R k1 = gw * tw * (p2 - eps);
         const R k2 = gw * (-eps * d2 + R(0.5) * tw * tw + p2 * d2);
@@ -787,26 +924,32 @@ __device__ __forceinline__ void coef_node(const Geo &g, const GeoR<R> &q, R kapp
             const R r = lk == 0 ? r_p0 : r_0p;
             const R id2 = lk == 0 ? q.idx2 : q.idy2;
             const R d = lk == 0 ? q.dx : q.dy;
-            const long k = lk == 0 ? i + (long)g.Nxa * j : g.Na + i + (long)g.Nxb * j;
-            const long nb = lk == 0 ? n + 1 : n + g.Nx;
             R ph = R(0);
-            if (abei) ph += d * abei[k];
-            if (ab) ph += d * ab[k];
-            const cplx<R> p1 = ld(psi, nb), d1 = ld(dpsi, nb);
-            const cplx<R> zp = grad_c(p0, ph, p1), zd = grad_c(d0, ph, d1);
+            if (he) ph += d * (lk == 0 ? E.a00 : E.b00);
+            if (hb) ph += d * (lk == 0 ? B.a00 : B.b00);
+            const cplx<R> p1 = lk == 0 ? p_r : p_u, d1 = lk == 0 ? d_r : d_u;
+            R s, cs;
+            sincos_r(ph, &s, &cs);  // U(-ph) = (cos ph, sin ph)
+            auto gc = [&](cplx<R> a0, cplx<R> a1) -> cplx<R> {  // grad_c(a0, ph, a1)
+                return {a1.re * cs + a1.im * s - a0.re, a1.im * cs - a1.re * s - a0.im};
+            };
+            auto gg = [&](cplx<R> a0, cplx<R> a1) -> R {  // g_grad(a0, ph, a1)
+                const R re = a1.re * cs + a1.im * s - a0.re;
+                const R im = a1.im * cs - a1.re * s - a0.im;
+                return re * re + im * im;
+            };
+            const cplx<R> zp = gc(p0, p1), zd = gc(d0, d1);
             const R zr = zp.re * zd.re - (-zp.im) * zd.im;  // Re(conj(zp) zd)
             const R w = r * id2;
             if (L == 5) {
-                C[2] += w * g_grad(d0, ph, d1);
+                C[2] += w * gg(d0, d1);
                 C[1] += w * R(2) * zr;
-                C[0] += w * g_grad(p0, ph, p1);
+                C[0] += w * gg(p0, p1);
             } else {
-                const R dph = d * dab[k], dph2 = dph * dph;
-                C[0] += w * g_grad(p0, ph, p1);
+                const R dph = d * (lk == 0 ? D.a00 : D.b00), dph2 = dph * dph;
+                C[0] += w * gg(p0, p1);
                 C[5] += w * R(2) * zr;
-                C[10] += w * g_grad(d0, ph, d1);
-                R s, cs;
-                sincos_r(ph, &s, &cs);  // U(-ph) = (cos ph, sin ph)
+                C[10] += w * gg(d0, d1);
                 auto mulU = [&](cplx<R> a, cplx<R> b) -> cplx<R> {  // a * U(-ph) * conj(b)
                     const cplx<R> t = {a.re * cs - a.im * s, a.re * s + a.im * cs};
                     return {t.re * b.re - t.im * (-b.im), t.re * (-b.im) + t.im * b.re};
@@ -827,14 +970,12 @@ __device__ __forceinline__ void coef_node(const Geo &g, const GeoR<R> &q, R kapp
         }
     }
     if (kappa2 > R(0) && i < g.Nxc && j < g.Nyc) {
-        const long a00 = i + (long)g.Nxa * j, a0p = i + (long)g.Nxa * (j + 1);
-        const long b00 = g.Na + i + (long)g.Nxb * j, bp0 = g.Na + i + 1 + (long)g.Nxb * j;
         if (L == 5) {
-            const R dB = q.idx * (ab[bp0] - ab[b00]) - q.idy * (ab[a0p] - ab[a00]) - H;
+            const R dB = q.idx * (B.bp0 - B.b00) - q.idy * (B.a0p - B.a00) - H;
             C[0] += kappa2 * dB * dB;
         } else {
-            const R BH = q.idx * (pv(abei, ab, bp0) - pv(abei, ab, b00)) - q.idy * (pv(abei, ab, a0p) - pv(abei, ab, a00)) - H;
-            const R dB = q.idx * (dab[bp0] - dab[b00]) - q.idy * (dab[a0p] - dab[a00]);
+            const R BH = q.idx * (pvv(E.bp0, B.bp0) - pvv(E.b00, B.b00)) - q.idy * (pvv(E.a0p, B.a0p) - pvv(E.a00, B.a00)) - H;
+            const R dB = q.idx * (D.bp0 - D.b00) - q.idy * (D.a0p - D.a00);
             C[0] += kappa2 * BH * BH;
             C[1] += kappa2 * R(2) * BH * dB;
             C[2] += kappa2 * dB * dB;
@@ -843,7 +984,7 @@ __device__ __forceinline__ void coef_node(const Geo &g, const GeoR<R> &q, R kapp
 }
 
 template <typename R, int L>
-__global__ void k_coef(Geo g, R kappa2, R eps, R H, const uint8_t *mt, const cplx<R> *psi, const cplx<R> *dpsi,
+__global__ void __launch_bounds__(256) k_coef(Geo g, R kappa2, R eps, R H, const uint8_t *mt, const cplx<R> *psi, const cplx<R> *dpsi,
                        const R *abei, const R *ab, const R *dab, double *partials) {
     const GeoR<R> q(g);
     double acc[L];
@@ -877,7 +1018,7 @@ __device__ __forceinline__ double np_mod_2pi(double x) {
 constexpr int VREC = 14;
 
 template <typename R>
-__global__ void k_vortex_cells(Geo g, const cplx<R> *psi, const R *ab, const R *abi, double H, double *rec,
+__global__ void __launch_bounds__(256) k_vortex_cells(Geo g, const cplx<R> *psi, const R *ab, const R *abi, double H, double *rec,
                                int capacity, int *count) {
     const double pi = 3.141592653589793;
     const double dxdyH = g.dx * g.dy * H;
@@ -915,7 +1056,7 @@ __global__ void k_vortex_cells(Geo g, const cplx<R> *psi, const R *ab, const R *
 }
 
 template <typename R>
-__global__ void k_phase_lock(cplx<R> *psi, int lockN, const int *ns) {
+__global__ void __launch_bounds__(256) k_phase_lock(cplx<R> *psi, int lockN, const int *ns) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l < lockN) {
         const int n = ns[l];
