@@ -70,7 +70,13 @@ __global__ void __launch_bounds__(256) k_batch_cg(const CGInst<R> *insts, double
             // 1. gradients, PR sums and the gradient norm
             double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // |g|^2, num_psi, den_psi, num_A, den_A
             for (long m = threadIdx.x; m < g.N; m += blockDim.x) {
-                const cplx<R> gj = jac_psi_node(g, q, I.eps, I.eps_map, I.mt, I.psi, (const R *)nullptr, I.ab, m);
+                int i, j;
+                node_ij(g, m, i, j);
+                R e, ga, gb;
+                cplx<R> gj;
+                bool ha, hb;
+                grad_node<R, withA>(g, q, I.kappa2, I.eps, I.eps_map, I.H, I.mt, I.psi, (const R *)nullptr, I.ab, m, i,
+                                    j, e, gj, ga, gb, ha, hb, R(0.5));
                 const cplx<R> gv = {dxdy * gj.re, dxdy * gj.im};
                 const cplx<R> gp = ld(jprev, m);
                 st(jac, m, gv);
@@ -78,12 +84,6 @@ __global__ void __launch_bounds__(256) k_batch_cg(const CGInst<R> *insts, double
                 acc[1] += double(gv.re * (gv.re - gp.re)) + double(gv.im * (gv.im - gp.im));
                 acc[2] += double(gp.re * gp.re) + double(gp.im * gp.im);
                 if (withA) {
-                    R ga, gb;
-                    bool ha, hb;
-                    jac_A_node(g, q, I.kappa2, I.H, I.mt, I.psi, (const R *)nullptr, I.ab, m, ga, gb, ha, hb,
-                               R(0.5));
-                    int i, j;
-                    node_ij(g, m, i, j);
                     if (ha) {
                         const long la = i + (long)g.Nxa * j;
                         const R v = s2 * ga, vp = jprevA[la];

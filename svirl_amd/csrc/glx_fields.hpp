@@ -541,7 +541,7 @@ __device__ __forceinline__ R curlcurl_b_v(const Geo &g, const GeoR<R> &q, R H, b
 }
 
 // The free-energy density, dF/dpsi and dF/dA of node n in one evaluation:
-// fe_node, jac_psi_node and jac_A_node (bscale 1) fused. Every value the
+// fe_node, jac_psi_node and jac_A_node fused. Every value the
 // three read (psi and its four neighbours, the cells, epsilon, the ten link
 // values of abei and ab) is loaded once, up front, from clamped indices, so
 // the loads of a node go out together instead of one dependent branch at a
@@ -551,7 +551,8 @@ __device__ __forceinline__ R curlcurl_b_v(const Geo &g, const GeoR<R> &q, R H, b
 template <typename R, bool withA>
 __device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kappa2, R eps, const R *eps_map, R H,
                                           const uint8_t *mt, const cplx<R> *psi, const R *abei, const R *ab, long n,
-                                          int i, int j, R &e, cplx<R> &gj, R &ga, R &gb, bool &has_a, bool &has_b) {
+                                          int i, int j, R &e, cplx<R> &gj, R &ga, R &gb, bool &has_a, bool &has_b,
+                                          R bscale = R(1)) {
     // loads
     const long row = (long)g.Nx * j;
     const cplx<R> p0 = ld(psi, n);
@@ -640,7 +641,7 @@ __device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kapp
         if (hb) dB += q.idx * (B.bp0 - B.b00) - q.idy * (B.a0p - B.a00);
         e += kappa2 * dB * dB;
     }
-    // jac_A_node (bscale 1)
+    // jac_A_node
     has_a = has_b = false;
     ga = gb = R(0);
     if (withA) {
@@ -648,6 +649,7 @@ __device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kapp
         has_b = i < g.Nxb && j < g.Nyb;
         if (has_a) {
             R t = curlcurl_a_v(g, q, H, he, E, hb, B, j);
+            if (j == 0 || j + 1 == g.Nya) t *= bscale;
             t *= kappa2;
             if (lr) {
                 const R r = R(0.5) * ((c.pm ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
@@ -657,6 +659,7 @@ __device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kapp
         }
         if (has_b) {
             R t = curlcurl_b_v(g, q, H, he, E, hb, B, i);
+            if (i == 0 || i + 1 == g.Nxb) t *= bscale;
             t *= kappa2;
             if (lu) {
                 const R r = R(0.5) * ((c.mp ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
