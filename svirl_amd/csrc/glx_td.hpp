@@ -1273,7 +1273,7 @@ struct SolveOut {
 template <typename R, bool GAUGE, bool SH = false>
 __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const GridSync &gs, const Team &tm, long w,
                                               long W, int pc, int qc, int pred, unsigned long long &epoch, int *lds) {
-    constexpr int KMAX = SH ? 2 : Fuse<R>::PSI;
+    constexpr int KMAX = Fuse<R>::PSI;
     const int o1 = (pc + 1) % 3, o2 = (pc + 2) % 3;
     PsiSweepArgs<R> A = T.psi;
     A.ab = pick3(T.Q, qc);
