@@ -36,7 +36,7 @@ from svirl_amd._native import BatchCGInstance, BatchInstance
 # (signature) -> (wg_per_cu, min_rows)
 _TUNED = {}
 
-CANDIDATES = [(0, 16), (2, 16), (0, 8), (0, 32), (1, 16)]
+CANDIDATES = [(0, 8), (0, 16), (0, 12), (0, 4), (2, 8)]
 
 
 class BatchTD:
