@@ -100,12 +100,6 @@ template <typename R> __device__ __forceinline__ R g_grad(cplx<R> p0, R ph, cplx
     const R im = p1.im * c - p1.re * s - p0.im;
     return re * re + im * im;
 }
-// psi1 U(ph) - psi0 (cg.h:304-311)
-template <typename R> __device__ __forceinline__ cplx<R> grad_c(cplx<R> p0, R ph, cplx<R> p1) {
-    R s, c;
-    sincos_r(ph, &s, &c);
-    return {p1.re * c + p1.im * s - p0.re, p1.im * c - p1.re * s - p0.im};
-}
 
 // ---------------------------------------------------------------- B field
 template <typename R>
@@ -417,12 +411,12 @@ __global__ void __launch_bounds__(256) k_jacobian_psi(Geo g, R kappa2, R eps, co
 // dF/dA of the a link (i, j) and the b link (i, j) at node n, before the
 // 2 dxdy factor (free_energy_jacobian_A, cg.h:124-301); has_a / has_b say
 // which links exist. The reference's field part on boundary links is twice
-// the derivative of F (ghost cell with dd = 2); bscale multiplies it there:
-// 1 keeps the reference's Jacobian, 1/2 gives the exact gradient.
+// the derivative of F (ghost cell with dd = 2); grad_node's bscale = 1/2
+// gives the exact gradient there.
 template <typename R>
 __device__ __forceinline__ void jac_A_node(const Geo &g, const GeoR<R> &q, R kappa2, R H, const uint8_t *mt,
                                            const cplx<R> *psi, const R *abei, const R *ab, long n, R &ga, R &gb,
-                                           bool &has_a, bool &has_b, R bscale = R(1)) {
+                                           bool &has_a, bool &has_b) {
     int i, j;
     node_ij(g, n, i, j);
     const cplx<R> p0 = ld(psi, n);
@@ -433,7 +427,6 @@ __device__ __forceinline__ void jac_A_node(const Geo &g, const GeoR<R> &q, R kap
     if (has_a) {
         const long m = i + (long)g.Nxa * j;
         R gj = curlcurl_a(g, q, H, abei, ab, i, j);
-        if (j == 0 || j + 1 == g.Nya) gj *= bscale;
         gj *= kappa2;
         if (c.pm || c.pp) {
             const R r = R(0.5) * ((c.pm ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
@@ -444,7 +437,6 @@ __device__ __forceinline__ void jac_A_node(const Geo &g, const GeoR<R> &q, R kap
     if (has_b) {
         const long m = g.Na + i + (long)g.Nxb * j;
         R gj = curlcurl_b(g, q, H, abei, ab, i, j);
-        if (i == 0 || i + 1 == g.Nxb) gj *= bscale;
         gj *= kappa2;
         if (c.mp || c.pp) {
             const R r = R(0.5) * ((c.mp ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
@@ -547,7 +539,9 @@ __device__ __forceinline__ R curlcurl_b_v(const Geo &g, const GeoR<R> &q, R H, b
 // the loads of a node go out together instead of one dependent branch at a
 // time; the sin/cos of the four link phases are evaluated once. Every
 // quantity keeps the operation order of the separate functions, so the
-// results are theirs bit for bit.
+// results are theirs bit for bit. bscale multiplies the field part of dF/dA
+// on boundary links: 1 keeps the reference's Jacobian (twice the derivative
+// there, ghost cell with dd = 2), 1/2 gives the exact gradient.
 template <typename R, bool withA>
 __device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kappa2, R eps, const R *eps_map, R H,
                                           const uint8_t *mt, const cplx<R> *psi, const R *abei, const R *ab, long n,
@@ -933,7 +927,7 @@ __device__ __forceinline__ void coef_node(const Geo &g, const GeoR<R> &q, R kapp
             const cplx<R> p1 = lk == 0 ? p_r : p_u, d1 = lk == 0 ? d_r : d_u;
             R s, cs;
             sincos_r(ph, &s, &cs);  // U(-ph) = (cos ph, sin ph)
-            auto gc = [&](cplx<R> a0, cplx<R> a1) -> cplx<R> {  // grad_c(a0, ph, a1)
+            auto gc = [&](cplx<R> a0, cplx<R> a1) -> cplx<R> {  // psi1 U(ph) - psi0 (cg.h:304-311)
                 return {a1.re * cs + a1.im * s - a0.re, a1.im * cs - a1.re * s - a0.im};
             };
             auto gg = [&](cplx<R> a0, cplx<R> a1) -> R {  // g_grad(a0, ph, a1)
