@@ -36,6 +36,7 @@ enum { CG_IT = 0, CG_DONE = 1, CG_BPSI = 2, CG_BA = 3, CG_G0 = 4, CG_GREL = 5, C
 
 // Sum of L doubles per thread over the workgroup, into out (thread 0).
 template <int L> __device__ __forceinline__ void block_sums(double (&acc)[L], double *lds, double *out) {
+#pragma unroll
     for (int k = 0; k < L; ++k) {
         const double s = block_sum(acc[k], lds);
         if (threadIdx.x == 0) out[k] = s;
@@ -51,7 +52,7 @@ __global__ void __launch_bounds__(256) k_batch_cg(const CGInst<R> *insts, double
     __shared__ double bc[4];   // beta_psi, beta_A, alpha_psi, alpha_A
     __shared__ int go, s_it;
     for (int k = blockIdx.x; k < n; k += gridDim.x) {
-        const CGInst<R> I = insts[k];
+        const CGInst<R> &I = insts[k];
         const Geo &g = I.g;
         const GeoR<R> q(g);
         const R dxdy = q.dx * q.dy, s2 = R(2) * q.dx * q.dy;
@@ -65,8 +66,11 @@ __global__ void __launch_bounds__(256) k_batch_cg(const CGInst<R> *insts, double
             __syncthreads();
             if (!go) break;
             const int itn = s_it;
-            cplx<R> *jac = I.jac[itn & 1], *jprev = I.jac[(itn + 1) & 1];
-            R *jacA = I.jacA[itn & 1], *jprevA = I.jacA[(itn + 1) & 1];
+            // (selects, not indexing: a run-time index into the instance
+            // record would move the whole record to scratch memory)
+            const bool odd = (itn & 1) != 0;
+            cplx<R> *jac = odd ? I.jac[1] : I.jac[0], *jprev = odd ? I.jac[0] : I.jac[1];
+            R *jacA = odd ? I.jacA[1] : I.jacA[0], *jprevA = odd ? I.jacA[0] : I.jacA[1];
             // 1. gradients, PR sums and the gradient norm
             double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // |g|^2, num_psi, den_psi, num_A, den_A
             for (long m = threadIdx.x; m < g.N; m += blockDim.x) {
@@ -154,8 +158,8 @@ __global__ void __launch_bounds__(256) k_batch_cg(const CGInst<R> *insts, double
                         c5[1][j] = sc * red[5 + j];
                         c5[2][j] = sc * red[10 + j];
                     }
-                    c5[3][0] = sc * red[15];
-                    c5[4][0] = sc * red[16];
+                    c5[3][0] = sc * red[15 % L];
+                    c5[4][0] = sc * red[16 % L];
                 } else {
                     for (int i = 0; i < 5; ++i) c5[i][0] = sc * red[i];
                 }

@@ -739,7 +739,9 @@ __device__ __forceinline__ void poly2(const double (&c)[5][5], double x, double 
     double xi[5] = {1.0, x, x * x, x * x * x, x * x * x * x};
     double yj[5] = {1.0, y, y * y, y * y * y, y * y * y * y};
     p = px = py = pxx = pxy = pyy = 0.0;
+#pragma unroll
     for (int i = 0; i < 5; ++i)
+#pragma unroll
         for (int j = 0; j < 5; ++j) {
             const double cij = c[i][j];
             if (cij == 0.0) continue;
