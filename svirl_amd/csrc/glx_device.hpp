@@ -162,9 +162,10 @@ template <> __device__ __forceinline__ int scaled_residual<float>(float r, doubl
 }
 
 // Lane i receives lane i-1 (up) or lane i+1 (down) of a wave64, by DPP
-// whole-wave shifts; the edge lane that has no source gets 0.
-__device__ __forceinline__ int dpp_up1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false); }
-__device__ __forceinline__ int dpp_dn1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false); }
+// whole-wave shifts; the edge lane that has no source gets 0 (bound_ctrl:
+// the hardware writes the zero, so no register has to be cleared first).
+__device__ __forceinline__ int dpp_up1(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, true); }
+__device__ __forceinline__ int dpp_dn1(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xF, 0xF, true); }
 __device__ __forceinline__ int shfl_up1(int v) { return dpp_up1(v); }
 __device__ __forceinline__ int shfl_dn1(int v) { return dpp_dn1(v); }
 __device__ __forceinline__ float shfl_up1(float v) { return __int_as_float(dpp_up1(__float_as_int(v))); }
