@@ -1,13 +1,14 @@
 // TDGL Jacobi sweeps for psi and A on gfx950.
 //
-// Layout of the work: the grid is cut into vertical strips of 62 columns;
-// one wave marches down a range of rows of one strip. Lanes 0 and 63 of a
-// wave hold the columns left and right of the strip (halos), lanes 1..62 own
-// one column each. Each lane keeps a three-row window of its column in
-// registers, so every field value is read from HBM once per sweep;
-// horizontal neighbours come from adjacent lanes by cross-lane shuffles.
-// Each lane evaluates sin/cos once per link it carries (2 per node per
-// sweep) and hands the values to its right neighbour.
+// Layout of the work: the grid is cut into vertical strips of 62 columns
+// (58 or 60 for passes that fuse three or two sweeps); one wave marches down
+// a range of rows of one strip. Lanes 0 and 63 of a wave hold the columns
+// left and right of the strip (halos), lanes 1..62 own one column each.
+// Each lane keeps a three-row window of its column in registers, so every
+// field value is read from HBM once per pass; horizontal neighbours come
+// from adjacent lanes by cross-lane shuffles. Each lane evaluates sin/cos
+// once per link it carries (2 per node per pass) and hands the values to
+// its right neighbour.
 //
 // Arithmetic follows svirl/cuda/td.h term by term (same operand order, no
 // reassociation); FP64 instantiations are compiled without FMA contraction.
@@ -80,17 +81,17 @@ template <typename R, typename Args> __device__ __forceinline__ R gauge_phase_b(
 }
 
 // Rows are loaded in batches of Pipe<R>::ROWS (single sweeps), ROWS2 (fused
-// psi pairs) or AROWS2 (fused A pairs): the loads of batch k+1 are issued
-// before batch k is computed, and the two batches live in separately named
-// registers (the loop body is unrolled over the pair), so no loaded value is
-// copied before it is used and 1 to 2 batches of loads per wave stay in
-// flight during the arithmetic. The persistent solves run two waves per
-// SIMD, which leaves registers for two-row batches (measured at 4097^2: FP32
-// psi pairs 93 -> 85 us per sweep, FP64 162 -> 155; deeper batches and
-// FP32 A pairs gain nothing). Loads beyond the wave's last row are clamped
-// to that row (cache hits, never used).
-// The batched kernel runs its solves with SH (shallow) batches, which keep
-// its register count within two waves per SIMD.
+// psi passes) or AROWS2 / AROWS3 (fused A passes of two / three sweeps): the
+// loads of batch k+1 are issued before batch k is computed, and the two
+// batches live in separately named registers (the loop body is unrolled
+// over the two), so no loaded value is copied before it is used and 1 to 2
+// batches of loads per wave stay in flight during the arithmetic. The
+// persistent solves run two waves per SIMD, which leaves registers for
+// two-row batches (measured at 4097^2 on two-sweep psi passes: FP32 93 ->
+// 85 us per sweep, FP64 162 -> 155; deeper batches and FP32 A passes gain
+// nothing). Loads beyond the wave's last row are clamped to that row (cache
+// hits, never used). The batched kernel runs its solves with SH (shallow)
+// batches, which keep its register count within two waves per SIMD.
 template <typename R, bool SH = false> struct Pipe { static constexpr int ROWS = 2, ROWS2 = 2, AROWS2 = 1, AROWS3 = 1; };
 template <> struct Pipe<double, false> { static constexpr int ROWS = 2, ROWS2 = 2, AROWS2 = 2, AROWS3 = 1; };
 template <> struct Pipe<float, true> { static constexpr int ROWS = 2, ROWS2 = 1, AROWS2 = 1, AROWS3 = 1; };
@@ -155,8 +156,8 @@ __device__ __forceinline__ PsiRow<R> psi_load_row(const PsiSweepArgs<R> &A, cons
 // mp = (c-1, j), pmc = (c, j-1), ppc = (c, j)); FP32 returns its reciprocal
 // (hardware estimate + one Newton step, within 1 ulp, one reciprocal for
 // both components instead of two IEEE divisions), FP64 the diagonal itself.
-// It depends on the sweep only through rhs, so the two sweeps of a fused
-// pair share it.
+// It depends on the sweep only through rhs, so the sweeps of a fused pass
+// share it.
 template <typename R>
 __device__ __forceinline__ R psi_den(const PsiSweepArgs<R> &A, const cplx<R> &rhs, R e, bool mm, bool mp, bool pmc,
                                      bool ppc) {
