@@ -302,6 +302,17 @@ def main():
             log(f"[{prec}] psi solve {out['psi_launch_us']:.1f} us/launch, {out['psi_gbs']:.1f} GB/s algorithmic; "
                 f"A solve {out['a_launch_us']:.1f} us/launch")
         out["F_td"] = gl.observables.free_energy_fp64()
+        # vortex detection on the final state (winding + triangulation on the
+        # GPU, ordering on the host); the second call is timed
+        for _ in range(2):
+            gl.solve.vortices_detected = False
+            t0 = time.perf_counter()
+            vx = gl.vortex_detector.vortices[0]
+            out["vortex_ms"] = 1e3 * (time.perf_counter() - t0)
+        out["vortices"] = int(len(vx))
+        out["vortex_cells"] = int(len(gl.vortex_detector.vortex_cells[0]))
+        log(f"[{prec}] vortex detection {out['vortex_ms']:.1f} ms: {out['vortex_cells']} cells flagged, "
+            f"{out['vortices']} vortices")
         if not args.skip_fp64:
             jx, jy = gl.observables.current_density
             out["psi_h"] = np.asarray(gl.vars.order_parameter, dtype=np.complex128).ravel()
@@ -367,6 +378,8 @@ def main():
         "sweeps_per_step": {"psi": r32["sweeps_psi"], "A": r32["sweeps_A"]},
         "a_solve": {"launch_us": r32.get("a_launch_us"), "achieved_GBs": r32.get("a_gbs")},
         "td_all_kernels_GBs": r32["all_gbs"],
+        "vortex_detector": {"fp32_ms": r32["vortex_ms"], "fp32_vortices": r32["vortices"],
+                            "fp32_cells_flagged": r32["vortex_cells"]},
         "cpu_baseline": cpu,
     }
     if batch:
@@ -383,6 +396,8 @@ def main():
                        "psi_launch_us": r64.get("psi_launch_us"),
                        "sweeps_per_step": {"psi": r64["sweeps_psi"], "A": r64["sweeps_A"]}, "F": r64["F_td"]}
         out["fp32_over_fp64"] = r32["node_steps_per_s"] / r64["node_steps_per_s"]
+        out["vortex_detector"].update({"fp64_ms": r64["vortex_ms"], "fp64_vortices": r64["vortices"],
+                                       "fp64_cells_flagged": r64["vortex_cells"]})
         out["fp32_F_rel_err_vs_fp64"] = abs(r32["F_td"] - r64["F_td"]) / abs(r64["F_td"])
         out["fp32_vs_fp64"] = {
             "after_steps": args.warmup + args.steps + args.profile_steps,

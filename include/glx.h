@@ -116,6 +116,12 @@ typedef struct glx_td_params {
     int32_t solveA;      /* kappa < inf */
     int32_t max_sweeps;  /* Jacobi cap per solve; the reference uses 1024 */
     int32_t eqn;         /* 0: psi then A, 1: psi only, 2: A only */
+    const void *a_src;   /* NULL: the A sweeps take their supercurrent source
+                          * from A^(2*floor(j/2)) (td.py:264-303); else a
+                          * buffer of Nab reals (not one of ab_bufs) that
+                          * every A sweep reads instead: svirl's fixed-vortex
+                          * path binds vpi + A^n there (td.py:256-266).
+                          * Not supported by glx_batch_td_run. */
 } glx_td_params;
 
 /* nsteps TDGL steps in one persistent launch: per step up to max_sweeps psi
@@ -244,6 +250,18 @@ int glx_current_density(int prec, glx_grid g, double kappa2, double H, const voi
 /* r_Du id Im(conj psi0 U psi1) on links (observables.h:159-235). */
 int glx_supercurrent_density(int prec, glx_grid g, const uint8_t *mt, const void *psi,
                              const void *abei, const void *ab, void *jsxjsy);
+/* B (cells), J and J_s (links) in one pass over the nodes: the results of
+ * glx_magnetic_field, glx_current_density and glx_supercurrent_density, to
+ * the bit, with each link value the three share read once (the three
+ * kernels of svirl/cuda/observables.h:4-235 behind observables.py:36-121,
+ * fused). Any of B, J, Js may be NULL to skip it; psi may be NULL when Js
+ * is. No counterpart in the reference: it serves J_n = J - J_s
+ * (normalcurrent_density, observables.py:113-121) and callers that read
+ * all three. */
+int glx_bulk_observables(int prec, glx_grid g, double kappa2, double H, const uint8_t *mt,
+                         const void *psi, const void *abei, const void *ab, void *B, void *J,
+                         void *Js);
+
 /* Total GL free energy, FP64 accumulation, returned on the host
  * (free_energy_pseudodensity, observables.h:250-362, + Reduction.gsum,
  * svirl/parallel/reduction.py:40-110; observables.py:124-149). */
@@ -262,6 +280,18 @@ int glx_free_energy(int prec, glx_grid g, double kappa2, double eps, const void 
  * and only `capacity` records are written). */
 int glx_vortex_cells(int prec, glx_grid g, const void *psi, const void *ab, const void *abi,
                      double H, double *rec, int32_t capacity, int32_t *count);
+
+/* Sub-cell vortex positions of `count` records of glx_vortex_cells (in
+ * device memory, any order), the triangulation loop of
+ * svirl/observables/vortex_detector.py:103-147 with its helpers
+ * find_zero, find_zero_line, find_intersection (:75-101), in FP64 with the
+ * reference's operation order. Writes 4 doubles per record to `xyv`:
+ * (x, y, round(v), keep), keep = 1.0 where the reference appends
+ * [x, y, round(v)] to its list (two zero points on each of the Re and Im
+ * lines, non-parallel lines, point within the cell's 3x3 neighbourhood),
+ * else 0.0. The reference's output is the kept rows in ascending cell id.
+ * Asynchronous on the library stream. */
+int glx_vortex_triangulate(glx_grid g, const double *rec, int32_t count, double *xyv);
 
 /* ------------------------------------------------------------- reductions */
 

@@ -712,12 +712,15 @@ class OracleState:
 
     Fields mirror the reference objects: psi (vars._psi), ab (vars._vp),
     abei (params._vpei; all zeros by default, params.py:36,287-303),
-    eps / eps_map (params._epsilon), kappa (params._kappa), sigma, H, mt.
+    eps / eps_map (params._epsilon), kappa (params._kappa), sigma, H, mt,
+    vpi (fixed_vortices._vpi, the irregular vector potential of pinned
+    vortices, or None) and lock_ns (fixed_vortices._phase_lock_ns or None).
     """
 
     def __init__(self, g, psi, ab, kappa=np.inf, sigma=1.0, H=0.0, eps=1.0,
                  eps_map=None, mt=None, abei=None, stop_psi=1e-6, stop_A=1e-6,
-                 langevin_psi=0.0, langevin_A=0.0, random_t=1, convergence_rtol=1e-6):
+                 langevin_psi=0.0, langevin_A=0.0, random_t=1, convergence_rtol=1e-6,
+                 vpi=None, lock_ns=None):
         self.g = g
         self.psi = np.array(psi, dtype=np.complex128)
         self.ab = np.array(ab, dtype=np.float64)
@@ -736,6 +739,8 @@ class OracleState:
         self.langevin_A = float(langevin_A)
         self.random_t = int(random_t) & 0xffffffff
         self.convergence_rtol = convergence_rtol
+        self.vpi = None if vpi is None else np.array(vpi, dtype=np.float64)
+        self.lock_ns = None if lock_ns is None else np.array(lock_ns, dtype=np.int64)
         # TD buffers: the reference's __gab_next persists between steps
         self._ab_next = np.zeros(g.Nab)
         self.sweeps = []  # (S_psi, S_A) per step
@@ -752,8 +757,22 @@ class OracleState:
         return self.kappa ** 2 if self.solveA else -1.0
 
     # -- TDGL ---------------------------------------------------------------
+    def ab_preprocess(self):
+        """td.py:118-140: vp += vpi over the first N entries only (xpy_r
+        with N = number of nodes, utils.h:117-124), before a psi solve."""
+        if self.vpi is not None:
+            N = self.g.N
+            self.ab[:N] = self.ab[:N] + self.vpi[:N]
+
+    def ab_postprocess(self):
+        """td.py:143-154: vp -= vpi over the first N entries (xmy_r)."""
+        if self.vpi is not None:
+            N = self.g.N
+            self.ab[:N] = self.ab[:N] - self.vpi[:N]
+
     def td_psi_solve(self, dt):
-        """td.py:157-218 (no fixed vortices / phase lock)."""
+        """td.py:157-218: the Jacobi loop, then the phase lock of the
+        listed nodes (td.py:205-216, td.h:376-388)."""
         g = self.g
         psi_rhs = self.psi.copy()
         cur = self.psi
@@ -768,6 +787,8 @@ class OracleState:
                 break
         self.random_t = (self.random_t + 1) & 0xffffffff
         self.psi = cur
+        if self.lock_ns is not None:
+            self.psi = phase_lock(self.psi, self.lock_ns)
         return nsw
 
     def td_A_solve(self, dt):
@@ -778,6 +799,12 @@ class OracleState:
         X = self.ab            # buffer object bound as abi_ab_rhs
         Y = self._ab_next
         src = X
+        if self.vpi is not None:
+            # td.py:256-266: vpi += vp over the first N entries; vpi is then
+            # the source of every sweep
+            N = g.N
+            self.vpi[:N] = self.vpi[:N] + X[:N]
+            src = self.vpi
         rhs = X.copy()
         cur, nxt = X, Y
         nsw = 0
@@ -793,14 +820,36 @@ class OracleState:
                 break
         self.random_t = (self.random_t + 1) & 0xffffffff
         self.ab, self._ab_next = cur, nxt
+        if self.vpi is not None:
+            # td.py:318-325: vpi -= vp (now A^{n+1}) over the first N entries
+            N = g.N
+            self.vpi[:N] = self.vpi[:N] - self.ab[:N]
         return nsw
 
     def td(self, dt=0.1, Nt=1000):
-        """td.py:342-366: per step, the psi solve then (if kappa < inf) A."""
+        """td.py:342-366: per step, the psi solve between the vpi pre- and
+        post-processing, then (if kappa < inf) the A solve."""
         for _ in range(int(Nt)):
+            self.ab_preprocess()
             s_psi = self.td_psi_solve(dt)
+            self.ab_postprocess()
             s_A = self.td_A_solve(dt) if self.solveA else 0
             self.sweeps.append((s_psi, s_A))
+
+    def td_order_parameter(self, dt=0.1, Nt=1000):
+        """td.py:221-231 (eqn='order_parameter'): the vpi preprocessing
+        once before the loop, the postprocessing after every step."""
+        self.ab_preprocess()
+        for _ in range(int(Nt)):
+            self.sweeps.append((self.td_psi_solve(dt), 0))
+            self.ab_postprocess()
+
+    def td_vector_potential(self, dt=0.1, Nt=1000):
+        """td.py:328-339 (eqn='vector_potential'): A solves only."""
+        if not self.solveA:
+            return
+        for _ in range(int(Nt)):
+            self.sweeps.append((0, self.td_A_solve(dt)))
 
     # -- observables ----------------------------------------------------------
     @property

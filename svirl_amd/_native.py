@@ -36,7 +36,7 @@ class TDParams(ctypes.Structure):
     _fields_ = [
         ("dt", c_double), ("eps", c_double), ("kappa2", c_double), ("rho", c_double), ("H", c_double),
         ("stop_psi", c_double), ("stop_A", c_double), ("langevin_psi", c_double), ("langevin_A", c_double),
-        ("rand_t", c_u32), ("solveA", c_i32), ("max_sweeps", c_i32), ("eqn", c_i32),
+        ("rand_t", c_u32), ("solveA", c_i32), ("max_sweeps", c_i32), ("eqn", c_i32), ("a_src", c_void_p),
     ]
 
 
@@ -91,6 +91,9 @@ _PROTOS = {
     "glx_free_energy": [c_int, Grid, c_double, c_double, c_void_p, c_double, c_void_p, c_void_p, c_void_p,
                         c_void_p, P_double],
     "glx_vortex_cells": [c_int, Grid, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_i32, P_i32],
+    "glx_vortex_triangulate": [Grid, c_void_p, c_i32, c_void_p],
+    "glx_bulk_observables": [c_int, Grid, c_double, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                             c_void_p, c_void_p],
     "glx_sum": [c_int, c_void_p, c_i64, c_i32, P_double],
     "glx_sum_v": [c_int, c_void_p, c_i64, c_i32, c_i32, P_double],
     "glx_pr_sums": [c_int, c_void_p, c_void_p, c_i64, P_double],

@@ -79,6 +79,8 @@ class BatchTD:
         arr = (BatchInstance * n)()
         bufs = []
         for k, td in enumerate(tds):
+            if td._has_fixed_vortices():
+                raise NotImplementedError("fixed vortices / phase lock are not supported in batched runs")
             g, prm, P, pc, Q, qc = td._prepare(0, td.dt)
             if prm.langevin_psi > 1e-32 or prm.langevin_A > 1e-32:
                 raise NotImplementedError("Langevin noise is not supported in batched runs")

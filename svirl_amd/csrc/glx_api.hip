@@ -312,6 +312,11 @@ int glx_supercurrent_density(int prec, glx_grid g, const uint8_t *mt, const void
     GLX_DISPATCH(prec, supercurrent_density(g_ctx.env, g, mt, psi, abei, ab, jsxjsy));
 }
 
+int glx_bulk_observables(int prec, glx_grid g, double kappa2, double H, const uint8_t *mt, const void *psi,
+                         const void *abei, const void *ab, void *B, void *J, void *Js) {
+    GLX_DISPATCH(prec, bulk_observables(g_ctx.env, g, kappa2, H, mt, psi, abei, ab, B, J, Js));
+}
+
 int glx_free_energy(int prec, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
                     const uint8_t *mt, const void *psi, const void *abei, const void *ab, double *F) {
     GLX_DISPATCH(prec, free_energy(g_ctx.env, g, kappa2, eps, eps_map, H, mt, psi, abei, ab, F));
@@ -320,6 +325,11 @@ int glx_free_energy(int prec, glx_grid g, double kappa2, double eps, const void 
 int glx_vortex_cells(int prec, glx_grid g, const void *psi, const void *ab, const void *abi, double H, double *rec,
                      int32_t capacity, int32_t *count) {
     GLX_DISPATCH(prec, vortex_cells(g_ctx.env, g, psi, ab, abi, H, rec, capacity, count));
+}
+
+int glx_vortex_triangulate(glx_grid g, const double *rec, int32_t count, double *xyv) {
+    GLX_REQUIRE_INIT();
+    return L<double>::vortex_triangulate(g_ctx.env, g, rec, count, xyv);
 }
 
 int glx_sum(int prec, const void *x, int64_t n, int32_t block_size, double *out) {

@@ -102,19 +102,25 @@ template <typename R> __device__ __forceinline__ R g_grad(cplx<R> p0, R ph, cplx
 }
 
 // ---------------------------------------------------------------- B field
+// B of cell (i, j) from abei + ab (observables.h:4-37).
+template <typename R>
+__device__ __forceinline__ R cell_field(const Geo &g, const GeoR<R> &q, const R *abei, const R *ab, int i, int j) {
+    R Bn = R(0);
+    for (int s = 0; s < 2; ++s) {
+        const R *v = s == 0 ? abei : ab;
+        if (!v) continue;
+        Bn += q.idx * (v[g.Na + i + 1 + (long)g.Nxb * j] - v[g.Na + i + (long)g.Nxb * j]) -
+              q.idy * (v[i + (long)g.Nxa * (j + 1)] - v[i + (long)g.Nxa * j]);
+    }
+    return Bn;
+}
+
 template <typename R>
 __global__ void __launch_bounds__(256) k_magnetic_field(Geo g, const R *abei, const R *ab, R *B) {
     const GeoR<R> q(g);
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.Nc; n += (long)gridDim.x * blockDim.x) {
         const int i = (int)(n % g.Nxc), j = (int)(n / g.Nxc);
-        R Bn = R(0);
-        for (int s = 0; s < 2; ++s) {
-            const R *v = s == 0 ? abei : ab;
-            if (!v) continue;
-            Bn += q.idx * (v[g.Na + i + 1 + (long)g.Nxb * j] - v[g.Na + i + (long)g.Nxb * j]) -
-                  q.idy * (v[i + (long)g.Nxa * (j + 1)] - v[i + (long)g.Nxa * j]);
-        }
-        B[n] = Bn;
+        B[n] = cell_field(g, q, abei, ab, i, j);
     }
 }
 
@@ -179,32 +185,62 @@ __global__ void __launch_bounds__(256) k_current_density(Geo g, R kappa2, R H, c
     }
 }
 
+// J_s on the a link (i, j) and the b link (i, j) of node n = (i, j)
+// (observables.h:159-235), written where the link exists.
+template <typename R>
+__device__ __forceinline__ void supercurrent_links(const Geo &g, const GeoR<R> &q, const uint8_t *mt,
+                                                   const cplx<R> *psi, const R *abei, const R *ab, long n, int i,
+                                                   int j, R *Js) {
+    const cplx<R> p0 = ld(psi, n);
+    const Cells c = cells_of(g, mt, i, j);
+    if (i < g.Nxa && j < g.Nya) {
+        const long m = i + (long)g.Nxa * j;
+        R jl = R(0);
+        if (c.pm || c.pp) {
+            const R r = R(0.5) * ((c.pm ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
+            jl = r * q.idx * js(p0, q.dx * pv(abei, ab, m), ld(psi, n + 1));
+        }
+        Js[m] = jl;
+    }
+    if (i < g.Nxb && j < g.Nyb) {
+        const long m = g.Na + i + (long)g.Nxb * j;
+        R jl = R(0);
+        if (c.mp || c.pp) {
+            const R r = R(0.5) * ((c.mp ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
+            jl = r * q.idy * js(p0, q.dy * pv(abei, ab, m), ld(psi, n + g.Nx));
+        }
+        Js[m] = jl;
+    }
+}
+
 template <typename R>
 __global__ void __launch_bounds__(256) k_supercurrent_density(Geo g, const uint8_t *mt, const cplx<R> *psi, const R *abei, const R *ab, R *Js) {
     const GeoR<R> q(g);
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
         int i, j;
         node_ij(g, n, i, j);
-        const cplx<R> p0 = ld(psi, n);
-        const Cells c = cells_of(g, mt, i, j);
-        if (i < g.Nxa && j < g.Nya) {
-            const long m = i + (long)g.Nxa * j;
-            R jl = R(0);
-            if (c.pm || c.pp) {
-                const R r = R(0.5) * ((c.pm ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
-                jl = r * q.idx * js(p0, q.dx * pv(abei, ab, m), ld(psi, n + 1));
-            }
-            Js[m] = jl;
+        supercurrent_links(g, q, mt, psi, abei, ab, n, i, j, Js);
+    }
+}
+
+// B, J and J_s in one pass over the nodes: node (i, j) writes cell (i, j)
+// and its a and b links, so the link values a node's three results share
+// are read once. Any of B, J, Js may be NULL (not computed). The values
+// are those of k_magnetic_field, k_current_density and
+// k_supercurrent_density, to the bit.
+template <typename R>
+__global__ void __launch_bounds__(256) k_bulk_observables(Geo g, R kappa2, R H, const uint8_t *mt, const cplx<R> *psi,
+                                                          const R *abei, const R *ab, R *B, R *J, R *Js) {
+    const GeoR<R> q(g);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
+        int i, j;
+        node_ij(g, n, i, j);
+        if (B && i < g.Nxc && j < g.Nyc) B[i + (long)g.Nxc * j] = cell_field(g, q, abei, ab, i, j);
+        if (J) {
+            if (i < g.Nxa && j < g.Nya) J[i + (long)g.Nxa * j] = kappa2 * curlcurl_a(g, q, H, abei, ab, i, j);
+            if (i < g.Nxb && j < g.Nyb) J[g.Na + i + (long)g.Nxb * j] = kappa2 * curlcurl_b(g, q, H, abei, ab, i, j);
         }
-        if (i < g.Nxb && j < g.Nyb) {
-            const long m = g.Na + i + (long)g.Nxb * j;
-            R jl = R(0);
-            if (c.mp || c.pp) {
-                const R r = R(0.5) * ((c.mp ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
-                jl = r * q.idy * js(p0, q.dy * pv(abei, ab, m), ld(psi, n + g.Nx));
-            }
-            Js[m] = jl;
-        }
+        if (Js) supercurrent_links(g, q, mt, psi, abei, ab, n, i, j, Js);
     }
 }
 
@@ -1052,6 +1088,86 @@ __global__ void __launch_bounds__(256) k_vortex_cells(Geo g, const cplx<R> *psi,
             }
         }
     }
+}
+
+// Sub-cell vortex position in each flagged cell, one thread per record of
+// k_vortex_cells (vortex_detector.py:103-147, with find_zero,
+// find_zero_line and find_intersection of :75-101): the corner values are
+// turned into a common gauge, the zero lines of Re and Im are taken where
+// the linearised field changes sign along the cell border, and the vortex
+// is their intersection. Writes (x, y, round(v), keep) per record; keep is
+// 1 when both zero lines have exactly two points, the lines are not
+// parallel (angle folded into [0, pi/2) above 1e-10) and the point lies in
+// the cell's 3x3 neighbourhood, as the reference's tests. FP64 throughout,
+// with the reference's operation order.
+// find_zero(x2, y2, f2, x1, y1, f1) when f changes sign from point 1 to 2
+// (also the closing edge: find_zero(x1, y1, f1, x4, y4, f4) is this with
+// point 4 first); the first two points are kept, k counts them all.
+__device__ __forceinline__ void zero_on_border(double x1, double y1, double f1, double x2, double y2, double f2,
+                                               int &k, double (&px)[2], double (&py)[2]) {
+    if (f2 * f1 < -1e-10) {
+        if (k < 2) {
+            px[k] = (f1 * x2 - x1 * f2) / (f1 - f2);
+            py[k] = (f1 * y2 - y1 * f2) / (f1 - f2);
+        }
+        ++k;
+    }
+}
+
+// psi * exp(-i th) with NumPy's complex product and exp (exp(0) = 1 exactly).
+__device__ __forceinline__ void rotate_corner(double re, double im, double th, double &ore, double &oim) {
+    double s, c;
+    sincos(th, &s, &c);
+    const double d = -s;
+    ore = re * c - im * d;
+    oim = re * d + im * c;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_vortex_triangulate(Geo g, const double *rec, int count, double *xyv) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const double *r = rec + (long)k * VREC;
+    const long n = (long)r[0];
+    const long i = n % g.Nxc, j = n / g.Nxc;
+    const double x = g.dx * double(i), y = g.dy * double(j);
+    const double inta_00 = g.dx * r[10], inta_0p = g.dx * r[11];
+    const double intb_00 = g.dy * r[12], intb_p0 = g.dy * r[13];
+    // corners 00, p0, pp, 0p in the gauge of corner 00
+    double re[4], im[4];
+    re[0] = r[2];
+    im[0] = r[3];
+    rotate_corner(r[4], r[5], 0.75 * inta_00 + 0.25 * (intb_00 + inta_0p - intb_p0), re[1], im[1]);
+    rotate_corner(r[6], r[7], 0.5 * (inta_00 + intb_p0) + 0.5 * (intb_00 + inta_0p), re[2], im[2]);
+    rotate_corner(r[8], r[9], 0.75 * intb_00 + 0.25 * (inta_00 + intb_p0 - inta_0p), re[3], im[3]);
+    const double cx[4] = {x, x + g.dx, x + g.dx, x}, cy[4] = {y, y, y + g.dy, y + g.dy};
+    double rx[2] = {0.0, 0.0}, ry[2] = {0.0, 0.0}, qx[2] = {0.0, 0.0}, qy[2] = {0.0, 0.0};
+    int nr = 0, nq = 0;
+    // edges 1-2, 2-3, 3-4, 4-1 of the border, in the reference's order
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int a = e, b = (e + 1) & 3;
+        zero_on_border(cx[a], cy[a], re[a], cx[b], cy[b], re[b], nr, rx, ry);
+        zero_on_border(cx[a], cy[a], im[a], cx[b], cy[b], im[b], nq, qx, qy);
+    }
+    double ix = 0.0, iy = 0.0, keep = 0.0;
+    if (nr == 2 && nq == 2) {
+        const double l1x1 = rx[0], l1y1 = ry[0], l1x2 = rx[1], l1y2 = ry[1];
+        const double l2x1 = qx[0], l2y1 = qy[0], l2x2 = qx[1], l2y2 = qy[1];
+        const double D = (l1x1 - l1x2) * (l2y1 - l2y2) - (l1y1 - l1y2) * (l2x1 - l2x2);
+        double ph = atan2(D, (l1y1 - l1y2) * (l2y1 - l2y2) - (l1x1 - l1x2) * (l2x1 - l2x2));
+        ph = fmod(fabs(ph), 1.5707963267948966);  // np.mod(|ph|, 0.5*np.pi)
+        if (fabs(ph) > 1e-10) {
+            ix = ((l1x1 * l1y2 - l1y1 * l1x2) * (l2x1 - l2x2) - (l1x1 - l1x2) * (l2x1 * l2y2 - l2y1 * l2x2)) / D;
+            iy = ((l1x1 * l1y2 - l1y1 * l1x2) * (l2y1 - l2y2) - (l1y1 - l1y2) * (l2x1 * l2y2 - l2y1 * l2x2)) / D;
+            if (x - g.dx < ix && ix < x + 2.0 * g.dx && y - g.dy < iy && iy < y + 2.0 * g.dy) keep = 1.0;
+        }
+    }
+    double *o = xyv + 4L * k;
+    o[0] = ix;
+    o[1] = iy;
+    o[2] = rint(r[1]);
+    o[3] = keep;
 }
 
 template <typename R>

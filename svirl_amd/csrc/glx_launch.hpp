@@ -55,10 +55,13 @@ template <typename R> struct L {
                                void *J);
     static int supercurrent_density(const Env &e, glx_grid g, const uint8_t *mt, const void *psi, const void *abei,
                                     const void *ab, void *Js);
+    static int bulk_observables(const Env &e, glx_grid g, double kappa2, double H, const uint8_t *mt, const void *psi,
+                                const void *abei, const void *ab, void *B, void *J, void *Js);
     static int free_energy(const Env &e, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
                            const uint8_t *mt, const void *psi, const void *abei, const void *ab, double *F);
     static int vortex_cells(const Env &e, glx_grid g, const void *psi, const void *ab, const void *abi, double H,
                             double *rec, int32_t capacity, int32_t *count);
+    static int vortex_triangulate(const Env &e, glx_grid g, const double *rec, int32_t count, double *xyv);
     static int sum(const Env &e, const void *x, int64_t n, int32_t bs, double *out);
     static int sum_v(const Env &e, const void *x, int64_t nv, int32_t ne, int32_t bs, double *out);
     static int pr_sums(const Env &e, const void *x, const void *y, int64_t n, double *out);

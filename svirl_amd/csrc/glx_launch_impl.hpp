@@ -135,11 +135,13 @@ template <typename R> struct RunForm { static constexpr bool has_gauge = false; 
 template <> struct RunForm<float> { static constexpr bool has_gauge = true; };
 
 template <typename R, bool G> static int solve_occupancy(int *per_cu) {
-    int a = 0, b = 0;
+    int a = 0, b = 0, c = 1 << 20;
     hipError_t st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_td_psi_solve<R, G>, 256, 0);
     if (st == hipSuccess) st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_td_a_solve<R, G>, 256, 0);
+    if constexpr (!G)  // the fixed-source A solve runs in the plain form only
+        if (st == hipSuccess) st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_td_a_solve<R, G, true>, 256, 0);
     if (st != hipSuccess) return check_hip(st, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-    *per_cu = a < b ? a : b;
+    *per_cu = a < b ? (a < c ? a : c) : (b < c ? b : c);
     return GLX_OK;
 }
 
@@ -186,6 +188,7 @@ static int td_run_form(const Env &e, glx_grid g, const glx_td_params *p, const v
     T.rhs_psi = p->langevin_psi > 1.0e-32 ? (cplx<R> *)rhs_psi : nullptr;
     T.rhs_ab = (p->solveA && p->langevin_A > 1.0e-32) ? (R *)rhs_ab : nullptr;
     T.max_sweeps = p->max_sweeps;
+    T.a_src = (const R *)p->a_src;
     T.sc.gs.arrive = e.arrive;
     T.sc.gs.rel = e.rel;
     T.sc.gs.err = e.err;
@@ -218,7 +221,14 @@ static int td_run_form(const Env &e, glx_grid g, const glx_td_params *p, const v
         if (do_A) {
             T.A.rand_t = rt++;
             if (prof) (void)hipEventRecord(e.tev[nev++], e.stream);
-            hipLaunchKernelGGL((k_td_a_solve<R, G>), dim3((unsigned)wg), dim3(256), 0, e.stream, T);
+            bool fixed = false;
+            if constexpr (!G) {
+                if (T.a_src) {
+                    hipLaunchKernelGGL((k_td_a_solve<R, G, true>), dim3((unsigned)wg), dim3(256), 0, e.stream, T);
+                    fixed = true;
+                }
+            }
+            if (!fixed) hipLaunchKernelGGL((k_td_a_solve<R, G>), dim3((unsigned)wg), dim3(256), 0, e.stream, T);
             GLX_LAUNCH_CHECK("k_td_a_solve");
             if (prof) (void)hipEventRecord(e.tev[nev++], e.stream);
         }
@@ -289,8 +299,11 @@ int L<R>::td_run(const Env &e, glx_grid g, const glx_td_params *p, const void *e
     if (wg <= 0) wg = std::min(maxg, 2 * e.num_cus);  // two workgroups per CU: the measured optimum
     if (wg > maxg) return set_error(GLX_E_ARG, "glx_td_run: grid %d exceeds co-resident limit %d", wg, maxg);
     const bool evolves_A = p->solveA && p->eqn != 1;
+    if (p->a_src && (p->a_src == Q[0] || p->a_src == Q[1] || p->a_src == Q[2]))
+        return set_error(GLX_E_ARG, "glx_td_run: a_src must not be one of the rotating A buffers");
+    // a fixed source is given in the plain form, so the run keeps that form
     if constexpr (RunForm<R>::has_gauge)
-        if (evolves_A)
+        if (evolves_A && !p->a_src)
             return td_run_form<R, true>(e, g, p, eps_map, mt, P, pc, Q, qc, rhs_psi, rhs_ab, nsteps, sweeps, wg);
     return td_run_form<R, false>(e, g, p, eps_map, mt, P, pc, Q, qc, rhs_psi, rhs_ab, nsteps, sweeps, wg);
 }
@@ -500,6 +513,8 @@ int L<R>::batch_td_run(const Env &e, glx_batch_instance *inst, int32_t n, int32_
             return set_error(GLX_E_ARG, "glx_batch_td_run: instance %d: invalid parameters", i);
         if (p.langevin_psi > 1.0e-32 || p.langevin_A > 1.0e-32)
             return set_error(GLX_E_ARG, "glx_batch_td_run: instance %d: Langevin noise is not supported in batches", i);
+        if (p.a_src)
+            return set_error(GLX_E_ARG, "glx_batch_td_run: instance %d: a fixed A source is not supported in batches", i);
         if (I.psi_cur < 0 || I.psi_cur > 2 || I.ab_cur < 0 || I.ab_cur > 2)
             return set_error(GLX_E_ARG, "glx_batch_td_run: instance %d: buffer index", i);
         for (int k = 0; k < 3; ++k)
@@ -645,6 +660,18 @@ int L<R>::supercurrent_density(const Env &e, glx_grid g, const uint8_t *mt, cons
 }
 
 template <typename R>
+int L<R>::bulk_observables(const Env &e, glx_grid g, double kappa2, double H, const uint8_t *mt, const void *psi,
+                           const void *abei, const void *ab, void *B, void *J, void *Js) {
+    if (!grid_ok(g) || (Js && !psi)) return set_error(GLX_E_ARG, "glx_bulk_observables: invalid arguments");
+    if (!B && !J && !Js) return GLX_OK;
+    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    hipLaunchKernelGGL(k_bulk_observables<R>, dim3(grid_blocks(G.N)), dim3(256), 0, e.stream, G, R(kappa2), R(H), mt,
+                       (const cplx<R> *)psi, (const R *)abei, (const R *)ab, (R *)B, (R *)J, (R *)Js);
+    GLX_LAUNCH_CHECK("k_bulk_observables");
+    return GLX_OK;
+}
+
+template <typename R>
 int L<R>::free_energy(const Env &e, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
                       const uint8_t *mt, const void *psi, const void *abei, const void *ab, double *F) {
     if (!grid_ok(g) || !psi || !F) return set_error(GLX_E_ARG, "glx_free_energy: invalid arguments");
@@ -678,6 +705,18 @@ int L<R>::vortex_cells(const Env &e, glx_grid g, const void *psi, const void *ab
     if (st != hipSuccess) return check_hip(st, "glx_vortex_cells readback");
     *count = c;
     if (c > capacity) return set_error(GLX_E_CAPACITY, "glx_vortex_cells: %d cells flagged, capacity %d", c, capacity);
+    return GLX_OK;
+}
+
+template <typename R>
+int L<R>::vortex_triangulate(const Env &e, glx_grid g, const double *rec, int32_t count, double *xyv) {
+    if (!grid_ok(g) || count < 0 || (count > 0 && (!rec || !xyv)))
+        return set_error(GLX_E_ARG, "glx_vortex_triangulate: invalid arguments");
+    if (count == 0) return GLX_OK;
+    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    hipLaunchKernelGGL(k_vortex_triangulate<R>, dim3((count + 255) / 256), dim3(256), 0, e.stream, G, rec, (int)count,
+                       xyv);
+    GLX_LAUNCH_CHECK("k_vortex_triangulate");
     return GLX_OK;
 }
 

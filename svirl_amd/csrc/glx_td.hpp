@@ -1226,6 +1226,7 @@ template <typename R> struct TDSolveArgs {
     int step;               // index into sweeps[]
     int solveA;             // batched runs: the instance has an A solve
     TDScratch sc;
+    const R *a_src;         // fixed supercurrent source of every A sweep (k_td_a_solve<R, G, true>)
 };
 
 // P[k] for a run-time k without indexing the kernel-argument array (which
@@ -1336,10 +1337,13 @@ __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const Gri
 // stores A^2, the source of sweep 3). Single sweeps (a_rows) are used at odd
 // j, at the last sweep allowed, and at j = pred - 1. As in psi_solve, a pass
 // whose earlier sweep met the stop rule is redone up to that sweep.
-template <typename R, bool GAUGE, bool SH = false>
+// FIXED: every sweep takes its source from T.a_src (svirl's fixed-vortex
+// path, td.py:256-266, binds vpi + A^n there); pairs may then start at any
+// j, and the three-sweep pass (which makes its own A^2 source) is not used.
+template <typename R, bool GAUGE, bool SH = false, bool FIXED = false>
 __device__ __forceinline__ SolveOut a_solve(const TDSolveArgs<R> &T, const GridSync &gs, const Team &tm, long w,
                                             long W, int pc, int qc, int pred, unsigned long long &epoch, int *lds) {
-    constexpr bool TRIPLE = !SH;
+    constexpr bool TRIPLE = !SH && !FIXED;
     const int o1 = (qc + 1) % 3, o2 = (qc + 2) % 3;
     ASweepArgs<R> A = T.A;
     A.psi = pick3(T.P, pc);
@@ -1367,10 +1371,11 @@ __device__ __forceinline__ SolveOut a_solve(const TDSolveArgs<R> &T, const GridS
         // source A^(2*floor(j/2)): the input on even sweeps, A^n on sweep 1,
         // the output buffer's previous contents on odd j >= 3 (A^(j-1), left
         // there by the single sweep j - 1 or by the three-sweep pass)
-        A.src = pick3(T.Q, (j % 2 == 0) ? in : (j == 1 ? qc : out));
+        if constexpr (FIXED) A.src = T.a_src;
+        else A.src = pick3(T.Q, (j % 2 == 0) ? in : (j == 1 ? qc : out));
         A.rhs = (T.rhs_ab && j > 0) ? T.rhs_ab : pick3(T.Q, qc);
         int k = 2;
-        if (j % 2 == 1) k = 1;
+        if (!FIXED && j % 2 == 1) k = 1;
         else if (TRIPLE && j == 0 && (pred & 1) && pred >= 3 && T.max_sweeps >= 3) k = 3;
         else if (j + 1 == pred || j + 2 > T.max_sweeps) k = 1;
         R r1 = R(0), r2 = R(0), r3 = R(0);
@@ -1421,7 +1426,7 @@ __global__ void __launch_bounds__(256) k_td_psi_solve(TDSolveArgs<R> T) {
     }
 }
 
-template <typename R, bool GAUGE>
+template <typename R, bool GAUGE, bool FIXED = false>
 __global__ void __launch_bounds__(256) k_td_a_solve(TDSolveArgs<R> T) {
     __shared__ int lds[16];
     const long W = (long)gridDim.x * (blockDim.x / WAVE);
@@ -1429,7 +1434,8 @@ __global__ void __launch_bounds__(256) k_td_a_solve(TDSolveArgs<R> T) {
     int *st = T.sc.state;
     unsigned long long epoch = *state64(st, ST_EPOCH);
     const Team tm{0, (int)gridDim.x, NREL, T.sc.gs.rel};
-    const SolveOut o = a_solve<R, GAUGE>(T, T.sc.gs, tm, w, W, st[ST_PSI], st[ST_AB], st[ST_PRED_A], epoch, lds);
+    const SolveOut o =
+        a_solve<R, GAUGE, false, FIXED>(T, T.sc.gs, tm, w, W, st[ST_PSI], st[ST_AB], st[ST_PRED_A], epoch, lds);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (!o.ok) {
             st[ST_ERRSTEP] = T.step + 1;

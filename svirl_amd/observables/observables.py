@@ -14,6 +14,7 @@ class Observables:
         self.vars = vars
         self.params = params
         self.cfg = par.cfg
+        self._edge2 = None  # second link-sized buffer for the fused pass
 
     def _grid(self):
         c = self.cfg
@@ -76,9 +77,39 @@ class Observables:
 
     @property
     def normalcurrent_density(self):
-        jx, jy = self.current_density
-        jsx, jsy = self.supercurrent_density
+        """J - J_s on links (observables.py:113-121); with A solved, J and
+        J_s come from one fused pass (glx_bulk_observables)."""
+        if not self.params.solveA:
+            jx, jy = self.current_density
+            jsx, jsy = self.supercurrent_density
+            return jx - jsx, jy - jsy
+        _, (jx, jy), (jsx, jsy) = self._bulk(False)
         return jx - jsx, jy - jsy
+
+    def bulk_observables(self):
+        """(B, (Jx, Jy), (Jsx, Jsy)) as magnetic_field, current_density and
+        supercurrent_density return them, from one pass over the nodes."""
+        return self._bulk(True)
+
+    def _bulk(self, want_B):
+        self._sync_device()
+        c = self.cfg
+        p = self.params
+        if self._edge2 is None:
+            self._edge2 = nat.DeviceArray(int(c.Na + c.Nb), c.dtype)
+        B = self.vars._tmp_cell_var_h() if want_B else None
+        J = self.vars._tmp_edge_var_h() if p.solveA else None
+        Js = self._edge2
+        nat.call("glx_bulk_observables", self.par.prec, self._grid(),
+                 float(p.gl_parameter_squared_h()) if p.solveA else 0.0, float(p.homogeneous_external_field),
+                 nat.ptr(self.mesh.material_tiling_h()), self.vars.order_parameter_h().ptr,
+                 nat.ptr(p.external_irregular_vector_potential_h()), nat.ptr(self.vars.vector_potential_h()),
+                 nat.ptr(B), nat.ptr(J), Js.ptr)
+        Bh = np.reshape(B.get(), (c.Nyc, c.Nxc)).T.copy() if want_B else None
+        js = self._edge_pair(Js)
+        # J = J_s when kappa = inf (observables.py:86-110)
+        jj = self._edge_pair(J) if J is not None else (js[0].copy(), js[1].copy())
+        return Bh, jj, js
 
     @property
     def free_energy(self):

@@ -63,10 +63,6 @@ class CG:
                     mt=nat.ptr(self.mesh.material_tiling_h()), abei=nat.ptr(p.external_irregular_vector_potential_h()),
                     ab=nat.ptr(self.vars.vector_potential_h()), psi=self.vars.order_parameter_h().ptr)
 
-    def _check_supported(self):
-        if self.fixed_vortices._vpi is not None or self.fixed_vortices._phase_lock_ns is not None:
-            raise NotImplementedError("fixed vortices / phase lock are not supported by the GPU CG path")
-
     # -- kernels exposed like the reference (used by its tests) -----------------------
     @property
     def _free_energy_jacobian_psi(self):
@@ -266,7 +262,9 @@ class CG:
         self.__loop(n_iter, True)
 
     def _solve(self, n_iter=1000):
-        self._check_supported()
+        """Fixed vortices enter the CG only through the external irregular
+        vector potential (params._vpei = external A + vpi, params.py:193-221)
+        and the phase lock not at all, as in the reference's cg.py."""
         if self.params.solveA:
             self.__minimization(n_iter)
         else:

@@ -5,7 +5,8 @@ Each launch advances several time steps with every Jacobi sweep and its
 stop test on the device (glx_td_run). The state buffers rotate among three
 device arrays per field instead of the reference's right-hand-side copy and
 pointer swap; vars._psi / vars._vp always point at the one holding the
-current state.
+current state. With fixed vortices or a phase lock the reference's per-step
+hooks run between one-step launches (TD._run_fixed).
 """
 import ctypes
 
@@ -118,10 +119,11 @@ class TD:
         return int(max(1, min(1000, 2.0e8 // max(1, int(self.cfg.N)))))
 
     # -- the time loop ------------------------------------------------------------------
+    def _has_fixed_vortices(self):
+        return self.fixed_vortices._vpi is not None or self.fixed_vortices._phase_lock_ns is not None
+
     def _prepare(self, eqn_code, dt):
         """Buffers and glx_td_params of a run (shared with batched runs)."""
-        if self.fixed_vortices._vpi is not None or self.fixed_vortices._phase_lock_ns is not None:
-            raise NotImplementedError("fixed vortices / phase lock are not supported by the GPU TD path")
         cfg = self.cfg
         v = self.vars
         v._psi.sync_d()
@@ -166,10 +168,11 @@ class TD:
         v._vp.need_dtoh_sync()
 
     # -- the time loop ------------------------------------------------------------------
-    def _run(self, nsteps, eqn_code, dt):
+    def _run(self, nsteps, eqn_code, dt, a_src=None):
         if nsteps <= 0:
             return
         g, prm, P, pc, Q, qc = self._prepare(eqn_code, dt)
+        prm.a_src = nat.ptr(a_src)
         p = self.params
         Pp = (ctypes.c_void_p * 3)(*[b.ptr for b in P])
         Qp = (ctypes.c_void_p * 3)(*[b.ptr for b in Q])
@@ -187,6 +190,84 @@ class TD:
             self._finish(P, cpc.value, Q, cqc.value, np.frombuffer(sw, dtype=np.int32), eqn_code)
             done += n
 
+    def _lock_ns_device(self):
+        ns = self.fixed_vortices._phase_lock_ns
+        if ns is None:
+            return None
+        cached = getattr(self, '_lock_cache', None)
+        if cached is None or cached[0].shape != ns.shape or not np.array_equal(cached[0], ns):
+            cached = (np.array(ns, copy=True), DeviceArray.from_host(ns, np.int32))
+            self._lock_cache = cached
+        return cached[1]
+
+    def _run_fixed(self, nsteps, eqn_code, dt):
+        """Fixed vortices and phase lock, with the reference's hooks around
+        one-step launches: vp += vpi before and vp -= vpi after each psi
+        solve (td.py:118-154; with eqn='order_parameter' the addition is
+        made once before the loop, td.py:221-231), |psi| at the locked nodes
+        after it (td.py:205-216), vpi += vp before and vpi -= vp after each
+        A solve (td.py:256-266, 318-325), all over the first N entries as
+        in the reference. The A solve reads its source from vpi
+        (glx_td_params.a_src) when vpi is set."""
+        fv = self.fixed_vortices
+        vpi = fv.irregular_vector_potential_h()
+        ns = self._lock_ns_device()
+        N = int(self.cfg.N)
+        prec = self.par.prec
+        v = self.vars
+
+        def xpy(x, y):
+            nat.call("glx_axpy", prec, x.ptr, y.ptr, x.ptr, 1.0, N)
+
+        def xmy(x, y):
+            nat.call("glx_axmy", prec, x.ptr, y.ptr, x.ptr, 1.0, N)
+
+        def psi_step():
+            self._run(1, 1, dt)
+            if ns is not None:
+                nat.call("glx_td_phase_lock", prec, v._psi.get_d_obj().ptr, int(ns.size), ns.ptr)
+            return self.sweeps.pop()[0]
+
+        def A_step():
+            if vpi is not None:
+                xpy(vpi, v._vp.get_d_obj())
+            self._run(1, 2, dt, a_src=vpi)
+            if vpi is not None:
+                xmy(vpi, v._vp.get_d_obj())
+            return self.sweeps.pop()[1]
+
+        v._psi.sync_d()
+        v._vp.sync_d()
+        if vpi is not None:
+            fv._vpi.sync_d()
+        if eqn_code == 1:
+            if vpi is not None:
+                xpy(v._vp.get_d_obj(), vpi)
+            for _ in range(nsteps):
+                self.sweeps.append((psi_step(), 0))
+                if vpi is not None:
+                    xmy(v._vp.get_d_obj(), vpi)
+        elif eqn_code == 2:
+            for _ in range(nsteps):
+                self.sweeps.append((0, A_step()))
+        else:
+            for _ in range(nsteps):
+                if vpi is not None:
+                    xpy(v._vp.get_d_obj(), vpi)
+                sp = psi_step()
+                if vpi is not None:
+                    xmy(v._vp.get_d_obj(), vpi)
+                self.sweeps.append((sp, A_step() if self.solveA else 0))
+        if vpi is not None:
+            fv._vpi.need_dtoh_sync()
+        v._vp.need_dtoh_sync()
+
+    def _steps(self, nsteps, eqn_code, dt):
+        if self._has_fixed_vortices():
+            self._run_fixed(nsteps, eqn_code, dt)
+        else:
+            self._run(nsteps, eqn_code, dt)
+
     def _solve(self, dt=None, Nt=None, T=None, eqn=None):
         """td.py:342-377: eqn None (psi then A each step), 'order_parameter'
         or 'vector_potential'. The A update uses the psi time step dt
@@ -194,7 +275,7 @@ class TD:
         if eqn == "order_parameter":
             self._set_iterator_options('order_parameter', dt=dt, Nt=Nt, T=T)
             self._stability_warnings(psi=True, A=False)
-            self._run(int(self.Nt), 1, self.dt)
+            self._steps(int(self.Nt), 1, self.dt)
         elif eqn == "vector_potential":
             if not self.solveA:
                 return
@@ -202,10 +283,10 @@ class TD:
             self._stability_warnings(psi=False, A=True)
             if self.dt is None:
                 raise TypeError("the vector-potential update uses the order-parameter time step, which is not set yet")
-            self._run(int(self.NtA), 2, self.dt)
+            self._steps(int(self.NtA), 2, self.dt)
         else:
             self._set_iterator_options('order_parameter', dt=dt, Nt=Nt, T=T)
             self._set_iterator_options('vector_potential', dt=dt, Nt=Nt, T=T)
             self._stability_warnings()
             self.td_energies = []
-            self._run(int(self.Nt), 0, self.dt)
+            self._steps(int(self.Nt), 0, self.dt)

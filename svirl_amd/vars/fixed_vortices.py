@@ -1,10 +1,11 @@
 """Vortex seeding and fixed (pinned) vortices (svirl/vars/fixed_vortices.py).
 
-Seeding phase windings into psi is host work and fully supported. Fixed
-vortices (an irregular vector potential and phase-locked nodes) are built on
-the host like the reference; the GPU TD and CG paths reject them
-(NotImplementedError) instead of reproducing the reference's partial-length
-vpi updates."""
+Seeding phase windings into psi is host work. Fixed vortices (an irregular
+vector potential vpi and phase-locked nodes) are built on the host like the
+reference; the TD solver applies them with the reference's per-step hooks,
+including their partial-length (first N entries) vpi updates, and the CG
+sees them through the external irregular vector potential. Batched runs
+reject them."""
 import numpy as np
 
 from svirl_amd.storage import GArray

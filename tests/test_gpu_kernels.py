@@ -263,6 +263,40 @@ def test_observables(prec, shape):
 
 
 @pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("shape", [(5, 4), (64, 9), (130, 33)])
+@pytest.mark.parametrize("mask,abei", [(True, True), (False, False)])
+def test_bulk_observables_equal_separate_kernels(prec, shape, mask, abei):
+    """The fused B, J, J_s pass gives the three separate kernels' values to
+    the bit (and so the oracle's, by test_observables), with any output
+    left out."""
+    rs, cs = _types(prec)
+    rng, g, mt, psi, ab, em = case(17 + shape[0], *shape, mask=mask)
+    ae = (0.1 * rng.normal(size=g.Nab)).astype(rs) if abei else None
+    d_psi, d_ab = D(psi.astype(cs), cs), D(ab.astype(rs), rs)
+    d_ae = D(ae, rs) if abei else None
+    d_mt = D(mt.astype(np.uint8), np.uint8) if mask else None
+    k2, H = 16.0, 0.2
+    B0, J0, S0 = (nat.DeviceArray.zeros(g.Nc, rs), nat.DeviceArray.zeros(g.Nab, rs),
+                  nat.DeviceArray.zeros(g.Nab, rs))
+    nat.call("glx_magnetic_field", prec, grid(g), nat.ptr(d_ae), d_ab.ptr, B0.ptr)
+    nat.call("glx_current_density", prec, grid(g), k2, H, nat.ptr(d_ae), d_ab.ptr, J0.ptr)
+    nat.call("glx_supercurrent_density", prec, grid(g), nat.ptr(d_mt), d_psi.ptr, nat.ptr(d_ae), d_ab.ptr, S0.ptr)
+    B1, J1, S1 = (nat.DeviceArray.zeros(g.Nc, rs), nat.DeviceArray.zeros(g.Nab, rs),
+                  nat.DeviceArray.zeros(g.Nab, rs))
+    nat.call("glx_bulk_observables", prec, grid(g), k2, H, nat.ptr(d_mt), d_psi.ptr, nat.ptr(d_ae), d_ab.ptr,
+             B1.ptr, J1.ptr, S1.ptr)
+    assert np.array_equal(B1.get(), B0.get())
+    assert np.array_equal(J1.get(), J0.get())
+    assert np.array_equal(S1.get(), S0.get())
+    # J alone (psi may be NULL then); the other buffers stay untouched
+    J2, S2 = nat.DeviceArray.zeros(g.Nab, rs), nat.DeviceArray.zeros(g.Nab, rs)
+    nat.call("glx_bulk_observables", prec, grid(g), k2, H, None, None, nat.ptr(d_ae), d_ab.ptr, None, J2.ptr, None)
+    assert np.array_equal(J2.get(), J0.get()) and not S2.get().any()
+    with pytest.raises(nat.GLXError):
+        nat.call("glx_bulk_observables", prec, grid(g), k2, H, None, None, None, d_ab.ptr, None, None, S2.ptr)
+
+
+@pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("shape", [(6, 5), (64, 9), (130, 33)])
 def test_cg_kernels(prec, shape):
     rs, cs = _types(prec)
@@ -373,6 +407,89 @@ def test_vortex_cells_match_oracle():
     assert list(r[:, 0].astype(int)) == list(cells)
     assert np.array_equal(np.round(r[:, 1]), np.round(vs))
     assert np.max(np.abs(r[:, 1] - vs)) < 1e-12
+
+
+@pytest.mark.parametrize("with_abi", [False, True])
+def test_vortex_triangulate_matches_oracle(with_abi):
+    """Sub-cell triangulation on the device against the oracle's transcription
+    of vortex_detector.py:75-147: same kept cells in the same order, equal
+    rounded vorticities, positions within 1e-10. Twenty seeded vortices of
+    both signs in a symmetric-gauge field, so the corner rotations matter."""
+    rng = np.random.default_rng(5)
+    g = O.Geom(131, 121, 0.5, 0.45)
+    x = g.dx * np.arange(g.Nx)[None, :]
+    y = g.dy * np.arange(g.Ny)[:, None]
+    psi = np.ones((g.Ny, g.Nx), complex)
+    for _ in range(20):
+        # |u| ~ distance near the core and ~1 away from it, so the corner
+        # values of a vortex cell clear the reference's 1e-10 sign-change test
+        x0, y0 = rng.uniform(2, g.dx * (g.Nx - 3)), rng.uniform(2, g.dy * (g.Ny - 3))
+        s = rng.choice([-1, 1])
+        u = (x - x0) + 0.15 * (y - y0) + 1j * s * (y - y0)
+        psi = psi * u / np.sqrt(np.abs(u) ** 2 + 1.0)
+    psi = psi.ravel()
+    H = 0.02
+    ab = O.symmetric_gauge(g, H)
+    abi = 0.01 * rng.standard_normal(g.Nab) if with_abi else None
+    ox, oy, ov = O.vortex_detector(g, psi, ab, H, abi)
+    assert len(ox) >= 10
+    cells, vs = O.vortex_cells(g, psi, ab, H)
+    cap = 256
+    rec = nat.DeviceArray.zeros(cap * 14, np.float64)
+    xyv = nat.DeviceArray.zeros(cap * 4, np.float64)
+    cnt = ctypes.c_int32()
+    d_psi, d_ab = D(psi, np.complex128), D(ab, np.float64)
+    d_abi = D(abi, np.float64) if with_abi else None
+    nat.call("glx_vortex_cells", nat.GLX_F64, grid(g), d_psi.ptr, d_ab.ptr, d_abi.ptr if with_abi else None, H,
+             rec.ptr, cap, ctypes.byref(cnt))
+    n = cnt.value
+    assert n == len(cells)
+    nat.call("glx_vortex_triangulate", grid(g), rec.ptr, n, xyv.ptr)
+    r = rec.get()[:n * 14].reshape(n, 14)
+    out = xyv.get()[:n * 4].reshape(n, 4)[np.argsort(r[:, 0], kind='stable')]
+    assert set(np.unique(out[:, 3])) <= {0.0, 1.0}
+    kept = out[out[:, 3] == 1.0]
+    assert len(kept) == len(ox)
+    assert np.array_equal(kept[:, 2], ov)
+    assert np.max(np.abs(kept[:, 0] - ox)) < 1e-10 and np.max(np.abs(kept[:, 1] - oy)) < 1e-10
+    # no records: no launch, no error
+    nat.call("glx_vortex_triangulate", grid(g), rec.ptr, 0, xyv.ptr)
+    with pytest.raises(nat.GLXError):
+        nat.call("glx_vortex_triangulate", grid(g), None, 3, xyv.ptr)
+
+
+def test_vortex_triangulate_every_cell_of_random_field():
+    """The triangulation run on every cell of a random field (records built
+    on the host, in shuffled order), so the rejections are exercised too:
+    border zero counts other than two, near-parallel lines and points
+    outside the 3x3 neighbourhood. Kept cells, vorticities and positions
+    against the oracle's triangulate (vortex_detector.py:75-147)."""
+    rng = np.random.default_rng(9)
+    g = O.Geom(41, 37, 0.5, 0.5)
+    psi = rng.standard_normal(g.N) + 1j * rng.standard_normal(g.N)
+    ab = 0.8 * rng.standard_normal(g.Nab)
+    vs = rng.uniform(-2.4, 2.4, g.Nc)
+    cells = np.arange(g.Nc)
+    ox = O.triangulate(g, cells, vs, psi, g.a2(ab).T, g.b2(ab).T)
+    i, j = cells % g.Nxc, cells // g.Nxc
+    q = i + g.Nx * j
+    rec = np.empty((g.Nc, 14))
+    rec[:, 0], rec[:, 1] = cells, vs
+    for k, off in enumerate((0, 1, 1 + g.Nx, g.Nx)):
+        rec[:, 2 + 2 * k], rec[:, 3 + 2 * k] = psi[q + off].real, psi[q + off].imag
+    rec[:, 10], rec[:, 11] = ab[i + g.Nxa * j], ab[i + g.Nxa * (j + 1)]
+    rec[:, 12], rec[:, 13] = ab[g.Na + i + g.Nxb * j], ab[g.Na + i + 1 + g.Nxb * j]
+    perm = rng.permutation(g.Nc)
+    d_rec = D(rec[perm].ravel(), np.float64)
+    xyv = nat.DeviceArray.zeros(4 * g.Nc, np.float64)
+    nat.call("glx_vortex_triangulate", grid(g), d_rec.ptr, g.Nc, xyv.ptr)
+    out = np.empty((g.Nc, 4))
+    out[perm] = xyv.get().reshape(g.Nc, 4)
+    kept = out[out[:, 3] == 1.0]
+    assert 0 < len(kept) < g.Nc
+    assert len(kept) == len(ox)
+    assert np.array_equal(kept[:, 2], ox[:, 2])
+    assert np.max(np.abs(kept[:, :2] - ox[:, :2])) < 1e-10
 
 
 def test_capacity_error():

@@ -34,11 +34,16 @@ def oracle_of(gl):
         ea, eb = gl.params.external_irregular_vector_potential
         abei = np.concatenate([gl.flatten_a_array(ea), gl.flatten_b_array(eb)]).astype(float)
     eps = float(gl.params.linear_coefficient_scalar_h()) if em is None else 1.0
+    fv = gl.params.fixed_vortices
+    vpi = None
+    if fv._vpi is not None:
+        ia, ib = fv.irregular_vector_potential
+        vpi = np.concatenate([gl.flatten_a_array(ia), gl.flatten_b_array(ib)]).astype(float)
     st = O.OracleState(g, psi, ab, kappa=float(gl.params.gl_parameter), sigma=float(gl.params.normal_conductivity),
                        H=float(gl.params.homogeneous_external_field), eps=eps, eps_map=em, mt=mt, abei=abei,
                        stop_psi=float(c.stop_criterion_order_parameter), stop_A=float(c.stop_criterion_vector_potential),
                        random_t=1 if c.random_seed is None else int(c.random_seed),
-                       convergence_rtol=c.convergence_rtol)
+                       convergence_rtol=c.convergence_rtol, vpi=vpi, lock_ns=fv._phase_lock_ns)
     return st
 
 
@@ -113,6 +118,82 @@ def test_vortex_detector_matches_oracle():
     assert list(gcells) == list(cells) and np.array_equal(np.round(gvs), np.round(vs))
     assert len(vx) == len(ox) and np.array_equal(vv, ov)
     assert np.allclose(vx, ox, atol=1e-10) and np.allclose(vy, oy, atol=1e-10)
+
+
+@pytest.mark.parametrize("kappa", [4.0, np.inf])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_bulk_observables_through_api(kappa, dtype):
+    """Observables.bulk_observables() and the fused normalcurrent_density
+    return what the separate properties return, to the bit; with kappa =
+    inf the current density is the supercurrent (observables.py:86-110)."""
+    gl = GLSolver(Lx=14, Ly=11, dx=0.5, dy=0.5, order_parameter='random', random_seed=4, gl_parameter=kappa,
+                  normal_conductivity=100.0, homogeneous_external_field=0.12, dtype=dtype)
+    mt = np.ones((gl.cfg.Nxc, gl.cfg.Nyc), bool)
+    mt[5:9, 4:8] = False
+    gl.mesh.material_tiling = mt
+    gl.solve.td(dt=0.1, Nt=5)
+    B, (jx, jy), (sx, sy) = gl.observables.bulk_observables()
+    assert np.array_equal(B, gl.observables.magnetic_field)
+    Jx, Jy = gl.observables.current_density
+    Sx, Sy = gl.observables.supercurrent_density
+    assert np.array_equal(jx, Jx) and np.array_equal(jy, Jy)
+    assert np.array_equal(sx, Sx) and np.array_equal(sy, Sy)
+    nx, ny = gl.observables.normalcurrent_density
+    assert np.array_equal(nx, Jx - Sx) and np.array_equal(ny, Jy - Sy)
+    if np.isinf(kappa):
+        assert np.array_equal(jx, sx) and not nx.any()
+
+
+def _vpi_of(gl):
+    ia, ib = gl.params.fixed_vortices.irregular_vector_potential
+    return np.concatenate([gl.flatten_a_array(ia), gl.flatten_b_array(ib)])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_td_fixed_vortices_and_phase_lock(dtype):
+    """examples/_example_fixed_vortices.py scaled down: pinned vortices (an
+    irregular vector potential vpi) and a phase lock, through all three eqn
+    modes, against the oracle's transcription of the reference's per-step
+    hooks (td.py:118-154, 205-231, 256-325), including the drift of vpi
+    and, with eqn='order_parameter', of A. FP64: sweep counts exact, fields
+    within 1e-10; FP32: within float tolerance."""
+    gl = GLSolver(Lx=16, Ly=14, dx=0.5, dy=0.5, order_parameter='random', random_level=0.1, random_seed=2,
+                  linear_coefficient=1.0, gl_parameter=1.0, normal_conductivity=10.0,
+                  fixed_vortices=(np.array([4.2, 4.2, 11.7, 11.7]), np.array([4.1, 9.6, 4.1, 9.6]), [1, -1, -1, 1]),
+                  phase_lock_radius=0.8, dtype=dtype)
+    assert gl.params.fixed_vortices._phase_lock_ns is not None
+    st = oracle_of(gl)
+    f64 = dtype == np.float64
+    tol = 1e-10 if f64 else 2e-4
+
+    def check():
+        if f64:
+            assert gl.solve._td.sweeps == st.sweeps
+        assert rel(gl_psi(gl), st.psi) < tol
+        assert rel(gl_ab(gl), st.ab) < tol
+        assert rel(_vpi_of(gl), st.vpi) < tol
+
+    gl.solve.td(dt=0.1, Nt=6)
+    st.td(dt=0.1, Nt=6)
+    check()
+    gl.solve.td(dt=0.1, Nt=3, eqn='order_parameter')
+    st.td_order_parameter(dt=0.1, Nt=3)
+    check()
+    gl.solve.td(dt=0.1, Nt=3, eqn='vector_potential')
+    st.td_vector_potential(dt=0.1, Nt=3)
+    check()
+    ns = gl.params.fixed_vortices._phase_lock_ns
+    p = gl_psi(gl)[ns]
+    assert np.all(p.imag == 0) and np.all(p.real >= 0)
+    if f64:
+        # the CG sees the pinned vortices through abei = vpi (as built at
+        # construction, params.py:193-221) and ignores the lock (cg.py)
+        st2 = oracle_of(gl)
+        assert st2.abei is not None and np.any(st2.abei != 0)
+        gl.solve.cg(n_iter=8)
+        st2.cg(n_iter=8)
+        assert np.allclose(gl.solve._cg.cg_energies, st2.cg_energies, rtol=1e-9)
+        assert rel(gl_psi(gl), st2.psi) < 1e-8
 
 
 def test_jacobians_finite_differences_through_api():

@@ -365,3 +365,88 @@ def test_aliasing_source_schedule():
         nxt[:] = vals
         cur, nxt = nxt, cur
     assert np.array_equal(cur, A[-1])
+
+
+# ---------------------------------------------------------------------------
+# fixed vortices and phase lock (td.py:118-154, 205-216, 256-266, 318-325)
+# ---------------------------------------------------------------------------
+
+def _chi_x(g, rng):
+    """A gauge function of x only: its gradient lives on the a links alone,
+    which are all inside the first N entries the vpi hooks touch."""
+    cx = rng.normal(size=g.Nx)
+    chi = np.tile(cx, (g.Ny, 1))
+    da = np.tile((cx[1:] - cx[:-1]) / g.dx, (g.Ny, 1))
+    return chi.ravel(), np.concatenate([da.ravel(), np.zeros(g.Nb)])
+
+
+def test_fixed_vortex_psi_hooks_gauge_covariant():
+    """A psi step with vpi = grad(chi) added around the solve equals, by
+    discrete gauge covariance, the plain step times exp(i chi); the
+    post-processing restores A. A sign error in either hook fails it."""
+    rng, g, mt, psi, ab, _, em = _case(61, Nx=9, Ny=8)
+    chi, grad = _chi_x(g, rng)
+    st1 = O.OracleState(g, psi, ab, kappa=np.inf, H=0.1, eps=1.0, eps_map=em, mt=mt)
+    st2 = O.OracleState(g, psi * np.exp(1j * chi), ab, kappa=np.inf, H=0.1, eps=1.0, eps_map=em, mt=mt, vpi=grad)
+    st1.td(dt=0.1, Nt=2)
+    st2.td(dt=0.1, Nt=2)
+    assert st2.sweeps == st1.sweeps
+    assert np.allclose(st2.psi, st1.psi * np.exp(1j * chi), atol=1e-11)
+    assert np.allclose(st2.ab, ab, rtol=0, atol=1e-14)
+
+
+def test_fixed_vortex_hooks_touch_first_N_entries_only():
+    """xpy_r/xmy_r run over N = number of nodes entries of the Nab-long
+    buffers: vpi values beyond index N never reach the psi solve."""
+    rng, g, mt, psi, ab, _, em = _case(62, Nx=8, Ny=9)
+    vpi = np.zeros(g.Nab)
+    vpi[g.N:] = rng.normal(size=g.Nab - g.N)
+    st1 = O.OracleState(g, psi, ab, kappa=np.inf, eps_map=em, mt=mt)
+    st2 = O.OracleState(g, psi, ab, kappa=np.inf, eps_map=em, mt=mt, vpi=vpi)
+    st1.td(dt=0.1, Nt=2)
+    st2.td(dt=0.1, Nt=2)
+    assert np.array_equal(st2.psi, st1.psi) and np.array_equal(st2.ab, st1.ab)
+    assert np.array_equal(st2.vpi, vpi)
+
+
+def test_fixed_vortex_A_solve_fixed_source():
+    """With vpi set, every A sweep reads vpi + A^n: the converged solve is a
+    fixed point of the sweep map with that source (with the aliased source
+    A^(2 floor(j/2)) it would be a fixed point with source A^{n+1} instead),
+    and vpi + A is conserved on the first N entries across the solve."""
+    rng, g, mt, psi, ab, _, _ = _case(63, Nx=9, Ny=9, mask=False)
+    vpi = 0.3 * rng.normal(size=g.Nab)
+    st = O.OracleState(g, psi, ab, kappa=2.0, sigma=10.0, H=0.1, stop_A=1e-13, vpi=vpi)
+    st.td_A_solve(0.1)
+    N = g.N
+    src = vpi.copy()
+    src[:N] = vpi[:N] + ab[:N]
+    again, _, _ = O.a_jacobi_sweep(g, 0.1, st.kappa2, st.rho, st.H, None, psi, src, ab, st.ab, stop_eps=1e-13)
+    assert np.max(np.abs(again - st.ab)) < 1e-11
+    alias, _, _ = O.a_jacobi_sweep(g, 0.1, st.kappa2, st.rho, st.H, None, psi, st.ab, ab, st.ab, stop_eps=1e-13)
+    assert np.max(np.abs(alias - st.ab)) > 1e-6
+    assert np.allclose(st.vpi[:N] + st.ab[:N], vpi[:N] + ab[:N], rtol=0, atol=1e-13)
+    assert np.array_equal(st.vpi[N:], vpi[N:])
+
+
+def test_phase_lock_after_solve_and_order_parameter_hooks():
+    """The lock takes |psi| at the listed nodes after each psi solve (the
+    unlocked step's magnitudes, the other nodes untouched); eqn =
+    'order_parameter' adds vpi once and removes it after every step
+    (td.py:221-231), so A drifts by -(Nt - 1) vpi."""
+    rng, g, mt, psi, ab, _, em = _case(64, Nx=8, Ny=8, mask=False)
+    ns = np.array([3, 10, 27, 40])
+    st1 = O.OracleState(g, psi, ab, kappa=np.inf, eps_map=em)
+    st2 = O.OracleState(g, psi, ab, kappa=np.inf, eps_map=em, lock_ns=ns)
+    st1.td(dt=0.1, Nt=1)
+    st2.td(dt=0.1, Nt=1)
+    other = np.setdiff1d(np.arange(g.N), ns)
+    assert np.array_equal(st2.psi[other], st1.psi[other])
+    assert np.all(st2.psi[ns].imag == 0) and np.all(st2.psi[ns].real >= 0)
+    assert np.allclose(st2.psi[ns].real, np.abs(st1.psi[ns]), rtol=1e-15)
+    vpi = 0.2 * rng.normal(size=g.Nab)
+    st3 = O.OracleState(g, psi, ab, kappa=np.inf, eps_map=em, vpi=vpi)
+    st3.td_order_parameter(dt=0.1, Nt=4)
+    N = g.N
+    assert np.allclose(st3.ab[:N], ab[:N] - 3 * vpi[:N], atol=1e-13)
+    assert np.array_equal(st3.ab[N:], ab[N:])
