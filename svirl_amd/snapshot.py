@@ -1,9 +1,10 @@
 """Checkpoint and resume of a GLSolver's state as a .npz file.
 
 Saved: the geometry and dtype (checked on load), the order parameter and
-vector potential (flat device layouts), the TD solver's random counter
-(TD._random_t, td.py:35-37) and the CG solver's hidden state that the
-reference carries between cg() calls (search directions and betas,
+vector potential (flat device layouts), the fixed-vortex potential vpi
+when set (the TD steps change it, td.py:256-325), the TD solver's random
+counter (TD._random_t, td.py:35-37) and the CG solver's hidden state that
+the reference carries between cg() calls (search directions and betas,
 cg.py:54-71). Loading into a solver built with the same arguments
 continues a run bit for bit. Only numeric arrays are written, and the
 loader never unpickles.
@@ -23,6 +24,10 @@ def save(gl, path):
     d = {"format": np.int64(FORMAT), "Nx": np.int64(c.Nx), "Ny": np.int64(c.Ny),
          "dx": np.float64(c.dx), "dy": np.float64(c.dy), "dtype": np.array(np.dtype(c.dtype).str),
          "psi": np.asarray(psi._data).copy(), "ab": np.asarray(vp._data).copy()}
+    fv = gl.params.fixed_vortices
+    if fv._vpi is not None:
+        fv._vpi.sync()
+        d["vpi"] = np.asarray(fv._vpi._data).copy()
     td = gl.solve._td
     if td is not None:
         d["td_random_t"] = np.uint32(td._random_t)
@@ -44,6 +49,9 @@ def load(gl, path):
             raise ValueError("snapshot geometry does not match the solver")
         if str(f["dtype"]) != np.dtype(c.dtype).str:
             raise ValueError("snapshot dtype does not match the solver")
+        fv = gl.params.fixed_vortices
+        if ("vpi" in f.files) != (fv._vpi is not None):
+            raise ValueError("snapshot and solver differ in having fixed vortices")
         psi = gl.vars._psi
         vp = gl.vars._vp
         psi.sync()
@@ -54,6 +62,11 @@ def load(gl, path):
         vp._data[:] = f["ab"]
         vp.need_htod_sync()
         vp.sync()
+        if fv._vpi is not None:
+            fv._vpi.sync()
+            fv._vpi._data[:] = f["vpi"]
+            fv._vpi.need_htod_sync()
+            fv._vpi.sync()
         if "td_random_t" in f.files:
             gl.solve._init_td()
             gl.solve._td._random_t = np.uint32(f["td_random_t"])

@@ -325,6 +325,75 @@ def test_snapshot_resume_is_bit_exact(tmp_path):
     assert a.solve._td.sweeps[-4:] == b.solve._td.sweeps[-4:]
 
 
+def test_snapshot_resume_with_fixed_vortices(tmp_path):
+    """The TD steps change vpi (td.py:256-325), so the snapshot carries it:
+    a resumed run matches the uninterrupted one to the bit, vpi included."""
+    from svirl_amd import GLSolver, snapshot
+
+    def make():
+        return GLSolver(Lx=12, Ly=12, dx=0.5, dy=0.5, order_parameter='random', random_level=0.2, random_seed=3,
+                        gl_parameter=1.5, normal_conductivity=10.0,
+                        fixed_vortices=([3.1, 8.7], [6.2, 6.3], [1, -1]), phase_lock_radius=0.6, dtype=np.float64)
+    a = make()
+    a.solve.td(dt=0.1, Nt=3)
+    p = str(tmp_path / "snap_fv.npz")
+    snapshot.save(a, p)
+    a.solve.td(dt=0.1, Nt=3)
+    b = make()
+    snapshot.load(b, p)
+    b.solve.td(dt=0.1, Nt=3)
+    assert np.array_equal(a.vars.order_parameter, b.vars.order_parameter)
+    assert np.array_equal(_vpi_of(a), _vpi_of(b)) and np.array_equal(gl_ab(a), gl_ab(b))
+    with pytest.raises(ValueError):
+        snapshot.load(GLSolver(Lx=12, Ly=12, dx=0.5, dy=0.5, gl_parameter=1.5, dtype=np.float64), p)
+
+
+def test_plotter_quantities():
+    """Every quantity svirl/plotter.py:18-64, 175-182 and 243 reads from a
+    solver, taken the way the plotter takes it, has the plotter's shape
+    and is finite, so a host-only plotting layer runs unchanged on the
+    drop-in."""
+    rng = np.random.default_rng(8)
+    gl = GLSolver(Lx=10, Ly=8, dx=0.5, dy=0.5, order_parameter='random', random_seed=1, gl_parameter=2.0,
+                  normal_conductivity=20.0, homogeneous_external_field=0.1,
+                  fixed_vortices=([4.2], [3.9], [1]), dtype=np.float64)
+    c = gl.cfg
+    mt = np.ones((c.Nxc, c.Nyc), bool)
+    mt[2:5, 2:4] = False
+    gl.mesh.material_tiling = mt
+    gl.params.linear_coefficient = 1.0 + 0.1 * rng.normal(size=(c.Nx, c.Ny))
+    gl.solve.td(dt=0.1, Nt=3)
+    node, cell, a_sh, b_sh = (c.Nx, c.Ny), (c.Nxc, c.Nyc), (c.Nxa, c.Nya), (c.Nxb, c.Nyb)
+    jx, jy = gl.observables.current_density
+    sx, sy = gl.observables.supercurrent_density
+    nx, ny = gl.observables.normalcurrent_density
+    va, vb = gl.vars.vector_potential
+    ia, ib = gl.params.fixed_vortices.irregular_vector_potential
+    q = {
+        'material_tiling': (gl.mesh.material_tiling.astype(c.dtype), cell),
+        'linear_coefficient': (gl.params.linear_coefficient, node),
+        'superfluid_density': (gl.observables.superfluid_density, node),
+        'order_parameter_phase': (np.angle(gl.vars.order_parameter) / (2.0 * np.pi), node),
+        'op_fv_phase': (np.angle(gl.vars.order_parameter) - gl.params.fixed_vortices.fixed_vortices_phase, node),
+        'magnetic_field': (gl.observables.magnetic_field - gl.params.homogeneous_external_field, cell),
+        'current_density': (gl.mesh.interpolate_ab_array_to_c_array_abs(jx, jy), cell),
+        'current_density_x': (jx, a_sh), 'current_density_y': (jy, b_sh),
+        'supercurrent_density': (gl.mesh.interpolate_ab_array_to_c_array_abs(sx, sy), cell),
+        'normalcurrent_density': (gl.mesh.interpolate_ab_array_to_c_array_abs(nx, ny), cell),
+        'vector_potential_x': (va, a_sh), 'vector_potential_y': (vb, b_sh),
+        'irregular_vector_potential_x': (ia, a_sh), 'irregular_vector_potential_y': (ib, b_sh),
+        'fixed_vortices_phase': (gl.params.fixed_vortices.fixed_vortices_phase, node),
+    }
+    for name, (v, shape) in q.items():
+        assert np.shape(v) == shape, name
+        assert np.all(np.isfinite(v)), name
+    assert np.all(q['current_density'][0] >= 0)
+    extent = [-0.5 * c.dx, c.Lx + 0.5 * c.dx, -0.5 * c.dy, c.Ly + 0.5 * c.dy]
+    assert np.isclose((extent[1] - extent[0]) / c.dx, c.Nx) and np.isclose((extent[3] - extent[2]) / c.dy, c.Ny)
+    vx, vy, vv = gl.vortex_detector.vortices
+    assert len(vx) == len(vy) == len(vv)
+
+
 @pytest.mark.parametrize("kappa", [5.0, np.inf])
 def test_cg_device_line_search(kappa):
     """The device line search (glx_cg_line_step, the FP32-mixed default)
