@@ -353,6 +353,37 @@ int glx_cg_pr_beta(int prec, const void *jac, const void *jac_prev, int64_t n, d
 /* z = alpha*x - y with alpha read from device memory (the reference's
  * axmy_c/axmy_r called with the device beta, cg.py:283-290). */
 int glx_axmy_dev(int prec, const void *x, const void *y, void *z, const double *alpha_dev, int64_t n);
+/* The direction update of one CG iteration fused into its line-search
+ * coefficients: dpsi_out = beta_dev[0]*dpsi - jac and (dab not NULL)
+ * dab_out = beta_dev[1]*dab - jacA (axmy_c/axmy_r with the device betas,
+ * cg.py:283-290, 491-530), formed where the coefficient pass reads the
+ * direction (free_energy_conjgrad_coef_psi / _coef, cg.h:314-731) and
+ * written on the way; the outputs must be buffers other than dpsi / dab.
+ * Then, with c != NULL: the 5 (dab NULL) or 17 coefficients, reduced in
+ * FP64, returned in c (the reference's host line search follows); with
+ * c == NULL: the device line search of glx_cg_line_step on them
+ * (alpha_dev written; the state is not updated: see
+ * glx_cg_update_gradient_pass). No counterpart in the reference as one
+ * call: it replaces the axmy launches and the coefficient launch of an
+ * iteration. */
+int glx_cg_direction_step(int prec, glx_grid g, double kappa2, double eps, double H,
+                          const uint8_t *mt, const void *psi, const void *abei, const void *ab,
+                          const void *dpsi, const void *jac, void *dpsi_out,
+                          const void *dab, const void *jacA, void *dab_out,
+                          const double *beta_dev, double *c, double *alpha_dev);
+/* The update of one CG iteration fused into the next gradient pass:
+ * glx_cg_gradient_pass evaluated at psi + alpha_dev[0]*dpsi and (dab not
+ * NULL) ab + alpha_dev[1]*dab (axpy_c/axpy_r, cg.py:296, 535-536, with the
+ * alphas read from device memory as doubles and rounded to the working
+ * precision), and the updated state written to psi_out / ab_out, buffers
+ * other than psi, dpsi, ab and dab, which then hold the state. No
+ * counterpart in the reference as one call. */
+int glx_cg_update_gradient_pass(int prec, glx_grid g, double kappa2, double eps,
+                                const void *eps_map, double H, const uint8_t *mt, const void *psi,
+                                const void *abei, const void *ab, const void *dpsi, const void *dab,
+                                const double *alpha_dev, void *psi_out, void *ab_out, void *jac,
+                                const void *jac_prev, void *jacA, const void *jacA_prev,
+                                double *beta_next_dev, double *F);
 /* The line search and update of one CG iteration on the device, for the
  * FP32-mixed form: the line-search coefficients (as glx_cg_coef, or
  * glx_cg_coef_psi when dab is NULL; cg.py:184-224, 325-375), their

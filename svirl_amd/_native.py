@@ -92,6 +92,8 @@ _PROTOS = {
                         c_void_p, P_double],
     "glx_vortex_cells": [c_int, Grid, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_i32, P_i32],
     "glx_vortex_triangulate": [Grid, c_void_p, c_i32, c_void_p],
+    "glx_cg_direction_step": [c_int, Grid, c_double, c_double, c_double] + [c_void_p] * 13,
+    "glx_cg_update_gradient_pass": [c_int, Grid, c_double, c_double, c_void_p, c_double] + [c_void_p] * 15,
     "glx_bulk_observables": [c_int, Grid, c_double, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_void_p],
     "glx_sum": [c_int, c_void_p, c_i64, c_i32, P_double],

@@ -327,6 +327,24 @@ int glx_vortex_cells(int prec, glx_grid g, const void *psi, const void *ab, cons
     GLX_DISPATCH(prec, vortex_cells(g_ctx.env, g, psi, ab, abi, H, rec, capacity, count));
 }
 
+int glx_cg_direction_step(int prec, glx_grid g, double kappa2, double eps, double H, const uint8_t *mt,
+                          const void *psi, const void *abei, const void *ab, const void *dpsi, const void *jac, void *dpsi_out,
+                          const void *dab, const void *jacA, void *dab_out, const double *beta_dev, double *c,
+                          double *alpha_dev) {
+    GLX_DISPATCH(prec, cg_direction_step(g_ctx.env, g, kappa2, eps, H, mt, psi, abei, ab, dpsi, jac, dpsi_out, dab,
+                                         jacA, dab_out, beta_dev, c, alpha_dev));
+}
+
+int glx_cg_update_gradient_pass(int prec, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
+                                const uint8_t *mt, const void *psi, const void *abei, const void *ab,
+                                const void *dpsi, const void *dab, const double *alpha_dev, void *psi_out,
+                                void *ab_out, void *jac, const void *jac_prev, void *jacA, const void *jacA_prev,
+                                double *beta_next_dev, double *F) {
+    GLX_DISPATCH(prec, cg_update_gradient_pass(g_ctx.env, g, kappa2, eps, eps_map, H, mt, psi, abei, ab, dpsi, dab,
+                                               alpha_dev, psi_out, ab_out, jac, jac_prev, jacA, jacA_prev,
+                                               beta_next_dev, F));
+}
+
 int glx_vortex_triangulate(glx_grid g, const double *rec, int32_t count, double *xyv) {
     GLX_REQUIRE_INIT();
     return L<double>::vortex_triangulate(g_ctx.env, g, rec, count, xyv);

@@ -578,16 +578,39 @@ __device__ __forceinline__ R curlcurl_b_v(const Geo &g, const GeoR<R> &q, R H, b
 // results are theirs bit for bit. bscale multiplies the field part of dF/dA
 // on boundary links: 1 keeps the reference's Jacobian (twice the derivative
 // there, ghost cell with dd = 2), 1/2 gives the exact gradient.
-template <typename R, bool withA>
+//
+// The CG state update fused into the gradient pass (StateUpdate, UPD): the
+// pass evaluates the updated state s + alpha d (k_axpy_dev's arithmetic)
+// wherever it reads psi or ab, and node n writes its own entries (psi(n),
+// and the links a(i, j), b(i, j) it owns) to psi_out / ab_out, buffers
+// other than the ones read. dab NULL: A is not updated.
+template <typename R> struct StateUpdate {
+    const cplx<R> *dpsi;
+    const R *dab;
+    cplx<R> *psi_out;
+    R *ab_out;
+    R ap, aa;  // alpha_psi, alpha_A rounded to the working precision
+};
+
+template <typename R, bool withA, bool UPD = false>
 __device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kappa2, R eps, const R *eps_map, R H,
                                           const uint8_t *mt, const cplx<R> *psi, const R *abei, const R *ab, long n,
                                           int i, int j, R &e, cplx<R> &gj, R &ga, R &gb, bool &has_a, bool &has_b,
-                                          R bscale = R(1)) {
+                                          R bscale = R(1), const StateUpdate<R> *S = nullptr) {
     // loads
     const long row = (long)g.Nx * j;
-    const cplx<R> p0 = ld(psi, n);
-    const cplx<R> pl = ld(psi, row + max(i - 1, 0)), pr = ld(psi, row + min(i + 1, g.Nx - 1));
-    const cplx<R> pd = ld(psi, (long)g.Nx * max(j - 1, 0) + i), pu = ld(psi, (long)g.Nx * min(j + 1, g.Ny - 1) + i);
+    auto lp = [&](long k) {
+        cplx<R> p = ld(psi, k);
+        if constexpr (UPD) {
+            const cplx<R> d = ld(S->dpsi, k);
+            p.re = S->ap * d.re + p.re;
+            p.im = S->ap * d.im + p.im;
+        }
+        return p;
+    };
+    const cplx<R> p0 = lp(n);
+    const cplx<R> pl = lp(row + max(i - 1, 0)), pr = lp(row + min(i + 1, g.Nx - 1));
+    const cplx<R> pd = lp((long)g.Nx * max(j - 1, 0) + i), pu = lp((long)g.Nx * min(j + 1, g.Ny - 1) + i);
     const R ee = eps_map ? eps_map[n] : eps;
     Cells c;
     c.mm = (i > 0) && (j > 0);
@@ -607,6 +630,25 @@ __device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kapp
     Links10<R> E{}, B{};
     if (he) E = load_links10(g, abei, i, j);
     if (hb) B = load_links10(g, ab, i, j);
+    if constexpr (UPD) {
+        st(S->psi_out, n, p0);
+        if (S->dab) {
+            const Links10<R> d = load_links10(g, S->dab, i, j);
+            const R a = S->aa;
+            B.a00 = a * d.a00 + B.a00;
+            B.am0 = a * d.am0 + B.am0;
+            B.a0m = a * d.a0m + B.a0m;
+            B.a0p = a * d.a0p + B.a0p;
+            B.amp = a * d.amp + B.amp;
+            B.b00 = a * d.b00 + B.b00;
+            B.b0m = a * d.b0m + B.b0m;
+            B.bm0 = a * d.bm0 + B.bm0;
+            B.bp0 = a * d.bp0 + B.bp0;
+            B.bpm = a * d.bpm + B.bpm;
+            if (i < g.Nxa) S->ab_out[i + (long)g.Nxa * j] = B.a00;
+            if (j < g.Nyb) S->ab_out[g.Na + i + (long)g.Nxb * j] = B.b00;
+        }
+    }
     auto pvv = [&](R ve, R vb) {  // pv(abei, ab, k) of loaded values
         R p = R(0);
         if (he) p += ve;
@@ -705,11 +747,16 @@ __device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kapp
 // buffer), and accumulates the free energy of the state and the two pairs
 // of Polak-Ribiere sums against the previous Jacobians. partials per
 // workgroup: F, num_psi, den_psi, num_A, den_A. withA: kappa < inf.
-template <typename R, bool withA>
+template <typename R, bool withA, bool UPD = false>
 __global__ void __launch_bounds__(256) k_cg_gradient_pass(Geo g, R kappa2, R eps, const R *eps_map, R H, const uint8_t *mt,
                                    const cplx<R> *psi, const R *abei, const R *ab, cplx<R> *jac,
-                                   const cplx<R> *jac_prev, R *jacA, const R *jacA_prev, double *partials) {
+                                   const cplx<R> *jac_prev, R *jacA, const R *jacA_prev, double *partials,
+                                   StateUpdate<R> S = {}, const double *alpha = nullptr) {
     const GeoR<R> q(g);
+    if constexpr (UPD) {
+        S.ap = R(alpha[0]);
+        S.aa = R(alpha[1]);
+    }
     const R dxdy = q.dx * q.dy, s2 = R(2) * q.dx * q.dy;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
@@ -718,7 +765,8 @@ __global__ void __launch_bounds__(256) k_cg_gradient_pass(Geo g, R kappa2, R eps
         R e, ga, gb;
         cplx<R> gj;
         bool ha, hb;
-        grad_node<R, withA>(g, q, kappa2, eps, eps_map, H, mt, psi, abei, ab, n, i, j, e, gj, ga, gb, ha, hb);
+        grad_node<R, withA, UPD>(g, q, kappa2, eps, eps_map, H, mt, psi, abei, ab, n, i, j, e, gj, ga, gb, ha, hb,
+                                 R(1), &S);
         // every load of the node before its stores
         const cplx<R> gp = ld(jac_prev, n);
         const long la = i + (long)g.Nxa * j, lb = g.Na + i + (long)g.Nxb * j;
@@ -901,16 +949,45 @@ template <typename R> __device__ __forceinline__ Links4<R> load_links4(const Geo
     return L;
 }
 
-template <typename R, int L>
+// The CG direction update fused into the coefficient pass: the direction
+// d = beta * d_old - jac (axmy with the device beta, cg.py:283-290, 491-530)
+// is formed wherever coef_node reads it, from d_old and jac, and node n
+// writes its own entries (d(n), and the links a(i, j), b(i, j) it owns)
+// to d_out, a buffer other than d_old. The arithmetic is k_axmy_dev's.
+template <typename R> struct DirUpdate {
+    const cplx<R> *dold, *jac;
+    const R *dAold, *jacA;
+    cplx<R> *dout;
+    R *dAout;
+    R bp, ba;  // beta_psi, beta_A rounded to the working precision
+};
+
+template <typename R>
+__device__ __forceinline__ cplx<R> dir_at(const DirUpdate<R> &U, long k) {
+    const cplx<R> o = ld(U.dold, k), gk = ld(U.jac, k);
+    return {U.bp * o.re - gk.re, U.bp * o.im - gk.im};
+}
+
+template <typename R, int L, bool DIR = false>
 __device__ __forceinline__ void coef_node(const Geo &g, const GeoR<R> &q, R kappa2, R eps_s, const R *eps_map, R H,
                                           const uint8_t *mt, const cplx<R> *psi, const cplx<R> *dpsi, const R *abei,
-                                          const R *ab, const R *dab, long n, R (&C)[L]) {
+                                          const R *ab, const R *dab, long n, R (&C)[L],
+                                          const DirUpdate<R> *U = nullptr) {
     int i, j;
     node_ij(g, n, i, j);
     // loads
     const long nr = (long)g.Nx * j + min(i + 1, g.Nx - 1), nu = (long)g.Nx * min(j + 1, g.Ny - 1) + i;
-    const cplx<R> p0 = ld(psi, n), d0 = ld(dpsi, n);
-    const cplx<R> p_r = ld(psi, nr), d_r = ld(dpsi, nr), p_u = ld(psi, nu), d_u = ld(dpsi, nu);
+    const cplx<R> p0 = ld(psi, n), p_r = ld(psi, nr), p_u = ld(psi, nu);
+    cplx<R> d0, d_r, d_u;
+    if constexpr (DIR) {
+        d0 = dir_at(*U, n);
+        d_r = dir_at(*U, nr);
+        d_u = dir_at(*U, nu);
+    } else {
+        d0 = ld(dpsi, n);
+        d_r = ld(dpsi, nr);
+        d_u = ld(dpsi, nu);
+    }
     const R eps = eps_map ? eps_map[n] : eps_s;
     Cells c;
     c.mm = (i > 0) && (j > 0);
@@ -930,7 +1007,24 @@ __device__ __forceinline__ void coef_node(const Geo &g, const GeoR<R> &q, R kapp
     Links4<R> E{}, B{}, D{};
     if (he) E = load_links4(g, abei, i, j);
     if (hb) B = load_links4(g, ab, i, j);
-    if (L == 17) D = load_links4(g, dab, i, j);
+    if (L == 17) {
+        if constexpr (DIR) {
+            const Links4<R> o = load_links4(g, U->dAold, i, j), gk = load_links4(g, U->jacA, i, j);
+            D.a00 = U->ba * o.a00 - gk.a00;
+            D.a0p = U->ba * o.a0p - gk.a0p;
+            D.b00 = U->ba * o.b00 - gk.b00;
+            D.bp0 = U->ba * o.bp0 - gk.bp0;
+        } else {
+            D = load_links4(g, dab, i, j);
+        }
+    }
+    if constexpr (DIR) {
+        st(U->dout, n, d0);
+        if (L == 17) {
+            if (i < g.Nxa) U->dAout[i + (long)g.Nxa * j] = D.a00;
+            if (j < g.Nyb) U->dAout[g.Na + i + (long)g.Nxb * j] = D.b00;
+        }
+    }
     auto pvv = [&](R ve, R vb) {  // pv(abei, ab, k) of loaded values
         R p = R(0);
         if (he) p += ve;
@@ -1030,6 +1124,30 @@ __global__ void __launch_bounds__(256) k_coef(Geo g, R kappa2, R eps, R H, const
 #pragma unroll
         for (int k = 0; k < L; ++k) C[k] = R(0);
         coef_node<R, L>(g, q, kappa2, eps, (const R *)nullptr, H, mt, psi, dpsi, abei, ab, dab, n, C);
+#pragma unroll
+        for (int k = 0; k < L; ++k) acc[k] += double(C[k]);
+    }
+    block_store_partials<L>(acc, partials);
+}
+
+// k_coef with the direction update fused in (DirUpdate): the coefficients
+// of F(psi + a d_psi, A + b d_A) for the new direction, which is written to
+// U.dout / U.dAout on the way. beta[0..1] are the device betas.
+template <typename R, int L>
+__global__ void __launch_bounds__(256) k_coef_dir(Geo g, R kappa2, R eps, R H, const uint8_t *mt, const cplx<R> *psi,
+                                                  const R *abei, const R *ab, DirUpdate<R> U, const double *beta,
+                                                  double *partials) {
+    const GeoR<R> q(g);
+    U.bp = R(beta[0]);
+    U.ba = R(beta[1]);
+    double acc[L];
+#pragma unroll
+    for (int k = 0; k < L; ++k) acc[k] = 0.0;
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
+        R C[L];
+#pragma unroll
+        for (int k = 0; k < L; ++k) C[k] = R(0);
+        coef_node<R, L, true>(g, q, kappa2, eps, (const R *)nullptr, H, mt, psi, nullptr, abei, ab, nullptr, n, C, &U);
 #pragma unroll
         for (int k = 0; k < L; ++k) acc[k] += double(C[k]);
     }

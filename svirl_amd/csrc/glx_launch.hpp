@@ -77,6 +77,15 @@ template <typename R> struct L {
     static int axmy_dev(const Env &e, const void *x, const void *y, void *z, const double *alpha_dev, int64_t n);
     static int cg_line_step(const Env &e, glx_grid g, double kappa2, double eps, double H, const uint8_t *mt, void *psi,
                             const void *dpsi, const void *abei, void *ab, const void *dab, double *alpha_dev);
+    static int cg_direction_step(const Env &e, glx_grid g, double kappa2, double eps, double H, const uint8_t *mt,
+                                 const void *psi, const void *abei, const void *ab, const void *dpsi, const void *jac,
+                                 void *dpsi_out, const void *dab, const void *jacA, void *dab_out,
+                                 const double *beta_dev, double *c, double *alpha_dev);
+    static int cg_update_gradient_pass(const Env &e, glx_grid g, double kappa2, double eps, const void *eps_map,
+                                       double H, const uint8_t *mt, const void *psi, const void *abei, const void *ab,
+                                       const void *dpsi, const void *dab, const double *alpha_dev, void *psi_out,
+                                       void *ab_out, void *jac, const void *jac_prev, void *jacA,
+                                       const void *jacA_prev, double *beta_next_dev, double *F);
     static int cg_gradient_pass(const Env &e, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
                                 const uint8_t *mt, const void *psi, const void *abei, const void *ab, void *jac,
                                 const void *jac_prev, void *jacA, const void *jacA_prev, double *beta_next_dev,

@@ -808,6 +808,67 @@ int L<R>::coef(const Env &e, glx_grid g, double kappa2, double eps, double H, co
     return fetch_result(e, c17, 17);
 }
 
+// The gradient pass, optionally with the state update fused in (S, alpha:
+// UPD; S.dpsi NULL: no update).
+template <typename R>
+static int gradient_pass_launch(const Env &e, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
+                                const uint8_t *mt, const void *psi, const void *abei, const void *ab, void *jac,
+                                const void *jac_prev, void *jacA, const void *jacA_prev, const StateUpdate<R> &S,
+                                const double *alpha, int *nb_out) {
+    const bool withA = kappa2 > 0.0;  // the psi Jacobian reads A in both cases
+    const bool upd = S.dpsi != nullptr;
+    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    const int nb = grid_blocks(G.N, RED_BLOCK, RED_MAX_BLOCKS);
+    *nb_out = nb;
+    const R k2 = R(kappa2), ep = R(eps), h = R(H);
+    const R *em = (const R *)eps_map, *ae = (const R *)abei, *a = (const R *)ab;
+    const cplx<R> *p = (const cplx<R> *)psi, *jp = (const cplx<R> *)jac_prev;
+    cplx<R> *jc = (cplx<R> *)jac;
+    R *jA = withA ? (R *)jacA : nullptr;
+    const R *jAp = withA ? (const R *)jacA_prev : nullptr;
+    if (withA && upd)
+        hipLaunchKernelGGL((k_cg_gradient_pass<R, true, true>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, k2, ep, em,
+                           h, mt, p, ae, a, jc, jp, jA, jAp, e.partials, S, alpha);
+    else if (withA)
+        hipLaunchKernelGGL((k_cg_gradient_pass<R, true>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, k2, ep, em, h, mt,
+                           p, ae, a, jc, jp, jA, jAp, e.partials, S, alpha);
+    else if (upd)
+        hipLaunchKernelGGL((k_cg_gradient_pass<R, false, true>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, k2, ep, em,
+                           h, mt, p, ae, a, jc, jp, jA, jAp, e.partials, S, alpha);
+    else
+        hipLaunchKernelGGL((k_cg_gradient_pass<R, false>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, k2, ep, em, h,
+                           mt, p, ae, a, jc, jp, jA, jAp, e.partials, S, alpha);
+    GLX_LAUNCH_CHECK("k_cg_gradient_pass");
+    return GLX_OK;
+}
+
+template <typename R>
+int L<R>::cg_update_gradient_pass(const Env &e, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
+                                  const uint8_t *mt, const void *psi, const void *abei, const void *ab,
+                                  const void *dpsi, const void *dab, const double *alpha_dev, void *psi_out,
+                                  void *ab_out, void *jac, const void *jac_prev, void *jacA, const void *jacA_prev,
+                                  double *beta_next_dev, double *F) {
+    const bool withA = kappa2 > 0.0;
+    if (!grid_ok(g) || !psi || !jac || !jac_prev || !beta_next_dev || !F || !dpsi || !alpha_dev || !psi_out)
+        return set_error(GLX_E_ARG, "glx_cg_update_gradient_pass: invalid arguments");
+    if (withA && (!(abei || ab) || !jacA || !jacA_prev))
+        return set_error(GLX_E_ARG, "glx_cg_update_gradient_pass: finite kappa needs the vector potential and its Jacobians");
+    if (dab && (!ab || !ab_out))
+        return set_error(GLX_E_ARG, "glx_cg_update_gradient_pass: an A direction needs ab and ab_out");
+    if (psi_out == psi || psi_out == dpsi || (dab && (ab_out == ab || ab_out == dab)))
+        return set_error(GLX_E_ARG, "glx_cg_update_gradient_pass: the updated state needs buffers of its own");
+    const StateUpdate<R> S{(const cplx<R> *)dpsi, (const R *)dab, (cplx<R> *)psi_out, (R *)ab_out, R(0), R(0)};
+    int nb = 0;
+    int rc = gradient_pass_launch<R>(e, g, kappa2, eps, eps_map, H, mt, psi, abei, ab, jac, jac_prev, jacA, jacA_prev,
+                                     S, alpha_dev, &nb);
+    if (rc) return rc;
+    const double dxdy = double(R(g.dx)) * double(R(g.dy));
+    hipLaunchKernelGGL(k_cg_gradient_finish<R>, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, dxdy, e.result,
+                       beta_next_dev);
+    GLX_LAUNCH_CHECK("k_cg_gradient_finish");
+    return fetch_result(e, F, 1);
+}
+
 template <typename R>
 int L<R>::cg_gradient_pass(const Env &e, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
                            const uint8_t *mt, const void *psi, const void *abei, const void *ab, void *jac,
@@ -817,17 +878,10 @@ int L<R>::cg_gradient_pass(const Env &e, glx_grid g, double kappa2, double eps, 
         return set_error(GLX_E_ARG, "glx_cg_gradient_pass: invalid arguments");
     if (withA && (!(abei || ab) || !jacA || !jacA_prev))
         return set_error(GLX_E_ARG, "glx_cg_gradient_pass: finite kappa needs the vector potential and its Jacobians");
-    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
-    const int nb = grid_blocks(G.N, RED_BLOCK, RED_MAX_BLOCKS);
-    if (withA)
-        hipLaunchKernelGGL((k_cg_gradient_pass<R, true>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, R(kappa2), R(eps),
-                           (const R *)eps_map, R(H), mt, (const cplx<R> *)psi, (const R *)abei, (const R *)ab,
-                           (cplx<R> *)jac, (const cplx<R> *)jac_prev, (R *)jacA, (const R *)jacA_prev, e.partials);
-    else
-        hipLaunchKernelGGL((k_cg_gradient_pass<R, false>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, R(kappa2), R(eps),
-                           (const R *)eps_map, R(H), mt, (const cplx<R> *)psi, (const R *)abei, (const R *)ab,
-                           (cplx<R> *)jac, (const cplx<R> *)jac_prev, (R *)nullptr, (const R *)nullptr, e.partials);
-    GLX_LAUNCH_CHECK("k_cg_gradient_pass");
+    int nb = 0;
+    int rc = gradient_pass_launch<R>(e, g, kappa2, eps, eps_map, H, mt, psi, abei, ab, jac, jac_prev, jacA, jacA_prev,
+                                     StateUpdate<R>{}, nullptr, &nb);
+    if (rc) return rc;
     const double dxdy = double(R(g.dx)) * double(R(g.dy));
     hipLaunchKernelGGL(k_cg_gradient_finish<R>, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, dxdy, e.result,
                        beta_next_dev);
@@ -890,6 +944,47 @@ int L<R>::cg_line_step(const Env &e, glx_grid g, double kappa2, double eps, doub
                            (const R *)ab, (R *)ab, (const double *)alpha_dev + 1, (long)G.Nab);
         GLX_LAUNCH_CHECK("k_axpy_dev");
     }
+    return GLX_OK;
+}
+
+template <typename R>
+int L<R>::cg_direction_step(const Env &e, glx_grid g, double kappa2, double eps, double H, const uint8_t *mt,
+                            const void *psi, const void *abei, const void *ab, const void *dpsi, const void *jac, void *dpsi_out,
+                            const void *dab, const void *jacA, void *dab_out, const double *beta_dev, double *c,
+                            double *alpha_dev) {
+    const bool withA = dab != nullptr;
+    if (!grid_ok(g) || !psi || !dpsi || !jac || !dpsi_out || !beta_dev || (!c && !alpha_dev))
+        return set_error(GLX_E_ARG, "glx_cg_direction_step: invalid arguments");
+    if (withA && (!ab || !jacA || !dab_out))
+        return set_error(GLX_E_ARG, "glx_cg_direction_step: a vector potential direction needs ab, jacA and dab_out");
+    if (dpsi_out == dpsi || (withA && dab_out == dab))
+        return set_error(GLX_E_ARG, "glx_cg_direction_step: the new direction needs a buffer of its own");
+    if (kappa2 > 0.0 && !(abei || ab)) return set_error(GLX_E_ARG, "glx_cg_direction_step: finite kappa needs ab");
+    const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    const int nb = grid_blocks(G.N, RED_BLOCK, RED_MAX_BLOCKS);
+    const double dxdy = double(R(g.dx)) * double(R(g.dy));
+    DirUpdate<R> U{(const cplx<R> *)dpsi, (const cplx<R> *)jac, (const R *)dab, (const R *)jacA,
+                   (cplx<R> *)dpsi_out, (R *)dab_out, R(0), R(0)};
+    const int nc = withA ? 17 : 5;
+    if (withA)
+        hipLaunchKernelGGL((k_coef_dir<R, 17>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, R(kappa2), R(eps), R(H), mt,
+                           (const cplx<R> *)psi, (const R *)abei, (const R *)ab, U, beta_dev, e.partials);
+    else
+        hipLaunchKernelGGL((k_coef_dir<R, 5>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, R(kappa2), R(eps), R(H), mt,
+                           (const cplx<R> *)psi, (const R *)abei, (const R *)ab, U, beta_dev, e.partials);
+    GLX_LAUNCH_CHECK("k_coef_dir");
+    if (c) {
+        hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, nc, dxdy, e.result);
+        GLX_LAUNCH_CHECK("k_finish_partials");
+        return fetch_result(e, c, nc);
+    }
+    if (withA)
+        hipLaunchKernelGGL(k_line_search<17>, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, dxdy, e.result,
+                           alpha_dev);
+    else
+        hipLaunchKernelGGL(k_line_search<5>, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, dxdy, e.result,
+                           alpha_dev);
+    GLX_LAUNCH_CHECK("k_line_search");
     return GLX_OK;
 }
 

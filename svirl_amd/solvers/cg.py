@@ -38,6 +38,7 @@ class CG:
         self.__reduction_vector_length = 17 if self.params.solveA else 5
         cd = cfg.dtype_complex
         self.__gdir_psi = DeviceArray.zeros(cfg.N, cd)
+        self.__gdir_psi_next = DeviceArray(cfg.N, cd)  # the fused direction update writes here
         self.__gjac_psi = DeviceArray.zeros(cfg.N, cd)
         self.__gjac_psi_prev = DeviceArray.zeros(cfg.N, cd)
         # beta stays in device memory, as in the reference: [beta_psi, beta_A,
@@ -45,6 +46,7 @@ class CG:
         self.__gbeta = DeviceArray.zeros(4, np.float64)
         if self.params.solveA:
             self.__gdir_A = DeviceArray.zeros(cfg.Nab, cfg.dtype)
+            self.__gdir_A_next = DeviceArray(cfg.Nab, cfg.dtype)
             self.__gjac_A = DeviceArray.zeros(cfg.Nab, cfg.dtype)
             self.__gjac_A_prev = DeviceArray.zeros(cfg.Nab, cfg.dtype)
         self.__c = np.zeros((5, 5), dtype=cfg.dtype) if self.params.solveA else np.zeros(5, dtype=cfg.dtype)
@@ -190,13 +192,14 @@ class CG:
 
     # -- minimisation loops ----------------------------------------------------------------------
     #
-    # Both loops follow cg.py:238-322 / 452-551 step for step, with the
-    # gradient, the energy and the Polak-Ribiere sums of a state fused into
-    # one pass (glx_cg_gradient_pass): the pass after the update of
-    # iteration i gives E_i for the stop test and the gradient and the
-    # (staged) beta of iteration i+1. The staged beta only becomes the
-    # solver's beta when iteration i+1 runs, so the state left behind when
-    # the loop stops is the reference's.
+    # Both loops follow cg.py:238-322 / 452-551 step for step in two passes
+    # per iteration. The direction update rides in the line-search
+    # coefficient pass (glx_cg_direction_step). The update of the state, its
+    # gradient, energy and Polak-Ribiere sums share the next pass
+    # (glx_cg_update_gradient_pass): it gives E_i for the stop test and the
+    # gradient and the (staged) beta of iteration i+1. The staged beta only
+    # becomes the solver's beta when iteration i+1 runs, so the state left
+    # behind when the loop stops is the reference's.
     def __gradient_pass(self, withA):
         a = self._args()
         F = ctypes.c_double()
@@ -206,10 +209,46 @@ class CG:
                  self._beta_ptr(2), ctypes.byref(F))
         return F.value
 
-    def __device_line_step(self, withA):
+    def __direction_step(self, withA, c=None):
+        """dir = beta dir - jac for psi (and A) fused into the line-search
+        coefficients (glx_cg_direction_step): with c (a ctypes array of 5 or
+        17 doubles) the coefficients come back to the host; without, the
+        device line search leaves the alphas in device memory. The new
+        directions are written to the second buffers, which then become the
+        current ones."""
         a = self._args()
-        nat.call("glx_cg_line_step", self.par.prec, self._grid(), a['k2'], a['eps'], a['H'], a['mt'], a['psi'],
-                 self.__gdir_psi.ptr, a['abei'], a['ab'], self.__gdir_A.ptr if withA else None, self.__galpha.ptr)
+        nat.call("glx_cg_direction_step", self.par.prec, self._grid(), a['k2'], a['eps'], a['H'], a['mt'], a['psi'],
+                 a['abei'], a['ab'], self.__gdir_psi.ptr, self.__gjac_psi.ptr, self.__gdir_psi_next.ptr,
+                 self.__gdir_A.ptr if withA else None, self.__gjac_A.ptr if withA else None,
+                 self.__gdir_A_next.ptr if withA else None, self._beta_ptr(0), c,
+                 None if c is not None else self.__galpha.ptr)
+        self.__gdir_psi, self.__gdir_psi_next = self.__gdir_psi_next, self.__gdir_psi
+        if withA:
+            self.__gdir_A, self.__gdir_A_next = self.__gdir_A_next, self.__gdir_A
+
+    def __update_gradient_pass(self, withA):
+        """The update state += alpha dir (alphas in device memory) fused into
+        the gradient pass of the updated state (glx_cg_update_gradient_pass);
+        the updated state goes to another buffer of the field's pool, which
+        becomes the current one."""
+        a = self._args()
+        v = self.vars
+        P, pc = v._psi.device_pool()
+        psi_out = P[(pc + 1) % 3]
+        ab_out = None
+        if withA:
+            Q, qc = v._vp.device_pool()
+            ab_out = Q[(qc + 1) % 3]
+        F = ctypes.c_double()
+        nat.call("glx_cg_update_gradient_pass", self.par.prec, self._grid(), a['k2'], a['eps'], a['epsmap'], a['H'],
+                 a['mt'], a['psi'], a['abei'], a['ab'], self.__gdir_psi.ptr, self.__gdir_A.ptr if withA else None,
+                 self.__galpha.ptr, psi_out.ptr, nat.ptr(ab_out), self.__gjac_psi.ptr, self.__gjac_psi_prev.ptr,
+                 self.__gjac_A.ptr if withA else None, self.__gjac_A_prev.ptr if withA else None,
+                 self._beta_ptr(2), ctypes.byref(F))
+        v._psi._gdata = psi_out
+        if withA:
+            v._vp._gdata = ab_out
+        return F.value
 
     def __commit_beta(self):
         nat.call("glx_memcpy_d2d", self._beta_ptr(0), self._beta_ptr(2), 16)
@@ -222,28 +261,31 @@ class CG:
         for i in range(n_iter):
             if i > 0:
                 self.__commit_beta()
-            self._axmy_beta(self.__gdir_psi, self.__gjac_psi, self.__gdir_psi, 0)
-            psi = self.vars.order_parameter_h()
+            cd = self.cfg.dtype
             if self.line_search == 'device':
-                if withA:
-                    self._axmy_beta(self.__gdir_A, self.__gjac_A, self.__gdir_A, 1)
-                self.__device_line_step(withA)
-                if withA:
-                    self.__gjac_A, self.__gjac_A_prev = self.__gjac_A_prev, self.__gjac_A
+                self.__direction_step(withA)
             elif withA:
-                self._axmy_beta(self.__gdir_A, self.__gjac_A, self.__gdir_A, 1)
-                self._free_energy_conjgrad_coef(self.__gdir_psi, self.__gdir_A)
+                out = (ctypes.c_double * 17)()
+                self.__direction_step(True, out)
+                g = np.array(out[:])
+                c = np.zeros((5, 5), dtype=cd)
+                c[0, :], c[1, :], c[2, :] = g[0:5], g[5:10], g[10:15]
+                c[3, 0], c[4, 0] = g[15], g[16]
+                self.__c = c
                 alpha_psi, alpha_A = self._cg_alpha_min()
-                ab = self.vars.vector_potential_h()
-                self._axpy(self.__gdir_psi, psi, psi, self.cfg.dtype(alpha_psi))
-                self._axpy(self.__gdir_A, ab, ab, self.cfg.dtype(alpha_A))
-                self.__gjac_A, self.__gjac_A_prev = self.__gjac_A_prev, self.__gjac_A
+                self.__galpha.set(np.array([float(cd(alpha_psi)), float(cd(alpha_A))]))
             else:
-                self._free_energy_conjgrad_coef_psi(self.__gdir_psi)
-                alpha0 = self._cg_alpha_psi_min()
-                self._axpy(self.__gdir_psi, psi, psi, self.cfg.dtype(alpha0))
+                out = (ctypes.c_double * 5)()
+                self.__direction_step(False, out)
+                self.__c = np.zeros(5, dtype=cd) if self.__c.ndim != 1 else self.__c
+                self.__c[:] = out[:]
+                self.__galpha.set(np.array([float(cd(self._cg_alpha_psi_min())), 0.0]))
+            if withA:
+                self.__gjac_A, self.__gjac_A_prev = self.__gjac_A_prev, self.__gjac_A
             self.__gjac_psi, self.__gjac_psi_prev = self.__gjac_psi_prev, self.__gjac_psi
-            E0 = self.cfg.dtype(self.__gradient_pass(withA))
+            # the update with this iteration's alphas, then the gradient,
+            # energy and Polak-Ribiere sums of the updated state
+            E0 = cd(self.__update_gradient_pass(withA))
             self.cg_energies.append(E0)
             if i > 0 and np.abs(self.cg_energies[i] / self.cg_energies[i - 1] - 1.0) < self.__convergence_rtol:
                 break

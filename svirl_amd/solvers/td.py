@@ -42,8 +42,6 @@ class TD:
         cfg.stop_criterion_vector_potential = cfg.dtype(cfg.stop_criterion_vector_potential)
         self.Nt = self.dt = self.T = None
         self.NtA = self.dtA = self.TA = None
-        self._P = None
-        self._Q = None
         self._rhs_psi = None
         self._rhs_ab = None
         self.sweeps = []          # (S_psi, S_A) per time step
@@ -90,16 +88,10 @@ class TD:
                 print('Warning (vector potential):  dt/(sigma*min(dx,dy)) = %g is too large' % d)
 
     # -- buffers ---------------------------------------------------------------------
-    def _rotation(self, garray, bufs_attr):
-        """Three same-sized device buffers, the first being the GArray's
-        current one; returns (buffers, index of the current one)."""
-        cur = garray.get_d_obj()
-        bufs = getattr(self, bufs_attr)
-        if bufs is None or all(b is not cur for b in bufs):
-            bufs = [cur, DeviceArray(cur.size, cur.dtype), DeviceArray(cur.size, cur.dtype)]
-            setattr(self, bufs_attr, bufs)
-        idx = [k for k, b in enumerate(bufs) if b is cur][0]
-        return bufs, idx
+    def _rotation(self, garray):
+        """The field's three rotation buffers and the index of the current
+        one (GArray.device_pool, shared with the CG)."""
+        return garray.device_pool()
 
     def _default_wg(self):
         """Persistent grid: two workgroups per CU (the measured optimum at
@@ -128,8 +120,8 @@ class TD:
         v = self.vars
         v._psi.sync_d()
         v._vp.sync_d()
-        P, pc = self._rotation(v._psi, '_P')
-        Q, qc = self._rotation(v._vp, '_Q')
+        P, pc = self._rotation(v._psi)
+        Q, qc = self._rotation(v._vp)
         p = self.params
         lp = float(p.order_parameter_Langevin_coefficient)
         lA = float(p.vector_potential_Langevin_coefficient)

@@ -154,6 +154,20 @@ class GArray:
             self.sync()
         return self._gdata
 
+    def device_pool(self):
+        """Three same-sized device buffers, one of them the current device
+        copy, and the index of that one. The TD solver rotates the field
+        among them and the CG writes its updated state into one that is not
+        current, so moving between solvers allocates nothing and never
+        writes the buffer a kernel reads. A device copy from elsewhere
+        starts a new set."""
+        cur = self._gdata
+        pool = getattr(self, '_pool', None)
+        if pool is None or all(b is not cur for b in pool):
+            pool = [cur, DeviceArray(cur.size, cur.dtype), DeviceArray(cur.size, cur.dtype)]
+            self._pool = pool
+        return pool, [k for k, b in enumerate(pool) if b is cur][0]
+
     def get_vec_h(self, sync=True):
         if sync:
             self.sync()

@@ -342,6 +342,130 @@ def test_cg_kernels(prec, shape):
     assert np.allclose(z.get(), 0.7 * X - Y, rtol=1e-6, atol=1e-6)
 
 
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("shape", [(6, 5), (64, 9), (130, 33)])
+@pytest.mark.parametrize("withA", [True, False])
+def test_cg_direction_step_equals_separate_passes(prec, shape, withA):
+    """glx_cg_direction_step (direction update fused into the coefficient
+    pass) against glx_axmy_dev followed by glx_cg_coef(_psi): the same new
+    directions and coefficients (FP64 to the bit); the device form gives
+    glx_cg_line_step's alphas on the separately updated direction and leaves
+    the state alone."""
+    rs, cs = _types(prec)
+    rng, g, mt, psi, ab, em = case(41 + shape[0], *shape)
+    ae = (0.1 * rng.normal(size=g.Nab)).astype(rs)
+    dold = (0.3 * (rng.normal(size=g.N) + 1j * rng.normal(size=g.N))).astype(cs)
+    jac = (0.2 * (rng.normal(size=g.N) + 1j * rng.normal(size=g.N))).astype(cs)
+    daold = (0.05 * rng.normal(size=g.Nab)).astype(rs)
+    jacA = (0.04 * rng.normal(size=g.Nab)).astype(rs)
+    k2, H, eps = (9.0 if withA else -1.0), 0.15, 0.9
+    beta = nat.DeviceArray.from_host(np.array([0.37, 0.61, 0.0, 0.0]), np.float64)
+    d_mt = D(mt.astype(np.uint8), np.uint8)
+    d_ae = D(ae, rs)
+    nc = 17 if withA else 5
+
+    def run(fused, device):
+        d_psi, d_ab = D(psi.astype(cs), cs), D(ab.astype(rs), rs)
+        d_do, d_jac, d_dao, d_jA = D(dold, cs), D(jac, cs), D(daold, rs), D(jacA, rs)
+        out_d, out_a = nat.DeviceArray.zeros(g.N, cs), nat.DeviceArray.zeros(g.Nab, rs)
+        alpha = nat.DeviceArray.zeros(2, np.float64)
+        c = None if device else (ctypes.c_double * nc)()
+        if fused:
+            nat.call("glx_cg_direction_step", prec, grid(g), k2, eps, H, d_mt.ptr, d_psi.ptr, d_ae.ptr, d_ab.ptr,
+                     d_do.ptr, d_jac.ptr, out_d.ptr, d_dao.ptr if withA else None, d_jA.ptr if withA else None,
+                     out_a.ptr if withA else None, beta.ptr, c, None if c is not None else alpha.ptr)
+            dn, dan = out_d, out_a
+        else:
+            nat.call("glx_axmy_dev", prec, d_do.ptr, d_jac.ptr, d_do.ptr, beta.ptr, 2 * g.N)
+            if withA:
+                nat.call("glx_axmy_dev", prec, d_dao.ptr, d_jA.ptr, d_dao.ptr, ctypes.c_void_p(beta.ptr + 8), g.Nab)
+            dn, dan = d_do, d_dao
+            if device:
+                nat.call("glx_cg_line_step", prec, grid(g), k2, eps, H, d_mt.ptr, d_psi.ptr, dn.ptr, d_ae.ptr,
+                         d_ab.ptr, dan.ptr if withA else None, alpha.ptr)
+            elif withA:
+                nat.call("glx_cg_coef", prec, grid(g), k2, eps, H, d_mt.ptr, d_psi.ptr, dn.ptr, d_ae.ptr, d_ab.ptr,
+                         dan.ptr, c)
+            else:
+                nat.call("glx_cg_coef_psi", prec, grid(g), k2, eps, H, d_mt.ptr, d_psi.ptr, dn.ptr, d_ae.ptr,
+                         d_ab.ptr, c)
+        return (dn.get(), dan.get() if withA else None, None if c is None else np.array(c[:]),
+                alpha.get(), d_psi.get(), d_ab.get())
+
+    for device in (False, True):
+        a, b = run(True, device), run(False, device)
+        f64 = prec == nat.GLX_F64
+        same = (lambda x, y: np.array_equal(x, y)) if f64 else (lambda x, y: np.allclose(x, y, rtol=1e-6, atol=1e-6))
+        assert same(a[0], b[0])
+        if withA:
+            assert same(a[1], b[1])
+        if not device:
+            assert np.allclose(a[2], b[2], rtol=1e-12 if f64 else 1e-5, atol=0 if f64 else 1e-5 * np.max(np.abs(b[2])))
+        else:
+            assert np.allclose(a[3], b[3], rtol=1e-10 if f64 else 1e-4)
+            assert np.array_equal(a[4], psi.astype(cs)) and np.array_equal(a[5], ab.astype(rs))
+    with pytest.raises(nat.GLXError):
+        dd = D(dold, cs)
+        nat.call("glx_cg_direction_step", prec, grid(g), k2, eps, H, None, dd.ptr, None, D(ab.astype(rs), rs).ptr,
+                 dd.ptr, D(jac, cs).ptr, dd.ptr, None, None, None, beta.ptr, (ctypes.c_double * 5)(), None)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("shape", [(6, 5), (64, 9), (130, 33)])
+@pytest.mark.parametrize("withA", [True, False])
+def test_cg_update_gradient_pass_equals_separate_passes(prec, shape, withA):
+    """glx_cg_update_gradient_pass against glx_axpy_dev-style updates
+    (glx_axpy with the same alphas) followed by glx_cg_gradient_pass: the
+    same updated state, Jacobians, energy and staged betas (FP64 to the
+    bit); the inputs are left untouched."""
+    rs, cs = _types(prec)
+    rng, g, mt, psi, ab, em = case(51 + shape[0], *shape)
+    ae = (0.1 * rng.normal(size=g.Nab)).astype(rs)
+    dpsi = (0.3 * (rng.normal(size=g.N) + 1j * rng.normal(size=g.N))).astype(cs)
+    dab = (0.05 * rng.normal(size=g.Nab)).astype(rs)
+    jprev = (0.2 * (rng.normal(size=g.N) + 1j * rng.normal(size=g.N))).astype(cs)
+    jAprev = (0.04 * rng.normal(size=g.Nab)).astype(rs)
+    k2, H, eps = (9.0 if withA else -1.0), 0.15, 0.0
+    al = np.array([0.23, -0.41]) if withA else np.array([0.23, 0.0])
+    d_mt, d_ae, d_em = D(mt.astype(np.uint8), np.uint8), D(ae, rs), D(em.astype(rs), rs)
+    d_al = D(al, np.float64)
+
+    def run(fused):
+        d_psi, d_ab = D(psi.astype(cs), cs), D(ab.astype(rs), rs)
+        d_dp, d_da = D(dpsi, cs), D(dab, rs)
+        jac, jacA = nat.DeviceArray.zeros(g.N, cs), nat.DeviceArray.zeros(g.Nab, rs)
+        jp, jAp = D(jprev, cs), D(jAprev, rs)
+        beta = nat.DeviceArray.zeros(2, np.float64)
+        F = ctypes.c_double()
+        if fused:
+            p_out, a_out = nat.DeviceArray.zeros(g.N, cs), nat.DeviceArray.zeros(g.Nab, rs)
+            nat.call("glx_cg_update_gradient_pass", prec, grid(g), k2, eps, d_em.ptr, H, d_mt.ptr, d_psi.ptr, d_ae.ptr,
+                     d_ab.ptr, d_dp.ptr, d_da.ptr if withA else None, d_al.ptr, p_out.ptr,
+                     a_out.ptr if withA else None, jac.ptr, jp.ptr, jacA.ptr if withA else None,
+                     jAp.ptr if withA else None, beta.ptr, ctypes.byref(F))
+            assert np.array_equal(d_psi.get(), psi.astype(cs)) and np.array_equal(d_ab.get(), ab.astype(rs))
+            st_psi, st_ab = p_out.get(), (a_out.get() if withA else d_ab.get())
+        else:
+            nat.call("glx_axpy", prec, d_dp.ptr, d_psi.ptr, d_psi.ptr, float(rs(al[0])), 2 * g.N)
+            if withA:
+                nat.call("glx_axpy", prec, d_da.ptr, d_ab.ptr, d_ab.ptr, float(rs(al[1])), g.Nab)
+            nat.call("glx_cg_gradient_pass", prec, grid(g), k2, eps, d_em.ptr, H, d_mt.ptr, d_psi.ptr, d_ae.ptr,
+                     d_ab.ptr, jac.ptr, jp.ptr, jacA.ptr if withA else None, jAp.ptr if withA else None, beta.ptr,
+                     ctypes.byref(F))
+            st_psi, st_ab = d_psi.get(), d_ab.get()
+        return st_psi, st_ab, jac.get(), (jacA.get() if withA else None), F.value, beta.get()
+
+    a, b = run(True), run(False)
+    f64 = prec == nat.GLX_F64
+    for x, y in zip(a, b):
+        if x is None:
+            continue
+        if f64:
+            assert np.array_equal(x, y)
+        else:
+            assert np.allclose(x, y, rtol=1e-5, atol=1e-5 * (1 + np.max(np.abs(y))))
+
+
 @pytest.mark.parametrize("n", [1, 5, 1000, 123457, 2048 * 2048 - 3])
 def test_reductions(n):
     """tests/at_reduction.py: sum and 5-vector sums vs numpy, atol 1e-10."""
