@@ -26,8 +26,8 @@ template <typename R> struct CGInst {
     const uint8_t *mt;        // per cell or NULL
     cplx<R> *psi;             // state
     R *ab;                    // state (NULL when kappa = inf)
-    cplx<R> *jac[2], *dir;    // psi gradient (two, alternating) and direction
-    R *jacA[2], *dirA;        // A gradient and direction
+    cplx<R> *jac[2], *dir[2]; // psi gradient and direction (two each, alternating by iteration)
+    R *jacA[2], *dirA[2];     // A gradient and direction
 };
 
 // Per-instance state, 8 doubles: iteration, stop flag, beta_psi, beta_A,
@@ -44,8 +44,15 @@ template <int L> __device__ __forceinline__ void block_sums(double (&acc)[L], do
     }
 }
 
+// FP32: at most 128 VGPRs, so that four workgroups share a CU and a
+// 1024-instance batch is resident at once on 256 CUs (at the 164 the
+// kernel would otherwise take, three fit and a launch runs in two rounds).
+// FP64 keeps the default bound (capping it spills the line search).
+template <typename R> struct BatchCGBound { static constexpr int MIN_WG = 1; };
+template <> struct BatchCGBound<float> { static constexpr int MIN_WG = 4; };
+
 template <typename R, int L>
-__global__ void __launch_bounds__(256) k_batch_cg(const CGInst<R> *insts, double *state, int n, int iters,
+__global__ void __launch_bounds__(256, BatchCGBound<R>::MIN_WG) k_batch_cg(const CGInst<R> *insts, double *state, int n, int iters,
                                                   int max_iters, double rtol) {
     __shared__ double lds[8];
     __shared__ double red[L];
@@ -71,6 +78,9 @@ __global__ void __launch_bounds__(256) k_batch_cg(const CGInst<R> *insts, double
             const bool odd = (itn & 1) != 0;
             cplx<R> *jac = odd ? I.jac[1] : I.jac[0], *jprev = odd ? I.jac[0] : I.jac[1];
             R *jacA = odd ? I.jacA[1] : I.jacA[0], *jprevA = odd ? I.jacA[0] : I.jacA[1];
+            // the direction of the previous iteration and this iteration's
+            cplx<R> *dold = odd ? I.dir[1] : I.dir[0], *dnew = odd ? I.dir[0] : I.dir[1];
+            R *dAold = odd ? I.dirA[1] : I.dirA[0], *dAnew = odd ? I.dirA[0] : I.dirA[1];
             // 1. gradients, PR sums and the gradient norm
             double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // |g|^2, num_psi, den_psi, num_A, den_A
             for (long m = threadIdx.x; m < g.N; m += blockDim.x) {
@@ -126,16 +136,9 @@ __global__ void __launch_bounds__(256) k_batch_cg(const CGInst<R> *insts, double
             }
             __syncthreads();
             if (!go) break;
-            // 2. directions
-            const R bpsi = R(bc[0]), bA = R(bc[1]);
-            for (long m = threadIdx.x; m < g.N; m += blockDim.x) {
-                const cplx<R> d = ld(I.dir, m), gv = ld(jac, m);
-                st(I.dir, m, cplx<R>{bpsi * d.re - gv.re, bpsi * d.im - gv.im});
-            }
-            if (withA)
-                for (long m = threadIdx.x; m < g.Nab; m += blockDim.x) I.dirA[m] = bA * I.dirA[m] - jacA[m];
-            __syncthreads();
-            // 3. line-search coefficients
+            // 2. line-search coefficients of the new direction d = beta d_old
+            // - g, which the pass forms where it reads it and writes to dnew
+            const DirUpdate<R> U{dold, jac, dAold, jacA, dnew, dAnew, R(bc[0]), R(bc[1])};
             double cacc[L];
 #pragma unroll
             for (int c = 0; c < L; ++c) cacc[c] = 0.0;
@@ -143,8 +146,8 @@ __global__ void __launch_bounds__(256) k_batch_cg(const CGInst<R> *insts, double
                 R C[L];
 #pragma unroll
                 for (int c = 0; c < L; ++c) C[c] = R(0);
-                coef_node<R, L>(g, q, withA ? I.kappa2 : R(-1), I.eps, I.eps_map, I.H, I.mt, I.psi, I.dir,
-                                (const R *)nullptr, I.ab, I.dirA, m, C);
+                coef_node<R, L, true>(g, q, withA ? I.kappa2 : R(-1), I.eps, I.eps_map, I.H, I.mt, I.psi, nullptr,
+                                      (const R *)nullptr, I.ab, nullptr, m, C, &U);
 #pragma unroll
                 for (int c = 0; c < L; ++c) cacc[c] += double(C[c]);
             }
@@ -171,14 +174,14 @@ __global__ void __launch_bounds__(256) k_batch_cg(const CGInst<R> *insts, double
                 bc[3] = y;
             }
             __syncthreads();
-            // 4. update
+            // 3. update
             const R apsi = R(bc[2]), aA = R(bc[3]);
             for (long m = threadIdx.x; m < g.N; m += blockDim.x) {
-                const cplx<R> p = ld(I.psi, m), d = ld(I.dir, m);
+                const cplx<R> p = ld(I.psi, m), d = ld(dnew, m);
                 st(I.psi, m, cplx<R>{apsi * d.re + p.re, apsi * d.im + p.im});
             }
             if (withA)
-                for (long m = threadIdx.x; m < g.Nab; m += blockDim.x) I.ab[m] = aA * I.dirA[m] + I.ab[m];
+                for (long m = threadIdx.x; m < g.Nab; m += blockDim.x) I.ab[m] = aA * dAnew[m] + I.ab[m];
             __syncthreads();
             if (threadIdx.x == 0) sv[CG_IT] = double(itn + 1);
             __syncthreads();

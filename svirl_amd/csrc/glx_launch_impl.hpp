@@ -533,13 +533,13 @@ template <typename R, int LL>
 static int batch_cg_form(const Env &e, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,
                          int32_t per_launch) {
     const bool withA = LL == 17;
-    // scratch per instance: jac[2], dir (N complex each) and, with A, jacA[2], dirA (Nab reals each)
+    // scratch per instance: jac[2], dir[2] (N complex each) and, with A, jacA[2], dirA[2] (Nab reals each)
     std::vector<size_t> off(n + 1, 0);
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     for (int i = 0; i < n; ++i) {
         const glx_grid &g = inst[i].g;
         const size_t N = (size_t)g.Nx * g.Ny, Nab = (size_t)(g.Nx - 1) * g.Ny + (size_t)g.Nx * (g.Ny - 1);
-        off[i + 1] = off[i] + 3 * al(N * sizeof(cplx<R>)) + (withA ? 3 * al(Nab * sizeof(R)) : 0);
+        off[i + 1] = off[i] + 4 * al(N * sizeof(cplx<R>)) + (withA ? 4 * al(Nab * sizeof(R)) : 0);
     }
     const size_t s_inst = al(sizeof(CGInst<R>) * n), s_state = al(sizeof(double) * 8 * n);
     char *blk = nullptr;
@@ -565,9 +565,9 @@ static int batch_cg_form(const Env &e, glx_batch_cg_instance *inst, int32_t n, i
         c.ab = withA ? (R *)I.ab : nullptr;
         char *w = work + off[i];
         for (int k = 0; k < 2; ++k) { c.jac[k] = (cplx<R> *)w; w += al(N * sizeof(cplx<R>)); }
-        c.dir = (cplx<R> *)w; w += al(N * sizeof(cplx<R>));
+        for (int k = 0; k < 2; ++k) { c.dir[k] = (cplx<R> *)w; w += al(N * sizeof(cplx<R>)); }
         for (int k = 0; k < 2; ++k) { c.jacA[k] = withA ? (R *)w : nullptr; if (withA) w += al(Nab * sizeof(R)); }
-        c.dirA = withA ? (R *)w : nullptr;
+        for (int k = 0; k < 2; ++k) { c.dirA[k] = withA ? (R *)w : nullptr; if (withA) w += al(Nab * sizeof(R)); }
     }
     st = hipMemcpyAsync(d_inst, h.data(), sizeof(CGInst<R>) * n, hipMemcpyHostToDevice, e.stream);
     if (st == hipSuccess) st = hipMemsetAsync(d_state, 0, sizeof(double) * 8 * n + off[n], e.stream);
