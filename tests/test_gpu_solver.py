@@ -325,6 +325,25 @@ def test_snapshot_resume_is_bit_exact(tmp_path):
     assert a.solve._td.sweeps[-4:] == b.solve._td.sweeps[-4:]
 
 
+@pytest.mark.parametrize("wg", [1, 4])
+def test_td_fixed_vortices_multi_workgroup(wg):
+    """The fixed-source A solve and the per-step hooks on a grid of several
+    strips, run by one workgroup and by a team of four (team barrier), in
+    FP64 against the oracle: sweep counts exact, fields within 1e-10."""
+    gl = GLSolver(Nx=131, Ny=70, dx=0.5, dy=0.5, order_parameter='random', random_level=0.3, random_seed=7,
+                  gl_parameter=2.0, normal_conductivity=20.0, homogeneous_external_field=0.05,
+                  fixed_vortices=([12.3, 40.1, 51.7], [10.2, 20.6, 27.9], [1, 1, -1]), phase_lock_radius=0.7,
+                  dtype=np.float64)
+    gl.solve._init_td()
+    gl.solve._td.wg = wg
+    st = oracle_of(gl)
+    gl.solve.td(dt=0.1, Nt=4)
+    st.td(dt=0.1, Nt=4)
+    assert gl.solve._td.sweeps == st.sweeps
+    assert rel(gl_psi(gl), st.psi) < 1e-10 and rel(gl_ab(gl), st.ab) < 1e-10
+    assert rel(_vpi_of(gl), st.vpi) < 1e-10
+
+
 def test_snapshot_resume_with_fixed_vortices(tmp_path):
     """The TD steps change vpi (td.py:256-325), so the snapshot carries it:
     a resumed run matches the uninterrupted one to the bit, vpi included."""
