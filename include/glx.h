@@ -135,7 +135,9 @@ typedef struct glx_td_params {
  * Replaces the host loops TD.__iterate / __iterate_order_parameter_gpu /
  * __iterate_vector_potential_gpu, svirl/solvers/td.py:157-366, with the
  * A-source aliasing of td.py:264-303 kept. wg / wg_size select the
- * persistent grid (0 = library default). */
+ * persistent grid of both solves (wg 0: the library default, two
+ * workgroups per CU for the psi solve and three for the A solve, within
+ * the co-resident limit). */
 int glx_td_run(int prec, glx_grid g, const glx_td_params *p, const void *eps_map,
                const uint8_t *mt, void *const psi_bufs[3], int32_t *psi_cur,
                void *const ab_bufs[3], int32_t *ab_cur, void *rhs_psi, void *rhs_ab,

@@ -168,7 +168,7 @@ template <typename R> int L<R>::td_max_grid(const Env &e, int32_t wg_size, int32
 template <typename R, bool G>
 static int td_run_form(const Env &e, glx_grid g, const glx_td_params *p, const void *eps_map, const uint8_t *mt,
                        void *const P[3], int32_t *pc, void *const Q[3], int32_t *qc, void *rhs_psi, void *rhs_ab,
-                       int32_t nsteps, int32_t *sweeps, int32_t wg) {
+                       int32_t nsteps, int32_t *sweeps, int32_t wg, int32_t wg_a) {
 
     TDSolveArgs<R> T{};
     T.psi = make_psi_args<R>(g, p->dt, p->eps, eps_map, mt, nullptr, p->stop_psi, p->langevin_psi);
@@ -224,11 +224,11 @@ static int td_run_form(const Env &e, glx_grid g, const glx_td_params *p, const v
             bool fixed = false;
             if constexpr (!G) {
                 if (T.a_src) {
-                    hipLaunchKernelGGL((k_td_a_solve<R, G, true>), dim3((unsigned)wg), dim3(256), 0, e.stream, T);
+                    hipLaunchKernelGGL((k_td_a_solve<R, G, true>), dim3((unsigned)wg_a), dim3(256), 0, e.stream, T);
                     fixed = true;
                 }
             }
-            if (!fixed) hipLaunchKernelGGL((k_td_a_solve<R, G>), dim3((unsigned)wg), dim3(256), 0, e.stream, T);
+            if (!fixed) hipLaunchKernelGGL((k_td_a_solve<R, G>), dim3((unsigned)wg_a), dim3(256), 0, e.stream, T);
             GLX_LAUNCH_CHECK("k_td_a_solve");
             if (prof) (void)hipEventRecord(e.tev[nev++], e.stream);
         }
@@ -296,7 +296,14 @@ int L<R>::td_run(const Env &e, glx_grid g, const glx_td_params *p, const void *e
     int32_t maxg = 0;
     int rc = td_max_grid(e, wg_size, &maxg);
     if (rc) return rc;
-    if (wg <= 0) wg = std::min(maxg, 2 * e.num_cus);  // two workgroups per CU: the measured optimum
+    // Default grids, the measured optima at 4097^2: two workgroups per CU
+    // for the psi solve, three for the A solve (maxg is the co-resident
+    // limit of every solve kernel); an explicit wg applies to both.
+    int32_t wg_a = wg;
+    if (wg <= 0) {
+        wg = std::min(maxg, 2 * e.num_cus);
+        wg_a = std::min(maxg, 3 * e.num_cus);
+    }
     if (wg > maxg) return set_error(GLX_E_ARG, "glx_td_run: grid %d exceeds co-resident limit %d", wg, maxg);
     const bool evolves_A = p->solveA && p->eqn != 1;
     if (p->a_src && (p->a_src == Q[0] || p->a_src == Q[1] || p->a_src == Q[2]))
@@ -304,8 +311,8 @@ int L<R>::td_run(const Env &e, glx_grid g, const glx_td_params *p, const void *e
     // a fixed source is given in the plain form, so the run keeps that form
     if constexpr (RunForm<R>::has_gauge)
         if (evolves_A && !p->a_src)
-            return td_run_form<R, true>(e, g, p, eps_map, mt, P, pc, Q, qc, rhs_psi, rhs_ab, nsteps, sweeps, wg);
-    return td_run_form<R, false>(e, g, p, eps_map, mt, P, pc, Q, qc, rhs_psi, rhs_ab, nsteps, sweeps, wg);
+            return td_run_form<R, true>(e, g, p, eps_map, mt, P, pc, Q, qc, rhs_psi, rhs_ab, nsteps, sweeps, wg, wg_a);
+    return td_run_form<R, false>(e, g, p, eps_map, mt, P, pc, Q, qc, rhs_psi, rhs_ab, nsteps, sweeps, wg, wg_a);
 }
 
 // ---- batched TD ---------------------------------------------------------------

@@ -94,8 +94,10 @@ class TD:
         return garray.device_pool()
 
     def _default_wg(self):
-        """Persistent grid: two workgroups per CU (the measured optimum at
-        4097^2), fewer for small grids (about 32 rows per wave)."""
+        """Persistent grid: fewer workgroups for small grids (about 32 rows
+        per wave); 0 (the library's default: two workgroups per CU for the
+        psi solve and three for the A solve, the measured optima at 4097^2)
+        once a grid fills two workgroups per CU."""
         cfg = self.cfg
         strips = -(-int(cfg.Nx) // 62)
         rows = strips * int(cfg.Ny)
@@ -103,7 +105,8 @@ class TD:
         maxg = ctypes.c_int32()
         nat.call("glx_td_max_grid", self.par.prec, 256, ctypes.byref(maxg))
         cus = nat.device_info()["num_cus"]
-        return int(max(1, min(maxg.value, 2 * cus, -(-waves // 4))))
+        wg = int(max(1, min(maxg.value, -(-waves // 4))))
+        return 0 if wg >= min(maxg.value, 2 * cus) else wg
 
     def _launch_steps(self):
         if self.steps_per_launch:
