@@ -224,6 +224,64 @@ def cpu_baseline(L, dt, steps):
     return g.N * steps / t, t, g.N, st.sweeps[1:]
 
 
+def _cpu_batch_task(spec):
+    """One oracle task of the batched baselines (runs in a worker or
+    inline): ('td', Nx, Ny, H, kappa, sigma, seed, n) -> n TD steps of a
+    Config 4 instance; ('cg', ...) -> n CG iterations of a Config 5
+    instance. Returns (nodes * n, seconds)."""
+    from oracle import gl_oracle as O
+    from svirl_amd import workloads
+    kind, Nx, Ny, H, kappa, sigma, seed, n = spec
+    g = O.Geom(Nx, Ny, 0.5, 0.5)
+    em = workloads.pinning_field(Nx, Ny, 0.5, seed=seed)
+    st = O.OracleState(g, O.random_psi(g.N, 1.0, seed=seed), O.symmetric_gauge(g, H), kappa=kappa, sigma=sigma, H=H,
+                       eps_map=np.ascontiguousarray(em.T).ravel())
+    t0 = time.perf_counter()
+    if kind == 'td':
+        st.td(dt=0.1, Nt=n)
+    else:
+        st.cg(n_iter=n)
+    return g.N * n, time.perf_counter() - t0
+
+
+def cpu_batch_baselines(workers, n_td=16, n_cg=8, cg_iters=3):
+    """The FP64 oracle on samples of Config 4 (one TD step of each of its
+    first n_td instances, drawn as workloads.config4 draws them) and Config
+    5 (cg_iters CG iterations of n_cg instances), on one core and on a pool
+    of `workers` processes. Runs before the GPU is touched, so the pool is
+    forked from a process without GPU state."""
+    import multiprocessing as mp
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    rng = np.random.default_rng(40)  # workloads.config4's draws
+    td = []
+    for k in range(n_td):
+        Lx, Ly = float(rng.choice((32, 64, 128, 256, 512))), float(rng.choice((32, 64, 128, 256, 512)))
+        H = float(rng.uniform(0.02, 0.5))
+        kappa = float(rng.choice([2.0, 5.0, np.inf]))
+        td.append(('td', int(round(Lx / 0.5)) + 1, int(round(Ly / 0.5)) + 1, H, kappa, 200.0, 40 * 1000 + k, 1))
+    rng = np.random.default_rng(50)  # workloads.config5's draws
+    cg = [('cg', 257, 257, float(rng.uniform(0.05, 0.3)), 5.0, 1.0, 50 * 10000 + k, cg_iters) for k in range(n_cg)]
+    out = {}
+    with mp.get_context("fork").Pool(workers) as pool:
+        for name, specs, unit in (("config4", td, "node-steps/s"), ("config5", cg, "node-iterations/s")):
+            t0 = time.perf_counter()
+            r1 = [_cpu_batch_task(s) for s in specs]
+            t1 = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            rp = pool.map(_cpu_batch_task, specs, chunksize=1)
+            tp = time.perf_counter() - t0
+            work = sum(w for w, _ in r1)
+            sample = (f"FP64 NumPy oracle, one TD step of each of the first {len(specs)} Config 4 instances "
+                      f"({work} nodes)" if name == "config4" else
+                      f"FP64 NumPy oracle, {cg_iters} CG iterations (reference line search) of {len(specs)} "
+                      f"Config 5 instances of 257^2")
+            out[name] = {"value": work / t1, "unit": unit, "cores": 1, "kind": "port",
+                         "pool_value": sum(w for w, _ in rp) / tp, "pool_cores": workers, "sample": sample}
+            log(f"[cpu] {name}: {work / t1:.4g} {unit} on 1 core, {sum(w for w, _ in rp) / tp:.4g} on {workers} "
+                f"processes ({sample})")
+    return out
+
+
 def pmc_traffic():
     """Measured HBM bytes per node per psi sweep of k_td_psi_solve<float>
     from the committed counter summary (FETCH_SIZE + WRITE_SIZE, scaled by
@@ -260,8 +318,14 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each instance runs on a "
-              "single GPU; see DESIGN.md for how the path would shard).", file=sys.stderr)
+              "single GPU; a multi-GPU run would split the 4097^2 grid into row slabs, one per GPU, or "
+              "the batched instances among GPUs).", file=sys.stderr)
         sys.exit(2)
+
+    cpu_batch = None
+    if not args.skip_cpu:
+        # before any GPU call: the pool's workers are forked from this process
+        cpu_batch = cpu_batch_baselines(max(1, min(16, os.cpu_count() or 1)))
 
     from svirl_amd import _native as nat
     from svirl_amd import workloads
@@ -382,6 +446,8 @@ def main():
                             "fp32_cells_flagged": r32["vortex_cells"]},
         "cpu_baseline": cpu,
     }
+    if cpu_batch:
+        out["cpu_baseline_batched"] = cpu_batch
     if batch:
         out["batch"] = batch
     if batch_cg:
