@@ -12,6 +12,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("GLX_LIB", os.path.join(_HERE, "_lib", "libglx.so"))
 
 GLX_F64, GLX_F32 = 0, 1
+GLX_E_ARG = -1       # error codes of include/glx.h
+GLX_E_HIP = -2
+GLX_E_NOINIT = -3
 GLX_E_TIMEOUT = -4
 GLX_E_CAPACITY = -5
 

@@ -628,3 +628,47 @@ def test_capacity_error():
         nat.call("glx_vortex_cells", nat.GLX_F64, grid(g), d_psi.ptr, d_ab.ptr, None, 0.0, rec.ptr, 1,
                  ctypes.byref(cnt))
     assert e.value.code == nat.GLX_E_CAPACITY and cnt.value == 2
+
+
+def test_argument_validation():
+    """Host-side validation of the C ABI: each call is rejected with
+    GLX_E_ARG and a message before anything is launched (no kernel runs)."""
+    def rejects(name, *args):
+        with pytest.raises(nat.GLXError) as e:
+            nat.call(name, *args)
+        assert e.value.code == nat.GLX_E_ARG, (name, e.value)
+        assert str(e.value)
+
+    F = ctypes.c_double()
+    x = nat.DeviceArray.zeros(64, np.float64)
+    big = nat.Grid(40000, 40000, 0.5, 0.5)   # more than 2^30 nodes
+    for g in (big, nat.Grid(1, 8, 0.5, 0.5), nat.Grid(8, 8, 0.0, 0.5), nat.Grid(8, 8, 0.5, -1.0)):
+        rejects("glx_free_energy", nat.GLX_F64, g, -1.0, 1.0, None, 0.0, None, x.ptr, None, x.ptr, ctypes.byref(F))
+    g = nat.Grid(8, 8, 0.5, 0.5)
+    rejects("glx_free_energy", nat.GLX_F64, g, -1.0, 1.0, None, 0.0, None, None, None, x.ptr, ctypes.byref(F))
+    rejects("glx_free_energy", 7, g, -1.0, 1.0, None, 0.0, None, x.ptr, None, x.ptr, ctypes.byref(F))  # precision
+    out = ctypes.c_double()
+    rejects("glx_sum", nat.GLX_F64, x.ptr, 64, 100, ctypes.byref(out))   # block size not a multiple of 64
+    rejects("glx_sum", nat.GLX_F64, x.ptr, -1, 0, ctypes.byref(out))
+    # persistent TD: bad buffer index, negative step count, grid above the co-resident limit
+    p = _td_params(0.1, 1.0, -1.0, 1.0, 0.0, 1e-6, 1e-6, False)
+    P = [nat.DeviceArray.zeros(g.Nx * g.Ny, np.complex128) for _ in range(3)]
+    Q = [nat.DeviceArray.zeros(2 * (g.Nx - 1) * g.Ny, np.float64) for _ in range(3)]
+    Pp = (ctypes.c_void_p * 3)(*[b.ptr for b in P])
+    Qp = (ctypes.c_void_p * 3)(*[b.ptr for b in Q])
+    sw = (ctypes.c_int32 * 8)()
+    for pc, qc, nsteps, wg in ((3, 0, 1, 0), (0, -1, 1, 0), (0, 0, -1, 0), (0, 0, 1, 1 << 20)):
+        rejects("glx_td_run", nat.GLX_F64, g, ctypes.byref(p), None, None, Pp, ctypes.byref(ctypes.c_int32(pc)), Qp,
+                ctypes.byref(ctypes.c_int32(qc)), None, None, nsteps, sw, wg, 256)
+    p.a_src = Q[1].ptr  # a fixed source may not be one of the rotating buffers
+    p.solveA = 1
+    p.kappa2 = 4.0
+    rejects("glx_td_run", nat.GLX_F64, g, ctypes.byref(p), None, None, Pp, ctypes.byref(ctypes.c_int32(0)), Qp,
+            ctypes.byref(ctypes.c_int32(0)), None, None, 1, sw, 0, 256)
+    # fused CG passes: outputs may not alias their inputs
+    d, j = nat.DeviceArray.zeros(g.Nx * g.Ny, np.complex128), nat.DeviceArray.zeros(g.Nx * g.Ny, np.complex128)
+    beta = nat.DeviceArray.zeros(4, np.float64)
+    rejects("glx_cg_update_gradient_pass", nat.GLX_F64, g, -1.0, 1.0, None, 0.0, None, P[0].ptr, None, Q[0].ptr,
+            d.ptr, None, beta.ptr, P[0].ptr, None, j.ptr, j.ptr, None, None, beta.ptr, ctypes.byref(F))
+    rejects("glx_vortex_triangulate", big, x.ptr, 1, x.ptr)
+    rejects("glx_bulk_observables", nat.GLX_F64, g, 4.0, 0.1, None, None, None, Q[0].ptr, None, None, x.ptr)
