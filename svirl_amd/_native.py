@@ -235,9 +235,14 @@ class DeviceArray:
             raise ValueError(f"size mismatch: device {self.size}, host {a.size}")
         call("glx_memcpy_h2d", self.ptr, a.ctypes.data, self.nbytes)
 
-    def get(self):
-        out = np.empty(self.size, dtype=self.dtype)
-        call("glx_memcpy_d2h", out.ctypes.data, self.ptr, self.nbytes)
+    def get(self, count=None):
+        """The buffer on the host (its first `count` elements if given)."""
+        n = self.size if count is None else int(count)
+        if n < 0 or n > self.size:
+            raise ValueError(f"count {n} outside [0, {self.size}]")
+        out = np.empty(n, dtype=self.dtype)
+        if n:
+            call("glx_memcpy_d2h", out.ctypes.data, self.ptr, n * self.dtype.itemsize)
         return out
 
     def fill(self, value):

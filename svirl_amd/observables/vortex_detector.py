@@ -56,7 +56,7 @@ class VortexDetector:
                     raise
                 self._capacity = int(count.value) + 64
         n = count.value
-        rec = self._rec.get()[:n * _VREC].reshape(n, _VREC) if n else np.zeros((0, _VREC))
+        rec = self._rec.get(n * _VREC).reshape(n, _VREC) if n else np.zeros((0, _VREC))
         return rec, np.argsort(rec[:, 0], kind='stable')
 
     @property
@@ -72,7 +72,7 @@ class VortexDetector:
                 if self._xyv is None or self._xyv.size < 4 * n:
                     self._xyv = nat.DeviceArray(4 * max(n, self._capacity), np.float64)
                 nat.call("glx_vortex_triangulate", self._grid(), self._rec.ptr, n, self._xyv.ptr)
-                xyv = self._xyv.get()[:4 * n].reshape(n, 4)[order]
+                xyv = self._xyv.get(4 * n).reshape(n, 4)[order]
                 det = xyv[xyv[:, 3] != 0.0, :3]
             else:
                 det = np.zeros((0, 3))
