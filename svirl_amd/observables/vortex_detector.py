@@ -57,7 +57,8 @@ class VortexDetector:
                 self._capacity = int(count.value) + 64
         n = count.value
         rec = self._rec.get(n * _VREC).reshape(n, _VREC) if n else np.zeros((0, _VREC))
-        return rec, np.argsort(rec[:, 0], kind='stable')
+        # cell ids are unique, so any sort gives the reference's ascending order
+        return rec, np.argsort(rec[:, 0])
 
     @property
     def vortices(self):
