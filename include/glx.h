@@ -378,7 +378,10 @@ int glx_cg_direction_step(int prec, glx_grid g, double kappa2, double eps, doubl
  * NULL) ab + alpha_dev[1]*dab (axpy_c/axpy_r, cg.py:296, 535-536, with the
  * alphas read from device memory as doubles and rounded to the working
  * precision), and the updated state written to psi_out / ab_out, buffers
- * other than psi, dpsi, ab and dab, which then hold the state. No
+ * other than psi, dpsi, ab and dab, which then hold the state. F NULL:
+ * nothing waits; the energy is copied to library-owned pinned memory and
+ * glx_cg_energy_wait returns it (work enqueued after this call, such as
+ * the next iteration's glx_cg_direction_step, may run meanwhile). No
  * counterpart in the reference as one call. */
 int glx_cg_update_gradient_pass(int prec, glx_grid g, double kappa2, double eps,
                                 const void *eps_map, double H, const uint8_t *mt, const void *psi,
@@ -386,6 +389,9 @@ int glx_cg_update_gradient_pass(int prec, glx_grid g, double kappa2, double eps,
                                 const double *alpha_dev, void *psi_out, void *ab_out, void *jac,
                                 const void *jac_prev, void *jacA, const void *jacA_prev,
                                 double *beta_next_dev, double *F);
+/* The energy of the last glx_cg_update_gradient_pass called with F NULL,
+ * once its copy has arrived (waits for that copy only). */
+int glx_cg_energy_wait(double *F);
 /* The line search and update of one CG iteration on the device, for the
  * FP32-mixed form: the line-search coefficients (as glx_cg_coef, or
  * glx_cg_coef_psi when dab is NULL; cg.py:184-224, 325-375), their

@@ -97,6 +97,7 @@ _PROTOS = {
     "glx_vortex_triangulate": [Grid, c_void_p, c_i32, c_void_p],
     "glx_cg_direction_step": [c_int, Grid, c_double, c_double, c_double] + [c_void_p] * 13,
     "glx_cg_update_gradient_pass": [c_int, Grid, c_double, c_double, c_void_p, c_double] + [c_void_p] * 15,
+    "glx_cg_energy_wait": [P_double],
     "glx_bulk_observables": [c_int, Grid, c_double, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_void_p],
     "glx_sum": [c_int, c_void_p, c_i64, c_i32, P_double],

@@ -115,6 +115,11 @@ int glx_init(int device) {
     e.sweeps = (int *)p; p += sweeps;
     e.sweeps_cap = SWEEPS_CAP;
     e.num_cus = prop.multiProcessorCount;
+    if ((st = hipHostMalloc((void **)&e.energy_host, 64, hipHostMallocDefault)) != hipSuccess)
+        return check_hip(st, "hipHostMalloc(energy)");
+    e.energy_host[0] = 0.0;
+    if ((st = hipEventCreateWithFlags(&e.ev_energy, hipEventDisableTiming)) != hipSuccess)
+        return check_hip(st, "hipEventCreate");
     g_ctx.prop = prop;
     g_ctx.device = device;
     g_ctx.ready = true;
@@ -343,6 +348,15 @@ int glx_cg_update_gradient_pass(int prec, glx_grid g, double kappa2, double eps,
     GLX_DISPATCH(prec, cg_update_gradient_pass(g_ctx.env, g, kappa2, eps, eps_map, H, mt, psi, abei, ab, dpsi, dab,
                                                alpha_dev, psi_out, ab_out, jac, jac_prev, jacA, jacA_prev,
                                                beta_next_dev, F));
+}
+
+int glx_cg_energy_wait(double *F) {
+    GLX_REQUIRE_INIT();
+    if (!F) return set_error(GLX_E_ARG, "glx_cg_energy_wait: NULL pointer");
+    hipError_t st = hipEventSynchronize(g_ctx.env.ev_energy);
+    if (st != hipSuccess) return check_hip(st, "hipEventSynchronize(energy)");
+    *F = g_ctx.env.energy_host[0];
+    return GLX_OK;
 }
 
 int glx_vortex_triangulate(glx_grid g, const double *rec, int32_t count, double *xyv) {

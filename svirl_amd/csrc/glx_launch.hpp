@@ -32,6 +32,10 @@ struct Env {
     int tev_cap;
     double *solve_ms;
     int *solve_n;
+    // asynchronous energy readback of glx_cg_update_gradient_pass: the copy
+    // goes to pinned host memory and ev_energy is recorded after it
+    double *energy_host;
+    hipEvent_t ev_energy;
 };
 
 template <typename R> struct L {

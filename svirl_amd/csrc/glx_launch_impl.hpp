@@ -856,7 +856,7 @@ int L<R>::cg_update_gradient_pass(const Env &e, glx_grid g, double kappa2, doubl
                                   void *ab_out, void *jac, const void *jac_prev, void *jacA, const void *jacA_prev,
                                   double *beta_next_dev, double *F) {
     const bool withA = kappa2 > 0.0;
-    if (!grid_ok(g) || !psi || !jac || !jac_prev || !beta_next_dev || !F || !dpsi || !alpha_dev || !psi_out)
+    if (!grid_ok(g) || !psi || !jac || !jac_prev || !beta_next_dev || !dpsi || !alpha_dev || !psi_out)
         return set_error(GLX_E_ARG, "glx_cg_update_gradient_pass: invalid arguments");
     if (withA && (!(abei || ab) || !jacA || !jacA_prev))
         return set_error(GLX_E_ARG, "glx_cg_update_gradient_pass: finite kappa needs the vector potential and its Jacobians");
@@ -873,7 +873,12 @@ int L<R>::cg_update_gradient_pass(const Env &e, glx_grid g, double kappa2, doubl
     hipLaunchKernelGGL(k_cg_gradient_finish<R>, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, dxdy, e.result,
                        beta_next_dev);
     GLX_LAUNCH_CHECK("k_cg_gradient_finish");
-    return fetch_result(e, F, 1);
+    if (F) return fetch_result(e, F, 1);
+    // asynchronous: the energy lands in pinned memory; glx_cg_energy_wait
+    // waits for this copy only, not for work enqueued after it
+    hipError_t st = hipMemcpyAsync(e.energy_host, e.result, sizeof(double), hipMemcpyDeviceToHost, e.stream);
+    if (st == hipSuccess) st = hipEventRecord(e.ev_energy, e.stream);
+    return check_hip(st, "glx_cg_update_gradient_pass energy readback");
 }
 
 template <typename R>

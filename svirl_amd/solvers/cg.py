@@ -209,24 +209,29 @@ class CG:
                  self._beta_ptr(2), ctypes.byref(F))
         return F.value
 
-    def __direction_step(self, withA, c=None):
+    def __direction_step(self, withA, c=None, beta_slot=0, swap=True):
         """dir = beta dir - jac for psi (and A) fused into the line-search
         coefficients (glx_cg_direction_step): with c (a ctypes array of 5 or
         17 doubles) the coefficients come back to the host; without, the
-        device line search leaves the alphas in device memory. The new
-        directions are written to the second buffers, which then become the
-        current ones."""
+        device line search leaves the alphas in device memory. beta comes
+        from the device slots beta_slot, beta_slot + 1 (0: the solver's
+        betas, 2: the staged ones). The new directions are written to the
+        second buffers, which become the current ones when swap is set."""
         a = self._args()
         nat.call("glx_cg_direction_step", self.par.prec, self._grid(), a['k2'], a['eps'], a['H'], a['mt'], a['psi'],
                  a['abei'], a['ab'], self.__gdir_psi.ptr, self.__gjac_psi.ptr, self.__gdir_psi_next.ptr,
                  self.__gdir_A.ptr if withA else None, self.__gjac_A.ptr if withA else None,
-                 self.__gdir_A_next.ptr if withA else None, self._beta_ptr(0), c,
+                 self.__gdir_A_next.ptr if withA else None, self._beta_ptr(beta_slot), c,
                  None if c is not None else self.__galpha.ptr)
+        if swap:
+            self.__swap_directions(withA)
+
+    def __swap_directions(self, withA):
         self.__gdir_psi, self.__gdir_psi_next = self.__gdir_psi_next, self.__gdir_psi
         if withA:
             self.__gdir_A, self.__gdir_A_next = self.__gdir_A_next, self.__gdir_A
 
-    def __update_gradient_pass(self, withA):
+    def __update_gradient_pass(self, withA, wait=True):
         """The update state += alpha dir (alphas in device memory) fused into
         the gradient pass of the updated state (glx_cg_update_gradient_pass);
         the updated state goes to another buffer of the field's pool, which
@@ -244,11 +249,11 @@ class CG:
                  a['mt'], a['psi'], a['abei'], a['ab'], self.__gdir_psi.ptr, self.__gdir_A.ptr if withA else None,
                  self.__galpha.ptr, psi_out.ptr, nat.ptr(ab_out), self.__gjac_psi.ptr, self.__gjac_psi_prev.ptr,
                  self.__gjac_A.ptr if withA else None, self.__gjac_A_prev.ptr if withA else None,
-                 self._beta_ptr(2), ctypes.byref(F))
+                 self._beta_ptr(2), ctypes.byref(F) if wait else None)
         v._psi._gdata = psi_out
         if withA:
             v._vp._gdata = ab_out
-        return F.value
+        return F.value if wait else None
 
     def __commit_beta(self):
         nat.call("glx_memcpy_d2d", self._beta_ptr(0), self._beta_ptr(2), 16)
@@ -258,13 +263,52 @@ class CG:
         self.vars._vp.sync_d()
         self.cg_energies = []
         self.__gradient_pass(withA)  # gradient of the starting state
+        if self.line_search == 'device':
+            self.__device_loop(n_iter, withA)
+        else:
+            self.__host_loop(n_iter, withA)
+        self.vars._psi.need_dtoh_sync()
+        if withA:
+            self.vars._vp.need_dtoh_sync()
+
+    def __converged(self, i):
+        return i > 0 and np.abs(self.cg_energies[i] / self.cg_energies[i - 1] - 1.0) < self.__convergence_rtol
+
+    def __device_loop(self, n_iter, withA):
+        """The loop with the device line search. The next iteration's
+        direction step (from the staged betas) is enqueued before this
+        iteration's energy is read, so the GPU does not wait for the stop
+        test; it writes only the second direction buffers and the alphas,
+        so when the loop stops there the directions and betas are left as
+        if it had not run."""
+        cd = self.cfg.dtype
+        if n_iter <= 0:
+            return
+        self.__direction_step(withA)
+        for i in range(n_iter):
+            if withA:
+                self.__gjac_A, self.__gjac_A_prev = self.__gjac_A_prev, self.__gjac_A
+            self.__gjac_psi, self.__gjac_psi_prev = self.__gjac_psi_prev, self.__gjac_psi
+            self.__update_gradient_pass(withA, wait=False)
+            more = i + 1 < n_iter
+            if more:
+                self.__direction_step(withA, beta_slot=2, swap=False)
+            F = ctypes.c_double()
+            nat.call("glx_cg_energy_wait", ctypes.byref(F))
+            self.cg_energies.append(cd(F.value))
+            if self.__converged(i):
+                break
+            if more:
+                self.__commit_beta()
+                self.__swap_directions(withA)
+
+    def __host_loop(self, n_iter, withA):
+        """The loop with the reference's host line search."""
+        cd = self.cfg.dtype
         for i in range(n_iter):
             if i > 0:
                 self.__commit_beta()
-            cd = self.cfg.dtype
-            if self.line_search == 'device':
-                self.__direction_step(withA)
-            elif withA:
+            if withA:
                 out = (ctypes.c_double * 17)()
                 self.__direction_step(True, out)
                 g = np.array(out[:])
@@ -285,13 +329,9 @@ class CG:
             self.__gjac_psi, self.__gjac_psi_prev = self.__gjac_psi_prev, self.__gjac_psi
             # the update with this iteration's alphas, then the gradient,
             # energy and Polak-Ribiere sums of the updated state
-            E0 = cd(self.__update_gradient_pass(withA))
-            self.cg_energies.append(E0)
-            if i > 0 and np.abs(self.cg_energies[i] / self.cg_energies[i - 1] - 1.0) < self.__convergence_rtol:
+            self.cg_energies.append(cd(self.__update_gradient_pass(withA)))
+            if self.__converged(i):
                 break
-        self.vars._psi.need_dtoh_sync()
-        if withA:
-            self.vars._vp.need_dtoh_sync()
 
     def __minimization_psi(self, n_iter=1000):
         """cg.py:238-322 (kappa = inf): the direction restarts from zero."""

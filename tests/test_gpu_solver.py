@@ -435,3 +435,36 @@ def test_cg_device_line_search(kappa):
     assert np.allclose(ea, eb, rtol=1e-8)
     assert np.all(np.diff(eb) <= 1e-12 * np.abs(eb[1:]))
     assert abs(ec[-1] - ea[-1]) <= 1e-5 * abs(ea[-1])
+    # a second call continues from the directions and betas the first left
+    # (the device loop enqueues each next direction step ahead of the stop
+    # test; when the loop ends there, that step must leave no trace)
+    for gl in (a, b):
+        gl.solve.cg(n_iter=7)
+    ea2, eb2 = (np.array(gl.solve._cg.cg_energies, dtype=np.float64) for gl in (a, b))
+    assert len(ea2) == len(eb2) == 7 and np.allclose(ea2, eb2, rtol=1e-8)
+    assert np.all(np.diff(eb2) <= 1e-12 * np.abs(eb2[1:])) and eb2[0] <= eb[-1] * (1 - 1e-14)
+
+
+def test_cg_device_loop_stop_leaves_state():
+    """A device-line-search run stopped by the energy test leaves the same
+    directions and betas as a run capped at the same iteration: the
+    direction step enqueued ahead of the stop test leaves no trace."""
+    def make():
+        gl = GLSolver(Nx=41, Ny=37, dx=0.5, dy=0.5, order_parameter='random', random_seed=9, gl_parameter=3.0,
+                      normal_conductivity=50.0, homogeneous_external_field=0.1, dtype=np.float64,
+                      convergence_rtol=1e-3)
+        gl.solve.td(dt=0.1, Nt=3)
+        gl.solve._init_cg()
+        gl.solve._cg.line_search = 'device'
+        return gl
+    a = make()
+    a.solve.cg(n_iter=200)
+    n = len(a.solve._cg.cg_energies)
+    assert 1 < n < 200                       # stopped by the energy test
+    b = make()
+    b.solve.cg(n_iter=n)                     # the same iterations, ended by the cap
+    assert a.solve._cg.cg_energies == b.solve._cg.cg_energies
+    sa, sb = a.solve._cg._state(), b.solve._cg._state()
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+    assert np.array_equal(a.vars.order_parameter, b.vars.order_parameter)
