@@ -104,6 +104,27 @@ def test_c2_like_td_finite_kappa_with_mask_and_eps_map():
     assert np.allclose(gl.solve._cg.cg_energies, st.cg_energies, rtol=1e-9)
 
 
+def test_large_grid_td_matches_oracle():
+    """Config 3's physics on a 769 x 641 grid (holes, pinning map, kappa = 5,
+    H = 0.05, dt = 0.05) in FP64: long row ranges per wave, fused passes
+    and a team of several dozen workgroups; sweep counts exact, fields
+    within 1e-10 of the oracle after three steps."""
+    from svirl_amd import workloads
+    gl = GLSolver(Nx=769, Ny=641, dx=0.5, dy=0.5, order_parameter='random', random_seed=3, gl_parameter=5.0,
+                  normal_conductivity=200.0, homogeneous_external_field=0.05, dtype=np.float64)
+    c = gl.cfg
+    gl.mesh.material_tiling = workloads.holes_mask(int(c.Nxc), int(c.Nyc), 0.5, 0.5, seed=5)
+    gl.vars.set_order_parameter_to_zero_outside_material()
+    gl.params.linear_coefficient = workloads.pinning_field(int(c.Nx), int(c.Ny), 0.5, seed=6)
+    st = oracle_of(gl)
+    gl.solve.td(dt=0.05, Nt=3)
+    st.td(dt=0.05, Nt=3)
+    assert gl.solve._td._default_wg() > 16
+    assert gl.solve._td.sweeps == st.sweeps
+    assert rel(gl_psi(gl), st.psi) < 1e-10 and rel(gl_ab(gl), st.ab) < 1e-10
+    assert gl.observables.free_energy == pytest.approx(st.free_energy(), rel=1e-10)
+
+
 def test_vortex_detector_matches_oracle():
     gl = GLSolver(Lx=24, Ly=24, dx=0.5, dy=0.5, order_parameter=1.0, gl_parameter=np.inf,
                   homogeneous_external_field=0.15, dtype=np.float64)
