@@ -609,9 +609,11 @@ template <typename R> struct ARow {
 template <typename R>
 __device__ __forceinline__ ARow<R> a_load_row(const ASweepArgs<R> &A, const ColCtx &X, const uint8_t *mtp, int j) {
     const uint32_t Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1, Na = Nxa * Ny;
-    const uint32_t jp = min((uint32_t)j + 1, Ny - 1), jq = min((uint32_t)j, Ny - 2);
+    // rows past the last one (the later sweeps of a fused pass load up to
+    // two beyond) are clamped like the others: their values are not used
+    const uint32_t jr = min((uint32_t)j, Ny - 1), jp = min((uint32_t)j + 1, Ny - 1), jq = min((uint32_t)j, Ny - 2);
     const uint32_t oc = X.cc * (uint32_t)sizeof(R), oa = X.ca * (uint32_t)sizeof(R);
-    const size_t ra = (size_t)Nxa * (uint32_t)j, rb = (size_t)Na + (size_t)Nx * jq;
+    const size_t ra = (size_t)Nxa * jr, rb = (size_t)Na + (size_t)Nx * jq;
     ARow<R> L;
     L.ap = at(A.ab + (size_t)Nxa * jp, oa);
     L.bc = at(A.ab + rb, oc);

@@ -136,6 +136,9 @@ def _td_params(dt, eps, k2, rho, H, stop_psi, stop_A, solveA, eqn=0, rand_t=1):
     ((70, 41), np.inf, False, False, 15, 3),
     ((130, 17), 3.0, True, False, 6, 0),
     ((9, 9), 2.0, False, True, 25, 1),
+    ((2, 2), 2.0, False, False, 4, 0),   # the smallest grids the C ABI takes:
+    ((3, 2), 2.0, False, True, 4, 0),    # one cell; odd A sweep counts, so the
+    ((2, 3), np.inf, False, False, 4, 0),  # three-sweep A pass runs too
 ])
 def test_td_run_matches_oracle(shape, kappa, mask, epsmap, nsteps, wg):
     """Persistent multi-step kernel vs td.py's loops: identical sweep
