@@ -24,6 +24,8 @@ CASES = [
     (65, 130, 5.0, 0.05, True, False),
     (9, 12, np.inf, 0.45, False, False),
     (200, 40, 5.0, 0.15, True, True),
+    (4, 4, 2.0, 0.30, True, True),       # the smallest grids GLSolver takes
+    (5, 4, 5.0, 0.25, False, False),
 ]
 
 
