@@ -15,7 +15,7 @@ nat = pytest.importorskip("svirl_amd._native")
 
 from oracle import gl_oracle as O  # noqa: E402
 
-CASES = [(17, 13, 5.0, 0.2), (33, 29, 5.0, 0.1), (65, 40, 5.0, 0.3)]
+CASES = [(17, 13, 5.0, 0.2), (33, 29, 5.0, 0.1), (65, 40, 5.0, 0.3), (4, 5, 5.0, 0.2)]
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -83,7 +83,7 @@ def test_first_iteration_is_oracle_steepest_descent_and_line_minimum():
     sims = [make(k, *c) for k, c in enumerate(CASES)]
     before = [state(gl) for gl in sims]
     iters, conv, _ = BatchCG(sims).cg(max_iter=1, rtol=0.0)
-    assert list(iters) == [1, 1, 1] and not conv.any()
+    assert list(iters) == [1] * len(CASES) and not conv.any()
     for gl, (g, psi0, ab0, mt, em, k2, H) in zip(sims, before):
         _, psi1, ab1, *_ = state(gl)
         d = -O.jacobian_psi(g, k2, 0.0, em, H, mt, psi0, None, ab0)
@@ -111,8 +111,11 @@ def test_falling_energy_from_random_state():
         b.cg(max_iter=25, rtol=0.0)
         for k, gl in enumerate(sims):
             f[k].append(F(*[state(gl)[i] for i in (0, 5, 4, 6, 3, 1, 2)]))
+    # a small instance can reach its minimum inside the run: from there the
+    # energy stays where it is (to rounding)
     for fk in f:
-        assert all(x1 < x0 for x0, x1 in zip(fk, fk[1:])), fk
+        assert fk[1] < fk[0], fk
+        assert all(x1 <= x0 + 1e-13 * abs(x0) for x0, x1 in zip(fk, fk[1:])), fk
 
 
 def test_ragged_convergence_from_uniform_state():
