@@ -332,6 +332,84 @@ def test_reduction_api():
             assert np.allclose(v.reshape(nv, 5).sum(axis=0), gl.par.red.test_sum_v(v, nv, 5, block_size=bs), atol=1e-10)
 
 
+def _device_free_bytes():
+    """Free device memory as the HIP runtime that libglx is linked to reports it."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    return free.value
+
+
+def _material(gl, kind, rng):
+    """The material patterns of tests/common.py:13-58 (apply_material_tiling)."""
+    c = gl.cfg
+    Nxc, Nyc = int(c.Nxc), int(c.Nyc)
+    mt = np.full((Nxc, Nyc), True)
+    if kind == 'empty':
+        mt[:] = False
+    elif kind.startswith('random'):
+        n = int(int(kind[6:]) / 100 * Nxc * Nyc)
+        mt[rng.integers(Nxc, size=n), rng.integers(Nyc, size=n)] = False
+    elif kind[0] == 'q':
+        q = int(kind[1]) - 1
+        xs = slice(None, Nxc // 2) if q % 2 == 0 else slice(Nxc // 2, None)
+        ys = slice(None, Nyc // 2) if q < 2 else slice(Nyc // 2, None)
+        mt[xs, ys] = False
+    elif kind == 'none':
+        mt = None
+    gl.mesh.material_tiling = mt
+    gl.vars.set_order_parameter_to_zero_outside_material()
+
+
+def test_constructor_destructor_cycles():
+    """tests/at_destructor.py: ten solvers of random size (4..1023 nodes a
+    side), kappa 1 or inf, randomized psi, external fields and a material
+    pattern, each built, stepped twice and deleted; every cycle succeeds and
+    the device memory they held is returned.
+
+    The steps use sigma = 200 and dt = 0.05: the A solve is a Jacobi
+    iteration that converges when dt kappa^2 / sigma is below about
+    dx dy / 4, and at sigma = 1, dt = 0.1 and dx, dy in [0.2, 0.4) it
+    diverges to NaN in the oracle as well (the reference algorithm's own
+    behaviour, not a property of the GPU path)."""
+    import gc
+    rng = np.random.default_rng(2024)
+    kinds = ['full', 'empty', 'q1', 'q2', 'q3', 'q4', 'none',
+             'random10', 'random30', 'random50', 'random90', 'random200', 'random400', 'random800']
+
+    # one-time device state (library init, code objects) before the count
+    for dt in (np.float64, np.float32):
+        gl = GLSolver(Nx=16, Ny=12, dx=0.3, dy=0.3, gl_parameter=2.0, normal_conductivity=200.0,
+                      homogeneous_external_field=0.05, dtype=dt)
+        gl.solve.td(dt=0.05, Nt=2)
+        gl.observables.free_energy
+        del gl
+    gc.collect()
+    free0 = _device_free_bytes()
+    for i in range(10):
+        Nx, Ny = (int(v) for v in rng.integers(4, 1024, size=2))
+        kappa = 1.0 if rng.random() > 0.5 else np.inf
+        gl = GLSolver(Nx=Nx, Ny=Ny, dx=0.2 + 0.2 * rng.random(), dy=0.2 + 0.2 * rng.random(),
+                      gl_parameter=kappa, normal_conductivity=200.0,
+                      dtype=np.float64 if i % 2 == 0 else np.float32)
+        gl.vars.order_parameter = 1.0
+        gl.vars.randomize_order_parameter(level=0.5, seed=i)
+        if not np.isposinf(gl.params.gl_parameter):
+            gl.params.gl_parameter = 1.0 + 3.0 * rng.random()
+            gl.params.external_field = 0.01 + 0.1 * rng.random()
+        gl.params.homogeneous_external_field = 0.01 + 0.1 * rng.random()
+        _material(gl, kinds[rng.integers(len(kinds))], rng)
+        gl.solve.td(dt=0.05, Nt=2)
+        assert np.isfinite(gl.observables.free_energy), (i, Nx, Ny)
+        del gl
+        gc.collect()
+        print(f"cycle {i}: {Nx}x{Ny}, device memory not returned so far "
+              f"{(free0 - _device_free_bytes()) / 2**20:.1f} MiB", flush=True)
+    leaked = free0 - _device_free_bytes()
+    assert leaked <= 16 * 2**20
+
+
 def test_precision_fp32_vs_fp64():
     """tests/at_precision.py in miniature: FP32-mixed and FP64 TD energies
     agree (the reference allows rtol 0.1; the FP32-mixed form is held to 1e-4)."""
