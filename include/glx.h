@@ -238,6 +238,16 @@ int glx_td_last_kernel_ms(float *ms);
 int glx_td_profile(int32_t enable);
 int glx_td_solve_ms(double *psi_ms, int32_t *psi_launches, double *a_ms, int32_t *a_launches);
 
+/* Failure-handling hook (no counterpart in the reference; for tests of the
+ * timeout path). The next glx_td_run or glx_batch_td_run call (including
+ * one made by glx_td_step / glx_td_psi_solve / glx_td_a_solve) starts with
+ * the grid barrier's error flag raised, as a barrier poll that ran past its
+ * 5 s bound would leave it: every multi-workgroup team leaves its solve at
+ * its first barrier, the kernels end normally, and the call returns
+ * GLX_E_TIMEOUT without updating *psi_cur / *ab_cur. Like a real timeout it
+ * leaves the run's output buffers undefined. The run after it is normal. */
+int glx_td_force_barrier_abort(void);
+
 /* Largest co-resident grid for glx_td_run at a workgroup size. */
 int glx_td_max_grid(int prec, int32_t wg_size, int32_t *max_wg);
 

@@ -227,6 +227,7 @@ int glx_td_run(int prec, glx_grid g, const glx_td_params *p, const void *eps_map
         g_ctx.env.tev = g_ctx.tev.data();
         g_ctx.env.tev_cap = (int)g_ctx.tev.size();
     }
+    struct ClearAbort { ~ClearAbort() { g_ctx.env.abort_next = 0; } } clear_abort;  // one run only
     GLX_DISPATCH(prec, td_run(g_ctx.env, g, p, eps_map, mt, psi_bufs, psi_cur, ab_bufs, ab_cur, rhs_psi, rhs_ab,
                               nsteps, sweeps, wg, wg_size));
 }
@@ -276,12 +277,19 @@ int glx_td_last_kernel_ms(float *ms) {
 
 int glx_batch_td_run(int prec, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
                      int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight) {
+    struct ClearAbort { ~ClearAbort() { g_ctx.env.abort_next = 0; } } clear_abort;  // one run only
     GLX_DISPATCH(prec, batch_td_run(g_ctx.env, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows, weight));
 }
 
 int glx_batch_cg_run(int prec, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,
                      int32_t iters_per_launch) {
     GLX_DISPATCH(prec, batch_cg_run(g_ctx.env, inst, n, max_iters, rtol, iters_per_launch));
+}
+
+int glx_td_force_barrier_abort(void) {
+    GLX_REQUIRE_INIT();
+    g_ctx.env.abort_next = 1;
+    return GLX_OK;
 }
 
 int glx_td_profile(int32_t enable) {

@@ -36,6 +36,10 @@ struct Env {
     // goes to pinned host memory and ev_energy is recorded after it
     double *energy_host;
     hipEvent_t ev_energy;
+    // the next TD run (single or batched) starts with the barrier error flag
+    // raised, so every team leaves at its first barrier as after a timeout
+    // (glx_td_force_barrier_abort; cleared by that run)
+    int abort_next;
 };
 
 template <typename R> struct L {

@@ -196,7 +196,7 @@ static int td_run_form(const Env &e, glx_grid g, const glx_td_params *p, const v
     T.sc.sweeps = e.sweeps;
     const long Nab = (long)(g.Nx - 1) * g.Ny + (long)g.Nx * (g.Ny - 1);
 
-    hipLaunchKernelGGL(k_td_reset, dim3(1), dim3(256), 0, e.stream, T.sc, (int)*pc, (int)*qc);
+    hipLaunchKernelGGL(k_td_reset, dim3(1), dim3(256), 0, e.stream, T.sc, (int)*pc, (int)*qc, e.abort_next ? 1 : 0);
     GLX_LAUNCH_CHECK("k_td_reset");
     if (G) {
         hipLaunchKernelGGL(k_gauge_shift<R>, dim3(grid_blocks(Nab)), dim3(256), 0, e.stream, T.Q[*qc], g.Nx, g.Ny, gH,
@@ -437,7 +437,7 @@ static int batch_run_form(const Env &e, glx_batch_instance *inst, int32_t n, int
     if (st == hipSuccess) st = hipMemcpyAsync(B.state, state.data(), sizeof(int) * 8 * n, hipMemcpyHostToDevice, e.stream);
     if (st != hipSuccess) return check_hip(st, "glx_batch_td_run upload");
     hipLaunchKernelGGL(k_batch_reset, dim3(std::max(1, grid_blocks(std::max(grid, n)))), dim3(256), 0, e.stream, e.arrive,
-                       grid, B.rel, n, e.err);
+                       grid, B.rel, n, e.err, e.abort_next ? 1 : 0);
     GLX_LAUNCH_CHECK("k_batch_reset");
     if (G)
         for (int i = 0; i < n; ++i) {

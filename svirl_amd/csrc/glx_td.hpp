@@ -1504,22 +1504,24 @@ __global__ void __launch_bounds__(256) k_batch_td_step(BatchArgs<R> B) {
     }
 }
 
-// Zero the arrival words of the first nb workgroups, n release words and
-// the error flag (a kernel, so that nothing here is a memset node).
-static __global__ void k_batch_reset(unsigned long long *arrive, int nb, unsigned long long *rel, int n, int *err) {
+// Zero the arrival words of the first nb workgroups and n release words, and
+// set the error flag to err0 (0, or 1 to make every team leave at its first
+// barrier); a kernel, so that nothing here is a memset node.
+static __global__ void k_batch_reset(unsigned long long *arrive, int nb, unsigned long long *rel, int n, int *err,
+                                     int err0) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nb; k += gridDim.x * blockDim.x) arrive[k] = 0ull;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) rel[(long)k * REL_STRIDE] = 0ull;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *err = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *err = err0;
 }
 
-// Zero the barrier words and run state before the first solve of a run and
-// set the buffer indices.
-static __global__ void k_td_reset(TDScratch sc, int pc, int qc) {
+// Zero the barrier words and run state before the first solve of a run, set
+// the buffer indices and the error flag (err0 as in k_batch_reset).
+static __global__ void k_td_reset(TDScratch sc, int pc, int qc, int err0) {
     const int t = threadIdx.x;
     for (int k = t; k < MAX_GRID_BLOCKS; k += blockDim.x) sc.gs.arrive[k] = 0ull;
     for (int k = t; k < NREL * REL_STRIDE; k += blockDim.x) sc.gs.rel[k] = 0ull;
     if (t == 0) {
-        *sc.gs.err = 0;
+        *sc.gs.err = err0;
         sc.state[ST_PSI] = pc;
         sc.state[ST_AB] = qc;
         *state64(sc.state, ST_EPOCH) = 0ull;

@@ -332,6 +332,35 @@ def test_reduction_api():
             assert np.allclose(v.reshape(nv, 5).sum(axis=0), gl.par.red.test_sum_v(v, nv, 5, block_size=bs), atol=1e-10)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_barrier_timeout_reports_and_keeps_state(dtype):
+    """The timeout path of the grid barrier, forced from the host: the run
+    starts with the barrier's error flag raised (as a poll past its 5 s
+    bound leaves it), every workgroup leaves at its first barrier, the call
+    raises GLX_E_TIMEOUT, the solver keeps the state it had before the run,
+    and the next run is the oracle's (FP64: sweep counts exact, psi to
+    1e-10)."""
+    from svirl_amd import _native as nat
+    gl = GLSolver(Nx=300, Ny=200, dx=0.5, dy=0.5, order_parameter='random', random_seed=5, gl_parameter=2.0,
+                  normal_conductivity=10.0, homogeneous_external_field=0.1, dtype=dtype)
+    # 300 x 200 nodes: 5 strips x 200 rows, a persistent grid of 8 workgroups
+    psi0 = np.array(gl.vars.order_parameter, copy=True)
+    st = oracle_of(gl)
+    nat.call("glx_td_force_barrier_abort")
+    with pytest.raises(nat.GLXError) as err:
+        gl.solve.td(dt=0.1, Nt=3)
+    assert err.value.code == nat.GLX_E_TIMEOUT and "timed out" in str(err.value)
+    assert gl.solve._td.sweeps == []
+    assert np.array_equal(gl.vars.order_parameter, psi0)
+    gl.solve.td(dt=0.1, Nt=3)
+    st.td(dt=0.1, Nt=3)
+    if dtype == np.float64:
+        assert gl.solve._td.sweeps == st.sweeps
+        assert rel(gl_psi(gl), st.psi) < 1e-10 and rel(gl_ab(gl), st.ab) < 1e-10
+    else:
+        assert rel(gl_psi(gl), st.psi) < 1e-4 and rel(gl_ab(gl), st.ab) < 1e-4
+
+
 def _device_free_bytes():
     """Free device memory as the HIP runtime that libglx is linked to reports it."""
     import ctypes
