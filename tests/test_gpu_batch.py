@@ -114,3 +114,27 @@ def test_batch_fp32_close_to_single_fp32():
         s1 = np.array(g1.solve._td.sweeps)
         assert np.all(np.abs(s0 - s1) <= 1)
         assert np.max(np.abs(gl.vars.order_parameter - g1.vars.order_parameter)) <= 1e-5
+
+
+def test_batch_barrier_timeout_keeps_states():
+    """glx_batch_td_run's timeout path, forced from the host (the run starts
+    with the barrier's error flag raised): the call raises GLX_E_TIMEOUT,
+    every instance keeps the state it had, and the next run matches the
+    oracle. The aborted call makes one step: a one-workgroup team does not
+    poll the flag and steps on, and a single step writes only buffers that
+    were not current."""
+    from svirl_amd.batch import BatchTD
+    sims = [make(k, *c) for k, c in enumerate(CASES)]
+    before = [np.array(gl.vars.order_parameter, copy=True) for gl in sims]
+    refs = [oracle_run(gl, 0.1, 4) for gl in sims]
+    b = BatchTD(sims, use_graph=False, autotune=False)
+    nat.call("glx_td_force_barrier_abort")
+    with pytest.raises(nat.GLXError) as err:
+        b.td(dt=0.1, Nt=1)
+    assert err.value.code == nat.GLX_E_TIMEOUT
+    for gl, p0 in zip(sims, before):
+        assert np.array_equal(gl.vars.order_parameter, p0)
+    b.td(dt=0.1, Nt=4)
+    for gl, st in zip(sims, refs):
+        assert [tuple(x) for x in gl.solve._td.sweeps] == st.sweeps
+        assert np.max(np.abs(gl.flatten_array(gl.vars.order_parameter) - st.psi)) <= 1e-11
