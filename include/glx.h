@@ -243,9 +243,12 @@ int glx_td_solve_ms(double *psi_ms, int32_t *psi_launches, double *a_ms, int32_t
  * one made by glx_td_step / glx_td_psi_solve / glx_td_a_solve) starts with
  * the grid barrier's error flag raised, as a barrier poll that ran past its
  * 5 s bound would leave it: every multi-workgroup team leaves its solve at
- * its first barrier, the kernels end normally, and the call returns
- * GLX_E_TIMEOUT without updating *psi_cur / *ab_cur. Like a real timeout it
- * leaves the run's output buffers undefined. The run after it is normal. */
+ * its first barrier (a one-workgroup team of a batch never polls the flag
+ * and runs on), the kernels end normally, and the call returns
+ * GLX_E_TIMEOUT without updating *psi_cur / *ab_cur or the instances'
+ * current indices. Like a real timeout it leaves the run's other buffers
+ * undefined (after one step, only buffers that were not current have been
+ * written). The run after it is normal. */
 int glx_td_force_barrier_abort(void);
 
 /* Largest co-resident grid for glx_td_run at a workgroup size. */
