@@ -186,6 +186,10 @@ __global__ void __launch_bounds__(256, BatchCGBound<R>::MIN_WG) k_batch_cg(const
             if (threadIdx.x == 0) sv[CG_IT] = double(itn + 1);
             __syncthreads();
         }
+        // At both breaks above each wave reads go after a barrier and leaves;
+        // without this barrier thread 0 could rewrite go / s_it for the next
+        // instance while a slower wave has yet to read them.
+        __syncthreads();
     }
 }
 
