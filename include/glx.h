@@ -176,6 +176,26 @@ typedef struct glx_batch_instance {
 int glx_batch_td_run(int prec, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
                      int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight);
 
+/* Bytes of the fixed device scratch array behind glx_batch_td_run_noalloc. */
+#define GLX_NOALLOC_SCRATCH_BYTES (8u << 20)
+
+/* glx_batch_td_run without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments and results as glx_batch_td_run with
+ * use_graph = 0: the run's descriptor table, team map, barrier words, state
+ * and sweep log live in a file-scope device array of
+ * GLX_NOALLOC_SCRATCH_BYTES (8 MiB, held by each GPU the library runs on
+ * until the process ends) instead of a per-call hipMalloc. A run that needs
+ * more (about 0.6 KB per instance plus 8 bytes per instance and step) fails
+ * with GLX_E_ARG before anything is launched; split nsteps over several
+ * calls. Host-side planning uses ordinary host memory. Every call on one
+ * GPU shares the array, and the array keeps the last call's contents: each
+ * call rewrites all it reads before its first step. Calls that overlap on
+ * one GPU (from several threads) race on it. */
+int glx_batch_td_run_noalloc(int prec, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
+                             int32_t wg_per_cu, int32_t min_rows, const double *weight);
+
 /* ------------------------------------------------------------ batched CG */
 
 /* One instance of a batched CG minimisation (no counterpart in the

@@ -42,9 +42,11 @@ CANDIDATES = [(0, 8), (0, 16), (0, 12), (0, 4), (2, 8)]
 class BatchTD:
     """Advance a list of GLSolver objects together (FP32 or FP64, all the
     same). use_graph: replay the step launches from a HIP graph.
-    autotune: pick the launch shape from CANDIDATES on the first steps."""
+    autotune: pick the launch shape from CANDIDATES on the first steps.
+    noalloc: run through glx_batch_td_run_noalloc (no device allocation per
+    call, its workspace is the library's fixed scratch array; no graph)."""
 
-    def __init__(self, solvers, use_graph=True, autotune=True, tune_steps=4):
+    def __init__(self, solvers, use_graph=True, autotune=True, tune_steps=4, noalloc=False):
         if not solvers:
             raise ValueError("BatchTD needs at least one solver")
         dt0 = solvers[0].cfg.dtype
@@ -52,7 +54,8 @@ class BatchTD:
             raise ValueError("all solvers of a batch must share one dtype")
         self.solvers = list(solvers)
         self.prec = nat.prec_of(dt0)
-        self.use_graph = bool(use_graph)
+        self.noalloc = bool(noalloc)
+        self.use_graph = bool(use_graph) and not self.noalloc
         self.autotune = bool(autotune)
         self.tune_steps = int(tune_steps)
         self.shape = None
@@ -100,8 +103,12 @@ class BatchTD:
         wt = None
         if self.weights is not None:
             wt = np.ascontiguousarray(self.weights, dtype=np.float64)
-        nat.call("glx_batch_td_run", self.prec, arr, n, nsteps, sw, 1 if self.use_graph else 0, shape[0], shape[1],
-                 wt.ctypes.data if wt is not None else None)
+        wp = wt.ctypes.data if wt is not None else None
+        if self.noalloc:
+            nat.call("glx_batch_td_run_noalloc", self.prec, arr, n, nsteps, sw, shape[0], shape[1], wp)
+        else:
+            nat.call("glx_batch_td_run", self.prec, arr, n, nsteps, sw, 1 if self.use_graph else 0, shape[0],
+                     shape[1], wp)
         kms = ctypes.c_float()
         nat.call("glx_td_last_kernel_ms", ctypes.byref(kms))
         self.last_kernel_ms = kms.value

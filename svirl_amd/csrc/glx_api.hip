@@ -24,11 +24,15 @@ struct Ctx {
     std::vector<hipEvent_t> tev;
     hipDeviceProp_t prop{};
     Env env{};
-    void *scratch = nullptr;
+    void *lib_scratch = nullptr;  // partials, barrier words, run state (hipMalloc at glx_init)
 };
 
 Ctx g_ctx;
 std::mutex g_mu;
+
+// Device addresses of `scratch` (below), one per device, taken at glx_init.
+constexpr int MAX_DEVICES = 64;
+void *g_scratch_dev[MAX_DEVICES] = {};
 thread_local char g_err[512] = "";
 
 constexpr int SWEEPS_CAP = 1 << 16;  // steps per persistent launch
@@ -57,6 +61,13 @@ static int ensure_ready() {
 
 using namespace glx;
 
+// Workspace of glx_batch_td_run_noalloc: a fixed device array, so that entry
+// point needs no device allocation. Its host-side name is not a device
+// address; glx_init stores the address hipGetSymbolAddress gives for the
+// current device in g_scratch_dev, and only that address is handed to
+// kernels and copies. Every call on one device shares the array.
+static __device__ __align__(256) unsigned char scratch[GLX_NOALLOC_SCRATCH_BYTES];
+
 #define GLX_REQUIRE_INIT() do { int _r = ensure_ready(); if (_r) return _r; } while (0)
 #define GLX_DISPATCH(prec, call)                                                   \
     do {                                                                           \
@@ -80,6 +91,7 @@ int glx_init(int device) {
     hipError_t st = hipGetDeviceCount(&n);
     if (st != hipSuccess) return check_hip(st, "hipGetDeviceCount");
     if (device < 0 || device >= n) return set_error(GLX_E_ARG, "device %d not present (%d devices)", device, n);
+    if (device >= MAX_DEVICES) return set_error(GLX_E_ARG, "device %d above the supported %d", device, MAX_DEVICES);
     if ((st = hipSetDevice(device)) != hipSuccess) return check_hip(st, "hipSetDevice");
     hipDeviceProp_t prop;
     if ((st = hipGetDeviceProperties(&prop, device)) != hipSuccess) return check_hip(st, "hipGetDeviceProperties");
@@ -92,7 +104,7 @@ int glx_init(int device) {
     g_ctx.env.last_kernel_ms = &g_ctx.last_kernel_ms;
     g_ctx.env.solve_ms = g_ctx.solve_ms;
     g_ctx.env.solve_n = g_ctx.solve_n;
-    // scratch layout: partials | result | arrive | rel | err | state | sweeps
+    // library scratch layout: partials | result | arrive | rel | err | state | sweeps
     const size_t partials = sizeof(double) * 1024 * 32;
     const size_t result = sizeof(double) * 64;
     const size_t arrive = sizeof(unsigned long long) * MAX_GRID_BLOCKS;
@@ -101,9 +113,9 @@ int glx_init(int device) {
     const size_t state = 512;
     const size_t sweeps = sizeof(int) * 2 * SWEEPS_CAP;
     const size_t total = partials + result + arrive + rel + err + state + sweeps;
-    if ((st = hipMalloc(&g_ctx.scratch, total)) != hipSuccess) return check_hip(st, "hipMalloc(scratch)");
-    if ((st = hipMemset(g_ctx.scratch, 0, total)) != hipSuccess) return check_hip(st, "hipMemset(scratch)");
-    char *p = (char *)g_ctx.scratch;
+    if ((st = hipMalloc(&g_ctx.lib_scratch, total)) != hipSuccess) return check_hip(st, "hipMalloc(library scratch)");
+    if ((st = hipMemset(g_ctx.lib_scratch, 0, total)) != hipSuccess) return check_hip(st, "hipMemset(library scratch)");
+    char *p = (char *)g_ctx.lib_scratch;
     Env &e = g_ctx.env;
     e.stream = g_ctx.stream;
     e.partials = (double *)p; p += partials;
@@ -120,6 +132,12 @@ int glx_init(int device) {
     e.energy_host[0] = 0.0;
     if ((st = hipEventCreateWithFlags(&e.ev_energy, hipEventDisableTiming)) != hipSuccess)
         return check_hip(st, "hipEventCreate");
+    if (!g_scratch_dev[device]) {  // with `device` current (hipSetDevice above)
+        void *sp = nullptr;
+        if ((st = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch))) != hipSuccess)
+            return check_hip(st, "hipGetSymbolAddress(scratch)");
+        g_scratch_dev[device] = sp;
+    }
     g_ctx.prop = prop;
     g_ctx.device = device;
     g_ctx.ready = true;
@@ -278,7 +296,17 @@ int glx_td_last_kernel_ms(float *ms) {
 int glx_batch_td_run(int prec, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
                      int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight) {
     struct ClearAbort { ~ClearAbort() { g_ctx.env.abort_next = 0; } } clear_abort;  // one run only
-    GLX_DISPATCH(prec, batch_td_run(g_ctx.env, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows, weight));
+    const Workspace ws{nullptr, 0, "glx_batch_td_run"};
+    GLX_DISPATCH(prec, batch_td_run(g_ctx.env, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows, weight, ws));
+}
+
+int glx_batch_td_run_noalloc(int prec, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
+                             int32_t wg_per_cu, int32_t min_rows, const double *weight) {
+    GLX_REQUIRE_INIT();
+    struct ClearAbort { ~ClearAbort() { g_ctx.env.abort_next = 0; } } clear_abort;  // one run only
+    const Workspace ws{g_scratch_dev[g_ctx.device], (size_t)GLX_NOALLOC_SCRATCH_BYTES, "glx_batch_td_run_noalloc"};
+    if (!ws.ptr) return set_error(GLX_E_NOINIT, "glx_batch_td_run_noalloc: no scratch address for device %d", g_ctx.device);
+    GLX_DISPATCH(prec, batch_td_run(g_ctx.env, inst, n, nsteps, sweeps, 0, wg_per_cu, min_rows, weight, ws));
 }
 
 int glx_batch_cg_run(int prec, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,

@@ -42,6 +42,15 @@ struct Env {
     int abort_next;
 };
 
+// Device memory a launcher uses in place of allocating its own per-call
+// block: ptr NULL = allocate as usual. `what` names the entry point in
+// error messages.
+struct Workspace {
+    void *ptr;
+    size_t bytes;
+    const char *what;
+};
+
 template <typename R> struct L {
     static int psi_sweep(const Env &e, glx_grid g, double dt, double eps, const void *eps_map, const uint8_t *mt,
                          const void *ab, void *psi_rhs, const void *psi, void *psi_next, double langevin_c,
@@ -53,7 +62,8 @@ template <typename R> struct L {
                       void *const P[3], int32_t *pc, void *const Q[3], int32_t *qc, void *rhs_psi, void *rhs_ab,
                       int32_t nsteps, int32_t *sweeps, int32_t wg, int32_t wg_size);
     static int batch_td_run(const Env &e, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
-                            int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight);
+                            int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight,
+                            const Workspace &ws);
     static int batch_cg_run(const Env &e, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,
                             int32_t iters_per_launch);
     static int td_max_grid(const Env &e, int32_t wg_size, int32_t *max_wg);

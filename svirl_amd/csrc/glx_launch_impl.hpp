@@ -380,13 +380,14 @@ template <typename R, bool G> static int batch_occupancy(int *per_cu) {
 
 template <typename R, bool G>
 static int batch_run_form(const Env &e, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
-                          int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight) {
+                          int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight,
+                          const Workspace &ws) {
     int per_cu = 0;
     int rc = batch_occupancy<R, G>(&per_cu);
     if (rc) return rc;
     if (wg_per_cu > 0 && wg_per_cu < per_cu) per_cu = wg_per_cu;
     const int Gb = std::min(per_cu * e.num_cus, MAX_GRID_BLOCKS);
-    if (n > Gb) return set_error(GLX_E_ARG, "glx_batch_td_run: %d instances exceed %d co-resident workgroups", n, Gb);
+    if (n > Gb) return set_error(GLX_E_ARG, "%s: %d instances exceed %d co-resident workgroups", ws.what, n, Gb);
     std::vector<long> rows(n);
     std::vector<double> work(n);
     for (int i = 0; i < n; ++i) {
@@ -414,10 +415,20 @@ static int batch_run_form(const Env &e, glx_batch_instance *inst, int32_t n, int
     const size_t s_args = al(sizeof(TDSolveArgs<R>) * n), s_tob = al(sizeof(int) * grid), s_teams = al(sizeof(int2) * n);
     const size_t s_rel = al(sizeof(unsigned long long) * REL_STRIDE * n), s_state = al(sizeof(int) * 8 * n);
     const size_t s_sw = al(sizeof(int) * 2 * (size_t)n * nsteps);
+    const size_t need = s_args + s_tob + s_teams + s_rel + s_state + s_sw;
     char *blk = nullptr;
-    hipError_t st = hipMalloc((void **)&blk, s_args + s_tob + s_teams + s_rel + s_state + s_sw);
-    if (st != hipSuccess) return check_hip(st, "hipMalloc(batch)");
-    struct Free { char *p; ~Free() { if (p) (void)hipFree(p); } } guard{blk};
+    hipError_t st = hipSuccess;
+    if (ws.ptr) {
+        // caller-provided device workspace (the no-allocation entry point)
+        if (need > ws.bytes)
+            return set_error(GLX_E_ARG, "%s: %d instances x %d steps need %zu bytes of scratch, above its %zu-byte maximum",
+                             ws.what, n, nsteps, need, ws.bytes);
+        blk = (char *)ws.ptr;
+    } else {
+        st = hipMalloc((void **)&blk, need);
+        if (st != hipSuccess) return check_hip(st, "hipMalloc(batch)");
+    }
+    struct Free { char *p; ~Free() { if (p) (void)hipFree(p); } } guard{ws.ptr ? nullptr : blk};
     char *q = blk;
     BatchArgs<R> B{};
     B.args = (const TDSolveArgs<R> *)q; q += s_args;
@@ -452,7 +463,7 @@ static int batch_run_form(const Env &e, glx_batch_instance *inst, int32_t n, int
     st = hipEventRecord(e.kev0, e.stream);
     if (st != hipSuccess) return check_hip(st, "hipEventRecord");
     int launched = 0;
-    if (use_graph && nsteps > 1) {
+    if (use_graph && nsteps > 1 && !ws.ptr) {
         const int K = std::min(nsteps, 64);
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
@@ -497,7 +508,7 @@ static int batch_run_form(const Env &e, glx_batch_instance *inst, int32_t n, int
     if (st != hipSuccess) return check_hip(st, "glx_batch_td_run readback");
     float kms = 0.f;
     if (hipEventElapsedTime(&kms, e.kev0, e.kev1) == hipSuccess && e.last_kernel_ms) *e.last_kernel_ms = kms;
-    if (host_err) return set_error(GLX_E_TIMEOUT, "glx_batch_td_run: a team barrier timed out");
+    if (host_err) return set_error(GLX_E_TIMEOUT, "%s: a team barrier timed out", ws.what);
     for (int i = 0; i < n; ++i) {
         inst[i].psi_cur = state[8 * i + ST_PSI];
         inst[i].ab_cur = state[8 * i + ST_AB];
@@ -508,30 +519,31 @@ static int batch_run_form(const Env &e, glx_batch_instance *inst, int32_t n, int
 
 template <typename R>
 int L<R>::batch_td_run(const Env &e, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
-                       int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight) {
-    if (!inst || n < 1 || nsteps < 0) return set_error(GLX_E_ARG, "glx_batch_td_run: invalid arguments");
-    if ((long)n * nsteps > (1L << 26)) return set_error(GLX_E_ARG, "glx_batch_td_run: n*nsteps too large");
+                       int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight,
+                       const Workspace &ws) {
+    if (!inst || n < 1 || nsteps < 0) return set_error(GLX_E_ARG, "%s: invalid arguments", ws.what);
+    if ((long)n * nsteps > (1L << 26)) return set_error(GLX_E_ARG, "%s: n*nsteps too large", ws.what);
     bool any_A = false;
     for (int i = 0; i < n; ++i) {
         const glx_batch_instance &I = inst[i];
         const glx_td_params &p = I.p;
-        if (!grid_ok(I.g)) return set_error(GLX_E_ARG, "glx_batch_td_run: instance %d: invalid grid", i);
+        if (!grid_ok(I.g)) return set_error(GLX_E_ARG, "%s: instance %d: invalid grid", ws.what, i);
         if (p.max_sweeps < 1 || p.stop_psi <= 0.0 || p.stop_A <= 0.0 || p.eqn != 0)
-            return set_error(GLX_E_ARG, "glx_batch_td_run: instance %d: invalid parameters", i);
+            return set_error(GLX_E_ARG, "%s: instance %d: invalid parameters", ws.what, i);
         if (p.langevin_psi > 1.0e-32 || p.langevin_A > 1.0e-32)
-            return set_error(GLX_E_ARG, "glx_batch_td_run: instance %d: Langevin noise is not supported in batches", i);
+            return set_error(GLX_E_ARG, "%s: instance %d: Langevin noise is not supported in batches", ws.what, i);
         if (p.a_src)
-            return set_error(GLX_E_ARG, "glx_batch_td_run: instance %d: a fixed A source is not supported in batches", i);
+            return set_error(GLX_E_ARG, "%s: instance %d: a fixed A source is not supported in batches", ws.what, i);
         if (I.psi_cur < 0 || I.psi_cur > 2 || I.ab_cur < 0 || I.ab_cur > 2)
-            return set_error(GLX_E_ARG, "glx_batch_td_run: instance %d: buffer index", i);
+            return set_error(GLX_E_ARG, "%s: instance %d: buffer index", ws.what, i);
         for (int k = 0; k < 3; ++k)
-            if (!I.psi[k] || !I.ab[k]) return set_error(GLX_E_ARG, "glx_batch_td_run: instance %d: NULL buffer", i);
+            if (!I.psi[k] || !I.ab[k]) return set_error(GLX_E_ARG, "%s: instance %d: NULL buffer", ws.what, i);
         any_A = any_A || p.solveA;
     }
     if (nsteps == 0) return GLX_OK;
     if constexpr (RunForm<R>::has_gauge)
-        if (any_A) return batch_run_form<R, true>(e, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows, weight);
-    return batch_run_form<R, false>(e, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows, weight);
+        if (any_A) return batch_run_form<R, true>(e, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows, weight, ws);
+    return batch_run_form<R, false>(e, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows, weight, ws);
 }
 
 // ---- batched CG -----------------------------------------------------------------
