@@ -19,8 +19,8 @@
  * Reference interfaces replaced (paths relative to the svirl repository):
  * the kernels are svirl/cuda/{td,cg,observables,utils,reduction,
  * block_reduction}.h, launched through pycuda by
- * svirl/parallel/startup.py:272-280 (get_function), with geometry baked in
- * at JIT time (startup.py:252-261); here the geometry is a glx_grid
+ * svirl/parallel/startup.py:74-82 (get_function), with geometry baked in
+ * at JIT time (startup.py:54-63); here the geometry is a glx_grid
  * argument. All calls run on one library-owned HIP stream; calls that return
  * host values synchronise it.
  */
@@ -46,7 +46,7 @@ enum {
 };
 
 /* Grid geometry; replaces the %(Nx)s, %(Ny)s, %(dx)s, %(dy)s template
- * constants of svirl/parallel/startup.py:252-258. */
+ * constants of svirl/parallel/startup.py:54-60. */
 typedef struct glx_grid {
     int32_t Nx, Ny;
     double dx, dy;
@@ -55,7 +55,7 @@ typedef struct glx_grid {
 /* ---------------------------------------------------------------- runtime */
 
 /* Select the device and create the library stream
- * (replaces Startup.__init_device, svirl/parallel/startup.py:285-291). */
+ * (replaces Startup.__init_device, svirl/parallel/startup.py:87-93). */
 int glx_init(int device);
 const char *glx_last_error(void);
 int glx_synchronize(void);
@@ -82,7 +82,7 @@ int glx_timer_stop(float *ms);
  * atomicMax(r2max, int(min(1e4*max|dpsi|/stop_eps, 1e8))). r2max is a device
  * int32 the caller zeroes. With langevin_c > 1e-32 and jstep == 0 the noisy
  * right-hand side is written back to psi_rhs.
- * Replaces iterate_order_parameter_jacobi_step, svirl/cuda/td.h:85-214,
+ * Replaces iterate_order_parameter_jacobi_step, svirl/cuda/td.h:4-133,
  * launched at svirl/solvers/td.py:168-187. */
 int glx_td_psi_sweep(int prec, glx_grid g, double dt, double eps, const void *eps_map,
                      const uint8_t *mt, const void *ab, void *psi_rhs, const void *psi,
@@ -92,14 +92,14 @@ int glx_td_psi_sweep(int prec, glx_grid g, double dt, double eps, const void *ep
 /* One Jacobi sweep of the vector potential. abi_ab_rhs is the supercurrent
  * source buffer and may alias ab_next (each thread reads its own source
  * links before it writes them, as the reference's ping-pong relies on).
- * Replaces iterate_vector_potential_jacobi_step, svirl/cuda/td.h:391-544,
+ * Replaces iterate_vector_potential_jacobi_step, svirl/cuda/td.h:310-463,
  * launched at svirl/solvers/td.py:277-300. */
 int glx_td_a_sweep(int prec, glx_grid g, double dt, double kappa2, double rho, double H,
                    const uint8_t *mt, const void *psi, const void *abi_ab_rhs, void *ab_rhs,
                    const void *ab, void *ab_next, double langevin_c, uint32_t jstep,
                    uint32_t rand_t, double stop_eps, int32_t *r2max);
 
-/* psi[n] <- |psi[n]| at lock_ns[0..lock_N) (svirl/cuda/td.h:376-388). */
+/* psi[n] <- |psi[n]| at lock_ns[0..lock_N) (svirl/cuda/td.h:295-307). */
 int glx_td_phase_lock(int prec, void *psi, int32_t lock_N, const int32_t *lock_ns);
 
 typedef struct glx_td_params {
@@ -310,7 +310,7 @@ int glx_free_energy(int prec, glx_grid g, double kappa2, double eps, const void 
  * records of 14 doubles: (cell n, v, psi00.re, .im, psip0.re, .im,
  * psipp.re, .im, psi0p.re, .im, a00, a0p, b00, bp0) where the link values
  * are ab + abi (abi may be NULL) for the host triangulation
- * (vortex_detector.py:256-296). Records come unordered; *count is the
+ * (vortex_detector.py:103-147). Records come unordered; *count is the
  * total number of flagged cells (may exceed capacity: then GLX_E_CAPACITY
  * and only `capacity` records are written). */
 int glx_vortex_cells(int prec, glx_grid g, const void *psi, const void *ab, const void *abi,

@@ -30,7 +30,7 @@ TWO_PI = 2.0 * np.pi
 
 class Geom:
     """Grid sizes and the kernel-side constants (svirl/__init__.py:100-123,
-    svirl/cuda/td.h:111-116, observables.h:14,51-53).
+    svirl/cuda/td.h:34-35, observables.h:14,51-53).
 
     The kernels recompute their constants as ``1.0/(dx*dx)`` etc. from the
     JIT-templated ``dx``; that is what is stored here.
@@ -95,7 +95,7 @@ def shift(X, dj, di):
 
 
 def node_cells(g, mt):
-    """Material flags of the four cells around every node (td.h:129-138).
+    """Material flags of the four cells around every node (td.h:48-57).
 
     Returns (mm, mp, pm, pp), each bool [j, i]: cells (i-1,j-1), (i-1,j),
     (i,j-1), (i,j), False where the cell lies outside the grid.
@@ -183,7 +183,7 @@ def grad_c(psi0, ph, psi1):
 
 
 def scaled_residual(r, stop_eps):
-    """int32(min(1e4 r / stop_eps, 1e8)) (td.h:205-213, 541-543)."""
+    """int32(min(1e4 r / stop_eps, 1e8)) (td.h:124-132, 541-543)."""
     with np.errstate(over='ignore', invalid='ignore'):
         rr = 1.0e4 * r / stop_eps
     rr = np.where(rr > 1.0e8, 1.0e8, rr)
@@ -204,10 +204,10 @@ def converged(r2max):
 
 def psi_jacobi_sweep(g, dt, eps, eps_map, mt, ab, psi_rhs, psi,
                      langevin_c=0.0, jstep=0, rand_t=1, stop_eps=1e-6):
-    """One backward-Euler Jacobi sweep for psi (td.h:85-214).
+    """One backward-Euler Jacobi sweep for psi (td.h:4-133).
 
     Returns (psi_next, psi_rhs_after, r2max_int). psi_rhs_after differs from
-    psi_rhs only when Langevin noise is added at jstep 0 (td.h:147-151).
+    psi_rhs only when Langevin noise is added at jstep 0 (td.h:66-70).
     ``ab`` is the regular vector potential (TD passes vars._vp only,
     td.py:120-142).
     """
@@ -249,7 +249,7 @@ def psi_jacobi_sweep(g, dt, eps, eps_map, mt, ab, psi_rhs, psi,
 def a_jacobi_sweep(g, dt, kappa2, rho, H, mt, psi, abi_ab_rhs, ab_rhs, ab,
                    langevin_c=0.0, jstep=0, rand_t=1, stop_eps=1e-6):
     """One Jacobi sweep of sigma dA/dt = J_s - kappa^2 curl curl A
-    (td.h:391-544). Returns (ab_next, ab_rhs_after, r2max_int).
+    (td.h:310-463). Returns (ab_next, ab_rhs_after, r2max_int).
 
     ``abi_ab_rhs`` is the supercurrent source buffer (its values as they
     were before this sweep wrote anything).
@@ -268,7 +268,7 @@ def a_jacobi_sweep(g, dt, kappa2, rho, H, mt, psi, abi_ab_rhs, ab_rhs, ab,
     src_a = g.a2(abi_ab_rhs); src_b = g.b2(abi_ab_rhs)
     ra = g.a2(rhs); rb = g.b2(rhs)
 
-    # ---- a links: i < Nx-1, all j (td.h:447-491)
+    # ---- a links: i < Nx-1, all j (td.h:366-410)
     Nxa, Ny = g.Nxa, g.Ny
     j = np.arange(Ny)[:, None] * np.ones((1, Nxa))
     rh = np.where(j == 0, 2.0 * kappa2 * H * g.idy,
@@ -285,7 +285,7 @@ def a_jacobi_sweep(g, dt, kappa2, rho, H, mt, psi, abi_ab_rhs, ab_rhs, ab,
               / (1.0 + 2.0 * dt_rho_kappa2 * g.idy2))
     res_a = np.abs(a_next - a)
 
-    # ---- b links: all i, j < Ny-1 (td.h:493-537)
+    # ---- b links: all i, j < Ny-1 (td.h:412-456)
     Nx, Nyb = g.Nx, g.Nyb
     i = np.ones((Nyb, 1)) * np.arange(Nx)[None, :]
     rh = np.where(i == 0, -2.0 * kappa2 * H * g.idx,
@@ -308,7 +308,7 @@ def a_jacobi_sweep(g, dt, kappa2, rho, H, mt, psi, abi_ab_rhs, ab_rhs, ab,
 
 
 def phase_lock(psi, lock_ns):
-    """td.h:376-388: psi[n] <- |psi[n]| at the listed nodes."""
+    """td.h:295-307: psi[n] <- |psi[n]| at the listed nodes."""
     out = np.array(psi, copy=True)
     out[lock_ns] = np.abs(out[lock_ns])
     return out
@@ -772,7 +772,7 @@ class OracleState:
 
     def td_psi_solve(self, dt):
         """td.py:157-218: the Jacobi loop, then the phase lock of the
-        listed nodes (td.py:205-216, td.h:376-388)."""
+        listed nodes (td.py:205-216, td.h:295-307)."""
         g = self.g
         psi_rhs = self.psi.copy()
         cur = self.psi
@@ -961,7 +961,7 @@ def vortex_cells(g, psi, ab, H):
 
 
 def triangulate(g, cells, vs, psi, a_ai, b_bi):
-    """vortex_detector.py:225-303: sub-cell vortex positions from the zero
+    """vortex_detector.py:75-147: sub-cell vortex positions from the zero
     lines of gauge-rotated Re/Im psi in each flagged cell.
     a_ai, b_bi are 2-D [i, j] arrays (host layout)."""
     def find_zero(x1, y1, f1, x2, y2, f2):

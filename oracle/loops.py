@@ -13,7 +13,7 @@ def _U(ph):
 
 
 def _cells(g, mt, i, j):
-    """td.h:129-138."""
+    """td.h:48-57."""
     Nx, Ny, Nxc = g.Nx, g.Ny, g.Nxc
     im, ip, jm, jp = i - 1, i + 1, j - 1, j + 1
     mm = (i > 0) and (j > 0)
@@ -55,7 +55,7 @@ def _residual(r, stop_eps):
 
 
 def psi_sweep(g, dt, eps, eps_map, mt, ab, psi_rhs, psi, stop_eps):
-    """td.h:85-214, Langevin off."""
+    """td.h:4-133, Langevin off."""
     Nx, Ny, Nxa, Nxb, Na = g.Nx, g.Ny, g.Nxa, g.Nxb, g.Na
     out = [0j] * g.N
     r2 = 0
@@ -94,7 +94,7 @@ def psi_sweep(g, dt, eps, eps_map, mt, ab, psi_rhs, psi, stop_eps):
 
 
 def a_sweep(g, dt, kappa2, rho, H, mt, psi, src, ab_rhs, ab, stop_eps):
-    """td.h:391-544, Langevin off. ``src`` is abi_ab_rhs."""
+    """td.h:310-463, Langevin off. ``src`` is abi_ab_rhs."""
     Nx, Ny, Nxa, Nya, Nxb, Nyb, Na = g.Nx, g.Ny, g.Nxa, g.Nya, g.Nxb, g.Nyb, g.Na
     dt_rho = dt * rho
     dt_rho_kappa2 = dt_rho * kappa2

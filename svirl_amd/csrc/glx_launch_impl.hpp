@@ -65,7 +65,7 @@ static ASweepArgs<R> make_a_args(const glx_grid &g, double dt, double kappa2, do
     A.Nx = g.Nx; A.Ny = g.Ny;
     A.dx = R(g.dx); A.dy = R(g.dy); A.idx = R(G.idx); A.idy = R(G.idy);
     A.idx2 = R(G.idx2); A.idy2 = R(G.idy2); A.idxy = R(G.idxy);
-    // scalar constants in the reference's expression order (td.h:419, 457, 484, 503, 530)
+    // scalar constants in the reference's expression order (td.h:338, 376, 403, 422, 449)
     const double dt_rho = double(R(dt)) * double(R(rho));
     const double dt_rho_R = double(R(dt_rho));
     const double dt_rho_kappa2 = dt_rho_R * double(R(kappa2));

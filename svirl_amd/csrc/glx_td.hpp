@@ -150,7 +150,7 @@ __device__ __forceinline__ PsiRow<R> psi_load_row(const PsiSweepArgs<R> &A, cons
     return L;
 }
 
-// Backward-Euler Jacobi update of one psi node (svirl/cuda/td.h:85-214),
+// Backward-Euler Jacobi update of one psi node (svirl/cuda/td.h:4-133),
 // in two parts. psi_den: the diagonal 1 + dt (|rhs|^2 - eps + sum of the
 // link weights / h^2) from the four adjacent cells (mm = (c-1, j-1),
 // mp = (c-1, j), pmc = (c, j-1), ppc = (c, j)); FP32 returns its reciprocal

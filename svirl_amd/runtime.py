@@ -31,7 +31,7 @@ class Startup:
 
     def get_function(self, function_name):
         """Kernels are not looked up by name here; kept for the error path
-        of svirl/parallel/startup.py:272-280."""
+        of svirl/parallel/startup.py:74-82."""
         raise ValueError("\n Function name %s not found" % function_name)
 
 

@@ -242,7 +242,7 @@ def test_coef_psi_polynomial_identity(kappa2):
 
 
 def test_coef17_expansion():
-    """tests/at_cg_coef.py:92-186: exact in alpha_psi when dA = 0, and the
+    """tests/at_cg_coef.py:33-125: exact in alpha_psi when dA = 0, and the
     alpha_A expansion error falls from P0 to P2 to P4 as |dA| shrinks."""
     rng, g, mt, psi, ab, ae, _ = _case(41, epsmap=False)
     k2, H, eps = 5.0, 0.07, 0.9
@@ -330,7 +330,7 @@ def test_vortex_imprint(s):
     H = 0 winds once around the cell holding (x0, y0); the detector reports
     that cell with vorticity -s and triangulates (x0, y0) exactly. (The
     zero lines are skewed on purpose: the reference's find_intersection
-    rejects exactly perpendicular lines, vortex_detector.py:245-252.)"""
+    rejects exactly perpendicular lines, vortex_detector.py:88-101.)"""
     g = O.Geom(17, 15, 0.5, 0.5)
     x0, y0 = 3.3, 4.1
     x = g.dx * np.arange(g.Nx)[None, :]
@@ -343,28 +343,6 @@ def test_vortex_imprint(s):
     vx, vy, vv = O.vortex_detector(g, psi, ab, 0.0)
     assert vx[0] == pytest.approx(x0, abs=1e-12) and vy[0] == pytest.approx(y0, abs=1e-12)
     assert vv[0] == -s
-
-
-def test_aliasing_source_schedule():
-    """td.py:264-303: sweep j reads its supercurrent source from A^(2*floor(j/2))."""
-    rng, g, mt, psi, ab, _, _ = _case(50, Nx=9, Ny=9, mask=False)
-    st = O.OracleState(g, psi, ab, kappa=3.0, sigma=5.0, H=0.1, stop_A=1e-14)
-    # replay with explicit per-sweep sources
-    A = [ab.copy()]
-    rhs = ab.copy()
-    for j in range(6):
-        src = A[2 * (j // 2)]
-        nxt, rhs, r = O.a_jacobi_sweep(g, 0.1, st.kappa2, st.rho, st.H, None, psi, src, rhs, A[-1], stop_eps=1e-14)
-        A.append(nxt)
-    st2 = O.OracleState(g, psi, ab, kappa=3.0, sigma=5.0, H=0.1, stop_A=1e-14)
-    # run a solve capped at 6 sweeps by monkeypatching the cap through a tiny stop rule
-    g2 = st2.g
-    X = st2.ab; Y = st2._ab_next; src = X; rhs = X.copy(); cur, nxt = X, Y
-    for j in range(6):
-        vals, rhs, r = O.a_jacobi_sweep(g2, 0.1, st2.kappa2, st2.rho, st2.H, None, psi, src.copy(), rhs, cur, stop_eps=1e-14)
-        nxt[:] = vals
-        cur, nxt = nxt, cur
-    assert np.array_equal(cur, A[-1])
 
 
 # ---------------------------------------------------------------------------
