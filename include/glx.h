@@ -258,6 +258,18 @@ int glx_td_last_kernel_ms(float *ms);
 int glx_td_profile(int32_t enable);
 int glx_td_solve_ms(double *psi_ms, int32_t *psi_launches, double *a_ms, int32_t *a_launches);
 
+/* Memory passes of the last glx_td_run call (no counterpart in the
+ * reference; a measurement hook, like glx_td_solve_ms): the psi solves'
+ * passes over the fields (a fused pass of two or three sweeps counts once)
+ * and the redone ones among them (a pass repeated up to an earlier sweep
+ * that met the stop rule), the same two for the A solves, and the A passes
+ * that move one more A buffer (a three-sweep pass stores A^2; an odd sweep
+ * j >= 3, or any sweep with a fixed source, reads a separate source). The
+ * compulsory bytes of a psi pass are (8s + e + m) per node, of an A pass
+ * (8s + m), plus 2s for each such extra buffer (s: bytes per real; e = s
+ * with an epsilon map; m = 1 with a mask). */
+int glx_td_passes(int32_t *psi_passes, int32_t *psi_redo, int32_t *a_passes, int32_t *a_redo, int32_t *a_xbuf);
+
 /* Failure-handling hook (no counterpart in the reference; for tests of the
  * timeout path). The next glx_td_run or glx_batch_td_run call (including
  * one made by glx_td_step / glx_td_psi_solve / glx_td_a_solve) starts with

@@ -21,6 +21,7 @@ struct Ctx {
     float last_kernel_ms = 0.f;
     double solve_ms[2] = {0.0, 0.0};
     int solve_n[2] = {0, 0};
+    int passes[5] = {0, 0, 0, 0, 0};
     std::vector<hipEvent_t> tev;
     hipDeviceProp_t prop{};
     Env env{};
@@ -104,6 +105,7 @@ int glx_init(int device) {
     g_ctx.env.last_kernel_ms = &g_ctx.last_kernel_ms;
     g_ctx.env.solve_ms = g_ctx.solve_ms;
     g_ctx.env.solve_n = g_ctx.solve_n;
+    g_ctx.env.passes = g_ctx.passes;
     // library scratch layout: partials | result | arrive | rel | err | state | sweeps
     const size_t partials = sizeof(double) * 1024 * 32;
     const size_t result = sizeof(double) * 64;
@@ -333,6 +335,17 @@ int glx_td_solve_ms(double *psi_ms, int32_t *psi_launches, double *a_ms, int32_t
     *psi_launches = g_ctx.solve_n[0];
     *a_ms = g_ctx.solve_ms[1];
     *a_launches = g_ctx.solve_n[1];
+    return GLX_OK;
+}
+
+int glx_td_passes(int32_t *psi_passes, int32_t *psi_redo, int32_t *a_passes, int32_t *a_redo, int32_t *a_xbuf) {
+    GLX_REQUIRE_INIT();
+    if (!psi_passes || !psi_redo || !a_passes || !a_redo || !a_xbuf) return set_error(GLX_E_ARG, "glx_td_passes: NULL");
+    *psi_passes = g_ctx.passes[0];
+    *psi_redo = g_ctx.passes[1];
+    *a_passes = g_ctx.passes[2];
+    *a_redo = g_ctx.passes[3];
+    *a_xbuf = g_ctx.passes[4];
     return GLX_OK;
 }
 

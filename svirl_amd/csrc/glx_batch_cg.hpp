@@ -142,16 +142,18 @@ __global__ void __launch_bounds__(256, BatchCGBound<R>::MIN_WG) k_batch_cg(const
             double cacc[L];
 #pragma unroll
             for (int c = 0; c < L; ++c) cacc[c] = 0.0;
+            double th = 0.0;  // largest link phase step |d dA| (the line search's bound)
             for (long m = threadIdx.x; m < g.N; m += blockDim.x) {
                 R C[L];
 #pragma unroll
                 for (int c = 0; c < L; ++c) C[c] = R(0);
                 coef_node<R, L, true>(g, q, withA ? I.kappa2 : R(-1), I.eps, I.eps_map, I.H, I.mt, I.psi, nullptr,
-                                      (const R *)nullptr, I.ab, nullptr, m, C, &U);
+                                      (const R *)nullptr, I.ab, nullptr, m, C, &U, withA ? &th : nullptr);
 #pragma unroll
                 for (int c = 0; c < L; ++c) cacc[c] += double(C[c]);
             }
             block_sums<L>(cacc, lds, red);
+            th = block_max(th, lds);
             if (threadIdx.x == 0) {
                 double c5[5][5] = {};
                 const double sc = double(q.dx) * double(q.dy);
@@ -167,7 +169,7 @@ __global__ void __launch_bounds__(256, BatchCGBound<R>::MIN_WG) k_batch_cg(const
                     for (int i = 0; i < 5; ++i) c5[i][0] = sc * red[i];
                 }
                 double x, y;
-                quartic2_min(c5, !withA, x, y);
+                quartic2_min(c5, !withA, x, y, alpha_A_bound(th));
                 sv[CG_APSI] = x;
                 sv[CG_AA] = y;
                 bc[2] = x;

@@ -209,6 +209,22 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
     return s;
 }
 
+// Maximum of one non-negative double per thread over the workgroup; result
+// valid in thread 0. `lds` needs blockDim.x/64 doubles.
+__device__ __forceinline__ double block_max(double v, double *lds) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, WAVE));
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    const int nw = (blockDim.x + WAVE - 1) / WAVE;
+    __syncthreads();
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double m = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < nw; ++k) m = fmax(m, lds[k]);
+    return m;
+}
+
 // Maximum of one int per thread over the workgroup; valid in thread 0.
 __device__ __forceinline__ int block_max_int(int v, int *lds) {
     v = wave_max_int(v);

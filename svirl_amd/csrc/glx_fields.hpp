@@ -843,7 +843,23 @@ __device__ __forceinline__ void poly2(const double (&c)[5][5], double x, double 
 // Armijo backtracking, in FP64. With one_d, y stays 0 (the psi-only
 // quartic). Plays the part of the reference's numpy root / scipy BFGS
 // line search (cg.py:227-235, 378-419) on the device.
-__device__ __forceinline__ void quartic2_min(const double (&c)[5][5], bool one_d, double &x, double &y) {
+//
+// The polynomial is exact in alpha_psi but a 4th-order expansion of the
+// link factors exp(i alpha_A d dA) in alpha_A (cg.h:477-731), whose quartic
+// coefficient is negative wherever neighbouring phases are aligned: far from
+// 0 it can fall without bound, and an unconstrained descent then runs off to
+// a step where the expansion means nothing (from psi = 1 it took |psi| to
+// 1e130). So |alpha_A| is kept within ymax, the step at which the largest
+// link phase moves by CG_PHASE_MAX (ymax = CG_PHASE_MAX / max|d dA|; there
+// the expansion's relative error is below 3e-4); trial points are projected
+// onto |y| <= ymax and must lower the polynomial.
+constexpr double CG_PHASE_MAX = 0.5;
+__device__ __forceinline__ double alpha_A_bound(double thmax) {
+    return thmax > 0.0 ? CG_PHASE_MAX / thmax : HUGE_VAL;
+}
+
+__device__ __forceinline__ void quartic2_min(const double (&c)[5][5], bool one_d, double &x, double &y,
+                                             double ymax = HUGE_VAL) {
     x = 0.0;
     y = 0.0;
     for (int it = 0; it < 80; ++it) {
@@ -863,17 +879,22 @@ __device__ __forceinline__ void quartic2_min(const double (&c)[5][5], bool one_d
         }
         const double slope = px * dx + py * dy;
         if (!(slope < 0.0)) { dx = -px; dy = -py; }
-        const double sl = px * dx + py * dy;
-        double t = 1.0, pn = 0.0, a, b, c2, d, e2;
+        double t = 1.0, pn = 0.0, a, b, c2, d, e2, sx = 0.0, sy = 0.0;
+        bool ok = false;
         for (int k = 0; k < 60; ++k) {
-            poly2(c, x + t * dx, y + t * dy, pn, a, b, c2, d, e2);
-            if (pn <= p + 1.0e-4 * t * sl) break;
+            sx = t * dx;
+            sy = fmin(fmax(y + t * dy, -ymax), ymax) - y;  // projected onto |y| <= ymax
+            poly2(c, x + sx, y + sy, pn, a, b, c2, d, e2);
+            if (pn < p && pn <= p + 1.0e-4 * fmin(px * sx + py * sy, 0.0)) {
+                ok = true;
+                break;
+            }
             t *= 0.5;
         }
-        const double sx = t * dx, sy = t * dy;
+        if (!ok) break;
         x += sx;
         y += sy;
-        if (fabs(sx) + fabs(sy) <= 1.0e-15 * (fabs(x) + fabs(y)) || !(pn < p)) break;
+        if (fabs(sx) + fabs(sy) <= 1.0e-15 * (fabs(x) + fabs(y))) break;
     }
 }
 
@@ -893,6 +914,12 @@ __global__ void __launch_bounds__(256) k_line_search(const double *partials, int
         if (threadIdx.x == 0) red[k] = s * scale;
         __syncthreads();
     }
+    // the coefficient pass left each block's largest link phase step after
+    // the sums (k_coef, k_coef_dir with L = 17)
+    double th = 0.0;
+    if (L == 17)
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) th = fmax(th, partials[(long)nb * L + b]);
+    th = block_max(th, lds);
     if (threadIdx.x == 0) {
         double c5[5][5] = {};
         if (L == 17) {
@@ -907,7 +934,7 @@ __global__ void __launch_bounds__(256) k_line_search(const double *partials, int
             for (int k = 0; k < 5; ++k) c5[k][0] = red[k];
         }
         double x, y;
-        quartic2_min(c5, L != 17, x, y);
+        quartic2_min(c5, L != 17, x, y, alpha_A_bound(th));
         alpha[0] = x;
         alpha[1] = y;
         for (int k = 0; k < L; ++k) c[k] = red[k];
@@ -968,11 +995,13 @@ __device__ __forceinline__ cplx<R> dir_at(const DirUpdate<R> &U, long k) {
     return {U.bp * o.re - gk.re, U.bp * o.im - gk.im};
 }
 
+// With L = 17 and th not NULL, *th also takes the largest link phase step
+// |d dA| of the node's links a(i, j), b(i, j) (for the line search's bound).
 template <typename R, int L, bool DIR = false>
 __device__ __forceinline__ void coef_node(const Geo &g, const GeoR<R> &q, R kappa2, R eps_s, const R *eps_map, R H,
                                           const uint8_t *mt, const cplx<R> *psi, const cplx<R> *dpsi, const R *abei,
                                           const R *ab, const R *dab, long n, R (&C)[L],
-                                          const DirUpdate<R> *U = nullptr) {
+                                          const DirUpdate<R> *U = nullptr, double *th = nullptr) {
     int i, j;
     node_ij(g, n, i, j);
     // loads
@@ -1017,6 +1046,7 @@ __device__ __forceinline__ void coef_node(const Geo &g, const GeoR<R> &q, R kapp
         } else {
             D = load_links4(g, dab, i, j);
         }
+        if (th) *th = fmax(*th, fmax(fabs(double(q.dx * D.a00)), fabs(double(q.dy * D.b00))));
     }
     if constexpr (DIR) {
         st(U->dout, n, d0);
@@ -1112,6 +1142,14 @@ __device__ __forceinline__ void coef_node(const Geo &g, const GeoR<R> &q, R kapp
     }
 }
 
+// Partials: the L sums of block b at partials[b L + k]; with L = 17 the
+// block's largest link phase step after all sums, at partials[nb L + b].
+__device__ __forceinline__ void block_store_phase_max(double th, double *partials, int L) {
+    __shared__ double lds[RED_BLOCK / WAVE];
+    th = block_max(th, lds);
+    if (threadIdx.x == 0) partials[(long)gridDim.x * L + blockIdx.x] = th;
+}
+
 template <typename R, int L>
 __global__ void __launch_bounds__(256) k_coef(Geo g, R kappa2, R eps, R H, const uint8_t *mt, const cplx<R> *psi, const cplx<R> *dpsi,
                        const R *abei, const R *ab, const R *dab, double *partials) {
@@ -1119,15 +1157,18 @@ __global__ void __launch_bounds__(256) k_coef(Geo g, R kappa2, R eps, R H, const
     double acc[L];
 #pragma unroll
     for (int k = 0; k < L; ++k) acc[k] = 0.0;
+    double th = 0.0;
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
         R C[L];
 #pragma unroll
         for (int k = 0; k < L; ++k) C[k] = R(0);
-        coef_node<R, L>(g, q, kappa2, eps, (const R *)nullptr, H, mt, psi, dpsi, abei, ab, dab, n, C);
+        coef_node<R, L>(g, q, kappa2, eps, (const R *)nullptr, H, mt, psi, dpsi, abei, ab, dab, n, C, nullptr,
+                        L == 17 ? &th : nullptr);
 #pragma unroll
         for (int k = 0; k < L; ++k) acc[k] += double(C[k]);
     }
     block_store_partials<L>(acc, partials);
+    if (L == 17) block_store_phase_max(th, partials, L);
 }
 
 // k_coef with the direction update fused in (DirUpdate): the coefficients
@@ -1143,15 +1184,18 @@ __global__ void __launch_bounds__(256) k_coef_dir(Geo g, R kappa2, R eps, R H, c
     double acc[L];
 #pragma unroll
     for (int k = 0; k < L; ++k) acc[k] = 0.0;
+    double th = 0.0;
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.N; n += (long)gridDim.x * blockDim.x) {
         R C[L];
 #pragma unroll
         for (int k = 0; k < L; ++k) C[k] = R(0);
-        coef_node<R, L, true>(g, q, kappa2, eps, (const R *)nullptr, H, mt, psi, nullptr, abei, ab, nullptr, n, C, &U);
+        coef_node<R, L, true>(g, q, kappa2, eps, (const R *)nullptr, H, mt, psi, nullptr, abei, ab, nullptr, n, C, &U,
+                              L == 17 ? &th : nullptr);
 #pragma unroll
         for (int k = 0; k < L; ++k) acc[k] += double(C[k]);
     }
     block_store_partials<L>(acc, partials);
+    if (L == 17) block_store_phase_max(th, partials, L);
 }
 
 // --------------------------------------------------------------- vortices

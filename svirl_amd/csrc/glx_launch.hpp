@@ -32,6 +32,9 @@ struct Env {
     int tev_cap;
     double *solve_ms;
     int *solve_n;
+    // memory passes of the last td_run (psi passes, psi redone, A passes, A
+    // redone, A passes with one more A buffer), or NULL
+    int *passes;
     // asynchronous energy readback of glx_cg_update_gradient_pass: the copy
     // goes to pinned host memory and ev_energy is recorded after it
     double *energy_host;

@@ -240,9 +240,9 @@ static int td_run_form(const Env &e, glx_grid g, const glx_td_params *p, const v
                            e.state, g.Nx, g.Ny, gH, g.dx, g.dy, 0.5 * Lx, 0.5 * Ly, 1.0);
         GLX_LAUNCH_CHECK("k_gauge_shift_state");
     }
-    int host_state[8] = {0};
+    int host_state[ST_WORDS] = {0};
     int host_err = 0;
-    hipError_t st = hipMemcpyAsync(host_state, e.state, sizeof(int) * 8, hipMemcpyDeviceToHost, e.stream);
+    hipError_t st = hipMemcpyAsync(host_state, e.state, sizeof(int) * ST_WORDS, hipMemcpyDeviceToHost, e.stream);
     if (st == hipSuccess) st = hipMemcpyAsync(&host_err, e.err, sizeof(int), hipMemcpyDeviceToHost, e.stream);
     if (st == hipSuccess) st = hipMemcpyAsync(sweeps, e.sweeps, sizeof(int) * 2 * nsteps, hipMemcpyDeviceToHost, e.stream);
     if (st == hipSuccess) st = hipStreamSynchronize(e.stream);
@@ -273,6 +273,8 @@ static int td_run_form(const Env &e, glx_grid g, const glx_td_params *p, const v
         for (int k = 0; k < nsteps; ++k) sweeps[2 * k + 1] = 0;
     *pc = host_state[ST_PSI];
     *qc = host_state[ST_AB];
+    if (e.passes)
+        for (int k = 0; k < 5; ++k) e.passes[k] = host_state[ST_PASS_PSI + k];
     return GLX_OK;
 }
 

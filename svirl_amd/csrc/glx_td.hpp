@@ -1210,6 +1210,11 @@ __global__ void __launch_bounds__(256) k_a_sweep(ASweepArgs<R> A, int *r2max, lo
 // memory, so there is no host round trip per step.
 
 enum { ST_PSI = 0, ST_AB = 1, ST_EPOCH = 2, ST_STEP = 4, ST_PRED_A = 5, ST_ERRSTEP = 6, ST_PRED_PSI = 7 };
+// Pass counters of a single-instance run (glx_td_passes): memory passes of
+// the psi and A solves, the redone ones among them, and the A passes that
+// move one more A buffer (the three-sweep pass stores A^2; an odd sweep
+// j >= 3, and every sweep with a fixed source, reads a separate source).
+enum { ST_PASS_PSI = 8, ST_REDO_PSI = 9, ST_PASS_A = 10, ST_REDO_A = 11, ST_XBUF_A = 12, ST_WORDS = 13 };
 
 struct TDScratch {
     GridSync gs;
@@ -1264,7 +1269,16 @@ struct SolveOut {
     int cur;    // buffer index of the final iterate
     int nsw;    // sweeps
     bool ok;    // false: a barrier timed out
+    int cnt;    // pass counters, packed (one register): PASS_ONE per memory
+                // pass (a fused pass counts once), REDO_ONE per pass redone
+                // up to an earlier sweep that met the stop rule, XBUF_ONE per
+                // A pass that moves one more A buffer
 };
+// a solve has at most 1024 sweeps, so each field fits its bits
+constexpr int PASS_ONE = 1, REDO_ONE = 1 << 11, XBUF_ONE = 1 << 13;
+__host__ __device__ inline int cnt_passes(int c) { return c & 0x7ff; }
+__host__ __device__ inline int cnt_redo(int c) { return (c >> 11) & 0x3; }
+__host__ __device__ inline int cnt_xbuf(int c) { return (c >> 13) & 0x7ff; }
 
 // The psi solve of one TD step on the team's waves (w of W), in passes of
 // KMAX fused sweeps (psi_rows3 or psi_rows2: several sweeps per pass over
@@ -1282,9 +1296,10 @@ __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const Gri
     PsiSweepArgs<R> A = T.psi;
     A.ab = pick3(T.Q, qc);
     A.rhs_out = T.rhs_psi;
-    int in = pc, out = o1, j = 0, nsw = 0;
+    int in = pc, out = o1, j = 0, nsw = 0, cnt = 0;
     bool done = false;
     auto pass = [&](int k, R &r1, R &r2, R &r3) {
+        cnt += PASS_ONE;
         if constexpr (KMAX == 3) {
             if (k == 3) {
                 for_rows_rm<3>(A.Nx, A.Ny, w, W,
@@ -1309,7 +1324,7 @@ __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const Gri
         int r = 0;
         if (!sweep_sync<KMAX>(gs, tm, scaled_residual<R>(r1, A.stop_eps), scaled_residual<R>(r2, A.stop_eps),
                               scaled_residual<R>(r3, A.stop_eps), epoch, lds, r))
-            return {in, nsw, false};
+            return {in, nsw, false, cnt};
         // the first sweep of the pass that met the stop rule, or k
         const int m = (k > 1 && res_conv(r, 0)) ? 1 : ((k > 2 && res_conv(r, 1)) ? 2 : k);
         if (m < k) {
@@ -1317,8 +1332,9 @@ __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const Gri
             // the pass had three sweeps)
             R d1 = R(0), d2 = R(0), d3 = R(0);
             pass(KMAX == 3 && m == 2 ? 2 : 1, d1, d2, d3);
+            cnt += REDO_ONE;
             int rdummy = 0;
-            if (!sweep_sync<0>(gs, tm, 0, 0, 0, epoch, lds, rdummy)) return {in, nsw, false};
+            if (!sweep_sync<0>(gs, tm, 0, 0, 0, epoch, lds, rdummy)) return {in, nsw, false, cnt};
             done = true;
         } else {
             done = res_conv(r, k - 1);
@@ -1328,7 +1344,7 @@ __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const Gri
         in = out;
         out = (out == o1) ? o2 : o1;
     }
-    return {in, nsw, true};
+    return {in, nsw, true, cnt};
 }
 
 // The A solve of one TD step on the team's waves, with the reference's
@@ -1350,9 +1366,10 @@ __device__ __forceinline__ SolveOut a_solve(const TDSolveArgs<R> &T, const GridS
     ASweepArgs<R> A = T.A;
     A.psi = pick3(T.P, pc);
     A.rhs_out = T.rhs_ab;
-    int in = qc, out = o1, j = 0, nsw = 0;
+    int in = qc, out = o1, j = 0, nsw = 0, cnt = 0;
     bool done = false;
     auto pass = [&](int k, R &r1, R &r2, R &r3) {
+        cnt += PASS_ONE;
         if constexpr (TRIPLE) {
             if (k == 3) {
                 for_rows_rm<3>(A.Nx, A.Ny, w, W,
@@ -1382,16 +1399,18 @@ __device__ __forceinline__ SolveOut a_solve(const TDSolveArgs<R> &T, const GridS
         else if (j + 1 == pred || j + 2 > T.max_sweeps) k = 1;
         R r1 = R(0), r2 = R(0), r3 = R(0);
         pass(k, r1, r2, r3);
+        cnt += (FIXED || k == 3 || (j % 2 == 1 && j >= 3)) ? XBUF_ONE : 0;
         int r = 0;
         if (!sweep_sync<TRIPLE ? 3 : 2>(gs, tm, scaled_residual<R>(r1, A.stop_eps), scaled_residual<R>(r2, A.stop_eps),
                                         scaled_residual<R>(r3, A.stop_eps), epoch, lds, r))
-            return {in, nsw, false};
+            return {in, nsw, false, cnt};
         const int m = (k > 1 && res_conv(r, 0)) ? 1 : ((k > 2 && res_conv(r, 1)) ? 2 : k);
         if (m < k) {
             R d1 = R(0), d2 = R(0), d3 = R(0);
             pass(TRIPLE && m == 2 ? 2 : 1, d1, d2, d3);  // m is 1 unless the pass had three sweeps
+            cnt += REDO_ONE + (FIXED ? XBUF_ONE : 0);
             int rdummy = 0;
-            if (!sweep_sync<0>(gs, tm, 0, 0, 0, epoch, lds, rdummy)) return {in, nsw, false};
+            if (!sweep_sync<0>(gs, tm, 0, 0, 0, epoch, lds, rdummy)) return {in, nsw, false, cnt};
             done = true;
         } else {
             done = res_conv(r, k - 1);
@@ -1401,7 +1420,7 @@ __device__ __forceinline__ SolveOut a_solve(const TDSolveArgs<R> &T, const GridS
         in = out;
         out = (out == o1) ? o2 : o1;
     }
-    return {in, nsw, true};
+    return {in, nsw, true, cnt};
 }
 
 // Single instance: one launch per solve over the whole grid; the state
@@ -1423,6 +1442,8 @@ __global__ void __launch_bounds__(256) k_td_psi_solve(TDSolveArgs<R> T) {
         }
         st[ST_PSI] = o.cur;
         st[ST_PRED_PSI] = o.nsw;
+        st[ST_PASS_PSI] += cnt_passes(o.cnt);
+        st[ST_REDO_PSI] += cnt_redo(o.cnt);
         *state64(st, ST_EPOCH) = epoch;
         T.sc.sweeps[2 * T.step] = o.nsw;
     }
@@ -1445,6 +1466,9 @@ __global__ void __launch_bounds__(256) k_td_a_solve(TDSolveArgs<R> T) {
         }
         st[ST_AB] = o.cur;
         st[ST_PRED_A] = o.nsw;
+        st[ST_PASS_A] += cnt_passes(o.cnt);
+        st[ST_REDO_A] += cnt_redo(o.cnt);
+        st[ST_XBUF_A] += cnt_xbuf(o.cnt);
         *state64(st, ST_EPOCH) = epoch;
         T.sc.sweeps[2 * T.step + 1] = o.nsw;
     }
@@ -1486,7 +1510,7 @@ __global__ void __launch_bounds__(256) k_batch_td_step(BatchArgs<R> B) {
     const int step = st[ST_STEP];
     const int qc = st[ST_AB];
     const SolveOut o = psi_solve<R, GAUGE, true>(T, B.gs, tm, w, W, st[ST_PSI], qc, st[ST_PRED_PSI], epoch, lds);
-    SolveOut oa{qc, 0, o.ok};
+    SolveOut oa{qc, 0, o.ok, 0};
     if (o.ok && T.solveA) oa = a_solve<R, GAUGE, true>(T, B.gs, tm, w, W, o.cur, qc, st[ST_PRED_A], epoch, lds);
     if ((int)blockIdx.x == tb.x && threadIdx.x == 0) {
         if (!oa.ok) {
@@ -1528,6 +1552,7 @@ static __global__ void k_td_reset(TDScratch sc, int pc, int qc, int err0) {
         sc.state[ST_ERRSTEP] = 0;
         sc.state[ST_PRED_PSI] = 0;
         sc.state[ST_PRED_A] = 0;
+        for (int k = ST_PASS_PSI; k < ST_WORDS; ++k) sc.state[k] = 0;
     }
 }
 
