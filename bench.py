@@ -432,6 +432,25 @@ def run_batch_cg(args):
     return res
 
 
+def stream_ceiling(n=1 << 26):
+    """The practical bandwidth ceiling of this device for the psi pass's
+    traffic mix (glx_stream_probe kind 2: 37 bytes per node, 6 reads and one
+    write, as a plain grid-stride stream), best over a few grid sizes, and
+    the plain read and copy rates beside it."""
+    from svirl_amd import _native as nat
+    out = {}
+    for name, kind, m in (("psi_mix", 2, n), ("read", 0, n // 2), ("copy", 1, n // 2)):
+        best = 0.0
+        for grid in (1024, 2048, 8192, 65536):
+            g = ctypes.c_double()
+            nat.call("glx_stream_probe", kind, m, grid, 5, ctypes.byref(g))
+            best = max(best, g.value)
+        out[name + "_GBs"] = best
+    log(f"[stream] psi-pass mix {out['psi_mix_GBs']:.0f} GB/s, read {out['read_GBs']:.0f} GB/s, "
+        f"copy {out['copy_GBs']:.0f} GB/s (best over grids)")
+    return out
+
+
 def calibrate_copy(n=1 << 26, reps=3):
     """FP32 axpy over n reals (reads 8n bytes, writes 4n), for scaling the
     rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
@@ -588,6 +607,7 @@ def main():
     if args.calibrate:
         calibrate_copy()
         log("[calib] 3 FP32 axpy dispatches over 2^26 reals")
+    ceiling = stream_ceiling()
     res = {}
     for prec in (["fp32"] if args.skip_fp64 else ["fp32", "fp64"]):
         res[prec] = bench_c3(args, prec)
@@ -631,6 +651,8 @@ def main():
             "physical_GBs": (traffic / (r32["psi_launch_us"] * 1e-6) / 1e9) if traffic else None,
             "physical_frac": (traffic / (r32["psi_launch_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS) if traffic else None,
             "per_sweep_GBs": r32.get("psi_per_sweep_GBs"),
+            "stream_ceiling": ceiling,
+            "frac_of_stream_ceiling": (r32["psi_gbs"] / ceiling["psi_mix_GBs"]) if r32.get("psi_gbs") else None,
             "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if traffic else None}
     out = {
         "metric": "TDGL node-steps/s at 4097^2 (FP32-mixed; FP64 and CG iters/s beside it)",

@@ -270,6 +270,16 @@ int glx_td_solve_ms(double *psi_ms, int32_t *psi_launches, double *a_ms, int32_t
  * with an epsilon map; m = 1 with a mask). */
 int glx_td_passes(int32_t *psi_passes, int32_t *psi_redo, int32_t *a_passes, int32_t *a_redo, int32_t *a_xbuf);
 
+/* Bandwidth of a plain streaming kernel on this device (no counterpart in
+ * the reference; a measurement hook giving the practical ceiling next to
+ * the HBM peak): kind 0 reads n float4, kind 1 copies n float4, kind 2
+ * streams the traffic of one FP32 psi pass over n nodes (psi, right-hand
+ * side, a, b, eps and a mask byte read, psi written: 37 bytes per node).
+ * grid workgroups of 256 threads, reps timed launches after one warm-up;
+ * *gbs receives bytes moved / time in GB/s. Allocates and frees its own
+ * buffers. */
+int glx_stream_probe(int32_t kind, int64_t n, int32_t grid, int32_t reps, double *gbs);
+
 /* Failure-handling hook (no counterpart in the reference; for tests of the
  * timeout path). The next glx_td_run or glx_batch_td_run call (including
  * one made by glx_td_step / glx_td_psi_solve / glx_td_a_solve) starts with

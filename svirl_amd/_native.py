@@ -91,6 +91,7 @@ _PROTOS = {
     "glx_batch_cg_run": [c_int, ctypes.POINTER(BatchCGInstance), c_i32, c_i32, c_double, c_i32],
     "glx_td_solve_ms": [ctypes.POINTER(c_double), P_i32, ctypes.POINTER(c_double), P_i32],
     "glx_td_passes": [P_i32, P_i32, P_i32, P_i32, P_i32],
+    "glx_stream_probe": [c_i32, c_i64, c_i32, c_i32, ctypes.POINTER(c_double)],
     "glx_magnetic_field": [c_int, Grid, c_void_p, c_void_p, c_void_p],
     "glx_current_density": [c_int, Grid, c_double, c_double, c_void_p, c_void_p, c_void_p],
     "glx_supercurrent_density": [c_int, Grid, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],

@@ -69,6 +69,32 @@ using namespace glx;
 // kernels and copies. Every call on one device shares the array.
 static __device__ __align__(256) unsigned char scratch[GLX_NOALLOC_SCRATCH_BYTES];
 
+// Streaming kernels of known traffic for glx_stream_probe (grid-stride
+// loops bounded by the element count).
+static __global__ void k_probe_read(const float4 *__restrict__ a, float *out, long n) {
+    float acc = 0.f;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float4 v = a[i];
+        acc += (v.x + v.y) + (v.z + v.w);
+    }
+    if (acc == 1.0e30f) out[0] = acc;  // keeps the loads; never true for the zeroed input
+}
+static __global__ void k_probe_copy(const float4 *__restrict__ a, float4 *__restrict__ b, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
+}
+// The traffic of one FP32 psi pass per node: psi and the right-hand side
+// (8 B each), a, b and eps (4 B each) and a mask byte read, psi out (8 B).
+static __global__ void k_probe_psi_mix(const float2 *__restrict__ p, const float2 *__restrict__ r,
+                                       const float *__restrict__ a, const float *__restrict__ b,
+                                       const float *__restrict__ e, const unsigned char *__restrict__ m,
+                                       float2 *__restrict__ o, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float2 pv = p[i], rv = r[i];
+        const float s = a[i] + b[i] + e[i] + (float)m[i];
+        o[i] = make_float2(pv.x * s + rv.x, pv.y * s + rv.y);
+    }
+}
+
 #define GLX_REQUIRE_INIT() do { int _r = ensure_ready(); if (_r) return _r; } while (0)
 #define GLX_DISPATCH(prec, call)                                                   \
     do {                                                                           \
@@ -346,6 +372,57 @@ int glx_td_passes(int32_t *psi_passes, int32_t *psi_redo, int32_t *a_passes, int
     *a_passes = g_ctx.passes[2];
     *a_redo = g_ctx.passes[3];
     *a_xbuf = g_ctx.passes[4];
+    return GLX_OK;
+}
+
+int glx_stream_probe(int32_t kind, int64_t n, int32_t grid, int32_t reps, double *gbs) {
+    GLX_REQUIRE_INIT();
+    if (kind < 0 || kind > 2 || n < 1024 || grid < 1 || reps < 1 || !gbs)
+        return set_error(GLX_E_ARG, "glx_stream_probe: invalid arguments");
+    // kind 0: read n float4; 1: copy n float4; 2: the psi-pass mix over n nodes
+    const size_t f4 = (size_t)n * sizeof(float4);
+    const size_t sizes[7] = {kind == 2 ? (size_t)n * 8 : f4, kind == 1 ? f4 : (kind == 2 ? (size_t)n * 8 : 64),
+                             kind == 2 ? (size_t)n * 4 : 0, kind == 2 ? (size_t)n * 4 : 0,
+                             kind == 2 ? (size_t)n * 4 : 0, kind == 2 ? (size_t)n : 0,
+                             kind == 2 ? (size_t)n * 8 : 0};
+    void *buf[7] = {};
+    hipError_t st = hipSuccess;
+    for (int k = 0; k < 7 && st == hipSuccess; ++k)
+        if (sizes[k]) {
+            st = hipMalloc(&buf[k], sizes[k]);
+            if (st == hipSuccess) st = hipMemsetAsync(buf[k], 0, sizes[k], g_ctx.stream);
+        }
+    double bytes = 0.0;
+    auto launch = [&]() {
+        if (kind == 0) {
+            hipLaunchKernelGGL(k_probe_read, dim3(grid), dim3(256), 0, g_ctx.stream, (const float4 *)buf[0], (float *)buf[1], (long)n);
+            bytes = double(f4);
+        } else if (kind == 1) {
+            hipLaunchKernelGGL(k_probe_copy, dim3(grid), dim3(256), 0, g_ctx.stream, (const float4 *)buf[0], (float4 *)buf[1], (long)n);
+            bytes = 2.0 * double(f4);
+        } else {
+            hipLaunchKernelGGL(k_probe_psi_mix, dim3(grid), dim3(256), 0, g_ctx.stream, (const float2 *)buf[0],
+                               (const float2 *)buf[1], (const float *)buf[2], (const float *)buf[3], (const float *)buf[4],
+                               (const unsigned char *)buf[5], (float2 *)buf[6], (long)n);
+            bytes = 37.0 * double(n);
+        }
+    };
+    float ms = 0.f;
+    if (st == hipSuccess) {
+        launch();  // warm-up
+        st = hipEventRecord(g_ctx.ev0, g_ctx.stream);
+        for (int k = 0; k < reps && st == hipSuccess; ++k) {
+            launch();
+            st = hipGetLastError();
+        }
+        if (st == hipSuccess) st = hipEventRecord(g_ctx.ev1, g_ctx.stream);
+        if (st == hipSuccess) st = hipEventSynchronize(g_ctx.ev1);
+        if (st == hipSuccess) st = hipEventElapsedTime(&ms, g_ctx.ev0, g_ctx.ev1);
+    }
+    for (int k = 0; k < 7; ++k)
+        if (buf[k]) (void)hipFree(buf[k]);
+    if (st != hipSuccess) return check_hip(st, "glx_stream_probe");
+    *gbs = bytes * reps / (double(ms) * 1e-3) / 1e9;
     return GLX_OK;
 }
 

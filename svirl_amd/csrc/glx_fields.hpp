@@ -1214,41 +1214,91 @@ __device__ __forceinline__ double np_mod_2pi(double x) {
 
 constexpr int VREC = 14;
 
+// Winding number of every cell (vortex_detector.py:55-72) with each node's
+// phase evaluated once. Cells are cut into strips of 63 columns; a wave
+// marches down a range of cell rows of one strip, lane l holding node column
+// i = 63 s + l of the two node rows j, j + 1 that bound cell row j (phase,
+// psi and the a link of each) and the b link of row j; lane l + 1 supplies
+// the right-hand corner values by a lane shift, so lanes 0..62 evaluate
+// their cells with the reference's expression, term by term. Flagged cells
+// are appended with one atomic per wave and row (ballot, then each lane's
+// slot is the base plus the flagged lanes below it). Bounds: node rows j + 1
+// <= Ny - 1 and b rows j <= Ny - 2 since j < Nyc; columns past the grid are
+// clamped to the last one for their loads and never evaluate a cell.
+constexpr int VSTRIP = WAVE - 1;
+
+inline long vortex_waves(const Geo &g) {
+    const long T = (long)((g.Nxc + VSTRIP - 1) / VSTRIP) * g.Nyc;
+    const long W = (T + 15) / 16;  // about 16 rows per wave
+    return W < 1 ? 1 : W;
+}
+
 template <typename R>
-__global__ void __launch_bounds__(256) k_vortex_cells(Geo g, const cplx<R> *psi, const R *ab, const R *abi, double H, double *rec,
-                               int capacity, int *count) {
+__global__ void __launch_bounds__(256) k_vortex_cells(Geo g, const cplx<R> *psi, const R *ab, const R *abi, double H,
+                                                      double *rec, int capacity, int *count, long W) {
     const double pi = 3.141592653589793;
     const double dxdyH = g.dx * g.dy * H;
-    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.Nc; n += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(n % g.Nxc), j = (int)(n / g.Nxc);
-        const long q00 = i + (long)g.Nx * j;
-        const cplx<R> s00 = ld(psi, q00), sp0 = ld(psi, q00 + 1), spp = ld(psi, q00 + 1 + g.Nx), s0p = ld(psi, q00 + g.Nx);
-        const double t00 = atan2(double(s00.im), double(s00.re));
-        const double tp0 = atan2(double(sp0.im), double(sp0.re));
-        const double tpp = atan2(double(spp.im), double(spp.re));
-        const double t0p = atan2(double(s0p.im), double(s0p.re));
-        const long ka00 = i + (long)g.Nxa * j, ka0p = i + (long)g.Nxa * (j + 1);
-        const long kb00 = g.Na + i + (long)g.Nxb * j, kbp0 = g.Na + i + 1 + (long)g.Nxb * j;
-        const double a00 = double(ab[ka00]), a0p = double(ab[ka0p]), b00 = double(ab[kb00]), bp0 = double(ab[kbp0]);
-        const double v = -(0.5 / pi) * (np_mod_2pi(tp0 - t00 - g.dx * a00 + pi) + np_mod_2pi(tpp - tp0 - g.dy * bp0 + pi) +
-                                          np_mod_2pi(t0p - tpp + g.dx * a0p + pi) + np_mod_2pi(t00 - t0p + g.dy * b00 + pi) -
-                                          4.0 * pi + dxdyH);
-        if (fabs(v) > 0.5 && fabs(v - rint(v)) < 0.1) {
-            const int slot = atomicAdd(count, 1);
-            if (slot < capacity) {
-                double *o = rec + (long)slot * VREC;
-                o[0] = double(n);
-                o[1] = v;
-                o[2] = double(s00.re); o[3] = double(s00.im);
-                o[4] = double(sp0.re); o[5] = double(sp0.im);
-                o[6] = double(spp.re); o[7] = double(spp.im);
-                o[8] = double(s0p.re); o[9] = double(s0p.im);
-                o[10] = a00 + (abi ? double(abi[ka00]) : 0.0);
-                o[11] = a0p + (abi ? double(abi[ka0p]) : 0.0);
-                o[12] = b00 + (abi ? double(abi[kb00]) : 0.0);
-                o[13] = bp0 + (abi ? double(abi[kbp0]) : 0.0);
+    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
+    if (w >= W) return;  // wave-uniform
+    const int lane = threadIdx.x & (WAVE - 1);
+    const long ns = (g.Nxc + VSTRIP - 1) / VSTRIP, T = ns * g.Nyc;
+    long r0 = (w * T) / W;
+    const long r1 = ((w + 1) * T) / W;
+    const R *bl = ab + g.Na;
+    while (r0 < r1) {
+        const int s = (int)(r0 / g.Nyc), ja = (int)(r0 % g.Nyc);
+        const int jb = (int)min((long)g.Nyc, (long)ja + (r1 - r0));
+        const int i = s * VSTRIP + lane;
+        const bool col = i < g.Nx, cell = lane < VSTRIP && i < g.Nxc;
+        const int ic = min(i, g.Nx - 1), iac = min(i, g.Nxa - 1);
+        cplx<R> p0 = ld(psi, ic + (long)g.Nx * ja);
+        double t0 = col ? atan2(double(p0.im), double(p0.re)) : 0.0;
+        double a0 = double(ab[iac + (long)g.Nxa * ja]);
+        for (int j = ja; j < jb; ++j) {
+            const cplx<R> p1 = ld(psi, ic + (long)g.Nx * (j + 1));
+            const double t1 = col ? atan2(double(p1.im), double(p1.re)) : 0.0;
+            const double a1 = double(ab[iac + (long)g.Nxa * (j + 1)]);
+            const double b0 = double(bl[ic + (long)g.Nxb * j]);
+            const double t0r = shfl_dn1(t0), t1r = shfl_dn1(t1), b0r = shfl_dn1(b0);
+            double v = 0.0;
+            bool flag = false;
+            if (cell) {
+                v = -(0.5 / pi) * (np_mod_2pi(t0r - t0 - g.dx * a0 + pi) + np_mod_2pi(t1r - t0r - g.dy * b0r + pi) +
+                                   np_mod_2pi(t1 - t1r + g.dx * a1 + pi) + np_mod_2pi(t0 - t1 + g.dy * b0 + pi) -
+                                   4.0 * pi + dxdyH);
+                flag = fabs(v) > 0.5 && fabs(v - rint(v)) < 0.1;
             }
+            const unsigned long long m = __ballot(flag);
+            if (m) {  // wave-uniform
+                const R p0r_re = shfl_dn1(p0.re), p0r_im = shfl_dn1(p0.im);
+                const R p1r_re = shfl_dn1(p1.re), p1r_im = shfl_dn1(p1.im);
+                int base = 0;
+                if (lane == 0) base = atomicAdd(count, (int)__popcll(m));
+                base = __shfl(base, 0, WAVE);
+                if (flag) {
+                    const int slot = base + (int)__popcll(m & ((1ull << lane) - 1ull));
+                    if (slot < capacity) {
+                        const long ka00 = i + (long)g.Nxa * j, ka0p = i + (long)g.Nxa * (j + 1);
+                        const long kb00 = g.Na + i + (long)g.Nxb * j, kbp0 = kb00 + 1;
+                        double *o = rec + (long)slot * VREC;
+                        o[0] = double(i + (long)g.Nxc * j);
+                        o[1] = v;
+                        o[2] = double(p0.re); o[3] = double(p0.im);
+                        o[4] = double(p0r_re); o[5] = double(p0r_im);
+                        o[6] = double(p1r_re); o[7] = double(p1r_im);
+                        o[8] = double(p1.re); o[9] = double(p1.im);
+                        o[10] = a0 + (abi ? double(abi[ka00]) : 0.0);
+                        o[11] = a1 + (abi ? double(abi[ka0p]) : 0.0);
+                        o[12] = b0 + (abi ? double(abi[kb00]) : 0.0);
+                        o[13] = b0r + (abi ? double(abi[kbp0]) : 0.0);
+                    }
+                }
+            }
+            p0 = p1;
+            t0 = t1;
+            a0 = a1;
         }
+        r0 += jb - ja;
     }
 }
 

@@ -717,8 +717,9 @@ int L<R>::vortex_cells(const Env &e, glx_grid g, const void *psi, const void *ab
     int *dcount = (int *)e.result;
     hipError_t st = hipMemsetAsync(dcount, 0, sizeof(int), e.stream);
     if (st != hipSuccess) return check_hip(st, "hipMemsetAsync(count)");
-    hipLaunchKernelGGL(k_vortex_cells<R>, dim3(grid_blocks(G.Nc)), dim3(256), 0, e.stream, G, (const cplx<R> *)psi,
-                       (const R *)ab, (const R *)abi, double(R(H)), rec, (int)capacity, dcount);
+    const long W = vortex_waves(G);
+    hipLaunchKernelGGL(k_vortex_cells<R>, dim3((unsigned)((W + 3) / 4)), dim3(256), 0, e.stream, G, (const cplx<R> *)psi,
+                       (const R *)ab, (const R *)abi, double(R(H)), rec, (int)capacity, dcount, W);
     GLX_LAUNCH_CHECK("k_vortex_cells");
     int c = 0;
     st = hipMemcpyAsync(&c, dcount, sizeof(int), hipMemcpyDeviceToHost, e.stream);
