@@ -26,6 +26,8 @@ struct Ctx {
     hipDeviceProp_t prop{};
     Env env{};
     void *lib_scratch = nullptr;  // partials, barrier words, run state (hipMalloc at glx_init)
+    void *grow = nullptr;         // Env::grow
+    size_t grow_bytes = 0;
 };
 
 Ctx g_ctx;
@@ -132,6 +134,8 @@ int glx_init(int device) {
     g_ctx.env.solve_ms = g_ctx.solve_ms;
     g_ctx.env.solve_n = g_ctx.solve_n;
     g_ctx.env.passes = g_ctx.passes;
+    g_ctx.env.grow = &g_ctx.grow;
+    g_ctx.env.grow_bytes = &g_ctx.grow_bytes;
     // library scratch layout: partials | result | arrive | rel | err | state | sweeps
     const size_t partials = sizeof(double) * 1024 * 32;
     const size_t result = sizeof(double) * 64;

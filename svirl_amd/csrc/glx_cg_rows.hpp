@@ -1,0 +1,362 @@
+// Row-marching forms of the two passes of a CG iteration over one grid:
+// the gradient pass (free energy, both Jacobians, Polak-Ribiere sums, the
+// state update optionally fused in) and the coefficient pass with the
+// direction update. Same layout of the work as the TD sweeps: strips of 62
+// columns, lanes 0 and 63 of a wave carry the columns either side, a wave
+// marches down a range of rows keeping a window of its column in
+// registers, and the values of the columns to the left and right come from
+// the neighbouring lanes. Every field value is loaded once per pass (the
+// node-parallel forms load psi five times and each link value up to four
+// times, through the caches), and each link's sin/cos is evaluated once:
+// the left and down link factors of a node are its neighbours' right and up
+// link factors with the orientation reversed, (sin, cos)(-x) = (-sin x,
+// cos x), which sincos_r returns bitwise. The node arithmetic is grad_eval /
+// coef_eval's, so the results equal the node-parallel forms to the bit;
+// only the order of the FP64 partial sums differs.
+#pragma once
+
+#include "glx_fields.hpp"
+#include "glx_td.hpp"
+
+namespace glx {
+
+// Waves of a row-marching CG pass: about 16 rows per wave, at most 4096
+// waves (1024 workgroups, the partials' capacity).
+inline long cg_row_waves(int Nx, int Ny) {
+    const long T = (long)nstrips_of<1>(Nx) * Ny;
+    long W = (T + 15) / 16;
+    if (W > 4 * RED_MAX_BLOCKS) W = 4 * RED_MAX_BLOCKS;
+    return W < 1 ? 1 : W;
+}
+
+// The link values a lane holds for one link family at one row: abei (E)
+// and ab (B), the latter with the state update applied (UPD).
+template <typename R> struct LinkPair {
+    R e, b;
+};
+
+template <typename R, bool withA, bool UPD>
+__global__ void __launch_bounds__(256) k_cg_grad_rows(Geo g, R kappa2, R eps, const R *eps_map, R H, const uint8_t *mt,
+                                                      const cplx<R> *psi, const R *abei, const R *ab, cplx<R> *jac,
+                                                      const cplx<R> *jac_prev, R *jacA, const R *jacA_prev,
+                                                      double *partials, StateUpdate<R> S, const double *alpha, long W) {
+    const GeoR<R> q(g);
+    if constexpr (UPD) {
+        S.ap = R(alpha[0]);
+        S.aa = R(alpha[1]);
+    }
+    const R dxdy = q.dx * q.dy, s2 = R(2) * q.dx * q.dy;
+    const bool he = abei != nullptr, hb = ab != nullptr, has_mt = mt != nullptr;
+    const bool upd_a = UPD && S.dab != nullptr;
+    const R *ae_b = he ? abei + g.Na : nullptr;
+    const R *ab_b = hb ? ab + g.Na : nullptr;
+    const R *da_b = upd_a ? S.dab + g.Na : nullptr;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
+    if (w < W) for_rows(g.Nx, g.Ny, w, W, [&](int s, int ja, int jb) {
+        const ColCtx X = col_ctx<1>(g.Nx, s);
+        const int c = X.c;
+        const uint32_t cc = X.cc, ca = X.ca;
+        // Raw loads of one marching step: psi at row j+1 and the a links
+        // there, the b links, cell flag, epsilon and previous gradients of
+        // row j. A step's loads are issued one step ahead, before the
+        // previous step's arithmetic, and the state update (UPD) is applied
+        // where they are consumed. Indices are clamped into the grid (rows
+        // past it are never used).
+        struct Step {
+            cplx<R> p, dp, gp;
+            R ae, ab, ad, be, bb, bd, ee, vpa, vpb;
+            uint8_t mc;
+        };
+        auto kpsi = [&](int j) { return (long)g.Nx * min(max(j, 0), g.Ny - 1) + cc; };
+        auto ka_of = [&](int j) { return ca + (long)g.Nxa * min(max(j, 0), g.Ny - 1); };
+        auto kb_of = [&](int j) { return g.Na + cc + (long)g.Nxb * min(max(j, 0), g.Ny - 2); };
+        auto kc_of = [&](int j) { return ca + (long)g.Nxc * min(max(j, 0), g.Ny - 2); };
+        auto lpsi = [&](int j, cplx<R> &p, cplx<R> &d) {
+            const long k = kpsi(j);
+            p = ld(psi, k);
+            if constexpr (UPD) d = ld(S.dpsi, k);
+        };
+        auto llink = [&](long k, R &e, R &b, R &d) {
+            if (he) e = abei[k];
+            if (hb) b = ab[k];
+            if (upd_a) d = S.dab[k];
+        };
+        auto fetch = [&](int j) {
+            Step t{};
+            lpsi(j + 1, t.p, t.dp);
+            llink(ka_of(j + 1), t.ae, t.ab, t.ad);
+            llink(kb_of(j), t.be, t.bb, t.bd);
+            t.mc = has_mt ? mt[kc_of(j)] : uint8_t(1);
+            const long n = kpsi(j);
+            t.ee = eps_map ? eps_map[n] : eps;
+            t.gp = ld(jac_prev, n);
+            if (withA) {
+                t.vpa = jacA_prev[ka_of(j)];
+                t.vpb = jacA_prev[kb_of(j)];
+            }
+            return t;
+        };
+        auto upd_psi = [&](cplx<R> p, const cplx<R> &d) {
+            if constexpr (UPD) {
+                p.re = S.ap * d.re + p.re;
+                p.im = S.ap * d.im + p.im;
+            }
+            return p;
+        };
+        auto upd_link = [&](R e, R b, R d) {
+            LinkPair<R> v{e, b};
+            if (upd_a) v.b = S.aa * d + v.b;
+            return v;
+        };
+        auto pv = [&](const LinkPair<R> &v) {  // pv(abei, ab, k) of loaded values
+            R p = R(0);
+            if (he) p += v.e;
+            if (hb) p += v.b;
+            return p;
+        };
+        // window at the first row: psi at rows ja-1, ja; a at rows ja-1,
+        // ja; b and the cell at row ja-1 and the up-link factors of row
+        // ja-1 (the down link of row ja)
+        cplx<R> Pd, Pc;
+        LinkPair<R> Ad, Ac, Bd;
+        bool md;
+        {
+            cplx<R> p{}, d{};
+            lpsi(ja - 1, p, d);
+            Pd = upd_psi(p, d);
+            lpsi(ja, p, d);
+            Pc = upd_psi(p, d);
+            R le = R(0), lb = R(0), ld_ = R(0);
+            llink(ka_of(ja - 1), le, lb, ld_);
+            Ad = upd_link(le, lb, ld_);
+            llink(ka_of(ja), le, lb, ld_);
+            Ac = upd_link(le, lb, ld_);
+            llink(kb_of(ja - 1), le, lb, ld_);
+            Bd = upd_link(le, lb, ld_);
+            md = has_mt ? mt[kc_of(ja - 1)] != 0 : true;
+        }
+        R sud = R(0), cud = R(1);
+        if (ja > 0) {
+            const bool mdl = shfl_up1((int)md) != 0;
+            const bool lu_d = (c > 0 && mdl) || (c + 1 < g.Nx && md);  // cells (c-1, ja-1), (c, ja-1)
+            if (X.col && lu_d) sincos_r(q.dy * pv(Bd), &sud, &cud);
+        }
+        Step nx = fetch(ja);
+        for (int j = ja; j < jb; ++j) {
+            const Step t = nx;
+            nx = fetch(min(j + 1, jb - 1));
+            const long n = (long)g.Nx * j + cc;
+            const cplx<R> Pu = upd_psi(t.p, t.dp);
+            const LinkPair<R> Au = upd_link(t.ae, t.ab, t.ad), Bc = upd_link(t.be, t.bb, t.bd);
+            const bool mc = t.mc != 0;
+            GradIn<R> in;
+            in.ee = t.ee;
+            const cplx<R> gp = t.gp;
+            const long ka = ca + (long)g.Nxa * j, kb = g.Na + cc + (long)g.Nxb * min(j, g.Ny - 2);
+            const R vpa = withA ? t.vpa : R(0), vpb = withA ? t.vpb : R(0);
+            // node values from the window and the neighbouring lanes
+            in.p0 = Pc;
+            in.pd = Pd;
+            in.pu = Pu;
+            in.pl.re = shfl_up1(Pc.re);
+            in.pl.im = shfl_up1(Pc.im);
+            in.pr.re = shfl_dn1(Pc.re);
+            in.pr.im = shfl_dn1(Pc.im);
+            in.c = cells_from(g, has_mt, c, j, shfl_up1((int)md) != 0, shfl_up1((int)mc) != 0, md, mc);
+            const Cells cl = in.c;
+            auto links = [&](bool e_part) {
+                auto v = [&](const LinkPair<R> &x) { return e_part ? x.e : x.b; };
+                Links10<R> L;
+                L.a00 = v(Ac);
+                L.am0 = shfl_up1(v(Ac));
+                L.a0m = v(Ad);
+                L.a0p = v(Au);
+                L.amp = shfl_up1(v(Au));
+                L.b00 = v(Bc);
+                L.b0m = v(Bd);
+                L.bm0 = shfl_up1(v(Bc));
+                L.bp0 = shfl_dn1(v(Bc));
+                L.bpm = shfl_dn1(v(Bd));
+                return L;
+            };
+            in.E = he ? links(true) : Links10<R>{};
+            in.B = hb ? links(false) : Links10<R>{};
+            // link factors: the right and up links here, the left one from
+            // the lane to the left, the down one from the row below
+            in.sr = in.su = R(0);
+            in.cr = in.cu = R(1);
+            if (X.col && (cl.pm || cl.pp)) sincos_r(q.dx * pv(Ac), &in.sr, &in.cr);
+            if (X.col && (cl.mp || cl.pp)) sincos_r(q.dy * pv(Bc), &in.su, &in.cu);
+            in.sl = -shfl_up1(in.sr);
+            in.cl = shfl_up1(in.cr);
+            in.sd = -sud;
+            in.cd = cud;
+            if (X.own) {
+                if constexpr (UPD) {
+                    st(S.psi_out, n, Pc);
+                    if (upd_a) {
+                        if (c < g.Nxa) S.ab_out[ka] = Ac.b;
+                        if (j < g.Nyb) S.ab_out[kb] = Bc.b;
+                    }
+                }
+                R e, ga, gb;
+                cplx<R> gj;
+                bool ha, hbb;
+                grad_eval<R, withA>(g, q, kappa2, H, he, hb, in, c, j, e, gj, ga, gb, ha, hbb, R(1));
+                acc[0] += double(e);
+                const cplx<R> gv = {dxdy * gj.re, dxdy * gj.im};
+                st(jac, n, gv);
+                acc[1] += double(gv.re * (gv.re - gp.re));
+                acc[1] += double(gv.im * (gv.im - gp.im));
+                acc[2] += double(gp.re * gp.re);
+                acc[2] += double(gp.im * gp.im);
+                if (withA) {
+                    if (ha) {
+                        const R v = s2 * ga;
+                        jacA[ka] = v;
+                        acc[3] += double(v * (v - vpa));
+                        acc[4] += double(vpa * vpa);
+                    }
+                    if (hbb) {
+                        const R v = s2 * gb;
+                        jacA[kb] = v;
+                        acc[3] += double(v * (v - vpb));
+                        acc[4] += double(vpb * vpb);
+                    }
+                }
+            }
+            // roll the window
+            Pd = Pc;
+            Pc = Pu;
+            Ad = Ac;
+            Ac = Au;
+            Bd = Bc;
+            md = mc;
+            sud = in.su;
+            cud = in.cu;
+        }
+    });
+    block_store_partials<5>(acc, partials);
+}
+
+// The line-search coefficients of the new direction d = beta d_old - g,
+// which the pass forms once per node where it reads it and writes to
+// U.dout / U.dAout (glx_cg_direction_step; k_coef_dir's results, node by
+// node). Partials as k_coef_dir leaves them: the L sums per workgroup, then
+// (L = 17) each workgroup's largest link phase step.
+template <typename R> struct LinkTriple {
+    R e, b, d;  // abei, ab, the new A direction
+};
+
+template <typename R, int L>
+__global__ void __launch_bounds__(256) k_coef_dir_rows(Geo g, R kappa2, R eps, const R *eps_map, R H, const uint8_t *mt,
+                                                       const cplx<R> *psi, const R *abei, const R *ab, DirUpdate<R> U,
+                                                       const double *beta, double *partials, long W) {
+    const GeoR<R> q(g);
+    U.bp = R(beta[0]);
+    U.ba = R(beta[1]);
+    const bool he = abei != nullptr, hb = ab != nullptr, has_mt = mt != nullptr;
+    double acc[L];
+#pragma unroll
+    for (int k = 0; k < L; ++k) acc[k] = 0.0;
+    double th = 0.0;
+    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
+    if (w < W) for_rows(g.Nx, g.Ny, w, W, [&](int s, int ja, int jb) {
+        const ColCtx X = col_ctx<1>(g.Nx, s);
+        const int c = X.c;
+        const uint32_t cc = X.cc, ca = X.ca;
+        // Raw loads of one marching step (psi, the old direction and the
+        // gradient at row j+1, the a links there, the b links, cell flag and
+        // epsilon of row j), issued one step ahead of their use; the new
+        // direction beta d_old - g is formed where they are consumed.
+        struct Lk {
+            R e, b, o, g;  // abei, ab, old A direction, A gradient
+        };
+        struct Step {
+            cplx<R> p, o, gr;
+            Lk a, b;
+            R eps;
+            uint8_t mc;
+        };
+        auto row = [&](int j) { return (long)g.Nx * min(max(j, 0), g.Ny - 1) + cc; };
+        auto ka_of = [&](int j) { return ca + (long)g.Nxa * min(max(j, 0), g.Ny - 1); };
+        auto kb_of = [&](int j) { return g.Na + cc + (long)g.Nxb * min(max(j, 0), g.Ny - 2); };
+        auto link = [&](long k) {
+            Lk v{R(0), R(0), R(0), R(0)};
+            if (he) v.e = abei[k];
+            if (hb) v.b = ab[k];
+            if (L == 17) {
+                v.o = U.dAold[k];
+                v.g = U.jacA[k];
+            }
+            return v;
+        };
+        auto dlink = [&](const Lk &v) { return LinkTriple<R>{v.e, v.b, L == 17 ? U.ba * v.o - v.g : R(0)}; };
+        auto dnode = [&](const cplx<R> &o, const cplx<R> &gk) {
+            return cplx<R>{U.bp * o.re - gk.re, U.bp * o.im - gk.im};
+        };
+        auto fetch = [&](int j) {
+            Step t;
+            const long k = row(j + 1);
+            t.p = ld(psi, k);
+            t.o = ld(U.dold, k);
+            t.gr = ld(U.jac, k);
+            t.a = link(ka_of(j + 1));
+            t.b = link(kb_of(j));
+            t.mc = has_mt ? mt[ca + (long)g.Nxc * min(max(j, 0), g.Ny - 2)] : uint8_t(1);
+            t.eps = eps_map ? eps_map[row(j)] : eps;
+            return t;
+        };
+        // window: psi, direction and the a links at row ja; the cell below
+        cplx<R> Pc = ld(psi, row(ja)), Dc = dir_at(U, row(ja));
+        LinkTriple<R> Ac = dlink(link(ka_of(ja)));
+        bool md = has_mt ? mt[ca + (long)g.Nxc * min(max(ja - 1, 0), g.Ny - 2)] != 0 : true;
+        Step nx = fetch(ja);
+        for (int j = ja; j < jb; ++j) {
+            const Step t = nx;
+            nx = fetch(min(j + 1, jb - 1));
+            const long n = (long)g.Nx * j + cc;
+            const cplx<R> Pu = t.p, Du = dnode(t.o, t.gr);
+            const LinkTriple<R> Au = dlink(t.a), Bc = dlink(t.b);
+            const bool mc = t.mc != 0;
+            CoefIn<R> in;
+            in.eps = t.eps;
+            in.p0 = Pc;
+            in.p_u = Pu;
+            in.p_r.re = shfl_dn1(Pc.re);
+            in.p_r.im = shfl_dn1(Pc.im);
+            in.d0 = Dc;
+            in.d_u = Du;
+            in.d_r.re = shfl_dn1(Dc.re);
+            in.d_r.im = shfl_dn1(Dc.im);
+            in.c = cells_from(g, has_mt, c, j, shfl_up1((int)md) != 0, shfl_up1((int)mc) != 0, md, mc);
+            in.E = Links4<R>{Ac.e, Au.e, Bc.e, shfl_dn1(Bc.e)};
+            in.B = Links4<R>{Ac.b, Au.b, Bc.b, shfl_dn1(Bc.b)};
+            in.D = Links4<R>{};
+            if (L == 17) in.D = Links4<R>{Ac.d, Au.d, Bc.d, shfl_dn1(Bc.d)};
+            if (X.own) {
+                st(U.dout, n, Dc);
+                if (L == 17) {
+                    if (c < g.Nxa) U.dAout[ca + (long)g.Nxa * j] = Ac.d;
+                    if (j < g.Nyb) U.dAout[g.Na + cc + (long)g.Nxb * j] = Bc.d;
+                    th = fmax(th, fmax(fabs(double(q.dx * Ac.d)), fabs(double(q.dy * Bc.d))));
+                }
+                R C[L];
+#pragma unroll
+                for (int k = 0; k < L; ++k) C[k] = R(0);
+                coef_eval<R, L>(q, kappa2, H, he, hb, in, c, j, c < g.Nxc && j < g.Nyc, C);
+#pragma unroll
+                for (int k = 0; k < L; ++k) acc[k] += double(C[k]);
+            }
+            Pc = Pu;
+            Dc = Du;
+            Ac = Au;
+            md = mc;
+        }
+    });
+    block_store_partials<L>(acc, partials);
+    if (L == 17) block_store_phase_max(th, partials, L);
+}
+
+}  // namespace glx

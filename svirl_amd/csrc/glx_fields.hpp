@@ -8,6 +8,8 @@
 // Per-node formulas follow svirl/cuda/observables.h and cg.h term by term.
 #pragma once
 
+#include <climits>
+
 #include "glx_device.hpp"
 
 namespace glx {
@@ -592,11 +594,133 @@ template <typename R> struct StateUpdate {
     R ap, aa;  // alpha_psi, alpha_A rounded to the working precision
 };
 
+// Everything grad_eval reads of node (i, j): psi there and at its four
+// neighbours, epsilon, the material flags of the four cells, the ten link
+// values of abei (E) and ab (B) around it, and the link factors (sin, cos)
+// of d A on the right link a(i, j) and the up link b(i, j) (valid where
+// that link touches material) and of -d A on the left link a(i-1, j) and
+// the down link b(i, j-1) (likewise). grad_node gathers them with loads;
+// the row-marching gradient pass takes most of them from its register
+// window and the neighbouring lanes.
+template <typename R> struct GradIn {
+    cplx<R> p0, pl, pr, pd, pu;
+    R ee;
+    Cells c;
+    Links10<R> E, B;
+    R sr, cr, su, cu, sl, cl, sd, cd;
+};
+
+// The free-energy density, dF/dpsi and dF/dA of node (i, j) from gathered
+// values (fe_node, jac_psi_node and jac_A_node fused, each term in the
+// separate functions' order).
+template <typename R, bool withA>
+__device__ __forceinline__ void grad_eval(const Geo &g, const GeoR<R> &q, R kappa2, R H, bool he, bool hb,
+                                          const GradIn<R> &in, int i, int j, R &e, cplx<R> &gj, R &ga, R &gb,
+                                          bool &has_a, bool &has_b, R bscale) {
+    const cplx<R> p0 = in.p0;
+    const Cells c = in.c;
+    const Links10<R> &E = in.E, &B = in.B;
+    const cplx<R> zero = {R(0), R(0)};
+    const bool lr = c.pm || c.pp, lu = c.mp || c.pp;  // the right link a(i, j) and the up link b(i, j) are active
+    const R sr = in.sr, cr = in.cr, su = in.su, cu = in.cu;
+    // |psi1 U - psi0|^2 and Im(conj psi0 U psi1) with U's (sin, cos) given
+    auto gg = [&](R s, R cs, cplx<R> p1) {
+        const R re = p1.re * cs + p1.im * s - p0.re;
+        const R im = p1.im * cs - p1.re * s - p0.im;
+        return re * re + im * im;
+    };
+    auto jsu = [&](R s, R cs, cplx<R> p1) {
+        return (p0.re * p1.im - p0.im * p1.re) * cs - (p0.re * p1.re + p0.im * p1.im) * s;
+    };
+    e = R(0);
+    gj = zero;
+    if (c.mm || c.mp || c.pm || c.pp) {
+        const R f_mm = c.mm ? R(1) : R(0), f_mp = c.mp ? R(1) : R(0), f_pm = c.pm ? R(1) : R(0), f_pp = c.pp ? R(1) : R(0);
+        const R r_m0 = R(0.5) * (f_mm + f_mp), r_p0 = R(0.5) * (f_pm + f_pp);
+        const R r_0m = R(0.5) * (f_mm + f_pm), r_0p = R(0.5) * (f_mp + f_pp);
+        const R gw = R(0.25) * (r_m0 + r_p0 + r_0m + r_0p);
+        // fe_node
+        const R p2 = p0.re * p0.re + p0.im * p0.im;
+        e += gw * (R(0.5) * p2 - in.ee) * p2;
+        if (lr) e += r_p0 * q.idx2 * gg(sr, cr, in.pr);
+        if (lu) e += r_0p * q.idy2 * gg(su, cu, in.pu);
+        // jac_psi_node
+        const R p = p0.re * p0.re + p0.im * p0.im - in.ee;
+        const R k = R(2) * gw * p;
+        gj.re += k * p0.re;
+        gj.im += k * p0.im;
+        auto add = [&](R r, R id2, R s, R cs, cplx<R> p1) {
+            const R tr = R(2) * (p0.re - (p1.re * cs + p1.im * s));
+            const R ti = R(2) * (p0.im - (p1.im * cs - p1.re * s));
+            const R w = r * id2;
+            gj.re += w * tr;
+            gj.im += w * ti;
+        };
+        if (c.mm || c.mp) add(r_m0, q.idx2, in.sl, in.cl, in.pl);
+        if (lr) add(r_p0, q.idx2, sr, cr, in.pr);
+        if (c.mm || c.pm) add(r_0m, q.idy2, in.sd, in.cd, in.pd);
+        if (lu) add(r_0p, q.idy2, su, cu, in.pu);
+    }
+    // fe_node: the cell's magnetic energy
+    if (kappa2 > R(0) && i < g.Nxc && j < g.Nyc) {
+        R dB = -H;
+        if (he) dB += q.idx * (E.bp0 - E.b00) - q.idy * (E.a0p - E.a00);
+        if (hb) dB += q.idx * (B.bp0 - B.b00) - q.idy * (B.a0p - B.a00);
+        e += kappa2 * dB * dB;
+    }
+    // jac_A_node
+    has_a = has_b = false;
+    ga = gb = R(0);
+    if (withA) {
+        has_a = i < g.Nxa && j < g.Nya;
+        has_b = i < g.Nxb && j < g.Nyb;
+        if (has_a) {
+            R t = curlcurl_a_v(g, q, H, he, E, hb, B, j);
+            if (j == 0 || j + 1 == g.Nya) t *= bscale;
+            t *= kappa2;
+            if (lr) {
+                const R r = R(0.5) * ((c.pm ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
+                t += -r * q.idx * jsu(sr, cr, in.pr);
+            }
+            ga = t;
+        }
+        if (has_b) {
+            R t = curlcurl_b_v(g, q, H, he, E, hb, B, i);
+            if (i == 0 || i + 1 == g.Nxb) t *= bscale;
+            t *= kappa2;
+            if (lu) {
+                const R r = R(0.5) * ((c.mp ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
+                t += -r * q.idy * jsu(su, cu, in.pu);
+            }
+            gb = t;
+        }
+    }
+}
+
+// Cell material flags of node (i, j) from the mask bytes of its four cells
+// (the four flags as read; outside the grid they are ignored).
+__device__ __forceinline__ Cells cells_from(const Geo &g, bool has_mt, int i, int j, bool m_mm, bool m_mp, bool m_pm,
+                                            bool m_pp) {
+    Cells c;
+    c.mm = (i > 0) && (j > 0);
+    c.mp = (i > 0) && (j + 1 < g.Ny);
+    c.pm = (i + 1 < g.Nx) && (j > 0);
+    c.pp = (i + 1 < g.Nx) && (j + 1 < g.Ny);
+    if (has_mt) {
+        c.mm = c.mm && m_mm;
+        c.mp = c.mp && m_mp;
+        c.pm = c.pm && m_pm;
+        c.pp = c.pp && m_pp;
+    }
+    return c;
+}
+
 template <typename R, bool withA, bool UPD = false>
 __device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kappa2, R eps, const R *eps_map, R H,
                                           const uint8_t *mt, const cplx<R> *psi, const R *abei, const R *ab, long n,
                                           int i, int j, R &e, cplx<R> &gj, R &ga, R &gb, bool &has_a, bool &has_b,
                                           R bscale = R(1), const StateUpdate<R> *S = nullptr) {
+    GradIn<R> in;
     // loads
     const long row = (long)g.Nx * j;
     auto lp = [&](long k) {
@@ -608,30 +732,30 @@ __device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kapp
         }
         return p;
     };
-    const cplx<R> p0 = lp(n);
-    const cplx<R> pl = lp(row + max(i - 1, 0)), pr = lp(row + min(i + 1, g.Nx - 1));
-    const cplx<R> pd = lp((long)g.Nx * max(j - 1, 0) + i), pu = lp((long)g.Nx * min(j + 1, g.Ny - 1) + i);
-    const R ee = eps_map ? eps_map[n] : eps;
-    Cells c;
-    c.mm = (i > 0) && (j > 0);
-    c.mp = (i > 0) && (j + 1 < g.Ny);
-    c.pm = (i + 1 < g.Nx) && (j > 0);
-    c.pp = (i + 1 < g.Nx) && (j + 1 < g.Ny);
+    in.p0 = lp(n);
+    in.pl = lp(row + max(i - 1, 0));
+    in.pr = lp(row + min(i + 1, g.Nx - 1));
+    in.pd = lp((long)g.Nx * max(j - 1, 0) + i);
+    in.pu = lp((long)g.Nx * min(j + 1, g.Ny - 1) + i);
+    in.ee = eps_map ? eps_map[n] : eps;
+    bool m_mm = true, m_mp = true, m_pm = true, m_pp = true;
     if (mt) {
         const int cx0 = min(i, g.Nxc - 1), cxm = max(i - 1, 0), cy0 = min(j, g.Nyc - 1), cym = max(j - 1, 0);
-        const bool m_mm = mt[cxm + (long)g.Nxc * cym] != 0, m_mp = mt[cxm + (long)g.Nxc * cy0] != 0;
-        const bool m_pm = mt[cx0 + (long)g.Nxc * cym] != 0, m_pp = mt[cx0 + (long)g.Nxc * cy0] != 0;
-        c.mm = c.mm && m_mm;
-        c.mp = c.mp && m_mp;
-        c.pm = c.pm && m_pm;
-        c.pp = c.pp && m_pp;
+        m_mm = mt[cxm + (long)g.Nxc * cym] != 0;
+        m_mp = mt[cxm + (long)g.Nxc * cy0] != 0;
+        m_pm = mt[cx0 + (long)g.Nxc * cym] != 0;
+        m_pp = mt[cx0 + (long)g.Nxc * cy0] != 0;
     }
+    in.c = cells_from(g, mt != nullptr, i, j, m_mm, m_mp, m_pm, m_pp);
+    const Cells c = in.c;
     const bool he = abei != nullptr, hb = ab != nullptr;
-    Links10<R> E{}, B{};
+    Links10<R> &E = in.E, &B = in.B;
+    E = Links10<R>{};
+    B = Links10<R>{};
     if (he) E = load_links10(g, abei, i, j);
     if (hb) B = load_links10(g, ab, i, j);
     if constexpr (UPD) {
-        st(S->psi_out, n, p0);
+        st(S->psi_out, n, in.p0);
         if (S->dab) {
             const Links10<R> d = load_links10(g, S->dab, i, j);
             const R a = S->aa;
@@ -655,91 +779,13 @@ __device__ __forceinline__ void grad_node(const Geo &g, const GeoR<R> &q, R kapp
         if (hb) p += vb;
         return p;
     };
-    const cplx<R> zero = {R(0), R(0)};
-    const bool lr = c.pm || c.pp, lu = c.mp || c.pp;  // the right link a(i, j) and the up link b(i, j) are active
-    R sr = R(0), cr = R(1), su = R(0), cu = R(1);
-    if (lr) sincos_r(q.dx * pvv(E.a00, B.a00), &sr, &cr);
-    if (lu) sincos_r(q.dy * pvv(E.b00, B.b00), &su, &cu);
-    // |psi1 U - psi0|^2 and Im(conj psi0 U psi1) with U's (sin, cos) given
-    auto gg = [&](R s, R cs, cplx<R> p1) {
-        const R re = p1.re * cs + p1.im * s - p0.re;
-        const R im = p1.im * cs - p1.re * s - p0.im;
-        return re * re + im * im;
-    };
-    auto jsu = [&](R s, R cs, cplx<R> p1) {
-        return (p0.re * p1.im - p0.im * p1.re) * cs - (p0.re * p1.re + p0.im * p1.im) * s;
-    };
-    e = R(0);
-    gj = zero;
-    if (c.mm || c.mp || c.pm || c.pp) {
-        const R f_mm = c.mm ? R(1) : R(0), f_mp = c.mp ? R(1) : R(0), f_pm = c.pm ? R(1) : R(0), f_pp = c.pp ? R(1) : R(0);
-        const R r_m0 = R(0.5) * (f_mm + f_mp), r_p0 = R(0.5) * (f_pm + f_pp);
-        const R r_0m = R(0.5) * (f_mm + f_pm), r_0p = R(0.5) * (f_mp + f_pp);
-        const R gw = R(0.25) * (r_m0 + r_p0 + r_0m + r_0p);
-        // fe_node
-        const R p2 = p0.re * p0.re + p0.im * p0.im;
-        e += gw * (R(0.5) * p2 - ee) * p2;
-        if (lr) e += r_p0 * q.idx2 * gg(sr, cr, pr);
-        if (lu) e += r_0p * q.idy2 * gg(su, cu, pu);
-        // jac_psi_node
-        const R p = p0.re * p0.re + p0.im * p0.im - ee;
-        const R k = R(2) * gw * p;
-        gj.re += k * p0.re;
-        gj.im += k * p0.im;
-        auto add = [&](R r, R id2, R s, R cs, cplx<R> p1) {
-            const R tr = R(2) * (p0.re - (p1.re * cs + p1.im * s));
-            const R ti = R(2) * (p0.im - (p1.im * cs - p1.re * s));
-            const R w = r * id2;
-            gj.re += w * tr;
-            gj.im += w * ti;
-        };
-        if (c.mm || c.mp) {
-            R s, cs;
-            sincos_r(-q.dx * pvv(E.am0, B.am0), &s, &cs);
-            add(r_m0, q.idx2, s, cs, pl);
-        }
-        if (lr) add(r_p0, q.idx2, sr, cr, pr);
-        if (c.mm || c.pm) {
-            R s, cs;
-            sincos_r(-q.dy * pvv(E.b0m, B.b0m), &s, &cs);
-            add(r_0m, q.idy2, s, cs, pd);
-        }
-        if (lu) add(r_0p, q.idy2, su, cu, pu);
-    }
-    // fe_node: the cell's magnetic energy
-    if (kappa2 > R(0) && i < g.Nxc && j < g.Nyc) {
-        R dB = -H;
-        if (he) dB += q.idx * (E.bp0 - E.b00) - q.idy * (E.a0p - E.a00);
-        if (hb) dB += q.idx * (B.bp0 - B.b00) - q.idy * (B.a0p - B.a00);
-        e += kappa2 * dB * dB;
-    }
-    // jac_A_node
-    has_a = has_b = false;
-    ga = gb = R(0);
-    if (withA) {
-        has_a = i < g.Nxa && j < g.Nya;
-        has_b = i < g.Nxb && j < g.Nyb;
-        if (has_a) {
-            R t = curlcurl_a_v(g, q, H, he, E, hb, B, j);
-            if (j == 0 || j + 1 == g.Nya) t *= bscale;
-            t *= kappa2;
-            if (lr) {
-                const R r = R(0.5) * ((c.pm ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
-                t += -r * q.idx * jsu(sr, cr, pr);
-            }
-            ga = t;
-        }
-        if (has_b) {
-            R t = curlcurl_b_v(g, q, H, he, E, hb, B, i);
-            if (i == 0 || i + 1 == g.Nxb) t *= bscale;
-            t *= kappa2;
-            if (lu) {
-                const R r = R(0.5) * ((c.mp ? R(1) : R(0)) + (c.pp ? R(1) : R(0)));
-                t += -r * q.idy * jsu(su, cu, pu);
-            }
-            gb = t;
-        }
-    }
+    in.sr = in.su = in.sl = in.sd = R(0);
+    in.cr = in.cu = in.cl = in.cd = R(1);
+    if (c.pm || c.pp) sincos_r(q.dx * pvv(E.a00, B.a00), &in.sr, &in.cr);
+    if (c.mp || c.pp) sincos_r(q.dy * pvv(E.b00, B.b00), &in.su, &in.cu);
+    if (c.mm || c.mp) sincos_r(-q.dx * pvv(E.am0, B.am0), &in.sl, &in.cl);
+    if (c.mm || c.pm) sincos_r(-q.dy * pvv(E.b0m, B.b0m), &in.sd, &in.cd);
+    grad_eval<R, withA>(g, q, kappa2, H, he, hb, in, i, j, e, gj, ga, gb, has_a, has_b, bscale);
 }
 
 // One pass of the CG loop over the state (phase 1 of an iteration): writes
@@ -995,66 +1041,27 @@ __device__ __forceinline__ cplx<R> dir_at(const DirUpdate<R> &U, long k) {
     return {U.bp * o.re - gk.re, U.bp * o.im - gk.im};
 }
 
-// With L = 17 and th not NULL, *th also takes the largest link phase step
-// |d dA| of the node's links a(i, j), b(i, j) (for the line search's bound).
-template <typename R, int L, bool DIR = false>
-__device__ __forceinline__ void coef_node(const Geo &g, const GeoR<R> &q, R kappa2, R eps_s, const R *eps_map, R H,
-                                          const uint8_t *mt, const cplx<R> *psi, const cplx<R> *dpsi, const R *abei,
-                                          const R *ab, const R *dab, long n, R (&C)[L],
-                                          const DirUpdate<R> *U = nullptr, double *th = nullptr) {
-    int i, j;
-    node_ij(g, n, i, j);
-    // loads
-    const long nr = (long)g.Nx * j + min(i + 1, g.Nx - 1), nu = (long)g.Nx * min(j + 1, g.Ny - 1) + i;
-    const cplx<R> p0 = ld(psi, n), p_r = ld(psi, nr), p_u = ld(psi, nu);
-    cplx<R> d0, d_r, d_u;
-    if constexpr (DIR) {
-        d0 = dir_at(*U, n);
-        d_r = dir_at(*U, nr);
-        d_u = dir_at(*U, nu);
-    } else {
-        d0 = ld(dpsi, n);
-        d_r = ld(dpsi, nr);
-        d_u = ld(dpsi, nu);
-    }
-    const R eps = eps_map ? eps_map[n] : eps_s;
+// Everything coef_eval reads of node (i, j): psi and the direction at the
+// node and its right and up neighbours, epsilon, the four cell flags, and
+// the four link values a(i, j), a(i, j+1), b(i, j), b(i+1, j) of abei (E),
+// ab (B) and the A direction (D).
+template <typename R> struct CoefIn {
+    cplx<R> p0, p_r, p_u, d0, d_r, d_u;
+    R eps;
     Cells c;
-    c.mm = (i > 0) && (j > 0);
-    c.mp = (i > 0) && (j + 1 < g.Ny);
-    c.pm = (i + 1 < g.Nx) && (j > 0);
-    c.pp = (i + 1 < g.Nx) && (j + 1 < g.Ny);
-    if (mt) {
-        const int cx0 = min(i, g.Nxc - 1), cxm = max(i - 1, 0), cy0 = min(j, g.Nyc - 1), cym = max(j - 1, 0);
-        const bool m_mm = mt[cxm + (long)g.Nxc * cym] != 0, m_mp = mt[cxm + (long)g.Nxc * cy0] != 0;
-        const bool m_pm = mt[cx0 + (long)g.Nxc * cym] != 0, m_pp = mt[cx0 + (long)g.Nxc * cy0] != 0;
-        c.mm = c.mm && m_mm;
-        c.mp = c.mp && m_mp;
-        c.pm = c.pm && m_pm;
-        c.pp = c.pp && m_pp;
-    }
-    const bool he = abei != nullptr, hb = ab != nullptr;
-    Links4<R> E{}, B{}, D{};
-    if (he) E = load_links4(g, abei, i, j);
-    if (hb) B = load_links4(g, ab, i, j);
-    if (L == 17) {
-        if constexpr (DIR) {
-            const Links4<R> o = load_links4(g, U->dAold, i, j), gk = load_links4(g, U->jacA, i, j);
-            D.a00 = U->ba * o.a00 - gk.a00;
-            D.a0p = U->ba * o.a0p - gk.a0p;
-            D.b00 = U->ba * o.b00 - gk.b00;
-            D.bp0 = U->ba * o.bp0 - gk.bp0;
-        } else {
-            D = load_links4(g, dab, i, j);
-        }
-        if (th) *th = fmax(*th, fmax(fabs(double(q.dx * D.a00)), fabs(double(q.dy * D.b00))));
-    }
-    if constexpr (DIR) {
-        st(U->dout, n, d0);
-        if (L == 17) {
-            if (i < g.Nxa) U->dAout[i + (long)g.Nxa * j] = D.a00;
-            if (j < g.Nyb) U->dAout[g.Na + i + (long)g.Nxb * j] = D.b00;
-        }
-    }
+    Links4<R> E, B, D;
+};
+
+// One node's contributions to the L line-search coefficients, added to C
+// (before the dxdy factor), from gathered values: the reference's terms in
+// its order, the sin/cos of each link phase evaluated once.
+template <typename R, int L>
+__device__ __forceinline__ void coef_eval(const GeoR<R> &q, R kappa2, R H, bool he, bool hb, const CoefIn<R> &in,
+                                          int i, int j, bool cell_ij, R (&C)[L]) {
+    const cplx<R> p0 = in.p0, p_r = in.p_r, p_u = in.p_u, d0 = in.d0, d_r = in.d_r, d_u = in.d_u;
+    const R eps = in.eps;
+    const Cells c = in.c;
+    const Links4<R> &E = in.E, &B = in.B, &D = in.D;
     auto pvv = [&](R ve, R vb) {  // pv(abei, ab, k) of loaded values
         R p = R(0);
         if (he) p += ve;
@@ -1128,7 +1135,7 @@ __device__ __forceinline__ void coef_node(const Geo &g, const GeoR<R> &q, R kapp
             }
         }
     }
-    if (kappa2 > R(0) && i < g.Nxc && j < g.Nyc) {
+    if (kappa2 > R(0) && cell_ij) {
         if (L == 5) {
             const R dB = q.idx * (B.bp0 - B.b00) - q.idy * (B.a0p - B.a00) - H;
             C[0] += kappa2 * dB * dB;
@@ -1140,6 +1147,68 @@ __device__ __forceinline__ void coef_node(const Geo &g, const GeoR<R> &q, R kapp
             C[2] += kappa2 * dB * dB;
         }
     }
+}
+
+// With L = 17 and th not NULL, *th also takes the largest link phase step
+// |d dA| of the node's links a(i, j), b(i, j) (for the line search's bound).
+template <typename R, int L, bool DIR = false>
+__device__ __forceinline__ void coef_node(const Geo &g, const GeoR<R> &q, R kappa2, R eps_s, const R *eps_map, R H,
+                                          const uint8_t *mt, const cplx<R> *psi, const cplx<R> *dpsi, const R *abei,
+                                          const R *ab, const R *dab, long n, R (&C)[L],
+                                          const DirUpdate<R> *U = nullptr, double *th = nullptr) {
+    int i, j;
+    node_ij(g, n, i, j);
+    CoefIn<R> in;
+    // loads
+    const long nr = (long)g.Nx * j + min(i + 1, g.Nx - 1), nu = (long)g.Nx * min(j + 1, g.Ny - 1) + i;
+    in.p0 = ld(psi, n);
+    in.p_r = ld(psi, nr);
+    in.p_u = ld(psi, nu);
+    if constexpr (DIR) {
+        in.d0 = dir_at(*U, n);
+        in.d_r = dir_at(*U, nr);
+        in.d_u = dir_at(*U, nu);
+    } else {
+        in.d0 = ld(dpsi, n);
+        in.d_r = ld(dpsi, nr);
+        in.d_u = ld(dpsi, nu);
+    }
+    in.eps = eps_map ? eps_map[n] : eps_s;
+    bool m_mm = true, m_mp = true, m_pm = true, m_pp = true;
+    if (mt) {
+        const int cx0 = min(i, g.Nxc - 1), cxm = max(i - 1, 0), cy0 = min(j, g.Nyc - 1), cym = max(j - 1, 0);
+        m_mm = mt[cxm + (long)g.Nxc * cym] != 0;
+        m_mp = mt[cxm + (long)g.Nxc * cy0] != 0;
+        m_pm = mt[cx0 + (long)g.Nxc * cym] != 0;
+        m_pp = mt[cx0 + (long)g.Nxc * cy0] != 0;
+    }
+    in.c = cells_from(g, mt != nullptr, i, j, m_mm, m_mp, m_pm, m_pp);
+    const bool he = abei != nullptr, hb = ab != nullptr;
+    in.E = Links4<R>{};
+    in.B = Links4<R>{};
+    in.D = Links4<R>{};
+    if (he) in.E = load_links4(g, abei, i, j);
+    if (hb) in.B = load_links4(g, ab, i, j);
+    if (L == 17) {
+        if constexpr (DIR) {
+            const Links4<R> o = load_links4(g, U->dAold, i, j), gk = load_links4(g, U->jacA, i, j);
+            in.D.a00 = U->ba * o.a00 - gk.a00;
+            in.D.a0p = U->ba * o.a0p - gk.a0p;
+            in.D.b00 = U->ba * o.b00 - gk.b00;
+            in.D.bp0 = U->ba * o.bp0 - gk.bp0;
+        } else {
+            in.D = load_links4(g, dab, i, j);
+        }
+        if (th) *th = fmax(*th, fmax(fabs(double(q.dx * in.D.a00)), fabs(double(q.dy * in.D.b00))));
+    }
+    if constexpr (DIR) {
+        st(U->dout, n, in.d0);
+        if (L == 17) {
+            if (i < g.Nxa) U->dAout[i + (long)g.Nxa * j] = in.D.a00;
+            if (j < g.Nyb) U->dAout[g.Na + i + (long)g.Nxb * j] = in.D.b00;
+        }
+    }
+    coef_eval<R, L>(q, kappa2, H, he, hb, in, i, j, i < g.Nxc && j < g.Nyc, C);
 }
 
 // Partials: the L sums of block b at partials[b L + k]; with L = 17 the
@@ -1200,10 +1269,32 @@ __global__ void __launch_bounds__(256) k_coef_dir(Geo g, R kappa2, R eps, R H, c
 
 // --------------------------------------------------------------- vortices
 
+// fmod(x, b) for b > 0 from the truncated quotient: the remainder x - n b
+// of the right n is a representable number, so one FMA gives it exactly;
+// n = trunc(x / b) is off by at most one for |x / b| < 2^20, which the
+// sign and size of the FMA's result reveal (a wrong n leaves a nonzero
+// remainder of the wrong sign or at least b in size). Larger or non-finite
+// x take the library fmod.
+__device__ __forceinline__ double fmod_pos(double x, double b) {
+    double n = trunc(x / b);
+    if (!(fabs(n) < 1048576.0)) return fmod(x, b);
+    double r = fma(-n, b, x);
+    if (x >= 0.0) {
+        if (r < 0.0) n -= 1.0;
+        else if (r >= b) n += 1.0;
+        else return r;
+    } else {
+        if (r > 0.0) n += 1.0;
+        else if (r <= -b) n -= 1.0;
+        else return r;
+    }
+    return fma(-n, b, x);
+}
+
 // np.mod(x, 2 pi) for float64 (fmod, shifted into [0, 2 pi)).
 __device__ __forceinline__ double np_mod_2pi(double x) {
     const double b = 6.283185307179586;  // 2.0*np.pi
-    double m = fmod(x, b);
+    double m = fmod_pos(x, b);
     if (m != 0.0) {
         if (m < 0.0) m += b;
     } else {
@@ -1214,44 +1305,73 @@ __device__ __forceinline__ double np_mod_2pi(double x) {
 
 constexpr int VREC = 14;
 
-// Winding number of every cell (vortex_detector.py:55-72) with each node's
-// phase evaluated once. Cells are cut into strips of 63 columns; a wave
-// marches down a range of cell rows of one strip, lane l holding node column
-// i = 63 s + l of the two node rows j, j + 1 that bound cell row j (phase,
-// psi and the a link of each) and the b link of row j; lane l + 1 supplies
-// the right-hand corner values by a lane shift, so lanes 0..62 evaluate
-// their cells with the reference's expression, term by term. Flagged cells
-// are appended with one atomic per wave and row (ballot, then each lane's
-// slot is the base plus the flagged lanes below it). Bounds: node rows j + 1
-// <= Ny - 1 and b rows j <= Ny - 2 since j < Nyc; columns past the grid are
-// clamped to the last one for their loads and never evaluate a cell.
+// Winding number of one cell (vortex_detector.py:55-72) from the phases of
+// its corners (i, j), (i+1, j), (i, j+1), (i+1, j+1) and its links a(i, j),
+// a(i, j+1), b(i, j), b(i+1, j), term by term in the reference's order.
+__device__ __forceinline__ double cell_winding(const Geo &g, double dxdyH, double t00, double tp0, double t0p,
+                                               double tpp, double a0, double a1, double b0, double bp) {
+    const double pi = 3.141592653589793;
+    return -(0.5 / pi) * (np_mod_2pi(tp0 - t00 - g.dx * a0 + pi) + np_mod_2pi(tpp - tp0 - g.dy * bp + pi) +
+                          np_mod_2pi(t0p - tpp + g.dx * a1 + pi) + np_mod_2pi(t00 - t0p + g.dy * b0 + pi) - 4.0 * pi +
+                          dxdyH);
+}
+
+// The reference keeps cells with |v| > 0.5 whose v is within 0.1 of an
+// integer (vortex_detector.py:67-72).
+__device__ __forceinline__ bool winding_flag(double v) { return fabs(v) > 0.5 && fabs(v - rint(v)) < 0.1; }
+
+// Vortex cells in three launches, none with an atomic: (1) k_vortex_flags
+// evaluates every cell and stores, per strip row, the 64-bit mask of the
+// flagged cells, and per wave their number; (2) k_vortex_scan turns the
+// per-wave numbers into the first record slot of each wave and the total;
+// (3) k_vortex_emit walks the masks again and writes a record for each set
+// bit, re-evaluating only those cells (the same expression on the same
+// values, so the same v). Records come out in strip, row, column order.
+//
+// Cells are cut into strips of 63 columns; in (1) a wave marches down a
+// range of cell rows of one strip, lane l holding node column i = 63 s + l
+// of the two node rows j, j + 1 that bound cell row j (phase, psi and the a
+// link of each) and the b link of row j; lane l + 1 supplies the right-hand
+// corner values by a lane shift, so lanes 0..62 evaluate their cells and
+// each node's phase is evaluated once. Bounds: node rows j + 1 <= Ny - 1
+// and b rows j <= Ny - 2 since j < Nyc; columns past the grid are clamped
+// to the last one for their loads and never evaluate a cell.
 constexpr int VSTRIP = WAVE - 1;
+constexpr long VORTEX_MAX_WAVES = 32768;  // per-wave counts and slots fit the partials buffer
+
+inline long vortex_rows(const Geo &g) { return (long)((g.Nxc + VSTRIP - 1) / VSTRIP) * g.Nyc; }
 
 inline long vortex_waves(const Geo &g) {
-    const long T = (long)((g.Nxc + VSTRIP - 1) / VSTRIP) * g.Nyc;
-    const long W = (T + 15) / 16;  // about 16 rows per wave
+    long W = (vortex_rows(g) + 15) / 16;  // about 16 rows per wave
+    if (W > VORTEX_MAX_WAVES) W = VORTEX_MAX_WAVES;
     return W < 1 ? 1 : W;
 }
 
+// Rows r0 <= r < r1 of the (strip, cell row) items r = s Nyc + j of wave w.
+__device__ __forceinline__ void vortex_range(const Geo &g, long w, long W, long &r0, long &r1) {
+    const long T = (long)((g.Nxc + VSTRIP - 1) / VSTRIP) * g.Nyc;
+    r0 = (w * T) / W;
+    r1 = ((w + 1) * T) / W;
+}
+
 template <typename R>
-__global__ void __launch_bounds__(256) k_vortex_cells(Geo g, const cplx<R> *psi, const R *ab, const R *abi, double H,
-                                                      double *rec, int capacity, int *count, long W) {
-    const double pi = 3.141592653589793;
+__global__ void __launch_bounds__(256) k_vortex_flags(Geo g, const cplx<R> *psi, const R *ab, double H,
+                                                      unsigned long long *bits, int *wcount, long W) {
     const double dxdyH = g.dx * g.dy * H;
     const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
     if (w >= W) return;  // wave-uniform
     const int lane = threadIdx.x & (WAVE - 1);
-    const long ns = (g.Nxc + VSTRIP - 1) / VSTRIP, T = ns * g.Nyc;
-    long r0 = (w * T) / W;
-    const long r1 = ((w + 1) * T) / W;
+    long r0, r1;
+    vortex_range(g, w, W, r0, r1);
     const R *bl = ab + g.Na;
+    int cnt = 0;
     while (r0 < r1) {
         const int s = (int)(r0 / g.Nyc), ja = (int)(r0 % g.Nyc);
         const int jb = (int)min((long)g.Nyc, (long)ja + (r1 - r0));
         const int i = s * VSTRIP + lane;
         const bool col = i < g.Nx, cell = lane < VSTRIP && i < g.Nxc;
         const int ic = min(i, g.Nx - 1), iac = min(i, g.Nxa - 1);
-        cplx<R> p0 = ld(psi, ic + (long)g.Nx * ja);
+        const cplx<R> p0 = ld(psi, ic + (long)g.Nx * ja);
         double t0 = col ? atan2(double(p0.im), double(p0.re)) : 0.0;
         double a0 = double(ab[iac + (long)g.Nxa * ja]);
         for (int j = ja; j < jb; ++j) {
@@ -1260,45 +1380,90 @@ __global__ void __launch_bounds__(256) k_vortex_cells(Geo g, const cplx<R> *psi,
             const double a1 = double(ab[iac + (long)g.Nxa * (j + 1)]);
             const double b0 = double(bl[ic + (long)g.Nxb * j]);
             const double t0r = shfl_dn1(t0), t1r = shfl_dn1(t1), b0r = shfl_dn1(b0);
-            double v = 0.0;
             bool flag = false;
-            if (cell) {
-                v = -(0.5 / pi) * (np_mod_2pi(t0r - t0 - g.dx * a0 + pi) + np_mod_2pi(t1r - t0r - g.dy * b0r + pi) +
-                                   np_mod_2pi(t1 - t1r + g.dx * a1 + pi) + np_mod_2pi(t0 - t1 + g.dy * b0 + pi) -
-                                   4.0 * pi + dxdyH);
-                flag = fabs(v) > 0.5 && fabs(v - rint(v)) < 0.1;
-            }
+            if (cell) flag = winding_flag(cell_winding(g, dxdyH, t0, t0r, t1, t1r, a0, a1, b0, b0r));
             const unsigned long long m = __ballot(flag);
-            if (m) {  // wave-uniform
-                const R p0r_re = shfl_dn1(p0.re), p0r_im = shfl_dn1(p0.im);
-                const R p1r_re = shfl_dn1(p1.re), p1r_im = shfl_dn1(p1.im);
-                int base = 0;
-                if (lane == 0) base = atomicAdd(count, (int)__popcll(m));
-                base = __shfl(base, 0, WAVE);
-                if (flag) {
-                    const int slot = base + (int)__popcll(m & ((1ull << lane) - 1ull));
-                    if (slot < capacity) {
-                        const long ka00 = i + (long)g.Nxa * j, ka0p = i + (long)g.Nxa * (j + 1);
-                        const long kb00 = g.Na + i + (long)g.Nxb * j, kbp0 = kb00 + 1;
-                        double *o = rec + (long)slot * VREC;
-                        o[0] = double(i + (long)g.Nxc * j);
-                        o[1] = v;
-                        o[2] = double(p0.re); o[3] = double(p0.im);
-                        o[4] = double(p0r_re); o[5] = double(p0r_im);
-                        o[6] = double(p1r_re); o[7] = double(p1r_im);
-                        o[8] = double(p1.re); o[9] = double(p1.im);
-                        o[10] = a0 + (abi ? double(abi[ka00]) : 0.0);
-                        o[11] = a1 + (abi ? double(abi[ka0p]) : 0.0);
-                        o[12] = b0 + (abi ? double(abi[kb00]) : 0.0);
-                        o[13] = b0r + (abi ? double(abi[kbp0]) : 0.0);
-                    }
-                }
-            }
-            p0 = p1;
+            if (lane == 0) bits[(long)s * g.Nyc + j] = m;
+            cnt += (int)__popcll(m);
             t0 = t1;
             a0 = a1;
         }
         r0 += jb - ja;
+    }
+    if (lane == 0) wcount[w] = cnt;
+}
+
+// Exclusive prefix sum of the W per-wave counts into slot (one workgroup of
+// 1024 threads, each scanning a contiguous chunk), the total into *total.
+static __global__ void __launch_bounds__(1024) k_vortex_scan(const int *wcount, int *slot, long W, int *total) {
+    __shared__ long part[1024];
+    const int t = threadIdx.x;
+    const long C = (W + 1023) / 1024;
+    const long k0 = min((long)t * C, W), k1 = min(k0 + C, W);
+    long sum = 0;
+    for (long k = k0; k < k1; ++k) sum += wcount[k];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+        const long v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    long run = part[t] - sum;
+    for (long k = k0; k < k1; ++k) {
+        const int c = wcount[k];
+        slot[k] = (int)min(run, (long)INT_MAX);
+        run += c;
+    }
+    if (t == 1023) *total = (int)min(part[1023], (long)INT_MAX);
+}
+
+// Records of the flagged cells: (cell n, v, psi at the four corners
+// counter-clockwise from (i, j), a(i, j), a(i, j+1), b(i, j), b(i+1, j))
+// with abi added to the links when given. Slots past `capacity` are not
+// written.
+template <typename R>
+__global__ void __launch_bounds__(256) k_vortex_emit(Geo g, const cplx<R> *psi, const R *ab, const R *abi, double H,
+                                                     const unsigned long long *bits, const int *slot, double *rec,
+                                                     int capacity, long W) {
+    const double dxdyH = g.dx * g.dy * H;
+    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
+    if (w >= W) return;  // wave-uniform
+    const int lane = threadIdx.x & (WAVE - 1);
+    long r0, r1;
+    vortex_range(g, w, W, r0, r1);
+    long base = slot[w];
+    for (long r = r0; r < r1; ++r) {
+        const unsigned long long m = bits[r];
+        if (!m) continue;  // wave-uniform
+        if ((m >> lane) & 1ull) {
+            const int s = (int)(r / g.Nyc), j = (int)(r % g.Nyc);
+            const int i = s * VSTRIP + lane;  // i < Nxc, j < Nyc: a cell
+            const long n00 = i + (long)g.Nx * j, n0p = n00 + g.Nx;
+            const cplx<R> p00 = ld(psi, n00), pp0 = ld(psi, n00 + 1), p0p = ld(psi, n0p), ppp = ld(psi, n0p + 1);
+            const long ka00 = i + (long)g.Nxa * j, ka0p = ka00 + g.Nxa;
+            const long kb00 = g.Na + i + (long)g.Nxb * j, kbp0 = kb00 + 1;
+            const double a0 = double(ab[ka00]), a1 = double(ab[ka0p]), b0 = double(ab[kb00]), bp = double(ab[kbp0]);
+            const double v = cell_winding(g, dxdyH, atan2(double(p00.im), double(p00.re)),
+                                          atan2(double(pp0.im), double(pp0.re)), atan2(double(p0p.im), double(p0p.re)),
+                                          atan2(double(ppp.im), double(ppp.re)), a0, a1, b0, bp);
+            const long k = base + (long)__popcll(m & ((1ull << lane) - 1ull));
+            if (k < capacity) {
+                double *o = rec + k * VREC;
+                o[0] = double(i + (long)g.Nxc * j);
+                o[1] = v;
+                o[2] = double(p00.re); o[3] = double(p00.im);
+                o[4] = double(pp0.re); o[5] = double(pp0.im);
+                o[6] = double(ppp.re); o[7] = double(ppp.im);
+                o[8] = double(p0p.re); o[9] = double(p0p.im);
+                o[10] = a0 + (abi ? double(abi[ka00]) : 0.0);
+                o[11] = a1 + (abi ? double(abi[ka0p]) : 0.0);
+                o[12] = b0 + (abi ? double(abi[kb00]) : 0.0);
+                o[13] = bp + (abi ? double(abi[kbp0]) : 0.0);
+            }
+        }
+        base += (long)__popcll(m);
     }
 }
 

@@ -43,6 +43,10 @@ struct Env {
     // raised, so every team leaves at its first barrier as after a timeout
     // (glx_td_force_barrier_abort; cleared by that run)
     int abort_next;
+    // a device buffer kept between calls and grown on demand (the vortex
+    // detector's per-row masks); grow_workspace() below
+    void **grow;
+    size_t *grow_bytes;
 };
 
 // Device memory a launcher uses in place of allocating its own per-call

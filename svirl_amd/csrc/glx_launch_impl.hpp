@@ -3,12 +3,15 @@
 #pragma once
 
 #include <algorithm>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "glx_fields.hpp"
 #include "glx_launch.hpp"
 #include "glx_td.hpp"
 #include "glx_batch_cg.hpp"
+#include "glx_cg_rows.hpp"
 
 namespace glx {
 
@@ -708,21 +711,52 @@ int L<R>::free_energy(const Env &e, glx_grid g, double kappa2, double eps, const
     return fetch_result(e, F, 1);
 }
 
+// The library's growable device buffer, at least `bytes` large. Growing
+// frees the old buffer after the device has finished with it (hipFree
+// waits for the device).
+static int grow_workspace(const Env &e, size_t bytes, void **out) {
+    if (*e.grow_bytes < bytes) {
+        if (*e.grow) {
+            hipError_t st = hipFree(*e.grow);
+            *e.grow = nullptr;
+            *e.grow_bytes = 0;
+            if (st != hipSuccess) return check_hip(st, "hipFree(workspace)");
+        }
+        hipError_t st = hipMalloc(e.grow, bytes);
+        if (st != hipSuccess) {
+            *e.grow = nullptr;
+            return check_hip(st, "hipMalloc(workspace)");
+        }
+        *e.grow_bytes = bytes;
+    }
+    *out = *e.grow;
+    return GLX_OK;
+}
+
 template <typename R>
 int L<R>::vortex_cells(const Env &e, glx_grid g, const void *psi, const void *ab, const void *abi, double H,
                        double *rec, int32_t capacity, int32_t *count) {
     if (!grid_ok(g) || !psi || !ab || !count || capacity < 0 || (capacity > 0 && !rec))
         return set_error(GLX_E_ARG, "glx_vortex_cells: invalid arguments");
     const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
+    const long T = vortex_rows(G), W = vortex_waves(G);
+    void *bits = nullptr;
+    int rc = grow_workspace(e, sizeof(unsigned long long) * (size_t)T, &bits);
+    if (rc) return rc;
+    int *wcount = (int *)e.partials, *slot = wcount + VORTEX_MAX_WAVES;
     int *dcount = (int *)e.result;
-    hipError_t st = hipMemsetAsync(dcount, 0, sizeof(int), e.stream);
-    if (st != hipSuccess) return check_hip(st, "hipMemsetAsync(count)");
-    const long W = vortex_waves(G);
-    hipLaunchKernelGGL(k_vortex_cells<R>, dim3((unsigned)((W + 3) / 4)), dim3(256), 0, e.stream, G, (const cplx<R> *)psi,
-                       (const R *)ab, (const R *)abi, double(R(H)), rec, (int)capacity, dcount, W);
-    GLX_LAUNCH_CHECK("k_vortex_cells");
+    const unsigned nb = (unsigned)((W + 3) / 4);
+    hipLaunchKernelGGL(k_vortex_flags<R>, dim3(nb), dim3(256), 0, e.stream, G, (const cplx<R> *)psi, (const R *)ab,
+                       double(R(H)), (unsigned long long *)bits, wcount, W);
+    GLX_LAUNCH_CHECK("k_vortex_flags");
+    hipLaunchKernelGGL(k_vortex_scan, dim3(1), dim3(1024), 0, e.stream, (const int *)wcount, slot, W, dcount);
+    GLX_LAUNCH_CHECK("k_vortex_scan");
+    hipLaunchKernelGGL(k_vortex_emit<R>, dim3(nb), dim3(256), 0, e.stream, G, (const cplx<R> *)psi, (const R *)ab,
+                       (const R *)abi, double(R(H)), (const unsigned long long *)bits, (const int *)slot, rec,
+                       (int)capacity, W);
+    GLX_LAUNCH_CHECK("k_vortex_emit");
     int c = 0;
-    st = hipMemcpyAsync(&c, dcount, sizeof(int), hipMemcpyDeviceToHost, e.stream);
+    hipError_t st = hipMemcpyAsync(&c, dcount, sizeof(int), hipMemcpyDeviceToHost, e.stream);
     if (st == hipSuccess) st = hipStreamSynchronize(e.stream);
     if (st != hipSuccess) return check_hip(st, "glx_vortex_cells readback");
     *count = c;
@@ -832,6 +866,29 @@ int L<R>::coef(const Env &e, glx_grid g, double kappa2, double eps, double H, co
 
 // The gradient pass, optionally with the state update fused in (S, alpha:
 // UPD; S.dpsi NULL: no update).
+// Waves of a row-marching pass cut to whole rounds of the chip: W is
+// lowered to a multiple of the waves the kernel keeps resident (its
+// occupancy x CUs), so that no round runs a fraction of the chip. The
+// occupancy is queried once per kernel.
+static long fit_row_waves(const void *fn, long W, int num_cus) {
+    static std::mutex mu;
+    static std::unordered_map<const void *, int> per_cu;
+    int per = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = per_cu.find(fn);
+        if (it != per_cu.end()) {
+            per = it->second;
+        } else {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, RED_BLOCK, 0) != hipSuccess) per = 0;
+            per_cu[fn] = per;
+        }
+    }
+    const long res = (long)per * num_cus * (RED_BLOCK / WAVE);
+    if (res <= 0 || W <= res) return W;
+    return (W / res) * res;
+}
+
 template <typename R>
 static int gradient_pass_launch(const Env &e, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
                                 const uint8_t *mt, const void *psi, const void *abei, const void *ab, void *jac,
@@ -840,7 +897,10 @@ static int gradient_pass_launch(const Env &e, glx_grid g, double kappa2, double 
     const bool withA = kappa2 > 0.0;  // the psi Jacobian reads A in both cases
     const bool upd = S.dpsi != nullptr;
     const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
-    const int nb = grid_blocks(G.N, RED_BLOCK, RED_MAX_BLOCKS);
+    const void *fn = withA ? (upd ? (const void *)k_cg_grad_rows<R, true, true> : (const void *)k_cg_grad_rows<R, true, false>)
+                           : (upd ? (const void *)k_cg_grad_rows<R, false, true> : (const void *)k_cg_grad_rows<R, false, false>);
+    const long W = fit_row_waves(fn, cg_row_waves(g.Nx, g.Ny), e.num_cus);
+    const int nb = (int)((W + 3) / 4);
     *nb_out = nb;
     const R k2 = R(kappa2), ep = R(eps), h = R(H);
     const R *em = (const R *)eps_map, *ae = (const R *)abei, *a = (const R *)ab;
@@ -849,18 +909,18 @@ static int gradient_pass_launch(const Env &e, glx_grid g, double kappa2, double 
     R *jA = withA ? (R *)jacA : nullptr;
     const R *jAp = withA ? (const R *)jacA_prev : nullptr;
     if (withA && upd)
-        hipLaunchKernelGGL((k_cg_gradient_pass<R, true, true>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, k2, ep, em,
-                           h, mt, p, ae, a, jc, jp, jA, jAp, e.partials, S, alpha);
+        hipLaunchKernelGGL((k_cg_grad_rows<R, true, true>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, k2, ep, em, h, mt,
+                           p, ae, a, jc, jp, jA, jAp, e.partials, S, alpha, W);
     else if (withA)
-        hipLaunchKernelGGL((k_cg_gradient_pass<R, true>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, k2, ep, em, h, mt,
-                           p, ae, a, jc, jp, jA, jAp, e.partials, S, alpha);
+        hipLaunchKernelGGL((k_cg_grad_rows<R, true, false>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, k2, ep, em, h,
+                           mt, p, ae, a, jc, jp, jA, jAp, e.partials, S, alpha, W);
     else if (upd)
-        hipLaunchKernelGGL((k_cg_gradient_pass<R, false, true>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, k2, ep, em,
-                           h, mt, p, ae, a, jc, jp, jA, jAp, e.partials, S, alpha);
+        hipLaunchKernelGGL((k_cg_grad_rows<R, false, true>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, k2, ep, em, h,
+                           mt, p, ae, a, jc, jp, jA, jAp, e.partials, S, alpha, W);
     else
-        hipLaunchKernelGGL((k_cg_gradient_pass<R, false>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, k2, ep, em, h,
-                           mt, p, ae, a, jc, jp, jA, jAp, e.partials, S, alpha);
-    GLX_LAUNCH_CHECK("k_cg_gradient_pass");
+        hipLaunchKernelGGL((k_cg_grad_rows<R, false, false>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, k2, ep, em, h,
+                           mt, p, ae, a, jc, jp, jA, jAp, e.partials, S, alpha, W);
+    GLX_LAUNCH_CHECK("k_cg_grad_rows");
     return GLX_OK;
 }
 
@@ -988,18 +1048,23 @@ int L<R>::cg_direction_step(const Env &e, glx_grid g, double kappa2, double eps,
         return set_error(GLX_E_ARG, "glx_cg_direction_step: the new direction needs a buffer of its own");
     if (kappa2 > 0.0 && !(abei || ab)) return set_error(GLX_E_ARG, "glx_cg_direction_step: finite kappa needs ab");
     const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
-    const int nb = grid_blocks(G.N, RED_BLOCK, RED_MAX_BLOCKS);
+    const long W = fit_row_waves(withA ? (const void *)k_coef_dir_rows<R, 17> : (const void *)k_coef_dir_rows<R, 5>,
+                                 cg_row_waves(g.Nx, g.Ny), e.num_cus);
+    const int nb = (int)((W + 3) / 4);
     const double dxdy = double(R(g.dx)) * double(R(g.dy));
     DirUpdate<R> U{(const cplx<R> *)dpsi, (const cplx<R> *)jac, (const R *)dab, (const R *)jacA,
                    (cplx<R> *)dpsi_out, (R *)dab_out, R(0), R(0)};
     const int nc = withA ? 17 : 5;
+    // the reference's coefficient kernels read only the scalar epsilon
     if (withA)
-        hipLaunchKernelGGL((k_coef_dir<R, 17>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, R(kappa2), R(eps), R(H), mt,
-                           (const cplx<R> *)psi, (const R *)abei, (const R *)ab, U, beta_dev, e.partials);
+        hipLaunchKernelGGL((k_coef_dir_rows<R, 17>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, R(kappa2), R(eps),
+                           (const R *)nullptr, R(H), mt, (const cplx<R> *)psi, (const R *)abei, (const R *)ab, U,
+                           beta_dev, e.partials, W);
     else
-        hipLaunchKernelGGL((k_coef_dir<R, 5>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, R(kappa2), R(eps), R(H), mt,
-                           (const cplx<R> *)psi, (const R *)abei, (const R *)ab, U, beta_dev, e.partials);
-    GLX_LAUNCH_CHECK("k_coef_dir");
+        hipLaunchKernelGGL((k_coef_dir_rows<R, 5>), dim3(nb), dim3(RED_BLOCK), 0, e.stream, G, R(kappa2), R(eps),
+                           (const R *)nullptr, R(H), mt, (const cplx<R> *)psi, (const R *)abei, (const R *)ab, U,
+                           beta_dev, e.partials, W);
+    GLX_LAUNCH_CHECK("k_coef_dir_rows");
     if (c) {
         hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(RED_BLOCK), 0, e.stream, e.partials, nb, nc, dxdy, e.result);
         GLX_LAUNCH_CHECK("k_finish_partials");
