@@ -8,7 +8,7 @@
 // One workgroup owns one instance at a time and runs whole CG iterations on
 // it: every reduction (PR sums, gradient norm, the 17 line-search
 // coefficients) is a workgroup reduction in FP64, the line search is solved
-// by thread 0 in FP64, and the phases are separated by workgroup barriers
+// by the first wave in FP64, and the phases are separated by workgroup barriers
 // only. Instances therefore iterate and stop independently (ragged batch);
 // the per-instance state (iteration, betas, initial gradient norm, stop
 // flag) lives in device memory so a run is split into launches of a few
@@ -154,7 +154,8 @@ __global__ void __launch_bounds__(256, BatchCGBound<R>::MIN_WG) k_batch_cg(const
             }
             block_sums<L>(cacc, lds, red);
             th = block_max(th, lds);
-            if (threadIdx.x == 0) {
+            th = __shfl(th, 0, WAVE);  // block_max leaves it in thread 0
+            if (threadIdx.x < WAVE) {  // the first wave runs the line search
                 double c5[5][5] = {};
                 const double sc = double(q.dx) * double(q.dy);
                 if (withA) {
@@ -169,11 +170,13 @@ __global__ void __launch_bounds__(256, BatchCGBound<R>::MIN_WG) k_batch_cg(const
                     for (int i = 0; i < 5; ++i) c5[i][0] = sc * red[i];
                 }
                 double x, y;
-                quartic2_min(c5, !withA, x, y, alpha_A_bound(th));
-                sv[CG_APSI] = x;
-                sv[CG_AA] = y;
-                bc[2] = x;
-                bc[3] = y;
+                quartic2_min_wave(c5, !withA, x, y, alpha_A_bound(th));
+                if (threadIdx.x == 0) {
+                    sv[CG_APSI] = x;
+                    sv[CG_AA] = y;
+                    bc[2] = x;
+                    bc[3] = y;
+                }
             }
             __syncthreads();
             // 3. update

@@ -21,7 +21,9 @@
 namespace glx {
 
 // Waves of a row-marching CG pass: about 16 rows per wave, at most 4096
-// waves (1024 workgroups, the partials' capacity).
+// waves (1024 workgroups, the partials' capacity). The passes hand out the
+// rows row-major (for_rows_rm): waves of consecutive index march the same
+// rows of adjacent strips, so the chip reads whole grid rows together.
 inline long cg_row_waves(int Nx, int Ny) {
     const long T = (long)nstrips_of<1>(Nx) * Ny;
     long W = (T + 15) / 16;
@@ -35,34 +37,46 @@ template <typename R> struct LinkPair {
     R e, b;
 };
 
+// Rows per batch of the double-buffered march (two batches in flight).
+template <typename R> struct CGRows { static constexpr int B = 1; };
+template <> struct CGRows<float> { static constexpr int B = 2; };
+
+// Every load of a marching step is issued unconditionally, from a valid
+// address: a buffer that is absent is replaced by one of at least its size
+// (psi holds 2 N reals, more than the Na + Nb links, the N nodes of an
+// epsilon map or the bytes of the cell flags) and its values are dropped by
+// a select where they are used. Straight-line loads let the compiler keep a
+// whole batch in flight while it computes the previous one.
+
 template <typename R, bool withA, bool UPD>
 __global__ void __launch_bounds__(256) k_cg_grad_rows(Geo g, R kappa2, R eps, const R *eps_map, R H, const uint8_t *mt,
                                                       const cplx<R> *psi, const R *abei, const R *ab, cplx<R> *jac,
                                                       const cplx<R> *jac_prev, R *jacA, const R *jacA_prev,
                                                       double *partials, StateUpdate<R> S, const double *alpha, long W) {
+    constexpr int B = CGRows<R>::B;
     const GeoR<R> q(g);
     if constexpr (UPD) {
         S.ap = R(alpha[0]);
         S.aa = R(alpha[1]);
     }
     const R dxdy = q.dx * q.dy, s2 = R(2) * q.dx * q.dy;
-    const bool he = abei != nullptr, hb = ab != nullptr, has_mt = mt != nullptr;
+    const bool he = abei != nullptr, hb = ab != nullptr, has_mt = mt != nullptr, has_em = eps_map != nullptr;
     const bool upd_a = UPD && S.dab != nullptr;
-    const R *ae_b = he ? abei + g.Na : nullptr;
-    const R *ab_b = hb ? ab + g.Na : nullptr;
-    const R *da_b = upd_a ? S.dab + g.Na : nullptr;
+    const R *any = (const R *)psi;
+    const R *abei_s = he ? abei : any, *ab_s = hb ? ab : any, *dab_s = upd_a ? S.dab : any;
+    const uint8_t *mt_s = has_mt ? mt : (const uint8_t *)psi;
+    const R *em_s = has_em ? eps_map : any;
+    const cplx<R> *dpsi_s = UPD ? S.dpsi : psi;
+    const R *jAp_s = withA ? jacA_prev : any;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
-    if (w < W) for_rows(g.Nx, g.Ny, w, W, [&](int s, int ja, int jb) {
+    if (w < W) for_rows_rm<1>(g.Nx, g.Ny, w, W, [&](int s, int ja, int jb) {
         const ColCtx X = col_ctx<1>(g.Nx, s);
         const int c = X.c;
         const uint32_t cc = X.cc, ca = X.ca;
-        // Raw loads of one marching step: psi at row j+1 and the a links
-        // there, the b links, cell flag, epsilon and previous gradients of
-        // row j. A step's loads are issued one step ahead, before the
-        // previous step's arithmetic, and the state update (UPD) is applied
-        // where they are consumed. Indices are clamped into the grid (rows
-        // past it are never used).
+        // Raw loads of marching step j: psi (and its update direction) at
+        // row j+1 and the a links there; the b links, cell flag, epsilon and
+        // previous gradients of row j. Indices are clamped into the grid.
         struct Step {
             cplx<R> p, dp, gp;
             R ae, ab, ad, be, bb, bd, ee, vpa, vpb;
@@ -72,29 +86,22 @@ __global__ void __launch_bounds__(256) k_cg_grad_rows(Geo g, R kappa2, R eps, co
         auto ka_of = [&](int j) { return ca + (long)g.Nxa * min(max(j, 0), g.Ny - 1); };
         auto kb_of = [&](int j) { return g.Na + cc + (long)g.Nxb * min(max(j, 0), g.Ny - 2); };
         auto kc_of = [&](int j) { return ca + (long)g.Nxc * min(max(j, 0), g.Ny - 2); };
-        auto lpsi = [&](int j, cplx<R> &p, cplx<R> &d) {
-            const long k = kpsi(j);
-            p = ld(psi, k);
-            if constexpr (UPD) d = ld(S.dpsi, k);
-        };
-        auto llink = [&](long k, R &e, R &b, R &d) {
-            if (he) e = abei[k];
-            if (hb) b = ab[k];
-            if (upd_a) d = S.dab[k];
-        };
         auto fetch = [&](int j) {
-            Step t{};
-            lpsi(j + 1, t.p, t.dp);
-            llink(ka_of(j + 1), t.ae, t.ab, t.ad);
-            llink(kb_of(j), t.be, t.bb, t.bd);
-            t.mc = has_mt ? mt[kc_of(j)] : uint8_t(1);
-            const long n = kpsi(j);
-            t.ee = eps_map ? eps_map[n] : eps;
+            Step t;
+            const long kp = kpsi(j + 1), ka = ka_of(j + 1), kb = kb_of(j), n = kpsi(j);
+            t.p = ld(psi, kp);
+            t.dp = ld(dpsi_s, kp);
+            t.ae = abei_s[ka];
+            t.ab = ab_s[ka];
+            t.ad = dab_s[ka];
+            t.be = abei_s[kb];
+            t.bb = ab_s[kb];
+            t.bd = dab_s[kb];
+            t.mc = mt_s[kc_of(j)];
+            t.ee = em_s[n];
             t.gp = ld(jac_prev, n);
-            if (withA) {
-                t.vpa = jacA_prev[ka_of(j)];
-                t.vpb = jacA_prev[kb_of(j)];
-            }
+            t.vpa = jAp_s[ka_of(j)];
+            t.vpb = jAp_s[kb];
             return t;
         };
         auto upd_psi = [&](cplx<R> p, const cplx<R> &d) {
@@ -105,7 +112,7 @@ __global__ void __launch_bounds__(256) k_cg_grad_rows(Geo g, R kappa2, R eps, co
             return p;
         };
         auto upd_link = [&](R e, R b, R d) {
-            LinkPair<R> v{e, b};
+            LinkPair<R> v{he ? e : R(0), hb ? b : R(0)};
             if (upd_a) v.b = S.aa * d + v.b;
             return v;
         };
@@ -122,19 +129,13 @@ __global__ void __launch_bounds__(256) k_cg_grad_rows(Geo g, R kappa2, R eps, co
         LinkPair<R> Ad, Ac, Bd;
         bool md;
         {
-            cplx<R> p{}, d{};
-            lpsi(ja - 1, p, d);
-            Pd = upd_psi(p, d);
-            lpsi(ja, p, d);
-            Pc = upd_psi(p, d);
-            R le = R(0), lb = R(0), ld_ = R(0);
-            llink(ka_of(ja - 1), le, lb, ld_);
-            Ad = upd_link(le, lb, ld_);
-            llink(ka_of(ja), le, lb, ld_);
-            Ac = upd_link(le, lb, ld_);
-            llink(kb_of(ja - 1), le, lb, ld_);
-            Bd = upd_link(le, lb, ld_);
-            md = has_mt ? mt[kc_of(ja - 1)] != 0 : true;
+            const long k0 = kpsi(ja - 1), k1 = kpsi(ja), a0 = ka_of(ja - 1), a1 = ka_of(ja), b0 = kb_of(ja - 1);
+            Pd = upd_psi(ld(psi, k0), ld(dpsi_s, k0));
+            Pc = upd_psi(ld(psi, k1), ld(dpsi_s, k1));
+            Ad = upd_link(abei_s[a0], ab_s[a0], dab_s[a0]);
+            Ac = upd_link(abei_s[a1], ab_s[a1], dab_s[a1]);
+            Bd = upd_link(abei_s[b0], ab_s[b0], dab_s[b0]);
+            md = has_mt ? mt_s[kc_of(ja - 1)] != 0 : true;
         }
         R sud = R(0), cud = R(1);
         if (ja > 0) {
@@ -142,16 +143,13 @@ __global__ void __launch_bounds__(256) k_cg_grad_rows(Geo g, R kappa2, R eps, co
             const bool lu_d = (c > 0 && mdl) || (c + 1 < g.Nx && md);  // cells (c-1, ja-1), (c, ja-1)
             if (X.col && lu_d) sincos_r(q.dy * pv(Bd), &sud, &cud);
         }
-        Step nx = fetch(ja);
-        for (int j = ja; j < jb; ++j) {
-            const Step t = nx;
-            nx = fetch(min(j + 1, jb - 1));
+        auto consume = [&](const Step &t, int j) {
             const long n = (long)g.Nx * j + cc;
             const cplx<R> Pu = upd_psi(t.p, t.dp);
             const LinkPair<R> Au = upd_link(t.ae, t.ab, t.ad), Bc = upd_link(t.be, t.bb, t.bd);
-            const bool mc = t.mc != 0;
+            const bool mc = has_mt ? t.mc != 0 : true;
             GradIn<R> in;
-            in.ee = t.ee;
+            in.ee = has_em ? t.ee : eps;
             const cplx<R> gp = t.gp;
             const long ka = ca + (long)g.Nxa * j, kb = g.Na + cc + (long)g.Nxb * min(j, g.Ny - 2);
             const R vpa = withA ? t.vpa : R(0), vpb = withA ? t.vpb : R(0);
@@ -235,6 +233,23 @@ __global__ void __launch_bounds__(256) k_cg_grad_rows(Geo g, R kappa2, R eps, co
             md = mc;
             sud = in.su;
             cud = in.cu;
+        };
+        // two batches of B steps in flight: the loads of one batch are
+        // issued before the other is computed
+        Step S0[B], S1[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) S0[k] = fetch(min(ja + k, jb - 1));
+        for (int j = ja; j < jb; j += 2 * B) {
+#pragma unroll
+            for (int k = 0; k < B; ++k) S1[k] = fetch(min(j + B + k, jb - 1));
+#pragma unroll
+            for (int k = 0; k < B; ++k)
+                if (j + k < jb) consume(S0[k], j + k);
+#pragma unroll
+            for (int k = 0; k < B; ++k) S0[k] = fetch(min(j + 2 * B + k, jb - 1));
+#pragma unroll
+            for (int k = 0; k < B; ++k)
+                if (j + B + k < jb) consume(S1[k], j + B + k);
         }
     });
     block_store_partials<5>(acc, partials);
@@ -253,23 +268,29 @@ template <typename R, int L>
 __global__ void __launch_bounds__(256) k_coef_dir_rows(Geo g, R kappa2, R eps, const R *eps_map, R H, const uint8_t *mt,
                                                        const cplx<R> *psi, const R *abei, const R *ab, DirUpdate<R> U,
                                                        const double *beta, double *partials, long W) {
+    constexpr int B = CGRows<R>::B;
     const GeoR<R> q(g);
     U.bp = R(beta[0]);
     U.ba = R(beta[1]);
-    const bool he = abei != nullptr, hb = ab != nullptr, has_mt = mt != nullptr;
+    const bool he = abei != nullptr, hb = ab != nullptr, has_mt = mt != nullptr, has_em = eps_map != nullptr;
+    const R *any = (const R *)psi;
+    const R *abei_s = he ? abei : any, *ab_s = hb ? ab : any;
+    const R *dAold_s = L == 17 ? U.dAold : any, *jacA_s = L == 17 ? U.jacA : any;
+    const uint8_t *mt_s = has_mt ? mt : (const uint8_t *)psi;
+    const R *em_s = has_em ? eps_map : any;
     double acc[L];
 #pragma unroll
     for (int k = 0; k < L; ++k) acc[k] = 0.0;
     double th = 0.0;
     const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
-    if (w < W) for_rows(g.Nx, g.Ny, w, W, [&](int s, int ja, int jb) {
+    if (w < W) for_rows_rm<1>(g.Nx, g.Ny, w, W, [&](int s, int ja, int jb) {
         const ColCtx X = col_ctx<1>(g.Nx, s);
         const int c = X.c;
         const uint32_t cc = X.cc, ca = X.ca;
-        // Raw loads of one marching step (psi, the old direction and the
+        // Raw loads of marching step j (psi, the old direction and the
         // gradient at row j+1, the a links there, the b links, cell flag and
-        // epsilon of row j), issued one step ahead of their use; the new
-        // direction beta d_old - g is formed where they are consumed.
+        // epsilon of row j); the new direction beta d_old - g is formed where
+        // they are consumed.
         struct Lk {
             R e, b, o, g;  // abei, ab, old A direction, A gradient
         };
@@ -282,17 +303,11 @@ __global__ void __launch_bounds__(256) k_coef_dir_rows(Geo g, R kappa2, R eps, c
         auto row = [&](int j) { return (long)g.Nx * min(max(j, 0), g.Ny - 1) + cc; };
         auto ka_of = [&](int j) { return ca + (long)g.Nxa * min(max(j, 0), g.Ny - 1); };
         auto kb_of = [&](int j) { return g.Na + cc + (long)g.Nxb * min(max(j, 0), g.Ny - 2); };
-        auto link = [&](long k) {
-            Lk v{R(0), R(0), R(0), R(0)};
-            if (he) v.e = abei[k];
-            if (hb) v.b = ab[k];
-            if (L == 17) {
-                v.o = U.dAold[k];
-                v.g = U.jacA[k];
-            }
-            return v;
+        auto kc_of = [&](int j) { return ca + (long)g.Nxc * min(max(j, 0), g.Ny - 2); };
+        auto link = [&](long k) { return Lk{abei_s[k], ab_s[k], dAold_s[k], jacA_s[k]}; };
+        auto dlink = [&](const Lk &v) {
+            return LinkTriple<R>{he ? v.e : R(0), hb ? v.b : R(0), L == 17 ? U.ba * v.o - v.g : R(0)};
         };
-        auto dlink = [&](const Lk &v) { return LinkTriple<R>{v.e, v.b, L == 17 ? U.ba * v.o - v.g : R(0)}; };
         auto dnode = [&](const cplx<R> &o, const cplx<R> &gk) {
             return cplx<R>{U.bp * o.re - gk.re, U.bp * o.im - gk.im};
         };
@@ -304,24 +319,21 @@ __global__ void __launch_bounds__(256) k_coef_dir_rows(Geo g, R kappa2, R eps, c
             t.gr = ld(U.jac, k);
             t.a = link(ka_of(j + 1));
             t.b = link(kb_of(j));
-            t.mc = has_mt ? mt[ca + (long)g.Nxc * min(max(j, 0), g.Ny - 2)] : uint8_t(1);
-            t.eps = eps_map ? eps_map[row(j)] : eps;
+            t.mc = mt_s[kc_of(j)];
+            t.eps = em_s[row(j)];
             return t;
         };
         // window: psi, direction and the a links at row ja; the cell below
-        cplx<R> Pc = ld(psi, row(ja)), Dc = dir_at(U, row(ja));
+        cplx<R> Pc = ld(psi, row(ja)), Dc = dnode(ld(U.dold, row(ja)), ld(U.jac, row(ja)));
         LinkTriple<R> Ac = dlink(link(ka_of(ja)));
-        bool md = has_mt ? mt[ca + (long)g.Nxc * min(max(ja - 1, 0), g.Ny - 2)] != 0 : true;
-        Step nx = fetch(ja);
-        for (int j = ja; j < jb; ++j) {
-            const Step t = nx;
-            nx = fetch(min(j + 1, jb - 1));
+        bool md = has_mt ? mt_s[kc_of(ja - 1)] != 0 : true;
+        auto consume = [&](const Step &t, int j) {
             const long n = (long)g.Nx * j + cc;
             const cplx<R> Pu = t.p, Du = dnode(t.o, t.gr);
             const LinkTriple<R> Au = dlink(t.a), Bc = dlink(t.b);
-            const bool mc = t.mc != 0;
+            const bool mc = has_mt ? t.mc != 0 : true;
             CoefIn<R> in;
-            in.eps = t.eps;
+            in.eps = has_em ? t.eps : eps;
             in.p0 = Pc;
             in.p_u = Pu;
             in.p_r.re = shfl_dn1(Pc.re);
@@ -353,6 +365,21 @@ __global__ void __launch_bounds__(256) k_coef_dir_rows(Geo g, R kappa2, R eps, c
             Dc = Du;
             Ac = Au;
             md = mc;
+        };
+        Step S0[B], S1[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) S0[k] = fetch(min(ja + k, jb - 1));
+        for (int j = ja; j < jb; j += 2 * B) {
+#pragma unroll
+            for (int k = 0; k < B; ++k) S1[k] = fetch(min(j + B + k, jb - 1));
+#pragma unroll
+            for (int k = 0; k < B; ++k)
+                if (j + k < jb) consume(S0[k], j + k);
+#pragma unroll
+            for (int k = 0; k < B; ++k) S0[k] = fetch(min(j + 2 * B + k, jb - 1));
+#pragma unroll
+            for (int k = 0; k < B; ++k)
+                if (j + B + k < jb) consume(S1[k], j + B + k);
         }
     });
     block_store_partials<L>(acc, partials);

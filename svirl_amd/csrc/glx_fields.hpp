@@ -268,6 +268,34 @@ __device__ __forceinline__ void block_store_partials(double (&acc)[L], double *p
     }
 }
 
+// red[k] = sum_b partials[b*L + k] for k < L into shared memory, one
+// workgroup: each thread loads the L values of its blocks together and the
+// L sums are reduced over the workgroup at once. Ends with a barrier.
+template <int L> __device__ __forceinline__ void reduce_partials(const double *partials, int nb, double *red) {
+    __shared__ double lds[RED_BLOCK / WAVE * L];
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    const int nw = blockDim.x / WAVE;
+    double acc[L];
+#pragma unroll
+    for (int k = 0; k < L; ++k) acc[k] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < L; ++k) acc[k] += partials[(long)b * L + k];
+    }
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+        const double v = wave_sum(acc[k]);
+        if (lane == 0) lds[wid * L + k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < L) {
+        double t = 0.0;
+        for (int w = 0; w < nw; ++w) t += lds[w * L + threadIdx.x];
+        red[threadIdx.x] = t;
+    }
+    __syncthreads();
+}
+
 // out[k] = scale * sum_b partials[b*L + k], one workgroup, fixed order.
 static __global__ void __launch_bounds__(256) k_finish_partials(const double *partials, int nb, int L, double scale, double *out) {
     __shared__ double lds[RED_BLOCK / WAVE];
@@ -848,14 +876,8 @@ __global__ void __launch_bounds__(256) k_cg_gradient_pass(Geo g, R kappa2, R eps
 // direction update.
 template <typename R>
 __global__ void __launch_bounds__(256) k_cg_gradient_finish(const double *partials, int nb, double dxdy, double *out, double *beta_next) {
-    __shared__ double lds[RED_BLOCK / WAVE];
-    double v[5];
-    for (int k = 0; k < 5; ++k) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partials[(long)b * 5 + k];
-        v[k] = block_sum(s, lds);
-        __syncthreads();
-    }
+    __shared__ double v[5];
+    reduce_partials<5>(partials, nb, v);
     if (threadIdx.x == 0) {
         out[0] = v[0] * dxdy;
         beta_next[0] = double(R(fmax(v[1] / v[2], 0.0)));
@@ -944,6 +966,49 @@ __device__ __forceinline__ void quartic2_min(const double (&c)[5][5], bool one_d
     }
 }
 
+// quartic2_min run by a whole wave, every lane with the same c: the
+// backtracking steps t = 2^-k, k < 60, are tried at once, lane k taking
+// 2^-k, and the first that passes the sufficient-decrease test is taken, as
+// the sequential loop would; each lane's arithmetic is that loop's k-th
+// iteration, so the result is the same to the bit. All 64 lanes must call.
+__device__ __forceinline__ void quartic2_min_wave(const double (&c)[5][5], bool one_d, double &x, double &y,
+                                                  double ymax = HUGE_VAL) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const double tl = ldexp(1.0, -lane);  // 0.5 applied lane times to 1.0
+    x = 0.0;
+    y = 0.0;
+    for (int it = 0; it < 80; ++it) {
+        double p, px, py, pxx, pxy, pyy;
+        poly2(c, x, y, p, px, py, pxx, pxy, pyy);
+        if (one_d) { py = 0.0; pxy = 0.0; pyy = 1.0; }
+        const double gn = fabs(px) + fabs(py);
+        if (!(gn > 0.0)) break;
+        double dx, dy;
+        const double det = pxx * pyy - pxy * pxy;
+        if (pxx > 0.0 && det > 0.0) {
+            dx = -(pyy * px - pxy * py) / det;
+            dy = -(pxx * py - pxy * px) / det;
+        } else {
+            dx = -px;
+            dy = -py;
+        }
+        const double slope = px * dx + py * dy;
+        if (!(slope < 0.0)) { dx = -px; dy = -py; }
+        double pn, a, b, c2, d, e2;
+        const double sx = tl * dx;
+        const double sy = fmin(fmax(y + tl * dy, -ymax), ymax) - y;
+        poly2(c, x + sx, y + sy, pn, a, b, c2, d, e2);
+        const bool pass = lane < 60 && pn < p && pn <= p + 1.0e-4 * fmin(px * sx + py * sy, 0.0);
+        const unsigned long long m = __ballot(pass);
+        if (!m) break;
+        const int k = __ffsll((long long)m) - 1;
+        const double tx = __shfl(sx, k, WAVE), ty = __shfl(sy, k, WAVE);
+        x += tx;
+        y += ty;
+        if (fabs(tx) + fabs(ty) <= 1.0e-15 * (fabs(x) + fabs(y))) break;
+    }
+}
+
 // Device line search of the FP32-mixed CG (glx_cg_line_step): the L
 // coefficient sums of k_coef (partials), times scale, packed as the
 // reference packs them (cg.py:369-373, c[i][0] for the psi-only quartic),
@@ -953,20 +1018,16 @@ template <int L>
 __global__ void __launch_bounds__(256) k_line_search(const double *partials, int nb, double scale, double *c, double *alpha) {
     __shared__ double lds[RED_BLOCK / WAVE];
     __shared__ double red[L];
-    for (int k = 0; k < L; ++k) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partials[(long)b * L + k];
-        s = block_sum(s, lds);
-        if (threadIdx.x == 0) red[k] = s * scale;
-        __syncthreads();
-    }
+    reduce_partials<L>(partials, nb, red);
+    if (threadIdx.x < L) red[threadIdx.x] *= scale;  // read by thread 0 after block_max's barriers
     // the coefficient pass left each block's largest link phase step after
     // the sums (k_coef, k_coef_dir with L = 17)
     double th = 0.0;
     if (L == 17)
         for (int b = threadIdx.x; b < nb; b += blockDim.x) th = fmax(th, partials[(long)nb * L + b]);
     th = block_max(th, lds);
-    if (threadIdx.x == 0) {
+    th = __shfl(th, 0, WAVE);  // block_max leaves it in thread 0
+    if (threadIdx.x < WAVE) {
         double c5[5][5] = {};
         if (L == 17) {
             for (int k = 0; k < 5; ++k) {
@@ -980,10 +1041,12 @@ __global__ void __launch_bounds__(256) k_line_search(const double *partials, int
             for (int k = 0; k < 5; ++k) c5[k][0] = red[k];
         }
         double x, y;
-        quartic2_min(c5, L != 17, x, y, alpha_A_bound(th));
-        alpha[0] = x;
-        alpha[1] = y;
-        for (int k = 0; k < L; ++k) c[k] = red[k];
+        quartic2_min_wave(c5, L != 17, x, y, alpha_A_bound(th));
+        if (threadIdx.x == 0) {
+            alpha[0] = x;
+            alpha[1] = y;
+            for (int k = 0; k < L; ++k) c[k] = red[k];
+        }
     }
 }
 
@@ -1269,14 +1332,14 @@ __global__ void __launch_bounds__(256) k_coef_dir(Geo g, R kappa2, R eps, R H, c
 
 // --------------------------------------------------------------- vortices
 
-// fmod(x, b) for b > 0 from the truncated quotient: the remainder x - n b
-// of the right n is a representable number, so one FMA gives it exactly;
-// n = trunc(x / b) is off by at most one for |x / b| < 2^20, which the
-// sign and size of the FMA's result reveal (a wrong n leaves a nonzero
-// remainder of the wrong sign or at least b in size). Larger or non-finite
-// x take the library fmod.
-__device__ __forceinline__ double fmod_pos(double x, double b) {
-    double n = trunc(x / b);
+// fmod(x, b) for b > 0 from an estimate of the truncated quotient: the
+// remainder x - n b of the right n is a representable number, so one FMA
+// gives it exactly; n = trunc(x * inv_b), inv_b the rounded 1 / b, is off by
+// at most one for |x / b| < 2^20, which the sign and size of the FMA's
+// result reveal (a wrong n leaves a nonzero remainder of the wrong sign or
+// at least b in size). Larger or non-finite x take the library fmod.
+__device__ __forceinline__ double fmod_pos(double x, double b, double inv_b) {
+    double n = trunc(x * inv_b);
     if (!(fabs(n) < 1048576.0)) return fmod(x, b);
     double r = fma(-n, b, x);
     if (x >= 0.0) {
@@ -1294,7 +1357,7 @@ __device__ __forceinline__ double fmod_pos(double x, double b) {
 // np.mod(x, 2 pi) for float64 (fmod, shifted into [0, 2 pi)).
 __device__ __forceinline__ double np_mod_2pi(double x) {
     const double b = 6.283185307179586;  // 2.0*np.pi
-    double m = fmod_pos(x, b);
+    double m = fmod_pos(x, b, 0.15915494309189535);  // 1 / b rounded
     if (m != 0.0) {
         if (m < 0.0) m += b;
     } else {
@@ -1322,7 +1385,7 @@ __device__ __forceinline__ bool winding_flag(double v) { return fabs(v) > 0.5 &&
 
 // Vortex cells in three launches, none with an atomic: (1) k_vortex_flags
 // evaluates every cell and stores, per strip row, the 64-bit mask of the
-// flagged cells, and per wave their number; (2) k_vortex_scan turns the
+// flagged cells (at s Nyc + j), and per wave their number; (2) k_vortex_scan turns the
 // per-wave numbers into the first record slot of each wave and the total;
 // (3) k_vortex_emit walks the masks again and writes a record for each set
 // bit, re-evaluating only those cells (the same expression on the same
@@ -1347,12 +1410,26 @@ inline long vortex_waves(const Geo &g) {
     return W < 1 ? 1 : W;
 }
 
-// Rows r0 <= r < r1 of the (strip, cell row) items r = s Nyc + j of wave w.
-__device__ __forceinline__ void vortex_range(const Geo &g, long w, long W, long &r0, long &r1) {
-    const long T = (long)((g.Nxc + VSTRIP - 1) / VSTRIP) * g.Nyc;
-    r0 = (w * T) / W;
-    r1 = ((w + 1) * T) / W;
+// The (strip, cell row range) items of wave w, handed out row-major as the
+// row-marching sweeps do: the cell rows are cut into S = max(1, W / ns)
+// segments and item k = (segment, strip) goes to wave k mod U, U =
+// min(W, S ns), so waves of consecutive index march the same rows of
+// adjacent strips. f(s, ja, jb) per item, in a fixed order.
+template <typename F> __device__ __forceinline__ void vortex_items(const Geo &g, long w, long W, F &&f) {
+    const long ns = (g.Nxc + VSTRIP - 1) / VSTRIP;
+    const long S = W >= ns ? W / ns : 1;
+    const long U = W < S * ns ? W : S * ns;
+    for (long k = w; k < S * ns; k += U) {
+        const long seg = k / ns;
+        const int ja = (int)((seg * g.Nyc) / S), jb = (int)(((seg + 1) * g.Nyc) / S);
+        if (ja < jb) f((int)(k % ns), ja, jb);
+    }
 }
+
+// Rows are taken VROWS at a time: their loads are issued together and
+// their phases evaluated together, so a wave waits for memory once per
+// VROWS rows.
+constexpr int VROWS = 8;
 
 template <typename R>
 __global__ void __launch_bounds__(256) k_vortex_flags(Geo g, const cplx<R> *psi, const R *ab, double H,
@@ -1361,62 +1438,108 @@ __global__ void __launch_bounds__(256) k_vortex_flags(Geo g, const cplx<R> *psi,
     const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
     if (w >= W) return;  // wave-uniform
     const int lane = threadIdx.x & (WAVE - 1);
-    long r0, r1;
-    vortex_range(g, w, W, r0, r1);
     const R *bl = ab + g.Na;
     int cnt = 0;
-    while (r0 < r1) {
-        const int s = (int)(r0 / g.Nyc), ja = (int)(r0 % g.Nyc);
-        const int jb = (int)min((long)g.Nyc, (long)ja + (r1 - r0));
+    vortex_items(g, w, W, [&](int s, int ja, int jb) {
         const int i = s * VSTRIP + lane;
         const bool col = i < g.Nx, cell = lane < VSTRIP && i < g.Nxc;
         const int ic = min(i, g.Nx - 1), iac = min(i, g.Nxa - 1);
         const cplx<R> p0 = ld(psi, ic + (long)g.Nx * ja);
         double t0 = col ? atan2(double(p0.im), double(p0.re)) : 0.0;
         double a0 = double(ab[iac + (long)g.Nxa * ja]);
-        for (int j = ja; j < jb; ++j) {
-            const cplx<R> p1 = ld(psi, ic + (long)g.Nx * (j + 1));
-            const double t1 = col ? atan2(double(p1.im), double(p1.re)) : 0.0;
-            const double a1 = double(ab[iac + (long)g.Nxa * (j + 1)]);
-            const double b0 = double(bl[ic + (long)g.Nxb * j]);
-            const double t0r = shfl_dn1(t0), t1r = shfl_dn1(t1), b0r = shfl_dn1(b0);
-            bool flag = false;
-            if (cell) flag = winding_flag(cell_winding(g, dxdyH, t0, t0r, t1, t1r, a0, a1, b0, b0r));
-            const unsigned long long m = __ballot(flag);
-            if (lane == 0) bits[(long)s * g.Nyc + j] = m;
-            cnt += (int)__popcll(m);
-            t0 = t1;
-            a0 = a1;
+        for (int j = ja; j < jb; j += VROWS) {
+            // rows j + k (clamped to jb - 1 for the loads: node row <= Nyc,
+            // b row <= Nyc - 1)
+            cplx<R> p1[VROWS];
+            R a1[VROWS], b0[VROWS];
+#pragma unroll
+            for (int k = 0; k < VROWS; ++k) {
+                const int jj = min(j + k, jb - 1);
+                p1[k] = ld(psi, ic + (long)g.Nx * (jj + 1));
+                a1[k] = ab[iac + (long)g.Nxa * (jj + 1)];
+                b0[k] = bl[ic + (long)g.Nxb * jj];
+            }
+            double t1[VROWS];
+#pragma unroll
+            for (int k = 0; k < VROWS; ++k) t1[k] = col ? atan2(double(p1[k].im), double(p1[k].re)) : 0.0;
+#pragma unroll
+            for (int k = 0; k < VROWS; ++k) {
+                if (j + k < jb) {  // wave-uniform
+                    const double a1k = double(a1[k]), b0k = double(b0[k]);
+                    const double t0r = shfl_dn1(t0), t1r = shfl_dn1(t1[k]), b0r = shfl_dn1(b0k);
+                    bool flag = false;
+                    if (cell) flag = winding_flag(cell_winding(g, dxdyH, t0, t0r, t1[k], t1r, a0, a1k, b0k, b0r));
+                    const unsigned long long m = __ballot(flag);
+                    if (lane == 0) bits[(long)s * g.Nyc + j + k] = m;
+                    cnt += (int)__popcll(m);
+                    t0 = t1[k];
+                    a0 = a1k;
+                }
+            }
         }
-        r0 += jb - ja;
-    }
+    });
     if (lane == 0) wcount[w] = cnt;
 }
 
-// Exclusive prefix sum of the W per-wave counts into slot (one workgroup of
-// 1024 threads, each scanning a contiguous chunk), the total into *total.
+// Exclusive prefix sum of the W <= 32768 per-wave counts into slot (one
+// workgroup of 1024 threads, each scanning a contiguous chunk of at most
+// 32), the total into *total.
 static __global__ void __launch_bounds__(1024) k_vortex_scan(const int *wcount, int *slot, long W, int *total) {
     __shared__ long part[1024];
     const int t = threadIdx.x;
     const long C = (W + 1023) / 1024;
-    const long k0 = min((long)t * C, W), k1 = min(k0 + C, W);
+    const long k0 = (long)t * C;
+    int v[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) v[k] = (k < C && k0 + k < W) ? wcount[k0 + k] : 0;
     long sum = 0;
-    for (long k = k0; k < k1; ++k) sum += wcount[k];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) sum += v[k];
     part[t] = sum;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
-        const long v = t >= off ? part[t - off] : 0;
+        const long x = t >= off ? part[t - off] : 0;
         __syncthreads();
-        part[t] += v;
+        part[t] += x;
         __syncthreads();
     }
     long run = part[t] - sum;
-    for (long k = k0; k < k1; ++k) {
-        const int c = wcount[k];
-        slot[k] = (int)min(run, (long)INT_MAX);
-        run += c;
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+        if (k < C && k0 + k < W) slot[k0 + k] = (int)min(run, (long)INT_MAX);
+        run += v[k];
     }
     if (t == 1023) *total = (int)min(part[1023], (long)INT_MAX);
+}
+
+// Exclusive prefix sum of v over the lanes of a wave.
+__device__ __forceinline__ int wave_excl_scan(int v) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const int y = __shfl_up(x, d, WAVE);
+        if (lane >= d) x += y;
+    }
+    return x - v;
+}
+
+// Position of the n-th (from 0) set bit of m; m has more than n set bits.
+__device__ __forceinline__ int nth_set_bit(unsigned long long m, int n) {
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const unsigned long long lo = m & ((1ull << w) - 1ull);
+        const int c = (int)__popcll(lo);
+        if (n >= c) {
+            n -= c;
+            m >>= w;
+            pos += w;
+        } else {
+            m = lo;
+        }
+    }
+    return pos;
 }
 
 // Records of the flagged cells: (cell n, v, psi at the four corners
@@ -1431,44 +1554,64 @@ __global__ void __launch_bounds__(256) k_vortex_emit(Geo g, const cplx<R> *psi, 
     const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
     if (w >= W) return;  // wave-uniform
     const int lane = threadIdx.x & (WAVE - 1);
-    long r0, r1;
-    vortex_range(g, w, W, r0, r1);
     long base = slot[w];
-    for (long r = r0; r < r1; ++r) {
-        const unsigned long long m = bits[r];
-        if (!m) continue;  // wave-uniform
-        if ((m >> lane) & 1ull) {
-            const int s = (int)(r / g.Nyc), j = (int)(r % g.Nyc);
-            const int i = s * VSTRIP + lane;  // i < Nxc, j < Nyc: a cell
-            const long n00 = i + (long)g.Nx * j, n0p = n00 + g.Nx;
-            const cplx<R> p00 = ld(psi, n00), pp0 = ld(psi, n00 + 1), p0p = ld(psi, n0p), ppp = ld(psi, n0p + 1);
-            const long ka00 = i + (long)g.Nxa * j, ka0p = ka00 + g.Nxa;
-            const long kb00 = g.Na + i + (long)g.Nxb * j, kbp0 = kb00 + 1;
-            const double a0 = double(ab[ka00]), a1 = double(ab[ka0p]), b0 = double(ab[kb00]), bp = double(ab[kbp0]);
-            const double v = cell_winding(g, dxdyH, atan2(double(p00.im), double(p00.re)),
-                                          atan2(double(pp0.im), double(pp0.re)), atan2(double(p0p.im), double(p0p.re)),
-                                          atan2(double(ppp.im), double(ppp.re)), a0, a1, b0, bp);
-            const long k = base + (long)__popcll(m & ((1ull << lane) - 1ull));
-            if (k < capacity) {
-                double *o = rec + k * VREC;
-                o[0] = double(i + (long)g.Nxc * j);
-                o[1] = v;
-                o[2] = double(p00.re); o[3] = double(p00.im);
-                o[4] = double(pp0.re); o[5] = double(pp0.im);
-                o[6] = double(ppp.re); o[7] = double(ppp.im);
-                o[8] = double(p0p.re); o[9] = double(p0p.im);
-                o[10] = a0 + (abi ? double(abi[ka00]) : 0.0);
-                o[11] = a1 + (abi ? double(abi[ka0p]) : 0.0);
-                o[12] = b0 + (abi ? double(abi[kb00]) : 0.0);
-                o[13] = bp + (abi ? double(abi[kbp0]) : 0.0);
+    vortex_items(g, w, W, [&](int s, int ja, int jb) {
+        const long r0 = (long)s * g.Nyc + ja, r1 = (long)s * g.Nyc + jb;
+        for (long rc = r0; rc < r1; rc += WAVE) {
+            // the masks of up to 64 rows in one load, lane k holding row rc + k
+            const unsigned long long mk = rc + lane < r1 ? bits[rc + lane] : 0ull;
+            const int ck = (int)__popcll(mk);
+            const int ek = wave_excl_scan(ck);  // flagged cells in the rows before row rc + lane
+            const int total = __shfl(ek + ck, WAVE - 1, WAVE);
+            // the flagged cells of these rows, one per lane: cell f lies in
+            // the last row k with ek <= f, at the (f - ek)-th set bit
+            for (int f0 = 0; f0 < total; f0 += WAVE) {
+                const int f = f0 + lane;
+                int k = 0;
+#pragma unroll
+                for (int step = WAVE / 2; step >= 1; step >>= 1) {
+                    const int e = __shfl(ek, min(k + step, WAVE - 1), WAVE);
+                    if (k + step < WAVE && e <= f) k += step;
+                }
+                const unsigned long long m = __shfl(mk, k, WAVE);
+                const int e = __shfl(ek, k, WAVE);
+                if (f < total) {
+                    const int j = ja + (int)(rc - r0) + k;
+                    const int i = s * VSTRIP + nth_set_bit(m, f - e);  // i < Nxc, j < Nyc: a cell
+                    const long n00 = i + (long)g.Nx * j, n0p = n00 + g.Nx;
+                    const cplx<R> p00 = ld(psi, n00), pp0 = ld(psi, n00 + 1), p0p = ld(psi, n0p),
+                                  ppp = ld(psi, n0p + 1);
+                    const long ka00 = i + (long)g.Nxa * j, ka0p = ka00 + g.Nxa;
+                    const long kb00 = g.Na + i + (long)g.Nxb * j, kbp0 = kb00 + 1;
+                    const double a0 = double(ab[ka00]), a1 = double(ab[ka0p]), b0 = double(ab[kb00]),
+                                 bp = double(ab[kbp0]);
+                    const double v = cell_winding(g, dxdyH, atan2(double(p00.im), double(p00.re)),
+                                                  atan2(double(pp0.im), double(pp0.re)),
+                                                  atan2(double(p0p.im), double(p0p.re)),
+                                                  atan2(double(ppp.im), double(ppp.re)), a0, a1, b0, bp);
+                    const long q = base + f;
+                    if (q < capacity) {
+                        double *o = rec + q * VREC;
+                        o[0] = double(i + (long)g.Nxc * j);
+                        o[1] = v;
+                        o[2] = double(p00.re); o[3] = double(p00.im);
+                        o[4] = double(pp0.re); o[5] = double(pp0.im);
+                        o[6] = double(ppp.re); o[7] = double(ppp.im);
+                        o[8] = double(p0p.re); o[9] = double(p0p.im);
+                        o[10] = a0 + (abi ? double(abi[ka00]) : 0.0);
+                        o[11] = a1 + (abi ? double(abi[ka0p]) : 0.0);
+                        o[12] = b0 + (abi ? double(abi[kb00]) : 0.0);
+                        o[13] = bp + (abi ? double(abi[kbp0]) : 0.0);
+                    }
+                }
             }
+            base += total;
         }
-        base += (long)__popcll(m);
-    }
+    });
 }
 
 // Sub-cell vortex position in each flagged cell, one thread per record of
-// k_vortex_cells (vortex_detector.py:103-147, with find_zero,
+// k_vortex_emit (vortex_detector.py:103-147, with find_zero,
 // find_zero_line and find_intersection of :75-101): the corner values are
 // turned into a common gauge, the zero lines of Re and Im are taken where
 // the linearised field changes sign along the cell border, and the vortex

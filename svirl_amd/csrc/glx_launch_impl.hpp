@@ -15,6 +15,35 @@
 
 namespace glx {
 
+// Waves of `fn` (256-thread workgroups) resident on the chip at once, from
+// its occupancy, queried once per kernel; 0 if the query fails.
+static long resident_waves(const void *fn, int num_cus) {
+    static std::mutex mu;
+    static std::unordered_map<const void *, int> per_cu;
+    int per = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = per_cu.find(fn);
+        if (it != per_cu.end()) {
+            per = it->second;
+        } else {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, RED_BLOCK, 0) != hipSuccess) per = 0;
+            per_cu[fn] = per;
+        }
+    }
+    return (long)per * num_cus * (RED_BLOCK / WAVE);
+}
+
+// Waves of a row-marching pass cut to whole rounds of the chip: W is
+// lowered to a multiple of the waves the kernel keeps resident, so that no
+// round runs a fraction of the chip.
+static long fit_row_waves(const void *fn, long W, int num_cus) {
+    const long res = resident_waves(fn, num_cus);
+    if (res <= 0 || W <= res) return W;
+    return (W / res) * res;
+}
+
+
 // Nodes are addressed with 32-bit offsets: Nx * Ny is capped at 2^30.
 inline bool grid_ok(const glx_grid &g) {
     return g.Nx >= 2 && g.Ny >= 2 && (long)g.Nx * g.Ny <= (1L << 30) && g.dx > 0.0 && g.dy > 0.0;
@@ -866,29 +895,6 @@ int L<R>::coef(const Env &e, glx_grid g, double kappa2, double eps, double H, co
 
 // The gradient pass, optionally with the state update fused in (S, alpha:
 // UPD; S.dpsi NULL: no update).
-// Waves of a row-marching pass cut to whole rounds of the chip: W is
-// lowered to a multiple of the waves the kernel keeps resident (its
-// occupancy x CUs), so that no round runs a fraction of the chip. The
-// occupancy is queried once per kernel.
-static long fit_row_waves(const void *fn, long W, int num_cus) {
-    static std::mutex mu;
-    static std::unordered_map<const void *, int> per_cu;
-    int per = 0;
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        auto it = per_cu.find(fn);
-        if (it != per_cu.end()) {
-            per = it->second;
-        } else {
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, RED_BLOCK, 0) != hipSuccess) per = 0;
-            per_cu[fn] = per;
-        }
-    }
-    const long res = (long)per * num_cus * (RED_BLOCK / WAVE);
-    if (res <= 0 || W <= res) return W;
-    return (W / res) * res;
-}
-
 template <typename R>
 static int gradient_pass_launch(const Env &e, glx_grid g, double kappa2, double eps, const void *eps_map, double H,
                                 const uint8_t *mt, const void *psi, const void *abei, const void *ab, void *jac,
