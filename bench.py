@@ -395,36 +395,54 @@ def run_batch_cg(args):
     """Config 5: n instances of 257^2 (kappa = 5) minimised by the batched
     CG, FP32-mixed then FP64, up to --cg-batch-iters iterations with the
     ragged stop ||grad F|| <= 1e-8 ||grad F||_0; the first iteration of
-    sampled instances against the oracle's exact line minimum."""
+    sampled instances against the oracle's exact line minimum. The
+    FP32-mixed run stops an instance whose gradient norm has not halved in
+    --cg-batch-stall iterations (its FP32 floor) and finishes it in FP64."""
     from svirl_amd import workloads
     from svirl_amd.batch import BatchCG
     z = gold("c5_sample")
     res = {"instances": args.cg_batch_n, "max_iters": args.cg_batch_iters, "rtol": 1e-8}
     F = {}
     for name, dtype in (("fp32", np.float32), ("fp64", np.float64)):
-        log(f"[c5] building {args.cg_batch_n} Config 5 instances ({name})")
-        sims = workloads.config5(n=args.cg_batch_n, dtype=dtype)
-        nodes = sum(int(s.cfg.N) for s in sims)
-        b = BatchCG(sims)
-        b.cg(max_iter=1, rtol=0.0)  # the first iteration, against the oracle
+        specs = workloads.config5_specs(n=args.cg_batch_n)
         vo = {}
         if args.cg_batch_n == 1024:
-            fr = [(sims[int(k)].observables.free_energy_fp64() - float(z[f"F1_{n}"])) / abs(float(z[f"F1_{n}"]))
-                  for n, k in enumerate(z["index"])]
+            # the first iteration of the sampled instances, against the oracle
+            # (instances of a batch are independent: a batch of the sampled
+            # ones computes what the full batch computes for them)
+            sample = [workloads.instance(specs[int(k)], dtype) for k in z["index"]]
+            BatchCG(sample).cg(max_iter=1, rtol=0.0)
+            fr = [(gl.observables.free_energy_fp64() - float(z[f"F1_{n}"])) / abs(float(z[f"F1_{n}"]))
+                  for n, gl in enumerate(sample)]
             vo = {"horizon": "1 iteration (exact line minimum along -grad F)", "instances": [int(k) for k in z["index"]],
                   "F_rel_max": float(np.max(np.abs(fr)))}
-        b.cg(max_iter=1, rtol=0.0)  # warm-up
+            del sample
+        # the timed run starts from the initial state, so ||grad F||_0 is the
+        # initial state's gradient norm
+        log(f"[c5] building {args.cg_batch_n} Config 5 instances ({name})")
+        sims = [workloads.instance(sp, dtype) for sp in specs]
+        nodes = sum(int(s.cfg.N) for s in sims)
+        b = BatchCG(sims)
+        mixed = dtype == np.float32
         t0 = time.perf_counter()
-        iters, conv, grel = b.cg(max_iter=args.cg_batch_iters, rtol=1e-8)
+        iters, conv, grel = b.cg(max_iter=args.cg_batch_iters, rtol=1e-8,
+                                 stall_iters=args.cg_batch_stall if mixed else 0, fp64_finish=mixed)
         t = time.perf_counter() - t0
         total = int(iters.sum())
         res[name] = {"instance_iters_per_s": total / t, "node_iters_per_s": float(np.sum(iters * (nodes / len(sims)))) / t,
                      "wall_s": t, "converged": int(conv.sum()), "iters_min": int(iters.min()),
                      "iters_max": int(iters.max()), "grad_rel_median": float(np.median(grel)), "vs_oracle": vo}
+        if mixed:
+            fin = b.fp64_iters > 0
+            res[name].update({"stall_iters": args.cg_batch_stall, "finished_in_fp64": int(fin.sum()),
+                              "converged_in_fp32": int((conv & ~fin).sum()),
+                              "fp32_iters_total": int((iters - b.fp64_iters).sum()),
+                              "fp64_finish_iters_total": int(b.fp64_iters.sum())})
         F[name] = np.array([s.observables.free_energy_fp64() for s in sims[:64]])
         log(f"[c5] {name}: {total / t:.4g} instance-iterations/s, {res[name]['node_iters_per_s']:.4g} "
             f"node-iterations/s, {int(conv.sum())} converged, median grad_rel {np.median(grel):.2e}; first step vs "
-            f"oracle F {vo.get('F_rel_max', float('nan')):.2e}")
+            f"oracle F {vo.get('F_rel_max', float('nan')):.2e}; wall {t:.1f} s"
+            + (f"; {res[name]['finished_in_fp64']} finished in FP64 after stalling" if mixed else ""))
         del sims, b
     res["fp32_vs_fp64_F_rel_max"] = float(np.max(np.abs(F["fp32"] - F["fp64"]) / np.abs(F["fp64"])))
     res["fp32_vs_fp64_F_rel_median"] = float(np.median(np.abs(F["fp32"] - F["fp64"]) / np.abs(F["fp64"])))
@@ -583,7 +601,8 @@ def main():
     ap.add_argument("--batch-solo-steps", type=int, default=10)
     ap.add_argument("--skip-batch-cg", action="store_true")
     ap.add_argument("--cg-batch-n", type=int, default=1024)
-    ap.add_argument("--cg-batch-iters", type=int, default=300)
+    ap.add_argument("--cg-batch-iters", type=int, default=3000)
+    ap.add_argument("--cg-batch-stall", type=int, default=300)
     ap.add_argument("--calibrate", action="store_true",
                     help="run FP32 axpy dispatches of known size first (counter calibration)")
     ap.add_argument("--cpu-psi-sweeps", type=int, default=3)

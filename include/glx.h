@@ -202,7 +202,8 @@ int glx_batch_td_run_noalloc(int prec, glx_batch_instance *inst, int32_t n, int3
  * reference; the batched form of cg.py:452-551 / 238-322). kappa2 <= 0
  * means kappa = inf (psi only, ab may be NULL); all instances of one call
  * must agree on that. eps_map (may be NULL) is used by the gradient and by
- * the line-search coefficients. iters, converged and grad_rel are outputs. */
+ * the line-search coefficients. iters, converged, grad_rel, grad0, stalled
+ * and fp64_iters are outputs. */
 typedef struct glx_batch_cg_instance {
     glx_grid g;
     double kappa2;
@@ -215,6 +216,9 @@ typedef struct glx_batch_cg_instance {
     int32_t iters;
     int32_t converged;
     double grad_rel;
+    double grad0;        /* out: ||grad F||_0 of the stop rule */
+    int32_t stalled;     /* out: 1 = stopped by the stall rule, not converged */
+    int32_t fp64_iters;  /* out: iterations of the FP64 finish (included in iters) */
 } glx_batch_cg_instance;
 
 /* Minimise n instances by Polak-Ribiere CG with an exact polynomial line
@@ -223,9 +227,17 @@ typedef struct glx_batch_cg_instance {
  * (or 5) FP64 line-search coefficients, an FP64 2-D quartic minimisation on
  * the device, update) and stops on its own when ||grad F|| <= rtol *
  * ||grad F||_0 or after max_iters (a ragged batch). iters_per_launch
- * iterations run per kernel launch (0 = 50). */
+ * iterations run per kernel launch (0 = 50).
+ * stall_iters > 0 adds a stall rule: an instance whose relative gradient
+ * norm has not halved within stall_iters iterations stops (stalled = 1).
+ * fp64_finish (FP32-mixed batches with a stall rule only): the stalled
+ * instances are copied into FP64 and continue there from their iteration
+ * count, with a steepest-descent step and the FP32 run's ||grad F||_0, to
+ * rtol or max_iters in all; the FP64 result is rounded back into the FP32
+ * psi and ab (so an FP32 re-evaluation of the gradient sits at FP32
+ * rounding, not at rtol). */
 int glx_batch_cg_run(int prec, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,
-                     int32_t iters_per_launch);
+                     int32_t iters_per_launch, int32_t stall_iters, int32_t fp64_finish);
 
 /* One psi Jacobi solve to the reference's stop rule, on the device
  * (TD.__iterate_order_parameter_gpu, td.py:157-218: the sweep loop, its

@@ -53,7 +53,7 @@ class BatchCGInstance(ctypes.Structure):
     """glx_batch_cg_instance."""
     _fields_ = [("g", Grid), ("kappa2", c_double), ("eps", c_double), ("H", c_double), ("eps_map", c_void_p),
                 ("mt", c_void_p), ("psi", c_void_p), ("ab", c_void_p), ("iters", c_i32), ("converged", c_i32),
-                ("grad_rel", c_double)]
+                ("grad_rel", c_double), ("grad0", c_double), ("stalled", c_i32), ("fp64_iters", c_i32)]
 
 
 # name -> (argtypes); every function returns int
@@ -88,7 +88,7 @@ _PROTOS = {
     "glx_td_force_barrier_abort": [],
     "glx_batch_td_run": [c_int, ctypes.POINTER(BatchInstance), c_i32, c_i32, P_i32, c_i32, c_i32, c_i32, c_void_p],
     "glx_batch_td_run_noalloc": [c_int, ctypes.POINTER(BatchInstance), c_i32, c_i32, P_i32, c_i32, c_i32, c_void_p],
-    "glx_batch_cg_run": [c_int, ctypes.POINTER(BatchCGInstance), c_i32, c_i32, c_double, c_i32],
+    "glx_batch_cg_run": [c_int, ctypes.POINTER(BatchCGInstance), c_i32, c_i32, c_double, c_i32, c_i32, c_i32],
     "glx_td_solve_ms": [ctypes.POINTER(c_double), P_i32, ctypes.POINTER(c_double), P_i32],
     "glx_td_passes": [P_i32, P_i32, P_i32, P_i32, P_i32],
     "glx_stream_probe": [c_i32, c_i64, c_i32, c_i32, ctypes.POINTER(c_double)],

@@ -163,9 +163,21 @@ class BatchCG:
         self.prec = nat.prec_of(dt0)
         self.last_kernel_ms = 0.0
 
-    def cg(self, max_iter=3000, rtol=1e-8, iters_per_launch=50):
+    def cg(self, max_iter=3000, rtol=1e-8, iters_per_launch=50, stall_iters=0, fp64_finish=False):
         """Returns (iterations, converged flags, final relative gradient
-        norms) per instance."""
+        norms) per instance.
+
+        stall_iters > 0 stops an instance whose relative gradient norm has
+        not halved within that many iterations. fp64_finish (FP32 batches,
+        with a stall rule) continues the stalled instances in FP64 from
+        where they stopped, against the FP32 run's initial gradient norm,
+        and rounds their result back into the FP32 state. After the call,
+        self.stalled and self.fp64_iters hold the per-instance stall flags
+        and FP64-finish iteration counts."""
+        if fp64_finish and self.prec != nat.GLX_F32:
+            raise ValueError("the FP64 finish is for FP32 batches")
+        if fp64_finish and stall_iters < 1:
+            raise ValueError("the FP64 finish needs a stall rule (stall_iters >= 1)")
         n = len(self.solvers)
         arr = (BatchCGInstance * n)()
         for k, gl in enumerate(self.solvers):
@@ -183,7 +195,8 @@ class BatchCG:
             I.mt = nat.ptr(gl.mesh.material_tiling_h())
             I.psi = v.order_parameter_h().ptr
             I.ab = nat.ptr(v.vector_potential_h()) if p.solveA else None
-        nat.call("glx_batch_cg_run", self.prec, arr, n, int(max_iter), float(rtol), int(iters_per_launch))
+        nat.call("glx_batch_cg_run", self.prec, arr, n, int(max_iter), float(rtol), int(iters_per_launch),
+                 int(stall_iters), int(bool(fp64_finish)))
         kms = ctypes.c_float()
         nat.call("glx_td_last_kernel_ms", ctypes.byref(kms))
         self.last_kernel_ms = kms.value
@@ -194,4 +207,6 @@ class BatchCG:
         iters = np.array([arr[k].iters for k in range(n)])
         conv = np.array([bool(arr[k].converged) for k in range(n)])
         grel = np.array([arr[k].grad_rel for k in range(n)])
+        self.stalled = np.array([bool(arr[k].stalled) for k in range(n)])
+        self.fp64_iters = np.array([arr[k].fp64_iters for k in range(n)])
         return iters, conv, grel

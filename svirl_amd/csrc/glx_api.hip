@@ -342,8 +342,8 @@ int glx_batch_td_run_noalloc(int prec, glx_batch_instance *inst, int32_t n, int3
 }
 
 int glx_batch_cg_run(int prec, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,
-                     int32_t iters_per_launch) {
-    GLX_DISPATCH(prec, batch_cg_run(g_ctx.env, inst, n, max_iters, rtol, iters_per_launch));
+                     int32_t iters_per_launch, int32_t stall_iters, int32_t fp64_finish) {
+    GLX_DISPATCH(prec, batch_cg_run(g_ctx.env, inst, n, max_iters, rtol, iters_per_launch, stall_iters, fp64_finish));
 }
 
 int glx_td_force_barrier_abort(void) {

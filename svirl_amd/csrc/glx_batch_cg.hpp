@@ -30,9 +30,16 @@ template <typename R> struct CGInst {
     R *jacA[2], *dirA[2];     // A gradient and direction
 };
 
-// Per-instance state, 8 doubles: iteration, stop flag, beta_psi, beta_A,
-// initial gradient norm, last relative gradient norm, last alphas.
-enum { CG_IT = 0, CG_DONE = 1, CG_BPSI = 2, CG_BA = 3, CG_G0 = 4, CG_GREL = 5, CG_APSI = 6, CG_AA = 7 };
+// Per-instance state, CG_NSTATE doubles: iteration, stop flag (1
+// converged, 2 stalled), beta_psi, beta_A, initial gradient norm, last
+// relative gradient norm, last alphas, the iteration the run started at,
+// and the stall rule's mark and count. A run starts at iteration CG_IT0
+// (0, or where an earlier run stopped) with a steepest-descent step, and
+// takes CG_G0 as ||grad F||_0 when it is already set (> 0).
+enum {
+    CG_IT = 0, CG_DONE = 1, CG_BPSI = 2, CG_BA = 3, CG_G0 = 4, CG_GREL = 5, CG_APSI = 6, CG_AA = 7,
+    CG_IT0 = 8, CG_MARK = 9, CG_SINCE = 10, CG_NSTATE = 12
+};
 
 // Sum of L doubles per thread over the workgroup, into out (thread 0).
 template <int L> __device__ __forceinline__ void block_sums(double (&acc)[L], double *lds, double *out) {
@@ -51,9 +58,12 @@ template <int L> __device__ __forceinline__ void block_sums(double (&acc)[L], do
 template <typename R> struct BatchCGBound { static constexpr int MIN_WG = 1; };
 template <> struct BatchCGBound<float> { static constexpr int MIN_WG = 4; };
 
+// Stall rule (stall_iters > 0): an instance whose relative gradient norm
+// has not halved within stall_iters iterations stops with flag 2 (the
+// FP32-mixed form stalls at the resolution of its gradient).
 template <typename R, int L>
 __global__ void __launch_bounds__(256, BatchCGBound<R>::MIN_WG) k_batch_cg(const CGInst<R> *insts, double *state, int n, int iters,
-                                                  int max_iters, double rtol) {
+                                                  int max_iters, double rtol, int stall_iters) {
     __shared__ double lds[8];
     __shared__ double red[L];
     __shared__ double bc[4];   // beta_psi, beta_A, alpha_psi, alpha_A
@@ -64,7 +74,7 @@ __global__ void __launch_bounds__(256, BatchCGBound<R>::MIN_WG) k_batch_cg(const
         const GeoR<R> q(g);
         const R dxdy = q.dx * q.dy, s2 = R(2) * q.dx * q.dy;
         const bool withA = L == 17;
-        double *sv = state + 8 * k;
+        double *sv = state + CG_NSTATE * k;
         for (int it = 0; it < iters; ++it) {
             if (threadIdx.x == 0) {
                 go = (sv[CG_DONE] == 0.0 && sv[CG_IT] < (double)max_iters) ? 1 : 0;
@@ -119,14 +129,26 @@ __global__ void __launch_bounds__(256, BatchCGBound<R>::MIN_WG) k_batch_cg(const
             block_sums<5>(acc, lds, red);
             if (threadIdx.x == 0) {
                 const double gn = sqrt(red[0]);
-                if (itn == 0) sv[CG_G0] = gn;
+                const int it0 = (int)sv[CG_IT0];
+                if (itn == it0 && !(sv[CG_G0] > 0.0)) sv[CG_G0] = gn;
                 const double g0 = sv[CG_G0];
-                sv[CG_GREL] = g0 > 0.0 ? gn / g0 : 0.0;
+                const double grel = g0 > 0.0 ? gn / g0 : 0.0;
+                sv[CG_GREL] = grel;
                 if (!(gn > rtol * g0)) {
                     sv[CG_DONE] = 1.0;
                     go = 0;
+                } else if (stall_iters > 0 && itn > it0 && !(grel < 0.5 * sv[CG_MARK]) &&
+                           sv[CG_SINCE] + 1.0 >= (double)stall_iters) {
+                    sv[CG_DONE] = 2.0;
+                    go = 0;
                 } else {
-                    if (itn > 0) {
+                    if (itn == it0 || grel < 0.5 * sv[CG_MARK]) {
+                        sv[CG_MARK] = grel;
+                        sv[CG_SINCE] = 0.0;
+                    } else {
+                        sv[CG_SINCE] += 1.0;
+                    }
+                    if (itn > it0) {
                         sv[CG_BPSI] = double(R(fmax(red[1] / red[2], 0.0)));
                         sv[CG_BA] = withA ? double(R(fmax(red[3] / red[4], 0.0))) : 0.0;
                     }

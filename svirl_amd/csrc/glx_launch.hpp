@@ -72,7 +72,7 @@ template <typename R> struct L {
                             int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight,
                             const Workspace &ws);
     static int batch_cg_run(const Env &e, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,
-                            int32_t iters_per_launch);
+                            int32_t iters_per_launch, int32_t stall_iters, int32_t fp64_finish);
     static int td_max_grid(const Env &e, int32_t wg_size, int32_t *max_wg);
     static int phase_lock(const Env &e, void *psi, int32_t n, const int32_t *ns);
     static int magnetic_field(const Env &e, glx_grid g, const void *abei, const void *ab, void *B);

@@ -584,7 +584,7 @@ int L<R>::batch_td_run(const Env &e, glx_batch_instance *inst, int32_t n, int32_
 
 template <typename R, int LL>
 static int batch_cg_form(const Env &e, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,
-                         int32_t per_launch) {
+                         int32_t per_launch, int32_t stall_iters, const double *init_state) {
     const bool withA = LL == 17;
     // scratch per instance: jac[2], dir[2] (N complex each) and, with A, jacA[2], dirA[2] (Nab reals each)
     std::vector<size_t> off(n + 1, 0);
@@ -594,7 +594,7 @@ static int batch_cg_form(const Env &e, glx_batch_cg_instance *inst, int32_t n, i
         const size_t N = (size_t)g.Nx * g.Ny, Nab = (size_t)(g.Nx - 1) * g.Ny + (size_t)g.Nx * (g.Ny - 1);
         off[i + 1] = off[i] + 4 * al(N * sizeof(cplx<R>)) + (withA ? 4 * al(Nab * sizeof(R)) : 0);
     }
-    const size_t s_inst = al(sizeof(CGInst<R>) * n), s_state = al(sizeof(double) * 8 * n);
+    const size_t s_inst = al(sizeof(CGInst<R>) * n), s_state = al(sizeof(double) * CG_NSTATE * n);
     char *blk = nullptr;
     hipError_t st = hipMalloc((void **)&blk, s_inst + s_state + off[n]);
     if (st != hipSuccess) return check_hip(st, "hipMalloc(batch cg)");
@@ -623,25 +623,32 @@ static int batch_cg_form(const Env &e, glx_batch_cg_instance *inst, int32_t n, i
         for (int k = 0; k < 2; ++k) { c.dirA[k] = withA ? (R *)w : nullptr; if (withA) w += al(Nab * sizeof(R)); }
     }
     st = hipMemcpyAsync(d_inst, h.data(), sizeof(CGInst<R>) * n, hipMemcpyHostToDevice, e.stream);
-    if (st == hipSuccess) st = hipMemsetAsync(d_state, 0, sizeof(double) * 8 * n + off[n], e.stream);
+    if (st == hipSuccess) st = hipMemsetAsync(d_state, 0, s_state + off[n], e.stream);
+    if (st == hipSuccess && init_state)
+        st = hipMemcpyAsync(d_state, init_state, sizeof(double) * CG_NSTATE * n, hipMemcpyHostToDevice, e.stream);
     if (st != hipSuccess) return check_hip(st, "glx_batch_cg_run upload");
     int per_cu = 0;
     st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_cg<R, LL>, 256, 0);
     if (st != hipSuccess) return check_hip(st, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     const int grid = std::max(1, std::min(n, per_cu * e.num_cus));
     const int chunk = per_launch > 0 ? per_launch : 50;
-    std::vector<double> hs(8 * (size_t)n);
+    int it_min = max_iters;  // fewest iterations any instance starts with
+    for (int i = 0; i < n; ++i) it_min = std::min(it_min, init_state ? (int)init_state[CG_NSTATE * i + CG_IT] : 0);
+    std::vector<double> hs(CG_NSTATE * (size_t)n);
+    if (init_state) std::copy(init_state, init_state + CG_NSTATE * (size_t)n, hs.begin());
+    else std::fill(hs.begin(), hs.end(), 0.0);
     st = hipEventRecord(e.kev0, e.stream);
     if (st != hipSuccess) return check_hip(st, "hipEventRecord");
-    for (int done = 0; done < max_iters; done += chunk) {
+    for (int done = it_min; done < max_iters; done += chunk) {
         hipLaunchKernelGGL((k_batch_cg<R, LL>), dim3(grid), dim3(256), 0, e.stream, d_inst, d_state, n,
-                           std::min(chunk, max_iters - done), max_iters, rtol);
+                           std::min(chunk, max_iters - done), max_iters, rtol, stall_iters);
         GLX_LAUNCH_CHECK("k_batch_cg");
-        st = hipMemcpyAsync(hs.data(), d_state, sizeof(double) * 8 * n, hipMemcpyDeviceToHost, e.stream);
+        st = hipMemcpyAsync(hs.data(), d_state, sizeof(double) * CG_NSTATE * n, hipMemcpyDeviceToHost, e.stream);
         if (st == hipSuccess) st = hipStreamSynchronize(e.stream);
         if (st != hipSuccess) return check_hip(st, "glx_batch_cg_run readback");
         bool all = true;
-        for (int i = 0; i < n && all; ++i) all = hs[8 * i + CG_DONE] != 0.0 || hs[8 * i + CG_IT] >= max_iters;
+        for (int i = 0; i < n && all; ++i)
+            all = hs[CG_NSTATE * i + CG_DONE] != 0.0 || hs[CG_NSTATE * i + CG_IT] >= max_iters;
         if (all) break;
     }
     st = hipEventRecord(e.kev1, e.stream);
@@ -650,17 +657,113 @@ static int batch_cg_form(const Env &e, glx_batch_cg_instance *inst, int32_t n, i
     float kms = 0.f;
     if (hipEventElapsedTime(&kms, e.kev0, e.kev1) == hipSuccess && e.last_kernel_ms) *e.last_kernel_ms = kms;
     for (int i = 0; i < n; ++i) {
-        inst[i].iters = (int32_t)hs[8 * i + CG_IT];
-        inst[i].converged = hs[8 * i + CG_DONE] != 0.0 ? 1 : 0;
-        inst[i].grad_rel = hs[8 * i + CG_GREL];
+        const double *sv = hs.data() + CG_NSTATE * i;
+        inst[i].iters = (int32_t)sv[CG_IT];
+        inst[i].converged = sv[CG_DONE] == 1.0 ? 1 : 0;
+        inst[i].stalled = sv[CG_DONE] == 2.0 ? 1 : 0;
+        inst[i].grad_rel = sv[CG_GREL];
+        inst[i].grad0 = sv[CG_G0];
+        inst[i].fp64_iters = 0;
     }
     return GLX_OK;
 }
 
+template <typename D, typename S> __global__ void __launch_bounds__(256) k_convert(D *dst, const S *src, long n) {
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
+        dst[k] = D(src[k]);
+}
+
+template <typename D, typename S> static int convert(const Env &e, D *dst, const S *src, long n) {
+    if (n <= 0) return GLX_OK;
+    hipLaunchKernelGGL((k_convert<D, S>), dim3(grid_blocks(n)), dim3(256), 0, e.stream, dst, src, n);
+    GLX_LAUNCH_CHECK("k_convert");
+    return GLX_OK;
+}
+
+// FP64 finish of the instances an FP32-mixed run left stalled: each is
+// copied into FP64 (psi, A and the epsilon map), continues from its
+// iteration count with a steepest-descent step and the FP32 run's
+// ||grad F||_0, to rtol or max_iters in all, and its FP64 result is
+// rounded back into the FP32 state.
+template <int LL>
+static int batch_cg_finish64(const Env &e, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,
+                             int32_t per_launch) {
+    const bool withA = LL == 17;
+    std::vector<int> idx;
+    for (int i = 0; i < n; ++i)
+        if (inst[i].stalled && inst[i].iters < max_iters) idx.push_back(i);
+    if (idx.empty()) return GLX_OK;
+    const int m = (int)idx.size();
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    auto sizes = [](const glx_grid &g, size_t &N, size_t &Nab) {
+        N = (size_t)g.Nx * g.Ny;
+        Nab = (size_t)(g.Nx - 1) * g.Ny + (size_t)g.Nx * (g.Ny - 1);
+    };
+    std::vector<size_t> off(m + 1, 0);
+    for (int k = 0; k < m; ++k) {
+        size_t N, Nab;
+        sizes(inst[idx[k]].g, N, Nab);
+        off[k + 1] = off[k] + al(N * 16) + (withA ? al(Nab * 8) : 0) + (inst[idx[k]].eps_map ? al(N * 8) : 0);
+    }
+    char *blk = nullptr;
+    hipError_t st = hipMalloc((void **)&blk, off[m]);
+    if (st != hipSuccess) return check_hip(st, "hipMalloc(fp64 finish)");
+    struct Free { char *p; ~Free() { if (p) (void)hipFree(p); } } guard{blk};
+    std::vector<glx_batch_cg_instance> sub(m);
+    std::vector<double> init(CG_NSTATE * (size_t)m, 0.0);
+    for (int k = 0; k < m; ++k) {
+        const glx_batch_cg_instance &I = inst[idx[k]];
+        size_t N, Nab;
+        sizes(I.g, N, Nab);
+        char *w = blk + off[k];
+        glx_batch_cg_instance &J = sub[k];
+        J = I;
+        J.psi = w;
+        w += al(N * 16);
+        int rc = convert(e, (double *)J.psi, (const float *)I.psi, 2 * (long)N);
+        if (!rc && withA) {
+            J.ab = w;
+            w += al(Nab * 8);
+            rc = convert(e, (double *)J.ab, (const float *)I.ab, (long)Nab);
+        }
+        if (!rc && I.eps_map) {
+            J.eps_map = w;
+            rc = convert(e, (double *)w, (const float *)I.eps_map, (long)N);
+        }
+        if (rc) return rc;
+        double *sv = init.data() + CG_NSTATE * k;
+        sv[CG_IT] = sv[CG_IT0] = I.iters;
+        sv[CG_G0] = I.grad0;
+    }
+    const float kms32 = e.last_kernel_ms ? *e.last_kernel_ms : 0.f;
+    int rc = batch_cg_form<double, LL>(e, sub.data(), m, max_iters, rtol, per_launch, 0, init.data());
+    if (rc) return rc;
+    if (e.last_kernel_ms) *e.last_kernel_ms += kms32;
+    for (int k = 0; k < m; ++k) {
+        glx_batch_cg_instance &I = inst[idx[k]];
+        const glx_batch_cg_instance &J = sub[k];
+        size_t N, Nab;
+        sizes(I.g, N, Nab);
+        rc = convert(e, (float *)I.psi, (const double *)J.psi, 2 * (long)N);
+        if (!rc && withA) rc = convert(e, (float *)I.ab, (const double *)J.ab, (long)Nab);
+        if (rc) return rc;
+        I.fp64_iters = J.iters - I.iters;
+        I.iters = J.iters;
+        I.converged = J.converged;
+        I.stalled = 0;
+        I.grad_rel = J.grad_rel;
+    }
+    st = hipStreamSynchronize(e.stream);  // before the FP64 block is freed
+    return st == hipSuccess ? GLX_OK : check_hip(st, "glx_batch_cg_run fp64 finish");
+}
+
 template <typename R>
 int L<R>::batch_cg_run(const Env &e, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,
-                       int32_t iters_per_launch) {
-    if (!inst || n < 1 || max_iters < 0 || !(rtol >= 0.0)) return set_error(GLX_E_ARG, "glx_batch_cg_run: invalid arguments");
+                       int32_t iters_per_launch, int32_t stall_iters, int32_t fp64_finish) {
+    if (!inst || n < 1 || max_iters < 0 || !(rtol >= 0.0) || stall_iters < 0)
+        return set_error(GLX_E_ARG, "glx_batch_cg_run: invalid arguments");
+    if (fp64_finish && (sizeof(R) != sizeof(float) || stall_iters < 1))
+        return set_error(GLX_E_ARG, "glx_batch_cg_run: the FP64 finish needs an FP32-mixed batch and a stall rule");
     const bool withA = inst[0].kappa2 > 0.0;
     for (int i = 0; i < n; ++i) {
         const glx_batch_cg_instance &I = inst[i];
@@ -669,9 +772,16 @@ int L<R>::batch_cg_run(const Env &e, glx_batch_cg_instance *inst, int32_t n, int
             return set_error(GLX_E_ARG, "glx_batch_cg_run: instances must all have kappa < inf or all kappa = inf");
         if (withA && !I.ab) return set_error(GLX_E_ARG, "glx_batch_cg_run: instance %d: NULL vector potential", i);
     }
+    for (int i = 0; i < n; ++i) {
+        inst[i].iters = inst[i].converged = inst[i].stalled = inst[i].fp64_iters = 0;
+        inst[i].grad_rel = inst[i].grad0 = 0.0;
+    }
     if (max_iters == 0) return GLX_OK;
-    if (withA) return batch_cg_form<R, 17>(e, inst, n, max_iters, rtol, iters_per_launch);
-    return batch_cg_form<R, 5>(e, inst, n, max_iters, rtol, iters_per_launch);
+    int rc = withA ? batch_cg_form<R, 17>(e, inst, n, max_iters, rtol, iters_per_launch, stall_iters, nullptr)
+                   : batch_cg_form<R, 5>(e, inst, n, max_iters, rtol, iters_per_launch, stall_iters, nullptr);
+    if (rc || !fp64_finish) return rc;
+    return withA ? batch_cg_finish64<17>(e, inst, n, max_iters, rtol, iters_per_launch)
+                 : batch_cg_finish64<5>(e, inst, n, max_iters, rtol, iters_per_launch);
 }
 
 template <typename R> int L<R>::phase_lock(const Env &e, void *psi, int32_t n, const int32_t *ns) {

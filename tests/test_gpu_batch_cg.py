@@ -138,3 +138,42 @@ def test_fp32_mixed_reaches_fp64_energy():
     for a, b in zip(s32, s64):
         fa, fb = a.observables.free_energy_fp64(), b.observables.free_energy_fp64()
         assert abs(fa - fb) <= 1e-5 * abs(fb)
+
+
+def test_fp32_mixed_stalls_then_fp64_finish_reaches_rtol():
+    """At rtol = 1e-6 the FP32-mixed gradient stalls above the stop test;
+    the stall rule stops those instances, and the FP64 finish continues each
+    from where it stopped (its iteration count and the FP32 run's initial
+    gradient norm) until the FP64 gradient passes the test. The result ends
+    at the FP64-only run's free energy, to FP32 rounding of the state."""
+    from svirl_amd.batch import BatchCG
+    s32 = [make(k, *c, dtype=np.float32, init=1.0) for k, c in enumerate(CASES)]
+    plain = [make(k, *c, dtype=np.float32, init=1.0) for k, c in enumerate(CASES)]
+    s64 = [make(k, *c, init=1.0) for k, c in enumerate(CASES)]
+    b0 = BatchCG(plain)
+    it0, cv0, _ = b0.cg(max_iter=3000, rtol=1e-6, stall_iters=50)
+    assert b0.stalled.any() and not (cv0 & b0.stalled).any(), (it0, cv0, b0.stalled)
+    b = BatchCG(s32)
+    iters, conv, grel = b.cg(max_iter=3000, rtol=1e-6, stall_iters=50, fp64_finish=True)
+    assert conv.all(), (iters, grel, b.fp64_iters)
+    assert np.all(grel <= 1e-6)
+    # the FP32 phase is the plain stalled run, the finish adds its own iterations
+    fin = b.fp64_iters > 0
+    assert np.array_equal(fin, b0.stalled), (b.fp64_iters, b0.stalled)
+    assert np.array_equal(iters[fin], it0[fin] + b.fp64_iters[fin]), (iters, it0, b.fp64_iters)
+    assert np.array_equal(iters[~fin], it0[~fin])
+    it64, cv64, _ = BatchCG(s64).cg(max_iter=3000, rtol=1e-6)
+    assert cv64.all()
+    for a, c in zip(s32, s64):
+        fa, fc = a.observables.free_energy_fp64(), c.observables.free_energy_fp64()
+        assert abs(fa - fc) <= 1e-6 * abs(fc), (fa, fc)
+
+
+def test_fp64_finish_argument_checks():
+    from svirl_amd.batch import BatchCG
+    s64 = [make(0, *CASES[0])]
+    with pytest.raises(ValueError):
+        BatchCG(s64).cg(max_iter=10, rtol=1e-8, stall_iters=5, fp64_finish=True)
+    s32 = [make(0, *CASES[0], dtype=np.float32)]
+    with pytest.raises(ValueError):
+        BatchCG(s32).cg(max_iter=10, rtol=1e-8, fp64_finish=True)
