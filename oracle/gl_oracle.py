@@ -1013,6 +1013,26 @@ def triangulate(g, cells, vs, psi, a_ai, b_bi):
     return np.zeros((0, 3))
 
 
+def fixed_vortices_phase(g, vpi):
+    """svirl/vars/fixed_vortices.py:130-137 (_ab_phase of the irregular
+    vector potential, :155-158): the phase at node (i, j) is the line
+    integral of vpi from node (0, 0), first up the column i = 0 along the
+    b links (dy times the running sum), then along row j over the a links
+    (dx times the running sum), the two added. Flat node layout."""
+    a = g.a2(vpi)                                                   # a[j, i]
+    b = g.b2(vpi)                                                   # b[j, i]
+    ca = g.dx * np.concatenate([np.zeros((g.Nya, 1)), a], axis=1).cumsum(axis=1)
+    cb = g.dy * np.concatenate([np.zeros((1, g.Nx)), b], axis=0).cumsum(axis=0)
+    return (np.repeat(cb[:, :1], g.Nx, axis=1) + ca).ravel()
+
+
+def release_fixed_vortices(g, psi, vpi):
+    """svirl/vars/fixed_vortices.py:63-77 (fixed_vortices_release): psi times
+    exp(-i phase) with the phase of the irregular vector potential; the
+    caller then drops vpi and the phase-locked nodes."""
+    return psi * np.exp(-1.0j * fixed_vortices_phase(g, vpi))
+
+
 def vortex_detector(g, psi, ab, H, abi=None):
     """vortex_detector.py:31-156: winding uses vars._vp only; the
     triangulation uses vp + vpi (abi = irregular vector potential)."""

@@ -45,46 +45,59 @@ class FixedVortices:
         self.vars._psi.need_htod_sync()
         self.vars._psi.sync()
 
+    def fixed_vortices_release(self):
+        """Frees the pinned vortices (fixed_vortices.py:63-77): psi takes the
+        phase of the irregular vector potential, exp(-i phase) with phase its
+        line integral from node (0, 0) (_ab_phase), so that the windings it
+        imposed become windings of psi itself; then the fixed vortices and
+        the phase lock are cleared."""
+        self.vars._psi.sync()
+        psi = self.vars._psi.get_h()
+        psi *= np.exp(-1.0j * self.fixed_vortices_phase)
+        self.vars._psi.need_htod_sync()
+        self.vars._psi.sync()
+        self.fixed_vortices = None
+        self.phase_lock_radius = None
+
     def _vortices_format(self, vortices):
-        """fixed_vortices.py:81-112."""
+        """Three equal-length arrays (x, y, vorticity) in the solver dtype
+        from (x, y), (x, y, vorticity) or a dict with 'x', 'y' and an
+        optional 'vorticity' (fixed_vortices.py:81-112). None stands for no
+        entries; a single value is repeated to the others' length; the
+        vorticity is 1 when only positions are given; entries with a NaN in
+        any of the three are dropped. (For a dict the reference reads the
+        vorticity as key 2, which raises; here it is key 'vorticity'.)"""
         if vortices is None:
-            vortices = [[], []]
-        assert isinstance(vortices, (list, tuple, dict))
-        assert len(vortices) in [2, 3]
-        if isinstance(vortices, (list, tuple)):
-            vx, vy = vortices[0], vortices[1]
-            vv = vortices[2] if len(vortices) == 3 else []
+            vortices = ([], [])
+        assert isinstance(vortices, (list, tuple, dict)) and len(vortices) in (2, 3)
+        if isinstance(vortices, dict):
+            parts = (vortices['x'], vortices['y'], vortices.get('vorticity'))
         else:
-            vx, vy = vortices['x'], vortices['y']
-            vv = vortices['vorticity'] if 'vorticity' in vortices else []
-        vx = np.array([] if vx is None else vx).flatten()
-        vy = np.array([] if vy is None else vy).flatten()
-        vv = np.array([] if vv is None else vv).flatten()
-        vN = max(vx.size, vy.size, vv.size)
-        if vx.size > 0 and vv.size == 0:
-            vv = np.array([1])
-        assert vx.size in [1, vN] and vy.size in [1, vN] and vv.size in [1, vN]
-        fvx, fvy, fvv = [], [], []
-        x = y = v = np.nan
-        for i in range(vN):
-            if i < vx.size: x = vx[i]
-            if i < vy.size: y = vy[i]
-            if i < vv.size: v = vv[i]
-            if np.isnan(x) or np.isnan(y) or np.isnan(v):
-                continue
-            fvx.append(x); fvy.append(y); fvv.append(v)
+            parts = tuple(vortices) + (None,) * (3 - len(vortices))
+        x, y, v = (np.asarray([] if q is None else q).ravel() for q in parts)
+        n = max(x.size, y.size, v.size)
+        if x.size and not v.size:
+            v = np.ones(1, dtype=int)
+        assert all(q.size in (1, n) for q in (x, y, v))
         dt = self.cfg.dtype
-        return np.array(fvx).astype(dt), np.array(fvy).astype(dt), np.array(fvv).astype(dt)
+        if n == 0:
+            return tuple(np.zeros(0, dtype=dt) for _ in range(3))
+        x, y, v = (np.broadcast_to(q, (n,)) for q in (x, y, v))
+        keep = ~(np.isnan(x) | np.isnan(y) | np.isnan(v))
+        return tuple(q[keep].astype(dt) for q in (x, y, v))
 
     def _fixed_vortices_correct(self):
+        """Rounds the vorticities to integers and, by the chosen correction,
+        moves each position to the nearest cell centre (where B lives) or
+        grid vertex (where psi lives) (fixed_vortices.py:115-128)."""
         cfg = self.cfg
         self.fixed_vortices_vorticity = np.round(self.fixed_vortices_vorticity)
-        if self.fixed_vortices_correction == 'cell centers':
-            self.fixed_vortices_x = cfg.dx * (np.round(self.fixed_vortices_x / cfg.dx + 0.5) - 0.5)
-            self.fixed_vortices_y = cfg.dy * (np.round(self.fixed_vortices_y / cfg.dy + 0.5) - 0.5)
-        elif self.fixed_vortices_correction == 'vertices':
-            self.fixed_vortices_x = cfg.dx * np.round(self.fixed_vortices_x / cfg.dx)
-            self.fixed_vortices_y = cfg.dy * np.round(self.fixed_vortices_y / cfg.dy)
+        offset = {'cell centers': 0.5, 'vertices': 0.0}.get(self.fixed_vortices_correction)
+        if offset is not None:
+            def snap(u, h):
+                return h * (np.round(u / h + offset) - offset)
+            self.fixed_vortices_x = snap(self.fixed_vortices_x, cfg.dx)
+            self.fixed_vortices_y = snap(self.fixed_vortices_y, cfg.dy)
 
     def _ab_phase(self, a, b):
         cfg = self.cfg

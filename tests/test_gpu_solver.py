@@ -645,3 +645,77 @@ def test_cg_device_loop_stop_leaves_state():
     for k in sa:
         assert np.array_equal(sa[k], sb[k]), k
     assert np.array_equal(a.vars.order_parameter, b.vars.order_parameter)
+
+
+def test_fixed_vortices_release_matches_oracle():
+    """fixed_vortices_release (fixed_vortices.py:63-77) after TD steps with
+    pinned vortices: psi takes exp(-i phase) of the irregular vector
+    potential as it stands after the steps, vpi and the phase lock are
+    dropped, and TD then continues as the oracle does without them."""
+    gl = GLSolver(Nx=131, Ny=70, dx=0.5, dy=0.5, order_parameter='random', random_level=0.3, random_seed=7,
+                  gl_parameter=2.0, normal_conductivity=20.0, homogeneous_external_field=0.05,
+                  fixed_vortices=([12.3, 40.1, 51.7], [10.2, 20.6, 27.9], [1, 1, -1]), phase_lock_radius=0.7,
+                  dtype=np.float64)
+    st = oracle_of(gl)
+    gl.solve.td(dt=0.1, Nt=4)
+    st.td(dt=0.1, Nt=4)
+    assert rel(_vpi_of(gl), st.vpi) < 1e-10
+    fv = gl.params.fixed_vortices
+    fv.fixed_vortices_release()
+    psi_rel = O.release_fixed_vortices(st.g, st.psi, st.vpi)
+    assert rel(gl_psi(gl), psi_rel) < 1e-10
+    assert fv._vpi is None and fv._phase_lock_ns is None and fv.fixed_vortices_x.size == 0
+    st.psi, st.vpi, st.lock_ns = psi_rel, None, None
+    gl.solve.td(dt=0.1, Nt=3)
+    st.td(dt=0.1, Nt=3)
+    assert gl.solve._td.sweeps[-3:] == st.sweeps[-3:]
+    assert rel(gl_psi(gl), st.psi) < 1e-10 and rel(gl_ab(gl), st.ab) < 1e-10
+
+
+def test_startup_get_function_runs_the_reference_kernels():
+    """gl.par.get_function(name) (startup.py:74-82) returns a callable with
+    the reference kernel's argument order: elementwise kernels against
+    numpy, the TD sweep and Jacobian against the oracle, the field against
+    the observables; launch-shape-dependent kernels and unknown names raise."""
+    from svirl_amd._native import DeviceArray
+    gl = GLSolver(Nx=33, Ny=29, dx=0.5, dy=0.5, order_parameter='random', random_seed=3, gl_parameter=5.0,
+                  normal_conductivity=100.0, homogeneous_external_field=0.15, dtype=np.float64)
+    par = gl.par
+    gl.vars._psi.sync_d()
+    gl.vars._vp.sync_d()
+    rng = np.random.default_rng(1)
+    x, y = rng.normal(size=50), rng.normal(size=50)
+    dx_, dy_, dz = DeviceArray.from_host(x), DeviceArray.from_host(y), DeviceArray.zeros(50, np.float64)
+    par.get_function("axpy_r")(dx_, dy_, dz, np.float64(0.7), np.int32(50), block=(128, 1, 1), grid=(1, 1))
+    assert np.array_equal(dz.get(), 0.7 * x + y)
+    al = DeviceArray.from_host(np.array([1.5]))
+    par.get_function("axmy_c")(dx_, dy_, dz, al, np.int32(25))
+    assert np.array_equal(dz.get(), 1.5 * x - y)
+    par.get_function("xpy_r")(dx_, dy_, np.int32(50))
+    assert np.array_equal(dx_.get(), x + y)
+    par.get_function("xmy_r")(dx_, dy_, np.int32(50))
+    assert np.array_equal(dx_.get(), (x + y) - y)
+    # magnetic field, as the observable computes it
+    c = gl.cfg
+    B = DeviceArray.zeros(int(c.Nxc) * int(c.Nyc), np.float64)
+    par.get_function("magnetic_field")(None, gl.vars.vector_potential_h(), B)
+    assert np.array_equal(np.reshape(B.get(), (c.Nyc, c.Nxc)).T, gl.observables.magnetic_field)
+    # one psi Jacobi sweep and the psi Jacobian, against the oracle
+    st = oracle_of(gl)
+    g = st.g
+    rhs = DeviceArray.from_host(st.psi.copy())
+    cur = DeviceArray.from_host(st.psi.copy())
+    nxt = DeviceArray.zeros(g.N, np.complex128)
+    r2 = DeviceArray.zeros(1, np.int32)
+    par.get_function("iterate_order_parameter_jacobi_step")(
+        np.float64(0.1), np.float64(st.eps_scalar), None, None, gl.vars.vector_potential_h(), rhs, cur, nxt,
+        np.float64(0.0), np.uint32(0), np.uint32(1), np.float64(1e-6), r2)
+    ref_next, _, _ = O.psi_jacobi_sweep(g, 0.1, st.eps_scalar, None, None, st.ab, st.psi.copy(), st.psi, 0.0, 0, 1, 1e-6)
+    assert rel(nxt.get(), ref_next) < 1e-12
+    G = DeviceArray.zeros(g.N, np.complex128)
+    par.get_function("free_energy_jacobian_psi")(np.float64(25.0), np.float64(st.eps_scalar), None, np.float64(0.15),
+                                                 None, gl.vars.order_parameter_h(), None, gl.vars.vector_potential_h(), G)
+    assert rel(G.get(), O.jacobian_psi(g, 25.0, st.eps_scalar, None, 0.15, None, st.psi, None, st.ab)) < 1e-12
+    for name in ("sum", "free_energy_conjgrad_coef", "no_such_kernel"):
+        with pytest.raises(ValueError):
+            par.get_function(name)
