@@ -284,13 +284,17 @@ struct Team {
 
 // Call from every thread of every workgroup of the team the same number of
 // times with the same `epoch` (barriers the team has passed; incremented
-// here). `v` is the workgroup's value (thread 0's is used): three 10-bit
-// fields (bits 0-9, 10-19, 20-29). On success returns true and sets `vmax`
-// (all threads) to the
-// field-wise max over the team; returns false when a poll timed out anywhere
-// in the grid. A one-workgroup team synchronises in the workgroup only.
+// here). `v` is the workgroup's value (thread 0's is used): NF fields of
+// 32 / NF bits, at most 10 (three 10-bit fields at bits 0-9, 10-19, 20-29;
+// four 8-bit fields). On success returns true and sets `vmax` (all threads)
+// to the field-wise max over the team; returns false when a poll timed out
+// anywhere in the grid. A one-workgroup team synchronises in the workgroup
+// only.
+template <int NF = 3>
 __device__ __forceinline__ bool team_sync_max(const GridSync &gs, const Team &tm, unsigned long long &epoch, int v,
                                               int &vmax) {
+    constexpr int FB = NF >= 4 ? 8 : 10;
+    constexpr unsigned long long FM = (1ull << FB) - 1ull;
     __shared__ int s_red[16];
     __shared__ int s_ok, s_max;
     ++epoch;
@@ -311,17 +315,17 @@ __device__ __forceinline__ bool team_sync_max(const GridSync &gs, const Team &tm
     }
     if (local == 0) {
         const unsigned long long t0 = wall_clock64();
-        int ok = 1, m0 = 0, m1 = 0, m2 = 0;
+        int ok = 1, mf[NF];
         for (;;) {
             int all = 1;
-            m0 = m1 = m2 = 0;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) mf[f] = 0;
             for (int k = threadIdx.x; k < tm.nb; k += blockDim.x) {
                 const unsigned long long a = ld_relaxed(gs.arrive + tm.b0 + k);
                 if ((a >> 32) != epoch) all = 0;
                 else {
-                    m0 = max(m0, (int)(unsigned)(a & 0x3ffull));
-                    m1 = max(m1, (int)(unsigned)((a >> 10) & 0x3ffull));
-                    m2 = max(m2, (int)(unsigned)((a >> 20) & 0x3ffull));
+#pragma unroll
+                    for (int f = 0; f < NF; ++f) mf[f] = max(mf[f], (int)(unsigned)((a >> (FB * f)) & FM));
                 }
             }
             int abort = 0;
@@ -331,10 +335,10 @@ __device__ __forceinline__ bool team_sync_max(const GridSync &gs, const Team &tm
             __builtin_amdgcn_s_sleep(1);
         }
         if (ok) {  // uniform over the workgroup
-            m0 = block_max_int(m0, s_red);
-            m1 = block_max_int(m1, s_red);
-            m2 = block_max_int(m2, s_red);
-            if (threadIdx.x == 0) s_max = (m2 << 20) | (m1 << 10) | m0;
+            int packed = 0;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) packed |= block_max_int(mf[f], s_red) << (FB * f);
+            if (threadIdx.x == 0) s_max = packed;
             __syncthreads();
             if (threadIdx.x < tm.nrel) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");

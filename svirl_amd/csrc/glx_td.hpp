@@ -97,8 +97,11 @@ template <> struct Pipe<double, false> { static constexpr int ROWS = 2, ROWS2 = 
 template <> struct Pipe<float, true> { static constexpr int ROWS = 2, ROWS2 = 1, AROWS2 = 1, AROWS3 = 1; };
 template <> struct Pipe<double, true> { static constexpr int ROWS = 1, ROWS2 = 1, AROWS2 = 1, AROWS3 = 1; };
 
-// Sweeps per fused psi pass in the persistent single-instance solve.
+// Sweeps per fused psi pass in the persistent single-instance solve (FP32:
+// four, psi_rowsK<4> with two-row batches, 181 VGPRs, two workgroups per
+// CU; FP64 keeps three, psi_rows3).
 template <typename R> struct Fuse { static constexpr int PSI = 3; };
+template <> struct Fuse<float> { static constexpr int PSI = 4; };
 
 // Per-lane column context of a strip: c may lie outside the grid (halo lanes
 // at the edges); cc and ca are c clamped into the node and link ranges, so
@@ -580,6 +583,181 @@ __device__ __forceinline__ void psi_rows3(const PsiSweepArgs<R> &A, int s, int j
     };
 
     const int t1 = jb + 2;  // rows t in [t0, t1)
+    PsiRow<R> P0[B], P1[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) P0[k] = psi_load_row(A, X, mtp, emp, min(t0 + k, t1 - 1));
+    for (int t = t0; t < t1; t += 2 * B) {
+#pragma unroll
+        for (int k = 0; k < B; ++k) P1[k] = psi_load_row(A, X, mtp, emp, min(t + B + k, t1 - 1));
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            if (t + k < t1) row(P0[k], t + k);
+#pragma unroll
+        for (int k = 0; k < B; ++k) P0[k] = psi_load_row(A, X, mtp, emp, min(t + 2 * B + k, t1 - 1));
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            if (t + B + k < t1) row(P1[k], t + B + k);
+    }
+}
+
+// The window a later sweep keeps of its row r: its input iterate at rows
+// r-1 and r, the link factors of a(c, r), of the link to the left and of
+// b(c, r-1), and the node's right-hand side, diagonal and material test.
+template <typename R> struct PsiStage {
+    cplx<R> m, c;
+    R ca, sa, cal, sal, cbd, sbd;
+    cplx<R> rhs;
+    R den;
+    bool in;
+};
+
+// K >= 2 fused psi sweeps in one pass, psi_rows3's pipeline with K stages:
+// at row t, stage s runs sweep j+s at row t-s on lanes s+1 .. 62-s from the
+// rows stage s-1 left in registers (stage 0 from the loaded rows), so the
+// strips are WAVE - 2K columns wide and each stage's lateral neighbours come
+// from the lanes either side. Link factors, cells and node diagonals are
+// computed once per row by stage 0 and handed down the stages. Writes
+// psi^{j+K} (and, at jstep 0, the Langevin right-hand side); r[s] gets the
+// lane's largest |delta| of sweep j+s.
+template <typename R, int K, bool GAUGE, bool SH = false>
+__device__ __forceinline__ void psi_rowsK(const PsiSweepArgs<R> &A, int s_, int ja, int jb, R (&r)[K]) {
+    static_assert(K >= 2, "two or more fused sweeps");
+    constexpr int B = 2;
+    const ColCtx X = col_ctx<K>(A.Nx, s_);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int c = X.c;
+    const bool own = X.own;  // lanes K .. 63-K: the last stage's outputs
+    const int Nx = A.Nx, Ny = A.Ny, Nxa = Nx - 1;
+    const R *b = A.ab + (long)Nxa * Ny;
+    const bool noisy = A.rhs_out != nullptr;
+    const bool noise_on = noisy && A.jstep == 0 && A.langevin_c > R(1.0e-32);
+    const bool has_mt = A.mt != nullptr;
+    const uint8_t *mtp = has_mt ? A.mt : reinterpret_cast<const uint8_t *>(A.ab);
+    const R *emp = A.eps_map ? A.eps_map : reinterpret_cast<const R *>(A.rhs);
+    const R gb = GAUGE ? gauge_phase_b<R>(A, c) : R(0);
+    const cplx<R> zero = {R(0), R(0)};
+    const int t0 = ja > K - 1 ? ja - (K - 1) : 0;
+
+    // stage 0 window: psi^j at rows t-1, t; U of b(t-1); cell (c, t-1)
+    cplx<R> pm = zero, pc = zero;
+    R cb1 = R(1), sb1 = R(0);
+    int cell1 = 0;
+    if (X.col) {
+        pc = ld(A.psi, c + (long)Nx * t0);
+        if (t0 > 0) {
+            pm = ld(A.psi, c + (long)Nx * (t0 - 1));
+            sincos_r(link_phase<R, GAUGE>(A.dy, b[c + (long)Nx * (t0 - 1)], gb), &sb1, &cb1);
+            if (c < Nxa) cell1 = has_mt ? int(A.mt[c + (long)Nxa * (t0 - 1)]) : 1;
+        }
+    }
+    mask_link(shfl_up1(cell1) != 0 || cell1 != 0, cb1, sb1);  // b(c, t0-1)
+    // stages 1 .. K-1 (S[s-1]), empty until the pipeline fills
+    PsiStage<R> S[K - 1];
+#pragma unroll
+    for (int k = 0; k < K - 1; ++k)
+        S[k] = PsiStage<R>{zero, zero, R(0), R(0), R(0), R(0), R(0), R(0), zero, R(1), false};
+
+    auto row = [&](const PsiRow<R> &cu, int t) {
+        const long rowoff = (long)Nx * t;
+        const bool real = t < Ny;  // t >= Ny: past the last row (later stages only)
+        const cplx<R> pp = (X.col && t + 1 < Ny) ? cu.pp : zero;
+        const R av = X.hasa ? cu.av : R(0);
+        const R bv = (X.col && t < Ny - 1) ? cu.bv : R(0);
+        const int cellc = (X.hasa && t < Ny - 1) ? (has_mt ? cu.cell : 1) : 0;
+        cplx<R> rhs = cu.rhs;
+        const R e = A.eps_map ? cu.e : A.eps;
+        R sa, ca, sb, cb;
+        sincos_r(link_phase<R, GAUGE>(A.dx, av, GAUGE ? gauge_phase_a<R>(A, t) : R(0)), &sa, &ca);
+        sincos_r(link_phase<R, GAUGE>(A.dy, bv, gb), &sb, &cb);
+        const bool mm = shfl_up1(cell1) != 0;  // cell (c-1, t-1)
+        const bool mp = shfl_up1(cellc) != 0;  // cell (c-1, t)
+        const bool pmc = cell1 != 0, ppc = cellc != 0;
+        mask_link(pmc || ppc, ca, sa);  // a(c, t): cells (c, t-1), (c, t)
+        mask_link(mp || ppc, cb, sb);   // b(c, t): cells (c-1, t), (c, t)
+        const R cal = shfl_up1(ca), sal = shfl_up1(sa);
+        const bool inside = mm || mp || pmc || ppc;
+
+        cplx<R> out[K];  // out[s]: sweep j+s's result at row t-s
+        // stage 0: sweep j at row t
+        R den = R(1);
+        {
+            cplx<R> q = zero;
+            const R pl_re = shfl_up1(pc.re), pl_im = shfl_up1(pc.im);
+            const R pr_re = shfl_dn1(pc.re), pr_im = shfl_dn1(pc.im);
+            if (X.col && lane >= 1 && lane < WAVE - 1 && real && inside) {
+                if (noise_on) psi_noise(A, (uint32_t)(c + rowoff), rhs);
+                den = psi_den(A, rhs, e, mm, mp, pmc, ppc);
+                q = psi_apply(A, rhs, den, pl_re, pl_im, pr_re, pr_im, pm, pp, cal, sal, ca, sa, cb1, sb1, cb, sb);
+            }
+            if (own && real && t >= ja && t < jb) {
+                if (noisy && A.jstep == 0) st(A.rhs_out, c + rowoff, rhs);
+                r[0] = fmax(r[0], fmax(fabs(q.re - pc.re), fabs(q.im - pc.im)));
+            }
+            out[0] = q;
+        }
+        // stages 1 .. K-1: sweep j+s at row t-s; its up neighbour is stage
+        // s-1's result of this row, its up link stage s-1's down link
+#pragma unroll
+        for (int st_ = 1; st_ < K; ++st_) {
+            const PsiStage<R> &W = S[st_ - 1];
+            const int rs = t - st_;
+            const R ucb = st_ == 1 ? cb1 : S[st_ - 2].cbd, usb = st_ == 1 ? sb1 : S[st_ - 2].sbd;
+            const R pl_re = shfl_up1(W.c.re), pl_im = shfl_up1(W.c.im);
+            const R pr_re = shfl_dn1(W.c.re), pr_im = shfl_dn1(W.c.im);
+            cplx<R> w = zero;
+            if (st_ < K - 1) {
+                if (rs >= 0 && rs < Ny) {
+                    if (X.col && lane >= st_ + 1 && lane < WAVE - 1 - st_ && W.in)
+                        w = psi_apply(A, W.rhs, W.den, pl_re, pl_im, pr_re, pr_im, W.m, out[st_ - 1], W.cal, W.sal,
+                                      W.ca, W.sa, W.cbd, W.sbd, ucb, usb);
+                    if (own && rs >= ja && rs < jb) r[st_] = fmax(r[st_], fmax(fabs(w.re - W.c.re), fabs(w.im - W.c.im)));
+                }
+            } else if (rs >= ja && own) {
+                if (W.in)
+                    w = psi_apply(A, W.rhs, W.den, pl_re, pl_im, pr_re, pr_im, W.m, out[st_ - 1], W.cal, W.sal, W.ca,
+                                  W.sa, W.cbd, W.sbd, ucb, usb);
+                st_at(A.psi_next + (size_t)Nx * rs, 2 * X.cc * (uint32_t)sizeof(R), w);
+                r[st_] = fmax(r[st_], fmax(fabs(w.re - W.c.re), fabs(w.im - W.c.im)));
+            }
+            out[st_] = w;
+        }
+        // roll the windows, the last stage first
+#pragma unroll
+        for (int st_ = K - 1; st_ >= 1; --st_) {
+            PsiStage<R> &W = S[st_ - 1];
+            W.m = W.c;
+            W.c = out[st_ - 1];
+            if (st_ >= 2) {
+                const PsiStage<R> &P = S[st_ - 2];
+                W.ca = P.ca;
+                W.sa = P.sa;
+                W.cal = P.cal;
+                W.sal = P.sal;
+                W.cbd = P.cbd;
+                W.sbd = P.sbd;
+                W.rhs = P.rhs;
+                W.den = P.den;
+                W.in = P.in;
+            } else {
+                W.ca = ca;
+                W.sa = sa;
+                W.cal = cal;
+                W.sal = sal;
+                W.cbd = cb1;
+                W.sbd = sb1;
+                W.rhs = rhs;
+                W.den = den;
+                W.in = inside;
+            }
+        }
+        pm = pc;
+        pc = pp;
+        cb1 = cb;
+        sb1 = sb;
+        cell1 = cellc;
+    };
+
+    const int t1 = jb + K - 1;  // rows t in [t0, t1)
     PsiRow<R> P0[B], P1[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) P0[k] = psi_load_row(A, X, mtp, emp, min(t0 + k, t1 - 1));
@@ -1251,9 +1429,20 @@ __device__ __forceinline__ unsigned long long *state64(int *state, int k) {
 // 10000 steers the solve, and r < 10000 iff the field is below 625 =
 // 10000 / 16). sweep_sync: the workgroup's largest residuals, team barrier,
 // team-wide fields in r_out; res_conv(r_out, k): sweep k of the pass met the
-// stop rule everywhere.
+// stop rule everywhere. Four sweeps go as 8-bit fields: r >> 6 (at most
+// 156) below 10000, at least 157 from 10000 on.
 __device__ __forceinline__ int res_field(int r) { return min(r, 16383) >> 4; }
 __device__ __forceinline__ bool res_conv(int packed, int k) { return ((packed >> (10 * k)) & 0x3ff) < 625; }
+__device__ __forceinline__ int res_field8(int r) { return r < 10000 ? (r >> 6) : max(157, min(r, 16383) >> 6); }
+__device__ __forceinline__ bool res_conv8(int packed, int k) { return ((packed >> (8 * k)) & 0xff) < 157; }
+
+__device__ __forceinline__ bool sweep_sync4(const GridSync &gs, const Team &tm, const int (&r)[4],
+                                            unsigned long long &epoch, int *lds, int &r_out) {
+    int packed = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) packed |= block_max_int(res_field8(r[k]), lds) << (8 * k);
+    return team_sync_max<4>(gs, tm, epoch, packed, r_out);
+}
 
 template <int K = 3>
 __device__ __forceinline__ bool sweep_sync(const GridSync &gs, const Team &tm, int r1, int r2, int r3,
@@ -1298,19 +1487,29 @@ __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const Gri
     A.rhs_out = T.rhs_psi;
     int in = pc, out = o1, j = 0, nsw = 0, cnt = 0;
     bool done = false;
-    auto pass = [&](int k, R &r1, R &r2, R &r3) {
+    auto pass = [&](int k, R (&r)[4]) {
         cnt += PASS_ONE;
-        if constexpr (KMAX == 3) {
+        if constexpr (KMAX == 4) {
+            if (k == 4) {
+                for_rows_rm<4>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) {
+                    R q[4] = {r[0], r[1], r[2], r[3]};
+                    psi_rowsK<R, 4, GAUGE, SH>(A, s, ja, jb, q);
+                    r[0] = q[0]; r[1] = q[1]; r[2] = q[2]; r[3] = q[3];
+                });
+                return;
+            }
+        }
+        if constexpr (KMAX >= 3) {
             if (k == 3) {
                 for_rows_rm<3>(A.Nx, A.Ny, w, W,
-                               [&](int s, int ja, int jb) { psi_rows3<R, GAUGE, SH>(A, s, ja, jb, r1, r2, r3); });
+                               [&](int s, int ja, int jb) { psi_rows3<R, GAUGE, SH>(A, s, ja, jb, r[0], r[1], r[2]); });
                 return;
             }
         }
         if (k == 2)
-            for_rows_rm<2>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { psi_rows2<R, GAUGE, SH>(A, s, ja, jb, r1, r2); });
+            for_rows_rm<2>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { psi_rows2<R, GAUGE, SH>(A, s, ja, jb, r[0], r[1]); });
         else
-            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { r1 = fmax(r1, psi_rows<R, GAUGE, SH>(A, s, ja, jb)); });
+            for_rows_rm<1>(A.Nx, A.Ny, w, W, [&](int s, int ja, int jb) { r[0] = fmax(r[0], psi_rows<R, GAUGE, SH>(A, s, ja, jb)); });
     };
     while (!done && j < T.max_sweeps) {
         A.psi = pick3(T.P, in);
@@ -1319,25 +1518,34 @@ __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const Gri
         A.jstep = (uint32_t)j;
         int k = (j == 0 && pred % KMAX != 0) ? pred % KMAX : KMAX;
         k = min(k, T.max_sweeps - j);
-        R r1 = R(0), r2 = R(0), r3 = R(0);
-        pass(k, r1, r2, r3);
+        R rr[4] = {R(0), R(0), R(0), R(0)};
+        pass(k, rr);
         int r = 0;
-        if (!sweep_sync<KMAX>(gs, tm, scaled_residual<R>(r1, A.stop_eps), scaled_residual<R>(r2, A.stop_eps),
-                              scaled_residual<R>(r3, A.stop_eps), epoch, lds, r))
-            return {in, nsw, false, cnt};
+        bool ok;
+        if constexpr (KMAX == 4) {
+            const int sr[4] = {scaled_residual<R>(rr[0], A.stop_eps), scaled_residual<R>(rr[1], A.stop_eps),
+                               scaled_residual<R>(rr[2], A.stop_eps), scaled_residual<R>(rr[3], A.stop_eps)};
+            ok = sweep_sync4(gs, tm, sr, epoch, lds, r);
+        } else {
+            ok = sweep_sync<KMAX>(gs, tm, scaled_residual<R>(rr[0], A.stop_eps), scaled_residual<R>(rr[1], A.stop_eps),
+                                  scaled_residual<R>(rr[2], A.stop_eps), epoch, lds, r);
+        }
+        if (!ok) return {in, nsw, false, cnt};
+        auto conv = [&](int q) { return KMAX == 4 ? res_conv8(r, q) : res_conv(r, q); };
         // the first sweep of the pass that met the stop rule, or k
-        const int m = (k > 1 && res_conv(r, 0)) ? 1 : ((k > 2 && res_conv(r, 1)) ? 2 : k);
+        int m = k;
+        for (int q = k - 2; q >= 0; --q)
+            if (conv(q)) m = q + 1;
         if (m < k) {
-            // redo the pass up to sweep m into the same output (m is 1 unless
-            // the pass had three sweeps)
-            R d1 = R(0), d2 = R(0), d3 = R(0);
-            pass(KMAX == 3 && m == 2 ? 2 : 1, d1, d2, d3);
+            // redo the pass up to sweep m into the same output
+            R d[4] = {R(0), R(0), R(0), R(0)};
+            pass(m, d);
             cnt += REDO_ONE;
             int rdummy = 0;
             if (!sweep_sync<0>(gs, tm, 0, 0, 0, epoch, lds, rdummy)) return {in, nsw, false, cnt};
             done = true;
         } else {
-            done = res_conv(r, k - 1);
+            done = conv(k - 1);
         }
         j += m;
         nsw = j;
