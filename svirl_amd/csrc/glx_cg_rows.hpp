@@ -62,14 +62,20 @@ __global__ void __launch_bounds__(256) k_cg_grad_rows(Geo g, R kappa2, R eps, co
     const R dxdy = q.dx * q.dy, s2 = R(2) * q.dx * q.dy;
     const bool he = abei != nullptr, hb = ab != nullptr, has_mt = mt != nullptr, has_em = eps_map != nullptr;
     const bool upd_a = UPD && S.dab != nullptr;
-    const R *any = (const R *)psi;
-    const R *abei_s = he ? abei : any, *ab_s = hb ? ab : any, *dab_s = upd_a ? S.dab : any;
-    const uint8_t *mt_s = has_mt ? mt : (const uint8_t *)psi;
-    const R *em_s = has_em ? eps_map : any;
+    // an absent buffer is replaced by one read at the same index anyway, so
+    // its loads hit lines already fetched (the links by A or abei, the cell
+    // flags by the first byte of A's link, epsilon by the real part of the
+    // previous gradient at the node)
+    const R *lk = hb ? ab : (he ? abei : (const R *)psi);
+    const R *abei_s = he ? abei : lk, *ab_s = hb ? ab : lk, *dab_s = upd_a ? S.dab : lk;
+    const uint8_t *mt_s = has_mt ? mt : (const uint8_t *)lk;
+    const int mt_st = has_mt ? 1 : (int)sizeof(R);
+    const R *em_s = has_em ? eps_map : (const R *)jac_prev;
+    const int em_st = has_em ? 1 : 2;
     const cplx<R> *dpsi_s = UPD ? S.dpsi : psi;
-    const R *jAp_s = withA ? jacA_prev : any;
+    const R *jAp_s = withA ? jacA_prev : lk;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
+    const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
     if (w < W) for_rows_rm<1>(g.Nx, g.Ny, w, W, [&](int s, int ja, int jb) {
         const ColCtx X = col_ctx<1>(g.Nx, s);
         const int c = X.c;
@@ -97,8 +103,8 @@ __global__ void __launch_bounds__(256) k_cg_grad_rows(Geo g, R kappa2, R eps, co
             t.be = abei_s[kb];
             t.bb = ab_s[kb];
             t.bd = dab_s[kb];
-            t.mc = mt_s[kc_of(j)];
-            t.ee = em_s[n];
+            t.mc = mt_s[kc_of(j) * mt_st];
+            t.ee = em_s[n * em_st];
             t.gp = ld(jac_prev, n);
             t.vpa = jAp_s[ka_of(j)];
             t.vpb = jAp_s[kb];
@@ -135,7 +141,7 @@ __global__ void __launch_bounds__(256) k_cg_grad_rows(Geo g, R kappa2, R eps, co
             Ad = upd_link(abei_s[a0], ab_s[a0], dab_s[a0]);
             Ac = upd_link(abei_s[a1], ab_s[a1], dab_s[a1]);
             Bd = upd_link(abei_s[b0], ab_s[b0], dab_s[b0]);
-            md = has_mt ? mt_s[kc_of(ja - 1)] != 0 : true;
+            md = has_mt ? mt[kc_of(ja - 1)] != 0 : true;
         }
         R sud = R(0), cud = R(1);
         if (ja > 0) {
@@ -273,16 +279,20 @@ __global__ void __launch_bounds__(256) k_coef_dir_rows(Geo g, R kappa2, R eps, c
     U.bp = R(beta[0]);
     U.ba = R(beta[1]);
     const bool he = abei != nullptr, hb = ab != nullptr, has_mt = mt != nullptr, has_em = eps_map != nullptr;
-    const R *any = (const R *)psi;
-    const R *abei_s = he ? abei : any, *ab_s = hb ? ab : any;
-    const R *dAold_s = L == 17 ? U.dAold : any, *jacA_s = L == 17 ? U.jacA : any;
-    const uint8_t *mt_s = has_mt ? mt : (const uint8_t *)psi;
-    const R *em_s = has_em ? eps_map : any;
+    // absent buffers replaced as in k_cg_grad_rows (epsilon by the real part
+    // of the gradient at the node)
+    const R *lk = hb ? ab : (he ? abei : (const R *)psi);
+    const R *abei_s = he ? abei : lk, *ab_s = hb ? ab : lk;
+    const R *dAold_s = L == 17 ? U.dAold : lk, *jacA_s = L == 17 ? U.jacA : lk;
+    const uint8_t *mt_s = has_mt ? mt : (const uint8_t *)lk;
+    const int mt_st = has_mt ? 1 : (int)sizeof(R);
+    const R *em_s = has_em ? eps_map : (const R *)U.jac;
+    const int em_st = has_em ? 1 : 2;
     double acc[L];
 #pragma unroll
     for (int k = 0; k < L; ++k) acc[k] = 0.0;
     double th = 0.0;
-    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
+    const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
     if (w < W) for_rows_rm<1>(g.Nx, g.Ny, w, W, [&](int s, int ja, int jb) {
         const ColCtx X = col_ctx<1>(g.Nx, s);
         const int c = X.c;
@@ -319,14 +329,14 @@ __global__ void __launch_bounds__(256) k_coef_dir_rows(Geo g, R kappa2, R eps, c
             t.gr = ld(U.jac, k);
             t.a = link(ka_of(j + 1));
             t.b = link(kb_of(j));
-            t.mc = mt_s[kc_of(j)];
-            t.eps = em_s[row(j)];
+            t.mc = mt_s[kc_of(j) * mt_st];
+            t.eps = em_s[row(j) * em_st];
             return t;
         };
         // window: psi, direction and the a links at row ja; the cell below
         cplx<R> Pc = ld(psi, row(ja)), Dc = dnode(ld(U.dold, row(ja)), ld(U.jac, row(ja)));
         LinkTriple<R> Ac = dlink(link(ka_of(ja)));
-        bool md = has_mt ? mt_s[kc_of(ja - 1)] != 0 : true;
+        bool md = has_mt ? mt[kc_of(ja - 1)] != 0 : true;
         auto consume = [&](const Step &t, int j) {
             const long n = (long)g.Nx * j + cc;
             const cplx<R> Pu = t.p, Du = dnode(t.o, t.gr);

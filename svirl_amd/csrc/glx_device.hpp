@@ -188,6 +188,18 @@ __device__ __forceinline__ int wave_max_int(int v) {
     for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, WAVE));
     return v;
 }
+// Workgroups are dispatched to the NXCD accelerator dies round-robin by
+// blockIdx, and each die has its own L2. The row-marching kernels number
+// their work by xcd_block(), which gives the workgroups of one die a contiguous
+// range, so that waves sharing halo rows, halo columns and cache lines of
+// adjacent strips mostly sit on the same die and share its L2.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_block() {
+    const int G = gridDim.x, b = blockIdx.x;
+    if (G % NXCD) return b;
+    return (b % NXCD) * (G / NXCD) + b / NXCD;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);

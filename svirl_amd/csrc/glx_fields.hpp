@@ -1435,7 +1435,7 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_vortex_flags(Geo g, const cplx<R> *psi, const R *ab, double H,
                                                       unsigned long long *bits, int *wcount, long W) {
     const double dxdyH = g.dx * g.dy * H;
-    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
+    const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
     if (w >= W) return;  // wave-uniform
     const int lane = threadIdx.x & (WAVE - 1);
     const R *bl = ab + g.Na;
@@ -1551,7 +1551,7 @@ __global__ void __launch_bounds__(256) k_vortex_emit(Geo g, const cplx<R> *psi, 
                                                      const unsigned long long *bits, const int *slot, double *rec,
                                                      int capacity, long W) {
     const double dxdyH = g.dx * g.dy * H;
-    const long w = (long)blockIdx.x * (blockDim.x / WAVE) + wave_id();
+    const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
     if (w >= W) return;  // wave-uniform
     const int lane = threadIdx.x & (WAVE - 1);
     long base = slot[w];

@@ -166,14 +166,48 @@ int L<R>::a_sweep(const Env &e, glx_grid g, double dt, double kappa2, double rho
 template <typename R> struct RunForm { static constexpr bool has_gauge = false; };
 template <> struct RunForm<float> { static constexpr bool has_gauge = true; };
 
-template <typename R, bool G> static int solve_occupancy(int *per_cu) {
+// Co-resident workgroups per CU of the psi solve kernel and of the A solve
+// kernels (the smaller of the plain and fixed-source forms).
+template <typename R, bool G> static int solve_occupancy2(int *psi_per_cu, int *a_per_cu) {
     int a = 0, b = 0, c = 1 << 20;
     hipError_t st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_td_psi_solve<R, G>, 256, 0);
     if (st == hipSuccess) st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_td_a_solve<R, G>, 256, 0);
     if constexpr (!G)  // the fixed-source A solve runs in the plain form only
         if (st == hipSuccess) st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_td_a_solve<R, G, true>, 256, 0);
     if (st != hipSuccess) return check_hip(st, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-    *per_cu = a < b ? (a < c ? a : c) : (b < c ? b : c);
+    *psi_per_cu = a;
+    *a_per_cu = b < c ? b : c;
+    return GLX_OK;
+}
+
+template <typename R, bool G> static int solve_occupancy(int *per_cu) {
+    int p = 0, q = 0;
+    int rc = solve_occupancy2<R, G>(&p, &q);
+    if (rc) return rc;
+    *per_cu = p < q ? p : q;
+    return GLX_OK;
+}
+
+// The co-resident limits of the two persistent solves separately (each
+// launch only needs its own grid resident): workgroups on the chip.
+template <typename R> static int solve_grids(const Env &e, int32_t *psi_max, int32_t *a_max) {
+    static int pp = 0, pa = 0;
+    if (pp <= 0 || pa <= 0) {
+        int p = 0, a = 0;
+        int rc = solve_occupancy2<R, false>(&p, &a);
+        if (rc) return rc;
+        if constexpr (RunForm<R>::has_gauge) {
+            int pg = 0, ag = 0;
+            rc = solve_occupancy2<R, true>(&pg, &ag);
+            if (rc) return rc;
+            if (pg < p) p = pg;
+            if (ag < a) a = ag;
+        }
+        pp = p;
+        pa = a;
+    }
+    *psi_max = std::min(pp * e.num_cus, MAX_GRID_BLOCKS);
+    *a_max = std::min(pa * e.num_cus, MAX_GRID_BLOCKS);
     return GLX_OK;
 }
 
@@ -331,12 +365,16 @@ int L<R>::td_run(const Env &e, glx_grid g, const glx_td_params *p, const void *e
     int rc = td_max_grid(e, wg_size, &maxg);
     if (rc) return rc;
     // Default grids, the measured optima at 4097^2: two workgroups per CU
-    // for the psi solve, three for the A solve (maxg is the co-resident
-    // limit of every solve kernel); an explicit wg applies to both.
+    // for the psi solve, three for the A solve, each within its own kernel's
+    // co-resident limit (maxg, the smaller of the two, bounds an explicit wg,
+    // which applies to both).
     int32_t wg_a = wg;
     if (wg <= 0) {
-        wg = std::min(maxg, 2 * e.num_cus);
-        wg_a = std::min(maxg, 3 * e.num_cus);
+        int32_t psi_max = 0, a_max = 0;
+        rc = solve_grids<R>(e, &psi_max, &a_max);
+        if (rc) return rc;
+        wg = std::min(psi_max, 2 * e.num_cus);
+        wg_a = std::min(a_max, 3 * e.num_cus);
     }
     if (wg > maxg) return set_error(GLX_E_ARG, "glx_td_run: grid %d exceeds co-resident limit %d", wg, maxg);
     const bool evolves_A = p->solveA && p->eqn != 1;

@@ -1326,17 +1326,6 @@ __device__ __forceinline__ void for_rows(int Nx, int Ny, long w, long W, F &&f) 
     }
 }
 
-// Workgroups are dispatched to the NXCD accelerator dies round-robin by
-// blockIdx, and each die has its own L2. The persistent solves number their
-// work by xcd_block(), which gives the workgroups of one die a contiguous
-// range, so that waves sharing halo rows, halo columns and cache lines of
-// adjacent strips mostly sit on the same die and share its L2.
-constexpr int NXCD = 8;
-__device__ __forceinline__ int xcd_block() {
-    const int G = gridDim.x, b = blockIdx.x;
-    if (G % NXCD) return b;
-    return (b % NXCD) * (G / NXCD) + b / NXCD;
-}
 
 // Row-major partition for the persistent solves: the rows are cut into
 // S = max(1, W / nstrips) segments and the work items k = (segment, strip)
