@@ -1469,7 +1469,9 @@ __host__ __device__ inline int cnt_xbuf(int c) { return (c >> 13) & 0x7ff; }
 template <typename R, bool GAUGE, bool SH = false>
 __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const GridSync &gs, const Team &tm, long w,
                                               long W, int pc, int qc, int pred, unsigned long long &epoch, int *lds) {
-    constexpr int KMAX = Fuse<R>::PSI;
+    // (the batched kernel, SH, keeps three-sweep passes: its teams are small
+    // and a fourth stage's wider halo costs more than it saves there)
+    constexpr int KMAX = SH ? 3 : Fuse<R>::PSI;
     const int o1 = (pc + 1) % 3, o2 = (pc + 2) % 3;
     PsiSweepArgs<R> A = T.psi;
     A.ab = pick3(T.Q, qc);
