@@ -360,14 +360,19 @@ def run_batch(args):
     if args.batch_n == 256:
         sims = workloads.config4(n=args.batch_n, dtype=np.float32)
         BatchTD(sims, autotune=False).td(dt=0.1, Nt=int(z["Nt"]))
-        fe, pe = [], []
+        fe, pe, je = [], [], []
         for n, k in enumerate(z["index"]):
             gl = sims[int(k)]
             fe.append(rel(gl.observables.free_energy_fp64(), float(z[f"F_{n}"])))
             pe.append(l2rel(gl.flatten_array(gl.vars.order_parameter).astype(np.complex128), z[f"psi_{n}"]))
+            jx, jy = gl.observables.current_density
+            je.append(l2rel(np.concatenate([gl.flatten_a_array(jx), gl.flatten_b_array(jy)]).astype(np.float64),
+                            z[f"J_{n}"]))
         res["vs_oracle"] = {"horizon": f"{int(z['Nt'])} TD steps", "instances": [int(k) for k in z["index"]],
-                            "F_rel_max": float(np.max(fe)), "psi_L2_rel_max": float(np.max(pe))}
-        log(f"[c4] FP32 vs oracle ({len(fe)} instances, {int(z['Nt'])} steps): F {np.max(fe):.2e}, psi {np.max(pe):.2e}")
+                            "F_rel_max": float(np.max(fe)), "psi_L2_rel_max": float(np.max(pe)),
+                            "J_L2_rel_max": float(np.max(je))}
+        log(f"[c4] FP32 vs oracle ({len(fe)} instances, {int(z['Nt'])} steps): F {np.max(fe):.2e}, psi {np.max(pe):.2e}, "
+            f"J {np.max(je):.2e}")
         del sims
     # beyond the oracle's horizon: the first 32 instances, FP32-mixed vs FP64, batched, same steps
     k = min(32, args.batch_n)
@@ -414,8 +419,13 @@ def run_batch_cg(args):
             BatchCG(sample).cg(max_iter=1, rtol=0.0)
             fr = [(gl.observables.free_energy_fp64() - float(z[f"F1_{n}"])) / abs(float(z[f"F1_{n}"]))
                   for n, gl in enumerate(sample)]
+            jr = []
+            for n, gl in enumerate(sample):
+                jx, jy = gl.observables.current_density
+                jr.append(l2rel(np.concatenate([gl.flatten_a_array(jx), gl.flatten_b_array(jy)]).astype(np.float64),
+                                z[f"J1_{n}"]))
             vo = {"horizon": "1 iteration (exact line minimum along -grad F)", "instances": [int(k) for k in z["index"]],
-                  "F_rel_max": float(np.max(np.abs(fr)))}
+                  "F_rel_max": float(np.max(np.abs(fr))), "J_L2_rel_max": float(np.max(jr))}
             del sample
         # the timed run starts from the initial state, so ||grad F||_0 is the
         # initial state's gradient norm
@@ -441,7 +451,8 @@ def run_batch_cg(args):
         F[name] = np.array([s.observables.free_energy_fp64() for s in sims[:64]])
         log(f"[c5] {name}: {total / t:.4g} instance-iterations/s, {res[name]['node_iters_per_s']:.4g} "
             f"node-iterations/s, {int(conv.sum())} converged, median grad_rel {np.median(grel):.2e}; first step vs "
-            f"oracle F {vo.get('F_rel_max', float('nan')):.2e}; wall {t:.1f} s"
+            f"oracle F {vo.get('F_rel_max', float('nan')):.2e}, J {vo.get('J_L2_rel_max', float('nan')):.2e}; "
+            f"wall {t:.1f} s"
             + (f"; {res[name]['finished_in_fp64']} finished in FP64 after stalling" if mixed else ""))
         del sims, b
     res["fp32_vs_fp64_F_rel_max"] = float(np.max(np.abs(F["fp32"] - F["fp64"]) / np.abs(F["fp64"])))

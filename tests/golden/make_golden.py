@@ -185,7 +185,8 @@ def c3_4097():
 
 def c4_sample():
     """The four smallest instances of Config 4 (workloads.config4_specs),
-    3 TD steps of dt = 0.1 each: sweep counts, F and the whole psi."""
+    3 TD steps of dt = 0.1 each: sweep counts, F, the whole psi and A, and
+    the current density (J, or J_s at kappa = inf)."""
     from svirl_amd import workloads
     specs = workloads.config4_specs()
     pick = sorted(range(len(specs)), key=lambda k: (specs[k]["Nx"] * specs[k]["Ny"], k))[:4]
@@ -198,7 +199,7 @@ def c4_sample():
                            kappa=s["kappa"], sigma=s["sigma"], H=s["H"], eps_map=em, random_t=s["seed"])
         st.td(dt=0.1, Nt=3)
         out.update({f"sweeps_{n}": np.array(st.sweeps), f"F_{n}": st.free_energy(), f"psi_{n}": st.psi,
-                    f"ab_{n}": st.ab})
+                    f"ab_{n}": st.ab, f"J_{n}": st.current_density()})
     return out
 
 
@@ -208,7 +209,8 @@ def c5_sample():
     (the oracle's Jacobians with the field part on boundary links halved,
     which makes dF/dA exact) and the step minimises the oracle's F along it
     (Nelder-Mead on (alpha_psi, alpha_A), then polished to 1e-13 in F).
-    Stores F before and at the minimum and the two step lengths."""
+    Stores F before and at the minimum, the two step lengths and the
+    current density J at the minimum."""
     import scipy.optimize
     from svirl_amd import workloads
     specs = workloads.config5_specs()
@@ -238,7 +240,9 @@ def c5_sample():
         s0 = np.array([1.0 / np.sqrt(np.vdot(d, d).real), 1.0 / np.sqrt(np.dot(dA, dA))]) * 0.1
         r = scipy.optimize.minimize(lambda y: f(y * s0), x0=np.ones(2), method='Nelder-Mead',
                                     options=dict(xatol=1e-12, fatol=1e-13 * abs(F0), maxiter=4000))
-        out.update({f"F0_{n}": F0, f"F1_{n}": float(r.fun), f"alpha_{n}": r.x * s0})
+        a1 = r.x * s0
+        out.update({f"F0_{n}": F0, f"F1_{n}": float(r.fun), f"alpha_{n}": a1,
+                    f"J1_{n}": O.current_density(g, k2, H, None, ab + a1[1] * dA)})
     return out
 
 

@@ -235,10 +235,18 @@ def test_c4_fp32_batch_against_oracle():
         st.td(dt=0.1, Nt=3)
         refs[k] = st
     BatchTD(sims, use_graph=True, autotune=False).td(dt=0.1, Nt=3)
-    for k, st in refs.items():
+    z = np.load(os.path.join(GOLD, "c4_sample.npz"))
+    assert [int(k) for k in z["index"]] == list(refs)
+    for n, (k, st) in enumerate(refs.items()):
         gl = sims[k]
         assert gl.observables.free_energy_fp64() == pytest.approx(st.free_energy(), rel=1e-5)
         assert l2rel(gl.flatten_array(gl.vars.order_parameter).astype(np.complex128), st.psi) < 1e-5
+        # the committed fixture is this oracle run
+        assert np.array_equal(z[f"psi_{n}"], st.psi) and np.array_equal(z[f"J_{n}"], st.current_density())
+        # J differentiates the float-stored A twice (see test_c2_fp32_mixed_against_oracle)
+        jx, jy = gl.observables.current_density
+        J = np.concatenate([gl.flatten_a_array(jx), gl.flatten_b_array(jy)]).astype(np.float64)
+        assert l2rel(J, z[f"J_{n}"]) < 3e-2
 
 
 # ---------------------------------------------------------------------------
