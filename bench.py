@@ -323,12 +323,12 @@ def run_batch(args):
     against the batched FP64 form."""
     from svirl_amd import _native as nat
     from svirl_amd import workloads
-    from svirl_amd.batch import CANDIDATES, BatchTD
+    from svirl_amd.batch import BatchTD
     log(f"[c4] building {args.batch_n} Config 4 instances")
     sims = workloads.config4(n=args.batch_n, dtype=np.float32)
     nodes = sum(int(s.cfg.N) for s in sims)
     b = BatchTD(sims, use_graph=True, autotune=True)
-    b.td(dt=0.1, Nt=b.tune_steps * (len(CANDIDATES) + 1) + 1)  # autotune on the first steps
+    b.td(dt=0.1, Nt=b.tune_need())  # autotune on the first steps
     nat.call("glx_synchronize")
     t0 = time.perf_counter()
     b.td(dt=0.1, Nt=args.batch_steps)
@@ -350,8 +350,9 @@ def run_batch(args):
     sw = np.array([np.mean(np.array(s.solve._td.sweeps), axis=0) for s in sims])
     res = {"instances": len(sims), "nodes": nodes, "node_steps_per_s": rate_b, "ms_per_step": 1e3 * tb / args.batch_steps,
            "per_instance_node_steps_per_s": rate_s, "speedup_vs_per_instance": rate_b / rate_s,
-           "tuned_shape": {"wg_per_cu": b.shape[0], "min_rows": b.shape[1]},
-           "tuning_ms_per_step": {f"{k[0]},{k[1]}": v for k, v in b.timings.items()},
+           "tuned_shape": {"wg_per_cu": b.shape[0], "min_rows": b.shape[1], "psi_depth": b.shape[2],
+                           "strip_columns": 64 - 2 * b.shape[2]},
+           "tuning_ms_per_step": {f"wg{k[0]},rows{k[1]},depth{k[2]}": v for k, v in b.timings.items()},
            "mean_sweeps_per_step": {"psi": float(sw[:, 0].mean()), "A": float(sw[:, 1].mean())},
            "dtype": "f32", "graph": True}
     del sims

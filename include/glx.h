@@ -168,13 +168,16 @@ typedef struct glx_batch_instance {
  * proportion to each instance's wave-rows times weight[i] (its relative
  * sweeps per step, e.g. from an earlier run; NULL = 1). With use_graph != 0,
  * the step launches are captured once into a HIP graph and replayed.
- * wg_per_cu (co-resident workgroups per CU, 0 = as many as fit) and
- * min_rows (fewest rows per wave when sizing teams, 0 = 16) are the tunable
- * launch shape. sweeps (host, nsteps*n*2 int32, may be NULL) receives
- * (S_psi, S_A) per step and instance. Instances must not share buffers; n
- * must not exceed the number of co-resident workgroups. */
+ * wg_per_cu (co-resident workgroups per CU, 0 = as many as fit),
+ * min_rows (fewest rows per wave when sizing teams, 0 = 16) and psi_depth
+ * (sweeps per psi pass, 2..4, i.e. strips of 64 - 2*psi_depth columns;
+ * 0 = 3) are the tunable tile shape; the results do not depend on them.
+ * sweeps (host, nsteps*n*2 int32, may be NULL) receives (S_psi, S_A) per
+ * step and instance. Instances must not share buffers; n must not exceed
+ * the number of co-resident workgroups. */
 int glx_batch_td_run(int prec, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
-                     int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight);
+                     int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight,
+                     int32_t psi_depth);
 
 /* Bytes of the fixed device scratch array behind glx_batch_td_run_noalloc. */
 #define GLX_NOALLOC_SCRATCH_BYTES (8u << 20)
@@ -194,7 +197,7 @@ int glx_batch_td_run(int prec, glx_batch_instance *inst, int32_t n, int32_t nste
  * call rewrites all it reads before its first step. Calls that overlap on
  * one GPU (from several threads) race on it. */
 int glx_batch_td_run_noalloc(int prec, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
-                             int32_t wg_per_cu, int32_t min_rows, const double *weight);
+                             int32_t wg_per_cu, int32_t min_rows, const double *weight, int32_t psi_depth);
 
 /* ------------------------------------------------------------ batched CG */
 

@@ -86,8 +86,10 @@ _PROTOS = {
     "glx_td_last_kernel_ms": [ctypes.POINTER(ctypes.c_float)],
     "glx_td_profile": [c_i32],
     "glx_td_force_barrier_abort": [],
-    "glx_batch_td_run": [c_int, ctypes.POINTER(BatchInstance), c_i32, c_i32, P_i32, c_i32, c_i32, c_i32, c_void_p],
-    "glx_batch_td_run_noalloc": [c_int, ctypes.POINTER(BatchInstance), c_i32, c_i32, P_i32, c_i32, c_i32, c_void_p],
+    "glx_batch_td_run": [c_int, ctypes.POINTER(BatchInstance), c_i32, c_i32, P_i32, c_i32, c_i32, c_i32, c_void_p,
+                         c_i32],
+    "glx_batch_td_run_noalloc": [c_int, ctypes.POINTER(BatchInstance), c_i32, c_i32, P_i32, c_i32, c_i32, c_void_p,
+                                 c_i32],
     "glx_batch_cg_run": [c_int, ctypes.POINTER(BatchCGInstance), c_i32, c_i32, c_double, c_i32, c_i32, c_i32],
     "glx_td_solve_ms": [ctypes.POINTER(c_double), P_i32, ctypes.POINTER(c_double), P_i32],
     "glx_td_passes": [P_i32, P_i32, P_i32, P_i32, P_i32],

@@ -8,13 +8,16 @@ by its own team of workgroups with team-local barriers, so each instance
 keeps its own Jacobi sweep counts and stop rule, exactly as if it had been
 run alone. The step launches are replayed from a captured HIP graph.
 
-The launch shape (workgroups per CU, fewest rows per wave when sizing the
-teams) is picked by a small autotuner: the first steps of a run time each
-candidate shape on the real state (the results do not depend on the shape)
-and the rest of the run uses the fastest; the choice is cached per batch
-signature. Teams are sized by each instance's wave-rows times its sweeps
-per step in the previous run, so instances that iterate more get more
-workgroups.
+The tile shape — sweeps per psi pass (the tile depth, which sets the
+strip width: 64 - 2 x depth columns per wave), fewest rows per wave when
+sizing the teams (the tile height) and workgroups per CU — is picked by an
+autotuner: the first steps of a run time candidate shapes on the real state
+(the results do not depend on the shape), one axis at a time from a default
+(coordinate descent: depth, then height, then occupancy, each axis keeping
+its fastest value), and the rest of the run uses the fastest; the choice is
+cached per batch signature. Teams are sized by each instance's wave-rows
+times its sweeps per step in the previous run, so instances that iterate
+more get more workgroups.
 
     from svirl_amd import GLSolver
     from svirl_amd.batch import BatchTD
@@ -33,16 +36,21 @@ import numpy as np
 from svirl_amd import _native as nat
 from svirl_amd._native import BatchCGInstance, BatchInstance
 
-# (signature) -> (wg_per_cu, min_rows)
+# (signature) -> shape (wg_per_cu, min_rows, psi_depth)
 _TUNED = {}
 
-CANDIDATES = [(0, 8), (0, 16), (0, 12), (0, 4), (2, 8)]
+DEFAULT_SHAPE = (0, 8, 3)
+# the tuner's axes, in the order it visits them: (index in the shape, values)
+AXES = ((2, (3, 4, 2)), (1, (8, 4, 12, 16, 24)), (0, (0, 2, 1, 3)))
+# shapes the tuner times at most: the default, then the other values of each axis
+TUNE_RUNS = 1 + sum(len(v) - 1 for _, v in AXES)
 
 
 class BatchTD:
     """Advance a list of GLSolver objects together (FP32 or FP64, all the
     same). use_graph: replay the step launches from a HIP graph.
-    autotune: pick the launch shape from CANDIDATES on the first steps.
+    autotune: pick the tile shape on the first steps (AXES; needs
+    tune_need() steps of the first td() call).
     noalloc: run through glx_batch_td_run_noalloc (no device allocation per
     call, its workspace is the library's fixed scratch array; no graph)."""
 
@@ -62,6 +70,10 @@ class BatchTD:
         self.timings = {}
         self.last_kernel_ms = 0.0
         self.weights = None   # per-instance sweeps per step of the last run (team sizing)
+
+    def tune_need(self):
+        """Steps the autotuner takes from the first td() call."""
+        return self.tune_steps * (TUNE_RUNS + 1) + 1
 
     def _signature(self):
         return tuple((int(s.cfg.Nx), int(s.cfg.Ny), bool(s.params.solveA)) for s in self.solvers) + (self.prec,)
@@ -105,10 +117,10 @@ class BatchTD:
             wt = np.ascontiguousarray(self.weights, dtype=np.float64)
         wp = wt.ctypes.data if wt is not None else None
         if self.noalloc:
-            nat.call("glx_batch_td_run_noalloc", self.prec, arr, n, nsteps, sw, shape[0], shape[1], wp)
+            nat.call("glx_batch_td_run_noalloc", self.prec, arr, n, nsteps, sw, shape[0], shape[1], wp, shape[2])
         else:
             nat.call("glx_batch_td_run", self.prec, arr, n, nsteps, sw, 1 if self.use_graph else 0, shape[0],
-                     shape[1], wp)
+                     shape[1], wp, shape[2])
         kms = ctypes.c_float()
         nat.call("glx_td_last_kernel_ms", ctypes.byref(kms))
         self.last_kernel_ms = kms.value
@@ -128,18 +140,29 @@ class BatchTD:
         sig = self._signature()
         if self.shape is None:
             self.shape = _TUNED.get(sig)
-        if self.shape is None and self.autotune and Nt >= self.tune_steps * (len(CANDIDATES) + 1) + 1:
+        if self.shape is None and self.autotune and Nt >= self.tune_need():
             if self.weights is None:  # learn the per-instance sweep counts first
-                self._run(tds, self.tune_steps, CANDIDATES[0])
+                self._run(tds, self.tune_steps, DEFAULT_SHAPE)
                 done += self.tune_steps
-            for cand in CANDIDATES:
-                ms = self._run(tds, self.tune_steps, cand)
-                self.timings[cand] = ms / self.tune_steps
-                done += self.tune_steps
-            self.shape = min(self.timings, key=self.timings.get)
+
+            def timed(shape):
+                nonlocal done
+                if shape not in self.timings:
+                    self.timings[shape] = self._run(tds, self.tune_steps, shape) / self.tune_steps
+                    done += self.tune_steps
+                return self.timings[shape]
+
+            best = DEFAULT_SHAPE
+            timed(best)
+            for axis, values in AXES:
+                for v in values:
+                    cand = best[:axis] + (v,) + best[axis + 1:]
+                    if timed(cand) < self.timings[best]:
+                        best = cand
+            self.shape = best
             _TUNED[sig] = self.shape
         if self.shape is None:
-            self.shape = CANDIDATES[0]
+            self.shape = DEFAULT_SHAPE
         if Nt - done > 0:
             self._run(tds, Nt - done, self.shape)
         for s in self.solvers:

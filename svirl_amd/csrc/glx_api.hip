@@ -326,19 +326,22 @@ int glx_td_last_kernel_ms(float *ms) {
 }
 
 int glx_batch_td_run(int prec, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
-                     int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight) {
+                     int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight,
+                     int32_t psi_depth) {
     struct ClearAbort { ~ClearAbort() { g_ctx.env.abort_next = 0; } } clear_abort;  // one run only
     const Workspace ws{nullptr, 0, "glx_batch_td_run"};
-    GLX_DISPATCH(prec, batch_td_run(g_ctx.env, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows, weight, ws));
+    GLX_DISPATCH(prec, batch_td_run(g_ctx.env, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows, weight,
+                                    psi_depth, ws));
 }
 
 int glx_batch_td_run_noalloc(int prec, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
-                             int32_t wg_per_cu, int32_t min_rows, const double *weight) {
+                             int32_t wg_per_cu, int32_t min_rows, const double *weight, int32_t psi_depth) {
     GLX_REQUIRE_INIT();
     struct ClearAbort { ~ClearAbort() { g_ctx.env.abort_next = 0; } } clear_abort;  // one run only
     const Workspace ws{g_scratch_dev[g_ctx.device], (size_t)GLX_NOALLOC_SCRATCH_BYTES, "glx_batch_td_run_noalloc"};
     if (!ws.ptr) return set_error(GLX_E_NOINIT, "glx_batch_td_run_noalloc: no scratch address for device %d", g_ctx.device);
-    GLX_DISPATCH(prec, batch_td_run(g_ctx.env, inst, n, nsteps, sweeps, 0, wg_per_cu, min_rows, weight, ws));
+    GLX_DISPATCH(prec, batch_td_run(g_ctx.env, inst, n, nsteps, sweeps, 0, wg_per_cu, min_rows, weight, psi_depth,
+                                    ws));
 }
 
 int glx_batch_cg_run(int prec, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,

@@ -70,7 +70,7 @@ template <typename R> struct L {
                       int32_t nsteps, int32_t *sweeps, int32_t wg, int32_t wg_size);
     static int batch_td_run(const Env &e, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
                             int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight,
-                            const Workspace &ws);
+                            int32_t psi_depth, const Workspace &ws);
     static int batch_cg_run(const Env &e, glx_batch_cg_instance *inst, int32_t n, int32_t max_iters, double rtol,
                             int32_t iters_per_launch, int32_t stall_iters, int32_t fp64_finish);
     static int td_max_grid(const Env &e, int32_t wg_size, int32_t *max_wg);

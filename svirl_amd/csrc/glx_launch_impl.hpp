@@ -445,17 +445,17 @@ static void plan_teams(const std::vector<long> &rows, const std::vector<double> 
     total(lo, &nb);
 }
 
-template <typename R, bool G> static int batch_occupancy(int *per_cu) {
-    hipError_t st = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_batch_td_step<R, G>, 256, 0);
+template <typename R, bool G, int KB> static int batch_occupancy(int *per_cu) {
+    hipError_t st = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_batch_td_step<R, G, KB>, 256, 0);
     return st == hipSuccess ? GLX_OK : check_hip(st, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
 }
 
-template <typename R, bool G>
+template <typename R, bool G, int KB>
 static int batch_run_form(const Env &e, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
                           int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight,
                           const Workspace &ws) {
     int per_cu = 0;
-    int rc = batch_occupancy<R, G>(&per_cu);
+    int rc = batch_occupancy<R, G, KB>(&per_cu);
     if (rc) return rc;
     if (wg_per_cu > 0 && wg_per_cu < per_cu) per_cu = wg_per_cu;
     const int Gb = std::min(per_cu * e.num_cus, MAX_GRID_BLOCKS);
@@ -541,7 +541,7 @@ static int batch_run_form(const Env &e, glx_batch_instance *inst, int32_t n, int
         hipGraphExec_t exec = nullptr;
         st = hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal);
         if (st != hipSuccess) return check_hip(st, "hipStreamBeginCapture");
-        for (int k = 0; k < K; ++k) hipLaunchKernelGGL((k_batch_td_step<R, G>), dim3(grid), dim3(256), 0, e.stream, B);
+        for (int k = 0; k < K; ++k) hipLaunchKernelGGL((k_batch_td_step<R, G, KB>), dim3(grid), dim3(256), 0, e.stream, B);
         st = hipStreamEndCapture(e.stream, &graph);
         if (st != hipSuccess) return check_hip(st, "hipStreamEndCapture");
         st = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -555,7 +555,7 @@ static int batch_run_form(const Env &e, glx_batch_instance *inst, int32_t n, int
         if (st != hipSuccess) return check_hip(st, "hipGraphLaunch");
     }
     for (; launched < nsteps; ++launched) {
-        hipLaunchKernelGGL((k_batch_td_step<R, G>), dim3(grid), dim3(256), 0, e.stream, B);
+        hipLaunchKernelGGL((k_batch_td_step<R, G, KB>), dim3(grid), dim3(256), 0, e.stream, B);
         GLX_LAUNCH_CHECK("k_batch_td_step");
     }
     st = hipEventRecord(e.kev1, e.stream);
@@ -592,8 +592,10 @@ static int batch_run_form(const Env &e, glx_batch_instance *inst, int32_t n, int
 template <typename R>
 int L<R>::batch_td_run(const Env &e, glx_batch_instance *inst, int32_t n, int32_t nsteps, int32_t *sweeps,
                        int32_t use_graph, int32_t wg_per_cu, int32_t min_rows, const double *weight,
-                       const Workspace &ws) {
+                       int32_t psi_depth, const Workspace &ws) {
     if (!inst || n < 1 || nsteps < 0) return set_error(GLX_E_ARG, "%s: invalid arguments", ws.what);
+    if (psi_depth == 0) psi_depth = 3;
+    if (psi_depth < 2 || psi_depth > 4) return set_error(GLX_E_ARG, "%s: psi_depth %d not in 2..4", ws.what, psi_depth);
     if ((long)n * nsteps > (1L << 26)) return set_error(GLX_E_ARG, "%s: n*nsteps too large", ws.what);
     bool any_A = false;
     for (int i = 0; i < n; ++i) {
@@ -613,9 +615,18 @@ int L<R>::batch_td_run(const Env &e, glx_batch_instance *inst, int32_t n, int32_
         any_A = any_A || p.solveA;
     }
     if (nsteps == 0) return GLX_OK;
+    auto run = [&](auto g, auto kb) {
+        return batch_run_form<R, decltype(g)::value, decltype(kb)::value>(e, inst, n, nsteps, sweeps, use_graph,
+                                                                         wg_per_cu, min_rows, weight, ws);
+    };
+    auto by_depth = [&](auto g) {
+        if (psi_depth == 2) return run(g, std::integral_constant<int, 2>{});
+        if (psi_depth == 4) return run(g, std::integral_constant<int, 4>{});
+        return run(g, std::integral_constant<int, 3>{});
+    };
     if constexpr (RunForm<R>::has_gauge)
-        if (any_A) return batch_run_form<R, true>(e, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows, weight, ws);
-    return batch_run_form<R, false>(e, inst, n, nsteps, sweeps, use_graph, wg_per_cu, min_rows, weight, ws);
+        if (any_A) return by_depth(std::true_type{});
+    return by_depth(std::false_type{});
 }
 
 // ---- batched CG -----------------------------------------------------------------

@@ -1466,12 +1466,13 @@ __host__ __device__ inline int cnt_xbuf(int c) { return (c >> 13) & 0x7ff; }
 // result and the sweep count are those of sweep-by-sweep iteration. The
 // first pass is shortened to pred mod KMAX sweeps (pred: the previous step's
 // count) so that the full passes end where the solve is likely to end.
-template <typename R, bool GAUGE, bool SH = false>
+template <typename R, bool GAUGE, bool SH = false, int KB = 0>
 __device__ __forceinline__ SolveOut psi_solve(const TDSolveArgs<R> &T, const GridSync &gs, const Team &tm, long w,
                                               long W, int pc, int qc, int pred, unsigned long long &epoch, int *lds) {
-    // (the batched kernel, SH, keeps three-sweep passes: its teams are small
-    // and a fourth stage's wider halo costs more than it saves there)
-    constexpr int KMAX = SH ? 3 : Fuse<R>::PSI;
+    // KB > 0: passes of at most KB sweeps (the batched kernel's tile depth,
+    // picked by its autotuner; strips are 64 - 2 KB columns wide)
+    constexpr int KMAX = KB > 0 ? KB : (SH ? 3 : Fuse<R>::PSI);
+    static_assert(KMAX >= 1 && KMAX <= 4, "one to four sweeps per pass");
     const int o1 = (pc + 1) % 3, o2 = (pc + 2) % 3;
     PsiSweepArgs<R> A = T.psi;
     A.ab = pick3(T.Q, qc);
@@ -1694,7 +1695,8 @@ template <typename R> struct BatchArgs {
     int n;
 };
 
-template <typename R, bool GAUGE>
+// KB: sweeps per psi pass (2, 3 or 4).
+template <typename R, bool GAUGE, int KB = 3>
 __global__ void __launch_bounds__(256) k_batch_td_step(BatchArgs<R> B) {
     __shared__ int lds[16];
     const int inst = B.team_of_block[blockIdx.x];
@@ -1708,7 +1710,7 @@ __global__ void __launch_bounds__(256) k_batch_td_step(BatchArgs<R> B) {
     unsigned long long epoch = *state64(st, ST_EPOCH);
     const int step = st[ST_STEP];
     const int qc = st[ST_AB];
-    const SolveOut o = psi_solve<R, GAUGE, true>(T, B.gs, tm, w, W, st[ST_PSI], qc, st[ST_PRED_PSI], epoch, lds);
+    const SolveOut o = psi_solve<R, GAUGE, true, KB>(T, B.gs, tm, w, W, st[ST_PSI], qc, st[ST_PRED_PSI], epoch, lds);
     SolveOut oa{qc, 0, o.ok, 0};
     if (o.ok && T.solveA) oa = a_solve<R, GAUGE, true>(T, B.gs, tm, w, W, o.cur, qc, st[ST_PRED_A], epoch, lds);
     if ((int)blockIdx.x == tb.x && threadIdx.x == 0) {

@@ -89,12 +89,15 @@ def test_batch_equals_single_runs_and_continues():
     sims = [make(k, *c) for k, c in enumerate(CASES)]
     alone = [make(k, *c) for k, c in enumerate(CASES)]
     b = BatchTD(sims, use_graph=True, autotune=True, tune_steps=1)
-    b.td(dt=0.1, Nt=7)
+    n1 = b.tune_need() + 1
+    b.td(dt=0.1, Nt=n1)
     assert b.weights is not None
+    # the tuner timed every pass depth (strip width) and several shapes
+    assert {k[2] for k in b.timings} == {2, 3, 4} and len(b.timings) >= 8
     b.td(dt=0.1, Nt=3)
-    assert b.shape is not None
+    assert b.shape in b.timings
     for gl, g1 in zip(sims, alone):
-        g1.solve.td(dt=0.1, Nt=7)
+        g1.solve.td(dt=0.1, Nt=n1)
         g1.solve.td(dt=0.1, Nt=3)
         assert gl.solve._td.sweeps == g1.solve._td.sweeps
         assert np.array_equal(gl.vars.order_parameter, g1.vars.order_parameter)
