@@ -1383,13 +1383,13 @@ __device__ __forceinline__ double cell_winding(const Geo &g, double dxdyH, doubl
 // integer (vortex_detector.py:67-72).
 __device__ __forceinline__ bool winding_flag(double v) { return fabs(v) > 0.5 && fabs(v - rint(v)) < 0.1; }
 
-// Vortex cells in three launches, none with an atomic: (1) k_vortex_flags
+// Vortex cells in two launches, neither with an atomic: (1) k_vortex_flags
 // evaluates every cell and stores, per strip row, the 64-bit mask of the
-// flagged cells (at s Nyc + j), and per wave their number; (2) k_vortex_scan turns the
-// per-wave numbers into the first record slot of each wave and the total;
-// (3) k_vortex_emit walks the masks again and writes a record for each set
-// bit, re-evaluating only those cells (the same expression on the same
-// values, so the same v). Records come out in strip, row, column order.
+// flagged cells (at s Nyc + j), and per wave their number; (2) k_vortex_emit
+// takes each wave's first record slot from those numbers, walks the masks
+// again and writes a record for each set bit, re-evaluating only those cells
+// (the same expression on the same values, so the same v). Records come out
+// in strip, row, column order.
 //
 // Cells are cut into strips of 63 columns; in (1) a wave marches down a
 // range of cell rows of one strip, lane l holding node column i = 63 s + l
@@ -1400,7 +1400,7 @@ __device__ __forceinline__ bool winding_flag(double v) { return fabs(v) > 0.5 &&
 // and b rows j <= Ny - 2 since j < Nyc; columns past the grid are clamped
 // to the last one for their loads and never evaluate a cell.
 constexpr int VSTRIP = WAVE - 1;
-constexpr long VORTEX_MAX_WAVES = 32768;  // per-wave counts and slots fit the partials buffer
+constexpr long VORTEX_MAX_WAVES = 32768;  // per-wave and per-workgroup counts fit the partials buffer
 
 inline long vortex_rows(const Geo &g) { return (long)((g.Nxc + VSTRIP - 1) / VSTRIP) * g.Nyc; }
 
@@ -1431,22 +1431,51 @@ template <typename F> __device__ __forceinline__ void vortex_items(const Geo &g,
 // VROWS rows.
 constexpr int VROWS = 8;
 
+// The flag test in FP32 with a guard band. Phases are FP32 atan2f of the
+// node values, each np.mod term a float remainder, and v is summed in
+// float: against the FP64 evaluation each term is off by a few 1e-6 rad at
+// most while every link term |dx a| stays below 1e3 (beyond, the cell is
+// decided in FP64), v by about 1e-6. A cell whose
+// terms all lie more than VGUARD from the wrap point of np.mod (0 and 2 pi)
+// and whose |v| and |v - round(v)| lie more than VGUARD from the thresholds
+// 0.5 and 0.1 gets the same flag in FP64; every other cell, and every cell
+// with a corner whose magnitude float cannot resolve (nonzero below 1e-30
+// or above 1e30), is decided by the FP64 expression itself.
+constexpr float VGUARD = 1e-3f;
+
+template <typename R> __device__ __forceinline__ bool phase_unresolved(const cplx<R> &p) {
+    const R m = fmax(fabs(p.re), fabs(p.im));
+    return m != R(0) && !(m >= R(1e-30) && m <= R(1e30));
+}
+
+__device__ __forceinline__ float wrap_2pi_guarded(float x, bool &amb) {
+    const float two_pi = 6.28318530717958647692f;
+    const float m = x - two_pi * floorf(x * 0.159154943091895335769f);
+    amb = amb || !(m > VGUARD && m < two_pi - VGUARD);
+    return m;
+}
+
 template <typename R>
 __global__ void __launch_bounds__(256) k_vortex_flags(Geo g, const cplx<R> *psi, const R *ab, double H,
-                                                      unsigned long long *bits, int *wcount, long W) {
+                                                      unsigned long long *bits, int *wcount, int *gcount, long W) {
+    __shared__ int wsum[256 / WAVE];
     const double dxdyH = g.dx * g.dy * H;
-    const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
-    if (w >= W) return;  // wave-uniform
+    const float pi_f = 3.14159265358979323846f;
+    const float dxf = float(g.dx), dyf = float(g.dy), dxdyHf = float(dxdyH);
+    const int L = xcd_block();
+    const long w = (long)L * (blockDim.x / WAVE) + wave_id();
     const int lane = threadIdx.x & (WAVE - 1);
     const R *bl = ab + g.Na;
     int cnt = 0;
+    if (w < W)  // wave-uniform
     vortex_items(g, w, W, [&](int s, int ja, int jb) {
         const int i = s * VSTRIP + lane;
         const bool col = i < g.Nx, cell = lane < VSTRIP && i < g.Nxc;
         const int ic = min(i, g.Nx - 1), iac = min(i, g.Nxa - 1);
-        const cplx<R> p0 = ld(psi, ic + (long)g.Nx * ja);
-        double t0 = col ? atan2(double(p0.im), double(p0.re)) : 0.0;
-        double a0 = double(ab[iac + (long)g.Nxa * ja]);
+        cplx<R> p0 = ld(psi, ic + (long)g.Nx * ja);
+        float f0 = col ? atan2f(float(p0.im), float(p0.re)) : 0.f;
+        bool u0 = col && phase_unresolved(p0);
+        R a0 = ab[iac + (long)g.Nxa * ja];
         for (int j = ja; j < jb; j += VROWS) {
             // rows j + k (clamped to jb - 1 for the loads: node row <= Nyc,
             // b row <= Nyc - 1)
@@ -1459,57 +1488,63 @@ __global__ void __launch_bounds__(256) k_vortex_flags(Geo g, const cplx<R> *psi,
                 a1[k] = ab[iac + (long)g.Nxa * (jj + 1)];
                 b0[k] = bl[ic + (long)g.Nxb * jj];
             }
-            double t1[VROWS];
+            float f1[VROWS];
 #pragma unroll
-            for (int k = 0; k < VROWS; ++k) t1[k] = col ? atan2(double(p1[k].im), double(p1[k].re)) : 0.0;
+            for (int k = 0; k < VROWS; ++k) f1[k] = col ? atan2f(float(p1[k].im), float(p1[k].re)) : 0.f;
 #pragma unroll
             for (int k = 0; k < VROWS; ++k) {
                 if (j + k < jb) {  // wave-uniform
-                    const double a1k = double(a1[k]), b0k = double(b0[k]);
-                    const double t0r = shfl_dn1(t0), t1r = shfl_dn1(t1[k]), b0r = shfl_dn1(b0k);
+                    const bool u1 = col && phase_unresolved(p1[k]);
+                    const float f0r = shfl_dn1(f0), f1r = shfl_dn1(f1[k]);
+                    const R b0r = shfl_dn1(b0[k]);
+                    const bool ur = shfl_dn1((int)(u0 || u1)) != 0;
+                    const R p0r_re = shfl_dn1(p0.re), p0r_im = shfl_dn1(p0.im);
+                    const R p1r_re = shfl_dn1(p1[k].re), p1r_im = shfl_dn1(p1[k].im);
                     bool flag = false;
-                    if (cell) flag = winding_flag(cell_winding(g, dxdyH, t0, t0r, t1[k], t1r, a0, a1k, b0k, b0r));
+                    if (cell) {
+                        bool amb = u0 || u1 || ur;
+                        const float sum = wrap_2pi_guarded(f0r - f0 - dxf * float(a0) + pi_f, amb) +
+                                          wrap_2pi_guarded(f1r - f0r - dyf * float(b0r) + pi_f, amb) +
+                                          wrap_2pi_guarded(f1[k] - f1r + dxf * float(a1[k]) + pi_f, amb) +
+                                          wrap_2pi_guarded(f0 - f1[k] + dyf * float(b0[k]) + pi_f, amb);
+                        // link terms beyond 1e3 rad carry float errors near the guard band
+                        amb = amb || !(fmaxf(fmaxf(fabsf(dxf * float(a0)), fabsf(dyf * float(b0r))),
+                                             fmaxf(fabsf(dxf * float(a1[k])), fabsf(dyf * float(b0[k])))) < 1e3f);
+                        const float v = -(0.5f / pi_f) * (sum - 4.0f * pi_f + dxdyHf);
+                        const float av = fabsf(v), dv = fabsf(v - rintf(v));
+                        amb = amb || !(fabsf(av - 0.5f) > VGUARD) || !(fabsf(dv - 0.1f) > VGUARD);
+                        if (!amb) {
+                            flag = av > 0.5f && dv < 0.1f;
+                        } else {  // the exact test, as k_vortex_emit evaluates it
+                            const double t00 = atan2(double(p0.im), double(p0.re));
+                            const double tp0 = atan2(double(p0r_im), double(p0r_re));
+                            const double t0p = atan2(double(p1[k].im), double(p1[k].re));
+                            const double tpp = atan2(double(p1r_im), double(p1r_re));
+                            flag = winding_flag(cell_winding(g, dxdyH, t00, tp0, t0p, tpp, double(a0), double(a1[k]),
+                                                             double(b0[k]), double(b0r)));
+                        }
+                    }
                     const unsigned long long m = __ballot(flag);
                     if (lane == 0) bits[(long)s * g.Nyc + j + k] = m;
                     cnt += (int)__popcll(m);
-                    t0 = t1[k];
-                    a0 = a1k;
+                    p0 = p1[k];
+                    f0 = f1[k];
+                    u0 = u1;
+                    a0 = a1[k];
                 }
             }
         }
     });
-    if (lane == 0) wcount[w] = cnt;
-}
-
-// Exclusive prefix sum of the W <= 32768 per-wave counts into slot (one
-// workgroup of 1024 threads, each scanning a contiguous chunk of at most
-// 32), the total into *total.
-static __global__ void __launch_bounds__(1024) k_vortex_scan(const int *wcount, int *slot, long W, int *total) {
-    __shared__ long part[1024];
-    const int t = threadIdx.x;
-    const long C = (W + 1023) / 1024;
-    const long k0 = (long)t * C;
-    int v[32];
-#pragma unroll
-    for (int k = 0; k < 32; ++k) v[k] = (k < C && k0 + k < W) ? wcount[k0 + k] : 0;
-    long sum = 0;
-#pragma unroll
-    for (int k = 0; k < 32; ++k) sum += v[k];
-    part[t] = sum;
+    if (lane == 0) {
+        if (w < W) wcount[w] = cnt;
+        wsum[wave_id()] = cnt;
+    }
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
-        const long x = t >= off ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += x;
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < (int)(blockDim.x / WAVE); ++k) t += wsum[k];
+        gcount[L] = t;
     }
-    long run = part[t] - sum;
-#pragma unroll
-    for (int k = 0; k < 32; ++k) {
-        if (k < C && k0 + k < W) slot[k0 + k] = (int)min(run, (long)INT_MAX);
-        run += v[k];
-    }
-    if (t == 1023) *total = (int)min(part[1023], (long)INT_MAX);
 }
 
 // Exclusive prefix sum of v over the lanes of a wave.
@@ -1545,16 +1580,42 @@ __device__ __forceinline__ int nth_set_bit(unsigned long long m, int n) {
 // Records of the flagged cells: (cell n, v, psi at the four corners
 // counter-clockwise from (i, j), a(i, j), a(i, j+1), b(i, j), b(i+1, j))
 // with abi added to the links when given. Slots past `capacity` are not
-// written.
+// written. Each workgroup finds its first slot itself, as the sum of the
+// counts of the workgroups before its own (gcount, one per workgroup of
+// the flag launch, which has the same grid; the first workgroup sums them
+// all and stores the total), so no separate scan launch is needed: the
+// counts are a few KB to 32 KB and stay in the caches.
 template <typename R>
 __global__ void __launch_bounds__(256) k_vortex_emit(Geo g, const cplx<R> *psi, const R *ab, const R *abi, double H,
-                                                     const unsigned long long *bits, const int *slot, double *rec,
-                                                     int capacity, long W) {
+                                                     const unsigned long long *bits, const int *wcount,
+                                                     const int *gcount, int *total,
+                                                     double *rec, int capacity, long W) {
+    __shared__ double lds[RED_BLOCK / WAVE];
+    __shared__ long s_base;
     const double dxdyH = g.dx * g.dy * H;
-    const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
+    const int L = xcd_block();
+    const long w0 = (long)L * (blockDim.x / WAVE);
+    {
+        const long lim = L == 0 ? (long)gridDim.x : (long)L;
+        long part = 0;
+        const int4 *gc4 = reinterpret_cast<const int4 *>(gcount);  // gcount is 16-byte aligned
+        for (long k = threadIdx.x; k < lim / 4; k += blockDim.x) {
+            const int4 q = gc4[k];
+            part += (long)q.x + q.y + q.z + q.w;
+        }
+        for (long k = (lim & ~3L) + threadIdx.x; k < lim; k += blockDim.x) part += gcount[k];
+        const double sum = block_sum(double(part), lds);  // integers below 2^53: exact
+        if (threadIdx.x == 0) {
+            if (L == 0) *total = (int)fmin(sum, double(INT_MAX));
+            s_base = L == 0 ? 0 : (long)sum;
+        }
+        __syncthreads();
+    }
+    const long w = w0 + wave_id();
     if (w >= W) return;  // wave-uniform
     const int lane = threadIdx.x & (WAVE - 1);
-    long base = slot[w];
+    long base = s_base;
+    for (long k = w0; k < w; ++k) base += wcount[k];  // the waves before this one in the workgroup
     vortex_items(g, w, W, [&](int s, int ja, int jb) {
         const long r0 = (long)s * g.Nyc + ja, r1 = (long)s * g.Nyc + jb;
         for (long rc = r0; rc < r1; rc += WAVE) {

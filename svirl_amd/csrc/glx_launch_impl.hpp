@@ -931,16 +931,15 @@ int L<R>::vortex_cells(const Env &e, glx_grid g, const void *psi, const void *ab
     void *bits = nullptr;
     int rc = grow_workspace(e, sizeof(unsigned long long) * (size_t)T, &bits);
     if (rc) return rc;
-    int *wcount = (int *)e.partials, *slot = wcount + VORTEX_MAX_WAVES;
+    int *wcount = (int *)e.partials, *gcount = wcount + VORTEX_MAX_WAVES;
     int *dcount = (int *)e.result;
     const unsigned nb = (unsigned)((W + 3) / 4);
     hipLaunchKernelGGL(k_vortex_flags<R>, dim3(nb), dim3(256), 0, e.stream, G, (const cplx<R> *)psi, (const R *)ab,
-                       double(R(H)), (unsigned long long *)bits, wcount, W);
+                       double(R(H)), (unsigned long long *)bits, wcount, gcount, W);
     GLX_LAUNCH_CHECK("k_vortex_flags");
-    hipLaunchKernelGGL(k_vortex_scan, dim3(1), dim3(1024), 0, e.stream, (const int *)wcount, slot, W, dcount);
-    GLX_LAUNCH_CHECK("k_vortex_scan");
     hipLaunchKernelGGL(k_vortex_emit<R>, dim3(nb), dim3(256), 0, e.stream, G, (const cplx<R> *)psi, (const R *)ab,
-                       (const R *)abi, double(R(H)), (const unsigned long long *)bits, (const int *)slot, rec,
+                       (const R *)abi, double(R(H)), (const unsigned long long *)bits, (const int *)wcount,
+                       (const int *)gcount, dcount, rec,
                        (int)capacity, W);
     GLX_LAUNCH_CHECK("k_vortex_emit");
     int c = 0;
