@@ -98,8 +98,9 @@ template <> struct Pipe<float, true> { static constexpr int ROWS = 2, ROWS2 = 1,
 template <> struct Pipe<double, true> { static constexpr int ROWS = 1, ROWS2 = 1, AROWS2 = 1, AROWS3 = 1; };
 
 // Sweeps per fused psi pass in the persistent single-instance solve (FP32:
-// four, psi_rowsK<4> with two-row batches, 181 VGPRs, two workgroups per
-// CU; FP64 keeps three, psi_rows3).
+// four, psi_rowsK<4> with each row's loads issued three rows ahead, 227
+// VGPRs, two workgroups per CU; FP64 keeps three, psi_rows3, whose
+// register footprint under psi_rowsK<3> would leave one wave per SIMD).
 template <typename R> struct Fuse { static constexpr int PSI = 3; };
 template <> struct Fuse<float> { static constexpr int PSI = 4; };
 
@@ -600,11 +601,26 @@ __device__ __forceinline__ void psi_rows3(const PsiSweepArgs<R> &A, int s, int j
     }
 }
 
-// The window a later sweep keeps of its row r: its input iterate at rows
-// r-1 and r, the link factors of a(c, r), of the link to the left and of
-// b(c, r-1), and the node's right-hand side, diagonal and material test.
+// A copy of v in registers of its own. The record ring keeps a row's
+// right-hand side for K - 1 more rows; without the copy the register
+// allocator keeps it in the load's destination, so the next load of that
+// row buffer needs other registers and the loop's back edge then copies
+// loads still in flight, which waits for all of them.
+template <typename R> __device__ __forceinline__ cplx<R> own_copy(cplx<R> v) {
+    if constexpr (std::is_same<R, float>::value) {
+        cplx<R> o;
+        asm("v_mov_b32 %0, %1" : "=v"(o.re) : "v"(v.re));
+        asm("v_mov_b32 %0, %1" : "=v"(o.im) : "v"(v.im));
+        return o;
+    } else {
+        return v;
+    }
+}
+
+// What stage 0 hands down the stages from its row r: the link factors of
+// a(c, r), of the link to its left and of b(c, r-1), and the node's
+// right-hand side, diagonal and material test.
 template <typename R> struct PsiStage {
-    cplx<R> m, c;
     R ca, sa, cal, sal, cbd, sbd;
     cplx<R> rhs;
     R den;
@@ -619,10 +635,19 @@ template <typename R> struct PsiStage {
 // computed once per row by stage 0 and handed down the stages. Writes
 // psi^{j+K} (and, at jstep 0, the Langevin right-hand side); r[s] gets the
 // lane's largest |delta| of sweep j+s.
+//
+// The loop body takes Q = 4 rows, and everything a row leaves for later
+// rows (stage 0's record, each stage's result, psi^j, the b link factors
+// and the cell flag) sits in a ring of Q entries indexed by the row's place
+// p in the body: row p writes entry p and reads entries p-1 .. p-K+1, all
+// compile-time indices. Each value is written once where it is computed
+// and read where it is needed, so no window is copied from row to row (a
+// rolled window costs a register move per value per row).
 template <typename R, int K, bool GAUGE, bool SH = false>
 __device__ __forceinline__ void psi_rowsK(const PsiSweepArgs<R> &A, int s_, int ja, int jb, R (&r)[K]) {
     static_assert(K >= 2, "two or more fused sweeps");
-    constexpr int B = 2;
+    constexpr int Q = 4;
+    static_assert(K <= Q, "a ring entry is read at most Q - 1 rows after it is written");
     const ColCtx X = col_ctx<K>(A.Nx, s_);
     const int lane = threadIdx.x & (WAVE - 1);
     const int c = X.c;
@@ -638,28 +663,61 @@ __device__ __forceinline__ void psi_rowsK(const PsiSweepArgs<R> &A, int s_, int 
     const cplx<R> zero = {R(0), R(0)};
     const int t0 = ja > K - 1 ? ja - (K - 1) : 0;
 
-    // stage 0 window: psi^j at rows t-1, t; U of b(t-1); cell (c, t-1)
-    cplx<R> pm = zero, pc = zero;
-    R cb1 = R(1), sb1 = R(0);
-    int cell1 = 0;
-    if (X.col) {
-        pc = ld(A.psi, c + (long)Nx * t0);
-        if (t0 > 0) {
-            pm = ld(A.psi, c + (long)Nx * (t0 - 1));
-            sincos_r(link_phase<R, GAUGE>(A.dy, b[c + (long)Nx * (t0 - 1)], gb), &sb1, &cb1);
-            if (c < Nxa) cell1 = has_mt ? int(A.mt[c + (long)Nxa * (t0 - 1)]) : 1;
-        }
-    }
-    mask_link(shfl_up1(cell1) != 0 || cell1 != 0, cb1, sb1);  // b(c, t0-1)
-    // stages 1 .. K-1 (S[s-1]), empty until the pipeline fills
-    PsiStage<R> S[K - 1];
+    // rings, entry (p mod Q) written by the row at place p of the body:
+    // pj[p]: psi^j at that row (written one row earlier, as its pp);
+    // cbq/sbq/cq: its b link factors and cell flag; rec: stage 0's record;
+    // o[s]: stage s's result. The entries before the first row hold what
+    // the pipeline starts from: psi^j at rows t0 - 1 and t0, b(c, t0-1),
+    // cell (c, t0-1); empty stages.
+    cplx<R> pj[Q], o[K - 1][Q];
+    R cbq[Q], sbq[Q];
+    int cq[Q];
+    PsiStage<R> rec[Q];
 #pragma unroll
-    for (int k = 0; k < K - 1; ++k)
-        S[k] = PsiStage<R>{zero, zero, R(0), R(0), R(0), R(0), R(0), R(0), zero, R(1), false};
+    for (int k = 0; k < Q; ++k) {
+        pj[k] = zero;
+        cbq[k] = R(0);
+        sbq[k] = R(0);
+        cq[k] = 0;
+        rec[k] = PsiStage<R>{R(0), R(0), R(0), R(0), R(0), R(0), zero, R(1), false};
+#pragma unroll
+        for (int st_ = 0; st_ < K - 1; ++st_) o[st_][k] = zero;
+    }
+    {
+        cplx<R> pm = zero, pc = zero;
+        R cb1 = R(1), sb1 = R(0);
+        int cell1 = 0;
+        if (X.col) {
+            pc = ld(A.psi, c + (long)Nx * t0);
+            if (t0 > 0) {
+                pm = ld(A.psi, c + (long)Nx * (t0 - 1));
+                sincos_r(link_phase<R, GAUGE>(A.dy, b[c + (long)Nx * (t0 - 1)], gb), &sb1, &cb1);
+                if (c < Nxa) cell1 = has_mt ? int(A.mt[c + (long)Nxa * (t0 - 1)]) : 1;
+            }
+        }
+        mask_link(shfl_up1(cell1) != 0 || cell1 != 0, cb1, sb1);  // b(c, t0-1)
+        pj[Q - 1] = pm;
+        pj[0] = pc;
+        cbq[Q - 1] = cb1;
+        sbq[Q - 1] = sb1;
+        cq[Q - 1] = cell1;
+    }
 
-    auto row = [&](const PsiRow<R> &cu, int t) {
+    // Every stage is computed on all lanes and its result selected (zero
+    // where the stage has no node), so a row is one basic block the
+    // scheduler can interleave across stages; only the last stage's store
+    // stays under a lane condition. The Langevin variant (NZ: the first
+    // sweep of a step with a right-hand side to record) is a second copy of
+    // the loop, so the common one carries no branch for it.
+    auto row = [&](const PsiRow<R> &cu, int t, auto place, auto nz) {
+        constexpr int p = decltype(place)::value;
+        constexpr bool NZ = decltype(nz)::value;
+        constexpr int p1 = (p + Q - 1) % Q, pn = (p + 1) % Q;
         const long rowoff = (long)Nx * t;
         const bool real = t < Ny;  // t >= Ny: past the last row (later stages only)
+        const cplx<R> pc = pj[p], pm = pj[p1];
+        const R cb1 = cbq[p1], sb1 = sbq[p1];
+        const int cell1 = cq[p1];
         const cplx<R> pp = (X.col && t + 1 < Ny) ? cu.pp : zero;
         const R av = X.hasa ? cu.av : R(0);
         const R bv = (X.col && t < Ny - 1) ? cu.bv : R(0);
@@ -679,100 +737,100 @@ __device__ __forceinline__ void psi_rowsK(const PsiSweepArgs<R> &A, int s_, int 
 
         cplx<R> out[K];  // out[s]: sweep j+s's result at row t-s
         // stage 0: sweep j at row t
-        R den = R(1);
+        R den;
         {
-            cplx<R> q = zero;
-            const R pl_re = shfl_up1(pc.re), pl_im = shfl_up1(pc.im);
-            const R pr_re = shfl_dn1(pc.re), pr_im = shfl_dn1(pc.im);
-            if (X.col && lane >= 1 && lane < WAVE - 1 && real && inside) {
-                if (noise_on) psi_noise(A, (uint32_t)(c + rowoff), rhs);
-                den = psi_den(A, rhs, e, mm, mp, pmc, ppc);
-                q = psi_apply(A, rhs, den, pl_re, pl_im, pr_re, pr_im, pm, pp, cal, sal, ca, sa, cb1, sb1, cb, sb);
+            const bool act = X.col && lane >= 1 && lane < WAVE - 1 && real && inside;
+            if constexpr (NZ) {
+                if (noise_on) {
+                    cplx<R> nr = rhs;
+                    psi_noise(A, (uint32_t)(c + rowoff), nr);
+                    rhs = act ? nr : rhs;
+                }
             }
-            if (own && real && t >= ja && t < jb) {
-                if (noisy && A.jstep == 0) st(A.rhs_out, c + rowoff, rhs);
-                r[0] = fmax(r[0], fmax(fabs(q.re - pc.re), fabs(q.im - pc.im)));
+            const R d = psi_den(A, rhs, e, mm, mp, pmc, ppc);
+            const cplx<R> qa = psi_apply(A, rhs, d, shfl_up1(pc.re), shfl_up1(pc.im), shfl_dn1(pc.re),
+                                         shfl_dn1(pc.im), pm, pp, cal, sal, ca, sa, cb1, sb1, cb, sb);
+            den = act ? d : R(1);
+            const cplx<R> q = act ? qa : zero;
+            const bool res = own && real && t >= ja && t < jb;
+            if constexpr (NZ) {
+                if (res) st(A.rhs_out, c + rowoff, rhs);
             }
+            r[0] = res ? fmax(r[0], fmax(fabs(q.re - pc.re), fabs(q.im - pc.im))) : r[0];
             out[0] = q;
         }
-        // stages 1 .. K-1: sweep j+s at row t-s; its up neighbour is stage
-        // s-1's result of this row, its up link stage s-1's down link
+        // stage 0's record of row t, for stage s at row t + s
+        rec[p] = PsiStage<R>{ca, sa, cal, sal, cb1, sb1, own_copy(rhs), den, inside};
+        // stages 1 .. K-1: sweep j+s at row t-s, from stage s-1's results of
+        // rows t-s-1 and t-s (computed at rows t-2 and t-1) and of row t-s+1
+        // (this row); its down link is row t-s's b(c, t-s-1), its up link
+        // row t-s+1's
 #pragma unroll
         for (int st_ = 1; st_ < K; ++st_) {
-            const PsiStage<R> &W = S[st_ - 1];
+            const PsiStage<R> &W = rec[(p + Q - st_) % Q];
+            const R ucb = rec[(p + Q - st_ + 1) % Q].cbd, usb = rec[(p + Q - st_ + 1) % Q].sbd;
+            const cplx<R> wc = o[st_ - 1][p1], wm = o[st_ - 1][(p + Q - 2) % Q];
             const int rs = t - st_;
-            const R ucb = st_ == 1 ? cb1 : S[st_ - 2].cbd, usb = st_ == 1 ? sb1 : S[st_ - 2].sbd;
-            const R pl_re = shfl_up1(W.c.re), pl_im = shfl_up1(W.c.im);
-            const R pr_re = shfl_dn1(W.c.re), pr_im = shfl_dn1(W.c.im);
-            cplx<R> w = zero;
+            const cplx<R> wa = psi_apply(A, W.rhs, W.den, shfl_up1(wc.re), shfl_up1(wc.im), shfl_dn1(wc.re),
+                                         shfl_dn1(wc.im), wm, out[st_ - 1], W.cal, W.sal, W.ca, W.sa, W.cbd, W.sbd,
+                                         ucb, usb);
+            const bool res = own && rs >= ja && rs < jb;  // rows of [ja, jb) lie in [0, Ny)
+            cplx<R> w;
             if (st_ < K - 1) {
-                if (rs >= 0 && rs < Ny) {
-                    if (X.col && lane >= st_ + 1 && lane < WAVE - 1 - st_ && W.in)
-                        w = psi_apply(A, W.rhs, W.den, pl_re, pl_im, pr_re, pr_im, W.m, out[st_ - 1], W.cal, W.sal,
-                                      W.ca, W.sa, W.cbd, W.sbd, ucb, usb);
-                    if (own && rs >= ja && rs < jb) r[st_] = fmax(r[st_], fmax(fabs(w.re - W.c.re), fabs(w.im - W.c.im)));
-                }
-            } else if (rs >= ja && own) {
-                if (W.in)
-                    w = psi_apply(A, W.rhs, W.den, pl_re, pl_im, pr_re, pr_im, W.m, out[st_ - 1], W.cal, W.sal, W.ca,
-                                  W.sa, W.cbd, W.sbd, ucb, usb);
-                st_at(A.psi_next + (size_t)Nx * rs, 2 * X.cc * (uint32_t)sizeof(R), w);
-                r[st_] = fmax(r[st_], fmax(fabs(w.re - W.c.re), fabs(w.im - W.c.im)));
+                const bool act = rs >= 0 && rs < Ny && X.col && lane >= st_ + 1 && lane < WAVE - 1 - st_ && W.in;
+                w = act ? wa : zero;
+            } else {
+                w = W.in ? wa : zero;
+                if (res) st_at(A.psi_next + (size_t)Nx * rs, 2 * X.cc * (uint32_t)sizeof(R), w);
             }
+            r[st_] = res ? fmax(r[st_], fmax(fabs(w.re - wc.re), fabs(w.im - wc.im))) : r[st_];
             out[st_] = w;
         }
-        // roll the windows, the last stage first
 #pragma unroll
-        for (int st_ = K - 1; st_ >= 1; --st_) {
-            PsiStage<R> &W = S[st_ - 1];
-            W.m = W.c;
-            W.c = out[st_ - 1];
-            if (st_ >= 2) {
-                const PsiStage<R> &P = S[st_ - 2];
-                W.ca = P.ca;
-                W.sa = P.sa;
-                W.cal = P.cal;
-                W.sal = P.sal;
-                W.cbd = P.cbd;
-                W.sbd = P.sbd;
-                W.rhs = P.rhs;
-                W.den = P.den;
-                W.in = P.in;
-            } else {
-                W.ca = ca;
-                W.sa = sa;
-                W.cal = cal;
-                W.sal = sal;
-                W.cbd = cb1;
-                W.sbd = sb1;
-                W.rhs = rhs;
-                W.den = den;
-                W.in = inside;
-            }
-        }
-        pm = pc;
-        pc = pp;
-        cb1 = cb;
-        sb1 = sb;
-        cell1 = cellc;
+        for (int st_ = 0; st_ < K - 1; ++st_) o[st_][p] = out[st_];
+        pj[pn] = pp;
+        cbq[p] = cb;
+        sbq[p] = sb;
+        cq[p] = cellc;
     };
 
+    // Q row buffers: before row t + p is evaluated, the loads of row
+    // t + p + Q - 1 go into the buffer row t + p - 1 has finished with, so
+    // each row's loads are in flight while the Q - 1 rows before it run.
+    // The body evaluates all Q rows unconditionally (up to Q - 1 rows past
+    // t1 - 1 in the last trip: their loads are clamped to row t1 - 1, and
+    // they store nothing and change no residual since every store and
+    // residual is limited to rows below jb), which keeps the loads and
+    // stores straight-line, so the memory counter waits stay exact across
+    // the loop's back edge instead of draining it.
+    using P0_ = std::integral_constant<int, 0>;
+    using P1_ = std::integral_constant<int, 1>;
+    using P2_ = std::integral_constant<int, 2>;
+    using P3_ = std::integral_constant<int, 3>;
+    static_assert(Q == 4, "the body below is written out for four rows");
     const int t1 = jb + K - 1;  // rows t in [t0, t1)
-    PsiRow<R> P0[B], P1[B];
+    auto run = [&](auto nz) {
+        PsiRow<R> P[Q];
 #pragma unroll
-    for (int k = 0; k < B; ++k) P0[k] = psi_load_row(A, X, mtp, emp, min(t0 + k, t1 - 1));
-    for (int t = t0; t < t1; t += 2 * B) {
-#pragma unroll
-        for (int k = 0; k < B; ++k) P1[k] = psi_load_row(A, X, mtp, emp, min(t + B + k, t1 - 1));
-#pragma unroll
-        for (int k = 0; k < B; ++k)
-            if (t + k < t1) row(P0[k], t + k);
-#pragma unroll
-        for (int k = 0; k < B; ++k) P0[k] = psi_load_row(A, X, mtp, emp, min(t + 2 * B + k, t1 - 1));
-#pragma unroll
-        for (int k = 0; k < B; ++k)
-            if (t + B + k < t1) row(P1[k], t + B + k);
-    }
+        for (int k = 0; k < Q - 1; ++k) {
+            P[k] = psi_load_row(A, X, mtp, emp, min(t0 + k, t1 - 1));
+            __builtin_amdgcn_sched_barrier(0);  // issue in row order, as the loop does
+        }
+        for (int t = t0; t < t1; t += Q) {
+            P[3] = psi_load_row(A, X, mtp, emp, min(t + 3, t1 - 1));
+            row(P[0], t, P0_{}, nz);
+            P[0] = psi_load_row(A, X, mtp, emp, min(t + 4, t1 - 1));
+            row(P[1], t + 1, P1_{}, nz);
+            P[1] = psi_load_row(A, X, mtp, emp, min(t + 5, t1 - 1));
+            row(P[2], t + 2, P2_{}, nz);
+            P[2] = psi_load_row(A, X, mtp, emp, min(t + 6, t1 - 1));
+            row(P[3], t + 3, P3_{}, nz);
+        }
+    };
+    if (noisy && A.jstep == 0)
+        run(std::true_type{});
+    else
+        run(std::false_type{});
 }
 
 // Raw loads of one row for the A sweep: a(c, j+1), b(c, j), psi(c, j+1), the
