@@ -291,7 +291,9 @@ __global__ void __launch_bounds__(256) k_coef_dir_rows(Geo g, R kappa2, R eps, c
     double acc[L];
 #pragma unroll
     for (int k = 0; k < L; ++k) acc[k] = 0.0;
-    double th = 0.0;
+    // the largest link phase step in the working precision: a maximum is
+    // exact in either, and widening it at the end gives the FP64 value
+    R th = R(0);
     const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
     if (w < W) for_rows_rm<1>(g.Nx, g.Ny, w, W, [&](int s, int ja, int jb) {
         const ColCtx X = col_ctx<1>(g.Nx, s);
@@ -337,6 +339,13 @@ __global__ void __launch_bounds__(256) k_coef_dir_rows(Geo g, R kappa2, R eps, c
         cplx<R> Pc = ld(psi, row(ja)), Dc = dnode(ld(U.dold, row(ja)), ld(U.jac, row(ja)));
         LinkTriple<R> Ac = dlink(link(ka_of(ja)));
         bool md = has_mt ? mt[kc_of(ja - 1)] != 0 : true;
+        // FP32: a lane's coefficients of the 2B rows of one trip are summed
+        // in FP32 (at most four terms each) and the sums added to the FP64
+        // accumulators once per trip; FP64 adds each node's terms directly
+        constexpr bool PART = std::is_same<R, float>::value;
+        R part[L];
+#pragma unroll
+        for (int k = 0; k < L; ++k) part[k] = R(0);
         auto consume = [&](const Step &t, int j) {
             const long n = (long)g.Nx * j + cc;
             const cplx<R> Pu = t.p, Du = dnode(t.o, t.gr);
@@ -362,14 +371,17 @@ __global__ void __launch_bounds__(256) k_coef_dir_rows(Geo g, R kappa2, R eps, c
                 if (L == 17) {
                     if (c < g.Nxa) U.dAout[ca + (long)g.Nxa * j] = Ac.d;
                     if (j < g.Nyb) U.dAout[g.Na + cc + (long)g.Nxb * j] = Bc.d;
-                    th = fmax(th, fmax(fabs(double(q.dx * Ac.d)), fabs(double(q.dy * Bc.d))));
+                    th = fmax(th, fmax(fabs(q.dx * Ac.d), fabs(q.dy * Bc.d)));
                 }
                 R C[L];
 #pragma unroll
                 for (int k = 0; k < L; ++k) C[k] = R(0);
                 coef_eval<R, L>(q, kappa2, H, he, hb, in, c, j, c < g.Nxc && j < g.Nyc, C);
 #pragma unroll
-                for (int k = 0; k < L; ++k) acc[k] += double(C[k]);
+                for (int k = 0; k < L; ++k) {
+                    if constexpr (PART) part[k] += C[k];
+                    else acc[k] += double(C[k]);
+                }
             }
             Pc = Pu;
             Dc = Du;
@@ -390,10 +402,17 @@ __global__ void __launch_bounds__(256) k_coef_dir_rows(Geo g, R kappa2, R eps, c
 #pragma unroll
             for (int k = 0; k < B; ++k)
                 if (j + B + k < jb) consume(S1[k], j + B + k);
+            if constexpr (PART) {
+#pragma unroll
+                for (int k = 0; k < L; ++k) {
+                    acc[k] += double(part[k]);
+                    part[k] = R(0);
+                }
+            }
         }
     });
     block_store_partials<L>(acc, partials);
-    if (L == 17) block_store_phase_max(th, partials, L);
+    if (L == 17) block_store_phase_max(double(th), partials, L);
 }
 
 }  // namespace glx

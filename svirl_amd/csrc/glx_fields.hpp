@@ -1104,6 +1104,14 @@ __device__ __forceinline__ cplx<R> dir_at(const DirUpdate<R> &U, long k) {
     return {U.bp * o.re - gk.re, U.bp * o.im - gk.im};
 }
 
+// x / N for the coefficients' constant divisors: FP64 divides as the
+// reference does; FP32 multiplies by the rounded reciprocal (within an ulp
+// of the quotient, against about ten instructions for an IEEE division).
+template <typename R, int N> __device__ __forceinline__ R div_const(R x) {
+    if constexpr (std::is_same<R, float>::value) return x * (1.0f / float(N));
+    else return x / R(N);
+}
+
 // Everything coef_eval reads of node (i, j): psi and the direction at the
 // node and its right and up neighbours, epsilon, the four cell flags, and
 // the four link values a(i, j), a(i, j+1), b(i, j), b(i+1, j) of abei (E),
@@ -1192,8 +1200,8 @@ __device__ __forceinline__ void coef_eval(const GeoR<R> &q, R kappa2, R H, bool 
                 for (int m = 0; m < 3; ++m) {
                     C[5 * m + 1] += w * R(2) * zz[m].im * dph;
                     C[5 * m + 2] += w * zz[m].re * dph2;
-                    C[5 * m + 3] += -r * id2 / R(3) * zz[m].im * dph2 * dph;
-                    C[5 * m + 4] += -r * id2 / R(12) * zz[m].re * dph2 * dph2;
+                    C[5 * m + 3] += div_const<R, 3>(-r * id2) * zz[m].im * dph2 * dph;
+                    C[5 * m + 4] += div_const<R, 12>(-r * id2) * zz[m].re * dph2 * dph2;
                 }
             }
         }

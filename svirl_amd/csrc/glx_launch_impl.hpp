@@ -35,10 +35,12 @@ static long resident_waves(const void *fn, int num_cus) {
 }
 
 // Waves of a row-marching pass cut to whole rounds of the chip: W is
-// lowered to a multiple of the waves the kernel keeps resident, so that no
-// round runs a fraction of the chip.
-static long fit_row_waves(const void *fn, long W, int num_cus) {
-    const long res = resident_waves(fn, num_cus);
+// lowered to a multiple of the waves the kernel keeps resident (at most
+// max_per_cu workgroups per CU when that is given), so that no round runs
+// a fraction of the chip.
+static long fit_row_waves(const void *fn, long W, int num_cus, int max_per_cu = 0) {
+    long res = resident_waves(fn, num_cus);
+    if (max_per_cu > 0) res = std::min(res, (long)max_per_cu * num_cus * (RED_BLOCK / WAVE));
     if (res <= 0 || W <= res) return W;
     return (W / res) * res;
 }
@@ -1063,7 +1065,11 @@ static int gradient_pass_launch(const Env &e, glx_grid g, double kappa2, double 
     const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
     const void *fn = withA ? (upd ? (const void *)k_cg_grad_rows<R, true, true> : (const void *)k_cg_grad_rows<R, true, false>)
                            : (upd ? (const void *)k_cg_grad_rows<R, false, true> : (const void *)k_cg_grad_rows<R, false, false>);
-    const long W = fit_row_waves(fn, cg_row_waves(g.Nx, g.Ny), e.num_cus);
+    // two workgroups per CU: with the three the gradient pass's registers
+    // allow, the rows in flight per XCD overflow its L2 before the
+    // neighbouring strips have read their halo lines (Config 3: 0.70 against
+    // 0.72 ms per CG iteration)
+    const long W = fit_row_waves(fn, cg_row_waves(g.Nx, g.Ny), e.num_cus, 2);
     const int nb = (int)((W + 3) / 4);
     *nb_out = nb;
     const R k2 = R(kappa2), ep = R(eps), h = R(H);
