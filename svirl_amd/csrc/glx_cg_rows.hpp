@@ -323,16 +323,25 @@ __global__ void __launch_bounds__(256) k_coef_dir_rows(Geo g, R kappa2, R eps, c
         auto dnode = [&](const cplx<R> &o, const cplx<R> &gk) {
             return cplx<R>{U.bp * o.re - gk.re, U.bp * o.im - gk.im};
         };
+        // each load of a step at a row pointer (uniform) plus the lane's
+        // byte offset (loop-invariant, 32-bit: grids hold at most 2^30 nodes)
+        const uint32_t o_n = cc * (uint32_t)sizeof(cplx<R>), o_a = ca * (uint32_t)sizeof(R),
+                       o_b = cc * (uint32_t)sizeof(R), o_m = ca * (uint32_t)mt_st,
+                       o_e = cc * (uint32_t)(em_st * sizeof(R));
+        auto link_at = [&](size_t rk, uint32_t o) {
+            return Lk{at(abei_s + rk, o), at(ab_s + rk, o), at(dAold_s + rk, o), at(jacA_s + rk, o)};
+        };
         auto fetch = [&](int j) {
             Step t;
-            const long k = row(j + 1);
-            t.p = ld(psi, k);
-            t.o = ld(U.dold, k);
-            t.gr = ld(U.jac, k);
-            t.a = link(ka_of(j + 1));
-            t.b = link(kb_of(j));
-            t.mc = mt_s[kc_of(j) * mt_st];
-            t.eps = em_s[row(j) * em_st];
+            const int jp = min(max(j + 1, 0), g.Ny - 1), jr = min(max(j, 0), g.Ny - 1), jq = min(max(j, 0), g.Ny - 2);
+            const size_t rn = (size_t)g.Nx * jp;
+            t.p = ld_at(psi + rn, o_n);
+            t.o = ld_at(U.dold + rn, o_n);
+            t.gr = ld_at(U.jac + rn, o_n);
+            t.a = link_at((size_t)g.Nxa * jp, o_a);
+            t.b = link_at((size_t)g.Na + (size_t)g.Nxb * jq, o_b);
+            t.mc = at(mt_s + (size_t)g.Nxc * jq * mt_st, o_m);
+            t.eps = at(em_s + (size_t)g.Nx * jr * em_st, o_e);
             return t;
         };
         // window: psi, direction and the a links at row ja; the cell below
@@ -383,9 +392,11 @@ __global__ void __launch_bounds__(256) k_coef_dir_rows(Geo g, R kappa2, R eps, c
                     else acc[k] += double(C[k]);
                 }
             }
-            Pc = Pu;
+            // values that outlive their row buffer get registers of their
+            // own (see own_copy), so the next batch can load into it
+            Pc = own_copy(Pu);
             Dc = Du;
-            Ac = Au;
+            Ac = LinkTriple<R>{own_copy(Au.e), own_copy(Au.b), Au.d};
             md = mc;
         };
         Step S0[B], S1[B];

@@ -1196,12 +1196,28 @@ __device__ __forceinline__ void coef_eval(const GeoR<R> &q, R kappa2, R H, bool 
                 const cplx<R> z1 = {za.re + zb.re, za.im + zb.im};
                 const cplx<R> z2 = mulU(d0, d1);
                 const cplx<R> zz[3] = {z0, z1, z2};
+                if constexpr (std::is_same<R, float>::value) {
+                    // FP32: the powers of the phase step once per link, then
+                    // w Im z and w Re z once per product, so each coefficient
+                    // takes one fma (the reference's per-term products
+                    // reassociated; FP64 keeps its order)
+                    const R k1 = R(2) * dph, k3 = R(-1.0f / 3.0f) * dph2 * dph, k4 = R(-1.0f / 12.0f) * dph2 * dph2;
 #pragma unroll
-                for (int m = 0; m < 3; ++m) {
-                    C[5 * m + 1] += w * R(2) * zz[m].im * dph;
-                    C[5 * m + 2] += w * zz[m].re * dph2;
-                    C[5 * m + 3] += div_const<R, 3>(-r * id2) * zz[m].im * dph2 * dph;
-                    C[5 * m + 4] += div_const<R, 12>(-r * id2) * zz[m].re * dph2 * dph2;
+                    for (int m = 0; m < 3; ++m) {
+                        const R wi = w * zz[m].im, wr = w * zz[m].re;
+                        C[5 * m + 1] = fmaf(wi, k1, C[5 * m + 1]);
+                        C[5 * m + 2] = fmaf(wr, dph2, C[5 * m + 2]);
+                        C[5 * m + 3] = fmaf(wi, k3, C[5 * m + 3]);
+                        C[5 * m + 4] = fmaf(wr, k4, C[5 * m + 4]);
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 3; ++m) {
+                        C[5 * m + 1] += w * R(2) * zz[m].im * dph;
+                        C[5 * m + 2] += w * zz[m].re * dph2;
+                        C[5 * m + 3] += div_const<R, 3>(-r * id2) * zz[m].im * dph2 * dph;
+                        C[5 * m + 4] += div_const<R, 12>(-r * id2) * zz[m].re * dph2 * dph2;
+                    }
                 }
             }
         }

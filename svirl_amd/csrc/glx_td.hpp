@@ -606,15 +606,17 @@ __device__ __forceinline__ void psi_rows3(const PsiSweepArgs<R> &A, int s, int j
 // allocator keeps it in the load's destination, so the next load of that
 // row buffer needs other registers and the loop's back edge then copies
 // loads still in flight, which waits for all of them.
-template <typename R> __device__ __forceinline__ cplx<R> own_copy(cplx<R> v) {
+template <typename R> __device__ __forceinline__ R own_copy(R v) {
     if constexpr (std::is_same<R, float>::value) {
-        cplx<R> o;
-        asm("v_mov_b32 %0, %1" : "=v"(o.re) : "v"(v.re));
-        asm("v_mov_b32 %0, %1" : "=v"(o.im) : "v"(v.im));
+        R o;
+        asm("v_mov_b32 %0, %1" : "=v"(o) : "v"(v));
         return o;
     } else {
         return v;
     }
+}
+template <typename R> __device__ __forceinline__ cplx<R> own_copy(cplx<R> v) {
+    return {own_copy(v.re), own_copy(v.im)};
 }
 
 // What stage 0 hands down the stages from its row r: the link factors of
