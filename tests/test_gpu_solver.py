@@ -149,6 +149,43 @@ def test_smallest_grids_match_oracle(shape, dtype):
     assert np.allclose(gl.solve._cg.cg_energies, st.cg_energies, rtol=1e-8 if f64 else 1e-4)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_td_langevin_matches_oracle(dtype):
+    """Langevin noise on psi and A (td.py:118-154, 256-325) through the
+    persistent solves, on a grid of several strips with holes: the first
+    pass of each psi solve draws the noise and records the noisy right-hand
+    side, the later passes read it. The first step's first pass is a full
+    pass (no earlier sweep count to shorten it), so FP32 runs the Langevin
+    copy of the four-sweep pass loop there, and the later steps' shortened
+    first passes (21-27 sweeps per step) the one-, two- and three-sweep
+    forms. Against the oracle over 4
+    steps: FP64 sweep counts exact and fields within 1e-10, FP32 within
+    float tolerance; with the noise off the fields differ."""
+    def make():
+        gl = GLSolver(Nx=130, Ny=97, dx=0.5, dy=0.5, order_parameter='random', random_seed=7, gl_parameter=5.0,
+                      normal_conductivity=100.0, homogeneous_external_field=0.1, dtype=dtype)
+        mt = np.ones((gl.cfg.Nxc, gl.cfg.Nyc), bool)
+        mt[40:52, 30:41] = False
+        gl.mesh.material_tiling = mt
+        gl.vars.set_order_parameter_to_zero_outside_material()
+        return gl
+    gl = make()
+    gl.params.order_parameter_Langevin_coefficient = 0.05
+    gl.params.vector_potential_Langevin_coefficient = 0.02
+    st = oracle_of(gl)
+    st.langevin_psi, st.langevin_A = 0.05, 0.02
+    gl.solve.td(dt=0.1, Nt=4)
+    st.td(dt=0.1, Nt=4)
+    f64 = dtype == np.float64
+    tol = 1e-10 if f64 else 1e-4
+    if f64:
+        assert gl.solve._td.sweeps == st.sweeps
+    assert rel(gl_psi(gl), st.psi) < tol and rel(gl_ab(gl), st.ab) < tol
+    quiet = make()
+    quiet.solve.td(dt=0.1, Nt=4)
+    assert rel(gl_psi(quiet), st.psi) > 100 * tol
+
+
 def test_fp32_large_grid_close_to_fp64():
     """The FP32-mixed form (gauge-shifted A, fused passes) against the FP64
     GPU form on Config 3's physics at 769 x 641 after 20 steps: free
