@@ -200,6 +200,22 @@ __device__ __forceinline__ cplx<R> psi_apply(const PsiSweepArgs<R> &A, const cpl
     else return {nre / den, nim / den};
 }
 
+// psi_apply for FP32 passes with two products folded in per row: the link
+// factors arrive scaled by dt/dx^2 (a links) and dt/dy^2 (b links), and rd
+// is rhs * den, so each component is one sum of the four link terms and
+// one fma (the same quantity as psi_apply's, rounded differently).
+template <typename R>
+__device__ __forceinline__ cplx<R> psi_apply_folded(const cplx<R> &rd, R den, R pl_re, R pl_im, R pr_re, R pr_im,
+                                                    const cplx<R> &pm, const cplx<R> &pp, R cal, R sal, R ca, R sa,
+                                                    R cbm, R sbm, R cb, R sb) {
+    const R m0r = cal * pl_re - sal * pl_im, m0i = cal * pl_im + sal * pl_re;
+    const R p0r = ca * pr_re + sa * pr_im, p0i = ca * pr_im - sa * pr_re;
+    const R q0mr = cbm * pm.re - sbm * pm.im, q0mi = cbm * pm.im + sbm * pm.re;
+    const R q0pr = cb * pp.re + sb * pp.im, q0pi = cb * pp.im - sb * pp.re;
+    const R sre = (m0r + p0r) + (q0mr + q0pr), sim = (m0i + p0i) + (q0mi + q0pi);
+    return {fma(sre, den, rd.re), fma(sim, den, rd.im)};
+}
+
 // Link factors of a link that touches no material cell become (0, 0).
 template <typename R> __device__ __forceinline__ void mask_link(bool on, R &c, R &s) {
     c = on ? c : R(0);
@@ -621,7 +637,8 @@ template <typename R> __device__ __forceinline__ cplx<R> own_copy(cplx<R> v) {
 
 // What stage 0 hands down the stages from its row r: the link factors of
 // a(c, r), of the link to its left and of b(c, r-1), and the node's
-// right-hand side, diagonal and material test.
+// right-hand side (FP32: rhs * den, psi_apply_folded), diagonal and
+// material test.
 template <typename R> struct PsiStage {
     R ca, sa, cal, sal, cbd, sbd;
     cplx<R> rhs;
@@ -664,6 +681,10 @@ __device__ __forceinline__ void psi_rowsK(const PsiSweepArgs<R> &A, int s_, int 
     const R gb = GAUGE ? gauge_phase_b<R>(A, c) : R(0);
     const cplx<R> zero = {R(0), R(0)};
     const int t0 = ja > K - 1 ? ja - (K - 1) : 0;
+    // FP32 folds dt/h^2 into the link factors and rhs * den into one value
+    // per row (psi_apply_folded); FP64 keeps the reference's operation order
+    constexpr bool FOLD = std::is_same<R, float>::value;
+    const R kxa = FOLD ? A.dt * A.idx2 : R(1), kyb = FOLD ? A.dt * A.idy2 : R(1);
 
     // rings, entry (p mod Q) written by the row at place p of the body:
     // pj[p]: psi^j at that row (written one row earlier, as its pp);
@@ -698,6 +719,10 @@ __device__ __forceinline__ void psi_rowsK(const PsiSweepArgs<R> &A, int s_, int 
             }
         }
         mask_link(shfl_up1(cell1) != 0 || cell1 != 0, cb1, sb1);  // b(c, t0-1)
+        if constexpr (FOLD) {
+            cb1 *= kyb;
+            sb1 *= kyb;
+        }
         pj[Q - 1] = pm;
         pj[0] = pc;
         cbq[Q - 1] = cb1;
@@ -734,12 +759,19 @@ __device__ __forceinline__ void psi_rowsK(const PsiSweepArgs<R> &A, int s_, int 
         const bool pmc = cell1 != 0, ppc = cellc != 0;
         mask_link(pmc || ppc, ca, sa);  // a(c, t): cells (c, t-1), (c, t)
         mask_link(mp || ppc, cb, sb);   // b(c, t): cells (c-1, t), (c, t)
+        if constexpr (FOLD) {
+            ca *= kxa;
+            sa *= kxa;
+            cb *= kyb;
+            sb *= kyb;
+        }
         const R cal = shfl_up1(ca), sal = shfl_up1(sa);
         const bool inside = mm || mp || pmc || ppc;
 
         cplx<R> out[K];  // out[s]: sweep j+s's result at row t-s
         // stage 0: sweep j at row t
         R den;
+        cplx<R> rd;  // what the later stages take of the right-hand side
         {
             const bool act = X.col && lane >= 1 && lane < WAVE - 1 && real && inside;
             if constexpr (NZ) {
@@ -750,9 +782,17 @@ __device__ __forceinline__ void psi_rowsK(const PsiSweepArgs<R> &A, int s_, int 
                 }
             }
             const R d = psi_den(A, rhs, e, mm, mp, pmc, ppc);
-            const cplx<R> qa = psi_apply(A, rhs, d, shfl_up1(pc.re), shfl_up1(pc.im), shfl_dn1(pc.re),
-                                         shfl_dn1(pc.im), pm, pp, cal, sal, ca, sa, cb1, sb1, cb, sb);
             den = act ? d : R(1);
+            cplx<R> qa;
+            if constexpr (FOLD) {
+                rd = {rhs.re * den, rhs.im * den};
+                qa = psi_apply_folded(rd, den, shfl_up1(pc.re), shfl_up1(pc.im), shfl_dn1(pc.re), shfl_dn1(pc.im), pm,
+                                      pp, cal, sal, ca, sa, cb1, sb1, cb, sb);
+            } else {
+                rd = own_copy(rhs);
+                qa = psi_apply(A, rhs, d, shfl_up1(pc.re), shfl_up1(pc.im), shfl_dn1(pc.re), shfl_dn1(pc.im), pm, pp,
+                               cal, sal, ca, sa, cb1, sb1, cb, sb);
+            }
             const cplx<R> q = act ? qa : zero;
             const bool res = own && real && t >= ja && t < jb;
             if constexpr (NZ) {
@@ -762,7 +802,7 @@ __device__ __forceinline__ void psi_rowsK(const PsiSweepArgs<R> &A, int s_, int 
             out[0] = q;
         }
         // stage 0's record of row t, for stage s at row t + s
-        rec[p] = PsiStage<R>{ca, sa, cal, sal, cb1, sb1, own_copy(rhs), den, inside};
+        rec[p] = PsiStage<R>{ca, sa, cal, sal, cb1, sb1, rd, den, inside};
         // stages 1 .. K-1: sweep j+s at row t-s, from stage s-1's results of
         // rows t-s-1 and t-s (computed at rows t-2 and t-1) and of row t-s+1
         // (this row); its down link is row t-s's b(c, t-s-1), its up link
@@ -773,9 +813,13 @@ __device__ __forceinline__ void psi_rowsK(const PsiSweepArgs<R> &A, int s_, int 
             const R ucb = rec[(p + Q - st_ + 1) % Q].cbd, usb = rec[(p + Q - st_ + 1) % Q].sbd;
             const cplx<R> wc = o[st_ - 1][p1], wm = o[st_ - 1][(p + Q - 2) % Q];
             const int rs = t - st_;
-            const cplx<R> wa = psi_apply(A, W.rhs, W.den, shfl_up1(wc.re), shfl_up1(wc.im), shfl_dn1(wc.re),
-                                         shfl_dn1(wc.im), wm, out[st_ - 1], W.cal, W.sal, W.ca, W.sa, W.cbd, W.sbd,
-                                         ucb, usb);
+            cplx<R> wa;
+            if constexpr (FOLD)
+                wa = psi_apply_folded(W.rhs, W.den, shfl_up1(wc.re), shfl_up1(wc.im), shfl_dn1(wc.re), shfl_dn1(wc.im),
+                                      wm, out[st_ - 1], W.cal, W.sal, W.ca, W.sa, W.cbd, W.sbd, ucb, usb);
+            else
+                wa = psi_apply(A, W.rhs, W.den, shfl_up1(wc.re), shfl_up1(wc.im), shfl_dn1(wc.re), shfl_dn1(wc.im),
+                               wm, out[st_ - 1], W.cal, W.sal, W.ca, W.sa, W.cbd, W.sbd, ucb, usb);
             const bool res = own && rs >= ja && rs < jb;  // rows of [ja, jb) lie in [0, Ny)
             cplx<R> w;
             if (st_ < K - 1) {
