@@ -1065,10 +1065,11 @@ static int gradient_pass_launch(const Env &e, glx_grid g, double kappa2, double 
     const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
     const void *fn = withA ? (upd ? (const void *)k_cg_grad_rows<R, true, true> : (const void *)k_cg_grad_rows<R, true, false>)
                            : (upd ? (const void *)k_cg_grad_rows<R, false, true> : (const void *)k_cg_grad_rows<R, false, false>);
-    // two workgroups per CU: with the three the gradient pass's registers
-    // allow, the rows in flight per XCD overflow its L2 before the
-    // neighbouring strips have read their halo lines (Config 3: 0.70 against
-    // 0.72 ms per CG iteration)
+    // two workgroups per CU, not the three the gradient pass's registers
+    // allow: measured on a 4097^2 grid with holes and an eps map, a CG
+    // iteration takes 0.70 ms this way against 0.72 ms with three (its
+    // traffic per node barely changes, so the gain is in how the passes
+    // overlap their loads, not in bytes)
     const long W = fit_row_waves(fn, cg_row_waves(g.Nx, g.Ny), e.num_cus, 2);
     const int nb = (int)((W + 3) / 4);
     *nb_out = nb;
