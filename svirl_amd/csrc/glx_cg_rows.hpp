@@ -75,9 +75,19 @@ __global__ void __launch_bounds__(256) k_cg_grad_rows(Geo g, R kappa2, R eps, co
     const cplx<R> *dpsi_s = UPD ? S.dpsi : psi;
     const R *jAp_s = withA ? jacA_prev : lk;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
-    if (w < W) for_rows_rm<1>(g.Nx, g.Ny, w, W, [&](int s, int ja, int jb) {
-        const ColCtx X = col_ctx<1>(g.Nx, s);
+    // FP32 (LOCK): a workgroup takes four adjacent strips over one band of
+    // rows; its waves march the same rows, kept in step by a barrier per
+    // trip of the row loop, so the cache lines the strips share at their
+    // edges are read by both while they are in the L2 (a CG iteration on a
+    // 4097^2 grid: 0.676 against 0.698 ms). A wave past the last strip
+    // repeats the last strip's loads and stores and sums nothing; every
+    // wave of a workgroup has the same band, hence the same trip count.
+    // FP64 keeps the wave-by-wave row-major partition, whose order of the
+    // FP64 partial sums the parity tests are pinned to.
+    constexpr bool LOCK = std::is_same<R, float>::value;
+    auto body = [&](int s, int ja, int jb, bool live) {
+        ColCtx X = col_ctx<1>(g.Nx, s);
+        X.own = X.own && live;
         const int c = X.c;
         const uint32_t cc = X.cc, ca = X.ca;
         // Raw loads of marching step j: psi (and its update direction) at
@@ -256,8 +266,22 @@ __global__ void __launch_bounds__(256) k_cg_grad_rows(Geo g, R kappa2, R eps, co
 #pragma unroll
             for (int k = 0; k < B; ++k)
                 if (j + B + k < jb) consume(S1[k], j + B + k);
+            if constexpr (LOCK) __syncthreads();
         }
-    });
+    };
+    if constexpr (LOCK) {
+        const long nst = nstrips_of<1>(g.Nx), ng = (nst + 3) / 4, nwg = gridDim.x;
+        const long nband = nwg >= ng ? nwg / ng : 1, U = nwg < nband * ng ? nwg : nband * ng;
+        for (long item = xcd_block(); item < nband * ng; item += U) {  // workgroup-uniform
+            const long band = item / ng;
+            const int ja = (int)((band * g.Ny) / nband), jb = (int)(((band + 1) * g.Ny) / nband);
+            const int s_raw = (int)((item % ng) * 4 + wave_id());
+            if (ja < jb) body(s_raw < nst ? s_raw : (int)nst - 1, ja, jb, s_raw < nst);
+        }
+    } else {
+        const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
+        if (w < W) for_rows_rm<1>(g.Nx, g.Ny, w, W, [&](int s, int ja, int jb) { body(s, ja, jb, true); });
+    }
     block_store_partials<5>(acc, partials);
 }
 
