@@ -347,11 +347,11 @@ int glx_free_energy(int prec, glx_grid g, double kappa2, double eps, const void 
  * records of 14 doubles: (cell n, v, psi00.re, .im, psip0.re, .im,
  * psipp.re, .im, psi0p.re, .im, a00, a0p, b00, bp0) where the link values
  * are ab + abi (abi may be NULL) for the host triangulation
- * (vortex_detector.py:103-147). Records come in a fixed order (strips of
- * 63 cell columns, then cell rows, then columns), not by cell id; *count
- * is the total number of flagged cells (may exceed capacity: then
- * GLX_E_CAPACITY and only the first `capacity` records are written). The
- * library keeps a device buffer of 8 bytes per 63 cells between calls. */
+ * (vortex_detector.py:103-147). Records come in ascending cell id n, the
+ * reference's loop order; *count is the total number of flagged cells (may
+ * exceed capacity: then GLX_E_CAPACITY and only the first `capacity`
+ * records are written). The library keeps a device buffer of 12 bytes per
+ * 63 cells plus 4 bytes per cell row between calls. */
 int glx_vortex_cells(int prec, glx_grid g, const void *psi, const void *ab, const void *abi,
                      double H, double *rec, int32_t capacity, int32_t *count);
 

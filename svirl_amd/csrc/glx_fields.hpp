@@ -1407,13 +1407,14 @@ __device__ __forceinline__ double cell_winding(const Geo &g, double dxdyH, doubl
 // integer (vortex_detector.py:67-72).
 __device__ __forceinline__ bool winding_flag(double v) { return fabs(v) > 0.5 && fabs(v - rint(v)) < 0.1; }
 
-// Vortex cells in two launches, neither with an atomic: (1) k_vortex_flags
-// evaluates every cell and stores, per strip row, the 64-bit mask of the
-// flagged cells (at s Nyc + j), and per wave their number; (2) k_vortex_emit
-// takes each wave's first record slot from those numbers, walks the masks
-// again and writes a record for each set bit, re-evaluating only those cells
-// (the same expression on the same values, so the same v). Records come out
-// in strip, row, column order.
+// Vortex cells without an atomic: (1) k_vortex_flags evaluates every cell
+// and stores, per strip row, the 64-bit mask of the flagged cells (at
+// s Nyc + j); (2) three small launches turn the masks into each strip
+// row's first rank in cell-id order (k_vortex_rowcount, k_vortex_rowscan,
+// k_vortex_rowprefix); (3) k_vortex_emit walks the masks again and writes
+// a record for each set bit at its rank, re-evaluating only those cells
+// (the same expression on the same values, so the same v). Records come
+// out in cell-id order, the reference's loop order.
 //
 // Cells are cut into strips of 63 columns; in (1) a wave marches down a
 // range of cell rows of one strip, lane l holding node column i = 63 s + l
@@ -1424,7 +1425,7 @@ __device__ __forceinline__ bool winding_flag(double v) { return fabs(v) > 0.5 &&
 // and b rows j <= Ny - 2 since j < Nyc; columns past the grid are clamped
 // to the last one for their loads and never evaluate a cell.
 constexpr int VSTRIP = WAVE - 1;
-constexpr long VORTEX_MAX_WAVES = 32768;  // per-wave and per-workgroup counts fit the partials buffer
+constexpr long VORTEX_MAX_WAVES = 32768;  // waves of the flag and emit launches
 
 inline long vortex_rows(const Geo &g) { return (long)((g.Nxc + VSTRIP - 1) / VSTRIP) * g.Nyc; }
 
@@ -1481,17 +1482,14 @@ __device__ __forceinline__ float wrap_2pi_guarded(float x, bool &amb) {
 
 template <typename R>
 __global__ void __launch_bounds__(256) k_vortex_flags(Geo g, const cplx<R> *psi, const R *ab, double H,
-                                                      unsigned long long *bits, int *wcount, int *gcount, long W) {
-    __shared__ int wsum[256 / WAVE];
+                                                      unsigned long long *bits, long W) {
     const double dxdyH = g.dx * g.dy * H;
     const float pi_f = 3.14159265358979323846f;
     const float dxf = float(g.dx), dyf = float(g.dy), dxdyHf = float(dxdyH);
-    const int L = xcd_block();
-    const long w = (long)L * (blockDim.x / WAVE) + wave_id();
+    const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
+    if (w >= W) return;  // wave-uniform
     const int lane = threadIdx.x & (WAVE - 1);
     const R *bl = ab + g.Na;
-    int cnt = 0;
-    if (w < W)  // wave-uniform
     vortex_items(g, w, W, [&](int s, int ja, int jb) {
         const int i = s * VSTRIP + lane;
         const bool col = i < g.Nx, cell = lane < VSTRIP && i < g.Nxc;
@@ -1550,7 +1548,6 @@ __global__ void __launch_bounds__(256) k_vortex_flags(Geo g, const cplx<R> *psi,
                     }
                     const unsigned long long m = __ballot(flag);
                     if (lane == 0) bits[(long)s * g.Nyc + j + k] = m;
-                    cnt += (int)__popcll(m);
                     p0 = p1[k];
                     f0 = f1[k];
                     u0 = u1;
@@ -1559,15 +1556,75 @@ __global__ void __launch_bounds__(256) k_vortex_flags(Geo g, const cplx<R> *psi,
             }
         }
     });
-    if (lane == 0) {
-        if (w < W) wcount[w] = cnt;
-        wsum[wave_id()] = cnt;
-    }
+}
+
+// The rank of each strip row's first flagged cell in cell-id order (row
+// by row, strips left to right within a row), from the masks, in three
+// small launches: the flagged cells of each cell row summed over the
+// strips; an exclusive scan over the rows, which also stores the total;
+// and the running sum over the strips of each row, stored per strip row
+// (pre[s Nyc + j], the layout of the masks). In the first and last a
+// workgroup takes 64 rows, each of its four waves a quarter of the strips
+// (a wave reads 64 consecutive rows of a strip), and the quarters are
+// combined through LDS.
+static __global__ void __launch_bounds__(256) k_vortex_rowcount(const unsigned long long *bits, long ns, int Nyc,
+                                                                int *rowcnt) {
+    __shared__ int part[4][WAVE];
+    const int r = threadIdx.x & (WAVE - 1), q = threadIdx.x / WAVE;
+    const long j = (long)blockIdx.x * WAVE + r;
+    int c = 0;
+    if (j < Nyc)
+        for (long s = q; s < ns; s += 4) c += (int)__popcll(bits[s * Nyc + j]);
+    part[q][r] = c;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int t = 0;
-        for (int k = 0; k < (int)(blockDim.x / WAVE); ++k) t += wsum[k];
-        gcount[L] = t;
+    if (q == 0 && j < Nyc) rowcnt[j] = part[0][r] + part[1][r] + part[2][r] + part[3][r];
+}
+
+// In place, one workgroup of 1024 threads, each a contiguous run of rows.
+// Counts and offsets stay below 2^30 (the cells of a grid).
+static __global__ void __launch_bounds__(1024) k_vortex_rowscan(int *rowcnt, int Nyc, int *total) {
+    __shared__ long part[1024];
+    const int t = threadIdx.x;
+    const long C = ((long)Nyc + 1023) / 1024, a = t * C, b = min((long)Nyc, a + C);
+    long sum = 0;
+    for (long j = a; j < b; ++j) sum += rowcnt[j];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan of the run sums
+        const long v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    long run = part[t] - sum;
+    for (long j = a; j < b; ++j) {
+        const int c = rowcnt[j];
+        rowcnt[j] = (int)run;
+        run += c;
+    }
+    if (t == 1023) *total = (int)min(part[1023], (long)INT_MAX);
+}
+
+// Wave q takes strips [q ns / 4, (q+1) ns / 4), so the quarters' sums give
+// each wave the offset of its first strip.
+static __global__ void __launch_bounds__(256) k_vortex_rowprefix(const unsigned long long *bits, long ns, int Nyc,
+                                                                 const int *rowoff, int *pre) {
+    __shared__ int part[4][WAVE];
+    const int r = threadIdx.x & (WAVE - 1), q = threadIdx.x / WAVE;
+    const long j = (long)blockIdx.x * WAVE + r;
+    const long s0 = q * ns / 4, s1 = (q + 1) * ns / 4;
+    int c = 0;
+    if (j < Nyc)
+        for (long s = s0; s < s1; ++s) c += (int)__popcll(bits[s * Nyc + j]);
+    part[q][r] = c;
+    __syncthreads();
+    if (j < Nyc) {
+        int run = rowoff[j];
+        for (int k = 0; k < q; ++k) run += part[k][r];
+        for (long s = s0; s < s1; ++s) {
+            pre[s * Nyc + j] = run;
+            run += (int)__popcll(bits[s * Nyc + j]);
+        }
     }
 }
 
@@ -1601,50 +1658,25 @@ __device__ __forceinline__ int nth_set_bit(unsigned long long m, int n) {
     return pos;
 }
 
-// Records of the flagged cells: (cell n, v, psi at the four corners
-// counter-clockwise from (i, j), a(i, j), a(i, j+1), b(i, j), b(i+1, j))
-// with abi added to the links when given. Slots past `capacity` are not
-// written. Each workgroup finds its first slot itself, as the sum of the
-// counts of the workgroups before its own (gcount, one per workgroup of
-// the flag launch, which has the same grid; the first workgroup sums them
-// all and stores the total), so no separate scan launch is needed: the
-// counts are a few KB to 32 KB and stay in the caches.
+// Records of the flagged cells in cell-id order: (cell n, v, psi at the
+// four corners counter-clockwise from (i, j), a(i, j), a(i, j+1), b(i, j),
+// b(i+1, j)) with abi added to the links when given. A cell's slot is its
+// strip row's rank (pre) plus its place among that row's set bits. Slots
+// past `capacity` are not written.
 template <typename R>
 __global__ void __launch_bounds__(256) k_vortex_emit(Geo g, const cplx<R> *psi, const R *ab, const R *abi, double H,
-                                                     const unsigned long long *bits, const int *wcount,
-                                                     const int *gcount, int *total,
-                                                     double *rec, int capacity, long W) {
-    __shared__ double lds[RED_BLOCK / WAVE];
-    __shared__ long s_base;
+                                                     const unsigned long long *bits, const int *pre, double *rec,
+                                                     int capacity, long W) {
     const double dxdyH = g.dx * g.dy * H;
-    const int L = xcd_block();
-    const long w0 = (long)L * (blockDim.x / WAVE);
-    {
-        const long lim = L == 0 ? (long)gridDim.x : (long)L;
-        long part = 0;
-        const int4 *gc4 = reinterpret_cast<const int4 *>(gcount);  // gcount is 16-byte aligned
-        for (long k = threadIdx.x; k < lim / 4; k += blockDim.x) {
-            const int4 q = gc4[k];
-            part += (long)q.x + q.y + q.z + q.w;
-        }
-        for (long k = (lim & ~3L) + threadIdx.x; k < lim; k += blockDim.x) part += gcount[k];
-        const double sum = block_sum(double(part), lds);  // integers below 2^53: exact
-        if (threadIdx.x == 0) {
-            if (L == 0) *total = (int)fmin(sum, double(INT_MAX));
-            s_base = L == 0 ? 0 : (long)sum;
-        }
-        __syncthreads();
-    }
-    const long w = w0 + wave_id();
+    const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
     if (w >= W) return;  // wave-uniform
     const int lane = threadIdx.x & (WAVE - 1);
-    long base = s_base;
-    for (long k = w0; k < w; ++k) base += wcount[k];  // the waves before this one in the workgroup
     vortex_items(g, w, W, [&](int s, int ja, int jb) {
         const long r0 = (long)s * g.Nyc + ja, r1 = (long)s * g.Nyc + jb;
         for (long rc = r0; rc < r1; rc += WAVE) {
             // the masks of up to 64 rows in one load, lane k holding row rc + k
             const unsigned long long mk = rc + lane < r1 ? bits[rc + lane] : 0ull;
+            const int pk = rc + lane < r1 ? pre[rc + lane] : 0;
             const int ck = (int)__popcll(mk);
             const int ek = wave_excl_scan(ck);  // flagged cells in the rows before row rc + lane
             const int total = __shfl(ek + ck, WAVE - 1, WAVE);
@@ -1660,6 +1692,7 @@ __global__ void __launch_bounds__(256) k_vortex_emit(Geo g, const cplx<R> *psi, 
                 }
                 const unsigned long long m = __shfl(mk, k, WAVE);
                 const int e = __shfl(ek, k, WAVE);
+                const long q = (long)__shfl(pk, k, WAVE) + (f - e);
                 if (f < total) {
                     const int j = ja + (int)(rc - r0) + k;
                     const int i = s * VSTRIP + nth_set_bit(m, f - e);  // i < Nxc, j < Nyc: a cell
@@ -1674,7 +1707,6 @@ __global__ void __launch_bounds__(256) k_vortex_emit(Geo g, const cplx<R> *psi, 
                                                   atan2(double(pp0.im), double(pp0.re)),
                                                   atan2(double(p0p.im), double(p0p.re)),
                                                   atan2(double(ppp.im), double(ppp.re)), a0, a1, b0, bp);
-                    const long q = base + f;
                     if (q < capacity) {
                         double *o = rec + q * VREC;
                         o[0] = double(i + (long)g.Nxc * j);
@@ -1690,7 +1722,6 @@ __global__ void __launch_bounds__(256) k_vortex_emit(Geo g, const cplx<R> *psi, 
                     }
                 }
             }
-            base += total;
         }
     });
 }

@@ -929,19 +929,29 @@ int L<R>::vortex_cells(const Env &e, glx_grid g, const void *psi, const void *ab
     if (!grid_ok(g) || !psi || !ab || !count || capacity < 0 || (capacity > 0 && !rec))
         return set_error(GLX_E_ARG, "glx_vortex_cells: invalid arguments");
     const Geo G = make_geo(g.Nx, g.Ny, g.dx, g.dy);
-    const long T = vortex_rows(G), W = vortex_waves(G);
-    void *bits = nullptr;
-    int rc = grow_workspace(e, sizeof(unsigned long long) * (size_t)T, &bits);
+    const long T = vortex_rows(G), W = vortex_waves(G), ns = T / G.Nyc;
+    // workspace: the masks (8 B per strip row), the ranks (4 B per strip
+    // row) and the per-row counts and offsets (4 B per cell row)
+    void *ws = nullptr;
+    int rc = grow_workspace(e, (sizeof(unsigned long long) + sizeof(int)) * (size_t)T + sizeof(int) * (size_t)G.Nyc, &ws);
     if (rc) return rc;
-    int *wcount = (int *)e.partials, *gcount = wcount + VORTEX_MAX_WAVES;
+    unsigned long long *bits = (unsigned long long *)ws;
+    int *pre = (int *)(bits + T), *rowcnt = pre + T;
     int *dcount = (int *)e.result;
-    const unsigned nb = (unsigned)((W + 3) / 4);
+    const unsigned nb = (unsigned)((W + 3) / 4), nr = (unsigned)((G.Nyc + WAVE - 1) / WAVE);
     hipLaunchKernelGGL(k_vortex_flags<R>, dim3(nb), dim3(256), 0, e.stream, G, (const cplx<R> *)psi, (const R *)ab,
-                       double(R(H)), (unsigned long long *)bits, wcount, gcount, W);
+                       double(R(H)), bits, W);
     GLX_LAUNCH_CHECK("k_vortex_flags");
+    hipLaunchKernelGGL(k_vortex_rowcount, dim3(nr), dim3(256), 0, e.stream, (const unsigned long long *)bits, ns,
+                       G.Nyc, rowcnt);
+    GLX_LAUNCH_CHECK("k_vortex_rowcount");
+    hipLaunchKernelGGL(k_vortex_rowscan, dim3(1), dim3(1024), 0, e.stream, rowcnt, G.Nyc, dcount);
+    GLX_LAUNCH_CHECK("k_vortex_rowscan");
+    hipLaunchKernelGGL(k_vortex_rowprefix, dim3(nr), dim3(256), 0, e.stream, (const unsigned long long *)bits, ns,
+                       G.Nyc, (const int *)rowcnt, pre);
+    GLX_LAUNCH_CHECK("k_vortex_rowprefix");
     hipLaunchKernelGGL(k_vortex_emit<R>, dim3(nb), dim3(256), 0, e.stream, G, (const cplx<R> *)psi, (const R *)ab,
-                       (const R *)abi, double(R(H)), (const unsigned long long *)bits, (const int *)wcount,
-                       (const int *)gcount, dcount, rec,
+                       (const R *)abi, double(R(H)), (const unsigned long long *)bits, (const int *)pre, rec,
                        (int)capacity, W);
     GLX_LAUNCH_CHECK("k_vortex_emit");
     int c = 0;

@@ -1,7 +1,7 @@
 """Vortex detector (svirl/observables/vortex_detector.py): the per-cell
 winding number (glx_vortex_cells) and the sub-cell triangulation of the
-flagged cells (glx_vortex_triangulate) run on the GPU; the host orders the
-results by cell id, as the reference's loop over cells does."""
+flagged cells (glx_vortex_triangulate) run on the GPU, which returns the
+cells in ascending cell id, the order of the reference's loop over cells."""
 import ctypes
 
 import numpy as np
@@ -21,7 +21,8 @@ class VortexDetector:
         self._capacity = 1024
         self._rec = None
         self._xyv = None
-        self.vortex_cells = None  # (cell ids ascending, vorticity) of the last detection
+        self._n_cells = 0
+        self._cells_cache = None
 
     def unflatten_c(self, n):
         return n % self.cfg.Nxc, n // self.cfg.Nxc
@@ -34,9 +35,8 @@ class VortexDetector:
         return nat.Grid(int(c.Nx), int(c.Ny), float(c.dx), float(c.dy))
 
     def _cells(self):
-        """Flagged cells from the GPU winding kernel: the records in device
-        order (left in self._rec on the device) and the permutation that
-        sorts them by cell id."""
+        """Runs the GPU winding kernel; the flagged cells' records stay on
+        the device (self._rec), in ascending cell id. Returns their number."""
         c = self.cfg
         g = self._grid()
         self.vars._psi.sync_d()
@@ -55,10 +55,20 @@ class VortexDetector:
                 if e.code != nat.GLX_E_CAPACITY:
                     raise
                 self._capacity = int(count.value) + 64
-        n = count.value
-        rec = self._rec.get(n * _VREC).reshape(n, _VREC) if n else np.zeros((0, _VREC))
-        # cell ids are unique, so any sort gives the reference's ascending order
-        return rec, np.argsort(rec[:, 0])
+        return int(count.value)
+
+    @property
+    def vortex_cells(self):
+        """(cell ids ascending, vorticity) of the flagged cells of the last
+        detection, copied from the device on first use (the positions need
+        only the triangulation's output)."""
+        if self.solver.vortices_detected is not True:
+            _ = self.vortices  # detection
+        if self._cells_cache is None:
+            n = self._n_cells
+            rec = self._rec.get(n * _VREC).reshape(n, _VREC) if n else np.zeros((0, _VREC))
+            self._cells_cache = (rec[:, 0].astype(np.int64), rec[:, 1].copy())
+        return self._cells_cache
 
     @property
     def vortices(self):
@@ -66,14 +76,13 @@ class VortexDetector:
         (vortex_detector.py:31-156), in ascending cell id."""
         if self.solver.vortices_detected is not True:
             c = self.cfg
-            rec, order = self._cells()
-            n = len(rec)
-            self.vortex_cells = (rec[order, 0].astype(np.int64), rec[order, 1].copy())
+            n = self._cells()
+            self._n_cells, self._cells_cache = n, None
             if n:
                 if self._xyv is None or self._xyv.size < 4 * n:
                     self._xyv = nat.DeviceArray(4 * max(n, self._capacity), np.float64)
                 nat.call("glx_vortex_triangulate", self._grid(), self._rec.ptr, n, self._xyv.ptr)
-                xyv = self._xyv.get(4 * n).reshape(n, 4)[order]
+                xyv = self._xyv.get(4 * n).reshape(n, 4)
                 det = xyv[xyv[:, 3] != 0.0, :3]
             else:
                 det = np.zeros((0, 3))
