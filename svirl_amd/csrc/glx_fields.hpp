@@ -1409,9 +1409,9 @@ __device__ __forceinline__ bool winding_flag(double v) { return fabs(v) > 0.5 &&
 
 // Vortex cells without an atomic: (1) k_vortex_flags evaluates every cell
 // and stores, per strip row, the 64-bit mask of the flagged cells (at
-// s Nyc + j); (2) three small launches turn the masks into each strip
-// row's first rank in cell-id order (k_vortex_rowcount, k_vortex_rowscan,
-// k_vortex_rowprefix); (3) k_vortex_emit walks the masks again and writes
+// s Nyc + j); (2) two small launches turn the masks into each strip row's
+// first rank in cell-id order (k_vortex_rowprefix, k_vortex_rowscan); (3)
+// k_vortex_emit walks the masks again and writes
 // a record for each set bit at its rank, re-evaluating only those cells
 // (the same expression on the same values, so the same v). Records come
 // out in cell-id order, the reference's loop order.
@@ -1559,25 +1559,36 @@ __global__ void __launch_bounds__(256) k_vortex_flags(Geo g, const cplx<R> *psi,
 }
 
 // The rank of each strip row's first flagged cell in cell-id order (row
-// by row, strips left to right within a row), from the masks, in three
-// small launches: the flagged cells of each cell row summed over the
-// strips; an exclusive scan over the rows, which also stores the total;
-// and the running sum over the strips of each row, stored per strip row
-// (pre[s Nyc + j], the layout of the masks). In the first and last a
-// workgroup takes 64 rows, each of its four waves a quarter of the strips
-// (a wave reads 64 consecutive rows of a strip), and the quarters are
-// combined through LDS.
-static __global__ void __launch_bounds__(256) k_vortex_rowcount(const unsigned long long *bits, long ns, int Nyc,
-                                                                int *rowcnt) {
+// by row, strips left to right within a row), from the masks, in two
+// small launches: k_vortex_rowprefix stores, per strip row, the flagged
+// cells of the strips to its left in the same cell row (pre[s Nyc + j],
+// the layout of the masks) and per cell row their total; k_vortex_rowscan
+// turns the totals into the rank of each cell row's first cell and stores
+// the grand total. A cell's rank is then rowoff[j] + pre[s Nyc + j] + its
+// place among the strip row's set bits. In k_vortex_rowprefix a workgroup
+// takes 64 rows, each of its four waves a contiguous quarter of the strips
+// (a wave reads 64 consecutive rows of a strip), and the quarters' sums,
+// exchanged through LDS, give each wave the offset of its first strip.
+static __global__ void __launch_bounds__(256) k_vortex_rowprefix(const unsigned long long *bits, long ns, int Nyc,
+                                                                 int *pre, int *rowcnt) {
     __shared__ int part[4][WAVE];
     const int r = threadIdx.x & (WAVE - 1), q = threadIdx.x / WAVE;
     const long j = (long)blockIdx.x * WAVE + r;
+    const long s0 = q * ns / 4, s1 = (q + 1) * ns / 4;
     int c = 0;
     if (j < Nyc)
-        for (long s = q; s < ns; s += 4) c += (int)__popcll(bits[s * Nyc + j]);
+        for (long s = s0; s < s1; ++s) c += (int)__popcll(bits[s * Nyc + j]);
     part[q][r] = c;
     __syncthreads();
-    if (q == 0 && j < Nyc) rowcnt[j] = part[0][r] + part[1][r] + part[2][r] + part[3][r];
+    if (j < Nyc) {
+        int run = 0;
+        for (int k = 0; k < q; ++k) run += part[k][r];
+        for (long s = s0; s < s1; ++s) {
+            pre[s * Nyc + j] = run;
+            run += (int)__popcll(bits[s * Nyc + j]);
+        }
+        if (q == 0) rowcnt[j] = part[0][r] + part[1][r] + part[2][r] + part[3][r];
+    }
 }
 
 // In place, one workgroup of 1024 threads, each a contiguous run of rows.
@@ -1603,29 +1614,6 @@ static __global__ void __launch_bounds__(1024) k_vortex_rowscan(int *rowcnt, int
         run += c;
     }
     if (t == 1023) *total = (int)min(part[1023], (long)INT_MAX);
-}
-
-// Wave q takes strips [q ns / 4, (q+1) ns / 4), so the quarters' sums give
-// each wave the offset of its first strip.
-static __global__ void __launch_bounds__(256) k_vortex_rowprefix(const unsigned long long *bits, long ns, int Nyc,
-                                                                 const int *rowoff, int *pre) {
-    __shared__ int part[4][WAVE];
-    const int r = threadIdx.x & (WAVE - 1), q = threadIdx.x / WAVE;
-    const long j = (long)blockIdx.x * WAVE + r;
-    const long s0 = q * ns / 4, s1 = (q + 1) * ns / 4;
-    int c = 0;
-    if (j < Nyc)
-        for (long s = s0; s < s1; ++s) c += (int)__popcll(bits[s * Nyc + j]);
-    part[q][r] = c;
-    __syncthreads();
-    if (j < Nyc) {
-        int run = rowoff[j];
-        for (int k = 0; k < q; ++k) run += part[k][r];
-        for (long s = s0; s < s1; ++s) {
-            pre[s * Nyc + j] = run;
-            run += (int)__popcll(bits[s * Nyc + j]);
-        }
-    }
 }
 
 // Exclusive prefix sum of v over the lanes of a wave.
@@ -1661,12 +1649,13 @@ __device__ __forceinline__ int nth_set_bit(unsigned long long m, int n) {
 // Records of the flagged cells in cell-id order: (cell n, v, psi at the
 // four corners counter-clockwise from (i, j), a(i, j), a(i, j+1), b(i, j),
 // b(i+1, j)) with abi added to the links when given. A cell's slot is its
-// strip row's rank (pre) plus its place among that row's set bits. Slots
-// past `capacity` are not written.
+// cell row's rank (rowoff) plus the cells of the strips to its left in that
+// row (pre) plus its place among its strip row's set bits. Slots past
+// `capacity` are not written.
 template <typename R>
 __global__ void __launch_bounds__(256) k_vortex_emit(Geo g, const cplx<R> *psi, const R *ab, const R *abi, double H,
-                                                     const unsigned long long *bits, const int *pre, double *rec,
-                                                     int capacity, long W) {
+                                                     const unsigned long long *bits, const int *pre,
+                                                     const int *rowoff, double *rec, int capacity, long W) {
     const double dxdyH = g.dx * g.dy * H;
     const long w = (long)xcd_block() * (blockDim.x / WAVE) + wave_id();
     if (w >= W) return;  // wave-uniform
@@ -1676,7 +1665,9 @@ __global__ void __launch_bounds__(256) k_vortex_emit(Geo g, const cplx<R> *psi, 
         for (long rc = r0; rc < r1; rc += WAVE) {
             // the masks of up to 64 rows in one load, lane k holding row rc + k
             const unsigned long long mk = rc + lane < r1 ? bits[rc + lane] : 0ull;
-            const int pk = rc + lane < r1 ? pre[rc + lane] : 0;
+            // rank of the row's first cell: its cell row's offset plus the
+            // strips to its left in that row
+            const int pk = rc + lane < r1 ? pre[rc + lane] + rowoff[ja + (rc - r0) + lane] : 0;
             const int ck = (int)__popcll(mk);
             const int ek = wave_excl_scan(ck);  // flagged cells in the rows before row rc + lane
             const int total = __shfl(ek + ck, WAVE - 1, WAVE);

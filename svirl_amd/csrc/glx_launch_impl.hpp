@@ -942,17 +942,14 @@ int L<R>::vortex_cells(const Env &e, glx_grid g, const void *psi, const void *ab
     hipLaunchKernelGGL(k_vortex_flags<R>, dim3(nb), dim3(256), 0, e.stream, G, (const cplx<R> *)psi, (const R *)ab,
                        double(R(H)), bits, W);
     GLX_LAUNCH_CHECK("k_vortex_flags");
-    hipLaunchKernelGGL(k_vortex_rowcount, dim3(nr), dim3(256), 0, e.stream, (const unsigned long long *)bits, ns,
-                       G.Nyc, rowcnt);
-    GLX_LAUNCH_CHECK("k_vortex_rowcount");
+    hipLaunchKernelGGL(k_vortex_rowprefix, dim3(nr), dim3(256), 0, e.stream, (const unsigned long long *)bits, ns,
+                       G.Nyc, pre, rowcnt);
+    GLX_LAUNCH_CHECK("k_vortex_rowprefix");
     hipLaunchKernelGGL(k_vortex_rowscan, dim3(1), dim3(1024), 0, e.stream, rowcnt, G.Nyc, dcount);
     GLX_LAUNCH_CHECK("k_vortex_rowscan");
-    hipLaunchKernelGGL(k_vortex_rowprefix, dim3(nr), dim3(256), 0, e.stream, (const unsigned long long *)bits, ns,
-                       G.Nyc, (const int *)rowcnt, pre);
-    GLX_LAUNCH_CHECK("k_vortex_rowprefix");
     hipLaunchKernelGGL(k_vortex_emit<R>, dim3(nb), dim3(256), 0, e.stream, G, (const cplx<R> *)psi, (const R *)ab,
-                       (const R *)abi, double(R(H)), (const unsigned long long *)bits, (const int *)pre, rec,
-                       (int)capacity, W);
+                       (const R *)abi, double(R(H)), (const unsigned long long *)bits, (const int *)pre,
+                       (const int *)rowcnt, rec, (int)capacity, W);
     GLX_LAUNCH_CHECK("k_vortex_emit");
     int c = 0;
     hipError_t st = hipMemcpyAsync(&c, dcount, sizeof(int), hipMemcpyDeviceToHost, e.stream);
