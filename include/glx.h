@@ -343,7 +343,8 @@ int glx_free_energy(int prec, glx_grid g, double kappa2, double eps, const void 
 
 /* Vortex cells: winding number of every cell as in
  * svirl/observables/vortex_detector.py:55-72 (np.angle, np.mod, np.round
- * semantics), computed in FP64. Flagged cells are written to `rec` as
+ * semantics), computed in FP64, with H first rounded to the precision
+ * `prec` as svirl's Params stores it (svirl/vars/params.py:131-132). Flagged cells are written to `rec` as
  * records of 14 doubles: (cell n, v, psi00.re, .im, psip0.re, .im,
  * psipp.re, .im, psi0p.re, .im, a00, a0p, b00, bp0) where the link values
  * are ab + abi (abi may be NULL) for the host triangulation

@@ -536,6 +536,48 @@ def test_vortex_cells_match_oracle():
     assert np.max(np.abs(r[:, 1] - vs)) < 1e-12
 
 
+@pytest.mark.parametrize("prec", PRECS)
+def test_vortex_cells_large_random_field(prec):
+    """Winding numbers of a random field at 1100 x 1300 nodes: 18 strips of
+    63 cells per cell row and 1299 cell rows, so the per-row ranks span many
+    strips and the row-offset scan runs over more rows than one pass of its
+    workgroup covers. About a fifth of the cells are flagged. Against the
+    oracle (FP32 inputs widened exactly, and H rounded to float as svirl's
+    Params stores it in a float32 run, params.py:132): the same cells in
+    ascending id, vorticities within 1e-12, and every record's corner psi
+    and link values those of its own cell, which pins where each record was
+    written."""
+    rng = np.random.default_rng(21)
+    g = O.Geom(1100, 1300, 0.5, 0.5)
+    rt, ct = _types(prec)
+    psi = (rng.standard_normal(g.N) + 1j * rng.standard_normal(g.N)).astype(ct)
+    ab = (0.8 * rng.standard_normal(g.Nab)).astype(rt)
+    H = 0.05
+    psi64, ab64 = psi.astype(np.complex128), ab.astype(np.float64)
+    cells, vs = O.vortex_cells(g, psi64, ab64, float(rt(H)))
+    assert 0.1 * g.Nc < len(cells) < 0.4 * g.Nc
+    cap = len(cells) + 16
+    rec = nat.DeviceArray.zeros(cap * 14, np.float64)
+    cnt = ctypes.c_int32()
+    d_psi, d_ab = D(psi, ct), D(ab, rt)
+    print(f"vortex cells {g.Nx}x{g.Ny} prec {prec}", flush=True)
+    nat.call("glx_vortex_cells", prec, grid(g), d_psi.ptr, d_ab.ptr, None, H, rec.ptr, cap, ctypes.byref(cnt))
+    assert cnt.value == len(cells)
+    r = rec.get()[:cnt.value * 14].reshape(-1, 14)
+    n = r[:, 0].astype(np.int64)
+    assert np.array_equal(n, cells)
+    assert np.max(np.abs(r[:, 1] - vs)) < 1e-12
+    i, j = n % g.Nxc, n // g.Nxc
+    Nxa, Na = g.Nx - 1, (g.Nx - 1) * g.Ny
+    corners = [i + g.Nx * j, i + 1 + g.Nx * j, i + 1 + g.Nx * (j + 1), i + g.Nx * (j + 1)]
+    for k, m in enumerate(corners):
+        assert np.array_equal(r[:, 2 + 2 * k], psi64.real[m])
+        assert np.array_equal(r[:, 3 + 2 * k], psi64.imag[m])
+    links = [i + Nxa * j, i + Nxa * (j + 1), Na + i + g.Nx * j, Na + i + 1 + g.Nx * j]
+    for k, m in enumerate(links):
+        assert np.array_equal(r[:, 10 + k], ab64[m])
+
+
 @pytest.mark.parametrize("with_abi", [False, True])
 def test_vortex_triangulate_matches_oracle(with_abi):
     """Sub-cell triangulation on the device against the oracle's transcription
