@@ -1234,6 +1234,11 @@ __device__ __forceinline__ void a_rows3(const ASweepArgs<R> &A0, int s, int ja, 
     ASweepArgs<R> tmp;
     const ASweepArgs<R> &A = a_row_args<R, SH>(A0, tmp);
     constexpr int B = Pipe<R, SH>::AROWS3;
+    // FP64 persistent solve: the link updates and sweep j+2's sincos are
+    // computed on all lanes and selected (stores and residuals stay under
+    // their lane conditions), so a row's sweeps can be interleaved; the same
+    // expressions, so the same values
+    constexpr bool BF = std::is_same<R, double>::value && !SH;
     const ColCtx X = col_ctx<3>(A.Nx, s);
     const int lane = threadIdx.x & (WAVE - 1);
     const int c = X.c;
@@ -1306,8 +1311,15 @@ __device__ __forceinline__ void a_rows3(const ASweepArgs<R> &A0, int s, int ja, 
 
         // sweep j at row t -> A^{j+1}(t)
         R va = R(0), vb = R(0);
-        if (mid1 && hasa_t) va = a_link_update(A, t, ra, jla, am, bm, brm, ap, bc, brc);
-        if (mid1 && hasb_t) vb = b_link_update(A, c, rb, jlb, blc, alc, alp, brc, ac, ap);
+        if constexpr (BF) {
+            const R xa = a_link_update(A, t, ra, jla, am, bm, brm, ap, bc, brc);
+            const R xb = b_link_update(A, c, rb, jlb, blc, alc, alp, brc, ac, ap);
+            va = (mid1 && hasa_t) ? xa : R(0);
+            vb = (mid1 && hasb_t) ? xb : R(0);
+        } else {
+            if (mid1 && hasa_t) va = a_link_update(A, t, ra, jla, am, bm, brm, ap, bc, brc);
+            if (mid1 && hasb_t) vb = b_link_update(A, c, rb, jlb, blc, alc, alp, brc, ac, ap);
+        }
         if (own && t >= ja && t < jb) {
             if (hasa_t) {
                 if (noisy && A.jstep == 0) A.rhs_out[na] = ra;
@@ -1324,7 +1336,13 @@ __device__ __forceinline__ void a_rows3(const ASweepArgs<R> &A0, int s, int ja, 
         {
             const R brc1 = shfl_dn1(bc1), brm1 = shfl_dn1(bm1);
             const R alc1 = shfl_up1(ac1), alp1 = shfl_up1(va), blc1 = shfl_up1(bc1);
-            if (mid2 && u >= 0 && u < Ny) {
+            if constexpr (BF) {
+                const R xa = a_link_update(A, u, ra1, jla1, am1, bm1, brm1, va, bc1, brc1);
+                const R xb = b_link_update(A, c, rb1, jlb1, blc1, alc1, alp1, brc1, ac1, va);
+                const bool m2 = mid2 && u >= 0 && u < Ny;
+                wa = (m2 && hasa) ? xa : R(0);
+                wb = (m2 && u < Ny - 1) ? xb : R(0);
+            } else if (mid2 && u >= 0 && u < Ny) {
                 if (hasa) wa = a_link_update(A, u, ra1, jla1, am1, bm1, brm1, va, bc1, brc1);
                 if (u < Ny - 1) wb = b_link_update(A, c, rb1, jlb1, blc1, alc1, alp1, brc1, ac1, va);
             }
@@ -1345,7 +1363,25 @@ __device__ __forceinline__ void a_rows3(const ASweepArgs<R> &A0, int s, int ja, 
             const R brc2 = shfl_dn1(bc2), brm2 = shfl_dn1(bm2);
             const R alc2 = shfl_up1(ac2), alp2 = shfl_up1(wa), blc2 = shfl_up1(bc2);
             const R pvr_re = shfl_dn1(pv.re), pvr_im = shfl_dn1(pv.im);
-            if (v >= ja && own) {
+            if constexpr (BF) {
+                R s2a, c2a, s2b, c2b;
+                sincos_r(link_phase<R, GAUGE>(A.dx, ac2, GAUGE ? gauge_phase_a<R>(A, v) : R(0)), &s2a, &c2a);
+                sincos_r(link_phase<R, GAUGE>(A.dy, bc2, gb), &s2b, &c2b);
+                const R jla2 = ena2 ? link_current(A.idx, pv, pvr_re, pvr_im, c2a, s2a) : R(0);
+                const R xa = a_link_update(A, v, ra2, jla2, am2, bm2, brm2, wa, bc2, brc2);
+                const R jlb2 = enb2 ? link_current(A.idy, pv, pu.re, pu.im, c2b, s2b) : R(0);
+                const R xb = b_link_update(A, c, rb2, jlb2, blc2, alc2, alp2, brc2, ac2, wa);
+                if (v >= ja && own) {
+                    if (hasa) {
+                        st_at(A.ab_next + (size_t)Nxa * v, oc, xa);
+                        r3 = fmax(r3, fabs(xa - ac2));
+                    }
+                    if (v < Ny - 1) {
+                        st_at(A.ab_next + (size_t)Na + (size_t)Nx * v, oc, xb);
+                        r3 = fmax(r3, fabs(xb - bc2));
+                    }
+                }
+            } else if (v >= ja && own) {
                 R s2a, c2a, s2b, c2b;
                 sincos_r(link_phase<R, GAUGE>(A.dx, ac2, GAUGE ? gauge_phase_a<R>(A, v) : R(0)), &s2a, &c2a);
                 sincos_r(link_phase<R, GAUGE>(A.dy, bc2, gb), &s2b, &c2b);
